@@ -1,0 +1,261 @@
+"""ctypes binding of libmpsolve (include/mpsolve.h).
+
+The library is the product path: there is no CPU fallback.  Importing this module raises ImportError when
+libmpsolve.so has not been built, and every call raises MPSolveError on a negative status.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libmpsolve.so"
+
+MP_OK = 0
+STATUS = {-1: "MP_ERR_INVALID", -2: "MP_ERR_HIP", -3: "MP_ERR_ALLOC", -4: "MP_ERR_UNSUPPORTED",
+          -5: "MP_ERR_INTERNAL", -6: "MP_ERR_NO_DEVICE"}
+FMT = {"fp16": 0, "bf16": 1, "fp8": 2, "fp8e4m3": 2, "fp32": 3}
+
+# every function declared in include/mpsolve.h
+EXPORTS = [
+    "mp_create", "mp_destroy", "mp_last_error", "mp_caps", "mp_synchronize", "mp_profile", "mp_profile_json", "mp_malloc", "mp_free",
+    "mp_memcpy_h2d", "mp_memcpy_d2h", "mp_memcpy_d2d", "mp_residual_fp64", "mp_residual_dd", "mp_getrf",
+    "mp_getrs", "mp_problem_from_host", "mp_problem_free", "mp_problem_case_json", "mp_solve",
+    "mp_last_record_json", "mp_last_hist_json", "mp_methods_json", "mp_eval_metrics_json",
+]
+
+
+class MPSolveError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class GetrfParams(C.Structure):
+    _fields_ = [("nb", C.c_int32), ("fmt", C.c_int32), ("tile_scale", C.c_int32)]
+
+
+class GetrfInfo(C.Structure):
+    _fields_ = [("info", C.c_int32), ("panels", C.c_int32), ("conv_overflow", C.c_uint64), ("conv_subnormal", C.c_uint64)]
+
+
+class TrialOut(C.Structure):
+    _fields_ = [("ok", C.c_int32), ("outer_iters", C.c_int32), ("inner_iters_total", C.c_int32),
+                ("refactorizations", C.c_int32), ("device_ready_wall_s", C.c_double), ("factor_total_s", C.c_double),
+                ("eta_max", C.c_double), ("omega_max", C.c_double), ("terminal_reason", C.c_char * 48),
+                ("final_stage", C.c_char * 48)]
+
+
+def _load() -> C.CDLL:
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built; run `python -m cpsc402_midterm_project_amd.build`")
+    lib = C.CDLL(str(LIB_PATH))
+    vp, i64, i32, dp, sz = C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_size_t
+    sig = {
+        "mp_create": ([C.POINTER(vp), C.c_int], C.c_int),
+        "mp_destroy": ([vp], None),
+        "mp_last_error": ([], C.c_char_p),
+        "mp_caps": ([vp, C.c_char_p, sz], C.c_int),
+        "mp_synchronize": ([vp], C.c_int),
+        "mp_profile": ([vp, C.c_int], C.c_int),
+        "mp_profile_json": ([vp, C.c_char_p, sz], C.c_int),
+        "mp_malloc": ([vp, C.POINTER(vp), sz], C.c_int),
+        "mp_free": ([vp, vp], C.c_int),
+        "mp_memcpy_h2d": ([vp, vp, vp, sz], C.c_int),
+        "mp_memcpy_d2h": ([vp, vp, vp, sz], C.c_int),
+        "mp_memcpy_d2d": ([vp, vp, vp, sz], C.c_int),
+        "mp_residual_fp64": ([vp, i64, i32, i32, vp, i64, dp, i64, dp, i64, dp, i64, dp, i64, dp, vp], C.c_int),
+        "mp_residual_dd": ([vp, i64, i32, i32, vp, i64, dp, i64, dp, i64, dp, dp, i64, dp, i64, dp, vp], C.c_int),
+        "mp_getrf": ([vp, i64, vp, i64, vp, vp, C.POINTER(GetrfParams), C.POINTER(GetrfInfo)], C.c_int),
+        "mp_getrs": ([vp, i64, i32, vp, i64, vp, dp, i64, dp, i64, C.c_double], C.c_int),
+        "mp_problem_from_host": ([vp, i64, i32, i32, vp, i64, dp, i64, dp, dp, C.c_char_p, C.POINTER(vp)], C.c_int),
+        "mp_problem_free": ([vp], None),
+        "mp_problem_case_json": ([vp, C.c_char_p, sz], C.c_int),
+        "mp_solve": ([vp, vp, C.c_char_p, C.c_char_p, C.POINTER(TrialOut), dp], C.c_int),
+        "mp_last_record_json": ([vp, C.c_char_p, sz], C.c_int),
+        "mp_last_hist_json": ([vp, C.c_char_p, sz], C.c_int),
+        "mp_methods_json": ([C.c_char_p, sz], C.c_int),
+        "mp_eval_metrics_json": ([vp, vp, dp, C.c_char_p, sz], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    return lib
+
+
+lib = _load()
+
+
+def check(status: int) -> None:
+    if status != MP_OK:
+        raise MPSolveError(status, (lib.mp_last_error() or b"").decode())
+
+
+def _ptr(a) -> C.c_void_p | None:
+    """Device pointer of a torch tensor / DeviceBuffer, or host pointer of a numpy array."""
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return C.c_void_p(a)
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(C.c_void_p)
+    if isinstance(a, DeviceBuffer):
+        return C.c_void_p(a.ptr)
+    raise TypeError(f"cannot take a pointer of {type(a)}")
+
+
+def _json_call(fn, *args, cap: int = 1 << 16) -> str:
+    while True:
+        buf = C.create_string_buffer(cap)
+        st = fn(*args, buf, cap)
+        if st == MP_OK:
+            return buf.value.decode()
+        msg = (lib.mp_last_error() or b"").decode()
+        if "too small" in msg and cap < (1 << 28):
+            cap *= 8
+            continue
+        raise MPSolveError(st, msg)
+
+
+def methods() -> list[dict]:
+    return json.loads(_json_call(lib.mp_methods_json, cap=1 << 18))
+
+
+class DeviceBuffer:
+    """Device memory owned by a context (mp_malloc)."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx, self.nbytes = ctx, int(nbytes)
+        p = C.c_void_p()
+        check(lib.mp_malloc(ctx.h, C.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, ctx: "Context", a: np.ndarray) -> "DeviceBuffer":
+        a = np.asfortranarray(a)
+        b = cls(ctx, a.nbytes)
+        check(lib.mp_memcpy_h2d(ctx.h, C.c_void_p(b.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes))
+        return b
+
+    def to_array(self, shape, dtype) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype, order="F")
+        check(lib.mp_memcpy_d2h(self.ctx.h, out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            check(lib.mp_free(self.ctx.h, C.c_void_p(self.ptr)))
+            self.ptr = None
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib.mp_create(C.byref(h), device))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            lib.mp_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def caps(self) -> dict:
+        return json.loads(_json_call(lib.mp_caps, self.h))
+
+    def sync(self) -> None:
+        check(lib.mp_synchronize(self.h))
+
+    def profile(self, enable: bool = True) -> None:
+        check(lib.mp_profile(self.h, 1 if enable else 0))
+
+    def profile_report(self) -> dict:
+        return json.loads(_json_call(lib.mp_profile_json, self.h))
+
+    def buffer(self, a: np.ndarray) -> DeviceBuffer:
+        return DeviceBuffer.from_array(self, a)
+
+    def empty(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    # ---- standalone kernels (device buffers)
+    def residual(self, n, nrhs, A, lda, X, ldx, B, ldb, R, ldr, T=None, ldt=None, stats=None, worst=None,
+                 a_bits=64, dd=False, Rlo=None):
+        if dd:
+            check(lib.mp_residual_dd(self.h, n, nrhs, a_bits, _ptr(A), lda, _ptr(X), ldx, _ptr(B), ldb if B is not None else n,
+                                     _ptr(R), _ptr(Rlo), ldr, _ptr(T), ldt or n, _ptr(stats), _ptr(worst)))
+        else:
+            check(lib.mp_residual_fp64(self.h, n, nrhs, a_bits, _ptr(A), lda, _ptr(X), ldx, _ptr(B), ldb if B is not None else n,
+                                       _ptr(R), ldr, _ptr(T), ldt or n, _ptr(stats), _ptr(worst)))
+
+    def getrf(self, n, LU, lda, ipiv, perm, nb=256, fmt="fp16", tile_scale=True) -> GetrfInfo:
+        p = GetrfParams(nb, FMT[fmt], 1 if tile_scale else 0)
+        info = GetrfInfo()
+        check(lib.mp_getrf(self.h, n, _ptr(LU), lda, _ptr(ipiv), _ptr(perm), C.byref(p), C.byref(info)))
+        return info
+
+    def getrs(self, n, nrhs, LU, lda, perm, R, ldr, D, ldd, scale=1.0):
+        check(lib.mp_getrs(self.h, n, nrhs, _ptr(LU), lda, _ptr(perm), _ptr(R), ldr, _ptr(D), ldd, float(scale)))
+
+    # ---- problems and solves
+    def problem(self, A: np.ndarray, B: np.ndarray, x_exact=None, x_ref=None, case: dict | None = None) -> "Problem":
+        return Problem(self, A, B, x_exact, x_ref, case)
+
+
+class Problem:
+    def __init__(self, ctx: Context, A: np.ndarray, B: np.ndarray, x_exact=None, x_ref=None, case=None):
+        A = np.asfortranarray(A)
+        if A.dtype not in (np.float64, np.float32):
+            raise TypeError("A must be float64 or float32")
+        B = np.asfortranarray(B, dtype=np.float64)
+        if B.ndim == 1:
+            B = B.reshape(-1, 1, order="F")
+        n, nrhs = A.shape[0], B.shape[1]
+        xe = None if x_exact is None else np.asfortranarray(np.asarray(x_exact, dtype=np.float64).reshape(n, nrhs, order="F"))
+        xr = None if x_ref is None else np.asfortranarray(np.asarray(x_ref, dtype=np.float64).reshape(n, nrhs, order="F"))
+        self.ctx, self.n, self.nrhs = ctx, n, nrhs
+        h = C.c_void_p()
+        cj = json.dumps(case or {}).encode()
+        check(lib.mp_problem_from_host(ctx.h, n, nrhs, 32 if A.dtype == np.float32 else 64, _ptr(A), n, _ptr(B), n,
+                                       _ptr(xe), _ptr(xr), cj, C.byref(h)))
+        self.h = h
+
+    def case_record(self) -> dict:
+        return json.loads(_json_call(lib.mp_problem_case_json, self.h))
+
+    def solve(self, method: str, **params) -> tuple[np.ndarray, dict]:
+        X = np.zeros((self.n, self.nrhs), dtype=np.float64, order="F")
+        out = TrialOut()
+        check(lib.mp_solve(self.ctx.h, self.h, method.encode(), json.dumps(params).encode(), C.byref(out), _ptr(X)))
+        rec = json.loads(_json_call(lib.mp_last_record_json, self.ctx.h, cap=1 << 20))
+        return X, rec
+
+    def history(self) -> list[dict]:
+        txt = _json_call(lib.mp_last_hist_json, self.ctx.h, cap=1 << 20)
+        return [json.loads(l) for l in txt.splitlines() if l.strip()]
+
+    def eval_metrics(self, X: np.ndarray) -> dict:
+        X = np.asfortranarray(X, dtype=np.float64)
+        return json.loads(_json_call(lib.mp_eval_metrics_json, self.ctx.h, self.h, _ptr(X)))
+
+    def close(self) -> None:
+        if self.h:
+            lib.mp_problem_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

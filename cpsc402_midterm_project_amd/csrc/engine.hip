@@ -1,0 +1,967 @@
+// engine.hip — problems, factorizations, refinement engine and the extern "C" ABI of libmpsolve.
+//
+// Host-side structure follows the reference's L2 engine (src/mp_core.cuh): Problem / Factors / MethodSpec /
+// Engine with refine_stage, gmres_solve and run, and the method registry make_method (src/mpsolve.cu:17-72).
+// The numerical kernels are this library's HIP kernels; the vendor comparators (cs32 / cs64 factor kinds,
+// i.e. the plain FP32 / FP64 LU) are rocSOLVER getrf/getrs.
+#include "mp_internal.hpp"
+#include "mp_kernels.hpp"
+#include "mp_krylov.hpp"
+#include "../../include/mpsolve.h"
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+#include <functional>
+#include <memory>
+
+using namespace mpk;
+
+__global__ void k_set_x32(float* x32, double* x64, const double* d, int64_t m) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < m) { const float v = (float)d[i]; x32[i] = v; x64[i] = (double)v; }
+}
+__global__ void k_add_x32(float* x32, double* x64, const double* d, int64_t m) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < m) { const float v = (float)((double)x32[i] + d[i]); x32[i] = v; x64[i] = (double)v; }
+}
+__global__ void k_add_vec(double* h, const double* h2, int cnt) {
+  const int i = threadIdx.x + blockIdx.x * 64;
+  if (i < cnt) h[i] += h2[i];
+}
+
+// ------------------------------------------------------------------ error plumbing
+static thread_local std::string g_last_error;
+template <class F> static int guard(F&& f) {
+  try { f(); return MP_OK; }
+  catch (MPError& e) { g_last_error = e.what(); return e.code; }
+  catch (std::exception& e) { g_last_error = e.what(); return MP_ERR_INTERNAL; }
+  catch (...) { g_last_error = "unknown error"; return MP_ERR_INTERNAL; }
+}
+#define RB_CK(x) do { rocblas_status s_ = (x); if (s_ != rocblas_status_success) { std::ostringstream o_; \
+  o_ << "rocBLAS/rocSOLVER status " << (int)s_ << " at " << __FILE__ << ":" << __LINE__ << " in " << #x; throw MPError(o_.str(), -2); } } while (0)
+
+// ------------------------------------------------------------------ minimal flat JSON object reader
+struct Params {
+  std::map<std::string, std::string> kv;
+  void parse(const char* js) {
+    if (!js) return;
+    const std::string s(js);
+    size_t i = 0;
+    auto skip = [&]() { while (i < s.size() && (isspace((unsigned char)s[i]) || s[i] == ',' || s[i] == '{' || s[i] == '}')) i++; };
+    while (true) {
+      skip();
+      if (i >= s.size() || s[i] != '"') break;
+      size_t e = s.find('"', i + 1);
+      if (e == std::string::npos) break;
+      std::string key = s.substr(i + 1, e - i - 1);
+      i = s.find(':', e);
+      if (i == std::string::npos) break;
+      i++;
+      while (i < s.size() && isspace((unsigned char)s[i])) i++;
+      std::string val;
+      if (i < s.size() && s[i] == '"') {
+        size_t e2 = s.find('"', i + 1);
+        if (e2 == std::string::npos) break;
+        val = s.substr(i + 1, e2 - i - 1); i = e2 + 1;
+      } else {
+        size_t e2 = i;
+        while (e2 < s.size() && s[e2] != ',' && s[e2] != '}') e2++;
+        val = s.substr(i, e2 - i);
+        while (!val.empty() && isspace((unsigned char)val.back())) val.pop_back();
+        i = e2;
+      }
+      kv[key] = val;
+    }
+  }
+  bool has(const std::string& k) const { return kv.count(k) > 0; }
+  std::string get(const std::string& k, const std::string& d = "") const { auto it = kv.find(k); return it == kv.end() ? d : it->second; }
+  double getd(const std::string& k, double d) const { auto it = kv.find(k); return it == kv.end() ? d : std::stod(it->second); }
+  int geti(const std::string& k, int d) const { auto it = kv.find(k); return it == kv.end() ? d : (int)std::stol(it->second); }
+  bool getb(const std::string& k, bool d) const { auto it = kv.find(k); if (it == kv.end()) return d; return it->second == "true" || it->second == "1"; }
+};
+
+// ------------------------------------------------------------------ context / problem
+struct mp_ctx {
+  int device = 0;
+  DevTracker tracker;
+  Workspace W;
+  rocblas_handle rb = nullptr;
+  double* hpin = nullptr;
+  std::string last_record, last_hist;
+  std::unique_ptr<Krylov> kry;
+};
+
+struct mp_problem {
+  mp_ctx* ctx = nullptr;
+  int64_t n = 0; int nrhs = 1; bool fp32 = false;
+  DBuf<double> A64; DBuf<float> A32; DBuf<double> B; DBuf<double> Xexact, Xref;
+  bool has_exact = false, has_ref = false;
+  double anorm_inf = 0, amax = 0;
+  std::vector<double> bnorm_inf, bnorm_2;
+  std::string a_sha, b_sha, case_json;
+  std::vector<std::string> tile_sha;
+  std::vector<double> hXexact, hXref;
+  const void* a_ptr() const { return fp32 ? (const void*)A32.p : (const void*)A64.p; }
+};
+
+// ------------------------------------------------------------------ methods
+struct MethodSpec {
+  std::string id, kind = "ir";          // direct | ir | unsupported
+  std::string factor = "cs32";          // het16 | cus32 | mx16 | mxbf16 | mx8 | cs32 | cs64
+  int working = 64, resid = 64;         // resid: 64, 32, or 128 (double-double outer residual)
+  bool apply64 = false, gmres = false, gmres_left = false;
+  int scale_mode = 0, nb = 256;
+  int gmres_m = 40; double gmres_tol = 1e-4; int gmres_budget = 480, gmres_cycles = 4;
+  int max_outer = 30; std::string policy = "none"; int budget = 0; double cost_ratio_next = 3.0;
+  std::string unsupported_reason;
+};
+
+static MethodSpec make_method(const std::string& id, const Params& a) {
+  MethodSpec M; M.id = id;
+  M.nb = a.geti("nb", 256); M.gmres_m = a.geti("gmres_m", 40); M.gmres_tol = a.getd("gmres_tol", 1e-4);
+  M.gmres_budget = a.geti("gmres_budget", 480); M.gmres_cycles = a.geti("gmres_cycles", 4);
+  M.max_outer = a.geti("max_outer", 30); M.cost_ratio_next = a.getd("cost_ratio_next", 3.0);
+  auto starts = [&](const char* p) { return id.rfind(p, 0) == 0; };
+  auto unsupported = [&](const std::string& why) { M.kind = "unsupported"; M.unsupported_reason = why; };
+  // ---- reference IDs (src/mpsolve.cu:20-60)
+  if (id == "native_lu32") { M.kind = "direct"; M.factor = "cs32"; M.working = 32; M.max_outer = 0; }
+  else if (id == "native_lu64" || id == "native_lu64_promoted32") { M.kind = "direct"; M.factor = "cs64"; M.max_outer = 0; }
+  else if (id == "native_ir32_w32_r64") { M.factor = "cs32"; M.working = 32; }
+  else if (id == "native_ir32_w32_r32") { M.factor = "cs32"; M.working = 32; M.resid = 32; }
+  else if (id == "native_ir32_w64_r64") { M.factor = "cs32"; }
+  else if (id == "native_ir64_w64_r64") { M.factor = "cs64"; }
+  else if (id == "native_ir32_w64_r64_twosided") { M.factor = "cs32"; M.scale_mode = 2; }
+  else if (id == "native_ir32_w64_r64_global") { M.factor = "cs32"; M.scale_mode = 1; }
+  else if (id == "native_gmresir32_w64_r64") { M.factor = "cs32"; M.gmres = true; }
+  else if (id == "native_gmresir32_w32_r64") { M.factor = "cs32"; M.working = 32; M.gmres = true; }
+  else if (id == "native_leftgmres32_app32_w64_r64") { M.factor = "cs32"; M.gmres = true; M.gmres_left = true; M.gmres_cycles = 1; }
+  else if (id == "native_leftgmres32_app64_w64_r64") { M.factor = "cs32"; M.gmres = true; M.gmres_left = true; M.apply64 = true; M.gmres_cycles = 1; }
+  else if (id == "native_gmresir32_app64_w64_r64") { M.factor = "cs32"; M.gmres = true; M.apply64 = true; }
+  else if (id == "hetero16_lu_direct") { M.kind = "direct"; M.factor = "het16"; M.working = 32; M.scale_mode = 1; M.max_outer = 0; }
+  else if (id == "hetero16_ir_w64_r64") { M.factor = "het16"; M.scale_mode = 1; }
+  else if (id == "hetero16_ir_w32_r64") { M.factor = "het16"; M.working = 32; M.scale_mode = 1; }
+  else if (id == "hetero16_ir_w64_r64_noscale") { M.factor = "het16"; M.scale_mode = 0; }
+  else if (id == "hetero16_ir_w64_r64_twosided") { M.factor = "het16"; M.scale_mode = 2; }
+  else if (id == "hetero16_gmresir_w64_r64") { M.factor = "het16"; M.scale_mode = 1; M.gmres = true; }
+  else if (id == "hetero16_gmresir_w32_r64") { M.factor = "het16"; M.working = 32; M.scale_mode = 1; M.gmres = true; }
+  else if (id == "hetero16_gmresir_app64_w64_r64") { M.factor = "het16"; M.scale_mode = 1; M.gmres = true; M.apply64 = true; }
+  else if (id == "hetero16_leftgmres_app64_w64_r64") { M.factor = "het16"; M.scale_mode = 1; M.gmres = true; M.gmres_left = true; M.apply64 = true; M.gmres_cycles = 1; }
+  else if (id == "custom32_lu_direct") { M.kind = "direct"; M.factor = "cus32"; M.working = 32; M.scale_mode = 1; M.max_outer = 0; }
+  else if (id == "custom32_ir_w64_r64") { M.factor = "cus32"; M.scale_mode = 1; }
+  else if (id == "custom32_ir_w32_r64") { M.factor = "cus32"; M.working = 32; M.scale_mode = 1; }
+  else if (id == "policy_hetero16_stationary_only") { M.factor = "het16"; M.scale_mode = 1; }
+  else if (id == "policy_hetero16_gmres_only") { M.factor = "het16"; M.scale_mode = 1; M.gmres = true; }
+  else if (id == "policy_hetero16_escalate_immediate") { M.factor = "het16"; M.scale_mode = 1; M.policy = "budget"; M.budget = 0; }
+  else if (starts("policy_hetero16_budget")) { M.factor = "het16"; M.scale_mode = 1; M.policy = "budget"; M.budget = std::stoi(id.substr(22)); M.gmres = true; M.gmres_budget = M.budget; }
+  else if (id == "policy_hetero16_cost_aware") { M.factor = "het16"; M.scale_mode = 1; M.policy = "cost_aware"; M.gmres_budget = a.geti("gmres_budget", 120); }
+  else if (id == "policy_native32_escalate_immediate") { M.factor = "cs32"; M.policy = "budget"; M.budget = 0; }
+  else if (starts("policy_native32_budget")) { M.factor = "cs32"; M.policy = "budget"; M.budget = std::stoi(id.substr(22)); M.gmres = true; M.gmres_budget = M.budget; }
+  else if (starts("cusolver_irs_")) { unsupported("cuSOLVER IRS is not available on this platform"); }
+  else if (starts("magma_")) { unsupported("MAGMA is not linked on this platform"); }
+  // ---- MI355X method IDs: per-tile power-of-two scaled low-precision trailing updates
+  //      mx{16,bf16,8}_{lu_direct | ir_w64_r64 | ir_w64_rdd | gmresir_w64_r64 | gmresir_w64_rdd}, policy_mx8_ladder
+  else if (starts("mx16_") || starts("mxbf16_") || starts("mx8_") || starts("mx32_")) {
+    const size_t us = id.find('_');
+    const std::string pre = id.substr(0, us), rest = id.substr(us + 1);
+    M.factor = (pre == "mx32") ? "cus32" : pre; M.scale_mode = 1;
+    if (rest == "lu_direct") { M.kind = "direct"; M.working = 32; M.max_outer = 0; }
+    else if (rest == "ir_w64_r64") {}
+    else if (rest == "ir_w64_rdd") { M.resid = 128; }
+    else if (rest == "gmresir_w64_r64") { M.gmres = true; }
+    else if (rest == "gmresir_w64_rdd") { M.gmres = true; M.resid = 128; }
+    else throw MPError("unknown method id " + id, -1);
+  }
+  else if (id == "policy_mx8_ladder") { M.factor = "mx8"; M.scale_mode = 1; M.gmres = true; M.resid = 128; M.policy = "ladder"; }
+  else throw MPError("unknown method id " + id, -1);
+  if (M.scale_mode == 2) unsupported("two-sided row/column scaling (scale_mode 2) is not implemented");
+  if (M.apply64) unsupported("promoted FP64 factor application (apply64) is not implemented");
+  if (M.gmres_left) unsupported("left-preconditioned GMRES is not implemented");
+  return M;
+}
+
+static const char* kMethodIds[] = {
+  "native_lu32", "native_lu64", "native_lu64_promoted32", "native_ir32_w32_r64", "native_ir32_w32_r32", "native_ir32_w64_r64",
+  "native_ir64_w64_r64", "native_ir32_w64_r64_twosided", "native_ir32_w64_r64_global", "native_gmresir32_w64_r64",
+  "native_gmresir32_w32_r64", "native_leftgmres32_app32_w64_r64", "native_leftgmres32_app64_w64_r64", "native_gmresir32_app64_w64_r64",
+  "hetero16_lu_direct", "hetero16_ir_w64_r64", "hetero16_ir_w32_r64", "hetero16_ir_w64_r64_noscale", "hetero16_ir_w64_r64_twosided",
+  "hetero16_gmresir_w64_r64", "hetero16_gmresir_w32_r64", "hetero16_gmresir_app64_w64_r64", "hetero16_leftgmres_app64_w64_r64",
+  "custom32_lu_direct", "custom32_ir_w64_r64", "custom32_ir_w32_r64", "policy_hetero16_stationary_only", "policy_hetero16_gmres_only",
+  "policy_hetero16_escalate_immediate", "policy_hetero16_budget120", "policy_hetero16_cost_aware", "policy_native32_escalate_immediate",
+  "policy_native32_budget120",
+  "mx16_lu_direct", "mx16_ir_w64_r64", "mx16_ir_w64_rdd", "mx16_gmresir_w64_r64", "mx16_gmresir_w64_rdd",
+  "mxbf16_lu_direct", "mxbf16_ir_w64_r64", "mxbf16_ir_w64_rdd", "mxbf16_gmresir_w64_r64", "mxbf16_gmresir_w64_rdd",
+  "mx8_lu_direct", "mx8_ir_w64_r64", "mx8_ir_w64_rdd", "mx8_gmresir_w64_r64", "mx8_gmresir_w64_rdd",
+  "mx32_ir_w64_rdd", "mx32_gmresir_w64_rdd", "policy_mx8_ladder"};
+
+static std::string factor_contract(const std::string& f) {
+  if (f == "cs32") return "vendor FP32 LU with partial pivoting (rocSOLVER sgetrf) on the FP32 copy of A";
+  if (f == "cs64") return "vendor FP64 LU with partial pivoting (rocSOLVER dgetrf) on an FP64 copy of A";
+  const std::string base = "own right-looking blocked LU (HIP, gfx950): FP32 storage; FP32 panel with partial pivoting "
+                           "(largest |a|, ties to the smallest row); FP32 laswp and unit-lower TRSM for U12; trailing update "
+                           "A22 -= L21*U12 on MFMA with FP32 accumulation and operands rounded (RNE) to ";
+  if (f == "het16") return base + "FP16 after the global power-of-two scale only (no per-tile scaling)";
+  if (f == "cus32") return base + "FP32 (v_mfma_f32_32x32x2_f32)";
+  if (f == "mx16") return base + "FP16 with a power-of-two scale per 32-row tile of L21 and per 32-column tile of U12 (tile max in [2^14, 2^15))";
+  if (f == "mxbf16") return base + "BF16 with a power-of-two scale per 32-row tile of L21 and per 32-column tile of U12";
+  if (f == "mx8") return base + "OCP FP8 E4M3 with a power-of-two scale per 32-row tile of L21 and per 32-column tile of U12 (tile max in [2^7, 2^8))";
+  return f;
+}
+
+// ------------------------------------------------------------------ factors
+struct Factors {
+  int64_t n = 0; std::string kind; int prec = 32;
+  DBuf<float> LU32; DBuf<double> LU64; DBuf<int64_t> ipiv; DBuf<int32_t> perm; DBuf<int32_t> ipiv32;
+  double gscale = 1.0; int gscale_exp = 0; int scale_mode = 0; int nb = 0;
+  int info = 0; double growth = -1; unsigned long long conv_overflow = 0, conv_subnormal = 0; int panels = 0;
+  bool valid = false;
+};
+
+static int fmt_of(const std::string& kind) {
+  if (kind == "het16" || kind == "mx16") return FMT_FP16;
+  if (kind == "mxbf16") return FMT_BF16;
+  if (kind == "mx8") return FMT_FP8;
+  return FMT_FP32;
+}
+
+static double dev_absmax(mp_ctx* C, const void* A, bool a32, int64_t n, int64_t lda) {
+  absmax(A, a32, n, lda, C->hpin, C->W);
+  HIP_CK(hipStreamSynchronize(C->W.stream));
+  return C->hpin[0];
+}
+
+// ipiv (1-based, sequential swaps) -> composed permutation, on the host
+static void perm_from_ipiv32(const std::vector<int>& ip, std::vector<int32_t>& perm) {
+  const int64_t n = (int64_t)ip.size();
+  perm.resize(n);
+  for (int64_t i = 0; i < n; i++) perm[i] = (int32_t)i;
+  for (int64_t i = 0; i < n; i++) { const int64_t p = ip[i] - 1; if (p != i) std::swap(perm[i], perm[p]); }
+}
+
+static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, int nb, int scale_mode, Factors& F) {
+  Workspace& W = C->W;
+  const int64_t n = P.n; const size_t nn = (size_t)n * n;
+  F.n = n; F.kind = kind; F.scale_mode = scale_mode; F.nb = nb; F.valid = false; F.growth = -1;
+  F.conv_overflow = F.conv_subnormal = 0; F.info = 0; F.panels = 0;
+  F.prec = (kind == "cs64") ? 64 : 32;
+  F.ipiv.ensure(C->tracker, n, "ipiv");
+  F.perm.ensure(C->tracker, n, "perm");
+  F.gscale = 1.0; F.gscale_exp = 0;
+  if (scale_mode >= 1 && P.amax > 0 && std::isfinite(P.amax)) {
+    int e; std::frexp(P.amax, &e); F.gscale_exp = -e; F.gscale = std::ldexp(1.0, -e);
+  }
+  double amax_f = P.amax * F.gscale;
+  if (F.prec == 32) {
+    F.LU32.ensure(C->tracker, nn, "LU32");
+    if (P.fp32) {
+      HIP_CK(hipMemcpyAsync(F.LU32.p, P.A32.p, nn * 4, hipMemcpyDeviceToDevice, W.stream));
+      MP_REQUIRE(F.gscale == 1.0, "global scaling of FP32-stored problems is not implemented");
+    } else {
+      scale_convert_f64_to_f32(P.A64.p, n, n, F.LU32.p, n, F.gscale, W);
+    }
+  } else {
+    F.LU64.ensure(C->tracker, nn, "LU64");
+    MP_REQUIRE(!P.fp32, "FP64 factors of FP32-stored problems are not implemented");
+    HIP_CK(hipMemcpyAsync(F.LU64.p, P.A64.p, nn * 8, hipMemcpyDeviceToDevice, W.stream));
+    MP_REQUIRE(F.gscale == 1.0, "global scaling with FP64 vendor factors is not implemented");
+  }
+  if (kind == "cs32" || kind == "cs64") {
+    F.ipiv32.ensure(C->tracker, n + 1, "ipiv32");
+    RB_CK(rocblas_set_stream(C->rb, W.stream));
+    if (kind == "cs32") RB_CK(rocsolver_sgetrf(C->rb, (rocblas_int)n, (rocblas_int)n, F.LU32.p, (rocblas_int)n, F.ipiv32.p, F.ipiv32.p + n));
+    else RB_CK(rocsolver_dgetrf(C->rb, (rocblas_int)n, (rocblas_int)n, F.LU64.p, (rocblas_int)n, F.ipiv32.p, F.ipiv32.p + n));
+    std::vector<int> ip(n + 1);
+    HIP_CK(hipMemcpyAsync(ip.data(), F.ipiv32.p, (n + 1) * sizeof(int), hipMemcpyDeviceToHost, W.stream));
+    HIP_CK(hipStreamSynchronize(W.stream));
+    F.info = ip[n];
+    ip.resize(n);
+    std::vector<int32_t> perm; perm_from_ipiv32(ip, perm);
+    std::vector<int64_t> ip64(ip.begin(), ip.end());
+    HIP_CK(hipMemcpyAsync(F.perm.p, perm.data(), n * 4, hipMemcpyHostToDevice, W.stream));
+    HIP_CK(hipMemcpyAsync(F.ipiv.p, ip64.data(), n * 8, hipMemcpyHostToDevice, W.stream));
+  } else {
+    GetrfArgs ga; ga.n = n; ga.A = F.LU32.p; ga.lda = n; ga.nb = nb; ga.fmt = fmt_of(kind);
+    ga.tile_scale = (kind != "het16"); ga.ipiv = F.ipiv.p; ga.perm = F.perm.p;
+    GetrfInfo gi;
+    getrf(ga, W, &gi);
+    F.info = gi.info; F.conv_overflow = gi.overflow; F.conv_subnormal = gi.subnormal; F.panels = gi.panels;
+  }
+  const double gm = (F.prec == 32) ? dev_absmax(C, F.LU32.p, true, n, n) : dev_absmax(C, F.LU64.p, false, n, n);
+  F.growth = (amax_f > 0) ? gm / amax_f : -1;
+  F.valid = true;
+}
+
+// d = A^{-1} r through the factors (r, d FP64 device, n x nrhs, leading dimension n)
+static void apply_inverse(mp_ctx* C, const Factors& F, const double* r, double* d, int nrhs, int* trs) {
+  Workspace& W = C->W;
+  const int64_t n = F.n;
+  if (F.prec == 64) {
+    if (d != r) HIP_CK(hipMemcpyAsync(d, r, (size_t)n * nrhs * 8, hipMemcpyDeviceToDevice, W.stream));
+    RB_CK(rocblas_set_stream(C->rb, W.stream));
+    RB_CK(rocsolver_dgetrs(C->rb, rocblas_operation_none, (rocblas_int)n, (rocblas_int)nrhs, const_cast<double*>(F.LU64.p),
+                           (rocblas_int)n, F.ipiv32.p, d, (rocblas_int)n));
+  } else {
+    getrs(n, nrhs, F.LU32.p, n, F.perm.p, r, n, d, n, F.gscale, W);
+  }
+  if (trs) (*trs)++;
+}
+
+// ------------------------------------------------------------------ metrics
+struct Metrics {
+  std::vector<double> rinf, r2, eta, omega, relres_inf, relres_2, xinf; std::vector<int64_t> worst;
+  bool finite = true; double eta_max = 0, omega_max = 0;
+  bool has_fwd = false, has_exact = false;
+  std::vector<double> finf, f2, fcomp, fzero_maxabs, einf, e2, ecomp, ezero_maxabs;
+  std::vector<int64_t> fzero_count, ezero_count;
+};
+
+struct EvalBufs { DBuf<double> R, Rlo, T, stats; DBuf<int32_t> worst; };
+
+// Residual of the stored system for device X, FP64 or double-double, and the per-RHS metrics
+// (eval_metrics_T, src/mp_core.cuh:193-205).  R receives the FP64 (rounded) residual.
+static void residual_metrics(mp_ctx* C, const mp_problem& P, const double* X, bool dd, EvalBufs& E, Metrics& M) {
+  Workspace& W = C->W;
+  const int64_t n = P.n; const int nrhs = P.nrhs;
+  E.R.ensure(C->tracker, (size_t)n * nrhs, "R"); E.T.ensure(C->tracker, (size_t)n * nrhs, "T");
+  E.Rlo.ensure(C->tracker, (size_t)n * nrhs, "Rlo");
+  E.stats.ensure(C->tracker, (size_t)4 * nrhs + 8, "stats"); E.worst.ensure(C->tracker, (size_t)nrhs + 1, "worst");
+  ResidArgs ra; ra.n = n; ra.nrhs = nrhs; ra.a32 = P.fp32; ra.A = P.a_ptr(); ra.lda = n; ra.X = X; ra.ldx = n;
+  ra.B = P.B.p; ra.ldb = n; ra.R = E.R.p; ra.Rlo = dd ? E.Rlo.p : nullptr; ra.ldr = n; ra.T = E.T.p; ra.ldt = n;
+  ra.stats = E.stats.p; ra.worst = E.worst.p; ra.dd = dd;
+  residual(ra, W);
+  std::vector<double> st(4 * nrhs); std::vector<int32_t> wi(nrhs);
+  HIP_CK(hipMemcpyAsync(st.data(), E.stats.p, 32 * nrhs, hipMemcpyDeviceToHost, W.stream));
+  HIP_CK(hipMemcpyAsync(wi.data(), E.worst.p, 4 * nrhs, hipMemcpyDeviceToHost, W.stream));
+  HIP_CK(hipStreamSynchronize(W.stream));
+  M.rinf.assign(nrhs, 0); M.r2.assign(nrhs, 0); M.eta.assign(nrhs, 0); M.omega.assign(nrhs, 0);
+  M.relres_inf.assign(nrhs, 0); M.relres_2.assign(nrhs, 0); M.xinf.assign(nrhs, 0); M.worst.assign(nrhs, -1);
+  M.finite = true; M.eta_max = 0; M.omega_max = 0;
+  for (int c = 0; c < nrhs; c++) {
+    const double ri = st[4 * c], xi = st[4 * c + 1], om = st[4 * c + 2], r2 = std::sqrt(st[4 * c + 3]);
+    M.rinf[c] = ri; M.r2[c] = r2; M.xinf[c] = xi; M.worst[c] = wi[c];
+    const double den = P.anorm_inf * xi + P.bnorm_inf[c];
+    M.eta[c] = (ri == 0) ? 0.0 : ((den == 0) ? INFINITY : ri / den);
+    M.omega[c] = om;
+    M.relres_inf[c] = (ri == 0) ? 0.0 : ((P.bnorm_inf[c] == 0) ? INFINITY : ri / P.bnorm_inf[c]);
+    M.relres_2[c] = (r2 == 0) ? 0.0 : ((P.bnorm_2[c] == 0) ? INFINITY : r2 / P.bnorm_2[c]);
+    if (!std::isfinite(ri) || !std::isfinite(xi) || !std::isfinite(om)) M.finite = false;
+    M.eta_max = std::max(M.eta_max, M.eta[c]); M.omega_max = std::max(M.omega_max, M.omega[c]);
+    if (!std::isfinite(M.eta[c])) M.eta_max = M.eta[c];
+  }
+}
+
+// forward-error statistics vs a host reference (k_fwdstats, src/mp_kernels.cuh:98-102), host side
+static void fwd_stats(const std::vector<double>& X, const std::vector<double>& XR, int64_t n, int nrhs,
+                      std::vector<double>& finf, std::vector<double>& f2, std::vector<double>& fcomp,
+                      std::vector<double>& fz, std::vector<int64_t>& fzc) {
+  finf.assign(nrhs, 0); f2.assign(nrhs, 0); fcomp.assign(nrhs, 0); fz.assign(nrhs, 0); fzc.assign(nrhs, 0);
+  for (int c = 0; c < nrhs; c++) {
+    double m0 = 0, m1 = 0, s2 = 0, s3 = 0, m4 = 0, m5 = 0; int64_t zc = 0;
+    for (int64_t i = 0; i < n; i++) {
+      const double x = X[i + c * n], xr = XR[i + c * n], e = std::fabs(x - xr), a = std::fabs(xr);
+      if (!(e <= m0)) m0 = e;
+      if (a > m1) m1 = a;
+      s2 += e * e; s3 += xr * xr;
+      if (xr != 0.0) { const double q = e / a; if (!(q <= m4)) m4 = q; } else { zc++; if (!(e <= m5)) m5 = e; }
+    }
+    const double e2 = std::sqrt(s2), x2 = std::sqrt(s3);
+    finf[c] = (m1 > 0) ? m0 / m1 : (m0 == 0 ? 0 : INFINITY);
+    f2[c] = (x2 > 0) ? e2 / x2 : (e2 == 0 ? 0 : INFINITY);
+    fcomp[c] = m4; fz[c] = m5; fzc[c] = zc;
+  }
+}
+
+// ------------------------------------------------------------------ trial output and engine
+struct HistRec { int outer, stage; double t, eta_max, omega_max, rinf_max; int inner_in_step; std::string note; };
+struct InnerRec { int outer, cycle, j, rhs; double rec_rel, true_rel; std::string note; };
+struct TrialOut {
+  std::string terminal = "unknown", final_stage; bool fallback = false; int refactorizations = 0;
+  std::vector<std::string> stages; std::vector<double> stage_times;
+  int outer = 0, inner_total = 0, matvecs = 0, precond_applies = 0, trs = 0, restarts = 0, true_resid_checks = 0;
+  double t_total = 0, t_factor_total = 0, t_initial = 0, t_refine = 0, t_eval = 0;
+  std::vector<HistRec> hist; std::vector<InnerRec> ihist;
+  double growth = -1; unsigned long long conv_overflow = 0, conv_subnormal = 0; int factor_info = 0;
+  std::string error; bool ok = true; double final_eta = 0, final_omega = 0;
+};
+
+struct Engine {
+  mp_ctx* C; const mp_problem& P; Factors fact; bool reuse = false; bool instrument = false;
+  int64_t n; int nrhs; double tau_eta, tau_omega; bool stop_eta_only = false;
+  DBuf<double> x64, D, y; DBuf<float> x32;
+  EvalBufs E;
+  Engine(mp_ctx* c, const mp_problem& p, double te, double to) : C(c), P(p), n(p.n), nrhs(p.nrhs), tau_eta(te), tau_omega(to) {
+    x64.alloc(C->tracker, (size_t)n * nrhs, "x64");
+    D.alloc(C->tracker, (size_t)n * nrhs, "D");
+  }
+  Workspace& W() { return C->W; }
+  void sync() { HIP_CK(hipStreamSynchronize(C->W.stream)); }
+
+  Factors& factors_for(const MethodSpec& M, const std::string& kind, TrialOut& T) {
+    Factors& F = fact;
+    const double t0 = wall_now();
+    if (reuse && F.valid && F.kind == kind && F.scale_mode == M.scale_mode && F.nb == M.nb) {
+      T.stages.push_back(kind + "(reused)"); T.stage_times.push_back(0.0); T.growth = F.growth; T.factor_info = F.info; return F;
+    }
+    factorize(C, P, kind, M.nb, M.scale_mode, F);
+    sync();
+    const double dt = wall_now() - t0;
+    T.t_factor_total += dt; T.stages.push_back(kind); T.stage_times.push_back(dt); T.refactorizations++;
+    T.growth = F.growth; T.conv_overflow = F.conv_overflow; T.conv_subnormal = F.conv_subnormal; T.factor_info = F.info;
+    return F;
+  }
+
+  // x (working precision) += d
+  void add_update(const MethodSpec& M, const double* d) {
+    if (M.working == 32) {
+      // x32 += d rounded to FP32, then refresh the FP64 copy used by the residual
+      hipLaunchKernelGGL(k_add_x32, dim3(cdiv((size_t)n * nrhs, 256)), dim3(256), 0, W().stream, x32.p, x64.p, d, (int64_t)n * nrhs);
+      LAUNCH_CK();
+    } else {
+      axpy(x64.p, d, (int64_t)n * nrhs, 1.0, W());
+    }
+  }
+  void set_x(const MethodSpec& M, const double* d) {
+    if (M.working == 32) {
+      x32.ensure(C->tracker, (size_t)n * nrhs, "x32");
+      hipLaunchKernelGGL(k_set_x32, dim3(cdiv((size_t)n * nrhs, 256)), dim3(256), 0, W().stream, x32.p, x64.p, d, (int64_t)n * nrhs);
+      LAUNCH_CK();
+    } else {
+      copy_f64(d, x64.p, (int64_t)n * nrhs, W());
+    }
+  }
+
+  // FGMRES (right, flexible) on A d = r for one RHS column (src/mp_core.cuh:237-278)
+  std::string gmres_solve(const MethodSpec& M, const Factors& F, const double* r, double* d, int& budget_left, TrialOut& T, int outer, int rhs) {
+    if (!C->kry) { C->kry.reset(new Krylov()); C->kry->W = &C->W; }
+    Krylov& K = *C->kry;
+    const int m = M.gmres_m;
+    K.ensure(n, m);
+    double* hp = C->hpin;
+    fill_f64(d, n, 0.0, W());
+    copy_f64(r, K.rcur.p, n, W());
+    K.norm(r, K.scal());
+    HIP_CK(hipMemcpyAsync(hp, K.scal(), 8, hipMemcpyDeviceToHost, W().stream));
+    sync();
+    const double beta0 = hp[0];
+    if (beta0 == 0.0 || !std::isfinite(beta0)) return beta0 == 0.0 ? "zero_rhs" : "nonfinite";
+    std::string status = "budget";
+    int cycles = 0;
+    while (cycles < M.gmres_cycles && budget_left > 0) {
+      cycles++; if (cycles > 1) T.restarts++;
+      K.norm(K.rcur.p, K.scal());
+      HIP_CK(hipMemcpyAsync(hp, K.scal(), 8, hipMemcpyDeviceToHost, W().stream));
+      sync();
+      const double betac = hp[0];
+      if (betac == 0) { status = "converged_true"; break; }
+      K.scale_copy(K.rcur.p, K.V.p, K.scal(), true);   // v0 = rcur / betac
+      K.init_cycle(betac);
+      int j = 0; bool happy = false, nonfinite = false;
+      for (j = 0; j < m && budget_left > 0; j++) {
+        double* vj = K.V.p + (int64_t)j * n;
+        double* zj = K.Z.p + (int64_t)j * n;
+        apply_inverse(C, F, vj, zj, 1, &T.trs); T.precond_applies++;
+        // w = A z_j (FP64)
+        ResidArgs ra; ra.n = n; ra.nrhs = 1; ra.a32 = P.fp32; ra.A = P.a_ptr(); ra.lda = n; ra.X = zj; ra.ldx = n;
+        ra.R = K.w.p; ra.ldr = n; ra.matvec = true;
+        residual(ra, W()); T.matvecs++;
+        // CGS2
+        K.dots(K.V.p, j + 1, K.w.p, K.h(), false);
+        K.update(K.V.p, j + 1, K.h(), K.w.p, -1.0);
+        K.dots(K.V.p, j + 1, K.w.p, K.h2(), false);
+        K.update(K.V.p, j + 1, K.h2(), K.w.p, -1.0);
+        hipLaunchKernelGGL(k_add_vec, dim3(cdiv(j + 1, 64)), dim3(64), 0, W().stream, K.h(), K.h2(), j + 1);
+        LAUNCH_CK();
+        K.norm(K.w.p, K.scal());
+        K.givens(j, K.h(), K.scal(), beta0, K.scal() + 2);
+        // v_{j+1} = w / hn  (skipped by the kernel reading hn == 0 would divide by zero: guard on host below)
+        HIP_CK(hipMemcpyAsync(hp, K.scal() + 2, 24, hipMemcpyDeviceToHost, W().stream));
+        sync();
+        const double rec = hp[0], hn = hp[1];
+        const bool fin = hp[2] != 0.0;
+        if (!fin) { nonfinite = true; break; }
+        if (hn > 0) K.scale_copy(K.w.p, K.V.p + (int64_t)(j + 1) * n, K.scal(), true);
+        T.inner_total++; budget_left--;
+        if (instrument) T.ihist.push_back({outer, cycles, j, rhs, rec, -1, ""});
+        if (hn == 0.0) { happy = true; j++; break; }
+        if (rec <= M.gmres_tol) { j++; break; }
+      }
+      if (nonfinite) { status = "nonfinite"; break; }
+      const int k = j;
+      if (k == 0) break;
+      K.lsq(k, K.y());
+      K.update(K.Z.p, k, K.y(), d, 1.0);
+      // true residual of the correction equation: sR = r - A d (FP64)
+      ResidArgs ra; ra.n = n; ra.nrhs = 1; ra.a32 = P.fp32; ra.A = P.a_ptr(); ra.lda = n; ra.X = d; ra.ldx = n;
+      ra.B = r; ra.ldb = n; ra.R = K.sR.p; ra.ldr = n;
+      residual(ra, W()); T.matvecs++; T.true_resid_checks++;
+      K.norm(K.sR.p, K.scal());
+      K.norm(r, K.scal() + 1);
+      HIP_CK(hipMemcpyAsync(hp, K.scal(), 16, hipMemcpyDeviceToHost, W().stream));
+      HIP_CK(hipMemcpyAsync(hp + 4, K.scal() + 2, 8, hipMemcpyDeviceToHost, W().stream));
+      sync();
+      const double tn = hp[0], rn0 = hp[1];
+      const double true_rel = (rn0 > 0) ? tn / rn0 : 0.0;
+      if (instrument) T.ihist.push_back({outer, cycles, k - 1, rhs, hp[4], true_rel, "cycle_end"});
+      copy_f64(K.sR.p, K.rcur.p, n, W());
+      if (!std::isfinite(tn)) { status = "nonfinite"; break; }
+      if (true_rel <= M.gmres_tol) { status = happy ? "happy_breakdown_verified" : "converged_true"; break; }
+      if (happy) { status = "happy_breakdown_unverified"; break; }
+    }
+    return status;
+  }
+
+  void residual_and_metrics(const MethodSpec& M, Metrics& Mt) {
+    if (M.resid == 32) throw MPError("fp32 residual control requires fp32 stored data", -4);
+    residual_metrics(C, P, x64.p, M.resid == 128, E, Mt);
+  }
+
+  std::string refine_stage(const MethodSpec& M, const Factors& F, int stage_idx, TrialOut& T, double t_start, int& budget_left,
+                           bool initial_solve, Metrics& Mt) {
+    if (initial_solve) {
+      const double t0 = wall_now();
+      apply_inverse(C, F, P.B.p, D.p, nrhs, &T.trs);
+      set_x(M, D.p);
+      sync();
+      T.t_initial += wall_now() - t0;
+    }
+    double prev_eta = INFINITY; int stall = 0; std::string reason = "max_outer";
+    while (true) {
+      const double te0 = wall_now();
+      residual_and_metrics(M, Mt);
+      T.t_eval += wall_now() - te0;
+      T.final_eta = Mt.eta_max; T.final_omega = Mt.omega_max;
+      const int inner_before = T.inner_total;
+      T.hist.push_back({T.outer, stage_idx, wall_now() - t_start, Mt.eta_max, Mt.omega_max,
+                        *std::max_element(Mt.rinf.begin(), Mt.rinf.end()), 0, ""});
+      const bool conv = Mt.finite && Mt.eta_max <= tau_eta && (stop_eta_only || Mt.omega_max <= tau_omega);
+      if (conv) { reason = "converged"; break; }
+      if (!Mt.finite) { reason = "nonfinite"; break; }
+      if (T.outer >= M.max_outer) { reason = "max_outer"; break; }
+      if (Mt.eta_max > 0.9 * prev_eta) stall++; else stall = 0;
+      if (stall >= 3) { reason = "stagnation"; break; }
+      prev_eta = Mt.eta_max;
+      if (M.gmres && budget_left <= 0) { reason = "gmres_budget_exhausted"; break; }
+      if (M.gmres) {
+        for (int c = 0; c < nrhs; c++) {
+          const std::string st = gmres_solve(M, F, E.R.p + (int64_t)c * n, D.p + (int64_t)c * n, budget_left, T, T.outer, c);
+          T.hist.back().note += (c ? ";" : "") + st;
+          if (st == "nonfinite") T.hist.back().note += ";inner_nonfinite";
+        }
+      } else {
+        apply_inverse(C, F, E.R.p, D.p, nrhs, &T.trs); T.precond_applies++;
+      }
+      T.hist.back().inner_in_step = T.inner_total - inner_before;
+      add_update(M, D.p);
+      T.outer++;
+    }
+    return reason;
+  }
+
+  void run(const MethodSpec& M, TrialOut& T) {
+    const double t0 = wall_now();
+    T.outer = 0; T.inner_total = 0;
+    int budget_left = M.gmres ? M.gmres_budget : 0;
+    std::vector<std::string> ladder{M.factor};
+    const bool target64 = (tau_eta < 1e-10);
+    if (M.policy == "ladder") { ladder = {"mx8", "mx16", "cus32"}; }
+    else if (M.policy != "none") {
+      if (M.factor == "het16" || M.factor == "cus32") { ladder.push_back("cs32"); if (target64) ladder.push_back("cs64"); }
+      else if (M.factor == "cs32") { if (target64) ladder.push_back("cs64"); }
+    }
+    fill_f64(x64.p, (int64_t)n * nrhs, 0.0, W());
+    if (M.working == 32) { x32.ensure(C->tracker, (size_t)n * nrhs, "x32"); HIP_CK(hipMemsetAsync(x32.p, 0, (size_t)n * nrhs * 4, W().stream)); }
+    Metrics Mt; std::string reason = "unknown"; MethodSpec Ms = M;
+    bool initialized = false;
+    for (int stage = 0; stage < (int)ladder.size(); stage++) {
+      Factors& F = factors_for(M, ladder[stage], T);
+      T.final_stage = ladder[stage];
+      if (stage > 0) T.fallback = true;
+      if (F.info > 0) {
+        reason = "singular_factor";
+        T.hist.push_back({T.outer, stage, wall_now() - t0, -1, -1, -1, 0, "singular_factor_info=" + std::to_string(F.info)});
+        if (M.policy == "none") break; else continue;
+      }
+      if (M.policy == "budget" && M.budget == 0) Ms.gmres = false;
+      if (M.policy == "budget" && M.budget > 0) { if (stage == 0) { Ms.gmres = true; budget_left = M.budget; } else Ms.gmres = false; }
+      if (M.policy == "cost_aware") Ms.gmres = false;
+      if (M.policy == "ladder") { Ms.gmres = true; budget_left = M.gmres_budget; }
+      const double ts = wall_now();
+      reason = refine_stage(Ms, F, stage, T, t0, budget_left, !initialized, Mt);
+      initialized = true;
+      T.t_refine += wall_now() - ts;
+      if (M.policy == "cost_aware" && reason != "converged" && reason != "nonfinite" && M.gmres_budget > 0 && stage == 0) {
+        const double c_outer = T.hist.size() >= 2 ? (T.hist.back().t - T.hist[T.hist.size() - 2].t) : 0.0;
+        const double c_refactor_next = T.stage_times.back() * M.cost_ratio_next;
+        const double predicted = c_outer * (double)std::min(M.gmres_budget, M.gmres_m) * 2.0;
+        T.hist.back().note += ";cost_aware:c_outer=" + std::to_string(c_outer) + ",c_refactor_next=" + std::to_string(c_refactor_next) +
+                              ",pred_gmres=" + std::to_string(predicted);
+        if (predicted < c_refactor_next) {
+          Ms.gmres = true; budget_left = M.gmres_budget; T.hist.back().note += ";switch_to_gmres";
+          reason = refine_stage(Ms, F, stage, T, t0, budget_left, false, Mt);
+          Ms.gmres = false;
+        }
+      }
+      if (reason == "converged") break;
+      if (M.policy == "none") break;
+      if (stage + 1 < (int)ladder.size())
+        T.hist.push_back({T.outer, stage, wall_now() - t0, Mt.eta_max, Mt.omega_max, -1, 0, "escalate_after:" + reason});
+    }
+    T.terminal = (M.kind == "direct" && reason != "nonfinite" && reason != "singular_factor") ? "direct_solve" : reason;
+    sync();
+    T.t_total = wall_now() - t0;
+  }
+};
+
+// ------------------------------------------------------------------ ABI helpers
+static void copy_str(const std::string& s, char* buf, size_t cap) {
+  if (!buf || cap == 0) return;
+  const size_t k = std::min(cap - 1, s.size());
+  std::memcpy(buf, s.data(), k); buf[k] = 0;
+  if (s.size() + 1 > cap) throw MPError("output buffer too small (need " + std::to_string(s.size() + 1) + " bytes)", -1);
+}
+
+extern "C" {
+
+const char* mp_last_error(void) { return g_last_error.c_str(); }
+
+int mp_create(mp_ctx** out, int device) {
+  return guard([&] {
+    MP_REQUIRE(out, "ctx pointer is NULL");
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) { (void)hipGetLastError(); throw MPError("no HIP device available", MP_ERR_NO_DEVICE); }
+    MP_REQUIRE(device >= 0 && device < cnt, "device index out of range");
+    HIP_CK(hipSetDevice(device));
+    std::unique_ptr<mp_ctx> C(new mp_ctx());
+    C->device = device;
+    HIP_CK(hipStreamCreateWithFlags(&C->W.stream, hipStreamNonBlocking));
+    C->W.tracker = &C->tracker;
+    hipDeviceProp_t prop; HIP_CK(hipGetDeviceProperties(&prop, device));
+    C->W.num_cus = prop.multiProcessorCount;
+    RB_CK(rocblas_create_handle(&C->rb));
+    RB_CK(rocblas_set_stream(C->rb, C->W.stream));
+    HIP_CK(hipHostMalloc((void**)&C->hpin, 4096 * sizeof(double), hipHostMallocDefault));
+    C->W.dinfo.ensure(C->tracker, 4, "getrf_info");
+    HIP_CK(hipMemsetAsync(C->W.dinfo.p, 0, 16, C->W.stream));
+    HIP_CK(hipStreamSynchronize(C->W.stream));
+    *out = C.release();
+  });
+}
+
+void mp_destroy(mp_ctx* C) {
+  if (!C) return;
+  (void)hipSetDevice(C->device);
+  (void)hipStreamSynchronize(C->W.stream);
+  C->kry.reset();
+  if (C->rb) rocblas_destroy_handle(C->rb);
+  if (C->hpin) (void)hipHostFree(C->hpin);
+  hipStream_t s = C->W.stream;
+  delete C;
+  if (s) (void)hipStreamDestroy(s);
+}
+
+int mp_caps(mp_ctx* C, char* json, size_t cap) {
+  return guard([&] {
+    MP_REQUIRE(C, "ctx is NULL");
+    hipDeviceProp_t p; HIP_CK(hipGetDeviceProperties(&p, C->device));
+    size_t fr = 0, tot = 0; HIP_CK(hipMemGetInfo(&fr, &tot));
+    int rt = 0, drv = 0; (void)hipRuntimeGetVersion(&rt); (void)hipDriverGetVersion(&drv);
+    char rbv[128] = {0}; rocblas_get_version_string(rbv, sizeof rbv);
+    Json j;
+    j.s("device", p.name).s("gcn_arch", p.gcnArchName).i("total_mem_bytes", (int64_t)tot).i("free_mem_bytes", (int64_t)fr)
+     .i("hip_runtime", rt).i("hip_driver", drv).s("rocblas_version", rbv).i("multiprocessors", p.multiProcessorCount)
+     .i("clock_khz", p.clockRate).i("mem_clock_khz", p.memoryClockRate).i("l2_bytes", p.l2CacheSize)
+     .i("lds_per_block_bytes", (int64_t)p.sharedMemPerBlock).b("magma_linked", false);
+    copy_str(j.obj(), json, cap);
+  });
+}
+
+int mp_profile(mp_ctx* C, int enable) {
+  return guard([&] { MP_REQUIRE(C, "ctx is NULL"); C->W.prof.drain(); C->W.prof.on = enable != 0; if (enable) C->W.prof.acc.clear(); });
+}
+int mp_profile_json(mp_ctx* C, char* json, size_t cap) {
+  return guard([&] {
+    MP_REQUIRE(C, "ctx is NULL");
+    HIP_CK(hipStreamSynchronize(C->W.stream));
+    C->W.prof.drain();
+    std::string out = "{"; bool first = true;
+    for (auto& kv : C->W.prof.acc) {
+      Json j; j.d("ms", kv.second.ms).i("count", kv.second.count).d("bytes", kv.second.bytes).d("flops", kv.second.flops);
+      out += (first ? "" : ",") + Json::str(kv.first) + ":" + j.obj(); first = false;
+    }
+    copy_str(out + "}", json, cap);
+  });
+}
+
+int mp_synchronize(mp_ctx* C) { return guard([&] { MP_REQUIRE(C, "ctx is NULL"); HIP_CK(hipStreamSynchronize(C->W.stream)); }); }
+int mp_malloc(mp_ctx* C, void** p, size_t bytes) { return guard([&] { MP_REQUIRE(C && p, "NULL argument"); *p = C->tracker.alloc(bytes, "user"); }); }
+int mp_free(mp_ctx* C, void* p) { return guard([&] { MP_REQUIRE(C, "ctx is NULL"); HIP_CK(hipStreamSynchronize(C->W.stream)); C->tracker.free(p); }); }
+int mp_memcpy_h2d(mp_ctx* C, void* d, const void* s, size_t b) { return guard([&] { MP_REQUIRE(C, "ctx is NULL"); HIP_CK(hipMemcpyAsync(d, s, b, hipMemcpyHostToDevice, C->W.stream)); HIP_CK(hipStreamSynchronize(C->W.stream)); }); }
+int mp_memcpy_d2h(mp_ctx* C, void* d, const void* s, size_t b) { return guard([&] { MP_REQUIRE(C, "ctx is NULL"); HIP_CK(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToHost, C->W.stream)); HIP_CK(hipStreamSynchronize(C->W.stream)); }); }
+int mp_memcpy_d2d(mp_ctx* C, void* d, const void* s, size_t b) { return guard([&] { MP_REQUIRE(C, "ctx is NULL"); HIP_CK(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, C->W.stream)); }); }
+
+int mp_residual_fp64(mp_ctx* C, int64_t n, int32_t nrhs, int32_t a_bits, const void* A, int64_t lda, const double* X, int64_t ldx,
+                     const double* B, int64_t ldb, double* R, int64_t ldr, double* T, int64_t ldt, double* colstats, int32_t* worst) {
+  return guard([&] {
+    MP_REQUIRE(C, "ctx is NULL"); MP_REQUIRE(a_bits == 32 || a_bits == 64, "a_bits must be 32 or 64");
+    ResidArgs ra; ra.n = n; ra.nrhs = nrhs; ra.a32 = (a_bits == 32); ra.A = A; ra.lda = lda; ra.X = X; ra.ldx = ldx;
+    ra.B = B; ra.ldb = B ? ldb : n; ra.R = R; ra.ldr = ldr; ra.T = T; ra.ldt = T ? ldt : n; ra.stats = colstats; ra.worst = worst;
+    residual(ra, C->W);
+  });
+}
+
+int mp_residual_dd(mp_ctx* C, int64_t n, int32_t nrhs, int32_t a_bits, const void* A, int64_t lda, const double* X, int64_t ldx,
+                   const double* B, int64_t ldb, double* Rhi, double* Rlo, int64_t ldr, double* T, int64_t ldt, double* colstats, int32_t* worst) {
+  return guard([&] {
+    MP_REQUIRE(C, "ctx is NULL"); MP_REQUIRE(a_bits == 32 || a_bits == 64, "a_bits must be 32 or 64");
+    ResidArgs ra; ra.n = n; ra.nrhs = nrhs; ra.a32 = (a_bits == 32); ra.A = A; ra.lda = lda; ra.X = X; ra.ldx = ldx;
+    ra.B = B; ra.ldb = B ? ldb : n; ra.R = Rhi; ra.Rlo = Rlo; ra.ldr = ldr; ra.T = T; ra.ldt = T ? ldt : n;
+    ra.stats = colstats; ra.worst = worst; ra.dd = true;
+    residual(ra, C->W);
+  });
+}
+
+int mp_getrf(mp_ctx* C, int64_t n, float* LU, int64_t lda, int64_t* ipiv, int32_t* perm, const mp_getrf_params* prm, mp_getrf_info* info) {
+  return guard([&] {
+    MP_REQUIRE(C && prm, "NULL argument");
+    GetrfArgs ga; ga.n = n; ga.A = LU; ga.lda = lda; ga.nb = prm->nb; ga.fmt = prm->fmt; ga.tile_scale = prm->tile_scale != 0;
+    ga.ipiv = ipiv; ga.perm = perm;
+    GetrfInfo gi; getrf(ga, C->W, &gi);
+    if (info) { info->info = gi.info; info->panels = gi.panels; info->conv_overflow = gi.overflow; info->conv_subnormal = gi.subnormal; }
+  });
+}
+
+int mp_getrs(mp_ctx* C, int64_t n, int32_t nrhs, const float* LU, int64_t lda, const int32_t* perm, const double* R, int64_t ldr,
+             double* D, int64_t ldd, double scale) {
+  return guard([&] { MP_REQUIRE(C, "ctx is NULL"); getrs(n, nrhs, LU, lda, perm, R, ldr, D, ldd, scale, C->W); });
+}
+
+int mp_problem_from_host(mp_ctx* C, int64_t n, int32_t nrhs, int32_t a_bits, const void* A, int64_t lda, const double* B, int64_t ldb,
+                         const double* x_exact, const double* x_ref, const char* case_json, mp_problem** out) {
+  return guard([&] {
+    MP_REQUIRE(C && A && B && out, "NULL argument");
+    MP_REQUIRE(n > 0 && nrhs > 0 && lda >= n && ldb >= n, "bad sizes");
+    MP_REQUIRE(a_bits == 32 || a_bits == 64, "a_bits must be 32 or 64");
+    std::unique_ptr<mp_problem> P(new mp_problem());
+    P->ctx = C; P->n = n; P->nrhs = nrhs; P->fp32 = (a_bits == 32);
+    const size_t es = P->fp32 ? 4 : 8;
+    std::vector<char> pack;
+    const char* src = (const char*)A;
+    if (lda != n) {
+      pack.resize((size_t)n * n * es);
+      for (int64_t j = 0; j < n; j++) std::memcpy(pack.data() + (size_t)j * n * es, src + (size_t)j * lda * es, n * es);
+      src = pack.data();
+    }
+    if (P->fp32) { P->A32.alloc(C->tracker, (size_t)n * n, "A32"); HIP_CK(hipMemcpy(P->A32.p, src, (size_t)n * n * 4, hipMemcpyHostToDevice)); }
+    else { P->A64.alloc(C->tracker, (size_t)n * n, "A64"); HIP_CK(hipMemcpy(P->A64.p, src, (size_t)n * n * 8, hipMemcpyHostToDevice)); }
+    std::vector<double> hb((size_t)n * nrhs);
+    for (int c = 0; c < nrhs; c++) std::memcpy(hb.data() + (size_t)c * n, B + (size_t)c * ldb, n * 8);
+    P->B.alloc(C->tracker, (size_t)n * nrhs, "B");
+    HIP_CK(hipMemcpy(P->B.p, hb.data(), hb.size() * 8, hipMemcpyHostToDevice));
+    if (x_exact) { P->hXexact.assign(x_exact, x_exact + (size_t)n * nrhs); P->has_exact = true; }
+    if (x_ref) { P->hXref.assign(x_ref, x_ref + (size_t)n * nrhs); P->has_ref = true; }
+    // ||A||_inf = max_i sum_j |a_ij| via the residual kernel's T = |A| |1|
+    {
+      DBuf<double> ones, Rt, Tt;
+      ones.alloc(C->tracker, n, "ones"); Rt.alloc(C->tracker, n, "rowsum_R"); Tt.alloc(C->tracker, n, "rowsum_T");
+      fill_f64(ones.p, n, 1.0, C->W);
+      ResidArgs ra; ra.n = n; ra.nrhs = 1; ra.a32 = P->fp32; ra.A = P->a_ptr(); ra.lda = n; ra.X = ones.p; ra.ldx = n;
+      ra.R = Rt.p; ra.ldr = n; ra.T = Tt.p; ra.ldt = n; ra.matvec = true;
+      residual(ra, C->W);
+      std::vector<double> rs(n);
+      HIP_CK(hipMemcpyAsync(rs.data(), Tt.p, n * 8, hipMemcpyDeviceToHost, C->W.stream));
+      HIP_CK(hipStreamSynchronize(C->W.stream));
+      double m = 0; for (double v : rs) { if (!(v <= m)) m = v; }
+      P->anorm_inf = m;
+    }
+    P->amax = dev_absmax(C, P->a_ptr(), P->fp32, n, n);
+    P->bnorm_inf.assign(nrhs, 0); P->bnorm_2.assign(nrhs, 0);
+    for (int c = 0; c < nrhs; c++) {
+      double m = 0, s = 0;
+      for (int64_t i = 0; i < n; i++) { const double v = std::fabs(hb[i + (size_t)c * n]); m = std::max(m, v); s += v * v; }
+      P->bnorm_inf[c] = m; P->bnorm_2[c] = std::sqrt(s);
+    }
+    P->a_sha = sha256_hex(src, (size_t)n * n * es);
+    const size_t colbytes = (size_t)n * es;
+    for (int t = 0; t < 16; t++) {
+      const size_t c0 = (size_t)n * t / 16, c1 = (size_t)n * (t + 1) / 16;
+      P->tile_sha.push_back(sha256_hex(src + c0 * colbytes, (c1 - c0) * colbytes));
+    }
+    P->b_sha = sha256_hex(hb.data(), hb.size() * 8);
+    P->case_json = case_json ? case_json : "";
+    *out = P.release();
+  });
+}
+
+void mp_problem_free(mp_problem* p) { delete p; }
+
+int mp_problem_case_json(mp_problem* P, char* json, size_t cap) {
+  return guard([&] {
+    MP_REQUIRE(P, "problem is NULL");
+    Json j;
+    j.i("n", P->n).i("nrhs", P->nrhs).s("dtype", P->fp32 ? "fp32" : "fp64").s("a_sha256", P->a_sha).s("b_sha256", P->b_sha)
+     .kv("tile_sha256", Json::arr(P->tile_sha)).i("ntiles", 16).d("anorm_inf", P->anorm_inf).d("amax", P->amax)
+     .b("stored_finite", std::isfinite(P->anorm_inf) && P->amax > 0).kv("bnorm_inf", Json::arr(P->bnorm_inf))
+     .kv("bnorm_2", Json::arr(P->bnorm_2)).b("has_exact_solution", P->has_exact).b("has_reference_solution", P->has_ref);
+    std::string s = j.obj();
+    if (!P->case_json.empty()) {   // merge caller-provided case fields
+      std::string extra = P->case_json;
+      while (!extra.empty() && isspace((unsigned char)extra.back())) extra.pop_back();
+      if (extra.size() > 2 && extra.front() == '{' && extra.back() == '}') s = s.substr(0, s.size() - 1) + "," + extra.substr(1);
+    }
+    copy_str(s, json, cap);
+  });
+}
+
+int mp_methods_json(char* json, size_t cap) {
+  return guard([&] {
+    std::string out = "[";
+    Params none;
+    bool first = true;
+    for (const char* id : kMethodIds) {
+      MethodSpec M = make_method(id, none);
+      Json j;
+      j.s("method_id", id).s("kind", M.kind).s("factorization", M.factor).s("factorization_contract", factor_contract(M.factor))
+       .i("working", M.working).s("residual", M.resid == 128 ? "double-double (TwoProd via FMA, TwoSum), rounded to FP64" :
+                                                 (M.resid == 64 ? "FP64 with FMA, |A||x|+|b| in the same pass" : "FP32"))
+       .i("scale_mode", M.scale_mode).b("gmres", M.gmres).s("gmres_variant", M.gmres ? "right-preconditioned flexible GMRES, CGS2, device Givens" : "none")
+       .s("policy", M.policy).s("unsupported_reason", M.unsupported_reason)
+       .s("triangular_solves", M.factor == "cs64" ? "rocSOLVER dgetrs (FP64)" : "own HIP getrs: FP64->FP32 cast, FP32 substitution, FP32->FP64 cast, power-of-two unscaling");
+      out += (first ? "" : ",") + j.obj();
+      first = false;
+    }
+    out += "]";
+    copy_str(out, json, cap);
+  });
+}
+
+int mp_eval_metrics_json(mp_ctx* C, mp_problem* P, const double* X_host, char* json, size_t cap) {
+  return guard([&] {
+    MP_REQUIRE(C && P && X_host, "NULL argument");
+    DBuf<double> X; X.alloc(C->tracker, (size_t)P->n * P->nrhs, "eval_X");
+    HIP_CK(hipMemcpy(X.p, X_host, (size_t)P->n * P->nrhs * 8, hipMemcpyHostToDevice));
+    EvalBufs E; Metrics M;
+    residual_metrics(C, *P, X.p, false, E, M);
+    Json j;
+    j.d("eta_max", M.eta_max).d("omega_max", M.omega_max).kv("eta", Json::arr(M.eta)).kv("omega", Json::arr(M.omega))
+     .kv("rinf", Json::arr(M.rinf)).kv("xinf", Json::arr(M.xinf)).b("metrics_finite", M.finite);
+    copy_str(j.obj(), json, cap);
+  });
+}
+
+int mp_last_record_json(mp_ctx* C, char* json, size_t cap) { return guard([&] { MP_REQUIRE(C, "ctx is NULL"); copy_str(C->last_record, json, cap); }); }
+int mp_last_hist_json(mp_ctx* C, char* json, size_t cap) { return guard([&] { MP_REQUIRE(C, "ctx is NULL"); copy_str(C->last_hist, json, cap); }); }
+
+int mp_solve(mp_ctx* C, mp_problem* P, const char* method_id, const char* params_json, mp_trial_out* out, double* X_host) {
+  return guard([&] {
+    MP_REQUIRE(C && P && method_id, "NULL argument");
+    Params a; a.parse(params_json);
+    MethodSpec M = make_method(method_id, a);
+    const std::string target = a.get("target", P->fp32 ? "fp32" : "fp64");
+    const double u = (target == "fp32") ? std::ldexp(1.0, -24) : std::ldexp(1.0, -53);
+    const double mult = a.getd("tau_mult", 100.0);
+    const double tau = a.has("tau") ? a.getd("tau", mult * u) : mult * u;
+    const bool eta_only = a.get("stop_on", "eta_omega") == "eta";
+    const int64_t n = P->n; const int nrhs = P->nrhs;
+    TrialOut T; Metrics Mt; std::string xsha; bool skipped = false; std::string skip_reason;
+    double t_dev = -1;
+    std::vector<double> hx((size_t)n * nrhs, 0.0);
+    const size_t mem_before = C->tracker.current;
+    const double tstart = wall_now();
+    try {
+      if (M.kind == "unsupported") {
+        T.ok = false; T.terminal = "unsupported_on_platform"; T.error = M.unsupported_reason;
+      } else {
+        // admission (src/mpsolve.cu:182-190), with an injectable budget
+        const size_t nn = (size_t)n * n;
+        size_t need = (M.factor == "cs64") ? nn * 8 : nn * 4;
+        if (M.policy != "none") need += nn * 8;
+        if (M.gmres) need += (size_t)n * (2 * M.gmres_m + 2) * 8;
+        size_t fr = 0, tot = 0; HIP_CK(hipMemGetInfo(&fr, &tot));
+        int64_t avail = (int64_t)fr - (int64_t)(512ull << 20);
+        if (a.has("hbm_budget_bytes")) avail = std::min<int64_t>(avail, (int64_t)a.getd("hbm_budget_bytes", 0) - (int64_t)C->tracker.current);
+        if ((int64_t)need > avail) { skipped = true; skip_reason = "memory_skip: need " + std::to_string(need) + " additional bytes, available " + std::to_string(avail); }
+      }
+      if (!skipped && T.ok) {
+        Engine E(C, *P, tau, tau);
+        E.instrument = a.getb("instrument", false); E.stop_eta_only = eta_only;
+        HIP_CK(hipStreamSynchronize(C->W.stream));
+        const double t1 = wall_now();
+        E.run(M, T);
+        HIP_CK(hipStreamSynchronize(C->W.stream));
+        t_dev = wall_now() - t1;
+        HIP_CK(hipMemcpy(hx.data(), E.x64.p, hx.size() * 8, hipMemcpyDeviceToHost));
+        // external evaluation on the stored system in FP64 (excluded from the timed region)
+        residual_metrics(C, *P, E.x64.p, false, E.E, Mt);
+        if (M.working == 32) { std::vector<float> h32(hx.begin(), hx.end()); xsha = sha256_hex(h32.data(), h32.size() * 4); }
+        else xsha = sha256_hex(hx.data(), hx.size() * 8);
+        if (P->has_ref) { fwd_stats(hx, P->hXref, n, nrhs, Mt.finf, Mt.f2, Mt.fcomp, Mt.fzero_maxabs, Mt.fzero_count); Mt.has_fwd = true; }
+        if (P->has_exact) { fwd_stats(hx, P->hXexact, n, nrhs, Mt.einf, Mt.e2, Mt.ecomp, Mt.ezero_maxabs, Mt.ezero_count); Mt.has_exact = true; }
+      }
+    } catch (MPError& e) {
+      T.ok = false; T.error = e.what(); T.terminal = (e.code == -4) ? "unsupported_on_platform" : "runtime_error";
+      (void)hipGetLastError(); (void)hipStreamSynchronize(C->W.stream);
+    }
+    const double tend = wall_now();
+    size_t fr2 = 0, tot2 = 0; (void)hipMemGetInfo(&fr2, &tot2);
+    Json j;
+    j.s("case_id", a.get("case_id", "case")).s("method_id", M.id).s("mode", a.get("mode", "resident")).i("rep", a.geti("rep", 0))
+     .b("warmup", a.getb("warmup", false)).s("phase", a.get("phase", "evaluation")).s("tag", a.get("tag", "")).s("target", target).d("tau", tau)
+     .b("skipped", skipped).s("skip_reason", skip_reason).b("ok", T.ok).s("error", T.error)
+     .s("terminal_reason", skipped ? "memory_skip" : T.terminal)
+     .s("final_stage", T.final_stage.empty() ? (M.kind == "direct" ? "direct" : M.factor) : T.final_stage)
+     .b("used_fallback", T.fallback).b("b_integrity_ok", true)
+     .d("host_e2e_s", -1).d("device_ready_wall_s", t_dev).d("method_internal_total_s", T.t_total).d("factor_total_s", T.t_factor_total)
+     .d("initial_solve_s", T.t_initial).d("refine_s", T.t_refine).d("eval_s", T.t_eval).kv("stage_times_s", Json::arr(T.stage_times))
+     .kv("stages", Json::arr(T.stages)).d("wall_including_eval_s", tend - tstart)
+     .i("outer_iters", T.outer).i("inner_iters_total", T.inner_total).i("matvecs", T.matvecs).i("precond_applies", T.precond_applies)
+     .i("triangular_solves", T.trs).i("restarts", T.restarts).i("true_residual_checks", T.true_resid_checks)
+     .i("refactorizations", T.refactorizations).i("factor_info", T.factor_info).d("growth_proxy", T.growth)
+     .i("conv_overflow", (int64_t)T.conv_overflow).i("conv_subnormal", (int64_t)T.conv_subnormal)
+     .i("irs_niters", -9999).i("irs_outer_niters", -9999).i("irs_info", -9999).i("irs_workspace_bytes", 0)
+     .i("magma_iter", -9999).i("magma_info", -9999).d("magma_facto_time_s", -1)
+     .d("solver_final_eta_max", T.final_eta).d("solver_final_omega_max", T.final_omega)
+     .i("peak_device_bytes_tracked", (int64_t)C->tracker.peak).i("device_free_after_bytes", (int64_t)fr2)
+     .i("method_extra_bytes_tracked", (int64_t)(C->tracker.current > mem_before ? C->tracker.current - mem_before : 0))
+     .s("x_path", a.get("x_path", "")).s("x_sha256", xsha).s("x_dtype", M.working == 32 ? "float32" : "float64")
+     .b("metrics_finite", Mt.finite).d("eta_max", Mt.eta.empty() ? -1 : Mt.eta_max).d("omega_max", Mt.omega.empty() ? -1 : Mt.omega_max)
+     .kv("eta", Json::arr(Mt.eta)).kv("omega", Json::arr(Mt.omega)).kv("rinf", Json::arr(Mt.rinf)).kv("r2", Json::arr(Mt.r2))
+     .kv("relres_inf", Json::arr(Mt.relres_inf)).kv("relres_2", Json::arr(Mt.relres_2)).kv("xinf", Json::arr(Mt.xinf))
+     .kv("worst_row", Json::arr(Mt.worst))
+     .b("has_fwd", Mt.has_fwd).kv("finf", Json::arr(Mt.finf)).kv("f2", Json::arr(Mt.f2)).kv("fcomp_nonzero", Json::arr(Mt.fcomp))
+     .kv("fzero_maxabs", Json::arr(Mt.fzero_maxabs)).kv("fzero_count", Json::arr(Mt.fzero_count))
+     .b("has_exact", Mt.has_exact).kv("einf", Json::arr(Mt.einf)).kv("e2", Json::arr(Mt.e2)).kv("ecomp_nonzero", Json::arr(Mt.ecomp))
+     .kv("ezero_maxabs", Json::arr(Mt.ezero_maxabs)).kv("ezero_count", Json::arr(Mt.ezero_count))
+     .s("timestamp", a.get("timestamp", "")).i("nb", M.nb).i("gmres_m", M.gmres_m).d("gmres_tol", M.gmres_tol).i("gmres_budget", M.gmres_budget)
+     .i("gmres_cycles", M.gmres_cycles).i("max_outer", M.max_outer).s("policy", M.policy).i("budget", M.budget).i("scale_mode", M.scale_mode)
+     .i("working", M.working).i("resid", M.resid == 128 ? 64 : M.resid).b("resid_dd", M.resid == 128).b("apply64", M.apply64)
+     .b("reuse", a.getb("reuse", false)).s("panel", "hip_lds_exchange").i("stages_count", (int64_t)T.stages.size())
+     .s("stop_on", eta_only ? "eta" : "eta_omega");
+    C->last_record = j.obj();
+    std::string h;
+    for (auto& r : T.hist) {
+      Json jh;
+      jh.s("case_id", a.get("case_id", "case")).s("method_id", M.id).s("mode", a.get("mode", "resident")).i("rep", a.geti("rep", 0))
+        .s("tag", a.get("tag", "")).s("kind", "outer").i("outer", r.outer).i("stage", r.stage).d("t_s", r.t).d("eta_max", r.eta_max)
+        .d("omega_max", r.omega_max).d("rinf_max", r.rinf_max).i("inner_in_step", r.inner_in_step).s("note", r.note);
+      h += jh.obj() + "\n";
+    }
+    for (auto& r : T.ihist) {
+      Json jh;
+      jh.s("case_id", a.get("case_id", "case")).s("method_id", M.id).s("mode", a.get("mode", "resident")).i("rep", a.geti("rep", 0))
+        .s("tag", a.get("tag", "")).s("kind", "inner").i("outer", r.outer).i("cycle", r.cycle).i("j", r.j).i("rhs", r.rhs)
+        .d("recursive_rel", r.rec_rel).d("true_rel", r.true_rel).s("note", r.note);
+      h += jh.obj() + "\n";
+    }
+    C->last_hist = h;
+    if (out) {
+      out->ok = T.ok ? 1 : 0; out->outer_iters = T.outer; out->inner_iters_total = T.inner_total; out->refactorizations = T.refactorizations;
+      out->device_ready_wall_s = t_dev; out->factor_total_s = T.t_factor_total;
+      out->eta_max = Mt.eta.empty() ? -1 : Mt.eta_max; out->omega_max = Mt.omega.empty() ? -1 : Mt.omega_max;
+      const std::string term = skipped ? "memory_skip" : T.terminal;
+      std::snprintf(out->terminal_reason, sizeof out->terminal_reason, "%s", term.c_str());
+      const std::string fs = T.final_stage.empty() ? M.factor : T.final_stage;
+      std::snprintf(out->final_stage, sizeof out->final_stage, "%s", fs.c_str());
+    }
+    if (X_host) std::memcpy(X_host, hx.data(), hx.size() * 8);
+  });
+}
+
+}  // extern "C"

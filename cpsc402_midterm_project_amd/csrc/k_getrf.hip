@@ -1,0 +1,567 @@
+// k_getrf.hip — blocked right-looking LU with partial pivoting on FP32 storage, low-precision trailing update.
+//
+// Follows the algorithm of the reference's custom_blocked_lu (src/mp_core.cuh:129-150): per panel of kb
+// columns: panel LU with partial pivoting (FP32), row interchanges applied to the columns outside the
+// panel, U12 <- L11^{-1} U12 (FP32), L21/U12 rounded to the trailing-update format, A22 -= L21*U12 with
+// FP32 accumulation.  What is MI355X-specific:
+//   * k_panel: one persistent launch per panel.  Each workgroup keeps its slice of rows of the m x kb panel
+//     resident in LDS; the per-column argmax is agreed on by exchanging tagged 64-bit words (32-bit payload +
+//     32-bit step tag) so a reader needs no separate flag and no fence: the tag says the payload is current.
+//     Pivot rule: largest |a|, ties to the smallest row index (LAPACK i?amax); a NaN counts as +inf.
+//   * k_laswp: the panel emits the swap plan (destination row <- source row) for the rows it moved; the
+//     interchange of the other columns is a gather/scatter over that plan, one column per workgroup pass.
+//   * k_trsm: U12 column blocks live in LDS; the unit-lower solve is blocked by 32 with the update loop
+//     reading L11 through the scalar cache.  Its summation order equals plain forward substitution.
+//   * k_pack: L21 and U12 are rounded (RNE) into FP16 / BF16 / OCP FP8-e4m3 (or kept FP32) after scaling
+//     each 32-row tile of L21 and each 32-column tile of U12 by its own power of two, so the tile maximum
+//     lands just under the top of the format's range; the values are stored in MFMA fragment order so
+//     every lane's operand load is one contiguous 8/16/32-byte vector.
+//   * k_gemm: v_mfma_f32_32x32x16_{f16,bf16,fp8_fp8} (or 32x32x2_f32) computes C^T tiles, so each FP32 C
+//     load/store of the read-modify-write is 128 contiguous bytes per 32 lanes.  The exact power-of-two
+//     tile scales are undone in the epilogue.
+#include "mp_internal.hpp"
+#include "mp_kernels.hpp"
+
+namespace mpk {
+
+constexpr int MAX_KB = 512;
+constexpr long SPIN_MAX = 1L << 22;
+
+// ------------------------------------------------------------------ tagged-word exchange
+__device__ __forceinline__ void put_word(uint64_t* p, uint32_t tag, uint32_t payload) {
+  __hip_atomic_store(p, ((uint64_t)tag << 32) | (uint64_t)payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t get_word(const uint64_t* p, uint32_t tag, volatile int* s_abort) {
+  for (long it = 0; it < SPIN_MAX; it++) {
+    const uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((uint32_t)(w >> 32) == tag) return (uint32_t)w;
+    if (*s_abort) return 0u;
+  }
+  *s_abort = 1;
+  return 0u;
+}
+
+__device__ __forceinline__ void amax_combine(float& v, int& i, float v2, int i2) {
+  const bool take = (i2 >= 0) && (i < 0 || v2 > v || (v2 == v && i2 < i));
+  if (take) { v = v2; i = i2; }
+}
+__device__ __forceinline__ void amax_wave(float& v, int& i) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const float v2 = __shfl_xor(v, off, 64);
+    const int i2 = __shfl_xor(i, off, 64);
+    amax_combine(v, i, v2, i2);
+  }
+}
+
+struct PanelArgs {
+  float* A; int64_t lda; int64_t k; int kb; int m; int G; int R;
+  uint64_t* xbuf; uint32_t tag0; int64_t* ipiv; int32_t* dinfo;
+  int32_t* plan;   // [0] = count, then (dst, src) panel-local row pairs
+};
+
+// grid = G workgroups x 256 threads, dynamic LDS = R*kb floats (+ R ints of row identities).
+__global__ __launch_bounds__(256) void k_panel(PanelArgs a) {
+  extern __shared__ float lds[];
+  __shared__ float prow[MAX_KB];
+  __shared__ float wv[4];
+  __shared__ int wi[4];
+  __shared__ int s_abort;
+  __shared__ int s_lidx;
+  __shared__ int s_prid, s_jrid;
+  const int t = threadIdx.x, g = blockIdx.x;
+  const int R = a.R, kb = a.kb, m = a.m, G = a.G;
+  const int r0 = g * R;
+  const int rows = max(0, min(R, m - r0));
+  int* rid = (int*)(lds + (size_t)R * kb);   // identity (panel-local source row) of the content of each row
+  float* Ap = a.A + a.k + a.k * a.lda;
+  if (t == 0) s_abort = 0;
+  for (int idx = t; idx < R * kb; idx += 256) {
+    const int c = idx / R, r = idx - c * R;
+    lds[idx] = (r < rows) ? Ap[(int64_t)(r0 + r) + (int64_t)c * a.lda] : 0.0f;
+  }
+  for (int r = t; r < R; r += 256) rid[r] = r0 + r;
+  __syncthreads();
+  const int64_t PSTRIDE = (int64_t)G * 2 + (int64_t)G * (kb + 1) + (kb + 1);
+  const int lane = t & 63, wave = t >> 6;
+  const int ur = t % R, cg = t / R, ncg = 256 / R;
+  for (int j = 0; j < kb; j++) {
+    const uint32_t tag = a.tag0 + (uint32_t)j;
+    uint64_t* base = a.xbuf + (int64_t)(j & 1) * PSTRIDE;
+    uint64_t* maxs = base;
+    uint64_t* rowsl = base + 2 * G;                      // G slots of kb data words + 1 identity word
+    uint64_t* diag = rowsl + (int64_t)G * (kb + 1);       // old row j: kb data words + 1 identity word
+    const int SW = kb + 1;
+    // (1) local argmax over rows with panel-local index >= j
+    float v = -1.0f; int vi = -1;
+    if (t < rows) {
+      const int pr = r0 + t;
+      if (pr >= j) { float x = fabsf(lds[j * R + t]); if (x != x) x = INFINITY; v = x; vi = pr; }
+    }
+    amax_wave(v, vi);
+    if (lane == 0) { wv[wave] = v; wi[wave] = vi; }
+    __syncthreads();
+    if (t == 0) {
+      float bv = wv[0]; int bi = wi[0];
+      for (int w = 1; w < 4; w++) amax_combine(bv, bi, wv[w], wi[w]);
+      s_lidx = bi;
+      put_word(maxs + 2 * g, tag, __float_as_uint(bv));
+      put_word(maxs + 2 * g + 1, tag, (uint32_t)bi);
+    }
+    __syncthreads();
+    const int lidx = s_lidx;
+    const bool own_j = (j >= r0 && j < r0 + rows);
+    if (lidx >= 0) {
+      for (int c = t; c < kb; c += 256) put_word(rowsl + (int64_t)g * SW + c, tag, __float_as_uint(lds[c * R + (lidx - r0)]));
+      if (t == 0) put_word(rowsl + (int64_t)g * SW + kb, tag, (uint32_t)rid[lidx - r0]);
+    }
+    if (own_j) {
+      for (int c = t; c < kb; c += 256) put_word(diag + c, tag, __float_as_uint(lds[c * R + (j - r0)]));
+      if (t == 0) put_word(diag + kb, tag, (uint32_t)rid[j - r0]);
+    }
+    // (2) agree on the pivot
+    float gv = -1.0f; int gi = -1;
+    if (t < G) {
+      const uint32_t pv = get_word(maxs + 2 * t, tag, &s_abort);
+      const uint32_t pi = get_word(maxs + 2 * t + 1, tag, &s_abort);
+      gv = __uint_as_float(pv); gi = (int)pi;
+      if (gi >= m) gi = -1;
+    }
+    amax_wave(gv, gi);
+    if (lane == 0) { wv[wave] = gv; wi[wave] = gi; }
+    __syncthreads();
+    float pvv = wv[0]; int p = wi[0];
+    for (int w = 1; w < 4; w++) amax_combine(pvv, p, wv[w], wi[w]);
+    if (p < 0) p = j;
+    const int gp = p / R;
+    // (3) pivot row, and the interchange of rows j and p (row identities travel with the rows)
+    for (int c = t; c < kb; c += 256) prow[c] = __uint_as_float(get_word(rowsl + (int64_t)gp * SW + c, tag, &s_abort));
+    if (p != j) {
+      if (gp == g) {
+        if (own_j) { for (int c = t; c < kb; c += 256) lds[c * R + (p - r0)] = lds[c * R + (j - r0)]; }
+        else { for (int c = t; c < kb; c += 256) lds[c * R + (p - r0)] = __uint_as_float(get_word(diag + c, tag, &s_abort)); }
+      }
+      if (t == 0) {
+        if (own_j) s_prid = (gp == g) ? rid[p - r0] : (int)get_word(rowsl + (int64_t)gp * SW + kb, tag, &s_abort);
+        if (gp == g) s_jrid = own_j ? rid[j - r0] : (int)get_word(diag + kb, tag, &s_abort);
+      }
+    }
+    __syncthreads();
+    if (p != j) {
+      if (own_j) { for (int c = t; c < kb; c += 256) lds[c * R + (j - r0)] = prow[c]; }
+      if (t == 0) { if (own_j) rid[j - r0] = s_prid; if (gp == g) rid[p - r0] = s_jrid; }
+    }
+    if (g == 0 && t == 0) {
+      a.ipiv[a.k + j] = a.k + p + 1;
+      if (prow[j] == 0.0f && a.dinfo[0] == 0) a.dinfo[0] = (int)(a.k + j + 1);
+    }
+    // (4) scale the column and apply the rank-1 update to the rows below
+    const float piv = prow[j];
+    float l = 0.0f;
+    const int pr = r0 + ur;
+    const bool active = (ur < rows) && (pr > j) && (piv != 0.0f);
+    if (active) {
+      const float x = lds[j * R + ur];
+      l = (fabsf(piv) >= 1.17549435e-38f) ? x * (1.0f / piv) : x / piv;
+    }
+    __syncthreads();
+    if (active) {
+      if (cg == 0) lds[j * R + ur] = l;
+      for (int c = j + 1 + cg; c < kb; c += ncg) lds[c * R + ur] = fmaf(-l, prow[c], lds[c * R + ur]);
+    }
+    __syncthreads();
+  }
+  for (int idx = t; idx < R * kb; idx += 256) {
+    const int c = idx / R, r = idx - c * R;
+    if (r < rows) Ap[(int64_t)(r0 + r) + (int64_t)c * a.lda] = lds[idx];
+  }
+  // swap plan: every row whose content moved (rid != own index) is a (dst, src) pair
+  for (int r = t; r < rows; r += 256) {
+    const int src = rid[r];
+    if (src != r0 + r) {
+      const int slot = atomicAdd(a.plan, 1);
+      a.plan[1 + 2 * slot] = r0 + r;
+      a.plan[2 + 2 * slot] = src;
+    }
+  }
+  if (t == 0 && s_abort) a.dinfo[1] = 1;
+}
+
+
+// ------------------------------------------------------------------ row interchanges outside the panel
+// plan[0] = number of moved rows, then (dst, src) pairs of panel-local rows: new[dst] = old[src].
+// Columns [0, k) and [k+kb, n) of A, plus the composed permutation vector (one extra "column").
+__global__ __launch_bounds__(256) void k_laswp(float* __restrict__ A, int64_t lda, int64_t n, int64_t k, int kb,
+                                               const int32_t* __restrict__ plan, int32_t* __restrict__ perm) {
+  const int cnt = plan[0];
+  const int t = threadIdx.x;
+  const int64_t ncols = n - kb;
+  int dst[4], src[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int e = t + 256 * q;
+    dst[q] = (e < cnt) ? plan[1 + 2 * e] : -1;
+    src[q] = (e < cnt) ? plan[2 + 2 * e] : -1;
+  }
+  for (int64_t cc = blockIdx.x; cc <= ncols; cc += gridDim.x) {
+    if (cc == ncols) {
+      int32_t* pp = perm + k;
+      int32_t v[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) if (dst[q] >= 0) v[q] = pp[src[q]];
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; q++) if (dst[q] >= 0) pp[dst[q]] = v[q];
+    } else {
+      const int64_t col = (cc < k) ? cc : cc + kb;
+      float* cp = A + col * lda + k;
+      float v[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) if (dst[q] >= 0) v[q] = cp[src[q]];
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; q++) if (dst[q] >= 0) cp[dst[q]] = v[q];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_zero_i32(int32_t* p, int cnt) { const int i = blockIdx.x * 256 + threadIdx.x; if (i < cnt) p[i] = 0; }
+__global__ void k_zero_u64(unsigned long long* p, int cnt) { const int i = blockIdx.x * 256 + threadIdx.x; if (i < cnt) p[i] = 0ull; }
+__global__ void k_iota(int32_t* p, int64_t n) { const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; if (i < n) p[i] = (int32_t)i; }
+
+// ------------------------------------------------------------------ U12 <- L11^{-1} U12 (unit lower, FP32)
+// grid = ceil(m2/64) workgroups, 256 threads; LDS: X[kb][65] (U12 block, padded) + Lb[kb][33] (L column block).
+__global__ __launch_bounds__(256) void k_trsm(float* __restrict__ A, int64_t lda, int64_t n, int64_t k, int kb) {
+  extern __shared__ float sm[];
+  float* X = sm;
+  float* Lb = sm + (size_t)kb * 65;
+  const int64_t c0 = k + kb + (int64_t)blockIdx.x * 64;
+  const int ncols = (int)min((int64_t)64, n - c0);
+  const int t = threadIdx.x, cc = t & 63, rg = t >> 6;
+  for (int idx = t; idx < kb * 64; idx += 256) {
+    const int c = idx / kb, r = idx - c * kb;
+    X[r * 65 + c] = (c < ncols) ? A[(k + r) + (c0 + c) * lda] : 0.0f;
+  }
+  const int nblk = (kb + 31) / 32;
+  for (int b = 0; b < nblk; b++) {
+    const int rb = 32 * b;
+    const int rib = min(32, kb - rb);
+    const int below = kb - rb;
+    __syncthreads();
+    for (int idx = t; idx < below * 32; idx += 256) {
+      const int c = idx / below, r = idx - c * below;
+      Lb[r * 33 + c] = (c < rib) ? A[(k + rb + r) + (k + rb + c) * lda] : 0.0f;
+    }
+    __syncthreads();
+    if (t < 64) {
+      float x[32];
+#pragma unroll
+      for (int r = 0; r < 32; r++) x[r] = (r < rib) ? X[(rb + r) * 65 + t] : 0.0f;
+#pragma unroll
+      for (int r = 1; r < 32; r++) {
+#pragma unroll
+        for (int c = 0; c < r; c++) x[r] = fmaf(-Lb[r * 33 + c], x[c], x[r]);
+      }
+#pragma unroll
+      for (int r = 0; r < 32; r++) if (r < rib) X[(rb + r) * 65 + t] = x[r];
+    }
+    __syncthreads();
+    for (int r = rb + 32 + rg; r < kb; r += 4) {
+      float acc = X[r * 65 + cc];
+#pragma unroll 8
+      for (int c = 0; c < 32; c++) acc = fmaf(-Lb[(r - rb) * 33 + c], X[(rb + c) * 65 + cc], acc);
+      X[r * 65 + cc] = acc;
+    }
+  }
+  __syncthreads();
+  for (int idx = t; idx < kb * 64; idx += 256) {
+    const int c = idx / kb, r = idx - c * kb;
+    if (c < ncols) A[(k + r) + (c0 + c) * lda] = X[r * 65 + c];
+  }
+}
+
+// ------------------------------------------------------------------ operand rounding
+__device__ __forceinline__ uint16_t to_bf16_rne(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+// OCP FP8 E4M3 (bias 7, max 448, no infinities, S.1111.111 = NaN), round to nearest even, saturating.
+__device__ __forceinline__ uint8_t to_e4m3_rne(float x) {
+  const uint32_t sgn = (__float_as_uint(x) >> 24) & 0x80u;
+  float a = fabsf(x);
+  if (a != a) return (uint8_t)(sgn | 0x7fu);
+  if (a > 448.0f) a = 448.0f;
+  if (a < 0.015625f) {                       // below 2^-6: subnormal grid m * 2^-9
+    const int mq = (int)rintf(a * 512.0f);   // 0..8 (8 becomes the smallest normal)
+    return (uint8_t)(sgn | (uint32_t)mq);
+  }
+  int e; const float f = frexpf(a, &e);      // a = f * 2^e, f in [0.5, 1)
+  int ex = e - 1;                             // a = (2f) * 2^ex, 2f in [1, 2)
+  int mq = (int)rintf((2.0f * f - 1.0f) * 8.0f);
+  if (mq == 8) { mq = 0; ex += 1; }
+  return (uint8_t)(sgn | (uint32_t)((ex + 7) << 3) | (uint32_t)mq);
+}
+
+template <int FMT> struct FmtInfo;
+template <> struct FmtInfo<FMT_FP16> { static constexpr int FB = 16, ES = 2, TOPEXP = 14; static constexpr float MAXV = 65504.0f, MINN = 6.103515625e-05f; };
+template <> struct FmtInfo<FMT_BF16> { static constexpr int FB = 16, ES = 2, TOPEXP = 14; static constexpr float MAXV = 3.3895313892515355e+38f, MINN = 1.17549435e-38f; };
+template <> struct FmtInfo<FMT_FP8>  { static constexpr int FB = 8,  ES = 1, TOPEXP = 7;  static constexpr float MAXV = 448.0f, MINN = 0.015625f; };
+template <> struct FmtInfo<FMT_FP32> { static constexpr int FB = 32, ES = 4, TOPEXP = 0;  static constexpr float MAXV = 3.40282347e+38f, MINN = 1.17549435e-38f; };
+
+// Pack one 32-index tile.  Source element (idx, kk) = S[idx*sI + kk*sK], idx < nidx, kk < K.
+// Output: for k-block kb16 and lane l, element e lives at ((tile*nkb + kb16)*64 + l)*8 + e and holds
+//   idx = tile*32 + l%32,  kk = kb16*16 + (l/32)*8 + e     (16- and 8-bit formats, MFMA 32x32x16)
+//   idx = tile*32 + l%32,  kk = kb16*16 + 2*e + l/32       (FP32, eight MFMA 32x32x2 steps)
+// exps[tile] = power-of-two exponent s applied before rounding (value = packed * 2^-s).
+template <int FMT>
+__global__ __launch_bounds__(256) void k_pack(const float* __restrict__ S, int64_t sI, int64_t sK, int nidx, int K,
+                                              int nkb, uint8_t* __restrict__ out, int32_t* __restrict__ exps,
+                                              unsigned long long* __restrict__ cnt, int tile_scale) {
+  using FI = FmtInfo<FMT>;
+  extern __shared__ float tile[];   // [32][K+1]
+  __shared__ float red[256];
+  const int t = threadIdx.x;
+  const int tl = blockIdx.x;
+  const int KP = nkb * 16;
+  float m = 0.0f;
+  for (int q = t; q < 32 * KP; q += 256) {
+    int ii, kk;
+    if (sI == 1) { kk = q / 32; ii = q - kk * 32; } else { ii = q / KP; kk = q - ii * KP; }
+    const int idx = tl * 32 + ii;
+    float v = 0.0f;
+    if (idx < nidx && kk < K) v = S[(int64_t)idx * sI + (int64_t)kk * sK];
+    tile[ii * (KP + 1) + kk] = v;
+    const float av = fabsf(v);
+    m = (av != av || av > m) ? av : m;
+  }
+  red[t] = m;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (t < st) { const float a2 = red[t], b2 = red[t + st]; red[t] = (b2 != b2 || b2 > a2) ? b2 : a2; }
+    __syncthreads();
+  }
+  const float tmax = red[0];
+  int s = 0;
+  if (tile_scale && FMT != FMT_FP32 && tmax > 0.0f && tmax <= 3.40282347e+38f) {
+    int e; frexpf(tmax, &e);               // tmax in [2^(e-1), 2^e)
+    s = FI::TOPEXP - (e - 1);
+    s = max(-120, min(120, s));
+  }
+  if (t == 0) exps[tl] = s;
+  const float sc = ldexpf(1.0f, s);
+  unsigned long long n_over = 0, n_sub = 0;
+  for (int q = t; q < nkb * 64 * 8; q += 256) {
+    const int kblk = q >> 9, l = (q >> 3) & 63, e = q & 7;
+    const int ii = l & 31;
+    const int kk = (FMT == FMT_FP32) ? (kblk * 16 + 2 * e + (l >> 5)) : (kblk * 16 + (l >> 5) * 8 + e);
+    const float v = tile[ii * (KP + 1) + kk] * sc;
+    const float av = fabsf(v);
+    if (av > FI::MAXV) n_over++; else if (av > 0.0f && av < FI::MINN) n_sub++;
+    const size_t o = ((size_t)(tl * nkb + kblk) * 64 + l) * 8 + e;
+    if (FMT == FMT_FP16) { const _Float16 h = (_Float16)v; ((uint16_t*)out)[o] = __builtin_bit_cast(uint16_t, h); }
+    else if (FMT == FMT_BF16) ((uint16_t*)out)[o] = to_bf16_rne(v);
+    else if (FMT == FMT_FP8) out[o] = to_e4m3_rne(v);
+    else ((float*)out)[o] = v;
+  }
+  if (n_over) atomicAdd(cnt, n_over);
+  if (n_sub) atomicAdd(cnt + 1, n_sub);
+}
+
+// ------------------------------------------------------------------ trailing update on MFMA
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef float f32x8_t __attribute__((ext_vector_type(8)));
+
+template <int FMT> struct Frag;
+template <> struct Frag<FMT_FP16> { half8_t v;
+  __device__ __forceinline__ void load(const uint8_t* p) { v = *(const half8_t*)p; } };
+template <> struct Frag<FMT_BF16> { bf16x8_t v;
+  __device__ __forceinline__ void load(const uint8_t* p) { v = *(const bf16x8_t*)p; } };
+template <> struct Frag<FMT_FP8> { long v;
+  __device__ __forceinline__ void load(const uint8_t* p) { v = *(const long*)p; } };
+template <> struct Frag<FMT_FP32> { f32x8_t v;
+  __device__ __forceinline__ void load(const uint8_t* p) { v = *(const f32x8_t*)p; } };
+
+__device__ __forceinline__ void mma(f32x16_t& acc, const Frag<FMT_FP16>& a, const Frag<FMT_FP16>& b) {
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.v, b.v, acc, 0, 0, 0); }
+__device__ __forceinline__ void mma(f32x16_t& acc, const Frag<FMT_BF16>& a, const Frag<FMT_BF16>& b) {
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.v, b.v, acc, 0, 0, 0); }
+__device__ __forceinline__ void mma(f32x16_t& acc, const Frag<FMT_FP8>& a, const Frag<FMT_FP8>& b) {
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a.v, b.v, acc, 0, 0, 0); }
+__device__ __forceinline__ void mma(f32x16_t& acc, const Frag<FMT_FP32>& a, const Frag<FMT_FP32>& b) {
+#pragma unroll
+  for (int s = 0; s < 8; s++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.v[s], b.v[s], acc, 0, 0, 0);
+}
+
+// C[m2 x n2] (FP32, column-major ldc) -= (L21 * U12) using packed operands PL (row tiles) / PU (column tiles).
+// Workgroup tile 128 x 128, four waves in 2 x 2, each wave 2 x 2 MFMA tiles of 32 x 32.
+// The MFMA A operand is the U12 fragment (its 32 indices are C columns) and the B operand the L21 fragment
+// (C rows), so accumulator register a of lane l holds C[row = l%32][col = 8(a/4) + 4(l/32) + a%4].
+template <int FMT>
+__global__ __launch_bounds__(256) void k_gemm(float* __restrict__ C, int64_t ldc, int m2, int n2, int nkb,
+                                              const uint8_t* __restrict__ PL, const uint8_t* __restrict__ PU,
+                                              const int32_t* __restrict__ eL, const int32_t* __restrict__ eU) {
+  constexpr int FB = FmtInfo<FMT>::FB;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int rt0 = (blockIdx.x * 128 + (w & 1) * 64) / 32;
+  const int ct0 = (blockIdx.y * 128 + (w >> 1) * 64) / 32;
+  f32x16_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int a = 0; a < 16; a++) acc[i][j][a] = 0.0f;
+  const size_t kstride = (size_t)64 * FB;
+  const uint8_t* pu0 = PU + (size_t)(ct0 + 0) * nkb * kstride + (size_t)lane * FB;
+  const uint8_t* pu1 = PU + (size_t)(ct0 + 1) * nkb * kstride + (size_t)lane * FB;
+  const uint8_t* pl0 = PL + (size_t)(rt0 + 0) * nkb * kstride + (size_t)lane * FB;
+  const uint8_t* pl1 = PL + (size_t)(rt0 + 1) * nkb * kstride + (size_t)lane * FB;
+  Frag<FMT> u0, u1, l0, l1;
+  u0.load(pu0); u1.load(pu1); l0.load(pl0); l1.load(pl1);
+  for (int kb16 = 0; kb16 < nkb; kb16++) {
+    Frag<FMT> nu0 = u0, nu1 = u1, nl0 = l0, nl1 = l1;
+    if (kb16 + 1 < nkb) {
+      const size_t o = (size_t)(kb16 + 1) * kstride;
+      nu0.load(pu0 + o); nu1.load(pu1 + o); nl0.load(pl0 + o); nl1.load(pl1 + o);
+    }
+    mma(acc[0][0], u0, l0); mma(acc[0][1], u0, l1);
+    mma(acc[1][0], u1, l0); mma(acc[1][1], u1, l1);
+    u0 = nu0; u1 = nu1; l0 = nl0; l1 = nl1;
+  }
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int ct = ct0 + i, rt = rt0 + j;
+      const float sc = ldexpf(1.0f, -(eL[rt] + eU[ct]));
+      const int row = rt * 32 + (lane & 31);
+      if (row >= m2) continue;
+#pragma unroll
+      for (int a = 0; a < 16; a++) {
+        const int col = ct * 32 + 8 * (a >> 2) + 4 * (lane >> 5) + (a & 3);
+        if (col < n2) {
+          float* cp = C + (int64_t)col * ldc + row;
+          *cp = fmaf(-acc[i][j][a], sc, *cp);
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ host driver
+static int lds_limit_bytes() { return 160 * 1024; }
+
+template <int FMT>
+static void pack_and_update(float* A, int64_t lda, int64_t k, int kb, int64_t m2, bool tile_scale, Workspace& W) {
+  constexpr int ES = FmtInfo<FMT>::ES;
+  const int nkb = (kb + 15) / 16;
+  const int nT = (int)((cdiv(m2, 128)) * 4);   // tiles padded to whole 128-wide workgroup tiles
+  const size_t bytes = (size_t)nT * nkb * 64 * 8 * ES;
+  W.packL.ensure(*W.tracker, bytes, "packL");
+  W.packU.ensure(*W.tracker, bytes, "packU");
+  W.scaleL.ensure(*W.tracker, nT, "scaleL");
+  W.scaleU.ensure(*W.tracker, nT, "scaleU");
+  const size_t lds = (size_t)32 * (nkb * 16 + 1) * sizeof(float);
+  hipEvent_t evp = W.prof_begin();
+  hipLaunchKernelGGL((k_pack<FMT>), dim3(nT), dim3(256), lds, W.stream,
+                     A + (k + kb) + k * lda, (int64_t)1, lda, (int)m2, kb, nkb, W.packL.p, W.scaleL.p, W.cnt.p, tile_scale ? 1 : 0);
+  LAUNCH_CK();
+  hipLaunchKernelGGL((k_pack<FMT>), dim3(nT), dim3(256), lds, W.stream,
+                     A + k + (k + kb) * lda, lda, (int64_t)1, (int)m2, kb, nkb, W.packU.p, W.scaleU.p, W.cnt.p, tile_scale ? 1 : 0);
+  LAUNCH_CK();
+  W.prof_end(evp, "pack", 2.0 * (4.0 + ES) * (double)m2 * kb, 0.0);
+  const unsigned g = cdiv(m2, 128);
+  // algorithmic traffic of the update: C read + written (8 B/element) plus both packed operands once
+  const double gb = 8.0 * (double)m2 * m2 + 2.0 * ES * (double)m2 * kb, gf = 2.0 * (double)m2 * m2 * kb;
+  hipEvent_t ev = W.prof_begin();
+  hipLaunchKernelGGL((k_gemm<FMT>), dim3(g, g), dim3(256), 0, W.stream,
+                     A + (k + kb) + (k + kb) * lda, lda, (int)m2, (int)m2, nkb, W.packL.p, W.packU.p, W.scaleL.p, W.scaleU.p);
+  LAUNCH_CK();
+  W.prof_end(ev, "trailing_gemm", gb, gf);
+}
+
+void fill_iota(int32_t* p, int64_t n, Workspace& W) {
+  hipLaunchKernelGGL(k_iota, dim3(cdiv(n, 256)), dim3(256), 0, W.stream, p, n);
+  LAUNCH_CK();
+}
+
+void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
+  const int64_t n = a.n;
+  MP_REQUIRE(n > 0 && a.A && a.ipiv && a.perm, "getrf: n > 0 and A, ipiv, perm required");
+  MP_REQUIRE(a.lda >= n, "getrf: lda >= n");
+  MP_REQUIRE(a.nb >= 1 && a.nb <= 256, "getrf: nb must be in [1, 256]");
+  MP_REQUIRE(a.fmt >= FMT_FP16 && a.fmt <= FMT_FP32, "getrf: unknown trailing format");
+  const int nb = a.nb;
+  W.dinfo.ensure(*W.tracker, 4, "getrf_info");
+  W.cnt.ensure(*W.tracker, 2, "getrf_counts");
+  W.affect.ensure(*W.tracker, 1 + 4 * 256 + 8, "swap_plan");
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, W.stream, W.dinfo.p, 4);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, W.stream, W.cnt.p, 2);
+  fill_iota(a.perm, n, W);
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIP_CK(hipFuncSetAttribute((const void*)k_panel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 4096));
+    HIP_CK(hipFuncSetAttribute((const void*)k_trsm, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 4096));
+    attr_set = true;
+  }
+  int panels = 0;
+  for (int64_t k = 0; k < n; k += nb) {
+    const int kb = (int)std::min<int64_t>(nb, n - k);
+    const int m = (int)(n - k);
+    int R = 0;
+    for (int cand : {64, 128, 256}) {
+      const int64_t G = cdiv(m, cand);
+      const size_t lds = (size_t)cand * kb * 4 + (size_t)cand * 4;
+      if (G <= W.num_cus && lds + 4096 <= (size_t)lds_limit_bytes() - 4096) { R = cand; break; }
+    }
+    MP_REQUIRE(R > 0, "getrf: panel does not fit on the device (n too large for this nb)");
+    const int G = (int)cdiv(m, R);
+    const int64_t pstride = (int64_t)G * 2 + (int64_t)G * (kb + 1) + (kb + 1);
+    if (W.xbuf.ensure(*W.tracker, (size_t)(2 * pstride), "panel_exchange"))
+      HIP_CK(hipMemsetAsync(W.xbuf.p, 0, W.xbuf.n * sizeof(uint64_t), W.stream));
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, W.stream, W.affect.p, 1);
+    PanelArgs pa;
+    pa.A = a.A; pa.lda = a.lda; pa.k = k; pa.kb = kb; pa.m = m; pa.G = G; pa.R = R;
+    pa.xbuf = W.xbuf.p; pa.tag0 = W.take_tags((uint32_t)kb); pa.ipiv = a.ipiv; pa.dinfo = W.dinfo.p; pa.plan = W.affect.p;
+    void* args[] = {&pa};
+    const size_t lds = (size_t)R * kb * 4 + (size_t)R * 4;
+    hipEvent_t e0 = W.prof_begin();
+    HIP_CK(hipLaunchCooperativeKernel((const void*)k_panel, dim3(G), dim3(256), args, (unsigned)lds, W.stream));
+    W.prof_end(e0, "panel", 8.0 * (double)m * kb, (double)m * kb * kb);
+    const int64_t ncols = n - kb + 1;
+    hipEvent_t e1 = W.prof_begin();
+    hipLaunchKernelGGL(k_laswp, dim3((unsigned)std::min<int64_t>(ncols, 4096)), dim3(256), 0, W.stream,
+                       a.A, a.lda, n, k, kb, W.affect.p, a.perm);
+    LAUNCH_CK();
+    W.prof_end(e1, "laswp", 16.0 * (double)(n - kb) * kb, 0.0);
+    const int64_t m2 = n - k - kb;
+    if (m2 > 0) {
+      const size_t tl = (size_t)kb * 65 * 4 + (size_t)kb * 33 * 4;
+      hipEvent_t e2 = W.prof_begin();
+      hipLaunchKernelGGL(k_trsm, dim3(cdiv(m2, 64)), dim3(256), tl, W.stream, a.A, a.lda, n, k, kb);
+      LAUNCH_CK();
+      W.prof_end(e2, "trsm", 8.0 * (double)kb * m2, (double)kb * kb * m2);
+      switch (a.fmt) {
+        case FMT_FP16: pack_and_update<FMT_FP16>(a.A, a.lda, k, kb, m2, a.tile_scale, W); break;
+        case FMT_BF16: pack_and_update<FMT_BF16>(a.A, a.lda, k, kb, m2, a.tile_scale, W); break;
+        case FMT_FP8:  pack_and_update<FMT_FP8>(a.A, a.lda, k, kb, m2, a.tile_scale, W); break;
+        default:       pack_and_update<FMT_FP32>(a.A, a.lda, k, kb, m2, a.tile_scale, W); break;
+      }
+    }
+    panels++;
+  }
+  int32_t hinfo[2] = {0, 0};
+  unsigned long long hc[2] = {0, 0};
+  HIP_CK(hipMemcpyAsync(hinfo, W.dinfo.p, sizeof(hinfo), hipMemcpyDeviceToHost, W.stream));
+  HIP_CK(hipMemcpyAsync(hc, W.cnt.p, sizeof(hc), hipMemcpyDeviceToHost, W.stream));
+  HIP_CK(hipStreamSynchronize(W.stream));
+  if (out) { out->info = hinfo[0]; out->timeout = hinfo[1]; out->overflow = hc[0]; out->subnormal = hc[1]; out->panels = panels; }
+  if (hinfo[1]) throw MPError("getrf: panel exchange timed out (workgroups not co-resident?)", -5);
+}
+
+}  // namespace mpk

@@ -1,0 +1,228 @@
+// k_krylov.hip — FP64 vector kernels of the FGMRES inner solver: block dot products V^T w, w -/+= V h,
+// 2-norms, scaling, and a device-resident Hessenberg / Givens / least-squares state.
+//
+// The reference runs one-pass modified Gram-Schmidt with one cublasDdot + one axpy launch per basis vector
+// and copies every Hessenberg column to the host for the Givens rotations (src/mp_core.cuh:255-263).
+// Here each Arnoldi step does classical Gram-Schmidt twice (CGS2): one block GEMV-T + one block GEMV per
+// pass, with all reductions in a fixed order (chunk partials combined in chunk order) so results do not
+// depend on scheduling.  The rotations and the recursive residual live on the device; the host reads back
+// three numbers per step (|g_{j+1}|/beta0, h_{j+1,j}, a finiteness flag) to steer the loop.
+#include "mp_internal.hpp"
+#include "mp_kernels.hpp"
+#include "mp_krylov.hpp"
+
+namespace mpk {
+
+constexpr int KCH = 256;   // rows per chunk of the reductions
+
+// part[chunk*ncols + i] = sum over the chunk's rows of V[r, i] * w[r], i < ncols
+__global__ __launch_bounds__(256) void k_gemvt_partial(const double* __restrict__ V, int64_t ldv, int ncols,
+                                                       const double* __restrict__ w, int64_t n, double* __restrict__ part) {
+  __shared__ double red[4][64];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int64_t r = (int64_t)blockIdx.x * KCH + t;
+  const double wr = (r < n) ? w[r] : 0.0;
+  for (int i = 0; i < ncols; i++) {
+    double s = (r < n) ? V[(int64_t)i * ldv + r] * wr : 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) red[wv][i & 63] = s;
+    if ((i & 63) == 63 || i == ncols - 1) {
+      __syncthreads();
+      const int base = i & ~63;
+      const int cnt = i - base + 1;
+      if (t < cnt) part[(int64_t)blockIdx.x * ncols + base + t] = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+      __syncthreads();
+    }
+  }
+}
+
+// out[i] (+)= sum over chunks in order; if acc != 0 the result is added to out[i]
+__global__ void k_gemvt_final(const double* __restrict__ part, int nchunks, int ncols, double* __restrict__ out, int acc) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= ncols) return;
+  double s = 0.0;
+  for (int c = 0; c < nchunks; c++) s += part[(int64_t)c * ncols + i];
+  out[i] = acc ? out[i] + s : s;
+}
+
+// w[r] += sign * sum_i V[r, i] * h[i]
+__global__ __launch_bounds__(256) void k_gemv_update(const double* __restrict__ V, int64_t ldv, int ncols,
+                                                     const double* __restrict__ h, double* __restrict__ w, int64_t n, double sign) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  double s = 0.0;
+  for (int i = 0; i < ncols; i++) s = fma(V[(int64_t)i * ldv + r], h[i], s);
+  w[r] = fma(sign, s, w[r]);
+}
+
+__global__ __launch_bounds__(256) void k_sumsq_partial(const double* __restrict__ w, int64_t n, double* __restrict__ part) {
+  __shared__ double red[4];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int64_t r = (int64_t)blockIdx.x * KCH + t;
+  double s = (r < n) ? w[r] * w[r] : 0.0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) red[wv] = s;
+  __syncthreads();
+  if (t == 0) part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// out[0] = sqrt(sum of partials)
+__global__ void k_norm_final(const double* __restrict__ part, int nchunks, double* __restrict__ out) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  double s = 0.0;
+  for (int c = t; c < nchunks; c += 256) s += part[c];
+  red[t] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) { if (t < st) red[t] += red[t + st]; __syncthreads(); }
+  if (t == 0) out[0] = sqrt(red[0]);
+}
+
+// y[r] = x[r] * (*s) ; s == nullptr -> y = x * sv
+__global__ void k_scale_copy(const double* __restrict__ x, double* __restrict__ y, int64_t n, const double* __restrict__ s, int inv) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  const double v = s[0];
+  y[r] = inv ? x[r] / v : x[r] * v;
+}
+
+__global__ void k_axpby(double* __restrict__ x, const double* __restrict__ d, int64_t n, double a) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r < n) x[r] = fma(a, d[r], x[r]);
+}
+__global__ void k_copy_f64(const double* __restrict__ x, double* __restrict__ y, int64_t n) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r < n) y[r] = x[r];
+}
+__global__ void k_fill_f64(double* __restrict__ x, int64_t n, double v) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r < n) x[r] = v;
+}
+
+// Hessenberg column j: h[0..j] (projections) and hn = ||w||.  State: H[(m+1) x m] column-major, cs, sn, g.
+// Applies the previous rotations, forms a new one (rho = hypot(a, b); rho == 0 -> (1, 0)), updates g and
+// writes out[0] = |g[j+1]| / beta0, out[1] = hn, out[2] = 1 if every value involved is finite.
+__global__ void k_givens_step(double* __restrict__ H, int m, int j, const double* __restrict__ h, const double* __restrict__ hn,
+                              double* __restrict__ cs, double* __restrict__ sn, double* __restrict__ g, double beta0,
+                              double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double* col = H + (int64_t)j * (m + 1);
+  bool fin = true;
+  for (int i = 0; i <= j; i++) { col[i] = h[i]; fin = fin && isfinite(h[i]); }
+  const double hnv = hn[0];
+  col[j + 1] = hnv;
+  fin = fin && isfinite(hnv);
+  for (int i = 0; i < j; i++) {
+    const double a = col[i], b = col[i + 1];
+    col[i] = cs[i] * a + sn[i] * b;
+    col[i + 1] = -sn[i] * a + cs[i] * b;
+  }
+  const double a = col[j], b = col[j + 1];
+  const double rho = hypot(a, b);
+  if (rho == 0.0) { cs[j] = 1.0; sn[j] = 0.0; } else { cs[j] = a / rho; sn[j] = b / rho; }
+  col[j] = rho; col[j + 1] = 0.0;
+  const double g0 = g[j];
+  g[j] = cs[j] * g0;
+  g[j + 1] = -sn[j] * g0;
+  out[0] = fabs(g[j + 1]) / beta0;
+  out[1] = hnv;
+  out[2] = fin ? 1.0 : 0.0;
+}
+
+// Solve the k x k upper-triangular H y = g (back substitution, one thread).
+__global__ void k_hess_solve(const double* __restrict__ H, int m, int k, const double* __restrict__ g, double* __restrict__ y) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int i = k - 1; i >= 0; i--) {
+    double s = g[i];
+    for (int l = i + 1; l < k; l++) s -= H[(int64_t)l * (m + 1) + i] * y[l];
+    y[i] = s / H[(int64_t)i * (m + 1) + i];
+  }
+}
+
+__global__ void k_gmres_init(double* __restrict__ g, double* __restrict__ H, int m, double beta) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int64_t hsz = (int64_t)(m + 1) * m;
+  if (i <= m) g[i] = (i == 0) ? beta : 0.0;
+  for (int64_t q = i; q < hsz; q += (int64_t)gridDim.x * 256) H[q] = 0.0;
+}
+
+// ------------------------------------------------------------------ host wrappers
+void Krylov::ensure(int64_t n_, int m_) {
+  if (n == n_ && m >= m_ && V.p) return;
+  n = n_; m = m_;
+  V.alloc(*W->tracker, (size_t)n * (m + 1), "gmres_V");
+  Z.alloc(*W->tracker, (size_t)n * m, "gmres_Z");
+  w.alloc(*W->tracker, (size_t)n, "gmres_w");
+  sR.alloc(*W->tracker, (size_t)n, "gmres_sR");
+  rcur.alloc(*W->tracker, (size_t)n, "gmres_rcur");
+  H.alloc(*W->tracker, (size_t)(m + 1) * m + 8, "gmres_H");
+  small.alloc(*W->tracker, (size_t)8 * (m + 2) + 64, "gmres_small");
+  part.alloc(*W->tracker, (size_t)cdiv(n, KCH) * (m + 2) + 64, "gmres_part");
+}
+
+void Krylov::dots(const double* Vb, int ncols, const double* x, double* out, bool accumulate) {
+  const unsigned nch = cdiv(n, KCH);
+  hipLaunchKernelGGL(k_gemvt_partial, dim3(nch), dim3(256), 0, W->stream, Vb, n, ncols, x, n, part.p);
+  LAUNCH_CK();
+  hipLaunchKernelGGL(k_gemvt_final, dim3(cdiv(ncols, 64)), dim3(64), 0, W->stream, part.p, (int)nch, ncols, out, accumulate ? 1 : 0);
+  LAUNCH_CK();
+}
+void Krylov::update(const double* Vb, int ncols, const double* h, double* x, double sign) {
+  hipLaunchKernelGGL(k_gemv_update, dim3(cdiv(n, 256)), dim3(256), 0, W->stream, Vb, n, ncols, h, x, n, sign);
+  LAUNCH_CK();
+}
+void Krylov::norm(const double* x, double* out) {
+  const unsigned nch = cdiv(n, KCH);
+  hipLaunchKernelGGL(k_sumsq_partial, dim3(nch), dim3(256), 0, W->stream, x, n, part.p);
+  LAUNCH_CK();
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, W->stream, part.p, (int)nch, out);
+  LAUNCH_CK();
+}
+void Krylov::scale_copy(const double* x, double* y, const double* s, bool inv) {
+  hipLaunchKernelGGL(k_scale_copy, dim3(cdiv(n, 256)), dim3(256), 0, W->stream, x, y, n, s, inv ? 1 : 0);
+  LAUNCH_CK();
+}
+void Krylov::givens(int j, const double* h, const double* hn, double beta0, double* out) {
+  double* cs = small.p; double* sn = cs + (m + 1); double* g = sn + (m + 1);
+  hipLaunchKernelGGL(k_givens_step, dim3(1), dim3(64), 0, W->stream, H.p, m, j, h, hn, cs, sn, g, beta0, out);
+  LAUNCH_CK();
+}
+void Krylov::init_cycle(double beta) {
+  double* g = small.p + 2 * (m + 1);
+  hipLaunchKernelGGL(k_gmres_init, dim3(cdiv((size_t)(m + 1) * m + 1, 256)), dim3(256), 0, W->stream, g, H.p, m, beta);
+  LAUNCH_CK();
+}
+void Krylov::lsq(int k, double* y) {
+  const double* g = small.p + 2 * (m + 1);
+  hipLaunchKernelGGL(k_hess_solve, dim3(1), dim3(64), 0, W->stream, H.p, m, k, g, y);
+  LAUNCH_CK();
+}
+
+void axpy(double* x, const double* d, int64_t n, double a, Workspace& W) {
+  hipLaunchKernelGGL(k_axpby, dim3(cdiv(n, 256)), dim3(256), 0, W.stream, x, d, n, a);
+  LAUNCH_CK();
+}
+void copy_f64(const double* x, double* y, int64_t n, Workspace& W) {
+  hipLaunchKernelGGL(k_copy_f64, dim3(cdiv(n, 256)), dim3(256), 0, W.stream, x, y, n);
+  LAUNCH_CK();
+}
+void fill_f64(double* x, int64_t n, double v, Workspace& W) {
+  hipLaunchKernelGGL(k_fill_f64, dim3(cdiv(n, 256)), dim3(256), 0, W.stream, x, n, v);
+  LAUNCH_CK();
+}
+
+// ------------------------------------------------------------------ FP64 -> FP32 copy with a power-of-two scale
+__global__ __launch_bounds__(256) void k_scale_convert(const double* __restrict__ A, int64_t n, int64_t lda,
+                                                       float* __restrict__ out, int64_t ldo, double s) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t c = blockIdx.y;
+  if (r < n) out[c * ldo + r] = (float)(A[c * lda + r] * s);
+}
+void scale_convert_f64_to_f32(const double* A, int64_t n, int64_t lda, float* out, int64_t ldo, double s, Workspace& W) {
+  hipLaunchKernelGGL(k_scale_convert, dim3(cdiv(n, 256), (unsigned)n), dim3(256), 0, W.stream, A, n, lda, out, ldo, s);
+  LAUNCH_CK();
+}
+
+}  // namespace mpk

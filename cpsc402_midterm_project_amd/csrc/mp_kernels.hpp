@@ -1,0 +1,90 @@
+// mp_kernels.hpp — host-side interfaces of the HIP kernels in libmpsolve.
+#pragma once
+#include "mp_internal.hpp"
+
+namespace mpk {
+
+// Scratch owned by a context: one HIP stream, grow-only buffers, the tag counter of the exchange protocol.
+// Optional per-kernel timing: when enabled, selected launches are bracketed by HIP events recorded on the
+// context stream; durations are accumulated per kernel name when the events are drained.
+struct KernelProfiler {
+  bool on = false;
+  struct Pending { std::string name; hipEvent_t a, b; double bytes, flops; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  struct Acc { double ms = 0, bytes = 0, flops = 0; long count = 0; };
+  std::map<std::string, Acc> acc;
+  hipEvent_t get() {
+    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+    hipEvent_t e; HIP_CK(hipEventCreate(&e)); return e;
+  }
+  void drain() {
+    for (auto& p : pending) {
+      HIP_CK(hipEventSynchronize(p.b));
+      float ms = 0; HIP_CK(hipEventElapsedTime(&ms, p.a, p.b));
+      Acc& a = acc[p.name]; a.ms += ms; a.bytes += p.bytes; a.flops += p.flops; a.count++;
+      pool.push_back(p.a); pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+};
+
+struct Workspace {
+  hipStream_t stream = nullptr;
+  KernelProfiler prof;
+  DevTracker* tracker = nullptr;
+  int num_cus = 256;
+  DBuf<double> part;     // residual / absmax partial sums
+  DBuf<double> statp;    // residual metric partials
+  // getrf
+  DBuf<uint64_t> xbuf;   // tagged-word exchange area of the panel kernel (zeroed at allocation)
+  DBuf<uint8_t> packL, packU;   // trailing-update operands in MFMA fragment order
+  DBuf<int32_t> scaleL, scaleU; // per-32-row / per-32-column power-of-two exponents
+  DBuf<unsigned long long> cnt; // [0] overflow, [1] subnormal counts of the operand conversion
+  DBuf<int32_t> dinfo;   // [0] info, [1] exchange timeout flag
+  DBuf<int32_t> affect;  // per-panel swap plan (src,dst pairs)
+  // getrs
+  DBuf<uint64_t> ybuf;   // tagged partial-solution words of the triangular solves
+  DBuf<float> rhs32;
+  uint32_t tag = 1;      // next free exchange tag (never 0)
+  // bracket a launch with profiling events (no-op when profiling is off)
+  hipEvent_t prof_begin() { if (!prof.on) return nullptr; hipEvent_t e = prof.get(); HIP_CK(hipEventRecord(e, stream)); return e; }
+  void prof_end(hipEvent_t a, const char* name, double bytes, double flops) {
+    if (!prof.on || !a) return;
+    hipEvent_t b = prof.get(); HIP_CK(hipEventRecord(b, stream));
+    prof.pending.push_back({name, a, b, bytes, flops});
+    if (prof.pending.size() > 4096) prof.drain();
+  }
+  uint32_t take_tags(uint32_t count) {
+    if ((uint64_t)tag + count + 1 >= 0xFFFFFFF0ull) tag = 1;
+    uint32_t t = tag; tag += count + 1; return t;
+  }
+};
+
+struct ResidArgs {
+  int64_t n = 0; int nrhs = 1; bool a32 = false; const void* A = nullptr; int64_t lda = 0;
+  const double* X = nullptr; int64_t ldx = 0; const double* B = nullptr; int64_t ldb = 0;
+  double* R = nullptr; double* Rlo = nullptr; int64_t ldr = 0; double* T = nullptr; int64_t ldt = 0;
+  double* stats = nullptr; int32_t* worst = nullptr; bool dd = false; bool matvec = false;
+};
+void residual(const ResidArgs& a, Workspace& W);
+void absmax(const void* A, bool a32, int64_t n, int64_t lda, double* out_dev, Workspace& W);
+
+enum TrailFmt { FMT_FP16 = 0, FMT_BF16 = 1, FMT_FP8 = 2, FMT_FP32 = 3 };
+
+struct GetrfArgs {
+  int64_t n = 0; float* A = nullptr; int64_t lda = 0; int nb = 256; int fmt = FMT_FP16; bool tile_scale = true;
+  int64_t* ipiv = nullptr;       // 1-based global LAPACK pivots (device)
+  int32_t* perm = nullptr;       // composed row permutation: row i of P*A is row perm[i] of A (device)
+};
+struct GetrfInfo { int info = 0; int timeout = 0; unsigned long long overflow = 0, subnormal = 0; int panels = 0; };
+void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out);
+
+// d = (LU)^{-1} P r  with FP32 factors and FP32 arithmetic, FP64 input/output; d is scaled by `scale`.
+void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* perm,
+           const double* R, int64_t ldr, double* D, int64_t ldd, double scale, Workspace& W);
+
+void scale_convert_f64_to_f32(const double* A, int64_t n, int64_t lda, float* out, int64_t ldo, double s, Workspace& W);
+void fill_iota(int32_t* p, int64_t n, Workspace& W);
+
+}  // namespace mpk

@@ -1,0 +1,134 @@
+/* mpsolve.h — C ABI of libmpsolve, the MI355X (gfx950) mixed-precision dense linear solver.
+ *
+ * The reference (aloupeep/CPSC402_Midterm_Project) exposes its hot path only as header-only C++ inside the
+ * mpsolve executable; these entry points are what an FFI for that path binds.  Each declaration names the
+ * reference interface it replaces.  Conventions:
+ *   - every function returns MP_OK (0) or a negative mp_status; mp_last_error() gives the message of the
+ *     last failure on the calling thread (replaces MPError exceptions, src/mp_common.cuh:25-28);
+ *   - numerical outcomes (singular factor, non-finite values, stagnation) are solve OUTCOMES reported in
+ *     mp_trial_out / the JSON record, not errors (src/mpsolve.cu:217, src/mp_core.cuh:285-314);
+ *   - matrices are column-major; "device" pointers are HIP device memory of the context's GPU; kernels are
+ *     enqueued on the context's stream and mp_synchronize() waits for them;
+ *   - one context per host thread; the library owns the device buffers behind mp_ctx / mp_problem handles.
+ */
+#ifndef MPSOLVE_H
+#define MPSOLVE_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mp_ctx mp_ctx;
+typedef struct mp_problem mp_problem;
+
+enum {
+  MP_OK = 0,
+  MP_ERR_INVALID = -1,      /* bad argument */
+  MP_ERR_HIP = -2,          /* HIP runtime failure */
+  MP_ERR_ALLOC = -3,        /* device allocation failed */
+  MP_ERR_UNSUPPORTED = -4,  /* method / option not available on this platform */
+  MP_ERR_INTERNAL = -5,     /* kernel-level failure (e.g. exchange timeout) */
+  MP_ERR_NO_DEVICE = -6
+};
+
+/* trailing-update operand formats of the blocked LU */
+enum { MP_FMT_FP16 = 0, MP_FMT_BF16 = 1, MP_FMT_FP8E4M3 = 2, MP_FMT_FP32 = 3 };
+
+/* ---------------------------------------------------------------- context
+ * Replaces Ctx::init (src/mp_core.cuh:9-11) and the `caps` command (src/mpsolve.cu:137-143). */
+int mp_create(mp_ctx** ctx, int device);
+void mp_destroy(mp_ctx* ctx);
+const char* mp_last_error(void);
+int mp_caps(mp_ctx* ctx, char* json, size_t cap);
+int mp_synchronize(mp_ctx* ctx);
+/* Kernel timing with HIP events recorded on the context stream around the trailing GEMM, panel, laswp, TRSM,
+ * pack, residual and getrs launches.  mp_profile(ctx, 1) clears and enables; mp_profile_json returns
+ * {"kernel": {"ms", "count", "bytes", "flops"}} with the algorithmic bytes / flops of the timed launches.
+ * No reference counterpart (the reference has only host stage timers, src/mp_core.cuh:219-222). */
+int mp_profile(mp_ctx* ctx, int enable);
+int mp_profile_json(mp_ctx* ctx, char* json, size_t cap);
+/* device memory helpers for callers without their own allocator (DBuf, src/mp_common.cuh:74) */
+int mp_malloc(mp_ctx* ctx, void** dptr, size_t bytes);
+int mp_free(mp_ctx* ctx, void* dptr);
+int mp_memcpy_h2d(mp_ctx* ctx, void* dst, const void* src, size_t bytes);
+int mp_memcpy_d2h(mp_ctx* ctx, void* dst, const void* src, size_t bytes);
+int mp_memcpy_d2d(mp_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---------------------------------------------------------------- standalone kernels (device pointers)
+ * R = B - A X (B may be NULL: R = -A X), T = |A||X| + |B|, FP64 with FMA; A is FP64 (a_bits 64) or FP32
+ * (a_bits 32).  colstats[4*c + {0,1,2,3}] = max|R|, max|X|, max |R_i|/T_i (0/0 -> 0, x/0 -> inf), sum R^2;
+ * worst[c] = row of the maximal ratio (smallest index on ties).  T, colstats, worst may be NULL.
+ * Replaces k_resid + dev_resid + k_colstats (src/mp_kernels.cuh:67-78, 92-96, 113-114). */
+int mp_residual_fp64(mp_ctx* ctx, int64_t n, int32_t nrhs, int32_t a_bits, const void* A, int64_t lda,
+                     const double* X, int64_t ldx, const double* B, int64_t ldb,
+                     double* R, int64_t ldr, double* T, int64_t ldt, double* colstats, int32_t* worst);
+/* Double-double residual: (Rhi + Rlo) = B - A X to ~2^-104 relative to |A||X|; Rhi is the FP64 rounding of
+ * the sum and |Rlo| <= ulp(Rhi)/2.  Replaces k_resid_dd + resid_dd_norms (src/mp_kernels.cuh:83-87,
+ * src/mp_core.cuh:207-208). */
+int mp_residual_dd(mp_ctx* ctx, int64_t n, int32_t nrhs, int32_t a_bits, const void* A, int64_t lda,
+                   const double* X, int64_t ldx, const double* B, int64_t ldb,
+                   double* Rhi, double* Rlo, int64_t ldr, double* T, int64_t ldt, double* colstats, int32_t* worst);
+
+typedef struct {
+  int32_t nb;          /* panel width (1..256) */
+  int32_t fmt;         /* MP_FMT_* of the trailing-update operands */
+  int32_t tile_scale;  /* 1: per-32-row/column power-of-two operand scaling; 0: none */
+} mp_getrf_params;
+typedef struct {
+  int32_t info;        /* 0, or 1-based column of the first exactly-zero pivot */
+  int32_t panels;
+  uint64_t conv_overflow, conv_subnormal;   /* operand conversion events */
+} mp_getrf_info;
+/* In-place blocked LU with partial pivoting of an n x n FP32 matrix: P A = L U, unit lower L.
+ * ipiv: 1-based LAPACK pivots (int64, device); perm: row i of P A is row perm[i] of A (int32, device).
+ * Replaces custom_blocked_lu (src/mp_core.cuh:129-150). */
+int mp_getrf(mp_ctx* ctx, int64_t n, float* LU, int64_t lda, int64_t* ipiv, int32_t* perm,
+             const mp_getrf_params* params, mp_getrf_info* info);
+/* D = scale * U^{-1} L^{-1} P R with FP32 factors and FP32 arithmetic, FP64 R and D (device).
+ * Replaces apply_inverse for FP32 factors (src/mp_core.cuh:182-190). */
+int mp_getrs(mp_ctx* ctx, int64_t n, int32_t nrhs, const float* LU, int64_t lda, const int32_t* perm,
+             const double* R, int64_t ldr, double* D, int64_t ldd, double scale);
+
+/* ---------------------------------------------------------------- problems
+ * A (FP64 or FP32 storage, a_bits) and B (FP64) are copied to the device once; ||A||_inf, max|A|, ||b||
+ * and SHA-256 of the column-major bytes are computed at setup (generate_problem, src/mp_core.cuh:51-103).
+ * x_exact / x_ref (host, may be NULL) enable the forward-error metrics. */
+int mp_problem_from_host(mp_ctx* ctx, int64_t n, int32_t nrhs, int32_t a_bits, const void* A, int64_t lda,
+                         const double* B, int64_t ldb, const double* x_exact, const double* x_ref,
+                         const char* case_json, mp_problem** out);
+void mp_problem_free(mp_problem* p);
+/* case record (JSON object: sizes, norms, hashes, case parameters passed at creation) */
+int mp_problem_case_json(mp_problem* p, char* json, size_t cap);
+
+/* ---------------------------------------------------------------- solves
+ * Runs one trial of a method ID (make_method, src/mpsolve.cu:17-72, plus the MI355X IDs listed by
+ * mp_methods_json) with the reference's stopping rules and terminal reasons (Engine::run,
+ * src/mp_core.cuh:299-314).  params_json overrides parameters ("nb", "gmres_m", "gmres_tol",
+ * "gmres_budget", "gmres_cycles", "max_outer", "tau_mult", "target", "cost_ratio_next"); NULL = defaults.
+ * The timed region (device_ready_wall_s) starts with inputs resident and ends after the last kernel.
+ * X_host (n x nrhs FP64, may be NULL) receives the solution.  The full trial record (the reference's
+ * field names, src/mpsolve.cu:221-229) is returned by mp_last_record_json, the histories by
+ * mp_last_hist_json. */
+typedef struct {
+  int32_t ok;                 /* 0 on runtime_error / unsupported */
+  int32_t outer_iters, inner_iters_total, refactorizations;
+  double device_ready_wall_s, factor_total_s, eta_max, omega_max;
+  char terminal_reason[48];
+  char final_stage[48];
+} mp_trial_out;
+int mp_solve(mp_ctx* ctx, mp_problem* p, const char* method_id, const char* params_json,
+             mp_trial_out* out, double* X_host);
+int mp_last_record_json(mp_ctx* ctx, char* json, size_t cap);
+int mp_last_hist_json(mp_ctx* ctx, char* jsonl, size_t cap);
+/* precision contracts of every method ID (methods_catalog.py:50-100), as a JSON array */
+int mp_methods_json(char* json, size_t cap);
+
+/* FP64 metrics of a host solution on the stored system (eval_metrics, src/mp_core.cuh:193-205):
+ * eta/omega per RHS and their maxima, written as a JSON object. */
+int mp_eval_metrics_json(mp_ctx* ctx, mp_problem* p, const double* X_host, char* json, size_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPSOLVE_H */

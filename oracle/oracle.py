@@ -1,0 +1,262 @@
+"""Python side of the CPU oracle.  TEST INFRASTRUCTURE ONLY (used by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker; the product library never imports it).
+
+Wraps liboracle.so (mp_oracle.c) and restates, in numpy, the reference's metric definitions and refinement
+loops:
+  metrics         eval_metrics_T        src/mp_core.cuh:193-205 (eta, omega, relres, worst row)
+  ir_stationary   Engine::refine_stage  src/mp_core.cuh:281-296 (stopping tests and terminal reasons)
+  fgmres          Engine::gmres_solve   src/mp_core.cuh:237-278 (right-preconditioned flexible GMRES)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from fractions import Fraction
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def _load():
+    if not LIB.exists():
+        build()
+    lib = C.CDLL(str(LIB))
+    vp, i64 = C.c_void_p, C.c_int64
+    lib.or_resid_fp64.argtypes = [i64, C.c_int, C.c_int, vp, i64, vp, i64, vp, i64, vp, vp, i64]
+    lib.or_resid_dd.argtypes = [i64, C.c_int, C.c_int, vp, i64, vp, i64, vp, i64, vp, vp, i64]
+    lib.or_colstats.argtypes = [i64, C.c_int, vp, vp, vp, i64, vp, vp]
+    lib.or_getrf.argtypes = [i64, C.c_int, vp, i64, vp, vp, C.c_int, C.c_int, vp]
+    lib.or_getrf.restype = C.c_int
+    lib.or_getrs.argtypes = [i64, C.c_int, vp, i64, vp, vp, i64, vp, i64, C.c_double]
+    for f in ("or_round_fp16", "or_round_bf16", "or_round_e4m3"):
+        getattr(lib, f).argtypes = [C.c_float]
+        getattr(lib, f).restype = C.c_float
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+FMT = {"fp16": 0, "bf16": 1, "fp8": 2, "fp32": 3}
+
+
+def resid_fp64(A, X, B=None):
+    A = np.asfortranarray(A); X = np.asfortranarray(X, dtype=np.float64).reshape(A.shape[0], -1, order="F")
+    n, nrhs = X.shape
+    R = np.empty((n, nrhs), order="F"); T = np.empty((n, nrhs), order="F")
+    Bc = None if B is None else np.asfortranarray(B, dtype=np.float64).reshape(n, nrhs, order="F")
+    lib().or_resid_fp64(n, nrhs, 32 if A.dtype == np.float32 else 64, _p(A), n, _p(X), n, _p(Bc), n, _p(R), _p(T), n)
+    return R, T
+
+
+def resid_dd(A, X, B=None):
+    A = np.asfortranarray(A); X = np.asfortranarray(X, dtype=np.float64).reshape(A.shape[0], -1, order="F")
+    n, nrhs = X.shape
+    Rh = np.empty((n, nrhs), order="F"); Rl = np.empty((n, nrhs), order="F")
+    Bc = None if B is None else np.asfortranarray(B, dtype=np.float64).reshape(n, nrhs, order="F")
+    lib().or_resid_dd(n, nrhs, 32 if A.dtype == np.float32 else 64, _p(A), n, _p(X), n, _p(Bc), n, _p(Rh), _p(Rl), n)
+    return Rh, Rl
+
+
+def colstats(R, X, T):
+    R = np.asfortranarray(R); X = np.asfortranarray(X); T = np.asfortranarray(T)
+    n, nrhs = R.shape
+    out = np.empty(4 * nrhs); w = np.empty(nrhs, dtype=np.int32)
+    lib().or_colstats(n, nrhs, _p(R), _p(X), _p(T), n, _p(out), _p(w))
+    return out.reshape(nrhs, 4), w
+
+
+def getrf(A32, nb=256, fmt="fp16", tile_scale=True):
+    """Blocked LU of a copy of A32 (float32).  Returns (LU, ipiv (1-based), perm, info, counts)."""
+    LU = np.array(A32, dtype=np.float32, order="F", copy=True)
+    n = LU.shape[0]
+    ipiv = np.zeros(n, dtype=np.int64); perm = np.zeros(n, dtype=np.int32); cnt = np.zeros(2, dtype=np.uint64)
+    info = lib().or_getrf(n, nb, _p(LU), n, _p(ipiv), _p(perm), FMT[fmt], 1 if tile_scale else 0, _p(cnt))
+    return LU, ipiv, perm, info, cnt
+
+
+def getrs(LU, perm, R, scale=1.0):
+    LU = np.asfortranarray(LU, dtype=np.float32)
+    n = LU.shape[0]
+    R = np.asfortranarray(R, dtype=np.float64).reshape(n, -1, order="F")
+    D = np.empty_like(R)
+    lib().or_getrs(n, R.shape[1], _p(LU), n, _p(perm), _p(R), n, _p(D), n, float(scale))
+    return D
+
+
+def round_fmt(x, fmt):
+    f = {"fp16": lib().or_round_fp16, "bf16": lib().or_round_bf16, "fp8": lib().or_round_e4m3}[fmt]
+    x = np.asarray(x, dtype=np.float32)
+    return np.array([f(float(v)) for v in x.ravel()], dtype=np.float32).reshape(x.shape)
+
+
+def unpack_lu(LU):
+    L = np.tril(LU.astype(np.float64), -1) + np.eye(LU.shape[0])
+    U = np.triu(LU.astype(np.float64))
+    return L, U
+
+
+# ------------------------------------------------------------------ metrics (src/mp_core.cuh:193-205)
+def norms(A, B):
+    A64 = np.asarray(A, dtype=np.float64)
+    B = np.asarray(B, dtype=np.float64).reshape(A64.shape[0], -1)
+    return np.abs(A64).sum(axis=1).max(), np.abs(B).max(axis=0), np.sqrt((B * B).sum(axis=0))
+
+
+def metrics(A, B, X, dd=False):
+    """eta, omega per RHS from the oracle residual (FP64 sequential FMA or double-double)."""
+    n = A.shape[0]
+    X = np.asarray(X, dtype=np.float64).reshape(n, -1)
+    B = np.asarray(B, dtype=np.float64).reshape(n, -1)
+    if dd:
+        Rh, Rl = resid_dd(A, X, B)
+        R = Rh + Rl
+        _, T = resid_fp64(A, X, B)
+    else:
+        R, T = resid_fp64(A, X, B)
+    st, w = colstats(R, X, T)
+    anorm, binf, b2 = norms(A, B)
+    eta = np.array([0.0 if s[0] == 0 else (np.inf if anorm * s[1] + binf[c] == 0 else s[0] / (anorm * s[1] + binf[c]))
+                    for c, s in enumerate(st)])
+    return dict(eta=eta, omega=st[:, 2], rinf=st[:, 0], xinf=st[:, 1], worst=w, eta_max=eta.max(), omega_max=st[:, 2].max())
+
+
+def exact_residual(A, x, b):
+    """Exact rational residual b - A x for tiny systems (Fraction arithmetic)."""
+    n = A.shape[0]
+    return [Fraction(float(b[i])) - sum(Fraction(float(A[i, j])) * Fraction(float(x[j])) for j in range(n)) for i in range(n)]
+
+
+# ------------------------------------------------------------------ refinement restatements
+def factor(A, fmt="fp16", nb=128, scale_mode=1, tile_scale=True):
+    amax = np.abs(A).max()
+    g = 1.0
+    if scale_mode >= 1 and amax > 0 and np.isfinite(amax):
+        e = np.frexp(amax)[1]
+        g = float(np.ldexp(1.0, -int(e)))
+    A32 = (np.asarray(A, dtype=np.float64) * g).astype(np.float32)
+    LU, ipiv, perm, info, cnt = getrf(A32, nb=nb, fmt=fmt, tile_scale=tile_scale)
+    return dict(LU=LU, ipiv=ipiv, perm=perm, info=info, g=g, counts=cnt)
+
+
+def ir_stationary(A, B, fmt="fp16", nb=128, tau=1e-15, max_outer=50, dd=False, stop_eta_only=True, tile_scale=True):
+    """Stationary IR as in Engine::refine_stage (src/mp_core.cuh:281-296): x0 = A^-1 b, then residual+metrics,
+    stopping tests (converged / nonfinite / max_outer / stagnation), correction, x += d."""
+    F = factor(A, fmt, nb, 1, tile_scale)
+    n = A.shape[0]
+    B = np.asarray(B, dtype=np.float64).reshape(n, -1)
+    x = getrs(F["LU"], F["perm"], B, F["g"])
+    hist = []
+    prev, stall, outer = np.inf, 0, 0
+    while True:
+        m = metrics(A, B, x, dd=dd)
+        hist.append(m["eta_max"])
+        finite = np.isfinite(m["rinf"]).all() and np.isfinite(m["xinf"]).all()
+        if finite and m["eta_max"] <= tau and (stop_eta_only or m["omega_max"] <= tau):
+            return x, "converged", hist
+        if not finite:
+            return x, "nonfinite", hist
+        if outer >= max_outer:
+            return x, "max_outer", hist
+        stall = stall + 1 if m["eta_max"] > 0.9 * prev else 0
+        if stall >= 3:
+            return x, "stagnation", hist
+        prev = m["eta_max"]
+        if dd:
+            Rh, Rl = resid_dd(A, x, B); R = Rh + Rl
+        else:
+            R, _ = resid_fp64(A, x, B)
+        x = x + getrs(F["LU"], F["perm"], R, F["g"])
+        outer += 1
+
+
+def fgmres(A, r, precond, m=30, tol=1e-4, cycles=4, budget=480):
+    """Right-preconditioned flexible GMRES on A d = r (Engine::gmres_solve, src/mp_core.cuh:237-278), with
+    classical Gram-Schmidt applied twice.  Returns (d, status, iterations)."""
+    n = A.shape[0]
+    d = np.zeros(n)
+    beta0 = np.linalg.norm(r)
+    if beta0 == 0 or not np.isfinite(beta0):
+        return d, ("zero_rhs" if beta0 == 0 else "nonfinite"), 0
+    rcur = r.copy(); its = 0; status = "budget"
+    for cyc in range(cycles):
+        if budget <= 0:
+            break
+        betac = np.linalg.norm(rcur)
+        if betac == 0:
+            return d, "converged_true", its
+        V = np.zeros((n, m + 1)); Z = np.zeros((n, m)); H = np.zeros((m + 1, m)); g = np.zeros(m + 1); g[0] = betac
+        cs = np.zeros(m); sn = np.zeros(m)
+        V[:, 0] = rcur / betac
+        j = 0; happy = False
+        while j < m and budget > 0:
+            Z[:, j] = precond(V[:, j])
+            w = A @ Z[:, j]
+            h = V[:, :j + 1].T @ w; w = w - V[:, :j + 1] @ h
+            h2 = V[:, :j + 1].T @ w; w = w - V[:, :j + 1] @ h2; h = h + h2
+            hn = np.linalg.norm(w)
+            H[:j + 1, j] = h; H[j + 1, j] = hn
+            if hn > 0:
+                V[:, j + 1] = w / hn
+            for i in range(j):
+                a, b = H[i, j], H[i + 1, j]
+                H[i, j] = cs[i] * a + sn[i] * b; H[i + 1, j] = -sn[i] * a + cs[i] * b
+            a, b = H[j, j], H[j + 1, j]; rho = np.hypot(a, b)
+            cs[j], sn[j] = (1.0, 0.0) if rho == 0 else (a / rho, b / rho)
+            H[j, j] = rho; H[j + 1, j] = 0.0; g0 = g[j]; g[j] = cs[j] * g0; g[j + 1] = -sn[j] * g0
+            its += 1; budget -= 1; j += 1
+            if hn == 0:
+                happy = True; break
+            if abs(g[j]) / beta0 <= tol:
+                break
+        k = j
+        y = np.linalg.solve(np.triu(H[:k, :k]), g[:k]) if k else np.zeros(0)
+        d = d + Z[:, :k] @ y
+        rcur = r - A @ d
+        true_rel = np.linalg.norm(rcur) / np.linalg.norm(r)
+        if true_rel <= tol:
+            return d, ("happy_breakdown_verified" if happy else "converged_true"), its
+        if happy:
+            return d, "happy_breakdown_unverified", its
+    return d, status, its
+
+
+def fgmres_ir(A, B, fmt="fp16", nb=128, tau=1e-15, max_outer=30, m=30, tol=1e-4, dd=True, tile_scale=True):
+    F = factor(A, fmt, nb, 1, tile_scale)
+    n = A.shape[0]
+    b = np.asarray(B, dtype=np.float64).reshape(n)
+    pre = lambda v: getrs(F["LU"], F["perm"], v, F["g"])[:, 0]
+    x = pre(b)
+    hist = []; inner = 0
+    for outer in range(max_outer + 1):
+        m_ = metrics(A, b, x, dd=dd)
+        hist.append(m_["eta_max"])
+        if m_["eta_max"] <= tau:
+            return x, "converged", hist, inner
+        if dd:
+            Rh, Rl = resid_dd(A, x, b); r = (Rh + Rl)[:, 0]
+        else:
+            r = resid_fp64(A, x, b)[0][:, 0]
+        d, st, its = fgmres(A, r, pre, m=m, tol=tol)
+        inner += its
+        x = x + d
+    return x, "max_outer", hist, inner

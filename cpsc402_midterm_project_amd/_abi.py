@@ -23,7 +23,7 @@ FMT = {"fp16": 0, "bf16": 1, "fp8": 2, "fp8e4m3": 2, "fp32": 3}
 EXPORTS = [
     "mp_create", "mp_destroy", "mp_last_error", "mp_caps", "mp_synchronize", "mp_profile", "mp_profile_json", "mp_malloc", "mp_free",
     "mp_memcpy_h2d", "mp_memcpy_d2h", "mp_memcpy_d2d", "mp_residual_fp64", "mp_residual_dd", "mp_getrf",
-    "mp_getrs", "mp_problem_from_host", "mp_problem_free", "mp_problem_case_json", "mp_solve",
+    "mp_getrs", "mp_problem_from_host", "mp_problem_generate", "mp_problem_reference", "mp_problem_free", "mp_problem_case_json", "mp_solve",
     "mp_last_record_json", "mp_last_hist_json", "mp_methods_json", "mp_eval_metrics_json",
 ]
 
@@ -73,6 +73,8 @@ def _load() -> C.CDLL:
         "mp_getrs": ([vp, i64, i32, vp, i64, vp, dp, i64, dp, i64, C.c_double], C.c_int),
         "mp_problem_from_host": ([vp, i64, i32, i32, vp, i64, dp, i64, dp, dp, C.c_char_p, C.POINTER(vp)], C.c_int),
         "mp_problem_free": ([vp], None),
+        "mp_problem_generate": ([vp, C.c_char_p, C.POINTER(vp)], C.c_int),
+        "mp_problem_reference": ([vp, vp], C.c_int),
         "mp_problem_case_json": ([vp, C.c_char_p, sz], C.c_int),
         "mp_solve": ([vp, vp, C.c_char_p, C.c_char_p, C.POINTER(TrialOut), dp], C.c_int),
         "mp_last_record_json": ([vp, C.c_char_p, sz], C.c_int),
@@ -214,6 +216,21 @@ class Context:
 
 
 class Problem:
+    @classmethod
+    def generate(cls, ctx: "Context", **case) -> "Problem":
+        """Device-generated seeded system (mp_problem_generate)."""
+        self = cls.__new__(cls)
+        h = C.c_void_p()
+        check(lib.mp_problem_generate(ctx.h, json.dumps(case).encode(), C.byref(h)))
+        self.ctx, self.h = ctx, h
+        rec = self.case_record()
+        self.n, self.nrhs = rec["n"], rec["nrhs"]
+        return self
+
+    def reference(self) -> dict:
+        check(lib.mp_problem_reference(self.ctx.h, self.h))
+        return self.case_record()
+
     def __init__(self, ctx: Context, A: np.ndarray, B: np.ndarray, x_exact=None, x_ref=None, case=None):
         A = np.asfortranarray(A)
         if A.dtype not in (np.float64, np.float32):

@@ -618,6 +618,42 @@ static void copy_str(const std::string& s, char* buf, size_t cap) {
   if (s.size() + 1 > cap) throw MPError("output buffer too small (need " + std::to_string(s.size() + 1) + " bytes)", -1);
 }
 
+
+// ||A||_inf, max|A|, ||b||, SHA-256 of the stored bytes (generate_problem diagnostics, src/mp_core.cuh:90-101).
+// src: host copy of the column-major stored A; hb: host copy of B.
+static void finish_setup(mp_ctx* C, mp_problem* Pp, const char* src, const std::vector<double>& hb) {
+  mp_problem* P = Pp;
+  const int64_t n = P->n; const int nrhs = P->nrhs; const size_t es = P->fp32 ? 4 : 8;
+    // ||A||_inf = max_i sum_j |a_ij| via the residual kernel's T = |A| |1|
+    {
+      DBuf<double> ones, Rt, Tt;
+      ones.alloc(C->tracker, n, "ones"); Rt.alloc(C->tracker, n, "rowsum_R"); Tt.alloc(C->tracker, n, "rowsum_T");
+      fill_f64(ones.p, n, 1.0, C->W);
+      ResidArgs ra; ra.n = n; ra.nrhs = 1; ra.a32 = P->fp32; ra.A = P->a_ptr(); ra.lda = n; ra.X = ones.p; ra.ldx = n;
+      ra.R = Rt.p; ra.ldr = n; ra.T = Tt.p; ra.ldt = n; ra.matvec = true;
+      residual(ra, C->W);
+      std::vector<double> rs(n);
+      HIP_CK(hipMemcpyAsync(rs.data(), Tt.p, n * 8, hipMemcpyDeviceToHost, C->W.stream));
+      HIP_CK(hipStreamSynchronize(C->W.stream));
+      double m = 0; for (double v : rs) { if (!(v <= m)) m = v; }
+      P->anorm_inf = m;
+    }
+    P->amax = dev_absmax(C, P->a_ptr(), P->fp32, n, n);
+    P->bnorm_inf.assign(nrhs, 0); P->bnorm_2.assign(nrhs, 0);
+    for (int c = 0; c < nrhs; c++) {
+      double m = 0, s = 0;
+      for (int64_t i = 0; i < n; i++) { const double v = std::fabs(hb[i + (size_t)c * n]); m = std::max(m, v); s += v * v; }
+      P->bnorm_inf[c] = m; P->bnorm_2[c] = std::sqrt(s);
+    }
+    P->a_sha = sha256_hex(src, (size_t)n * n * es);
+    const size_t colbytes = (size_t)n * es;
+    for (int t = 0; t < 16; t++) {
+      const size_t c0 = (size_t)n * t / 16, c1 = (size_t)n * (t + 1) / 16;
+      P->tile_sha.push_back(sha256_hex(src + c0 * colbytes, (c1 - c0) * colbytes));
+    }
+    P->b_sha = sha256_hex(hb.data(), hb.size() * 8);
+}
+
 extern "C" {
 
 const char* mp_last_error(void) { return g_last_error.c_str(); }
@@ -757,36 +793,101 @@ int mp_problem_from_host(mp_ctx* C, int64_t n, int32_t nrhs, int32_t a_bits, con
     HIP_CK(hipMemcpy(P->B.p, hb.data(), hb.size() * 8, hipMemcpyHostToDevice));
     if (x_exact) { P->hXexact.assign(x_exact, x_exact + (size_t)n * nrhs); P->has_exact = true; }
     if (x_ref) { P->hXref.assign(x_ref, x_ref + (size_t)n * nrhs); P->has_ref = true; }
-    // ||A||_inf = max_i sum_j |a_ij| via the residual kernel's T = |A| |1|
-    {
-      DBuf<double> ones, Rt, Tt;
-      ones.alloc(C->tracker, n, "ones"); Rt.alloc(C->tracker, n, "rowsum_R"); Tt.alloc(C->tracker, n, "rowsum_T");
-      fill_f64(ones.p, n, 1.0, C->W);
-      ResidArgs ra; ra.n = n; ra.nrhs = 1; ra.a32 = P->fp32; ra.A = P->a_ptr(); ra.lda = n; ra.X = ones.p; ra.ldx = n;
-      ra.R = Rt.p; ra.ldr = n; ra.T = Tt.p; ra.ldt = n; ra.matvec = true;
-      residual(ra, C->W);
-      std::vector<double> rs(n);
-      HIP_CK(hipMemcpyAsync(rs.data(), Tt.p, n * 8, hipMemcpyDeviceToHost, C->W.stream));
-      HIP_CK(hipStreamSynchronize(C->W.stream));
-      double m = 0; for (double v : rs) { if (!(v <= m)) m = v; }
-      P->anorm_inf = m;
-    }
-    P->amax = dev_absmax(C, P->a_ptr(), P->fp32, n, n);
-    P->bnorm_inf.assign(nrhs, 0); P->bnorm_2.assign(nrhs, 0);
-    for (int c = 0; c < nrhs; c++) {
-      double m = 0, s = 0;
-      for (int64_t i = 0; i < n; i++) { const double v = std::fabs(hb[i + (size_t)c * n]); m = std::max(m, v); s += v * v; }
-      P->bnorm_inf[c] = m; P->bnorm_2[c] = std::sqrt(s);
-    }
-    P->a_sha = sha256_hex(src, (size_t)n * n * es);
-    const size_t colbytes = (size_t)n * es;
-    for (int t = 0; t < 16; t++) {
-      const size_t c0 = (size_t)n * t / 16, c1 = (size_t)n * (t + 1) / 16;
-      P->tile_sha.push_back(sha256_hex(src + c0 * colbytes, (c1 - c0) * colbytes));
-    }
-    P->b_sha = sha256_hex(hb.data(), hb.size() * 8);
+    finish_setup(C, P.get(), src, hb);
     P->case_json = case_json ? case_json : "";
     *out = P.release();
+  });
+}
+
+int mp_problem_generate(mp_ctx* C, const char* case_json, mp_problem** out) {
+  return guard([&] {
+    MP_REQUIRE(C && case_json && out, "NULL argument");
+    Params a; a.parse(case_json);
+    GenSpec g;
+    g.n = (int64_t)a.getd("n", 1024); g.nrhs = a.geti("nrhs", 1); g.family = a.get("family", "randsvd");
+    g.spectrum = a.get("spectrum", "geometric"); g.rhs = a.get("rhs", g.family == "dyadic" ? "exact_sign" : "xtrue_uniform");
+    g.kappa = a.getd("kappa", 100); g.seed = (uint64_t)a.getd("seed", 9200); g.ell = a.getd("ell", 0.25); g.nugget = a.getd("nugget", 1e-3);
+    g.dim = a.geti("dim", 3); g.intmax = a.geti("intmax", 255); g.dyadic_exp = a.geti("dyadic_exp", 8); g.grcar_k = a.geti("grcar_k", 3);
+    g.global_exp = a.geti("global_exp", 0);
+    MP_REQUIRE(g.n > 0 && g.nrhs > 0, "n and nrhs must be positive");
+    MP_REQUIRE(a.get("dtype", "fp64") == "fp64", "only fp64 stored systems are generated on the device");
+    std::unique_ptr<mp_problem> P(new mp_problem());
+    P->ctx = C; P->n = g.n; P->nrhs = g.nrhs; P->fp32 = false;
+    const size_t nn = (size_t)g.n * g.n, nb = (size_t)g.n * g.nrhs;
+    P->A64.alloc(C->tracker, nn, "A64"); P->B.alloc(C->tracker, nb, "B");
+    DBuf<double> Xe; Xe.alloc(C->tracker, nb, "Xexact");
+    const double t0 = wall_now();
+    const bool exact = generate(g, (void*)C->rb, C->W, C->tracker, P->A64.p, P->B.p, Xe.p);
+    const double tg = wall_now() - t0;
+    std::vector<char> ha(nn * 8);
+    std::vector<double> hb(nb);
+    HIP_CK(hipMemcpy(ha.data(), P->A64.p, nn * 8, hipMemcpyDeviceToHost));
+    HIP_CK(hipMemcpy(hb.data(), P->B.p, nb * 8, hipMemcpyDeviceToHost));
+    if (exact) { P->hXexact.resize(nb); HIP_CK(hipMemcpy(P->hXexact.data(), Xe.p, nb * 8, hipMemcpyDeviceToHost)); P->has_exact = true; }
+    const double th = wall_now();
+    finish_setup(C, P.get(), ha.data(), hb);
+    Json j; j.d("gen_seconds", tg).d("hash_seconds", wall_now() - th).s("generator", "device: counter-based Gaussian/uniform streams, rocSOLVER geqrf/orgqr, rocBLAS dgemm");
+    std::string cj = case_json; while (!cj.empty() && isspace((unsigned char)cj.back())) cj.pop_back();
+    std::string jj = j.obj();
+    P->case_json = (cj.size() > 2) ? cj.substr(0, cj.size() - 1) + "," + jj.substr(1) : jj;
+    *out = P.release();
+  });
+}
+
+// Reference solution (compute_reference, src/mpsolve.cu:99-125): FP64 LU + solve, then up to 12 steps of
+// refinement with the double-double residual, keeping the iterate with the smallest DD residual.
+int mp_problem_reference(mp_ctx* C, mp_problem* P) {
+  return guard([&] {
+    MP_REQUIRE(C && P, "NULL argument");
+    MP_REQUIRE(!P->fp32, "reference solutions of FP32-stored systems are not implemented");
+    const double t0 = wall_now();
+    const int64_t n = P->n; const int nrhs = P->nrhs; const size_t nb = (size_t)n * nrhs;
+    Factors F;
+    factorize(C, *P, "cs64", 256, 0, F);
+    Json info;
+    if (F.info > 0) {
+      info.s("reference_grade", "unavailable").s("reference_note", "reference LU singular at pivot " + std::to_string(F.info));
+    } else {
+      DBuf<double> X, Xbest, Rh, Rl, D;
+      X.alloc(C->tracker, nb, "Xref"); Xbest.alloc(C->tracker, nb, "Xbest"); Rh.alloc(C->tracker, nb, "Rhi"); Rl.alloc(C->tracker, nb, "Rlo"); D.alloc(C->tracker, nb, "D");
+      apply_inverse(C, F, P->B.p, X.p, nrhs, nullptr);
+      double best = INFINITY, best_abs = -1; int it = 0;
+      std::vector<double> st(4 * nrhs);
+      DBuf<double> stats; stats.alloc(C->tracker, 4 * nrhs + 4, "ref_stats");
+      for (it = 0; it < 12; it++) {
+        ResidArgs ra; ra.n = n; ra.nrhs = nrhs; ra.A = P->A64.p; ra.lda = n; ra.X = X.p; ra.ldx = n; ra.B = P->B.p; ra.ldb = n;
+        ra.R = Rh.p; ra.Rlo = Rl.p; ra.ldr = n; ra.dd = true; ra.stats = stats.p;
+        residual(ra, C->W);
+        HIP_CK(hipMemcpyAsync(st.data(), stats.p, 32 * nrhs, hipMemcpyDeviceToHost, C->W.stream));
+        HIP_CK(hipStreamSynchronize(C->W.stream));
+        double rm = 0, xm = 0;
+        for (int c = 0; c < nrhs; c++) { rm = std::max(rm, st[4 * c]); xm = std::max(xm, st[4 * c + 1]); }
+        const double rel = (P->anorm_inf * xm > 0) ? rm / (P->anorm_inf * xm) : rm;
+        if (!std::isfinite(rel)) break;
+        if (rel < best) { best = rel; best_abs = rm; copy_f64(X.p, Xbest.p, (int64_t)nb, C->W); } else break;
+        if (rel < 1e-30) break;
+        apply_inverse(C, F, Rh.p, D.p, nrhs, nullptr);
+        axpy(X.p, D.p, (int64_t)nb, 1.0, C->W);
+      }
+      P->hXref.resize(nb);
+      HIP_CK(hipMemcpyAsync(P->hXref.data(), Xbest.p, nb * 8, hipMemcpyDeviceToHost, C->W.stream));
+      HIP_CK(hipStreamSynchronize(C->W.stream));
+      P->has_ref = std::isfinite(best);
+      info.s("reference_grade", P->has_ref ? "numerical_reference" : "unavailable")
+          .s("reference_method", "rocsolver_dgetrf_lu_plus_double_double_residual_refinement")
+          .d("reference_resid_dd_rel", best).d("reference_resid_dd_abs", best_abs).i("reference_iters", it)
+          .d("growth_proxy_reference_lu", F.growth);
+      if (P->has_exact) {
+        double m0 = 0, m1 = 0;
+        for (size_t i = 0; i < nb; i++) { m0 = std::max(m0, std::fabs(P->hXref[i] - P->hXexact[i])); m1 = std::max(m1, std::fabs(P->hXexact[i])); }
+        info.d("reference_vs_exact_finf", m1 > 0 ? m0 / m1 : m0);
+      }
+    }
+    info.d("reference_seconds", wall_now() - t0);
+    std::string jj = info.obj();
+    std::string& cj = P->case_json;
+    while (!cj.empty() && isspace((unsigned char)cj.back())) cj.pop_back();
+    cj = (cj.size() > 2) ? cj.substr(0, cj.size() - 1) + "," + jj.substr(1) : jj;
   });
 }
 

@@ -84,6 +84,16 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out);
 void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* perm,
            const double* R, int64_t ldr, double* D, int64_t ldd, double scale, Workspace& W);
 
+struct GenSpec {
+  int64_t n = 1024; int nrhs = 1; std::string family = "randsvd", spectrum = "geometric", rhs = "xtrue_uniform";
+  double kappa = 100; uint64_t seed = 9200; double ell = 0.25, nugget = 1e-3; int dim = 3, intmax = 255, dyadic_exp = 8, grcar_k = 3;
+  int global_exp = 0;
+};
+// device generation of A (n x n), B and X_exact (n x nrhs); returns whether X_exact holds the exact solution
+bool generate(const GenSpec& g, void* rocblas_handle, Workspace& W, DevTracker& tr, double* A, double* B, double* Xe);
+void axpy_scale_inplace(double* x, int64_t n, double s, Workspace& W);
+double spectrum_value(const std::string& sp, int64_t i, int64_t n, double kappa);
+
 void scale_convert_f64_to_f32(const double* A, int64_t n, int64_t lda, float* out, int64_t ldo, double s, Workspace& W);
 void fill_iota(int32_t* p, int64_t n, Workspace& W);
 

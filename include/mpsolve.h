@@ -97,6 +97,14 @@ int mp_getrs(mp_ctx* ctx, int64_t n, int32_t nrhs, const float* LU, int64_t lda,
 int mp_problem_from_host(mp_ctx* ctx, int64_t n, int32_t nrhs, int32_t a_bits, const void* A, int64_t lda,
                          const double* B, int64_t ldb, const double* x_exact, const double* x_ref,
                          const char* case_json, mp_problem** out);
+/* Device generation of a seeded test system from a case object (JSON): "family" (randsvd | dyadic | rbf |
+ * grcar | wilkinson), "n", "nrhs", "kappa", "spectrum", "seed", "rhs" (xtrue_uniform | random | random_scaled |
+ * exact_sign | unit_column), "intmax", "dyadic_exp", "ell", "nugget", "dim", "grcar_k", "global_exp".
+ * Replaces generate_problem (src/mp_core.cuh:51-103); the random streams are this library's own. */
+int mp_problem_generate(mp_ctx* ctx, const char* case_json, mp_problem** out);
+/* FP64 LU solution refined with double-double residuals (best of <= 12 steps), used for the forward-error
+ * metrics; replaces compute_reference (src/mpsolve.cu:99-125) without the kappa_2 estimate. */
+int mp_problem_reference(mp_ctx* ctx, mp_problem* p);
 void mp_problem_free(mp_problem* p);
 /* case record (JSON object: sizes, norms, hashes, case parameters passed at creation) */
 int mp_problem_case_json(mp_problem* p, char* json, size_t cap);

@@ -78,8 +78,13 @@ def test_getrf_exact_bitwise(ctx, oracle, fmt, n, nb):
     np.testing.assert_array_equal(LU, LUo)
 
 
-@pytest.mark.parametrize("fmt,tol", [("fp16", 3e-3), ("bf16", 2e-2), ("fp8", 0.2), ("fp32", 1e-5)])
-def test_getrf_random_backward_error(ctx, oracle, fmt, tol):
+@pytest.mark.parametrize("fmt", ["fp16", "bf16", "fp8", "fp32"])
+def test_getrf_random_backward_error(ctx, oracle, fmt):
+    """Random Gaussian matrix: the GPU factors have the backward error of the same algorithm run by the oracle
+    (within a factor 4 either way: too small would mean the operand rounding was skipped; the MFMA sums in
+    FP32 where the oracle sums in FP64 then rounds), |L| <= 1, and the composed permutation equals the LAPACK
+    swap sequence.  The first panel involves no trailing update, so its pivots must equal the oracle's; after
+    that a near-tie resolved differently by the two summation orders changes every later pivot."""
     n, nb = 1024, 128
     print(f"[getrf random] fmt={fmt} n={n}", flush=True)
     rng = np.random.default_rng(5)
@@ -89,12 +94,13 @@ def test_getrf_random_backward_error(ctx, oracle, fmt, tol):
     LU = dLU.to_array((n, n), np.float32); pm = dpm.to_array((n,), np.int32); ipiv = dip.to_array((n,), np.int64)
     L, U = oracle.unpack_lu(LU)
     err = np.abs(A[pm].astype(np.float64) - L @ U).max() / np.abs(A).max()
-    assert info.info == 0 and err < tol, err
-    assert np.abs(np.tril(LU, -1)).max() <= 1.0
     LUo, ipo, pmo, _, _ = oracle.getrf(A, nb, fmt, True)
-    agree = np.mean(ipiv == ipo)
-    assert agree > 0.9, agree
-    # perm is the composition of the LAPACK swaps
+    Lo, Uo = oracle.unpack_lu(LUo)
+    err_o = np.abs(A[pmo].astype(np.float64) - Lo @ Uo).max() / np.abs(A).max()
+    assert info.info == 0
+    assert 0.25 * err_o <= err <= 4.0 * err_o, (err, err_o)
+    assert np.abs(np.tril(LU, -1)).max() <= 1.0
+    np.testing.assert_array_equal(ipiv[:nb], ipo[:nb])
     p = np.arange(n)
     for i in range(n):
         j = ipiv[i] - 1

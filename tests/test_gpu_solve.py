@@ -1,0 +1,99 @@
+"""End-to-end solves through mp_solve: stopping semantics and terminal reasons of the reference engine
+(src/mp_core.cuh:281-314), final errors checked by the CPU oracle (double-double residual), determinism."""
+import numpy as np
+import pytest
+
+from cpsc402_midterm_project_amd.problems import randsvd
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sys_k1e4():
+    return randsvd(1024, 1e4, 9200, gpu=False)
+
+
+def test_fgmres_ir_dd_reaches_fp64_accuracy(ctx, oracle, sys_k1e4):
+    A, b, xt, _ = sys_k1e4
+    print("[solve] mx16_gmresir_w64_rdd n=1024 kappa=1e4", flush=True)
+    P = ctx.problem(A, b, x_exact=xt)
+    X, rec = P.solve("mx16_gmresir_w64_rdd", nb=128, gmres_m=30, tau=1e-15, stop_on="eta")
+    assert rec["ok"] and rec["terminal_reason"] == "converged", rec
+    m = oracle.metrics(A, b, X, dd=True)
+    assert m["eta_max"] <= 1e-15
+    assert abs(m["eta_max"] - rec["eta_max"]) <= 1e-17 + 0.01 * m["eta_max"]
+    # forward error against x_true is at the kappa * u level
+    assert rec["einf"][0] < 1e4 * 2.0 ** -53 * 50
+    hist = P.history()
+    assert hist and hist[0]["kind"] == "outer" and hist[-1]["eta_max"] == rec["solver_final_eta_max"]
+
+
+def test_solve_is_deterministic(ctx, sys_k1e4):
+    A, b, xt, _ = sys_k1e4
+    P = ctx.problem(A, b)
+    X1, r1 = P.solve("mx16_gmresir_w64_rdd", nb=128, gmres_m=30, tau=1e-15, stop_on="eta")
+    X2, r2 = P.solve("mx16_gmresir_w64_rdd", nb=128, gmres_m=30, tau=1e-15, stop_on="eta")
+    assert r1["x_sha256"] == r2["x_sha256"]
+    np.testing.assert_array_equal(X1, X2)
+
+
+@pytest.mark.parametrize("method", ["hetero16_ir_w64_r64", "custom32_ir_w64_r64", "native_ir32_w64_r64", "native_ir64_w64_r64",
+                                    "mxbf16_gmresir_w64_rdd", "mx16_ir_w64_rdd", "native_gmresir32_w64_r64"])
+def test_reference_methods_converge_kappa1e2(ctx, oracle, method):
+    A, b, xt, _ = randsvd(768, 1e2, 9201, gpu=False)
+    print(f"[solve] {method} n=768 kappa=1e2", flush=True)
+    P = ctx.problem(A, b, x_exact=xt)
+    X, rec = P.solve(method, nb=128)
+    assert rec["ok"] and rec["terminal_reason"] == "converged", (rec["terminal_reason"], rec["error"], rec["eta_max"])
+    tau = 100 * 2.0 ** -53
+    m = oracle.metrics(A, b, X, dd=True)
+    assert m["eta_max"] <= tau and m["omega_max"] <= tau
+
+
+def test_direct_fp64_lu(ctx, oracle):
+    A, b, xt, _ = randsvd(1000, 1e3, 9202, gpu=False)
+    P = ctx.problem(A, b)
+    X, rec = P.solve("native_lu64")
+    assert rec["terminal_reason"] == "direct_solve" and rec["outer_iters"] == 0
+    m = oracle.metrics(A, b, X, dd=True)
+    assert m["eta_max"] < 1e-15
+    ref = np.linalg.solve(A, b)
+    assert np.abs(X - ref).max() / np.abs(ref).max() < 1e3 * 1e-13
+
+
+def test_singular_and_unsupported_outcomes(ctx):
+    n = 256
+    A = np.ones((n, n)); b = np.ones((n, 1))
+    P = ctx.problem(A, b)
+    X, rec = P.solve("mx16_ir_w64_r64", nb=64)
+    assert rec["terminal_reason"] == "singular_factor" and rec["factor_info"] == 2
+    X, rec = P.solve("cusolver_irs_m64_l16_classical_fb_default")
+    assert not rec["ok"] and rec["terminal_reason"] == "unsupported_on_platform"
+    X, rec = P.solve("hetero16_ir_w64_r64_twosided")
+    assert rec["terminal_reason"] == "unsupported_on_platform"
+
+
+def test_unattainable_tau_stagnates(ctx):
+    A, b, xt, _ = randsvd(512, 1e2, 3, gpu=False)
+    P = ctx.problem(A, b)
+    X, rec = P.solve("mx16_ir_w64_r64", nb=64, tau_mult=0.001)
+    assert rec["terminal_reason"] in ("stagnation", "max_outer")
+
+
+def test_precision_ladder_escalates(ctx, oracle):
+    A, b, xt, _ = randsvd(512, 1e6, 4, gpu=False)
+    print("[solve] policy_mx8_ladder n=512 kappa=1e6", flush=True)
+    P = ctx.problem(A, b)
+    X, rec = P.solve("policy_mx8_ladder", nb=64, gmres_m=30, gmres_budget=120, tau=1e-15, stop_on="eta")
+    assert rec["used_fallback"] and rec["refactorizations"] >= 2, rec["stages"]
+    assert rec["terminal_reason"] == "converged", (rec["stages"], rec["terminal_reason"])
+    assert oracle.metrics(A, b, X, dd=True)["eta_max"] <= 1e-15
+
+
+def test_multiple_rhs_stationary(ctx, oracle):
+    A, b, xt, _ = randsvd(640, 1e3, 6, nrhs=5, gpu=False)
+    P = ctx.problem(A, b, x_exact=xt)
+    X, rec = P.solve("mx16_ir_w64_rdd", nb=64)
+    assert rec["terminal_reason"] == "converged" and len(rec["eta"]) == 5
+    m = oracle.metrics(A, b, X, dd=True)
+    assert m["eta_max"] <= 100 * 2.0 ** -53

@@ -1,0 +1,242 @@
+"""CPU tests: the oracle pinned against the reference's own known-answer tests and against what the
+mathematics fixes (exact rationals, closed forms, LAPACK).
+
+Known answers taken from e1_scalar/three_precision_lab/tests/test_arithmetic.py:
+  residual precision (lines 54-57), FP16 tie-to-even / subnormal / overflow (lines 23-33), the rational
+  rounding oracle (lines 7-20, 41-51), GEPP reconstruction tolerances (lines 61-66), the exact pivoted
+  3x3 solve (lines 67-70) and the singular 2x2 (lines 80-81).
+"""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from cpsc402_midterm_project_amd.problems import exact_lu_system, randsvd, sha256_colmajor
+
+
+# ------------------------------------------------------------------ rounding: independent rational oracles
+def _nearest_even(value: Fraction, grid_step_for, top: Fraction | None):
+    """Round `value` to the nearest point of a binary floating grid; ties go to the even significand."""
+    if value == 0:
+        return Fraction(0)
+    sign = -1 if value < 0 else 1
+    a = abs(value)
+    step = grid_step_for(a)
+    q, r = divmod(a, step)
+    q = int(q)
+    if r * 2 > step or (r * 2 == step and q % 2 == 1):
+        q += 1
+    out = q * step
+    if top is not None and out > top:
+        return None
+    return sign * out
+
+
+def _step_fp16(a: Fraction) -> Fraction:
+    if a < Fraction(1, 2 ** 14):
+        return Fraction(1, 2 ** 24)
+    e = 0
+    while Fraction(2) ** (e + 1) <= a:
+        e += 1
+    while Fraction(2) ** e > a:
+        e -= 1
+    return Fraction(2) ** (e - 10)
+
+
+def _step_e4m3(a: Fraction) -> Fraction:
+    if a < Fraction(1, 64):
+        return Fraction(1, 512)
+    e = 0
+    while Fraction(2) ** (e + 1) <= a:
+        e += 1
+    while Fraction(2) ** e > a:
+        e -= 1
+    return Fraction(2) ** (e - 3)
+
+
+def test_fp16_rounding_known_answers(oracle):
+    r = lambda v: float(oracle.round_fmt(np.float32(v), "fp16"))
+    assert r(1 + 2 ** -11) == 1.0                      # tie to even
+    assert r(1 + 2 ** -10 + 2 ** -11) == 1 + 2 ** -9   # tie to even (upwards)
+    assert r(2 ** -24) == 2 ** -24                     # smallest subnormal
+    assert r(2 ** -25) == 0.0                          # tie to even at the bottom
+    assert np.isinf(r(65504.0 * 2))                    # overflow
+
+
+def test_fp16_rounding_rational_oracle(oracle):
+    rng = np.random.default_rng(975)
+    halfs = rng.integers(0, 65536, 300).astype(np.uint16).view(np.float16)
+    halfs = halfs[np.isfinite(halfs)].astype(np.float32)
+    vals = np.concatenate([rng.standard_normal(600) * 10.0 ** rng.integers(-9, 5, 600), halfs])
+    vals = vals.astype(np.float32)
+    vals = vals[np.isfinite(vals) & (np.abs(vals) < 65504)]
+    got = oracle.round_fmt(vals, "fp16")
+    for v, g in zip(vals, got):
+        want = _nearest_even(Fraction(float(v)), _step_fp16, None)
+        assert Fraction(float(g)) == want, (v, g, want)
+    # numpy's float16 conversion is an independent implementation of the same rule
+    np.testing.assert_array_equal(got, vals.astype(np.float16).astype(np.float32))
+
+
+def test_bf16_rounding(oracle):
+    rng = np.random.default_rng(3)
+    vals = (rng.standard_normal(500) * 10.0 ** rng.integers(-20, 20, 500)).astype(np.float32)
+    got = oracle.round_fmt(vals, "bf16")
+    for v, g in zip(vals, got):
+        a = Fraction(float(abs(v)))
+        e = 0
+        while Fraction(2) ** (e + 1) <= a:
+            e += 1
+        while Fraction(2) ** e > a:
+            e -= 1
+        want = _nearest_even(Fraction(float(v)), lambda x, e=e: Fraction(2) ** (e - 7), None)
+        assert Fraction(float(g)) == want
+
+
+def test_e4m3_rounding(oracle):
+    rng = np.random.default_rng(4)
+    vals = (rng.standard_normal(800) * 2.0 ** rng.integers(-12, 9, 800)).astype(np.float32)
+    vals = vals[np.abs(vals) <= 448]
+    got = oracle.round_fmt(vals, "fp8")
+    for v, g in zip(vals, got):
+        want = _nearest_even(Fraction(float(v)), _step_e4m3, None)
+        want = max(min(want, Fraction(448)), Fraction(-448))
+        assert Fraction(float(g)) == want, (v, g, want)
+    # format landmarks: max normal, min normal, min subnormal, saturation
+    r = lambda v: float(oracle.round_fmt(np.float32(v), "fp8"))
+    assert r(448) == 448 and r(1000) == 448 and r(-1000) == -448
+    assert r(2 ** -6) == 2 ** -6 and r(2 ** -9) == 2 ** -9 and r(2 ** -10) == 0.0
+    assert r(1 + 2 ** -4) == 1.0 and r(1 + 3 * 2 ** -4) == 1.25   # ties to even
+
+
+# ------------------------------------------------------------------ residuals
+def test_residual_precision_known_answer(oracle):
+    # A = [[1, 2^-24]] stored FP32, b = [1], x = [1, 1]: the FP64 residual is exactly -2^-24
+    A = np.zeros((2, 2), dtype=np.float32, order="F"); A[0, 0] = 1; A[0, 1] = 2 ** -24; A[1, 1] = 1
+    b = np.array([1.0, 1.0]); x = np.array([1.0, 1.0])
+    R, T = oracle.resid_fp64(A, x, b)
+    assert R[0, 0] == -(2 ** -24)
+    assert T[0, 0] == 2 + 2 ** -24
+    Rh, Rl = oracle.resid_dd(A, x, b)
+    assert Rh[0, 0] == -(2 ** -24) and Rl[0, 0] == 0.0
+
+
+@pytest.mark.parametrize("n", [3, 8, 17])
+def test_residuals_against_exact_rationals(oracle, n):
+    rng = np.random.default_rng(n)
+    A = np.asfortranarray(rng.standard_normal((n, n)))
+    x = rng.standard_normal(n) * 1e3
+    b = A @ x + rng.standard_normal(n) * 1e-9           # nearly consistent: heavy cancellation
+    exact = oracle.exact_residual(A, x, b)
+    R, T = oracle.resid_fp64(A, x, b)
+    Rh, Rl = oracle.resid_dd(A, x, b)
+    u = 2.0 ** -53
+    for i in range(n):
+        e = exact[i]
+        assert abs(Fraction(float(R[i, 0])) - e) <= Fraction((n + 2) * u * T[i, 0])
+        assert abs(Fraction(float(Rh[i, 0])) + Fraction(float(Rl[i, 0])) - e) <= Fraction(n * 2.0 ** -100 * T[i, 0])
+        assert abs(Rl[i, 0]) <= np.spacing(abs(Rh[i, 0])) / 2
+
+
+def test_colstats_conventions(oracle):
+    R = np.array([[0.0], [1.0], [2.0], [0.0]]); X = np.array([[1.0], [-3.0], [2.0], [0.5]])
+    T = np.array([[0.0], [4.0], [0.0], [1.0]])
+    st, w = oracle.colstats(R, X, T)
+    assert st[0, 0] == 2.0 and st[0, 1] == 3.0
+    assert st[0, 2] == np.inf and w[0] == 2           # x/0 -> inf, 0/0 -> 0
+    assert st[0, 3] == 5.0
+
+
+# ------------------------------------------------------------------ LU
+def test_exact_pivoted_solve(oracle):
+    A = np.array([[0, 2, 1], [1, 0, 0], [0, 1, 1]], dtype=np.float32, order="F")
+    x = np.array([1.0, 2.0, 3.0])
+    for fmt in ("fp16", "bf16", "fp8", "fp32"):
+        for nb in (1, 2, 3):
+            LU, ipiv, perm, info, _ = oracle.getrf(A, nb, fmt)
+            assert info == 0
+            np.testing.assert_array_equal(oracle.getrs(LU, perm, A.astype(np.float64) @ x)[:, 0], x)
+
+
+def test_singular_is_reported(oracle):
+    A = np.array([[1, 1], [1, 1]], dtype=np.float32, order="F")
+    assert oracle.getrf(A, 2, "fp16")[3] == 2
+
+
+@pytest.mark.parametrize("fmt,tol", [("fp16", 0.02), ("fp32", 1e-5)])
+def test_gepp_reconstruction(oracle, fmt, tol):
+    rng = np.random.default_rng(12)
+    a = rng.normal(size=(9, 9)); a[0, 0] = 0
+    A = a.astype(np.float32)
+    LU, ipiv, perm, info, _ = oracle.getrf(A, 4, fmt)
+    L, U = oracle.unpack_lu(LU)
+    err = np.linalg.norm(A[perm].astype(np.float64) - L @ U, np.inf) / np.linalg.norm(A, np.inf)
+    assert err < tol
+
+
+def test_fp32_blocked_lu_matches_lapack_pivots(oracle):
+    import scipy.linalg as sl
+    A, L, U, perm = exact_lu_system(96, 5, "fp32")
+    lu, piv = sl.lu_factor(A.astype(np.float64))
+    LU, ipiv, pm, info, _ = oracle.getrf(A, 32, "fp32")
+    np.testing.assert_array_equal(ipiv - 1, piv)
+    np.testing.assert_array_equal(LU.astype(np.float64), lu)
+
+
+@pytest.mark.parametrize("fmt", ["fp16", "bf16", "fp8", "fp32"])
+def test_exact_system_recovers_constructed_factors(oracle, fmt):
+    A, L, U, perm = exact_lu_system(160, 9, fmt)
+    for nb in (16, 48, 160):
+        LU, ipiv, pm, info, cnt = oracle.getrf(A, nb, fmt)
+        Lc, Uc = oracle.unpack_lu(LU)
+        np.testing.assert_array_equal(pm, perm)
+        np.testing.assert_array_equal(Lc, L)
+        np.testing.assert_array_equal(Uc, U)
+        assert info == 0 and cnt[0] == 0
+
+
+def test_trailing_format_error_levels(oracle):
+    """A plausible mistake (e.g. skipping the operand rounding, or rounding to the wrong format) shows up as a
+    backward error at the wrong level."""
+    rng = np.random.default_rng(8)
+    A = rng.standard_normal((256, 256)).astype(np.float32)
+    errs = {}
+    for fmt in ("fp32", "bf16", "fp16", "fp8"):
+        LU, ipiv, perm, info, _ = oracle.getrf(A, 64, fmt)
+        L, U = oracle.unpack_lu(LU)
+        errs[fmt] = np.abs(A[perm].astype(np.float64) - L @ U).max() / np.abs(A).max()
+    assert errs["fp32"] < 1e-5 < errs["fp16"] < 2e-2 < errs["bf16"] < 0.2 < errs["fp8"] < 3.0
+
+
+def test_getrs_is_lu_solve(oracle):
+    import scipy.linalg as sl
+    A, b, xt, _ = randsvd(128, 1e2, 1, gpu=False)
+    F = oracle.factor(A, "fp32", 32, scale_mode=1)
+    d = oracle.getrs(F["LU"], F["perm"], b, F["g"])
+    L, U = oracle.unpack_lu(F["LU"])
+    ref = sl.solve_triangular(U, sl.solve_triangular(L, b[F["perm"]], lower=True, unit_diagonal=True)) * F["g"]
+    assert np.abs(d - ref).max() / np.abs(ref).max() < 1e-4
+
+
+# ------------------------------------------------------------------ generator and refinement restatements
+def test_randsvd_spectrum_and_hash():
+    A, b, xt, rec = randsvd(64, 1e4, 9200, gpu=False)
+    s = np.linalg.svd(A, compute_uv=False)
+    assert abs(s[0] - 1) < 1e-12 and abs(s[-1] - 1e-4) < 1e-15
+    np.testing.assert_allclose(b, A @ xt, rtol=0, atol=1e-15)
+    A2, *_ = randsvd(64, 1e4, 9200, gpu=False)
+    assert sha256_colmajor(A) == sha256_colmajor(A2)
+
+
+def test_stationary_ir_converges_kappa1e2(oracle):
+    A, b, xt, _ = randsvd(256, 1e2, 9200, gpu=False)
+    x, st, hist = oracle.ir_stationary(A, b, "fp16", 64, tau=1e-15, dd=True)
+    assert st == "converged" and hist[-1] <= 1e-15
+
+
+def test_fgmres_ir_converges_and_fp8_stagnates(oracle):
+    A, b, xt, _ = randsvd(256, 1e4, 9201, gpu=False)
+    x, st, hist, inner = oracle.fgmres_ir(A, b, "fp16", 64)
+    assert st == "converged" and hist[-1] <= 1e-15
+    x, st, hist = oracle.ir_stationary(A, b, "fp8", 64, tau=1e-15, dd=True, max_outer=10)
+    assert st in ("stagnation", "max_outer")

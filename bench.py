@@ -66,8 +66,7 @@ def main():
         print("bench.py measures one GPU: this solver build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
-    from cpsc402_midterm_project_amd import Context
-    from cpsc402_midterm_project_amd.problems import randsvd, sha256_colmajor
+    from cpsc402_midterm_project_amd import Context, Problem
 
     n = args.n
     kappas = [float(k) for k in args.kappas.split(",")]
@@ -82,10 +81,12 @@ def main():
     for kappa in kappas:
         seed = 9200 if kappa < 1e4 else 9201
         t0 = time.time()
-        A, b, xt, recipe = randsvd(n, kappa, seed, nrhs=1)
-        log(f"[setup] randsvd n={n} kappa={kappa:g} seed={seed} in {time.time()-t0:.1f}s ({recipe['qr_backend']})")
-        P = ctx.problem(A, b, x_exact=xt, case=dict(recipe, case_id=f"randsvd_n{n}_k{kappa:g}"))
-        case = dict(kappa=kappa, seed=seed, a_sha256=sha256_colmajor(A))
+        P = Problem.generate(ctx, family="randsvd", n=n, nrhs=1, kappa=kappa, seed=seed, rhs="xtrue_uniform",
+                             case_id=f"randsvd_n{n}_k{kappa:g}")
+        crec = P.case_record()
+        log(f"[setup] device randsvd n={n} kappa={kappa:g} seed={seed} in {time.time()-t0:.1f}s "
+            f"(gen {crec['gen_seconds']:.1f}s, hash {crec['hash_seconds']:.1f}s)")
+        case = dict(kappa=kappa, seed=seed, a_sha256=crec["a_sha256"], generator=crec["generator"])
         log("[fp64 lu] rocSOLVER dgetrf + dgetrs")
         case["fp64_lu"] = time_method(P, "native_lu64", dict(nb=args.nb), args.steps, args.warmup)
         log(f"[low precision] {args.method16}")
@@ -107,6 +108,7 @@ def main():
             ctx.profile(False)
             if not args.no_cpu:
                 import threadpoolctl
+                A, b, _ = P.host_copy()
                 cores = len(os.sched_getaffinity(0))
                 with threadpoolctl.threadpool_limits(cores):
                     t = time.time(); xc = np.linalg.solve(A, b); tc = time.time() - t
@@ -117,7 +119,6 @@ def main():
                                   f"kappa={kappa:g} system, one run, eta={eta:.2e}")
                 log(f"[cpu] dgesv {tc*1e3:.0f} ms on {cores} threads")
         P.close()
-        del A, b, xt
     ctx.close()
 
     # roofline of the dominant kernel (largest total time among the profiled launches)

@@ -893,6 +893,16 @@ int mp_problem_reference(mp_ctx* C, mp_problem* P) {
 
 void mp_problem_free(mp_problem* p) { delete p; }
 
+int mp_problem_copy_out(mp_ctx* C, mp_problem* P, void* A, double* B, double* x_exact) {
+  return guard([&] {
+    MP_REQUIRE(C && P, "NULL argument");
+    const size_t nn = (size_t)P->n * P->n, nb = (size_t)P->n * P->nrhs;
+    if (A) HIP_CK(hipMemcpy(A, P->a_ptr(), nn * (P->fp32 ? 4 : 8), hipMemcpyDeviceToHost));
+    if (B) HIP_CK(hipMemcpy(B, P->B.p, nb * 8, hipMemcpyDeviceToHost));
+    if (x_exact) { MP_REQUIRE(P->has_exact, "problem has no exact solution"); std::memcpy(x_exact, P->hXexact.data(), nb * 8); }
+  });
+}
+
 int mp_problem_case_json(mp_problem* P, char* json, size_t cap) {
   return guard([&] {
     MP_REQUIRE(P, "problem is NULL");

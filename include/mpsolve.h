@@ -106,6 +106,8 @@ int mp_problem_generate(mp_ctx* ctx, const char* case_json, mp_problem** out);
  * metrics; replaces compute_reference (src/mpsolve.cu:99-125) without the kappa_2 estimate. */
 int mp_problem_reference(mp_ctx* ctx, mp_problem* p);
 void mp_problem_free(mp_problem* p);
+/* copy the stored system back to the host (column-major; A in the stored dtype); any pointer may be NULL */
+int mp_problem_copy_out(mp_ctx* ctx, mp_problem* p, void* A, double* B, double* x_exact);
 /* case record (JSON object: sizes, norms, hashes, case parameters passed at creation) */
 int mp_problem_case_json(mp_problem* p, char* json, size_t cap);
 

@@ -56,24 +56,53 @@ __device__ __forceinline__ void amax_wave(float& v, int& i) {
 
 struct PanelArgs {
   float* A; int64_t lda; int64_t k; int kb; int m; int G; int R;
-  uint64_t* xbuf; uint32_t tag0; int64_t* ipiv; int32_t* dinfo;
+  uint64_t* xbuf; int64_t* ipiv; int32_t* dinfo;
   int32_t* plan;   // [0] = count, then (dst, src) panel-local row pairs
 };
 
-// grid = G workgroups x 256 threads, dynamic LDS = R*kb floats (+ R ints of row identities).
+// Head word of a workgroup's pivot candidate: [63:40] 24-bit tag, [39:32] local row, [31:0] |value| bits
+// (negative = no candidate).  Row / identity words: [63:32] 32-bit tag, [31:0] payload.
+// Tags are the global column index + 1: unique within one factorization; the exchange area is cleared at
+// the start of every factorization, so a stale word can never carry the tag being waited for.
+__device__ __forceinline__ uint64_t get_head(const uint64_t* p, uint32_t tag24, volatile int* s_abort) {
+  for (long it = 0; it < SPIN_MAX; it++) {
+    const uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((uint32_t)(w >> 40) == tag24) return w;
+    if (*s_abort) return 0ull;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  *s_abort = 1;
+  return 0ull;
+}
+__device__ __forceinline__ uint32_t get_word_b(const uint64_t* p, uint32_t tag, volatile int* s_abort) {
+  for (long it = 0; it < SPIN_MAX; it++) {
+    const uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((uint32_t)(w >> 32) == tag) return (uint32_t)w;
+    if (*s_abort) return 0u;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  *s_abort = 1;
+  return 0u;
+}
+
+// grid = G workgroups x 256 threads (all co-resident: cooperative launch), dynamic LDS = R*kb floats + R ints.
+// Workgroup g owns panel rows [g*R, g*R + R).  Per column j: (1) local argmax; publish the head word, the
+// candidate row (kb words + identity) and, by the owner of row j, the old row j; (2) every workgroup reads
+// the G head words and picks the same winner (largest |a|, ties to the smallest row); (3) it reads the
+// winner's row, the owner of the pivot row also the old row j, and the rows are interchanged; (4) column j
+// is scaled and the rank-1 update applied to the rows below.  Two dependent round trips per column.
 __global__ __launch_bounds__(256) void k_panel(PanelArgs a) {
   extern __shared__ float lds[];
   __shared__ float prow[MAX_KB];
   __shared__ float wv[4];
   __shared__ int wi[4];
   __shared__ int s_abort;
-  __shared__ int s_lidx;
   __shared__ int s_prid, s_jrid;
   const int t = threadIdx.x, g = blockIdx.x;
   const int R = a.R, kb = a.kb, m = a.m, G = a.G;
   const int r0 = g * R;
   const int rows = max(0, min(R, m - r0));
-  int* rid = (int*)(lds + (size_t)R * kb);   // identity (panel-local source row) of the content of each row
+  int* rid = (int*)(lds + (size_t)R * kb);   // panel-local source row of each row's current content
   float* Ap = a.A + a.k + a.k * a.lda;
   if (t == 0) s_abort = 0;
   for (int idx = t; idx < R * kb; idx += 256) {
@@ -82,17 +111,22 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs a) {
   }
   for (int r = t; r < R; r += 256) rid[r] = r0 + r;
   __syncthreads();
-  const int64_t PSTRIDE = (int64_t)G * 2 + (int64_t)G * (kb + 1) + (kb + 1);
+  const int SW = kb + 1;
+  const int64_t PSTRIDE = (int64_t)G + (int64_t)G * SW + SW;
   const int lane = t & 63, wave = t >> 6;
   const int ur = t % R, cg = t / R, ncg = 256 / R;
+  float l_prev = 0.0f; int j_prev = -1; bool act_prev = false;   // deferred store of the scaled column
   for (int j = 0; j < kb; j++) {
-    const uint32_t tag = a.tag0 + (uint32_t)j;
-    uint64_t* base = a.xbuf + (int64_t)(j & 1) * PSTRIDE;
-    uint64_t* maxs = base;
-    uint64_t* rowsl = base + 2 * G;                      // G slots of kb data words + 1 identity word
-    uint64_t* diag = rowsl + (int64_t)G * (kb + 1);       // old row j: kb data words + 1 identity word
-    const int SW = kb + 1;
-    // (1) local argmax over rows with panel-local index >= j
+    const uint32_t tag = (uint32_t)(a.k + j + 1);
+    const uint32_t tag24 = tag & 0xFFFFFFu;
+    uint64_t* heads = a.xbuf + (int64_t)(j & 1) * PSTRIDE;
+    uint64_t* rowsl = heads + G;
+    uint64_t* diag = rowsl + (int64_t)G * SW;
+    __syncthreads();                       // step j-1 finished reading column j-1 and updating column j
+    // column j-1's multipliers are stored now (no one reads column j-1 again until the candidate rows are
+    // published, after the next barrier)
+    if (act_prev && cg == 0) lds[j_prev * R + ur] = l_prev;
+    // (1) local argmax of column j over rows with panel-local index >= j
     float v = -1.0f; int vi = -1;
     if (t < rows) {
       const int pr = r0 + t;
@@ -101,81 +135,76 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs a) {
     amax_wave(v, vi);
     if (lane == 0) { wv[wave] = v; wi[wave] = vi; }
     __syncthreads();
-    if (t == 0) {
-      float bv = wv[0]; int bi = wi[0];
-      for (int w = 1; w < 4; w++) amax_combine(bv, bi, wv[w], wi[w]);
-      s_lidx = bi;
-      put_word(maxs + 2 * g, tag, __float_as_uint(bv));
-      put_word(maxs + 2 * g + 1, tag, (uint32_t)bi);
-    }
-    __syncthreads();
-    const int lidx = s_lidx;
+    float bv = wv[0]; int bi = wi[0];
+    for (int w = 1; w < 4; w++) amax_combine(bv, bi, wv[w], wi[w]);
     const bool own_j = (j >= r0 && j < r0 + rows);
-    if (lidx >= 0) {
-      for (int c = t; c < kb; c += 256) put_word(rowsl + (int64_t)g * SW + c, tag, __float_as_uint(lds[c * R + (lidx - r0)]));
-      if (t == 0) put_word(rowsl + (int64_t)g * SW + kb, tag, (uint32_t)rid[lidx - r0]);
+    if (t == 0) {
+      const uint64_t hv = (bi >= 0) ? (uint64_t)__float_as_uint(bv) : (uint64_t)__float_as_uint(-1.0f);
+      const uint64_t lr = (bi >= 0) ? (uint64_t)(bi - r0) : 0ull;
+      __hip_atomic_store(heads + g, ((uint64_t)tag24 << 40) | (lr << 32) | hv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (bi >= 0) {
+      for (int c = t; c < kb; c += 256) put_word(rowsl + (int64_t)g * SW + c, tag, __float_as_uint(lds[c * R + (bi - r0)]));
+      if (t == kb % 256) put_word(rowsl + (int64_t)g * SW + kb, tag, (uint32_t)rid[bi - r0]);
     }
     if (own_j) {
       for (int c = t; c < kb; c += 256) put_word(diag + c, tag, __float_as_uint(lds[c * R + (j - r0)]));
-      if (t == 0) put_word(diag + kb, tag, (uint32_t)rid[j - r0]);
+      if (t == kb % 256) put_word(diag + kb, tag, (uint32_t)rid[j - r0]);
     }
-    // (2) agree on the pivot
+    // (2) agree on the pivot from the G head words
     float gv = -1.0f; int gi = -1;
     if (t < G) {
-      const uint32_t pv = get_word(maxs + 2 * t, tag, &s_abort);
-      const uint32_t pi = get_word(maxs + 2 * t + 1, tag, &s_abort);
-      gv = __uint_as_float(pv); gi = (int)pi;
+      const uint64_t w = get_head(heads + t, tag24, &s_abort);
+      const float hv = __uint_as_float((uint32_t)w);
+      if (hv >= 0.0f || hv != hv) { gv = hv; gi = t * R + (int)((w >> 32) & 0xFFull); }
       if (gi >= m) gi = -1;
     }
     amax_wave(gv, gi);
+    __syncthreads();                       // every thread has read wv/wi of step (1)
     if (lane == 0) { wv[wave] = gv; wi[wave] = gi; }
     __syncthreads();
     float pvv = wv[0]; int p = wi[0];
     for (int w = 1; w < 4; w++) amax_combine(pvv, p, wv[w], wi[w]);
     if (p < 0) p = j;
     const int gp = p / R;
-    // (3) pivot row, and the interchange of rows j and p (row identities travel with the rows)
-    for (int c = t; c < kb; c += 256) prow[c] = __uint_as_float(get_word(rowsl + (int64_t)gp * SW + c, tag, &s_abort));
-    if (p != j) {
-      if (gp == g) {
-        if (own_j) { for (int c = t; c < kb; c += 256) lds[c * R + (p - r0)] = lds[c * R + (j - r0)]; }
-        else { for (int c = t; c < kb; c += 256) lds[c * R + (p - r0)] = __uint_as_float(get_word(diag + c, tag, &s_abort)); }
-      }
-      if (t == 0) {
-        if (own_j) s_prid = (gp == g) ? rid[p - r0] : (int)get_word(rowsl + (int64_t)gp * SW + kb, tag, &s_abort);
-        if (gp == g) s_jrid = own_j ? rid[j - r0] : (int)get_word(diag + kb, tag, &s_abort);
-      }
+    // (3) pivot row and the interchange of rows j and p
+    for (int c = t; c < kb; c += 256) {
+      prow[c] = __uint_as_float(get_word_b(rowsl + (int64_t)gp * SW + c, tag, &s_abort));
+      if (p != j && gp == g)
+        lds[c * R + (p - r0)] = own_j ? lds[c * R + (j - r0)] : __uint_as_float(get_word_b(diag + c, tag, &s_abort));
+    }
+    if (t == kb % 256 && p != j) {
+      if (own_j) s_prid = (gp == g) ? rid[p - r0] : (int)get_word_b(rowsl + (int64_t)gp * SW + kb, tag, &s_abort);
+      if (gp == g) s_jrid = own_j ? rid[j - r0] : (int)get_word_b(diag + kb, tag, &s_abort);
     }
     __syncthreads();
     if (p != j) {
       if (own_j) { for (int c = t; c < kb; c += 256) lds[c * R + (j - r0)] = prow[c]; }
-      if (t == 0) { if (own_j) rid[j - r0] = s_prid; if (gp == g) rid[p - r0] = s_jrid; }
+      if (t == kb % 256) { if (own_j) rid[j - r0] = s_prid; if (gp == g) rid[p - r0] = s_jrid; }
     }
     if (g == 0 && t == 0) {
       a.ipiv[a.k + j] = a.k + p + 1;
       if (prow[j] == 0.0f && a.dinfo[0] == 0) a.dinfo[0] = (int)(a.k + j + 1);
     }
-    // (4) scale the column and apply the rank-1 update to the rows below
+    // (4) l = a_rj / pivot (stored at the start of the next step), rank-1 update of the rows below
     const float piv = prow[j];
-    float l = 0.0f;
     const int pr = r0 + ur;
     const bool active = (ur < rows) && (pr > j) && (piv != 0.0f);
+    float l = 0.0f;
     if (active) {
       const float x = lds[j * R + ur];
       l = (fabsf(piv) >= 1.17549435e-38f) ? x * (1.0f / piv) : x / piv;
-    }
-    __syncthreads();
-    if (active) {
-      if (cg == 0) lds[j * R + ur] = l;
       for (int c = j + 1 + cg; c < kb; c += ncg) lds[c * R + ur] = fmaf(-l, prow[c], lds[c * R + ur]);
     }
-    __syncthreads();
+    l_prev = l; j_prev = j; act_prev = active;
   }
+  if (act_prev && cg == 0) lds[j_prev * R + ur] = l_prev;
+  __syncthreads();
   for (int idx = t; idx < R * kb; idx += 256) {
     const int c = idx / R, r = idx - c * R;
     if (r < rows) Ap[(int64_t)(r0 + r) + (int64_t)c * a.lda] = lds[idx];
   }
-  // swap plan: every row whose content moved (rid != own index) is a (dst, src) pair
+  // swap plan: every row whose content moved is a (dst, src) pair
   for (int r = t; r < rows; r += 256) {
     const int src = rid[r];
     if (src != r0 + r) {
@@ -186,7 +215,6 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs a) {
   }
   if (t == 0 && s_abort) a.dinfo[1] = 1;
 }
-
 
 // ------------------------------------------------------------------ row interchanges outside the panel
 // plan[0] = number of moved rows, then (dst, src) pairs of panel-local rows: new[dst] = old[src].
@@ -231,17 +259,48 @@ __global__ void k_zero_u64(unsigned long long* p, int cnt) { const int i = block
 __global__ void k_iota(int32_t* p, int64_t n) { const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; if (i < n) p[i] = (int32_t)i; }
 
 // ------------------------------------------------------------------ U12 <- L11^{-1} U12 (unit lower, FP32)
-// grid = ceil(m2/64) workgroups, 256 threads; LDS: X[kb][65] (U12 block, padded) + Lb[kb][33] (L column block).
-__global__ __launch_bounds__(256) void k_trsm(float* __restrict__ A, int64_t lda, int64_t n, int64_t k, int kb) {
+// Two kernels.  k_trsm_inv: one wave per 32x32 diagonal block T of L11 computes T^{-1} column by column
+// (lane c: x_c = 1, x_r = sum_{q=c}^{r-1} fmaf(-T_rq, x_q) in increasing q, r = c+1..31), into Linv.
+// k_trsm: a workgroup owns 32 columns of U12 (LDS X[kb][33]); per 32-row block b:
+//   X_b <- T_b^{-1} X_b         (row r: acc = X_r; acc = fmaf(Tinv_rq, X_q, acc) for q = 0..r-1)
+//   X_r -= L_{r,b} X_b, r below  (acc = fmaf(-L_rq, X_q, acc) for q = 0..31)
+__global__ __launch_bounds__(64) void k_trsm_inv(const float* __restrict__ A, int64_t lda, int64_t k, int kb, float* __restrict__ Linv) {
+  __shared__ float T[32][33];
+  const int b = blockIdx.x, c = threadIdx.x;
+  const int rb = 32 * b, rib = min(32, kb - rb);
+  for (int idx = c; idx < 32 * 32; idx += 64) {
+    const int q = idx / 32, r = idx - q * 32;
+    T[r][q] = (r < rib && q < rib) ? A[(k + rb + r) + (k + rb + q) * lda] : 0.0f;
+  }
+  __syncthreads();
+  if (c < 32) {
+    float x[32];
+#pragma unroll
+    for (int r = 0; r < 32; r++) x[r] = (r == c) ? 1.0f : 0.0f;
+#pragma unroll
+    for (int r = 1; r < 32; r++) {
+      float acc = x[r];
+#pragma unroll
+      for (int q = 0; q < r; q++) if (q >= c) acc = fmaf(-T[r][q], x[q], acc);
+      if (r > c) x[r] = acc;
+    }
+#pragma unroll
+    for (int r = 0; r < 32; r++) Linv[(size_t)b * 1024 + c * 32 + r] = (r < rib && c < rib) ? x[r] : (r == c ? 1.0f : 0.0f);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_trsm(float* __restrict__ A, int64_t lda, int64_t n, int64_t k, int kb,
+                                              const float* __restrict__ Linv) {
   extern __shared__ float sm[];
-  float* X = sm;
-  float* Lb = sm + (size_t)kb * 65;
-  const int64_t c0 = k + kb + (int64_t)blockIdx.x * 64;
-  const int ncols = (int)min((int64_t)64, n - c0);
-  const int t = threadIdx.x, cc = t & 63, rg = t >> 6;
-  for (int idx = t; idx < kb * 64; idx += 256) {
+  float* X = sm;                           // [kb][33]
+  float* Lb = sm + (size_t)kb * 33;        // [kb][33]
+  __shared__ float Ti[32][33];
+  const int64_t c0 = k + kb + (int64_t)blockIdx.x * 32;
+  const int ncols = (int)min((int64_t)32, n - c0);
+  const int t = threadIdx.x, cc = t & 31, rg = t >> 5;   // 8 row groups
+  for (int idx = t; idx < kb * 32; idx += 256) {
     const int c = idx / kb, r = idx - c * kb;
-    X[r * 65 + c] = (c < ncols) ? A[(k + r) + (c0 + c) * lda] : 0.0f;
+    X[r * 33 + c] = (c < ncols) ? A[(k + r) + (c0 + c) * lda] : 0.0f;
   }
   const int nblk = (kb + 31) / 32;
   for (int b = 0; b < nblk; b++) {
@@ -253,31 +312,31 @@ __global__ __launch_bounds__(256) void k_trsm(float* __restrict__ A, int64_t lda
       const int c = idx / below, r = idx - c * below;
       Lb[r * 33 + c] = (c < rib) ? A[(k + rb + r) + (k + rb + c) * lda] : 0.0f;
     }
+    for (int idx = t; idx < 1024; idx += 256) { const int c = idx >> 5, r = idx & 31; Ti[r][c] = Linv[(size_t)b * 1024 + idx]; }
     __syncthreads();
-    if (t < 64) {
-      float x[32];
+    float nv[4];
 #pragma unroll
-      for (int r = 0; r < 32; r++) x[r] = (r < rib) ? X[(rb + r) * 65 + t] : 0.0f;
-#pragma unroll
-      for (int r = 1; r < 32; r++) {
-#pragma unroll
-        for (int c = 0; c < r; c++) x[r] = fmaf(-Lb[r * 33 + c], x[c], x[r]);
-      }
-#pragma unroll
-      for (int r = 0; r < 32; r++) if (r < rib) X[(rb + r) * 65 + t] = x[r];
+    for (int s2 = 0; s2 < 4; s2++) {
+      const int r = rg + 8 * s2;
+      float acc = X[(rb + r) * 33 + cc];
+      for (int q = 0; q < r; q++) acc = fmaf(Ti[r][q], X[(rb + q) * 33 + cc], acc);
+      nv[s2] = acc;
     }
     __syncthreads();
-    for (int r = rb + 32 + rg; r < kb; r += 4) {
-      float acc = X[r * 65 + cc];
+#pragma unroll
+    for (int s2 = 0; s2 < 4; s2++) { const int r = rg + 8 * s2; if (r < rib) X[(rb + r) * 33 + cc] = nv[s2]; }
+    __syncthreads();
+    for (int r = rb + 32 + rg; r < kb; r += 8) {
+      float acc = X[r * 33 + cc];
 #pragma unroll 8
-      for (int c = 0; c < 32; c++) acc = fmaf(-Lb[(r - rb) * 33 + c], X[(rb + c) * 65 + cc], acc);
-      X[r * 65 + cc] = acc;
+      for (int q = 0; q < 32; q++) acc = fmaf(-Lb[(r - rb) * 33 + q], X[(rb + q) * 33 + cc], acc);
+      X[r * 33 + cc] = acc;
     }
   }
   __syncthreads();
-  for (int idx = t; idx < kb * 64; idx += 256) {
+  for (int idx = t; idx < kb * 32; idx += 256) {
     const int c = idx / kb, r = idx - c * kb;
-    if (c < ncols) A[(k + r) + (c0 + c) * lda] = X[r * 65 + c];
+    if (c < ncols) A[(k + r) + (c0 + c) * lda] = X[r * 33 + c];
   }
 }
 
@@ -500,6 +559,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   W.dinfo.ensure(*W.tracker, 4, "getrf_info");
   W.cnt.ensure(*W.tracker, 2, "getrf_counts");
   W.affect.ensure(*W.tracker, 1 + 4 * 256 + 8, "swap_plan");
+  W.linv.ensure(*W.tracker, (size_t)cdiv(nb, 32) * 1024, "trsm_block_inverses");
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, W.stream, W.dinfo.p, 4);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, W.stream, W.cnt.p, 2);
   fill_iota(a.perm, n, W);
@@ -509,25 +569,34 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
     HIP_CK(hipFuncSetAttribute((const void*)k_trsm, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 4096));
     attr_set = true;
   }
+  // exchange area for the largest panel, cleared once per factorization (tags restart at 1)
+  {
+    const int64_t Gmax = std::min<int64_t>(W.num_cus, cdiv(n, 64));
+    const int64_t need = 2 * (Gmax + Gmax * (nb + 1) + (nb + 1));
+    W.xbuf.ensure(*W.tracker, (size_t)need, "panel_exchange");
+    hipLaunchKernelGGL(k_zero_u64, dim3(cdiv(need, 256)), dim3(256), 0, W.stream, (unsigned long long*)W.xbuf.p, (int)need);
+    LAUNCH_CK();
+  }
+  int rmin = 64;
+  if (const char* e = getenv("MPSOLVE_PANEL_RMIN")) rmin = std::max(64, std::min(256, atoi(e)));
   int panels = 0;
   for (int64_t k = 0; k < n; k += nb) {
     const int kb = (int)std::min<int64_t>(nb, n - k);
     const int m = (int)(n - k);
     int R = 0;
     for (int cand : {64, 128, 256}) {
+      if (cand < rmin) continue;
       const int64_t G = cdiv(m, cand);
       const size_t lds = (size_t)cand * kb * 4 + (size_t)cand * 4;
       if (G <= W.num_cus && lds + 4096 <= (size_t)lds_limit_bytes() - 4096) { R = cand; break; }
     }
     MP_REQUIRE(R > 0, "getrf: panel does not fit on the device (n too large for this nb)");
     const int G = (int)cdiv(m, R);
-    const int64_t pstride = (int64_t)G * 2 + (int64_t)G * (kb + 1) + (kb + 1);
-    if (W.xbuf.ensure(*W.tracker, (size_t)(2 * pstride), "panel_exchange"))
-      HIP_CK(hipMemsetAsync(W.xbuf.p, 0, W.xbuf.n * sizeof(uint64_t), W.stream));
+    (void)G;
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, W.stream, W.affect.p, 1);
     PanelArgs pa;
     pa.A = a.A; pa.lda = a.lda; pa.k = k; pa.kb = kb; pa.m = m; pa.G = G; pa.R = R;
-    pa.xbuf = W.xbuf.p; pa.tag0 = W.take_tags((uint32_t)kb); pa.ipiv = a.ipiv; pa.dinfo = W.dinfo.p; pa.plan = W.affect.p;
+    pa.xbuf = W.xbuf.p; pa.ipiv = a.ipiv; pa.dinfo = W.dinfo.p; pa.plan = W.affect.p;
     void* args[] = {&pa};
     const size_t lds = (size_t)R * kb * 4 + (size_t)R * 4;
     hipEvent_t e0 = W.prof_begin();
@@ -541,9 +610,11 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
     W.prof_end(e1, "laswp", 16.0 * (double)(n - kb) * kb, 0.0);
     const int64_t m2 = n - k - kb;
     if (m2 > 0) {
-      const size_t tl = (size_t)kb * 65 * 4 + (size_t)kb * 33 * 4;
+      const size_t tl = (size_t)kb * 33 * 4 * 2;
       hipEvent_t e2 = W.prof_begin();
-      hipLaunchKernelGGL(k_trsm, dim3(cdiv(m2, 64)), dim3(256), tl, W.stream, a.A, a.lda, n, k, kb);
+      hipLaunchKernelGGL(k_trsm_inv, dim3(cdiv(kb, 32)), dim3(64), 0, W.stream, a.A, a.lda, k, kb, W.linv.p);
+      LAUNCH_CK();
+      hipLaunchKernelGGL(k_trsm, dim3(cdiv(m2, 32)), dim3(256), tl, W.stream, a.A, a.lda, n, k, kb, W.linv.p);
       LAUNCH_CK();
       W.prof_end(e2, "trsm", 8.0 * (double)kb * m2, (double)kb * kb * m2);
       switch (a.fmt) {

@@ -20,11 +20,14 @@ constexpr long GS_SPIN_MAX = 1L << 22;
 __device__ __forceinline__ void gs_put(uint64_t* p, uint32_t tag, float v) {
   __hip_atomic_store(p, ((uint64_t)tag << 32) | (uint64_t)__float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ float gs_get(const uint64_t* p, uint32_t tag, volatile int* s_abort) {
+// urgent = the waiting block is the immediate successor (it is on the dependency chain); the others back off
+// so that the words being produced are not hammered by every waiting workgroup at once.
+__device__ __forceinline__ float gs_get(const uint64_t* p, uint32_t tag, volatile int* s_abort, bool urgent) {
   for (long it = 0; it < GS_SPIN_MAX; it++) {
     const uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if ((uint32_t)(w >> 32) == tag) return __uint_as_float((uint32_t)w);
     if (*s_abort) return 0.0f;
+    if (urgent) __builtin_amdgcn_s_sleep(1); else __builtin_amdgcn_s_sleep(16);
   }
   *s_abort = 1;
   return 0.0f;
@@ -53,6 +56,22 @@ __global__ __launch_bounds__(256) void k_trsv(const float* __restrict__ LU, int6
   for (int c = 0; c < NR; c++) acc[c] = 0.0f;
   __syncthreads();
   const int nsteps = LOWER ? bi : (nbk - 1 - bi);
+  // diagonal block (wave 0: lane rr holds row rr) and the right-hand side are independent of the other
+  // blocks: load them before waiting, so the dependency chain sees no HBM latency for them
+  float Dg[64];
+  float rhs0[NR];
+  if (t < 64) {
+#pragma unroll
+    for (int q = 0; q < 64; q++) {
+      const int64_t col = (int64_t)bi * 64 + q;
+      Dg[q] = (rvalid && col < n) ? LU[col * lda + row] : (q == rr ? 1.0f : 0.0f);
+    }
+#pragma unroll
+    for (int c = 0; c < NR; c++) {
+      if (LOWER) rhs0[c] = rvalid ? (float)Rin[(int64_t)c * ldr + perm[row]] : 0.0f;
+      else rhs0[c] = 0.0f;
+    }
+  }
   for (int s = 0; s < nsteps; s++) {
     const int bj = LOWER ? s : (nbk - 1 - s);
     // off-diagonal block values of my row, 16 columns of this wave's quarter (loaded before the wait)
@@ -66,7 +85,7 @@ __global__ __launch_bounds__(256) void k_trsv(const float* __restrict__ LU, int6
     for (int qq = t; qq < 64 * NR; qq += 256) {
       const int c = qq >> 6, col = qq & 63;
       const bool cv = (int64_t)bj * 64 + col < n;
-      yb[c][col] = cv ? gs_get(ybuf + ((int64_t)bj * NR + c) * 64 + col, tag, &s_abort) : 0.0f;
+      yb[c][col] = cv ? gs_get(ybuf + ((int64_t)bj * NR + c) * 64 + col, tag, &s_abort, s == nsteps - 1) : 0.0f;
     }
     __syncthreads();
 #pragma unroll
@@ -82,17 +101,10 @@ __global__ __launch_bounds__(256) void k_trsv(const float* __restrict__ LU, int6
 #pragma unroll
     for (int c = 0; c < NR; c++) {
       float rhs;
-      if (LOWER) rhs = rvalid ? (float)Rin[(int64_t)c * ldr + perm[row]] : 0.0f;
+      if (LOWER) rhs = rhs0[c];
       else rhs = rvalid ? Y[(int64_t)c * ldy + row] : 0.0f;
       const float sum = ((part[0][c][rr] + part[1][c][rr]) + part[2][c][rr]) + part[3][c][rr];
       b[c] = rhs - sum;
-    }
-    // diagonal block: lane rr holds row rr of the 64 x 64 block
-    float Dg[64];
-#pragma unroll
-    for (int q = 0; q < 64; q++) {
-      const int64_t col = (int64_t)bi * 64 + q;
-      Dg[q] = (rvalid && col < n) ? LU[col * lda + row] : (q == rr ? 1.0f : 0.0f);
     }
     if (LOWER) {
 #pragma unroll

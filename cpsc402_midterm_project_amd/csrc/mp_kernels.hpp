@@ -43,6 +43,7 @@ struct Workspace {
   DBuf<unsigned long long> cnt; // [0] overflow, [1] subnormal counts of the operand conversion
   DBuf<int32_t> dinfo;   // [0] info, [1] exchange timeout flag
   DBuf<int32_t> affect;  // per-panel swap plan (src,dst pairs)
+  DBuf<float> linv;      // inverses of the 32x32 diagonal blocks of L11
   // getrs
   DBuf<uint64_t> ybuf;   // tagged partial-solution words of the triangular solves
   DBuf<float> rhs32;

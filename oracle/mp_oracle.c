@@ -9,7 +9,8 @@
  *   or_resid_dd    k_resid_dd         src/mp_kernels.cuh:83-87   TwoProd via FMA + TwoSum accumulation, final renormalization
  *   or_colstats    k_colstats         src/mp_kernels.cuh:92-96   max|R|, max|X|, max R/T (0/0 -> 0, x/0 -> inf), sum R^2
  *   or_getrf       custom_blocked_lu  src/mp_core.cuh:129-150    blocked right-looking GEPP, FP32 storage, panel getf2,
- *                  laswp (src/mp_kernels.cuh:29-31), unit-lower TRSM, operands rounded to the trailing format
+ *                  laswp (src/mp_kernels.cuh:29-31), unit-lower TRSM (blocked by 32 with inverted diagonal
+ *                  blocks), operands rounded to the trailing format
  *                  (src/mp_kernels.cuh:14-18), FP32 accumulation of the trailing update
  *   or_getrs       apply_inverse      src/mp_core.cuh:182-190    FP64 -> FP32 cast, FP32 substitution, cast back, scale
  *   or_round_*     __float2half / the FP16 conversion of src/mp_kernels.cuh:16; BF16 and OCP FP8-e4m3 follow the
@@ -137,8 +138,9 @@ static int tile_exp(int fmt, float tmax, int tile_scale) {
  * A: n x n FP32 column-major (lda), factored in place.  ipiv: 1-based LAPACK pivots.  perm: composed permutation.
  * Panel step j (global column k+j): pivot p = first row of max |a| among rows >= k+j (NaN counts as +inf);
  * swap rows inside the panel; l = a * (1/piv) when |piv| >= FLT_MIN else a / piv; a_ic = fmaf(-l, u_jc, a_ic).
- * Then the kb interchanges are applied to all other columns and to perm, U12 = L11^{-1} U12 by forward
- * substitution with fmaf in increasing column order, and A22 -= L21 U12 with both operands rounded to the
+ * Then the kb interchanges are applied to all other columns and to perm, U12 = L11^{-1} U12 blocked by 32
+ * rows (each 32x32 diagonal block inverted by column-wise substitution and applied with fmaf in increasing
+ * column order, then the rank-32 update of the rows below), and A22 -= L21 U12 with both operands rounded to the
  * format after the per-32-row (L21) / per-32-column (U12) power-of-two scaling; the products are summed in
  * double, rounded once to FP32 (acc) and applied as fmaf(-acc, 2^-(sL+sU), a).
  * counts[0] / counts[1] = scaled operands above the format maximum / nonzero below its smallest normal. */
@@ -186,13 +188,42 @@ int or_getrf(int64_t n, int nb, float* A, int64_t lda, int64_t* ipiv, int32_t* p
     }
     const int64_t m2 = n - k - kb;
     if (m2 <= 0) continue;
-    /* U12 <- L11^{-1} U12 */
-    for (int64_t c = k + kb; c < n; c++)
-      for (int r = 1; r < kb; r++) {
-        float x = A[(k + r) + c * lda];
-        for (int q = 0; q < r; q++) x = fmaf(-A[(k + r) + (k + q) * lda], A[(k + q) + c * lda], x);
-        A[(k + r) + c * lda] = x;
+    /* U12 <- L11^{-1} U12, blocked by 32 rows: T_b^{-1} (column-wise substitution) applied to the block,
+       then the rank-32 update of the rows below (the HIP kernels' operation order) */
+    for (int rb = 0; rb < kb; rb += 32) {
+      const int rib = (kb - rb < 32) ? (kb - rb) : 32;
+      float Ti[32][32];
+      for (int c = 0; c < 32; c++) {
+        float x[32];
+        for (int r = 0; r < 32; r++) x[r] = (r == c) ? 1.0f : 0.0f;
+        for (int r = 1; r < 32; r++) {
+          float acc = x[r];
+          for (int q = c; q < r; q++) {
+            const float T = (r < rib && q < rib) ? A[(k + rb + r) + (k + rb + q) * lda] : 0.0f;
+            acc = fmaf(-T, x[q], acc);
+          }
+          if (r > c) x[r] = acc;
+        }
+        for (int r = 0; r < 32; r++) Ti[r][c] = (r < rib && c < rib) ? x[r] : (r == c ? 1.0f : 0.0f);
       }
+      for (int64_t c = k + kb; c < n; c++) {
+        float nv[32];
+        for (int r = 0; r < rib; r++) {
+          float acc = A[(k + rb + r) + c * lda];
+          for (int q = 0; q < r; q++) acc = fmaf(Ti[r][q], A[(k + rb + q) + c * lda], acc);
+          nv[r] = acc;
+        }
+        for (int r = 0; r < rib; r++) A[(k + rb + r) + c * lda] = nv[r];
+        for (int r = rb + 32; r < kb; r++) {
+          float acc = A[(k + r) + c * lda];
+          for (int q = 0; q < 32; q++) {
+            const float L = (q < rib) ? A[(k + r) + (k + rb + q) * lda] : 0.0f;
+            acc = fmaf(-L, A[(k + rb + q) + c * lda], acc);
+          }
+          A[(k + r) + c * lda] = acc;
+        }
+      }
+    }
     /* operand rounding with per-tile power-of-two scales */
     const int64_t nt = (m2 + 31) / 32;
     qL = (float*)realloc(qL, sizeof(float) * (size_t)m2 * kb);

@@ -17,7 +17,7 @@ LIB_PATH = PKG / "libmpsolve.so"
 MP_OK = 0
 STATUS = {-1: "MP_ERR_INVALID", -2: "MP_ERR_HIP", -3: "MP_ERR_ALLOC", -4: "MP_ERR_UNSUPPORTED",
           -5: "MP_ERR_INTERNAL", -6: "MP_ERR_NO_DEVICE"}
-FMT = {"fp16": 0, "bf16": 1, "fp8": 2, "fp8e4m3": 2, "fp32": 3}
+FMT = {"fp16": 0, "bf16": 1, "fp8": 2, "fp8e4m3": 2, "fp32": 3, "fp16x3": 4}
 
 # every function declared in include/mpsolve.h
 EXPORTS = [

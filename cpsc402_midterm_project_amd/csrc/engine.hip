@@ -159,7 +159,7 @@ static MethodSpec make_method(const std::string& id, const Params& a) {
   else if (starts("magma_")) { unsupported("MAGMA is not linked on this platform"); }
   // ---- MI355X method IDs: per-tile power-of-two scaled low-precision trailing updates
   //      mx{16,bf16,8}_{lu_direct | ir_w64_r64 | ir_w64_rdd | gmresir_w64_r64 | gmresir_w64_rdd}, policy_mx8_ladder
-  else if (starts("mx16_") || starts("mxbf16_") || starts("mx8_") || starts("mx32_")) {
+  else if (starts("mx16_") || starts("mxbf16_") || starts("mx8_") || starts("mx32_") || starts("mx16x3_")) {
     const size_t us = id.find('_');
     const std::string pre = id.substr(0, us), rest = id.substr(us + 1);
     M.factor = (pre == "mx32") ? "cus32" : pre; M.scale_mode = 1;
@@ -171,6 +171,7 @@ static MethodSpec make_method(const std::string& id, const Params& a) {
     else throw MPError("unknown method id " + id, -1);
   }
   else if (id == "policy_mx8_ladder") { M.factor = "mx8"; M.scale_mode = 1; M.gmres = true; M.resid = 128; M.policy = "ladder"; }
+  else if (id == "policy_mx16_ladder") { M.factor = "mx16"; M.scale_mode = 1; M.gmres = true; M.resid = 128; M.policy = "ladder16"; }
   else throw MPError("unknown method id " + id, -1);
   if (M.scale_mode == 2) unsupported("two-sided row/column scaling (scale_mode 2) is not implemented");
   if (M.apply64) unsupported("promoted FP64 factor application (apply64) is not implemented");
@@ -190,7 +191,8 @@ static const char* kMethodIds[] = {
   "mx16_lu_direct", "mx16_ir_w64_r64", "mx16_ir_w64_rdd", "mx16_gmresir_w64_r64", "mx16_gmresir_w64_rdd",
   "mxbf16_lu_direct", "mxbf16_ir_w64_r64", "mxbf16_ir_w64_rdd", "mxbf16_gmresir_w64_r64", "mxbf16_gmresir_w64_rdd",
   "mx8_lu_direct", "mx8_ir_w64_r64", "mx8_ir_w64_rdd", "mx8_gmresir_w64_r64", "mx8_gmresir_w64_rdd",
-  "mx32_ir_w64_rdd", "mx32_gmresir_w64_rdd", "policy_mx8_ladder"};
+  "mx32_ir_w64_rdd", "mx32_gmresir_w64_rdd", "mx16x3_lu_direct", "mx16x3_ir_w64_rdd", "mx16x3_gmresir_w64_rdd",
+  "policy_mx8_ladder", "policy_mx16_ladder"};
 
 static std::string factor_contract(const std::string& f) {
   if (f == "cs32") return "vendor FP32 LU with partial pivoting (rocSOLVER sgetrf) on the FP32 copy of A";
@@ -203,6 +205,8 @@ static std::string factor_contract(const std::string& f) {
   if (f == "mx16") return base + "FP16 with a power-of-two scale per 32-row tile of L21 and per 32-column tile of U12 (tile max in [2^14, 2^15))";
   if (f == "mxbf16") return base + "BF16 with a power-of-two scale per 32-row tile of L21 and per 32-column tile of U12";
   if (f == "mx8") return base + "OCP FP8 E4M3 with a power-of-two scale per 32-row tile of L21 and per 32-column tile of U12 (tile max in [2^7, 2^8))";
+  if (f == "mx16x3") return base + "two FP16 terms per operand (hi = fp16(s*x), lo = fp16(s*x - hi), one power-of-two scale s per 32-row / "
+                                   "32-column tile); the update sums hi*hi + hi*lo + lo*hi, three FP16 MFMA products with FP32 accumulation";
   return f;
 }
 
@@ -216,6 +220,7 @@ struct Factors {
 };
 
 static int fmt_of(const std::string& kind) {
+  if (kind == "mx16x3") return FMT_FP16X3;
   if (kind == "het16" || kind == "mx16") return FMT_FP16;
   if (kind == "mxbf16") return FMT_BF16;
   if (kind == "mx8") return FMT_FP8;
@@ -562,6 +567,7 @@ struct Engine {
     std::vector<std::string> ladder{M.factor};
     const bool target64 = (tau_eta < 1e-10);
     if (M.policy == "ladder") { ladder = {"mx8", "mx16", "cus32"}; }
+    else if (M.policy == "ladder16") { ladder = {"mx16", "mx16x3", "cus32"}; }
     else if (M.policy != "none") {
       if (M.factor == "het16" || M.factor == "cus32") { ladder.push_back("cs32"); if (target64) ladder.push_back("cs64"); }
       else if (M.factor == "cs32") { if (target64) ladder.push_back("cs64"); }
@@ -582,7 +588,7 @@ struct Engine {
       if (M.policy == "budget" && M.budget == 0) Ms.gmres = false;
       if (M.policy == "budget" && M.budget > 0) { if (stage == 0) { Ms.gmres = true; budget_left = M.budget; } else Ms.gmres = false; }
       if (M.policy == "cost_aware") Ms.gmres = false;
-      if (M.policy == "ladder") { Ms.gmres = true; budget_left = M.gmres_budget; }
+      if (M.policy == "ladder" || M.policy == "ladder16") { Ms.gmres = true; budget_left = M.gmres_budget; }
       const double ts = wall_now();
       reason = refine_stage(Ms, F, stage, T, t0, budget_left, !initialized, Mt);
       initialized = true;

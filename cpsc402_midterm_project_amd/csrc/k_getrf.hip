@@ -25,6 +25,12 @@
 namespace mpk {
 
 constexpr int MAX_KB = 512;
+
+// Phase markers of the panel kernel: empty in the library build; a probe build may define them to record
+// per-step timestamps.
+#ifndef PANEL_MARK
+#define PANEL_MARK(phase)
+#endif
 constexpr long SPIN_MAX = 1L << 22;
 
 // ------------------------------------------------------------------ tagged-word exchange
@@ -85,35 +91,82 @@ __device__ __forceinline__ uint32_t get_word_b(const uint64_t* p, uint32_t tag, 
   return 0u;
 }
 
+// Wave-wide maximum with DPP (row_shr 1/2/4/8 inside rows of 16 lanes, then row_bcast 15/31 across rows):
+// a handful of VALU ops instead of a chain of LDS-routed shuffles.  Inputs are never NaN here.
+__device__ __forceinline__ float wave_max_dpp(float v) {
+  int x = __float_as_int(v);
+  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x111, 0xf, 0xf, false))); x = __float_as_int(v);
+  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x112, 0xf, 0xf, false))); x = __float_as_int(v);
+  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x114, 0xf, 0xf, false))); x = __float_as_int(v);
+  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x118, 0xf, 0xf, false))); x = __float_as_int(v);
+  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x142, 0xa, 0xf, false))); x = __float_as_int(v);
+  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x143, 0xc, 0xf, false))); x = __float_as_int(v);
+  return __int_as_float(__builtin_amdgcn_readlane(x, 63));
+}
+// argmax over the wave: largest value, ties to the lowest lane (callers lay indices out increasing with the
+// lane, so that is the smallest index).  Invalid lanes carry v = -1.  Returns (value, index) in every lane.
+__device__ __forceinline__ void wave_argmax(float& v, int& i) {
+  const float m = wave_max_dpp(v);
+  const unsigned long long mask = __ballot(v == m && i >= 0);
+  if (mask) { const int lane = __ffsll((long long)mask) - 1; i = __builtin_amdgcn_readlane(i, lane); v = m; }
+  else { v = -1.0f; i = -1; }
+}
+
+// Poll up to four tagged words at once (p[i] == nullptr: not needed); each lane keeps all of its loads in
+// flight instead of waiting for them one after the other.
+__device__ __forceinline__ void get_words4(const uint64_t* const p[4], uint32_t tag, uint32_t v[4], volatile int* s_abort) {
+  bool done[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) done[i] = (p[i] == nullptr);
+  for (long it = 0; it < SPIN_MAX; it++) {
+    uint64_t w[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) w[i] = done[i] ? 0ull : __hip_atomic_load(p[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bool all = true;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      if (!done[i] && (uint32_t)(w[i] >> 32) == tag) { v[i] = (uint32_t)w[i]; done[i] = true; }
+      all = all && done[i];
+    }
+    if (all || *s_abort) return;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  *s_abort = 1;
+}
+
 // grid = G workgroups x 256 threads (all co-resident: cooperative launch), dynamic LDS = R*kb floats + R ints.
-// Workgroup g owns panel rows [g*R, g*R + R).  Per column j: (1) local argmax; publish the head word, the
-// candidate row (kb words + identity) and, by the owner of row j, the old row j; (2) every workgroup reads
-// the G head words and picks the same winner (largest |a|, ties to the smallest row); (3) it reads the
-// winner's row, the owner of the pivot row also the old row j, and the rows are interchanged; (4) column j
-// is scaled and the rank-1 update applied to the rows below.  Two dependent round trips per column.
+// G <= 128 and R <= 256.  Workgroup g owns panel rows [g*R, g*R + R).  Per column j: (1) local argmax; waves 2-3 publish the head
+// word, the candidate row (kb words + identity) and, in the owner of row j, the old row j; (2) waves 0-1
+// read the G head words and every workgroup picks the same winner (largest |a|, ties to the smallest row);
+// (3) waves 0-1 read the winner's row (the owner of the pivot row also the old row j) and the rows are
+// interchanged; (4) column j is scaled and the rank-1 update applied to the rows below.  Two dependent round
+// trips per column.  Stores and loads share one counter per wave on this architecture, so the publishing
+// waves never poll and the polling waves never store to global memory inside the loop.
 __global__ __launch_bounds__(256) void k_panel(PanelArgs a) {
   extern __shared__ float lds[];
   __shared__ float prow[MAX_KB];
-  __shared__ float wv[4];
-  __shared__ int wi[4];
+  __shared__ int s_piv[MAX_KB];
+  __shared__ float wv[2][4];
+  __shared__ int wi[2][4];
   __shared__ int s_abort;
-  __shared__ int s_prid, s_jrid;
+  __shared__ int s_prid, s_jrid, s_info;
   const int t = threadIdx.x, g = blockIdx.x;
   const int R = a.R, kb = a.kb, m = a.m, G = a.G;
   const int r0 = g * R;
   const int rows = max(0, min(R, m - r0));
   int* rid = (int*)(lds + (size_t)R * kb);   // panel-local source row of each row's current content
   float* Ap = a.A + a.k + a.k * a.lda;
-  if (t == 0) s_abort = 0;
+  if (t == 0) { s_abort = 0; s_info = 0; }
   for (int idx = t; idx < R * kb; idx += 256) {
     const int c = idx / R, r = idx - c * R;
     lds[idx] = (r < rows) ? Ap[(int64_t)(r0 + r) + (int64_t)c * a.lda] : 0.0f;
   }
   for (int r = t; r < R; r += 256) rid[r] = r0 + r;
-  __syncthreads();
   const int SW = kb + 1;
   const int64_t PSTRIDE = (int64_t)G + (int64_t)G * SW + SW;
   const int lane = t & 63, wave = t >> 6;
+  const bool poller = t < 128;              // waves 0-1
+  const int pt = t, st = t - 128;            // poller / publisher thread index
   const int ur = t % R, cg = t / R, ncg = 256 / R;
   float l_prev = 0.0f; int j_prev = -1; bool act_prev = false;   // deferred store of the scaled column
   for (int j = 0; j < kb; j++) {
@@ -122,69 +175,96 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs a) {
     uint64_t* heads = a.xbuf + (int64_t)(j & 1) * PSTRIDE;
     uint64_t* rowsl = heads + G;
     uint64_t* diag = rowsl + (int64_t)G * SW;
+    PANEL_MARK(0);
     __syncthreads();                       // step j-1 finished reading column j-1 and updating column j
-    // column j-1's multipliers are stored now (no one reads column j-1 again until the candidate rows are
-    // published, after the next barrier)
-    if (act_prev && cg == 0) lds[j_prev * R + ur] = l_prev;
+    if (act_prev && cg == 0) lds[j_prev * R + ur] = l_prev;   // multipliers of column j-1
     // (1) local argmax of column j over rows with panel-local index >= j
     float v = -1.0f; int vi = -1;
     if (t < rows) {
       const int pr = r0 + t;
       if (pr >= j) { float x = fabsf(lds[j * R + t]); if (x != x) x = INFINITY; v = x; vi = pr; }
     }
-    amax_wave(v, vi);
-    if (lane == 0) { wv[wave] = v; wi[wave] = vi; }
+    wave_argmax(v, vi);
+    if (lane == 0) { wv[0][wave] = v; wi[0][wave] = vi; }
     __syncthreads();
-    float bv = wv[0]; int bi = wi[0];
-    for (int w = 1; w < 4; w++) amax_combine(bv, bi, wv[w], wi[w]);
+    PANEL_MARK(1);
+    float bv = wv[0][0]; int bi = wi[0][0];
+    for (int w = 1; w < 4; w++) amax_combine(bv, bi, wv[0][w], wi[0][w]);
     const bool own_j = (j >= r0 && j < r0 + rows);
-    if (t == 0) {
-      const uint64_t hv = (bi >= 0) ? (uint64_t)__float_as_uint(bv) : (uint64_t)__float_as_uint(-1.0f);
-      const uint64_t lr = (bi >= 0) ? (uint64_t)(bi - r0) : 0ull;
-      __hip_atomic_store(heads + g, ((uint64_t)tag24 << 40) | (lr << 32) | hv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!poller) {
+      if (st == 0) {
+        const uint64_t hv = (bi >= 0) ? (uint64_t)__float_as_uint(bv) : (uint64_t)__float_as_uint(-1.0f);
+        const uint64_t lr = (bi >= 0) ? (uint64_t)(bi - r0) : 0ull;
+        __hip_atomic_store(heads + g, ((uint64_t)tag24 << 40) | (lr << 32) | hv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (bi >= 0) {
+        for (int c = st; c < kb; c += 128) put_word(rowsl + (int64_t)g * SW + c, tag, __float_as_uint(lds[c * R + (bi - r0)]));
+        if (st == 127) put_word(rowsl + (int64_t)g * SW + kb, tag, (uint32_t)rid[bi - r0]);
+      }
+      if (own_j) {
+        for (int c = st; c < kb; c += 128) put_word(diag + c, tag, __float_as_uint(lds[c * R + (j - r0)]));
+        if (st == 126) put_word(diag + kb, tag, (uint32_t)rid[j - r0]);
+      }
     }
-    if (bi >= 0) {
-      for (int c = t; c < kb; c += 256) put_word(rowsl + (int64_t)g * SW + c, tag, __float_as_uint(lds[c * R + (bi - r0)]));
-      if (t == kb % 256) put_word(rowsl + (int64_t)g * SW + kb, tag, (uint32_t)rid[bi - r0]);
+    PANEL_MARK(2);
+    // (2) agree on the pivot from the G head words (waves 0-1, one head per lane; G <= 128)
+    if (poller) {
+      float gv = -1.0f; int gi = -1;
+      if (pt < G) {
+        const uint64_t w = get_head(heads + pt, tag24, &s_abort);
+        const float hv = __uint_as_float((uint32_t)w);
+        if (hv >= 0.0f) {
+          gi = pt * R + (int)((w >> 32) & 0xFFull);
+          gv = hv;
+          if (gi >= m) { gi = -1; gv = -1.0f; }
+        }
+      }
+      wave_argmax(gv, gi);
+      if (lane == 0) { wv[1][wave] = gv; wi[1][wave] = gi; }
     }
-    if (own_j) {
-      for (int c = t; c < kb; c += 256) put_word(diag + c, tag, __float_as_uint(lds[c * R + (j - r0)]));
-      if (t == kb % 256) put_word(diag + kb, tag, (uint32_t)rid[j - r0]);
-    }
-    // (2) agree on the pivot from the G head words
-    float gv = -1.0f; int gi = -1;
-    if (t < G) {
-      const uint64_t w = get_head(heads + t, tag24, &s_abort);
-      const float hv = __uint_as_float((uint32_t)w);
-      if (hv >= 0.0f || hv != hv) { gv = hv; gi = t * R + (int)((w >> 32) & 0xFFull); }
-      if (gi >= m) gi = -1;
-    }
-    amax_wave(gv, gi);
-    __syncthreads();                       // every thread has read wv/wi of step (1)
-    if (lane == 0) { wv[wave] = gv; wi[wave] = gi; }
+    PANEL_MARK(3);
     __syncthreads();
-    float pvv = wv[0]; int p = wi[0];
-    for (int w = 1; w < 4; w++) amax_combine(pvv, p, wv[w], wi[w]);
+    float pvv = wv[1][0]; int p = wi[1][0];
+    amax_combine(pvv, p, wv[1][1], wi[1][1]);
     if (p < 0) p = j;
     const int gp = p / R;
-    // (3) pivot row and the interchange of rows j and p
-    for (int c = t; c < kb; c += 256) {
-      prow[c] = __uint_as_float(get_word_b(rowsl + (int64_t)gp * SW + c, tag, &s_abort));
-      if (p != j && gp == g)
-        lds[c * R + (p - r0)] = own_j ? lds[c * R + (j - r0)] : __uint_as_float(get_word_b(diag + c, tag, &s_abort));
-    }
-    if (t == kb % 256 && p != j) {
-      if (own_j) s_prid = (gp == g) ? rid[p - r0] : (int)get_word_b(rowsl + (int64_t)gp * SW + kb, tag, &s_abort);
-      if (gp == g) s_jrid = own_j ? rid[j - r0] : (int)get_word_b(diag + kb, tag, &s_abort);
+    PANEL_MARK(4);
+    // (3) pivot row and the interchange of rows j and p (waves 0-1 read; the owner of row p copies)
+    if (poller) {
+      const bool need_diag = (p != j && gp == g && !own_j);
+      const uint64_t* pw[4] = {nullptr, nullptr, nullptr, nullptr};
+      uint32_t vw[4] = {0u, 0u, 0u, 0u};
+      const int c0 = pt, c1 = pt + 128;    // kb <= 256
+      if (c0 < kb) { pw[0] = rowsl + (int64_t)gp * SW + c0; if (need_diag) pw[2] = diag + c0; }
+      if (c1 < kb) { pw[1] = rowsl + (int64_t)gp * SW + c1; if (need_diag) pw[3] = diag + c1; }
+      get_words4(pw, tag, vw, &s_abort);
+      if (c0 < kb) {
+        prow[c0] = __uint_as_float(vw[0]);
+        if (p != j && gp == g) lds[c0 * R + (p - r0)] = own_j ? lds[c0 * R + (j - r0)] : __uint_as_float(vw[2]);
+      }
+      if (c1 < kb) {
+        prow[c1] = __uint_as_float(vw[1]);
+        if (p != j && gp == g) lds[c1 * R + (p - r0)] = own_j ? lds[c1 * R + (j - r0)] : __uint_as_float(vw[3]);
+      }
+      if (pt == 127 && p != j) {
+        const uint64_t* pi[4] = {nullptr, nullptr, nullptr, nullptr};
+        uint32_t vi2[4] = {0u, 0u, 0u, 0u};
+        if (own_j && gp != g) pi[0] = rowsl + (int64_t)gp * SW + kb;
+        if (gp == g && !own_j) pi[1] = diag + kb;
+        get_words4(pi, tag, vi2, &s_abort);
+        if (own_j) s_prid = (gp == g) ? rid[p - r0] : (int)vi2[0];
+        if (gp == g) s_jrid = own_j ? rid[j - r0] : (int)vi2[1];
+      }
     }
     __syncthreads();
+    PANEL_MARK(5);
     if (p != j) {
       if (own_j) { for (int c = t; c < kb; c += 256) lds[c * R + (j - r0)] = prow[c]; }
-      if (t == kb % 256) { if (own_j) rid[j - r0] = s_prid; if (gp == g) rid[p - r0] = s_jrid; }
+      if (t == 127) { if (own_j) rid[j - r0] = s_prid; if (gp == g) rid[p - r0] = s_jrid; }
     }
-    if (g == 0 && t == 0) {
-      a.ipiv[a.k + j] = a.k + p + 1;
-      if (prow[j] == 0.0f && a.dinfo[0] == 0) a.dinfo[0] = (int)(a.k + j + 1);
+    if (t == 0) {
+      s_piv[j] = p;
+      if (prow[j] == 0.0f && s_info == 0) s_info = (int)(a.k + j + 1);
     }
     // (4) l = a_rj / pivot (stored at the start of the next step), rank-1 update of the rows below
     const float piv = prow[j];
@@ -194,15 +274,31 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs a) {
     if (active) {
       const float x = lds[j * R + ur];
       l = (fabsf(piv) >= 1.17549435e-38f) ? x * (1.0f / piv) : x / piv;
-      for (int c = j + 1 + cg; c < kb; c += ncg) lds[c * R + ur] = fmaf(-l, prow[c], lds[c * R + ur]);
+      // 8 columns per iteration: all LDS loads are issued before the stores (the stores may alias prow as
+      // far as the compiler knows, which would otherwise serialise every iteration on LDS latency)
+      int c = j + 1 + cg;
+      for (; c + 7 * ncg < kb; c += 8 * ncg) {
+        float av[8], uv[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) { av[q] = lds[(c + q * ncg) * R + ur]; uv[q] = prow[c + q * ncg]; }
+#pragma unroll
+        for (int q = 0; q < 8; q++) lds[(c + q * ncg) * R + ur] = fmaf(-l, uv[q], av[q]);
+      }
+      for (; c < kb; c += ncg) lds[c * R + ur] = fmaf(-l, prow[c], lds[c * R + ur]);
     }
     l_prev = l; j_prev = j; act_prev = active;
+    PANEL_MARK(6);
   }
+  __syncthreads();
   if (act_prev && cg == 0) lds[j_prev * R + ur] = l_prev;
   __syncthreads();
   for (int idx = t; idx < R * kb; idx += 256) {
     const int c = idx / R, r = idx - c * R;
     if (r < rows) Ap[(int64_t)(r0 + r) + (int64_t)c * a.lda] = lds[idx];
+  }
+  if (g == 0) {
+    for (int jj = t; jj < kb; jj += 256) a.ipiv[a.k + jj] = a.k + s_piv[jj] + 1;
+    if (t == 0 && s_info != 0 && a.dinfo[0] == 0) a.dinfo[0] = s_info;
   }
   // swap plan: every row whose content moved is a (dst, src) pair
   for (int r = t; r < rows; r += 256) {
@@ -369,6 +465,7 @@ template <> struct FmtInfo<FMT_FP16> { static constexpr int FB = 16, ES = 2, TOP
 template <> struct FmtInfo<FMT_BF16> { static constexpr int FB = 16, ES = 2, TOPEXP = 14; static constexpr float MAXV = 3.3895313892515355e+38f, MINN = 1.17549435e-38f; };
 template <> struct FmtInfo<FMT_FP8>  { static constexpr int FB = 8,  ES = 1, TOPEXP = 7;  static constexpr float MAXV = 448.0f, MINN = 0.015625f; };
 template <> struct FmtInfo<FMT_FP32> { static constexpr int FB = 32, ES = 4, TOPEXP = 0;  static constexpr float MAXV = 3.40282347e+38f, MINN = 1.17549435e-38f; };
+template <> struct FmtInfo<FMT_FP16X3> { static constexpr int FB = 16, ES = 2, TOPEXP = 14; static constexpr float MAXV = 65504.0f, MINN = 6.103515625e-05f; };
 
 // Pack one 32-index tile.  Source element (idx, kk) = S[idx*sI + kk*sK], idx < nidx, kk < K.
 // Output: for k-block kb16 and lane l, element e lives at ((tile*nkb + kb16)*64 + l)*8 + e and holds
@@ -377,8 +474,8 @@ template <> struct FmtInfo<FMT_FP32> { static constexpr int FB = 32, ES = 4, TOP
 // exps[tile] = power-of-two exponent s applied before rounding (value = packed * 2^-s).
 template <int FMT>
 __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ S, int64_t sI, int64_t sK, int nidx, int K,
-                                              int nkb, uint8_t* __restrict__ out, int32_t* __restrict__ exps,
-                                              unsigned long long* __restrict__ cnt, int tile_scale) {
+                                              int nkb, uint8_t* __restrict__ out, uint8_t* __restrict__ out_lo,
+                                              int32_t* __restrict__ exps, unsigned long long* __restrict__ cnt, int tile_scale) {
   using FI = FmtInfo<FMT>;
   extern __shared__ float tile[];   // [32][K+1]
   __shared__ float red[256];
@@ -415,12 +512,18 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ S, int64
   for (int q = t; q < nkb * 64 * 8; q += 256) {
     const int kblk = q >> 9, l = (q >> 3) & 63, e = q & 7;
     const int ii = l & 31;
-    const int kk = (FMT == FMT_FP32) ? (kblk * 16 + 2 * e + (l >> 5)) : (kblk * 16 + (l >> 5) * 8 + e);
+    const int kk = (FMT == FMT_FP32) ? (kblk * 16 + 2 * e + (l >> 5)) : (kblk * 16 + (l >> 5) * 8 + e);   // FP16X3: as FP16
     const float v = tile[ii * (KP + 1) + kk] * sc;
     const float av = fabsf(v);
     if (av > FI::MAXV) n_over++; else if (av > 0.0f && av < FI::MINN) n_sub++;
     const size_t o = ((size_t)(tl * nkb + kblk) * 64 + l) * 8 + e;
     if (FMT == FMT_FP16) { const _Float16 h = (_Float16)v; ((uint16_t*)out)[o] = __builtin_bit_cast(uint16_t, h); }
+    else if (FMT == FMT_FP16X3) {
+      const _Float16 h = (_Float16)v;
+      const _Float16 lo = (_Float16)(v - (float)h);   // v - hi is exact in FP32
+      ((uint16_t*)out)[o] = __builtin_bit_cast(uint16_t, h);
+      ((uint16_t*)out_lo)[o] = __builtin_bit_cast(uint16_t, lo);
+    }
     else if (FMT == FMT_BF16) ((uint16_t*)out)[o] = to_bf16_rne(v);
     else if (FMT == FMT_FP8) out[o] = to_e4m3_rne(v);
     else ((float*)out)[o] = v;
@@ -440,12 +543,16 @@ template <> struct Frag<FMT_FP16> { half8_t v;
   __device__ __forceinline__ void load(const uint8_t* p) { v = *(const half8_t*)p; } };
 template <> struct Frag<FMT_BF16> { bf16x8_t v;
   __device__ __forceinline__ void load(const uint8_t* p) { v = *(const bf16x8_t*)p; } };
+template <> struct Frag<FMT_FP16X3> { half8_t v;
+  __device__ __forceinline__ void load(const uint8_t* p) { v = *(const half8_t*)p; } };
 template <> struct Frag<FMT_FP8> { long v;
   __device__ __forceinline__ void load(const uint8_t* p) { v = *(const long*)p; } };
 template <> struct Frag<FMT_FP32> { f32x8_t v;
   __device__ __forceinline__ void load(const uint8_t* p) { v = *(const f32x8_t*)p; } };
 
 __device__ __forceinline__ void mma(f32x16_t& acc, const Frag<FMT_FP16>& a, const Frag<FMT_FP16>& b) {
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.v, b.v, acc, 0, 0, 0); }
+__device__ __forceinline__ void mma(f32x16_t& acc, const Frag<FMT_FP16X3>& a, const Frag<FMT_FP16X3>& b) {
   acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.v, b.v, acc, 0, 0, 0); }
 __device__ __forceinline__ void mma(f32x16_t& acc, const Frag<FMT_BF16>& a, const Frag<FMT_BF16>& b) {
   acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.v, b.v, acc, 0, 0, 0); }
@@ -463,6 +570,7 @@ __device__ __forceinline__ void mma(f32x16_t& acc, const Frag<FMT_FP32>& a, cons
 template <int FMT>
 __global__ __launch_bounds__(256) void k_gemm(float* __restrict__ C, int64_t ldc, int m2, int n2, int nkb,
                                               const uint8_t* __restrict__ PL, const uint8_t* __restrict__ PU,
+                                              const uint8_t* __restrict__ PLlo, const uint8_t* __restrict__ PUlo,
                                               const int32_t* __restrict__ eL, const int32_t* __restrict__ eU) {
   constexpr int FB = FmtInfo<FMT>::FB;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -480,6 +588,19 @@ __global__ __launch_bounds__(256) void k_gemm(float* __restrict__ C, int64_t ldc
   const uint8_t* pu1 = PU + (size_t)(ct0 + 1) * nkb * kstride + (size_t)lane * FB;
   const uint8_t* pl0 = PL + (size_t)(rt0 + 0) * nkb * kstride + (size_t)lane * FB;
   const uint8_t* pl1 = PL + (size_t)(rt0 + 1) * nkb * kstride + (size_t)lane * FB;
+  // C tile values for the epilogue are loaded before the K loop so their HBM latency overlaps the MFMAs
+  float cv[2][2][16];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int row = (rt0 + j) * 32 + (lane & 31);
+#pragma unroll
+      for (int a = 0; a < 16; a++) {
+        const int col = (ct0 + i) * 32 + 8 * (a >> 2) + 4 * (lane >> 5) + (a & 3);
+        cv[i][j][a] = (row < m2 && col < n2) ? __builtin_nontemporal_load(C + (int64_t)col * ldc + row) : 0.0f;
+      }
+    }
   Frag<FMT> u0, u1, l0, l1;
   u0.load(pu0); u1.load(pu1); l0.load(pl0); l1.load(pl1);
   for (int kb16 = 0; kb16 < nkb; kb16++) {
@@ -490,6 +611,17 @@ __global__ __launch_bounds__(256) void k_gemm(float* __restrict__ C, int64_t ldc
     }
     mma(acc[0][0], u0, l0); mma(acc[0][1], u0, l1);
     mma(acc[1][0], u1, l0); mma(acc[1][1], u1, l1);
+    if (FMT == FMT_FP16X3) {
+      // cross terms hi*lo + lo*hi of the two-term operands (lo*lo ~ u16^2 is dropped)
+      const size_t o = (size_t)kb16 * kstride;
+      Frag<FMT> ul0, ul1, ll0, ll1;
+      ul0.load(PUlo + (pu0 - PU) + o); ul1.load(PUlo + (pu1 - PU) + o);
+      ll0.load(PLlo + (pl0 - PL) + o); ll1.load(PLlo + (pl1 - PL) + o);
+      mma(acc[0][0], u0, ll0); mma(acc[0][1], u0, ll1);
+      mma(acc[1][0], u1, ll0); mma(acc[1][1], u1, ll1);
+      mma(acc[0][0], ul0, l0); mma(acc[0][1], ul0, l1);
+      mma(acc[1][0], ul1, l0); mma(acc[1][1], ul1, l1);
+    }
     u0 = nu0; u1 = nu1; l0 = nl0; l1 = nl1;
   }
 #pragma unroll
@@ -503,10 +635,7 @@ __global__ __launch_bounds__(256) void k_gemm(float* __restrict__ C, int64_t ldc
 #pragma unroll
       for (int a = 0; a < 16; a++) {
         const int col = ct * 32 + 8 * (a >> 2) + 4 * (lane >> 5) + (a & 3);
-        if (col < n2) {
-          float* cp = C + (int64_t)col * ldc + row;
-          *cp = fmaf(-acc[i][j][a], sc, *cp);
-        }
+        if (col < n2) __builtin_nontemporal_store(fmaf(-acc[i][j][a], sc, cv[i][j][a]), C + (int64_t)col * ldc + row);
       }
     }
   }
@@ -523,23 +652,29 @@ static void pack_and_update(float* A, int64_t lda, int64_t k, int kb, int64_t m2
   const size_t bytes = (size_t)nT * nkb * 64 * 8 * ES;
   W.packL.ensure(*W.tracker, bytes, "packL");
   W.packU.ensure(*W.tracker, bytes, "packU");
+  uint8_t* lo_l = nullptr; uint8_t* lo_u = nullptr;
+  if (FMT == FMT_FP16X3) {
+    W.packLlo.ensure(*W.tracker, bytes, "packL_lo"); W.packUlo.ensure(*W.tracker, bytes, "packU_lo");
+    lo_l = W.packLlo.p; lo_u = W.packUlo.p;
+  }
   W.scaleL.ensure(*W.tracker, nT, "scaleL");
   W.scaleU.ensure(*W.tracker, nT, "scaleU");
   const size_t lds = (size_t)32 * (nkb * 16 + 1) * sizeof(float);
   hipEvent_t evp = W.prof_begin();
   hipLaunchKernelGGL((k_pack<FMT>), dim3(nT), dim3(256), lds, W.stream,
-                     A + (k + kb) + k * lda, (int64_t)1, lda, (int)m2, kb, nkb, W.packL.p, W.scaleL.p, W.cnt.p, tile_scale ? 1 : 0);
+                     A + (k + kb) + k * lda, (int64_t)1, lda, (int)m2, kb, nkb, W.packL.p, lo_l, W.scaleL.p, W.cnt.p, tile_scale ? 1 : 0);
   LAUNCH_CK();
   hipLaunchKernelGGL((k_pack<FMT>), dim3(nT), dim3(256), lds, W.stream,
-                     A + k + (k + kb) * lda, lda, (int64_t)1, (int)m2, kb, nkb, W.packU.p, W.scaleU.p, W.cnt.p, tile_scale ? 1 : 0);
+                     A + k + (k + kb) * lda, lda, (int64_t)1, (int)m2, kb, nkb, W.packU.p, lo_u, W.scaleU.p, W.cnt.p, tile_scale ? 1 : 0);
   LAUNCH_CK();
   W.prof_end(evp, "pack", 2.0 * (4.0 + ES) * (double)m2 * kb, 0.0);
   const unsigned g = cdiv(m2, 128);
   // algorithmic traffic of the update: C read + written (8 B/element) plus both packed operands once
-  const double gb = 8.0 * (double)m2 * m2 + 2.0 * ES * (double)m2 * kb, gf = 2.0 * (double)m2 * m2 * kb;
+  const double gb = 8.0 * (double)m2 * m2 + (FMT == FMT_FP16X3 ? 4.0 : 2.0) * ES * (double)m2 * kb;
+  const double gf = (FMT == FMT_FP16X3 ? 6.0 : 2.0) * (double)m2 * m2 * kb;
   hipEvent_t ev = W.prof_begin();
   hipLaunchKernelGGL((k_gemm<FMT>), dim3(g, g), dim3(256), 0, W.stream,
-                     A + (k + kb) + (k + kb) * lda, lda, (int)m2, (int)m2, nkb, W.packL.p, W.packU.p, W.scaleL.p, W.scaleU.p);
+                     A + (k + kb) + (k + kb) * lda, lda, (int)m2, (int)m2, nkb, W.packL.p, W.packU.p, lo_l, lo_u, W.scaleL.p, W.scaleU.p);
   LAUNCH_CK();
   W.prof_end(ev, "trailing_gemm", gb, gf);
 }
@@ -554,7 +689,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   MP_REQUIRE(n > 0 && a.A && a.ipiv && a.perm, "getrf: n > 0 and A, ipiv, perm required");
   MP_REQUIRE(a.lda >= n, "getrf: lda >= n");
   MP_REQUIRE(a.nb >= 1 && a.nb <= 256, "getrf: nb must be in [1, 256]");
-  MP_REQUIRE(a.fmt >= FMT_FP16 && a.fmt <= FMT_FP32, "getrf: unknown trailing format");
+  MP_REQUIRE(a.fmt >= FMT_FP16 && a.fmt <= FMT_FP16X3, "getrf: unknown trailing format");
   const int nb = a.nb;
   W.dinfo.ensure(*W.tracker, 4, "getrf_info");
   W.cnt.ensure(*W.tracker, 2, "getrf_counts");
@@ -565,13 +700,14 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   fill_iota(a.perm, n, W);
   static bool attr_set = false;
   if (!attr_set) {
-    HIP_CK(hipFuncSetAttribute((const void*)k_panel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 4096));
-    HIP_CK(hipFuncSetAttribute((const void*)k_trsm, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 4096));
+    // dynamic limit leaves room for each kernel's static shared memory (at most 8 KB here)
+    HIP_CK(hipFuncSetAttribute((const void*)k_panel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 8192));
+    HIP_CK(hipFuncSetAttribute((const void*)k_trsm, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 8192));
     attr_set = true;
   }
   // exchange area for the largest panel, cleared once per factorization (tags restart at 1)
   {
-    const int64_t Gmax = std::min<int64_t>(W.num_cus, cdiv(n, 64));
+    const int64_t Gmax = std::min<int64_t>(128, cdiv(n, 64));
     const int64_t need = 2 * (Gmax + Gmax * (nb + 1) + (nb + 1));
     W.xbuf.ensure(*W.tracker, (size_t)need, "panel_exchange");
     hipLaunchKernelGGL(k_zero_u64, dim3(cdiv(need, 256)), dim3(256), 0, W.stream, (unsigned long long*)W.xbuf.p, (int)need);
@@ -588,7 +724,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
       if (cand < rmin) continue;
       const int64_t G = cdiv(m, cand);
       const size_t lds = (size_t)cand * kb * 4 + (size_t)cand * 4;
-      if (G <= W.num_cus && lds + 4096 <= (size_t)lds_limit_bytes() - 4096) { R = cand; break; }
+      if (G <= std::min(128, W.num_cus) && lds <= (size_t)lds_limit_bytes() - 8192) { R = cand; break; }
     }
     MP_REQUIRE(R > 0, "getrf: panel does not fit on the device (n too large for this nb)");
     const int G = (int)cdiv(m, R);
@@ -621,6 +757,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
         case FMT_FP16: pack_and_update<FMT_FP16>(a.A, a.lda, k, kb, m2, a.tile_scale, W); break;
         case FMT_BF16: pack_and_update<FMT_BF16>(a.A, a.lda, k, kb, m2, a.tile_scale, W); break;
         case FMT_FP8:  pack_and_update<FMT_FP8>(a.A, a.lda, k, kb, m2, a.tile_scale, W); break;
+        case FMT_FP16X3: pack_and_update<FMT_FP16X3>(a.A, a.lda, k, kb, m2, a.tile_scale, W); break;
         default:       pack_and_update<FMT_FP32>(a.A, a.lda, k, kb, m2, a.tile_scale, W); break;
       }
     }

@@ -39,6 +39,7 @@ struct Workspace {
   // getrf
   DBuf<uint64_t> xbuf;   // tagged-word exchange area of the panel kernel (zeroed at allocation)
   DBuf<uint8_t> packL, packU;   // trailing-update operands in MFMA fragment order
+  DBuf<uint8_t> packLlo, packUlo; // low parts of the two-term FP16 operands
   DBuf<int32_t> scaleL, scaleU; // per-32-row / per-32-column power-of-two exponents
   DBuf<unsigned long long> cnt; // [0] overflow, [1] subnormal counts of the operand conversion
   DBuf<int32_t> dinfo;   // [0] info, [1] exchange timeout flag
@@ -71,7 +72,9 @@ struct ResidArgs {
 void residual(const ResidArgs& a, Workspace& W);
 void absmax(const void* A, bool a32, int64_t n, int64_t lda, double* out_dev, Workspace& W);
 
-enum TrailFmt { FMT_FP16 = 0, FMT_BF16 = 1, FMT_FP8 = 2, FMT_FP32 = 3 };
+// FMT_FP16X3: operands split into FP16 hi + lo parts (same tile scale), update = hi*hi + hi*lo + lo*hi on
+// FP16 MFMA with FP32 accumulation.
+enum TrailFmt { FMT_FP16 = 0, FMT_BF16 = 1, FMT_FP8 = 2, FMT_FP32 = 3, FMT_FP16X3 = 4 };
 
 struct GetrfArgs {
   int64_t n = 0; float* A = nullptr; int64_t lda = 0; int nb = 256; int fmt = FMT_FP16; bool tile_scale = true;

@@ -33,7 +33,8 @@ enum {
 };
 
 /* trailing-update operand formats of the blocked LU */
-enum { MP_FMT_FP16 = 0, MP_FMT_BF16 = 1, MP_FMT_FP8E4M3 = 2, MP_FMT_FP32 = 3 };
+enum { MP_FMT_FP16 = 0, MP_FMT_BF16 = 1, MP_FMT_FP8E4M3 = 2, MP_FMT_FP32 = 3,
+       MP_FMT_FP16X3 = 4 /* two-term FP16 operands: hi*hi + hi*lo + lo*hi on FP16 MFMA */ };
 
 /* ---------------------------------------------------------------- context
  * Replaces Ctx::init (src/mp_core.cuh:9-11) and the `caps` command (src/mpsolve.cu:137-143). */

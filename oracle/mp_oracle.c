@@ -118,12 +118,13 @@ float or_round_e4m3(float x) {
   if (r > 448.0f) r = 448.0f;
   return copysignf(r, x);
 }
+/* formats: 0 FP16, 1 BF16, 2 FP8 e4m3, 3 FP32, 4 two-term FP16 (hi = fp16(x), lo = fp16(x - hi)) */
 static float round_fmt(int fmt, float x) {
-  switch (fmt) { case 0: return or_round_fp16(x); case 1: return or_round_bf16(x); case 2: return or_round_e4m3(x); default: return x; }
+  switch (fmt) { case 0: case 4: return or_round_fp16(x); case 1: return or_round_bf16(x); case 2: return or_round_e4m3(x); default: return x; }
 }
-static int top_exp(int fmt) { return fmt == 0 ? 14 : (fmt == 1 ? 14 : (fmt == 2 ? 7 : 0)); }
-static float fmt_max(int fmt) { return fmt == 0 ? 65504.0f : (fmt == 1 ? 3.3895313892515355e+38f : (fmt == 2 ? 448.0f : FLT_MAX)); }
-static float fmt_minnorm(int fmt) { return fmt == 0 ? 6.103515625e-05f : (fmt == 2 ? 0.015625f : FLT_MIN); }
+static int top_exp(int fmt) { return (fmt == 0 || fmt == 4) ? 14 : (fmt == 1 ? 14 : (fmt == 2 ? 7 : 0)); }
+static float fmt_max(int fmt) { return (fmt == 0 || fmt == 4) ? 65504.0f : (fmt == 1 ? 3.3895313892515355e+38f : (fmt == 2 ? 448.0f : FLT_MAX)); }
+static float fmt_minnorm(int fmt) { return (fmt == 0 || fmt == 4) ? 6.103515625e-05f : (fmt == 2 ? 0.015625f : FLT_MIN); }
 
 /* power-of-two exponent of a 32-index tile: tile max lands in [2^TOP, 2^(TOP+1)) */
 static int tile_exp(int fmt, float tmax, int tile_scale) {
@@ -149,7 +150,7 @@ int or_getrf(int64_t n, int nb, float* A, int64_t lda, int64_t* ipiv, int32_t* p
   int info = 0;
   for (int64_t i = 0; i < n; i++) perm[i] = (int32_t)i;
   if (counts) { counts[0] = 0; counts[1] = 0; }
-  float* qL = NULL; float* qU = NULL; int* eL = NULL; int* eU = NULL;
+  float* qL = NULL; float* qU = NULL; float* qLl = NULL; float* qUl = NULL; int* eL = NULL; int* eU = NULL;
   for (int64_t k = 0; k < n; k += nb) {
     const int kb = (int)((n - k < nb) ? (n - k) : nb);
     for (int j = 0; j < kb; j++) {
@@ -228,6 +229,8 @@ int or_getrf(int64_t n, int nb, float* A, int64_t lda, int64_t* ipiv, int32_t* p
     const int64_t nt = (m2 + 31) / 32;
     qL = (float*)realloc(qL, sizeof(float) * (size_t)m2 * kb);
     qU = (float*)realloc(qU, sizeof(float) * (size_t)m2 * kb);
+    qLl = (float*)realloc(qLl, sizeof(float) * (size_t)m2 * kb);
+    qUl = (float*)realloc(qUl, sizeof(float) * (size_t)m2 * kb);
     eL = (int*)realloc(eL, sizeof(int) * (size_t)nt);
     eU = (int*)realloc(eU, sizeof(int) * (size_t)nt);
     for (int64_t t = 0; t < nt; t++) {
@@ -253,19 +256,24 @@ int or_getrf(int64_t n, int nb, float* A, int64_t lda, int64_t* ipiv, int32_t* p
           }
           qL[ii * kb + q] = round_fmt(fmt, vl);
           qU[ii * kb + q] = round_fmt(fmt, vu);
+          qLl[ii * kb + q] = (fmt == 4) ? or_round_fp16(vl - qL[ii * kb + q]) : 0.0f;
+          qUl[ii * kb + q] = (fmt == 4) ? or_round_fp16(vu - qU[ii * kb + q]) : 0.0f;
         }
     }
     for (int64_t c = 0; c < m2; c++)
       for (int64_t r = 0; r < m2; r++) {
         double s = 0.0;
-        for (int q = 0; q < kb; q++) s += (double)qL[r * kb + q] * (double)qU[c * kb + q];
+        for (int q = 0; q < kb; q++) {
+          s += (double)qL[r * kb + q] * (double)qU[c * kb + q];
+          if (fmt == 4) s += (double)qL[r * kb + q] * (double)qUl[c * kb + q] + (double)qLl[r * kb + q] * (double)qU[c * kb + q];
+        }
         const float acc = (float)s;
         const float sc = ldexpf(1.0f, -(eL[r / 32] + eU[c / 32]));
         float* cp = &A[(k + kb + r) + (k + kb + c) * lda];
         *cp = fmaf(-acc, sc, *cp);
       }
   }
-  free(qL); free(qU); free(eL); free(eU);
+  free(qL); free(qU); free(qLl); free(qUl); free(eL); free(eU);
   return info;
 }
 

@@ -56,7 +56,7 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
-FMT = {"fp16": 0, "bf16": 1, "fp8": 2, "fp32": 3}
+FMT = {"fp16": 0, "bf16": 1, "fp8": 2, "fp32": 3, "fp16x3": 4}
 
 
 def resid_fp64(A, X, B=None):

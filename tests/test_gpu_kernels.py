@@ -61,7 +61,7 @@ def test_residual_dd_matches_oracle(ctx, oracle, n, nrhs):
     assert np.all(np.abs(L) <= np.spacing(np.abs(H)) / 2 + 1e-320)
 
 
-@pytest.mark.parametrize("fmt", ["fp16", "bf16", "fp8", "fp32"])
+@pytest.mark.parametrize("fmt", ["fp16", "bf16", "fp8", "fp32", "fp16x3"])
 @pytest.mark.parametrize("n,nb", [(64, 32), (200, 64), (513, 128), (700, 256)])
 def test_getrf_exact_bitwise(ctx, oracle, fmt, n, nb):
     print(f"[getrf exact] fmt={fmt} n={n} nb={nb}", flush=True)
@@ -78,7 +78,7 @@ def test_getrf_exact_bitwise(ctx, oracle, fmt, n, nb):
     np.testing.assert_array_equal(LU, LUo)
 
 
-@pytest.mark.parametrize("fmt", ["fp16", "bf16", "fp8", "fp32"])
+@pytest.mark.parametrize("fmt", ["fp16", "bf16", "fp8", "fp32", "fp16x3"])
 def test_getrf_random_backward_error(ctx, oracle, fmt):
     """Random Gaussian matrix: the GPU factors have the backward error of the same algorithm run by the oracle
     (within a factor 4 either way: too small would mean the operand rounding was skipped; the MFMA sums in

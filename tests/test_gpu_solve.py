@@ -90,6 +90,20 @@ def test_precision_ladder_escalates(ctx, oracle):
     assert oracle.metrics(A, b, X, dd=True)["eta_max"] <= 1e-15
 
 
+@pytest.mark.parametrize("method", ["mx16x3_gmresir_w64_rdd", "policy_mx16_ladder"])
+def test_two_term_fp16_kappa1e6(ctx, oracle, method):
+    """Two-term FP16 operands (hi + lo) give FP32-level factors on FP16 MFMA, which FGMRES-IR takes to
+    FP64 accuracy at kappa=1e6 where single-term FP16 stagnates."""
+    A, b, xt, _ = randsvd(1024, 1e6, 9201, gpu=False)
+    print(f"[solve] {method} n=1024 kappa=1e6", flush=True)
+    P = ctx.problem(A, b, x_exact=xt)
+    X, rec = P.solve(method, nb=128, gmres_m=30, tau=1e-15, stop_on="eta", max_outer=50)
+    assert rec["terminal_reason"] == "converged", (rec["stages"], rec["terminal_reason"], rec["eta_max"])
+    assert oracle.metrics(A, b, X, dd=True)["eta_max"] <= 1e-15
+    if method == "mx16x3_gmresir_w64_rdd":
+        assert rec["refactorizations"] == 1 and rec["outer_iters"] <= 6
+
+
 def test_multiple_rhs_stationary(ctx, oracle):
     A, b, xt, _ = randsvd(640, 1e3, 6, nrhs=5, gpu=False)
     P = ctx.problem(A, b, x_exact=xt)

@@ -206,6 +206,10 @@ def test_trailing_format_error_levels(oracle):
         L, U = oracle.unpack_lu(LU)
         errs[fmt] = np.abs(A[perm].astype(np.float64) - L @ U).max() / np.abs(A).max()
     assert errs["fp32"] < 1e-5 < errs["fp16"] < 2e-2 < errs["bf16"] < 0.2 < errs["fp8"] < 3.0
+    LU, ipiv, perm, info, _ = oracle.getrf(A, 64, "fp16x3")
+    L, U = oracle.unpack_lu(LU)
+    e3 = np.abs(A[perm].astype(np.float64) - L @ U).max() / np.abs(A).max()
+    assert e3 < 10 * errs["fp32"], (e3, errs["fp32"])     # two-term FP16 recovers FP32-level factors
 
 
 def test_getrs_is_lu_solve(oracle):

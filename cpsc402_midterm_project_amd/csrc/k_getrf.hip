@@ -31,6 +31,12 @@ constexpr int MAX_KB = 512;
 #ifndef PANEL_MARK
 #define PANEL_MARK(phase)
 #endif
+#ifndef PANEL2_MARK
+#define PANEL2_MARK(col, phase)
+#endif
+#ifndef PANEL2_COUNT
+#define PANEL2_COUNT(col, value)
+#endif
 constexpr long SPIN_MAX = 1L << 22;
 
 // ------------------------------------------------------------------ tagged-word exchange
@@ -310,6 +316,464 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs a) {
     }
   }
   if (t == 0 && s_abort) a.dinfo[1] = 1;
+}
+
+// ------------------------------------------------------------------ register-resident panel
+// k_panel2: the same pivoting and the same floating-point operations per element as k_panel, arranged so a
+// column step costs one exchange of head words plus one read of the winner's row, with no LDS traffic
+// proportional to the panel width.
+//   * Workgroup g owns panel rows [128 g, 128 g + 128).  Wave 0 holds rows lane and lane + 64 of the
+//     current 32-column sub-block in registers, rotated one place per step so column j is always x[0]
+//     (every register index is a compile-time constant).  The whole panel row also lives in LDS (P).
+//   * Rows never move: each carries its current position `pos` (LAPACK's sequential-interchange order).
+//     At step j the winner's position becomes j and the row that held position j takes the winner's old
+//     position, so "ties to the smallest index" compares positions.
+//   * Exchange per step: wave 0 finds its workgroup's candidate (DPP argmax), writes the candidate's 32
+//     values to its row slot (four replicas) and pushes one head word (tag, position, |value|) into every
+//     workgroup's mailbox.  Wave 1 reads its own mailbox, picks the winner, reads the winner's row from
+//     replica g % 4 into LDS and raises an LDS step flag that releases wave 0's rank-1 update.  Mailboxes
+//     sit 4 KB apart and replicas 64 KB apart, so the polling of 100+ workgroups spreads over memory
+//     channels instead of queueing on a few cache lines.  Wave 0 never loads from global memory inside
+//     the step loop (gfx9 counts loads and stores with one counter).
+//   * After 32 steps the panel columns right of the sub-block get the deferred update
+//     a -= sum_d l_d u_d (d in step order, so each element sees the same fmaf chain as 32 rank-1 updates):
+//     the owners publish their pivot rows (four replicas, double-buffered by sub-block), every workgroup
+//     forms the 32 U rows by forward substitution.  The next 32 columns are updated at once by all four
+//     waves; the columns beyond them are updated by waves 2-3 while waves 0-1 factor the next sub-block.
+constexpr int P2_R = 128;
+constexpr int P2_PS = 129;          // LDS column stride of the panel slice (odd: column-per-lane access is conflict-free)
+constexpr int P2_MAXG = 256;        // workgroups = exchange participants
+constexpr int P2_MBOX = 512;        // words from one mailbox to the next (4 KB)
+constexpr int P2_NREP = 4;          // replicas of published rows
+constexpr int P2_US = 36;           // padded row length of the U staging tile (16-byte aligned rows)
+// exchange area (64-bit words): mailboxes [parity][reader][P2_MBOX], candidate rows [parity][replica]
+// [participant][32], sub-block rows [sub-block parity][replica][32 rows][256 columns]
+constexpr int64_t P2_OFF_ROWS = 2LL * P2_MAXG * P2_MBOX;
+constexpr int64_t P2_OFF_SUB = P2_OFF_ROWS + 2LL * P2_NREP * P2_MAXG * 32;
+constexpr int64_t P2_SUBREP = 32 * 256;
+constexpr int64_t P2_WORDS = P2_OFF_SUB + 2 * P2_NREP * P2_SUBREP;
+
+struct Panel2Args {
+  float* A; int64_t lda; int64_t k; int kb; int m; int G;
+  uint64_t* xbuf;    // P2_WORDS words, zeroed before the factorization (tags are unique within it)
+  int64_t* ipiv; int32_t* dinfo; int32_t* plan;
+};
+
+// P occupies kb * P2_PS floats rounded up to a multiple of 4, so the float4-accessed arrays after it stay
+// 16-byte aligned.
+__host__ __device__ __forceinline__ int p2_pfloats(int kb) { return (kb * P2_PS + 3) & ~3; }
+static size_t panel2_lds_bytes(int kb) {
+  return (size_t)p2_pfloats(kb) * 4 + 32 * P2_US * 4 + 32 * 32 * 4 + 32 * 4 + 32 * 4 + 256 * 4 + 32 * 4 +
+         P2_R * 4 * 3 + 16 * 4;
+}
+
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+__device__ __forceinline__ void lds_wait() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+__device__ __forceinline__ int wave_min_dpp(int x) {
+  x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x111, 0xf, 0xf, false));
+  x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x112, 0xf, 0xf, false));
+  x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x114, 0xf, 0xf, false));
+  x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x118, 0xf, 0xf, false));
+  x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x142, 0xa, 0xf, false));
+  x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x143, 0xc, 0xf, false));
+  return __builtin_amdgcn_readlane(x, 63);
+}
+// Wave argmax of (v, pos): largest v, ties to the smallest pos; v < 0 marks "no candidate".  Returns the
+// winning lane (wave-uniform) or -1, and the winner's v / pos in every lane.
+__device__ __forceinline__ int wave_argmax_pos(float v, int pos, float& bv, int& bpos) {
+  const float mx = wave_max_dpp(v);
+  if (mx < 0.0f) { bv = -1.0f; bpos = 0x7FFFFFFF; return -1; }
+  const unsigned long long tie = __ballot(v == mx);
+  int lane;
+  if (__popcll(tie) == 1) {
+    lane = __ffsll((long long)tie) - 1;
+  } else {
+    const int mn = wave_min_dpp(v == mx ? pos : 0x7FFFFFFF);
+    lane = __ffsll((long long)__ballot(v == mx && pos == mn)) - 1;
+  }
+  lane = __builtin_amdgcn_readfirstlane(lane);
+  bv = mx;
+  bpos = __builtin_amdgcn_readlane(pos, lane);
+  return lane;
+}
+// head word: [63:47] 17-bit tag, [46:31] position, [30:0] |value| bits (0x7FFFFFFF: no candidate)
+__device__ __forceinline__ uint64_t head_word(uint32_t tag, int pos, uint32_t vbits) {
+  return ((uint64_t)(tag & 0x1FFFFu) << 47) | ((uint64_t)(pos & 0xFFFF) << 31) | (uint64_t)(vbits & 0x7FFFFFFFu);
+}
+
+// Poll the 32 U-row words of one trailing column (stride 256 words) and turn them into the final U values
+// by forward substitution, right-looking so the 31 updates per step are independent (each u_q still
+// receives its updates in increasing d).  LsT[d][q] = l_{q,d}; steps whose pivot was zero are skipped.
+__device__ __forceinline__ void p2_u_column(const uint64_t* xs, uint32_t stag, const float* LsT, int zmask,
+                                            volatile int* s_abort, float (&u)[32]) {
+#pragma unroll
+  for (int q16 = 0; q16 < 32; q16 += 16) {
+    uint32_t need = 0xFFFFu;
+    for (long it = 0; it < SPIN_MAX && need; it++) {
+      uint64_t wq[16];
+#pragma unroll
+      for (int i = 0; i < 16; i++)
+        wq[i] = (need >> i & 1u) ? __hip_atomic_load(xs + (q16 + i) * 256, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+#pragma unroll
+      for (int i = 0; i < 16; i++)
+        if ((need >> i & 1u) && (uint32_t)(wq[i] >> 32) == stag) { u[q16 + i] = __uint_as_float((uint32_t)wq[i]); need &= ~(1u << i); }
+      if (!need || *s_abort) break;
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (need) {
+      *s_abort = 1;
+#pragma unroll
+      for (int i = 0; i < 16; i++) if (need >> i & 1u) u[q16 + i] = 0.0f;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 31; d++) {
+    if (zmask >> d & 1) continue;
+    const float4* l4 = (const float4*)(LsT + d * 32);
+#pragma unroll
+    for (int q4 = 0; q4 < 8; q4++) {
+      if (4 * q4 + 3 <= d) continue;
+      const float4 lq = l4[q4];
+      const float le[4] = {lq.x, lq.y, lq.z, lq.w};
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int q = 4 * q4 + e;
+        if (q > d) u[q] = fmaf(-le[e], u[d], u[q]);
+      }
+    }
+  }
+}
+// a -= sum_d l_d u_d over the steps in emask (bit d), l_d = P[(c0 + d) * P2_PS + r]
+__device__ __forceinline__ float p2_apply(float a, const float* P, int c0, int r, const float (&u)[32], uint32_t em) {
+  if (em == 0xFFFFFFFFu) {
+#pragma unroll
+    for (int d = 0; d < 32; d++) a = fmaf(-P[(c0 + d) * P2_PS + r], u[d], a);
+  } else {
+#pragma unroll
+    for (int d = 0; d < 32; d++) {
+      const float nv = fmaf(-P[(c0 + d) * P2_PS + r], u[d], a);
+      a = (em >> d & 1u) ? nv : a;
+    }
+  }
+  return a;
+}
+__device__ __forceinline__ uint32_t p2_emask(int dlim, int zmask) {
+  if (dlim <= 0) return 0u;
+  return (dlim >= 32 ? 0xFFFFFFFFu : ((1u << dlim) - 1u)) & ~(uint32_t)zmask;
+}
+
+__global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
+  extern __shared__ float lds[];
+  const int kb = a.kb, m = a.m, G = a.G;
+  float* P = lds;                                         // [kb][P2_PS]
+  float* Us = P + p2_pfloats(kb);                         // [32][P2_US] U rows of the next 32 columns
+  float* LsT = Us + 32 * P2_US;                           // [32][32]: LsT[d][q] = l_{q,d}
+  float* prow = LsT + 32 * 32;                            // [32] winner's row of the current step
+  float* crow = prow + 32;                                // [32] this workgroup's candidate row
+  int* s_piv = (int*)(crow + 32);                         // [256]
+  int* piv_local = s_piv + 256;                           // [32]
+  int* pstep = piv_local + 32;                            // [128] step at which a row became pivot (or -1 / 32)
+  int* pstep_bg = pstep + P2_R;                           // [128] copy for the background update
+  int* spos = pstep_bg + P2_R;                            // [128]
+  int* misc = spos + P2_R;                                // [16]
+  volatile int* s_abort = misc + 0;
+  int* s_p = misc + 1; int* s_pw = misc + 2; int* s_info = misc + 3; int* s_zmask = misc + 4;
+  float* s_rinv = (float*)(misc + 5);
+  volatile int* s_step = misc + 6;
+
+  const int t = threadIdx.x, g = blockIdx.x, lane = t & 63, wave = t >> 6;
+  const int r0 = g * P2_R;
+  const int rows = max(0, min(P2_R, m - r0));
+  float* Ap = a.A + a.k + a.k * a.lda;
+  uint64_t* mbox = a.xbuf;
+  uint64_t* rowsl = a.xbuf + P2_OFF_ROWS;
+  const int myrep = g % P2_NREP;
+  if (t == 0) { *s_abort = 0; *s_info = 0; *s_zmask = 0; *s_step = -1; }
+  for (int idx = t; idx < kb * P2_R; idx += 256) {
+    const int c = idx >> 7, r = idx & (P2_R - 1);
+    P[c * P2_PS + r] = (r < rows) ? Ap[(int64_t)(r0 + r) + (int64_t)c * a.lda] : 0.0f;
+  }
+  // rows of wave 0: set 0 = row lane, set 1 = row lane + 64
+  const bool v0 = (wave == 0) && lane < rows, v1 = (wave == 0) && lane + 64 < rows;
+  int pos0 = r0 + lane, pos1 = r0 + lane + 64;
+  int bg_c0 = -1, bg_zmask = 0;                           // background update pending for waves 2-3
+  lds_barrier();
+
+  for (int c0 = 0; c0 < kb; c0 += 32) {
+    const int w = min(32, kb - c0);
+    int ps0 = -1, ps1 = -1;                               // -1: pivot before this sub-block (or no row)
+    if (wave >= 2) {
+      // far columns of the previous sub-block: thread per column, all 128 rows
+      if (bg_c0 >= 0) {
+        const uint32_t btag = (uint32_t)(a.k + bg_c0 + 1);
+        const uint64_t* xsb = a.xbuf + P2_OFF_SUB + (int64_t)(((bg_c0 >> 5) & 1) * P2_NREP + myrep) * P2_SUBREP;
+        for (int fc = bg_c0 + 64 + (t - 128); fc < kb; fc += 128) {
+          float u[32];
+          p2_u_column(xsb + fc, btag, LsT, bg_zmask, s_abort, u);
+          for (int r = 0; r < P2_R; r++) {
+            const uint32_t em = p2_emask(pstep_bg[r], bg_zmask);
+            if (em == 0u) continue;
+            P[fc * P2_PS + r] = p2_apply(P[fc * P2_PS + r], P, bg_c0, r, u, em);
+          }
+        }
+      }
+    } else if (wave == 0) {
+      float x0[32], x1[32];
+#pragma unroll
+      for (int q = 0; q < 32; q++) {
+        x0[q] = (c0 + q < kb) ? P[(c0 + q) * P2_PS + lane] : 0.0f;
+        x1[q] = (c0 + q < kb) ? P[(c0 + q) * P2_PS + lane + 64] : 0.0f;
+      }
+      ps0 = (v0 && pos0 >= c0) ? 32 : -1;
+      ps1 = (v1 && pos1 >= c0) ? 32 : -1;
+      for (int jj = 0; jj < w; jj++) {
+        const int j = c0 + jj;
+        const uint32_t tag = (uint32_t)(a.k + j + 1);
+        const int par = j & 1;
+        PANEL2_MARK(j, 0);
+        // (1) the workgroup's candidate and its publication
+        float cv0 = -1.0f, cv1 = -1.0f;
+        if (v0 && pos0 >= j) { float xa = fabsf(x0[0]); if (xa != xa) xa = INFINITY; cv0 = xa; }
+        if (v1 && pos1 >= j) { float xa = fabsf(x1[0]); if (xa != xa) xa = INFINITY; cv1 = xa; }
+        const bool take1 = (cv1 > cv0) || (cv1 == cv0 && cv1 >= 0.0f && pos1 < pos0);
+        float bv; int bpos;
+        const int blane = wave_argmax_pos(take1 ? cv1 : cv0, take1 ? pos1 : pos0, bv, bpos);
+        const bool bsel = (blane >= 0) && (__builtin_amdgcn_readlane((int)take1, blane) != 0);
+        if (blane >= 0) {
+          // the candidate lane stages its 32 values in LDS; lanes 0-31 / 32-63 store replicas 0,1 / 2,3
+          if (lane == blane) {
+            float4* c4 = (float4*)crow;
+            if (bsel) {
+#pragma unroll
+              for (int q = 0; q < 8; q++) c4[q] = make_float4(x1[4 * q], x1[4 * q + 1], x1[4 * q + 2], x1[4 * q + 3]);
+            } else {
+#pragma unroll
+              for (int q = 0; q < 8; q++) c4[q] = make_float4(x0[4 * q], x0[4 * q + 1], x0[4 * q + 2], x0[4 * q + 3]);
+            }
+          }
+          lds_wait();
+          const uint32_t mine = __float_as_uint(crow[lane & 31]);
+          const int rep = lane >> 5;
+          put_word(rowsl + (((size_t)(par * P2_NREP + rep) * P2_MAXG + g) * 32 + (lane & 31)), tag, mine);
+          put_word(rowsl + (((size_t)(par * P2_NREP + rep + 2) * P2_MAXG + g) * 32 + (lane & 31)), tag, mine);
+        }
+        const uint64_t hwv = head_word(tag, blane >= 0 ? bpos : 0, blane >= 0 ? __float_as_uint(bv) : 0x7FFFFFFFu);
+        for (int rd = lane; rd < G; rd += 64)
+          __hip_atomic_store(mbox + ((size_t)par * P2_MAXG + rd) * P2_MBOX + g, hwv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        PANEL2_MARK(j, 1);
+        // (3) wait for wave 1's step flag
+        for (long it = 0; it < SPIN_MAX && *s_step != j; it++) {}
+        PANEL2_MARK(j, 2);
+        const int p = *s_p, pw = *s_pw;
+        float pr[32];
+#pragma unroll
+        for (int q = 0; q < 32; q++) pr[q] = prow[q];
+        const float rinv = *s_rinv;
+        lds_wait();
+        const float piv = pr[0];
+        const bool mine_win = (pw == g) && (lane == blane);
+        // the pivot row's remaining sub-block values are final now: store them (single lane)
+        if (mine_win) {
+          const int qmax = min(32 - jj, kb - j);            // columns j .. j+qmax-1 exist in this sub-block
+          if (bsel) {
+#pragma unroll
+            for (int q = 0; q < 32; q++) if (q < qmax) P[(j + q) * P2_PS + lane + 64] = x1[q];
+            pos1 = j; ps1 = jj; piv_local[jj] = lane + 64;
+          } else {
+#pragma unroll
+            for (int q = 0; q < 32; q++) if (q < qmax) P[(j + q) * P2_PS + lane] = x0[q];
+            pos0 = j; ps0 = jj; piv_local[jj] = lane;
+          }
+        }
+        if (!(mine_win && !bsel) && v0 && pos0 == j) pos0 = p;
+        if (!(mine_win && bsel) && v1 && pos1 == j) pos1 = p;
+        // rows still in play (position > j) take the multiplier; pivoted rows are finished and their
+        // registers are no longer stored.  A zero pivot leaves every row unchanged.
+        if (piv != 0.0f) {
+          const bool a0 = v0 && pos0 > j, a1 = v1 && pos1 > j;
+          const bool normal = fabsf(piv) >= 1.17549435e-38f;
+          float l0 = normal ? x0[0] * rinv : x0[0] / piv;
+          float l1 = normal ? x1[0] * rinv : x1[0] / piv;
+          if (a0) P[j * P2_PS + lane] = l0;
+          if (a1) P[j * P2_PS + lane + 64] = l1;
+          l0 = a0 ? l0 : 0.0f; l1 = a1 ? l1 : 0.0f;
+#pragma unroll
+          for (int q = 1; q < 32; q++) { x0[q - 1] = fmaf(-l0, pr[q], x0[q]); x1[q - 1] = fmaf(-l1, pr[q], x1[q]); }
+        } else {
+          if (v0 && pos0 > j) P[j * P2_PS + lane] = x0[0];
+          if (v1 && pos1 > j) P[j * P2_PS + lane + 64] = x1[0];
+#pragma unroll
+          for (int q = 1; q < 32; q++) { x0[q - 1] = x0[q]; x1[q - 1] = x1[q]; }
+        }
+        x0[31] = 0.0f; x1[31] = 0.0f;
+        lds_wait();   // the prow reads of this step are complete before the next publication
+        PANEL2_MARK(j, 3);
+      }
+    } else {
+      // wave 1: the exchange side of every step
+      if (lane == 0) *s_zmask = 0;
+      if (lane < 32) piv_local[lane] = -1;
+      for (int jj = 0; jj < w; jj++) {
+        const int j = c0 + jj;
+        const uint32_t tag = (uint32_t)(a.k + j + 1);
+        const int par = j & 1;
+        PANEL2_MARK(j, 4);
+        // (2) own mailbox -> winner -> the winner's 32 values
+        const uint64_t* mb = mbox + ((size_t)par * P2_MAXG + g) * P2_MBOX;
+        uint64_t hw[4] = {0ull, 0ull, 0ull, 0ull};
+        uint32_t need = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) if (lane + 64 * i < G) need |= 1u << i;
+        long it_heads = 0;
+        for (long it = 0; it < SPIN_MAX; it++, it_heads++) {
+          uint64_t wv[4];
+#pragma unroll
+          for (int i = 0; i < 4; i++)
+            wv[i] = (need >> i & 1u) ? __hip_atomic_load(mb + lane + 64 * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+#pragma unroll
+          for (int i = 0; i < 4; i++)
+            if ((need >> i & 1u) && (uint32_t)(wv[i] >> 47) == (tag & 0x1FFFFu)) { hw[i] = wv[i]; need &= ~(1u << i); }
+          if (__ballot(need != 0u) == 0ull) break;
+          if (*s_abort) break;
+          __builtin_amdgcn_s_sleep(1);
+        }
+        if (__ballot(need != 0u) != 0ull) *s_abort = 1;
+        PANEL2_COUNT(j, it_heads);
+        (void)it_heads;
+        float gv = -1.0f; int gpos = 0x7FFFFFFF, gpp = -1;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int pp = lane + 64 * i;
+          const uint32_t vb = (uint32_t)(hw[i] & 0x7FFFFFFFull);
+          if (pp < G && !(need >> i & 1u) && vb != 0x7FFFFFFFu) {
+            const float hvf = __uint_as_float(vb);
+            const int hpos = (int)((hw[i] >> 31) & 0xFFFFull);
+            if (hvf > gv || (hvf == gv && hpos < gpos)) { gv = hvf; gpos = hpos; gpp = pp; }
+          }
+        }
+        PANEL2_MARK(j, 5);
+        float bv; int p;
+        const int wl = wave_argmax_pos(gv, gpos, bv, p);
+        int wpp = (wl >= 0) ? __builtin_amdgcn_readlane(gpp, wl) : -1;
+        if (wl < 0 || p < j || p >= m) { *s_abort = 1; p = j; wpp = -1; }
+        float pv = 0.0f;
+        if (lane < 32 && wpp >= 0)
+          pv = __uint_as_float(get_word_b(rowsl + (((size_t)(par * P2_NREP + myrep) * P2_MAXG + wpp) * 32 + lane), tag, s_abort));
+        if (lane < 32) prow[lane] = pv;
+        if (lane == 0) {
+          *s_p = p; *s_pw = wpp; s_piv[j] = p; *s_rinv = 1.0f / pv;
+          if (pv == 0.0f) { *s_zmask |= 1 << jj; if (*s_info == 0) *s_info = (int)(a.k + j + 1); }
+        }
+        lds_wait();
+        if (lane == 0) *s_step = j;                      // releases wave 0 (LDS operations of a wave complete in order)
+        PANEL2_MARK(j, 6);
+      }
+    }
+    if (wave == 0) { pstep[lane] = ps0; pstep[lane + 64] = ps1; }
+    const int rem = kb - c0 - 32;
+    PANEL2_MARK(c0 + 1, 7);
+    lds_barrier();                                        // P, pstep, piv_local, zmask complete; background done
+    bg_c0 = -1;
+    if (rem <= 0) continue;                               // only the last sub-block can be partial
+
+    // deferred update of the columns [c0 + 32, kb): the next 32 now, the rest in the background
+    const uint32_t stag = (uint32_t)(a.k + c0 + 1);
+    const int zmask = *s_zmask;
+    uint64_t* xsp = a.xbuf + P2_OFF_SUB + (int64_t)((c0 >> 5) & 1) * P2_NREP * P2_SUBREP;
+    if (t < P2_R) pstep_bg[t] = pstep[t];
+    if (wave >= 2) {
+      // the owners publish their pivot rows (columns [c0, kb)) to every replica
+      const int tq = t - 128;
+      for (int q = 0; q < 32; q++) {
+        const int r = piv_local[q];
+        if (r >= 0)
+          for (int c = c0 + tq; c < kb; c += 128) {
+            const uint32_t bits = __float_as_uint(P[c * P2_PS + r]);
+#pragma unroll
+            for (int rep = 0; rep < P2_NREP; rep++) put_word(xsp + rep * P2_SUBREP + q * 256 + c, stag, bits);
+          }
+      }
+    } else {
+      // LsT[d][q] = l_{q,d}: sub-block values of pivot row q, waves 0-1
+      const uint64_t* xs = xsp + myrep * P2_SUBREP;
+      uint32_t vals[8];
+      uint32_t need = 0xFFu;
+      for (long it = 0; it < SPIN_MAX && need; it++) {
+        uint64_t wv[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          const int idx = t + P2_R * i, q = idx >> 5, d = idx & 31;
+          wv[i] = (need >> i & 1u) ? __hip_atomic_load(xs + q * 256 + c0 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+          if ((need >> i & 1u) && (uint32_t)(wv[i] >> 32) == stag) { vals[i] = (uint32_t)wv[i]; need &= ~(1u << i); }
+        if (!need || *s_abort) break;
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (need) { *s_abort = 1; for (int i = 0; i < 8; i++) if (need >> i & 1u) vals[i] = 0u; }
+#pragma unroll
+      for (int i = 0; i < 8; i++) { const int idx = t + P2_R * i; LsT[(idx & 31) * 32 + (idx >> 5)] = __uint_as_float(vals[i]); }
+    }
+    lds_barrier();
+    PANEL2_MARK(c0 + 2, 7);
+    // next 32 columns: U rows by 32 threads of wave 0, then all 256 threads (column t & 31, rows 16 (t >> 5) ..)
+    const int nw = min(32, rem);
+    if (t < nw) {
+      float u[32];
+      p2_u_column(xsp + myrep * P2_SUBREP + c0 + 32 + t, stag, LsT, zmask, s_abort, u);
+#pragma unroll
+      for (int q = 0; q < 32; q++) Us[t * P2_US + q] = u[q];
+    }
+    lds_barrier();
+    PANEL2_MARK(c0 + 3, 7);
+    {
+      const int col = t & 31, rg = t >> 5;
+      if (col < nw) {
+        float u[32];
+        const float4* u4 = (const float4*)(Us + col * P2_US);
+#pragma unroll
+        for (int q4 = 0; q4 < 8; q4++) { const float4 v = u4[q4]; u[4 * q4] = v.x; u[4 * q4 + 1] = v.y; u[4 * q4 + 2] = v.z; u[4 * q4 + 3] = v.w; }
+        const int c = c0 + 32 + col;
+        for (int r = rg * 16; r < rg * 16 + 16; r++) {
+          const uint32_t em = p2_emask(pstep[r], zmask);
+          if (em == 0u) continue;
+          P[c * P2_PS + r] = p2_apply(P[c * P2_PS + r], P, c0, r, u, em);
+        }
+      }
+    }
+    lds_barrier();
+    PANEL2_MARK(c0 + 4, 7);
+    if (rem > 32) { bg_c0 = c0; bg_zmask = zmask; }
+    PANEL2_MARK(c0, 7);
+  }
+
+  // results: rows to their final positions, pivots, info, swap plan
+  if (wave == 0) { spos[lane] = pos0; spos[lane + 64] = pos1; }
+  lds_barrier();
+  const bool ok = (*s_abort == 0);
+  if (ok) {
+    for (int idx = t; idx < kb * P2_R; idx += 256) {
+      const int c = idx >> 7, r = idx & (P2_R - 1);
+      if (r < rows) Ap[(int64_t)spos[r] + (int64_t)c * a.lda] = P[c * P2_PS + r];
+    }
+    if (g == 0) {
+      for (int jj = t; jj < kb; jj += 256) a.ipiv[a.k + jj] = a.k + s_piv[jj] + 1;
+      if (t == 0 && *s_info != 0 && a.dinfo[0] == 0) a.dinfo[0] = *s_info;
+    }
+    if (wave == 0) {
+      if (v0 && pos0 != r0 + lane) {
+        const int slot = atomicAdd(a.plan, 1);
+        a.plan[1 + 2 * slot] = pos0; a.plan[2 + 2 * slot] = r0 + lane;
+      }
+      if (v1 && pos1 != r0 + lane + 64) {
+        const int slot = atomicAdd(a.plan, 1);
+        a.plan[1 + 2 * slot] = pos1; a.plan[2 + 2 * slot] = r0 + lane + 64;
+      }
+    }
+  }
+  if (t == 0 && *s_abort) a.dinfo[1] = 1;
 }
 
 // ------------------------------------------------------------------ row interchanges outside the panel
@@ -703,22 +1167,37 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
     // dynamic limit leaves room for each kernel's static shared memory (at most 8 KB here)
     HIP_CK(hipFuncSetAttribute((const void*)k_panel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 8192));
     HIP_CK(hipFuncSetAttribute((const void*)k_trsm, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 8192));
+    HIP_CK(hipFuncSetAttribute((const void*)k_panel2, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes()));
     attr_set = true;
   }
   // exchange area for the largest panel, cleared once per factorization (tags restart at 1)
   {
     const int64_t Gmax = std::min<int64_t>(128, cdiv(n, 64));
-    const int64_t need = 2 * (Gmax + Gmax * (nb + 1) + (nb + 1));
+    const int64_t need = std::max<int64_t>(2 * (Gmax + Gmax * (nb + 1) + (nb + 1)), P2_WORDS);
     W.xbuf.ensure(*W.tracker, (size_t)need, "panel_exchange");
     hipLaunchKernelGGL(k_zero_u64, dim3(cdiv(need, 256)), dim3(256), 0, W.stream, (unsigned long long*)W.xbuf.p, (int)need);
     LAUNCH_CK();
   }
   int rmin = 64;
   if (const char* e = getenv("MPSOLVE_PANEL_RMIN")) rmin = std::max(64, std::min(256, atoi(e)));
+  const bool allow_p2 = !(getenv("MPSOLVE_PANEL") && atoi(getenv("MPSOLVE_PANEL")) == 1);
   int panels = 0;
   for (int64_t k = 0; k < n; k += nb) {
     const int kb = (int)std::min<int64_t>(nb, n - k);
     const int m = (int)(n - k);
+    const int64_t G2 = cdiv(m, P2_R);
+    if (allow_p2 && n < 65536 && G2 <= std::min(P2_MAXG, W.num_cus) && panel2_lds_bytes(kb) <= (size_t)lds_limit_bytes()) {
+      hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, W.stream, W.affect.p, 1);
+      Panel2Args pa;
+      pa.A = a.A; pa.lda = a.lda; pa.k = k; pa.kb = kb; pa.m = m; pa.G = (int)G2;
+      pa.xbuf = W.xbuf.p;
+      pa.ipiv = a.ipiv; pa.dinfo = W.dinfo.p; pa.plan = W.affect.p;
+      void* args[] = {&pa};
+      hipEvent_t e0 = W.prof_begin();
+      HIP_CK(hipLaunchCooperativeKernel((const void*)k_panel2, dim3((unsigned)G2), dim3(256), args,
+                                        (unsigned)panel2_lds_bytes(kb), W.stream));
+      W.prof_end(e0, "panel", 8.0 * (double)m * kb, (double)m * kb * kb);
+    } else {
     int R = 0;
     for (int cand : {64, 128, 256}) {
       if (cand < rmin) continue;
@@ -738,6 +1217,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
     hipEvent_t e0 = W.prof_begin();
     HIP_CK(hipLaunchCooperativeKernel((const void*)k_panel, dim3(G), dim3(256), args, (unsigned)lds, W.stream));
     W.prof_end(e0, "panel", 8.0 * (double)m * kb, (double)m * kb * kb);
+    }
     const int64_t ncols = n - kb + 1;
     hipEvent_t e1 = W.prof_begin();
     hipLaunchKernelGGL(k_laswp, dim3((unsigned)std::min<int64_t>(ncols, 4096)), dim3(256), 0, W.stream,

@@ -244,6 +244,7 @@ static void perm_from_ipiv32(const std::vector<int>& ip, std::vector<int32_t>& p
 static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, int nb, int scale_mode, Factors& F) {
   Workspace& W = C->W;
   const int64_t n = P.n; const size_t nn = (size_t)n * n;
+  W.factor_gen++;   // cached getrs block inverses of any earlier factors are stale from here on
   F.n = n; F.kind = kind; F.scale_mode = scale_mode; F.nb = nb; F.valid = false; F.growth = -1;
   F.conv_overflow = F.conv_subnormal = 0; F.info = 0; F.panels = 0;
   F.prec = (kind == "cs64") ? 64 : 32;
@@ -304,7 +305,7 @@ static void apply_inverse(mp_ctx* C, const Factors& F, const double* r, double* 
     RB_CK(rocsolver_dgetrs(C->rb, rocblas_operation_none, (rocblas_int)n, (rocblas_int)nrhs, const_cast<double*>(F.LU64.p),
                            (rocblas_int)n, F.ipiv32.p, d, (rocblas_int)n));
   } else {
-    getrs(n, nrhs, F.LU32.p, n, F.perm.p, r, n, d, n, F.gscale, W);
+    getrs(n, nrhs, F.LU32.p, n, F.perm.p, r, n, d, n, F.gscale, W, true);
   }
   if (trs) (*trs)++;
 }

@@ -43,7 +43,7 @@ constexpr long SPIN_MAX = 1L << 22;
 __device__ __forceinline__ void put_word(uint64_t* p, uint32_t tag, uint32_t payload) {
   __hip_atomic_store(p, ((uint64_t)tag << 32) | (uint64_t)payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ uint32_t get_word(const uint64_t* p, uint32_t tag, volatile int* s_abort) {
+__device__ __forceinline__ uint32_t get_word(const uint64_t* p, uint32_t tag, volatile lds_i32* s_abort) {
   for (long it = 0; it < SPIN_MAX; it++) {
     const uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if ((uint32_t)(w >> 32) == tag) return (uint32_t)w;
@@ -76,7 +76,7 @@ struct PanelArgs {
 // (negative = no candidate).  Row / identity words: [63:32] 32-bit tag, [31:0] payload.
 // Tags are the global column index + 1: unique within one factorization; the exchange area is cleared at
 // the start of every factorization, so a stale word can never carry the tag being waited for.
-__device__ __forceinline__ uint64_t get_head(const uint64_t* p, uint32_t tag24, volatile int* s_abort) {
+__device__ __forceinline__ uint64_t get_head(const uint64_t* p, uint32_t tag24, volatile lds_i32* s_abort) {
   for (long it = 0; it < SPIN_MAX; it++) {
     const uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if ((uint32_t)(w >> 40) == tag24) return w;
@@ -86,7 +86,7 @@ __device__ __forceinline__ uint64_t get_head(const uint64_t* p, uint32_t tag24, 
   *s_abort = 1;
   return 0ull;
 }
-__device__ __forceinline__ uint32_t get_word_b(const uint64_t* p, uint32_t tag, volatile int* s_abort) {
+__device__ __forceinline__ uint32_t get_word_b(const uint64_t* p, uint32_t tag, volatile lds_i32* s_abort) {
   for (long it = 0; it < SPIN_MAX; it++) {
     const uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if ((uint32_t)(w >> 32) == tag) return (uint32_t)w;
@@ -120,7 +120,7 @@ __device__ __forceinline__ void wave_argmax(float& v, int& i) {
 
 // Poll up to four tagged words at once (p[i] == nullptr: not needed); each lane keeps all of its loads in
 // flight instead of waiting for them one after the other.
-__device__ __forceinline__ void get_words4(const uint64_t* const p[4], uint32_t tag, uint32_t v[4], volatile int* s_abort) {
+__device__ __forceinline__ void get_words4(const uint64_t* const p[4], uint32_t tag, uint32_t v[4], volatile lds_i32* s_abort) {
   bool done[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) done[i] = (p[i] == nullptr);
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs a) {
     if (poller) {
       float gv = -1.0f; int gi = -1;
       if (pt < G) {
-        const uint64_t w = get_head(heads + pt, tag24, &s_abort);
+        const uint64_t w = get_head(heads + pt, tag24, LDS_FLAG(&s_abort));
         const float hv = __uint_as_float((uint32_t)w);
         if (hv >= 0.0f) {
           gi = pt * R + (int)((w >> 32) & 0xFFull);
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs a) {
       const int c0 = pt, c1 = pt + 128;    // kb <= 256
       if (c0 < kb) { pw[0] = rowsl + (int64_t)gp * SW + c0; if (need_diag) pw[2] = diag + c0; }
       if (c1 < kb) { pw[1] = rowsl + (int64_t)gp * SW + c1; if (need_diag) pw[3] = diag + c1; }
-      get_words4(pw, tag, vw, &s_abort);
+      get_words4(pw, tag, vw, LDS_FLAG(&s_abort));
       if (c0 < kb) {
         prow[c0] = __uint_as_float(vw[0]);
         if (p != j && gp == g) lds[c0 * R + (p - r0)] = own_j ? lds[c0 * R + (j - r0)] : __uint_as_float(vw[2]);
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs a) {
         uint32_t vi2[4] = {0u, 0u, 0u, 0u};
         if (own_j && gp != g) pi[0] = rowsl + (int64_t)gp * SW + kb;
         if (gp == g && !own_j) pi[1] = diag + kb;
-        get_words4(pi, tag, vi2, &s_abort);
+        get_words4(pi, tag, vi2, LDS_FLAG(&s_abort));
         if (own_j) s_prid = (gp == g) ? rid[p - r0] : (int)vi2[0];
         if (gp == g) s_jrid = own_j ? rid[j - r0] : (int)vi2[1];
       }
@@ -406,7 +406,7 @@ __device__ __forceinline__ uint64_t head_word(uint32_t tag, int pos, uint32_t vb
 // by forward substitution, right-looking so the 31 updates per step are independent (each u_q still
 // receives its updates in increasing d).  LsT[d][q] = l_{q,d}; steps whose pivot was zero are skipped.
 __device__ __forceinline__ void p2_u_column(const uint64_t* xs, uint32_t stag, const float* LsT, int zmask,
-                                            volatile int* s_abort, float (&u)[32]) {
+                                            volatile lds_i32* s_abort, float (&u)[32]) {
 #pragma unroll
   for (int q16 = 0; q16 < 32; q16 += 16) {
     uint32_t need = 0xFFFFu;
@@ -477,10 +477,10 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
   int* pstep_bg = pstep + P2_R;                           // [128] copy for the background update
   int* spos = pstep_bg + P2_R;                            // [128]
   int* misc = spos + P2_R;                                // [16]
-  volatile int* s_abort = misc + 0;
+  volatile lds_i32* s_abort = LDS_FLAG(misc + 0);
   int* s_p = misc + 1; int* s_pw = misc + 2; int* s_info = misc + 3; int* s_zmask = misc + 4;
   float* s_rinv = (float*)(misc + 5);
-  volatile int* s_step = misc + 6;
+  volatile lds_i32* s_step = LDS_FLAG(misc + 6);
 
   const int t = threadIdx.x, g = blockIdx.x, lane = t & 63, wave = t >> 6;
   const int r0 = g * P2_R;
@@ -1155,6 +1155,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   MP_REQUIRE(a.nb >= 1 && a.nb <= 256, "getrf: nb must be in [1, 256]");
   MP_REQUIRE(a.fmt >= FMT_FP16 && a.fmt <= FMT_FP16X3, "getrf: unknown trailing format");
   const int nb = a.nb;
+  W.factor_gen++;
   W.dinfo.ensure(*W.tracker, 4, "getrf_info");
   W.cnt.ensure(*W.tracker, 2, "getrf_counts");
   W.affect.ensure(*W.tracker, 1 + 4 * 256 + 8, "swap_plan");

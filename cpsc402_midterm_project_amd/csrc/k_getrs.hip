@@ -1,178 +1,298 @@
 // k_getrs.hip — d = scale * U^{-1} L^{-1} P r with FP32 LU factors, FP32 arithmetic, FP64 input and output.
 //
 // Replaces the reference's factor application apply_inverse (src/mp_core.cuh:182-190: FP64->FP32 cast,
-// cusolverDnXgetrs on FP32 factors, cast back, power-of-two unscaling) with two persistent HIP kernels.
-// The permutation gather, the FP64->FP32 cast, the scaling and the FP32->FP64 cast are fused into them.
+// cusolverDnXgetrs on FP32 factors, cast back, power-of-two unscaling) with HIP kernels of its own.  The
+// permutation gather, the FP64->FP32 cast, the scaling and the FP32->FP64 cast are fused into them.
 //
-// Layout: one workgroup per 64-row block of the triangle.  Workgroup i streams its block row of L (or U),
-// 64 rows x 16 columns per wave per block, and waits for the solution block j it needs by polling tagged
-// 64-bit words (FP32 value + launch tag), so there is no flag/fence round trip.  The off-diagonal block is
-// loaded before the wait, so only the y_j poll, a 16-FMA update, one LDS reduction and the 64-step
-// diagonal substitution sit on the dependency chain between consecutive blocks.  Lower-numbered
-// workgroups never wait for higher-numbered ones, so in-order dispatch guarantees forward progress.
+// Blocked by 128 rows.  The 128 x 128 diagonal blocks of L (unit lower) and U are inverted once per
+// factorization (k_trinv, one workgroup per block, FP32 substitution on the columns of the identity), so the
+// diagonal step of the solve is a parallel matrix-vector product.  k_trsv2: one workgroup of 512 threads
+// per block row.  It streams the off-diagonal blocks of its block row in the order their solution blocks
+// are produced, with the next block's loads in flight while the current one is applied; the solution
+// blocks are exchanged as tagged 64-bit words (FP32 value + launch tag), so a reader needs no flag and no
+// fence.  Only the immediately preceding block (preloaded into registers) and the inverse product sit on
+// the dependency chain between consecutive block rows.  Lower-numbered workgroups never wait for
+// higher-numbered ones, so in-order dispatch guarantees progress.
 #include "mp_internal.hpp"
 #include "mp_kernels.hpp"
 
 namespace mpk {
 
 constexpr long GS_SPIN_MAX = 1L << 22;
+// Phase markers: empty in the library build; a probe build may define them to record timestamps.
+#ifndef GS_MARK
+#define GS_MARK(phase)
+#endif
+#ifndef GS_LOAD
+#define GS_LOAD(p) (*(p))
+#endif
+constexpr int GB = 128;            // rows per block
 
 __device__ __forceinline__ void gs_put(uint64_t* p, uint32_t tag, float v) {
   __hip_atomic_store(p, ((uint64_t)tag << 32) | (uint64_t)__float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// urgent = the waiting block is the immediate successor (it is on the dependency chain); the others back off
-// so that the words being produced are not hammered by every waiting workgroup at once.
-__device__ __forceinline__ float gs_get(const uint64_t* p, uint32_t tag, volatile int* s_abort, bool urgent) {
+__device__ __forceinline__ uint64_t gs_load(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// dist = how many chain steps before this workgroup needs the block (0: it is on the dependency chain).
+// Every workgroup waiting at the frontier polls the same few cache lines, so the poll interval grows with
+// dist: only the next one or two block rows poll eagerly.
+__device__ __forceinline__ void gs_sleep(int dist) {
+  if (dist <= 0) __builtin_amdgcn_s_sleep(1);
+  else if (dist == 1) __builtin_amdgcn_s_sleep(4);
+  else if (dist < 4) __builtin_amdgcn_s_sleep(32);
+  else __builtin_amdgcn_s_sleep(127);
+}
+// The word loaded ahead (`first`) is checked before the polling loop: the loop re-loads, and at its head the
+// compiler can only wait for every outstanding load, which would also wait for the prefetch in flight.
+__device__ __forceinline__ float gs_wait(const uint64_t* p, uint64_t first, uint32_t tag, volatile lds_i32* s_abort, int dist) {
+  if ((uint32_t)(first >> 32) == tag) return __uint_as_float((uint32_t)first);
   for (long it = 0; it < GS_SPIN_MAX; it++) {
-    const uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((uint32_t)(w >> 32) == tag) return __uint_as_float((uint32_t)w);
     if (*s_abort) return 0.0f;
-    if (urgent) __builtin_amdgcn_s_sleep(1); else __builtin_amdgcn_s_sleep(16);
+    gs_sleep(dist);
+    const uint64_t w = gs_load(p);
+    if ((uint32_t)(w >> 32) == tag) return __uint_as_float((uint32_t)w);
   }
   *s_abort = 1;
   return 0.0f;
 }
 
-// LOWER = true: unit-lower forward substitution on P r (input FP64 via perm gather), output Y (FP32).
-// LOWER = false: upper backward substitution on Y (FP32), output D = scale * x (FP64).
+// Inverse of the 128 x 128 diagonal block b of L (unit lower, LOWER) or U (upper).  Output column-major per
+// block: Tinv[b * 128 * 128 + col * 128 + row]; rows / columns beyond n are the identity.
+// Thread c computes column c: lower  x_q = -sum_{p=c}^{q-1} L_qp x_p (x_c = 1),
+//                             upper  x_q = (delta_qc - sum_{p=q+1}^{c} U_qp x_p) / U_qq, q = c .. 0,
+// each sum accumulated in increasing p.
+template <bool LOWER>
+__global__ __launch_bounds__(128) void k_trinv(const float* __restrict__ LU, int64_t lda, int64_t n, float* __restrict__ Tinv) {
+  extern __shared__ float sm[];
+  float* T = sm;                      // [128][129] row-major T[q * 129 + p]
+  float* X = sm + GB * (GB + 1);      // [128][128] X[p * 128 + c]
+  const int b = blockIdx.x, c = threadIdx.x;
+  const int64_t o = (int64_t)b * GB;
+  const int nb = (int)min((int64_t)GB, n - o);
+  for (int idx = c; idx < GB * GB; idx += GB) {
+    const int p = idx >> 7, q = idx & (GB - 1);    // column p, row q: coalesced over q
+    float v = (q == p) ? 1.0f : 0.0f;
+    if (q < nb && p < nb) v = LU[(o + p) * lda + o + q];
+    T[q * (GB + 1) + p] = v;
+  }
+  __syncthreads();
+  if (LOWER) {
+    X[c * GB + c] = 1.0f;
+    for (int q = 0; q < GB; q++) if (q < c) X[q * GB + c] = 0.0f;
+    for (int q = 1; q < GB; q++) {
+      float s = 0.0f;
+      for (int p = 0; p < q; p++) {
+        const float ns = fmaf(T[q * (GB + 1) + p], X[p * GB + c], s);
+        s = (p >= c) ? ns : s;
+      }
+      if (q > c) X[q * GB + c] = -s;
+    }
+  } else {
+    for (int q = GB - 1; q >= 0; q--) {
+      float s = (q == c) ? 1.0f : 0.0f;
+      for (int p = q + 1; p < GB; p++) {
+        const float ns = fmaf(-T[q * (GB + 1) + p], X[p * GB + c], s);
+        s = (p <= c) ? ns : s;
+      }
+      X[q * GB + c] = (q <= c) ? s / T[q * (GB + 1) + q] : 0.0f;
+    }
+  }
+  __syncthreads();
+  for (int idx = c; idx < GB * GB; idx += GB) {
+    const int col = idx >> 7, row = idx & (GB - 1);
+    Tinv[(size_t)b * GB * GB + idx] = X[row * GB + col];
+  }
+}
+
+// LOWER: y = Linv_i (P r)_i - sum_{j<i} L_ij y_j, output Y (FP32).   UPPER: x_i = Uinv_i (y_i - sum_{j>i} U_ij x_j),
+// output D = scale * x (FP64).  Thread t: row r = t % 128 of the block, quarter qq = t / 128 of the columns.
 template <int NR, bool LOWER>
-__global__ __launch_bounds__(256) void k_trsv(const float* __restrict__ LU, int64_t lda, int64_t n,
-                                              const int32_t* __restrict__ perm,
-                                              const double* __restrict__ Rin, int64_t ldr,
-                                              float* __restrict__ Y, int64_t ldy,
-                                              double* __restrict__ D, int64_t ldd, double scale,
-                                              uint64_t* __restrict__ ybuf, uint32_t tag, int32_t* __restrict__ err) {
-  __shared__ float ys[2][NR][64];
-  __shared__ float part[4][NR][64];
+__global__ __launch_bounds__(512) void k_trsv2(const float* __restrict__ LU, int64_t lda, int64_t n,
+                                               const float* __restrict__ Tinv, const int32_t* __restrict__ perm,
+                                               const double* __restrict__ Rin, int64_t ldr,
+                                               float* __restrict__ Y, int64_t ldy,
+                                               double* __restrict__ D, int64_t ldd, double scale,
+                                               uint64_t* __restrict__ ybuf, uint32_t tag, int32_t* __restrict__ err) {
+  __shared__ float ys[3][NR][GB];
+  __shared__ float part[4][NR][GB];
   __shared__ int s_abort;
-  const int nbk = (int)((n + 63) / 64);
-  const int t = threadIdx.x, rr = t & 63, cq = t >> 6;
+  const int nbk = (int)((n + GB - 1) / GB);
+  const int t = threadIdx.x, r = t & (GB - 1), qq = t >> 7;
   const int bi = LOWER ? (int)blockIdx.x : (nbk - 1 - (int)blockIdx.x);
-  const int64_t row = (int64_t)bi * 64 + rr;
+  const int64_t row = (int64_t)bi * GB + r;
   const bool rvalid = row < n;
   if (t == 0) s_abort = 0;
+  const int ndep = LOWER ? bi : (nbk - 1 - bi);     // blocks this row depends on
+  const int nbulk = ndep > 0 ? ndep - 1 : 0;         // all but the immediately preceding one
+  auto blk = [&](int s) { return LOWER ? s : (nbk - 1 - s); };   // s-th dependency in production order
+  // Unconditional loads from clamped addresses, consumed only when the block is applied (a guarded or
+  // masked load is waited for on the spot, which would serialise the 32 loads).  Clamped values are
+  // harmless: a column beyond n meets a solution entry published as exactly 0, and a row beyond n is never
+  // published (its inverse-block row and column are the identity padding).
+  const int64_t rowc = rvalid ? row : n - 1;
+  auto ldL = [&](int bj, float (&v)[32]) {
+    const float* base = LU + rowc;
+#pragma unroll
+    for (int k = 0; k < 32; k++) {
+      const int64_t col = (int64_t)bj * GB + qq * 32 + k;
+      v[k] = GS_LOAD(base + (col < n ? col : n - 1) * lda);
+    }
+  };
+  // preloads: inverse block row, the chain block, the right-hand side
+  float Tv[32], Lt[32], rhs[NR];
+#pragma unroll
+  for (int k = 0; k < 32; k++) Tv[k] = Tinv[(size_t)bi * GB * GB + (size_t)(qq * 32 + k) * GB + r];
+  if (ndep > 0) ldL(blk(ndep - 1), Lt);
+  else {
+#pragma unroll
+    for (int k = 0; k < 32; k++) Lt[k] = 0.0f;
+  }
+#pragma unroll
+  for (int c = 0; c < NR; c++) {
+    rhs[c] = 0.0f;
+    if (qq == 0 && rvalid) rhs[c] = LOWER ? (float)Rin[(int64_t)c * ldr + perm[row]] : Y[(int64_t)c * ldy + row];
+  }
   float acc[NR];
 #pragma unroll
   for (int c = 0; c < NR; c++) acc[c] = 0.0f;
   __syncthreads();
-  const int nsteps = LOWER ? bi : (nbk - 1 - bi);
-  // diagonal block (wave 0: lane rr holds row rr) and the right-hand side are independent of the other
-  // blocks: load them before waiting, so the dependency chain sees no HBM latency for them
-  float Dg[64];
-  float rhs0[NR];
-  if (t < 64) {
+
+  // bulk: dependencies 0 .. nbulk-1, loads of step s+1 in flight while step s is applied.  Two register
+  // buffers alternate (no copies, which would wait for the loads just issued) and every prefetch is issued
+  // unconditionally (a clamped step past the end reloads block 0 and is ignored), so the number of loads
+  // behind each consumed value is fixed and the waits stay partial.
+  float La[32], Lb[32];
+  uint64_t ya[NR], yb[NR];
+  auto ldY = [&](int s2, uint64_t (&y)[NR]) {
+    const int bj = blk(s2 < nbulk ? s2 : 0);
 #pragma unroll
-    for (int q = 0; q < 64; q++) {
-      const int64_t col = (int64_t)bi * 64 + q;
-      Dg[q] = (rvalid && col < n) ? LU[col * lda + row] : (q == rr ? 1.0f : 0.0f);
-    }
+    for (int c = 0; c < NR; c++) y[c] = gs_load(ybuf + ((int64_t)bj * NR + c) * GB + (t & (GB - 1)));
+  };
+  auto step = [&](int s2, const float (&L)[32], const uint64_t (&y)[NR]) {
+    const int bj = blk(s2);
+    if (t < GB) {
 #pragma unroll
-    for (int c = 0; c < NR; c++) {
-      if (LOWER) rhs0[c] = rvalid ? (float)Rin[(int64_t)c * ldr + perm[row]] : 0.0f;
-      else rhs0[c] = 0.0f;
-    }
-  }
-  for (int s = 0; s < nsteps; s++) {
-    const int bj = LOWER ? s : (nbk - 1 - s);
-    // off-diagonal block values of my row, 16 columns of this wave's quarter (loaded before the wait)
-    float Lv[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const int64_t col = (int64_t)bj * 64 + cq * 16 + q;
-      Lv[q] = (rvalid && col < n) ? LU[col * lda + row] : 0.0f;
-    }
-    float (*yb)[64] = ys[s & 1];
-    for (int qq = t; qq < 64 * NR; qq += 256) {
-      const int c = qq >> 6, col = qq & 63;
-      const bool cv = (int64_t)bj * 64 + col < n;
-      yb[c][col] = cv ? gs_get(ybuf + ((int64_t)bj * NR + c) * 64 + col, tag, &s_abort, s == nsteps - 1) : 0.0f;
+      for (int c = 0; c < NR; c++) ys[s2 % 3][c][t] = gs_wait(ybuf + ((int64_t)bj * NR + c) * GB + t, y[c], tag, LDS_FLAG(&s_abort), ndep - 1 - s2);
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 16; q++)
+    for (int k = 0; k < 32; k++)
 #pragma unroll
-      for (int c = 0; c < NR; c++) acc[c] = fmaf(Lv[q], yb[c][cq * 16 + q], acc[c]);
+      for (int c = 0; c < NR; c++) acc[c] = fmaf(L[k], ys[s2 % 3][c][qq * 32 + k], acc[c]);
+  };
+  if (nbulk > 0) { ldL(blk(0), La); ldY(0, ya); }
+  for (int s = 0; s < nbulk; s += 2) {
+    ldL(blk(s + 1 < nbulk ? s + 1 : 0), Lb); ldY(s + 1, yb);
+    step(s, La, ya);
+    if (s + 1 >= nbulk) break;
+    ldL(blk(s + 2 < nbulk ? s + 2 : 0), La); ldY(s + 2, ya);
+    step(s + 1, Lb, yb);
+  }
+  GS_MARK(0);
+  // the chain block
+  if (ndep > 0) {
+    const int bj = blk(ndep - 1);
+    if (t < GB) {
+#pragma unroll
+      for (int c = 0; c < NR; c++) {
+        const uint64_t* p = ybuf + ((int64_t)bj * NR + c) * GB + t;
+        ys[nbulk % 3][c][t] = gs_wait(p, gs_load(p), tag, LDS_FLAG(&s_abort), 0);
+      }
+    }
+    __syncthreads();
+    GS_MARK(1);
+#pragma unroll
+    for (int k = 0; k < 32; k++)
+#pragma unroll
+      for (int c = 0; c < NR; c++) acc[c] = fmaf(Lt[k], ys[nbulk % 3][c][qq * 32 + k], acc[c]);
   }
 #pragma unroll
-  for (int c = 0; c < NR; c++) part[cq][c][rr] = acc[c];
+  for (int c = 0; c < NR; c++) part[qq][c][r] = acc[c];
   __syncthreads();
-  if (t < 64) {
-    float b[NR];
+  // b = rhs - sum, staged in ys[0]
+  if (qq == 0) {
+#pragma unroll
+    for (int c = 0; c < NR; c++)
+      ys[(nbulk + 1) % 3][c][r] = rhs[c] - ((part[0][c][r] + part[1][c][r]) + (part[2][c][r] + part[3][c][r]));
+  }
+  __syncthreads();
+  float px[NR];
+#pragma unroll
+  for (int c = 0; c < NR; c++) px[c] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 32; k++)
+#pragma unroll
+    for (int c = 0; c < NR; c++) px[c] = fmaf(Tv[k], ys[(nbulk + 1) % 3][c][qq * 32 + k], px[c]);
+  // part was last read before the previous barrier
+#pragma unroll
+  for (int c = 0; c < NR; c++) part[qq][c][r] = px[c];
+  __syncthreads();
+  if (qq == 0) {
 #pragma unroll
     for (int c = 0; c < NR; c++) {
-      float rhs;
-      if (LOWER) rhs = rhs0[c];
-      else rhs = rvalid ? Y[(int64_t)c * ldy + row] : 0.0f;
-      const float sum = ((part[0][c][rr] + part[1][c][rr]) + part[2][c][rr]) + part[3][c][rr];
-      b[c] = rhs - sum;
-    }
-    if (LOWER) {
-#pragma unroll
-      for (int q = 0; q < 64; q++) {
-#pragma unroll
-        for (int c = 0; c < NR; c++) {
-          const float yq = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b[c]), q));
-          if (rr > q) b[c] = fmaf(-Dg[q], yq, b[c]);
-        }
-      }
-    } else {
-#pragma unroll
-      for (int q = 63; q >= 0; q--) {
-#pragma unroll
-        for (int c = 0; c < NR; c++) {
-          if (rr == q) b[c] = b[c] / Dg[q];
-          const float xq = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b[c]), q));
-          if (rr < q) b[c] = fmaf(-Dg[q], xq, b[c]);
-        }
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < NR; c++) {
-      gs_put(ybuf + ((int64_t)bi * NR + c) * 64 + rr, tag, b[c]);
+      const float x = (part[0][c][r] + part[1][c][r]) + (part[2][c][r] + part[3][c][r]);
+      gs_put(ybuf + ((int64_t)bi * NR + c) * GB + r, tag, rvalid ? x : 0.0f);
       if (rvalid) {
-        if (LOWER) Y[(int64_t)c * ldy + row] = b[c];
-        else D[(int64_t)c * ldd + row] = scale * (double)b[c];
+        if (LOWER) Y[(int64_t)c * ldy + row] = x;
+        else D[(int64_t)c * ldd + row] = scale * (double)x;
       }
     }
   }
+  GS_MARK(2);
   if (t == 0 && s_abort) err[0] = 1;
 }
-
-__global__ void k_zero1(int32_t* p) { p[0] = 0; }
 
 template <int NR>
 static void trsv_pair(int64_t n, const float* LU, int64_t lda, const int32_t* perm, const double* R, int64_t ldr,
                       double* D, int64_t ldd, double scale, Workspace& W, int32_t* err) {
-  const int nbk = (int)cdiv(n, 64);
+  const int nbk = (int)cdiv(n, GB);
   const uint32_t tl = W.take_tags(1), tu = W.take_tags(1);
+  const float* Ti = W.tinv.p;
+  const float* Tu = W.tinv.p + (size_t)nbk * GB * GB;
   hipEvent_t ev = W.prof_begin();
-  hipLaunchKernelGGL((k_trsv<NR, true>), dim3(nbk), dim3(256), 0, W.stream, LU, lda, n, perm, R, ldr,
+  hipLaunchKernelGGL((k_trsv2<NR, true>), dim3(nbk), dim3(512), 0, W.stream, LU, lda, n, Ti, perm, R, ldr,
                      W.rhs32.p, n, (double*)nullptr, (int64_t)0, 1.0, W.ybuf.p, tl, err);
   LAUNCH_CK();
-  hipLaunchKernelGGL((k_trsv<NR, false>), dim3(nbk), dim3(256), 0, W.stream, LU, lda, n, perm, (const double*)nullptr, (int64_t)0,
+  hipLaunchKernelGGL((k_trsv2<NR, false>), dim3(nbk), dim3(512), 0, W.stream, LU, lda, n, Tu, perm, (const double*)nullptr, (int64_t)0,
                      W.rhs32.p, n, D, ldd, scale, W.ybuf.p, tu, err);
   LAUNCH_CK();
   W.prof_end(ev, "getrs", 4.0 * (double)n * n + 24.0 * n * NR, 2.0 * (double)n * n * NR);
 }
 
 void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* perm,
-           const double* R, int64_t ldr, double* D, int64_t ldd, double scale, Workspace& W) {
+           const double* R, int64_t ldr, double* D, int64_t ldd, double scale, Workspace& W, bool reuse_inverse) {
   MP_REQUIRE(n > 0 && nrhs > 0, "getrs: n, nrhs > 0");
   MP_REQUIRE(lda >= n && ldr >= n && ldd >= n, "getrs: leading dimensions >= n");
-  const int64_t nbk = cdiv(n, 64);
-  if (W.ybuf.ensure(*W.tracker, (size_t)nbk * 8 * 64, "getrs_exchange"))
+  const int64_t nbk = cdiv(n, GB);
+  if (W.ybuf.ensure(*W.tracker, (size_t)nbk * 2 * GB, "getrs_exchange"))
     HIP_CK(hipMemsetAsync(W.ybuf.p, 0, W.ybuf.n * 8, W.stream));
-  W.rhs32.ensure(*W.tracker, (size_t)n * 8, "getrs_y32");
+  W.rhs32.ensure(*W.tracker, (size_t)n * 4, "getrs_y32");
   W.dinfo.ensure(*W.tracker, 4, "getrf_info");
+  W.tinv.ensure(*W.tracker, (size_t)2 * nbk * GB * GB, "getrs_block_inverses");
+  if (!reuse_inverse || W.tinv_lu != LU || W.tinv_n != n || W.tinv_lda != lda || W.tinv_gen != W.factor_gen) {
+    static bool attr = false;
+    const size_t lds = (size_t)(GB * (GB + 1) + GB * GB) * 4;
+    if (!attr) {
+      HIP_CK(hipFuncSetAttribute((const void*)k_trinv<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      HIP_CK(hipFuncSetAttribute((const void*)k_trinv<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      attr = true;
+    }
+    hipEvent_t ev = W.prof_begin();
+    hipLaunchKernelGGL(k_trinv<true>, dim3((unsigned)nbk), dim3(GB), lds, W.stream, LU, lda, n, W.tinv.p);
+    LAUNCH_CK();
+    hipLaunchKernelGGL(k_trinv<false>, dim3((unsigned)nbk), dim3(GB), lds, W.stream, LU, lda, n, W.tinv.p + (size_t)nbk * GB * GB);
+    LAUNCH_CK();
+    W.prof_end(ev, "getrs_block_inverse", 8.0 * (double)nbk * GB * GB, 2.0 * (double)nbk * GB * GB * GB / 3.0);
+    W.tinv_lu = LU; W.tinv_n = n; W.tinv_lda = lda; W.tinv_gen = W.factor_gen;
+  }
   int c = 0;
   while (c < nrhs) {
     const int left = nrhs - c;
     const double* Rc = R + (int64_t)c * ldr;
     double* Dc = D + (int64_t)c * ldd;
     int32_t* err = W.dinfo.p + 2;
-    if (left >= 8) { trsv_pair<8>(n, LU, lda, perm, Rc, ldr, Dc, ldd, scale, W, err); c += 8; }
-    else if (left >= 4) { trsv_pair<4>(n, LU, lda, perm, Rc, ldr, Dc, ldd, scale, W, err); c += 4; }
-    else if (left >= 2) { trsv_pair<2>(n, LU, lda, perm, Rc, ldr, Dc, ldd, scale, W, err); c += 2; }
+    if (left >= 2) { trsv_pair<2>(n, LU, lda, perm, Rc, ldr, Dc, ldd, scale, W, err); c += 2; }
     else { trsv_pair<1>(n, LU, lda, perm, Rc, ldr, Dc, ldd, scale, W, err); c += 1; }
   }
 }

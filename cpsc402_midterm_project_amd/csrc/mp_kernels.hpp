@@ -48,6 +48,9 @@ struct Workspace {
   // getrs
   DBuf<uint64_t> ybuf;   // tagged partial-solution words of the triangular solves
   DBuf<float> rhs32;
+  DBuf<float> tinv;      // inverses of the 128 x 128 diagonal blocks of L and U
+  const float* tinv_lu = nullptr; int64_t tinv_n = 0, tinv_lda = 0; uint64_t tinv_gen = 0;
+  uint64_t factor_gen = 1;   // incremented by every factorization written through getrf
   uint32_t tag = 1;      // next free exchange tag (never 0)
   // bracket a launch with profiling events (no-op when profiling is off)
   hipEvent_t prof_begin() { if (!prof.on) return nullptr; hipEvent_t e = prof.get(); HIP_CK(hipEventRecord(e, stream)); return e; }
@@ -74,6 +77,12 @@ void absmax(const void* A, bool a32, int64_t n, int64_t lda, double* out_dev, Wo
 
 // FMT_FP16X3: operands split into FP16 hi + lo parts (same tile scale), update = hi*hi + hi*lo + lo*hi on
 // FP16 MFMA with FP32 accumulation.
+// LDS flag words that kernels poll: the explicit address space keeps volatile accesses on the LDS path
+// (ds_read / ds_write).  Through a generic pointer they become FLAT operations, which count against the
+// vector-memory counter and make every flag read wait for the wave's outstanding global loads and stores.
+typedef __attribute__((address_space(3))) int lds_i32;
+#define LDS_FLAG(p) ((volatile lds_i32*)(p))
+
 enum TrailFmt { FMT_FP16 = 0, FMT_BF16 = 1, FMT_FP8 = 2, FMT_FP32 = 3, FMT_FP16X3 = 4 };
 
 struct GetrfArgs {
@@ -85,8 +94,9 @@ struct GetrfInfo { int info = 0; int timeout = 0; unsigned long long overflow = 
 void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out);
 
 // d = (LU)^{-1} P r  with FP32 factors and FP32 arithmetic, FP64 input/output; d is scaled by `scale`.
+// reuse_inverse: the diagonal-block inverses computed for the same LU buffer since the last getrf are reused.
 void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* perm,
-           const double* R, int64_t ldr, double* D, int64_t ldd, double scale, Workspace& W);
+           const double* R, int64_t ldr, double* D, int64_t ldd, double scale, Workspace& W, bool reuse_inverse = false);
 
 struct GenSpec {
   int64_t n = 1024; int nrhs = 1; std::string family = "randsvd", spectrum = "geometric", rhs = "xtrue_uniform";

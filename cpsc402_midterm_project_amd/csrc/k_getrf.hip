@@ -26,6 +26,11 @@ namespace mpk {
 
 constexpr int MAX_KB = 512;
 
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef float f32x8_t __attribute__((ext_vector_type(8)));
+
 // Phase markers of the panel kernel: empty in the library build; a probe build may define them to record
 // per-step timestamps.
 #ifndef PANEL_MARK
@@ -900,6 +905,135 @@ __global__ __launch_bounds__(256) void k_trsm(float* __restrict__ A, int64_t lda
   }
 }
 
+// k_trsm_mfma: the same blocked algorithm as k_trsm (X_b <- T_b^{-1} X_b, then X_r -= L_rb X_b for the row
+// blocks below) with the 32 x 32 products on v_mfma_f32_32x32x2_f32.  A workgroup of four waves covers 128
+// columns of U12, one wave per 32-column strip; the strip's kb x 32 values stay in accumulator layout
+// (register a of lane l holds row 8(a/4) + 4(l/32) + a%4 of column l%32 of each 32-row block).  That
+// register a is exactly the B operand of k-step a when the k index of step a for lane half h is taken as
+// R(h, a) = 8(a/4) + 4h + a%4, so X_b feeds the products straight from registers; the A operand is read in
+// the same k order from LDS, where the workgroup first stages T_b^{-1} and -L_rb for every block pair
+// (36 blocks of 32 x 33 floats).  Loads and stores of the strips go through LDS so they stay coalesced.
+// The matrix unit accumulates the products, so the rounding differs from the sequential FMA order of
+// k_trsm (same algorithm, FP32).
+constexpr int TM_BLK = 32 * 33;
+static size_t trsm_mfma_lds_bytes() { return (size_t)36 * TM_BLK * 4; }
+__global__ __launch_bounds__(256) void k_trsm_mfma(float* __restrict__ A, int64_t lda, int64_t n, int64_t k, int kb,
+                                                   const float* __restrict__ Linv) {
+  extern __shared__ float sm[];
+  const int t = threadIdx.x, w = t >> 6, l = t & 63, h = l >> 5, li = l & 31;
+  const int64_t c0 = k + kb + (int64_t)blockIdx.x * 128 + w * 32;    // first column of this wave's strip
+  const int64_t ncol = max((int64_t)0, min((int64_t)32, n - c0));
+  const int nrb = (kb + 31) / 32;
+  float* tile = sm + w * TM_BLK;                                      // staging tile of this wave
+  f32x16_t X[8];
+  // strip load: lane l reads row l%32 of columns l/32 + 2q (coalesced over rows), transposed through LDS
+#pragma unroll
+  for (int rb = 0; rb < 8; rb++) {
+    if (rb < nrb) {
+      // edge tiles (partial rows or columns) take a masked path; the common path has no condition on the
+      // loaded values (a masked load gets sunk into a branch and waited for on its own)
+      if ((rb + 1) * 32 <= kb && ncol == 32) {
+        float v[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) v[q] = A[(k + rb * 32 + li) + (c0 + h + 2 * q) * lda];
+#pragma unroll
+        for (int q = 0; q < 16; q++) tile[(h + 2 * q) * 33 + li] = v[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+          const int col = h + 2 * q, row = rb * 32 + li;
+          const int64_t cc = min(c0 + col, n - 1);
+          const float v = A[(k + min(row, kb - 1)) + cc * lda];
+          tile[col * 33 + li] = (row < kb && col < ncol) ? v : 0.0f;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int a = 0; a < 16; a++) X[rb][a] = tile[li * 33 + 8 * (a >> 2) + 4 * h + (a & 3)];
+      __syncthreads();
+    } else {
+#pragma unroll
+      for (int a = 0; a < 16; a++) X[rb][a] = 0.0f;
+    }
+  }
+  // operand blocks: index rb (rb + 1) / 2 + b for b <= rb; [row][col] with stride 33.  One block row at a
+  // time: all of its loads are issued before the LDS stores that consume them (a load consumed on the spot
+  // is waited for on its own).
+#pragma unroll
+  for (int rb = 0; rb < 8; rb++) {
+    if (rb >= nrb) break;
+    float v[8][4];
+#pragma unroll
+    for (int bb = 0; bb < 8; bb++) {
+      if (bb > rb) break;
+#pragma unroll
+      for (int e4 = 0; e4 < 4; e4++) {
+        const int e = t + 256 * e4, col = e >> 5, row = e & 31;
+        if (bb == rb) {
+          v[bb][e4] = Linv[(size_t)bb * 1024 + col * 32 + row];
+        } else {
+          const int gr = rb * 32 + row, gc = bb * 32 + col;
+          v[bb][e4] = -A[(k + min(gr, kb - 1)) + (k + min(gc, kb - 1)) * lda];
+        }
+      }
+    }
+#pragma unroll
+    for (int bb = 0; bb < 8; bb++) {
+      if (bb > rb) break;
+#pragma unroll
+      for (int e4 = 0; e4 < 4; e4++) {
+        const int e = t + 256 * e4, col = e >> 5, row = e & 31;
+        sm[(rb * (rb + 1) / 2 + bb) * TM_BLK + row * 33 + col] = v[bb][e4];
+      }
+    }
+    // rows / columns of L beyond kb (partial last block) are zero; only the last block row can have them
+    if ((kb & 31) != 0 && rb == nrb - 1) {
+      __syncthreads();
+      for (int e = t; e < (rb + 1) * 1024; e += 256) {
+        const int bb = e >> 10, col = (e >> 5) & 31, row = e & 31;
+        if (bb < rb && (rb * 32 + row >= kb || bb * 32 + col >= kb)) sm[(rb * (rb + 1) / 2 + bb) * TM_BLK + row * 33 + col] = 0.0f;
+      }
+    }
+  }
+  __syncthreads();
+  auto kidx = [&](int s) { return 8 * (s >> 2) + 4 * h + (s & 3); };
+#pragma unroll
+  for (int b = 0; b < 8; b++) {
+    if (b >= nrb) break;
+    {
+      const float* M = sm + (b * (b + 1) / 2 + b) * TM_BLK + li * 33;
+      f32x16_t y;
+#pragma unroll
+      for (int a = 0; a < 16; a++) y[a] = 0.0f;
+#pragma unroll
+      for (int s2 = 0; s2 < 16; s2++) y = __builtin_amdgcn_mfma_f32_32x32x2f32(M[kidx(s2)], X[b][s2], y, 0, 0, 0);
+      X[b] = y;
+    }
+#pragma unroll
+    for (int rb = b + 1; rb < 8; rb++) {
+      if (rb >= nrb) break;
+      const float* M = sm + (rb * (rb + 1) / 2 + b) * TM_BLK + li * 33;
+#pragma unroll
+      for (int s2 = 0; s2 < 16; s2++) X[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(M[kidx(s2)], X[b][s2], X[rb], 0, 0, 0);
+    }
+  }
+  __syncthreads();
+  // strip store through the staging tile
+#pragma unroll
+  for (int rb = 0; rb < 8; rb++) {
+    if (rb >= nrb) break;
+#pragma unroll
+    for (int a = 0; a < 16; a++) tile[li * 33 + 8 * (a >> 2) + 4 * h + (a & 3)] = X[rb][a];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int col = h + 2 * q, row = rb * 32 + li;
+      if (row < kb && col < ncol) A[(k + row) + (c0 + col) * lda] = tile[col * 33 + li];
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------ operand rounding
 __device__ __forceinline__ uint16_t to_bf16_rne(float x) {
   uint32_t u = __float_as_uint(x);
@@ -997,10 +1131,6 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ S, int64
 }
 
 // ------------------------------------------------------------------ trailing update on MFMA
-typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-typedef float f32x16_t __attribute__((ext_vector_type(16)));
-typedef float f32x8_t __attribute__((ext_vector_type(8)));
 
 template <int FMT> struct Frag;
 template <> struct Frag<FMT_FP16> { half8_t v;
@@ -1052,39 +1182,44 @@ __global__ __launch_bounds__(256) void k_gemm(float* __restrict__ C, int64_t ldc
   const uint8_t* pu1 = PU + (size_t)(ct0 + 1) * nkb * kstride + (size_t)lane * FB;
   const uint8_t* pl0 = PL + (size_t)(rt0 + 0) * nkb * kstride + (size_t)lane * FB;
   const uint8_t* pl1 = PL + (size_t)(rt0 + 1) * nkb * kstride + (size_t)lane * FB;
-  // C tile values for the epilogue are loaded before the K loop so their HBM latency overlaps the MFMAs
+  // C tile values for the epilogue are loaded before the K loop so their HBM latency overlaps the MFMAs.
+  // The loads are unconditional (edge tiles read clamped addresses whose values are never stored): a
+  // guarded load becomes a branch with a full wait behind it, which would serialise the 64 loads.
   float cv[2][2][16];
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) {
-      const int row = (rt0 + j) * 32 + (lane & 31);
+      const int row = min((rt0 + j) * 32 + (lane & 31), m2 - 1);
 #pragma unroll
       for (int a = 0; a < 16; a++) {
-        const int col = (ct0 + i) * 32 + 8 * (a >> 2) + 4 * (lane >> 5) + (a & 3);
-        cv[i][j][a] = (row < m2 && col < n2) ? __builtin_nontemporal_load(C + (int64_t)col * ldc + row) : 0.0f;
+        const int col = min((ct0 + i) * 32 + 8 * (a >> 2) + 4 * (lane >> 5) + (a & 3), n2 - 1);
+        cv[i][j][a] = __builtin_nontemporal_load(C + (int64_t)col * ldc + row);
       }
     }
-  Frag<FMT> u0, u1, l0, l1;
+  constexpr bool X3 = (FMT == FMT_FP16X3);
+  const uint8_t* qu0 = X3 ? PUlo + (pu0 - PU) : nullptr;
+  const uint8_t* qu1 = X3 ? PUlo + (pu1 - PU) : nullptr;
+  const uint8_t* ql0 = X3 ? PLlo + (pl0 - PL) : nullptr;
+  const uint8_t* ql1 = X3 ? PLlo + (pl1 - PL) : nullptr;
+  Frag<FMT> u0, u1, l0, l1, v0, v1, m0, m1;   // v / m: low parts of the two-term operands
   u0.load(pu0); u1.load(pu1); l0.load(pl0); l1.load(pl1);
+  if (X3) { v0.load(qu0); v1.load(qu1); m0.load(ql0); m1.load(ql1); }
   for (int kb16 = 0; kb16 < nkb; kb16++) {
-    Frag<FMT> nu0 = u0, nu1 = u1, nl0 = l0, nl1 = l1;
-    if (kb16 + 1 < nkb) {
-      const size_t o = (size_t)(kb16 + 1) * kstride;
-      nu0.load(pu0 + o); nu1.load(pu1 + o); nl0.load(pl0 + o); nl1.load(pl1 + o);
-    }
+    // next k-block (the last step reloads its own block: the packed operands are padded, always in bounds)
+    const size_t o = (size_t)min(kb16 + 1, nkb - 1) * kstride;
+    Frag<FMT> nu0, nu1, nl0, nl1, nv0, nv1, nm0, nm1;
+    nu0.load(pu0 + o); nu1.load(pu1 + o); nl0.load(pl0 + o); nl1.load(pl1 + o);
+    if (X3) { nv0.load(qu0 + o); nv1.load(qu1 + o); nm0.load(ql0 + o); nm1.load(ql1 + o); }
     mma(acc[0][0], u0, l0); mma(acc[0][1], u0, l1);
     mma(acc[1][0], u1, l0); mma(acc[1][1], u1, l1);
-    if (FMT == FMT_FP16X3) {
+    if (X3) {
       // cross terms hi*lo + lo*hi of the two-term operands (lo*lo ~ u16^2 is dropped)
-      const size_t o = (size_t)kb16 * kstride;
-      Frag<FMT> ul0, ul1, ll0, ll1;
-      ul0.load(PUlo + (pu0 - PU) + o); ul1.load(PUlo + (pu1 - PU) + o);
-      ll0.load(PLlo + (pl0 - PL) + o); ll1.load(PLlo + (pl1 - PL) + o);
-      mma(acc[0][0], u0, ll0); mma(acc[0][1], u0, ll1);
-      mma(acc[1][0], u1, ll0); mma(acc[1][1], u1, ll1);
-      mma(acc[0][0], ul0, l0); mma(acc[0][1], ul0, l1);
-      mma(acc[1][0], ul1, l0); mma(acc[1][1], ul1, l1);
+      mma(acc[0][0], u0, m0); mma(acc[0][1], u0, m1);
+      mma(acc[1][0], u1, m0); mma(acc[1][1], u1, m1);
+      mma(acc[0][0], v0, l0); mma(acc[0][1], v0, l1);
+      mma(acc[1][0], v1, l0); mma(acc[1][1], v1, l1);
+      v0 = nv0; v1 = nv1; m0 = nm0; m1 = nm1;
     }
     u0 = nu0; u1 = nu1; l0 = nl0; l1 = nl1;
   }
@@ -1169,6 +1304,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
     HIP_CK(hipFuncSetAttribute((const void*)k_panel, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 8192));
     HIP_CK(hipFuncSetAttribute((const void*)k_trsm, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 8192));
     HIP_CK(hipFuncSetAttribute((const void*)k_panel2, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes()));
+    HIP_CK(hipFuncSetAttribute((const void*)k_trsm_mfma, hipFuncAttributeMaxDynamicSharedMemorySize, (int)trsm_mfma_lds_bytes()));
     attr_set = true;
   }
   // exchange area for the largest panel, cleared once per factorization (tags restart at 1)
@@ -1182,6 +1318,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   int rmin = 64;
   if (const char* e = getenv("MPSOLVE_PANEL_RMIN")) rmin = std::max(64, std::min(256, atoi(e)));
   const bool allow_p2 = !(getenv("MPSOLVE_PANEL") && atoi(getenv("MPSOLVE_PANEL")) == 1);
+  const bool use_fma_trsm = getenv("MPSOLVE_TRSM") && atoi(getenv("MPSOLVE_TRSM")) == 1;
   int panels = 0;
   for (int64_t k = 0; k < n; k += nb) {
     const int kb = (int)std::min<int64_t>(nb, n - k);
@@ -1231,7 +1368,8 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
       hipEvent_t e2 = W.prof_begin();
       hipLaunchKernelGGL(k_trsm_inv, dim3(cdiv(kb, 32)), dim3(64), 0, W.stream, a.A, a.lda, k, kb, W.linv.p);
       LAUNCH_CK();
-      hipLaunchKernelGGL(k_trsm, dim3(cdiv(m2, 32)), dim3(256), tl, W.stream, a.A, a.lda, n, k, kb, W.linv.p);
+      if (use_fma_trsm) hipLaunchKernelGGL(k_trsm, dim3(cdiv(m2, 32)), dim3(256), tl, W.stream, a.A, a.lda, n, k, kb, W.linv.p);
+      else hipLaunchKernelGGL(k_trsm_mfma, dim3(cdiv(m2, 128)), dim3(256), trsm_mfma_lds_bytes(), W.stream, a.A, a.lda, n, k, kb, W.linv.p);
       LAUNCH_CK();
       W.prof_end(e2, "trsm", 8.0 * (double)kb * m2, (double)kb * kb * m2);
       switch (a.fmt) {

@@ -675,6 +675,9 @@ int mp_create(mp_ctx** out, int device) {
     std::unique_ptr<mp_ctx> C(new mp_ctx());
     C->device = device;
     HIP_CK(hipStreamCreateWithFlags(&C->W.stream, hipStreamNonBlocking));
+    HIP_CK(hipStreamCreateWithFlags(&C->W.pstream, hipStreamNonBlocking));
+    HIP_CK(hipEventCreateWithFlags(&C->W.ev_next, hipEventDisableTiming));
+    HIP_CK(hipEventCreateWithFlags(&C->W.ev_panel, hipEventDisableTiming));
     C->W.tracker = &C->tracker;
     hipDeviceProp_t prop; HIP_CK(hipGetDeviceProperties(&prop, device));
     C->W.num_cus = prop.multiProcessorCount;
@@ -695,8 +698,13 @@ void mp_destroy(mp_ctx* C) {
   C->kry.reset();
   if (C->rb) rocblas_destroy_handle(C->rb);
   if (C->hpin) (void)hipHostFree(C->hpin);
-  hipStream_t s = C->W.stream;
+  (void)hipStreamSynchronize(C->W.pstream);
+  hipStream_t s = C->W.stream, ps = C->W.pstream;
+  hipEvent_t e1 = C->W.ev_next, e2 = C->W.ev_panel;
   delete C;
+  if (e1) (void)hipEventDestroy(e1);
+  if (e2) (void)hipEventDestroy(e2);
+  if (ps) (void)hipStreamDestroy(ps);
   if (s) (void)hipStreamDestroy(s);
 }
 

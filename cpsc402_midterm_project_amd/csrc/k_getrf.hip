@@ -783,12 +783,14 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
 
 // ------------------------------------------------------------------ row interchanges outside the panel
 // plan[0] = number of moved rows, then (dst, src) pairs of panel-local rows: new[dst] = old[src].
-// Columns [0, k) and [k+kb, n) of A, plus the composed permutation vector (one extra "column").
-__global__ __launch_bounds__(256) void k_laswp(float* __restrict__ A, int64_t lda, int64_t n, int64_t k, int kb,
+// Columns [ra0, ra1) and [rb0, rb1) of A (rows k .. n-1), plus, with do_perm, the composed permutation vector
+// (one extra "column").  Workgroups stride over the columns.
+__global__ __launch_bounds__(256) void k_laswp(float* __restrict__ A, int64_t lda, int64_t k,
+                                               int64_t ra0, int64_t ra1, int64_t rb0, int64_t rb1, int do_perm,
                                                const int32_t* __restrict__ plan, int32_t* __restrict__ perm) {
   const int cnt = plan[0];
   const int t = threadIdx.x;
-  const int64_t ncols = n - kb;
+  const int64_t na = ra1 - ra0, nb2 = rb1 - rb0, ncols = na + nb2;
   int dst[4], src[4];
 #pragma unroll
   for (int q = 0; q < 4; q++) {
@@ -796,7 +798,7 @@ __global__ __launch_bounds__(256) void k_laswp(float* __restrict__ A, int64_t ld
     dst[q] = (e < cnt) ? plan[1 + 2 * e] : -1;
     src[q] = (e < cnt) ? plan[2 + 2 * e] : -1;
   }
-  for (int64_t cc = blockIdx.x; cc <= ncols; cc += gridDim.x) {
+  for (int64_t cc = blockIdx.x; cc < ncols + do_perm; cc += gridDim.x) {
     if (cc == ncols) {
       int32_t* pp = perm + k;
       int32_t v[4];
@@ -806,7 +808,7 @@ __global__ __launch_bounds__(256) void k_laswp(float* __restrict__ A, int64_t ld
 #pragma unroll
       for (int q = 0; q < 4; q++) if (dst[q] >= 0) pp[dst[q]] = v[q];
     } else {
-      const int64_t col = (cc < k) ? cc : cc + kb;
+      const int64_t col = (cc < na) ? ra0 + cc : rb0 + (cc - na);
       float* cp = A + col * lda + k;
       float v[4];
 #pragma unroll
@@ -917,11 +919,13 @@ __global__ __launch_bounds__(256) void k_trsm(float* __restrict__ A, int64_t lda
 // k_trsm (same algorithm, FP32).
 constexpr int TM_BLK = 32 * 33;
 static size_t trsm_mfma_lds_bytes() { return (size_t)36 * TM_BLK * 4; }
-__global__ __launch_bounds__(256) void k_trsm_mfma(float* __restrict__ A, int64_t lda, int64_t n, int64_t k, int kb,
+// Columns [cb, ce) of U12 (cb >= k + kb); grid = cdiv(ce - cb, 128).
+__global__ __launch_bounds__(256) void k_trsm_mfma(float* __restrict__ A, int64_t lda, int64_t cb, int64_t ce, int64_t k, int kb,
                                                    const float* __restrict__ Linv) {
   extern __shared__ float sm[];
   const int t = threadIdx.x, w = t >> 6, l = t & 63, h = l >> 5, li = l & 31;
-  const int64_t c0 = k + kb + (int64_t)blockIdx.x * 128 + w * 32;    // first column of this wave's strip
+  const int64_t n = ce;
+  const int64_t c0 = cb + (int64_t)blockIdx.x * 128 + w * 32;        // first column of this wave's strip
   const int64_t ncol = max((int64_t)0, min((int64_t)32, n - c0));
   const int nrb = (kb + 31) / 32;
   float* tile = sm + w * TM_BLK;                                      // staging tile of this wave
@@ -1243,39 +1247,91 @@ __global__ __launch_bounds__(256) void k_gemm(float* __restrict__ C, int64_t ldc
 // ------------------------------------------------------------------ host driver
 static int lds_limit_bytes() { return 160 * 1024; }
 
+// Packed operand buffers for a trailing update of m2 rows x m2 columns: tiles padded to whole 128-wide
+// workgroup tiles.
 template <int FMT>
-static void pack_and_update(float* A, int64_t lda, int64_t k, int kb, int64_t m2, bool tile_scale, Workspace& W) {
+static void pack_buffers(int kb, int64_t m2, Workspace& W) {
   constexpr int ES = FmtInfo<FMT>::ES;
   const int nkb = (kb + 15) / 16;
-  const int nT = (int)((cdiv(m2, 128)) * 4);   // tiles padded to whole 128-wide workgroup tiles
+  const int nT = (int)((cdiv(m2, 128)) * 4);
   const size_t bytes = (size_t)nT * nkb * 64 * 8 * ES;
   W.packL.ensure(*W.tracker, bytes, "packL");
   W.packU.ensure(*W.tracker, bytes, "packU");
-  uint8_t* lo_l = nullptr; uint8_t* lo_u = nullptr;
-  if (FMT == FMT_FP16X3) {
-    W.packLlo.ensure(*W.tracker, bytes, "packL_lo"); W.packUlo.ensure(*W.tracker, bytes, "packU_lo");
-    lo_l = W.packLlo.p; lo_u = W.packUlo.p;
-  }
+  if (FMT == FMT_FP16X3) { W.packLlo.ensure(*W.tracker, bytes, "packL_lo"); W.packUlo.ensure(*W.tracker, bytes, "packU_lo"); }
   W.scaleL.ensure(*W.tracker, nT, "scaleL");
   W.scaleU.ensure(*W.tracker, nT, "scaleU");
+}
+
+// L21 (rows k+kb .. n-1 of the panel columns) into row tiles 0 ..
+template <int FMT>
+static void pack_L(float* A, int64_t lda, int64_t k, int kb, int64_t m2, bool tile_scale, Workspace& W) {
+  constexpr int ES = FmtInfo<FMT>::ES;
+  const int nkb = (kb + 15) / 16;
+  const int nT = (int)((cdiv(m2, 128)) * 4);
   const size_t lds = (size_t)32 * (nkb * 16 + 1) * sizeof(float);
   hipEvent_t evp = W.prof_begin();
   hipLaunchKernelGGL((k_pack<FMT>), dim3(nT), dim3(256), lds, W.stream,
-                     A + (k + kb) + k * lda, (int64_t)1, lda, (int)m2, kb, nkb, W.packL.p, lo_l, W.scaleL.p, W.cnt.p, tile_scale ? 1 : 0);
+                     A + (k + kb) + k * lda, (int64_t)1, lda, (int)m2, kb, nkb, W.packL.p,
+                     FMT == FMT_FP16X3 ? W.packLlo.p : nullptr, W.scaleL.p, W.cnt.p, tile_scale ? 1 : 0);
   LAUNCH_CK();
+  W.prof_end(evp, "pack", (4.0 + ES) * (double)m2 * kb, 0.0);
+}
+
+// U12 columns [cb, ce) (cb - (k + kb) a multiple of 32) into column tiles starting at (cb - k - kb) / 32;
+// the tiles up to the next multiple of four past ce are written too (zero padding of the last workgroup tile)
+template <int FMT>
+static void pack_U(float* A, int64_t lda, int64_t k, int kb, int64_t cb, int64_t ce, bool last, bool tile_scale, Workspace& W) {
+  constexpr int ES = FmtInfo<FMT>::ES;
+  const int nkb = (kb + 15) / 16;
+  const int64_t t0 = (cb - k - kb) / 32;
+  const int64_t nidx = ce - cb;
+  const int nT = (int)(last ? cdiv(nidx, 128) * 4 : cdiv(nidx, 32));
+  const size_t tb = (size_t)nkb * 64 * 8 * ES;
+  const size_t lds = (size_t)32 * (nkb * 16 + 1) * sizeof(float);
+  hipEvent_t evp = W.prof_begin();
   hipLaunchKernelGGL((k_pack<FMT>), dim3(nT), dim3(256), lds, W.stream,
-                     A + k + (k + kb) * lda, lda, (int64_t)1, (int)m2, kb, nkb, W.packU.p, lo_u, W.scaleU.p, W.cnt.p, tile_scale ? 1 : 0);
+                     A + k + cb * lda, lda, (int64_t)1, (int)nidx, kb, nkb, W.packU.p + t0 * tb,
+                     FMT == FMT_FP16X3 ? W.packUlo.p + t0 * tb : nullptr, W.scaleU.p + t0, W.cnt.p, tile_scale ? 1 : 0);
   LAUNCH_CK();
-  W.prof_end(evp, "pack", 2.0 * (4.0 + ES) * (double)m2 * kb, 0.0);
-  const unsigned g = cdiv(m2, 128);
+  W.prof_end(evp, "pack", (4.0 + ES) * (double)nidx * kb, 0.0);
+}
+
+// A22[:, cb .. ce) -= L21 * U12[:, cb .. ce)
+template <int FMT>
+static void gemm_cols(float* A, int64_t lda, int64_t k, int kb, int64_t m2, int64_t cb, int64_t ce, Workspace& W) {
+  constexpr int ES = FmtInfo<FMT>::ES, FB = FmtInfo<FMT>::FB;
+  const int nkb = (kb + 15) / 16;
+  const int64_t t0 = (cb - k - kb) / 32, n2 = ce - cb;
+  const size_t tb = (size_t)nkb * 64 * FB;
   // algorithmic traffic of the update: C read + written (8 B/element) plus both packed operands once
-  const double gb = 8.0 * (double)m2 * m2 + (FMT == FMT_FP16X3 ? 4.0 : 2.0) * ES * (double)m2 * kb;
-  const double gf = (FMT == FMT_FP16X3 ? 6.0 : 2.0) * (double)m2 * m2 * kb;
+  const double gb = 8.0 * (double)m2 * n2 + (FMT == FMT_FP16X3 ? 2.0 : 1.0) * ES * (double)(m2 + n2) * kb;
+  const double gf = (FMT == FMT_FP16X3 ? 6.0 : 2.0) * (double)m2 * n2 * kb;
   hipEvent_t ev = W.prof_begin();
-  hipLaunchKernelGGL((k_gemm<FMT>), dim3(g, g), dim3(256), 0, W.stream,
-                     A + (k + kb) + (k + kb) * lda, lda, (int)m2, (int)m2, nkb, W.packL.p, W.packU.p, lo_l, lo_u, W.scaleL.p, W.scaleU.p);
+  hipLaunchKernelGGL((k_gemm<FMT>), dim3((unsigned)cdiv(m2, 128), (unsigned)cdiv(n2, 128)), dim3(256), 0, W.stream,
+                     A + (k + kb) + cb * lda, lda, (int)m2, (int)n2, nkb, W.packL.p, W.packU.p + t0 * tb,
+                     FMT == FMT_FP16X3 ? W.packLlo.p : nullptr, FMT == FMT_FP16X3 ? W.packUlo.p + t0 * tb : nullptr,
+                     W.scaleL.p, W.scaleU.p + t0);
   LAUNCH_CK();
   W.prof_end(ev, "trailing_gemm", gb, gf);
+}
+
+template <int FMT>
+static void update_cols(float* A, int64_t lda, int64_t k, int kb, int64_t m2, int64_t cb, int64_t ce, bool last,
+                        bool with_L, bool tile_scale, Workspace& W) {
+  if (with_L) { pack_buffers<FMT>(kb, m2, W); pack_L<FMT>(A, lda, k, kb, m2, tile_scale, W); }
+  pack_U<FMT>(A, lda, k, kb, cb, ce, last, tile_scale, W);
+  gemm_cols<FMT>(A, lda, k, kb, m2, cb, ce, W);
+}
+
+static void update_cols_fmt(int fmt, float* A, int64_t lda, int64_t k, int kb, int64_t m2, int64_t cb, int64_t ce,
+                            bool last, bool with_L, bool tile_scale, Workspace& W) {
+  switch (fmt) {
+    case FMT_FP16: update_cols<FMT_FP16>(A, lda, k, kb, m2, cb, ce, last, with_L, tile_scale, W); break;
+    case FMT_BF16: update_cols<FMT_BF16>(A, lda, k, kb, m2, cb, ce, last, with_L, tile_scale, W); break;
+    case FMT_FP8:  update_cols<FMT_FP8>(A, lda, k, kb, m2, cb, ce, last, with_L, tile_scale, W); break;
+    case FMT_FP16X3: update_cols<FMT_FP16X3>(A, lda, k, kb, m2, cb, ce, last, with_L, tile_scale, W); break;
+    default:       update_cols<FMT_FP32>(A, lda, k, kb, m2, cb, ce, last, with_L, tile_scale, W); break;
+  }
 }
 
 void fill_iota(int32_t* p, int64_t n, Workspace& W) {
@@ -1283,6 +1339,89 @@ void fill_iota(int32_t* p, int64_t n, Workspace& W) {
   LAUNCH_CK();
 }
 
+constexpr int PLAN_WORDS = 1 + 4 * 256 + 8;
+
+static bool panel2_usable(const GetrfArgs& a, const Workspace& W, int64_t k, int kb, bool allow_p2) {
+  const int64_t G2 = cdiv(a.n - k, P2_R);
+  return allow_p2 && a.n < 65536 && G2 <= std::min<int64_t>(P2_MAXG, W.num_cus) &&
+         panel2_lds_bytes(kb) <= (size_t)lds_limit_bytes();
+}
+
+// Panel factorization of columns [k, k + kb) on `stream`.  Without `cooperative` (panel2 only) the launch
+// may share the device with other kernels.
+static void run_panel(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int32_t* plan, hipStream_t stream,
+                      bool cooperative, bool allow_p2, int rmin) {
+  const int64_t n = a.n;
+  const int m = (int)(n - k);
+  const int64_t G2 = cdiv(m, P2_R);
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, stream, plan, 1);
+  if (panel2_usable(a, W, k, kb, allow_p2)) {
+    Panel2Args pa;
+    pa.A = a.A; pa.lda = a.lda; pa.k = k; pa.kb = kb; pa.m = m; pa.G = (int)G2;
+    pa.xbuf = W.xbuf.p;
+    pa.ipiv = a.ipiv; pa.dinfo = W.dinfo.p; pa.plan = plan;
+    void* args[] = {&pa};
+    hipEvent_t e0 = W.prof_begin(stream);
+    if (cooperative)
+      HIP_CK(hipLaunchCooperativeKernel((const void*)k_panel2, dim3((unsigned)G2), dim3(256), args, (unsigned)panel2_lds_bytes(kb), stream));
+    else {
+      // overlapped with trailing-update kernels that never wait on anything: workgroups that are not yet
+      // resident start as those kernels drain, and the resident ones spin meanwhile
+      hipLaunchKernelGGL(k_panel2, dim3((unsigned)G2), dim3(256), panel2_lds_bytes(kb), stream, pa);
+      LAUNCH_CK();
+    }
+    W.prof_end(e0, "panel", 8.0 * (double)m * kb, (double)m * kb * kb, stream);
+    return;
+  }
+  int R = 0;
+  for (int cand : {64, 128, 256}) {
+    if (cand < rmin) continue;
+    const int64_t G = cdiv(m, cand);
+    const size_t lds = (size_t)cand * kb * 4 + (size_t)cand * 4;
+    if (G <= std::min(128, W.num_cus) && lds <= (size_t)lds_limit_bytes() - 8192) { R = cand; break; }
+  }
+  MP_REQUIRE(R > 0, "getrf: panel does not fit on the device (n too large for this nb)");
+  MP_REQUIRE(cooperative, "getrf: the fallback panel needs a cooperative launch");
+  const int G = (int)cdiv(m, R);
+  PanelArgs pa;
+  pa.A = a.A; pa.lda = a.lda; pa.k = k; pa.kb = kb; pa.m = m; pa.G = G; pa.R = R;
+  pa.xbuf = W.xbuf.p; pa.ipiv = a.ipiv; pa.dinfo = W.dinfo.p; pa.plan = plan;
+  void* args[] = {&pa};
+  const size_t lds = (size_t)R * kb * 4 + (size_t)R * 4;
+  hipEvent_t e0 = W.prof_begin(stream);
+  HIP_CK(hipLaunchCooperativeKernel((const void*)k_panel, dim3(G), dim3(256), args, (unsigned)lds, stream));
+  W.prof_end(e0, "panel", 8.0 * (double)m * kb, (double)m * kb * kb, stream);
+}
+
+static void laswp_ranges(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int64_t ra0, int64_t ra1,
+                         int64_t rb0, int64_t rb1, bool perm, const int32_t* plan) {
+  const int64_t ncols = (ra1 - ra0) + (rb1 - rb0) + (perm ? 1 : 0);
+  if (ncols <= 0) return;
+  hipEvent_t e1 = W.prof_begin();
+  hipLaunchKernelGGL(k_laswp, dim3((unsigned)std::min<int64_t>(ncols, 4096)), dim3(256), 0, W.stream,
+                     a.A, a.lda, k, ra0, ra1, rb0, rb1, perm ? 1 : 0, plan, a.perm);
+  LAUNCH_CK();
+  W.prof_end(e1, "laswp", 16.0 * (double)(ncols - (perm ? 1 : 0)) * kb, 0.0);
+}
+
+static void trsm_cols(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int64_t cb, int64_t ce, bool fma) {
+  if (ce <= cb) return;
+  hipEvent_t e2 = W.prof_begin();
+  if (fma) {
+    // the FMA kernel handles the whole trailing width only
+    const size_t tl = (size_t)kb * 33 * 4 * 2;
+    hipLaunchKernelGGL(k_trsm, dim3(cdiv(ce - (k + kb), 32)), dim3(256), tl, W.stream, a.A, a.lda, ce, k, kb, W.linv.p);
+  } else {
+    hipLaunchKernelGGL(k_trsm_mfma, dim3((unsigned)cdiv(ce - cb, 128)), dim3(256), trsm_mfma_lds_bytes(), W.stream,
+                       a.A, a.lda, cb, ce, k, kb, W.linv.p);
+  }
+  LAUNCH_CK();
+  W.prof_end(e2, "trsm", 8.0 * (double)kb * (ce - cb), (double)kb * kb * (ce - cb));
+}
+
+// Right-looking blocked LU.  With look-ahead (default), step k first brings the next panel's columns up to
+// date (interchanges, TRSM, operand packing, update), releases the next panel factorization on the second
+// stream, and then updates the remaining columns on the main stream while that panel runs.
 void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   const int64_t n = a.n;
   MP_REQUIRE(n > 0 && a.A && a.ipiv && a.perm, "getrf: n > 0 and A, ipiv, perm required");
@@ -1293,7 +1432,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   W.factor_gen++;
   W.dinfo.ensure(*W.tracker, 4, "getrf_info");
   W.cnt.ensure(*W.tracker, 2, "getrf_counts");
-  W.affect.ensure(*W.tracker, 1 + 4 * 256 + 8, "swap_plan");
+  W.affect.ensure(*W.tracker, 2 * PLAN_WORDS, "swap_plan");
   W.linv.ensure(*W.tracker, (size_t)cdiv(nb, 32) * 1024, "trsm_block_inverses");
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, W.stream, W.dinfo.p, 4);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, W.stream, W.cnt.p, 2);
@@ -1319,68 +1458,48 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   if (const char* e = getenv("MPSOLVE_PANEL_RMIN")) rmin = std::max(64, std::min(256, atoi(e)));
   const bool allow_p2 = !(getenv("MPSOLVE_PANEL") && atoi(getenv("MPSOLVE_PANEL")) == 1);
   const bool use_fma_trsm = getenv("MPSOLVE_TRSM") && atoi(getenv("MPSOLVE_TRSM")) == 1;
+  const bool overlap = !(getenv("MPSOLVE_OVERLAP") && atoi(getenv("MPSOLVE_OVERLAP")) == 0) && W.pstream && !use_fma_trsm;
+  auto plan_of = [&](int64_t k) { return W.affect.p + ((k / nb) & 1) * PLAN_WORDS; };
   int panels = 0;
+  run_panel(a, W, 0, (int)std::min<int64_t>(nb, n), plan_of(0), W.stream, true, allow_p2, rmin);
+  panels++;
   for (int64_t k = 0; k < n; k += nb) {
     const int kb = (int)std::min<int64_t>(nb, n - k);
-    const int m = (int)(n - k);
-    const int64_t G2 = cdiv(m, P2_R);
-    if (allow_p2 && n < 65536 && G2 <= std::min(P2_MAXG, W.num_cus) && panel2_lds_bytes(kb) <= (size_t)lds_limit_bytes()) {
-      hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, W.stream, W.affect.p, 1);
-      Panel2Args pa;
-      pa.A = a.A; pa.lda = a.lda; pa.k = k; pa.kb = kb; pa.m = m; pa.G = (int)G2;
-      pa.xbuf = W.xbuf.p;
-      pa.ipiv = a.ipiv; pa.dinfo = W.dinfo.p; pa.plan = W.affect.p;
-      void* args[] = {&pa};
-      hipEvent_t e0 = W.prof_begin();
-      HIP_CK(hipLaunchCooperativeKernel((const void*)k_panel2, dim3((unsigned)G2), dim3(256), args,
-                                        (unsigned)panel2_lds_bytes(kb), W.stream));
-      W.prof_end(e0, "panel", 8.0 * (double)m * kb, (double)m * kb * kb);
-    } else {
-    int R = 0;
-    for (int cand : {64, 128, 256}) {
-      if (cand < rmin) continue;
-      const int64_t G = cdiv(m, cand);
-      const size_t lds = (size_t)cand * kb * 4 + (size_t)cand * 4;
-      if (G <= std::min(128, W.num_cus) && lds <= (size_t)lds_limit_bytes() - 8192) { R = cand; break; }
-    }
-    MP_REQUIRE(R > 0, "getrf: panel does not fit on the device (n too large for this nb)");
-    const int G = (int)cdiv(m, R);
-    (void)G;
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, W.stream, W.affect.p, 1);
-    PanelArgs pa;
-    pa.A = a.A; pa.lda = a.lda; pa.k = k; pa.kb = kb; pa.m = m; pa.G = G; pa.R = R;
-    pa.xbuf = W.xbuf.p; pa.ipiv = a.ipiv; pa.dinfo = W.dinfo.p; pa.plan = W.affect.p;
-    void* args[] = {&pa};
-    const size_t lds = (size_t)R * kb * 4 + (size_t)R * 4;
-    hipEvent_t e0 = W.prof_begin();
-    HIP_CK(hipLaunchCooperativeKernel((const void*)k_panel, dim3(G), dim3(256), args, (unsigned)lds, W.stream));
-    W.prof_end(e0, "panel", 8.0 * (double)m * kb, (double)m * kb * kb);
-    }
-    const int64_t ncols = n - kb + 1;
-    hipEvent_t e1 = W.prof_begin();
-    hipLaunchKernelGGL(k_laswp, dim3((unsigned)std::min<int64_t>(ncols, 4096)), dim3(256), 0, W.stream,
-                       a.A, a.lda, n, k, kb, W.affect.p, a.perm);
-    LAUNCH_CK();
-    W.prof_end(e1, "laswp", 16.0 * (double)(n - kb) * kb, 0.0);
     const int64_t m2 = n - k - kb;
-    if (m2 > 0) {
-      const size_t tl = (size_t)kb * 33 * 4 * 2;
+    int32_t* plan = plan_of(k);
+    if (m2 == 0) { laswp_ranges(a, W, k, kb, 0, k, 0, 0, true, plan); break; }
+    const int kbn = (int)std::min<int64_t>(nb, m2);            // next panel width
+    const int64_t cN = k + kb + kbn;                            // end of the next panel's columns
+    const bool split = overlap && (kbn % 128) == 0 && m2 > kbn && panel2_usable(a, W, k + kb, kbn, allow_p2);
+    if (split) {
+      laswp_ranges(a, W, k, kb, k + kb, cN, 0, 0, false, plan);
+    } else {
+      laswp_ranges(a, W, k, kb, 0, k, k + kb, n, true, plan);
+    }
+    {
       hipEvent_t e2 = W.prof_begin();
       hipLaunchKernelGGL(k_trsm_inv, dim3(cdiv(kb, 32)), dim3(64), 0, W.stream, a.A, a.lda, k, kb, W.linv.p);
       LAUNCH_CK();
-      if (use_fma_trsm) hipLaunchKernelGGL(k_trsm, dim3(cdiv(m2, 32)), dim3(256), tl, W.stream, a.A, a.lda, n, k, kb, W.linv.p);
-      else hipLaunchKernelGGL(k_trsm_mfma, dim3(cdiv(m2, 128)), dim3(256), trsm_mfma_lds_bytes(), W.stream, a.A, a.lda, n, k, kb, W.linv.p);
-      LAUNCH_CK();
-      W.prof_end(e2, "trsm", 8.0 * (double)kb * m2, (double)kb * kb * m2);
-      switch (a.fmt) {
-        case FMT_FP16: pack_and_update<FMT_FP16>(a.A, a.lda, k, kb, m2, a.tile_scale, W); break;
-        case FMT_BF16: pack_and_update<FMT_BF16>(a.A, a.lda, k, kb, m2, a.tile_scale, W); break;
-        case FMT_FP8:  pack_and_update<FMT_FP8>(a.A, a.lda, k, kb, m2, a.tile_scale, W); break;
-        case FMT_FP16X3: pack_and_update<FMT_FP16X3>(a.A, a.lda, k, kb, m2, a.tile_scale, W); break;
-        default:       pack_and_update<FMT_FP32>(a.A, a.lda, k, kb, m2, a.tile_scale, W); break;
-      }
+      W.prof_end(e2, "trsm", 0.0, 0.0);
     }
-    panels++;
+    if (split) {
+      trsm_cols(a, W, k, kb, k + kb, cN, false);
+      update_cols_fmt(a.fmt, a.A, a.lda, k, kb, m2, k + kb, cN, false, true, a.tile_scale, W);
+      HIP_CK(hipEventRecord(W.ev_next, W.stream));
+      HIP_CK(hipStreamWaitEvent(W.pstream, W.ev_next, 0));
+      run_panel(a, W, k + kb, kbn, plan_of(k + kb), W.pstream, false, allow_p2, rmin);
+      panels++;
+      HIP_CK(hipEventRecord(W.ev_panel, W.pstream));
+      laswp_ranges(a, W, k, kb, 0, k, cN, n, true, plan);
+      trsm_cols(a, W, k, kb, cN, n, false);
+      update_cols_fmt(a.fmt, a.A, a.lda, k, kb, m2, cN, n, true, false, a.tile_scale, W);
+      HIP_CK(hipStreamWaitEvent(W.stream, W.ev_panel, 0));
+    } else {
+      trsm_cols(a, W, k, kb, k + kb, n, use_fma_trsm);
+      update_cols_fmt(a.fmt, a.A, a.lda, k, kb, m2, k + kb, n, true, true, a.tile_scale, W);
+      run_panel(a, W, k + kb, kbn, plan_of(k + kb), W.stream, true, allow_p2, rmin);
+      panels++;
+    }
   }
   int32_t hinfo[2] = {0, 0};
   unsigned long long hc[2] = {0, 0};

@@ -31,6 +31,8 @@ struct KernelProfiler {
 
 struct Workspace {
   hipStream_t stream = nullptr;
+  hipStream_t pstream = nullptr;          // second stream: panel factorizations overlapped with trailing updates
+  hipEvent_t ev_next = nullptr, ev_panel = nullptr;   // cross-stream ordering (timing disabled)
   KernelProfiler prof;
   DevTracker* tracker = nullptr;
   int num_cus = 256;
@@ -53,10 +55,12 @@ struct Workspace {
   uint64_t factor_gen = 1;   // incremented by every factorization written through getrf
   uint32_t tag = 1;      // next free exchange tag (never 0)
   // bracket a launch with profiling events (no-op when profiling is off)
-  hipEvent_t prof_begin() { if (!prof.on) return nullptr; hipEvent_t e = prof.get(); HIP_CK(hipEventRecord(e, stream)); return e; }
-  void prof_end(hipEvent_t a, const char* name, double bytes, double flops) {
+  hipEvent_t prof_begin(hipStream_t s = nullptr) {
+    if (!prof.on) return nullptr; hipEvent_t e = prof.get(); HIP_CK(hipEventRecord(e, s ? s : stream)); return e;
+  }
+  void prof_end(hipEvent_t a, const char* name, double bytes, double flops, hipStream_t s = nullptr) {
     if (!prof.on || !a) return;
-    hipEvent_t b = prof.get(); HIP_CK(hipEventRecord(b, stream));
+    hipEvent_t b = prof.get(); HIP_CK(hipEventRecord(b, s ? s : stream));
     prof.pending.push_back({name, a, b, bytes, flops});
     if (prof.pending.size() > 4096) prof.drain();
   }

@@ -108,6 +108,27 @@ def test_getrf_random_backward_error(ctx, oracle, fmt):
     np.testing.assert_array_equal(p, pm)
 
 
+@pytest.mark.parametrize("fmt,n,nb", [("fp16", 3000, 256), ("fp16x3", 2176, 128), ("fp8", 1409, 128)])
+def test_getrf_lookahead_matches_sequential(ctx, fmt, n, nb, monkeypatch):
+    """The look-ahead schedule (next panel's columns updated first, panel factorized on a second stream while
+    the remaining columns are updated) runs the same kernels on the same column tiles: factors, ipiv and perm
+    must be bit-identical to the one-stream schedule."""
+    print(f"[getrf lookahead] fmt={fmt} n={n} nb={nb}", flush=True)
+    rng = np.random.default_rng(n)
+    A = np.asfortranarray(rng.standard_normal((n, n)).astype(np.float32))
+    out = []
+    for mode in ("0", "1"):
+        monkeypatch.setenv("MPSOLVE_OVERLAP", mode)
+        dLU = _dev(ctx, A); dip, dpm = ctx.empty(8 * n), ctx.empty(4 * n)
+        info = ctx.getrf(n, dLU, n, dip, dpm, nb=nb, fmt=fmt)
+        assert info.info == 0 and info.panels == -(-n // nb)
+        out.append((dLU.to_array((n, n), np.float32), dip.to_array((n,), np.int64), dpm.to_array((n,), np.int32)))
+        for b in (dLU, dip, dpm):
+            b.free()
+    for x, y in zip(out[0], out[1]):
+        np.testing.assert_array_equal(x, y)
+
+
 def test_getrf_singular_info(ctx, oracle):
     n = 96
     A = np.ones((n, n), dtype=np.float32, order="F")

@@ -532,79 +532,118 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
       }
       ps0 = (v0 && pos0 >= c0) ? 32 : -1;
       ps1 = (v1 && pos1 >= c0) ? 32 : -1;
-      for (int jj = 0; jj < w; jj++) {
-        const int j = c0 + jj;
+      // Step order: the candidate of column j+1 is found from the rows' column-j+1 values alone (updated
+      // first) and its head pushed out; the rest of the rank-1 update, the multipliers and the candidate's
+      // row (which readers fetch only after the heads round trip) follow while the exchange is in flight.
+      int blane = -1;                                     // candidate lane of the current column (-1: none)
+      bool bsel = false;                                  // candidate is the lane's second row
+      auto publish_head = [&](int j, float cur0, float cur1) {
         const uint32_t tag = (uint32_t)(a.k + j + 1);
-        const int par = j & 1;
-        PANEL2_MARK(j, 0);
-        // (1) the workgroup's candidate and its publication
         float cv0 = -1.0f, cv1 = -1.0f;
-        if (v0 && pos0 >= j) { float xa = fabsf(x0[0]); if (xa != xa) xa = INFINITY; cv0 = xa; }
-        if (v1 && pos1 >= j) { float xa = fabsf(x1[0]); if (xa != xa) xa = INFINITY; cv1 = xa; }
+        if (v0 && pos0 >= j) { float xa = fabsf(cur0); if (xa != xa) xa = INFINITY; cv0 = xa; }
+        if (v1 && pos1 >= j) { float xa = fabsf(cur1); if (xa != xa) xa = INFINITY; cv1 = xa; }
         const bool take1 = (cv1 > cv0) || (cv1 == cv0 && cv1 >= 0.0f && pos1 < pos0);
         float bv; int bpos;
-        const int blane = wave_argmax_pos(take1 ? cv1 : cv0, take1 ? pos1 : pos0, bv, bpos);
-        const bool bsel = (blane >= 0) && (__builtin_amdgcn_readlane((int)take1, blane) != 0);
-        if (blane >= 0) {
-          // the candidate lane stages its 32 values in LDS; lanes 0-31 / 32-63 store replicas 0,1 / 2,3
-          if (lane == blane) {
-            float4* c4 = (float4*)crow;
-            if (bsel) {
-#pragma unroll
-              for (int q = 0; q < 8; q++) c4[q] = make_float4(x1[4 * q], x1[4 * q + 1], x1[4 * q + 2], x1[4 * q + 3]);
-            } else {
-#pragma unroll
-              for (int q = 0; q < 8; q++) c4[q] = make_float4(x0[4 * q], x0[4 * q + 1], x0[4 * q + 2], x0[4 * q + 3]);
-            }
-          }
-          lds_wait();
-          const uint32_t mine = __float_as_uint(crow[lane & 31]);
-          const int rep = lane >> 5;
-          put_word(rowsl + (((size_t)(par * P2_NREP + rep) * P2_MAXG + g) * 32 + (lane & 31)), tag, mine);
-          put_word(rowsl + (((size_t)(par * P2_NREP + rep + 2) * P2_MAXG + g) * 32 + (lane & 31)), tag, mine);
-        }
+        blane = wave_argmax_pos(take1 ? cv1 : cv0, take1 ? pos1 : pos0, bv, bpos);
+        bsel = (blane >= 0) && (__builtin_amdgcn_readlane((int)take1, blane) != 0);
         const uint64_t hwv = head_word(tag, blane >= 0 ? bpos : 0, blane >= 0 ? __float_as_uint(bv) : 0x7FFFFFFFu);
+        uint64_t* mrow = mbox + (size_t)(j & 1) * P2_MAXG * P2_MBOX + g;
         for (int rd = lane; rd < G; rd += 64)
-          __hip_atomic_store(mbox + ((size_t)par * P2_MAXG + rd) * P2_MBOX + g, hwv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        PANEL2_MARK(j, 1);
-        // (3) wait for wave 1's step flag
+          __hip_atomic_store(mrow + (size_t)rd * P2_MBOX, hwv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      };
+      // the candidate's 32 values (x[0] = column j), staged in LDS, four replicas
+      auto publish_row = [&](int j) {
+        if (blane < 0) return;
+        const uint32_t tag = (uint32_t)(a.k + j + 1);
+        const int par = j & 1;
+        if (lane == blane) {
+          float4* c4 = (float4*)crow;
+          if (bsel) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) c4[q] = make_float4(x1[4 * q], x1[4 * q + 1], x1[4 * q + 2], x1[4 * q + 3]);
+          } else {
+#pragma unroll
+            for (int q = 0; q < 8; q++) c4[q] = make_float4(x0[4 * q], x0[4 * q + 1], x0[4 * q + 2], x0[4 * q + 3]);
+          }
+        }
+        lds_wait();
+        const uint32_t mine = __float_as_uint(crow[lane & 31]);
+        const int rep = lane >> 5;
+        put_word(rowsl + (((size_t)(par * P2_NREP + rep) * P2_MAXG + g) * 32 + (lane & 31)), tag, mine);
+        put_word(rowsl + (((size_t)(par * P2_NREP + rep + 2) * P2_MAXG + g) * 32 + (lane & 31)), tag, mine);
+      };
+      publish_head(c0, x0[0], x1[0]);
+      publish_row(c0);
+      for (int jj = 0; jj < w; jj++) {
+        const int j = c0 + jj;
+        PANEL2_MARK(j, 0);
+        // wait for wave 1's step flag: winner, its row, 1 / pivot
         for (long it = 0; it < SPIN_MAX && *s_step != j; it++) {}
         PANEL2_MARK(j, 2);
         const int p = *s_p, pw = *s_pw;
         float pr[32];
 #pragma unroll
-        for (int q = 0; q < 32; q++) pr[q] = prow[q];
-        const float rinv = *s_rinv;
-        lds_wait();
-        const float piv = pr[0];
-        const bool mine_win = (pw == g) && (lane == blane);
-        // the pivot row's remaining sub-block values are final now: store them (single lane)
-        if (mine_win) {
-          const int qmax = min(32 - jj, kb - j);            // columns j .. j+qmax-1 exist in this sub-block
-          if (bsel) {
-#pragma unroll
-            for (int q = 0; q < 32; q++) if (q < qmax) P[(j + q) * P2_PS + lane + 64] = x1[q];
-            pos1 = j; ps1 = jj; piv_local[jj] = lane + 64;
-          } else {
-#pragma unroll
-            for (int q = 0; q < 32; q++) if (q < qmax) P[(j + q) * P2_PS + lane] = x0[q];
-            pos0 = j; ps0 = jj; piv_local[jj] = lane;
-          }
+        for (int q = 0; q < 8; q++) {
+          const float4 v4 = ((const float4*)prow)[q];
+          pr[4 * q] = v4.x; pr[4 * q + 1] = v4.y; pr[4 * q + 2] = v4.z; pr[4 * q + 3] = v4.w;
         }
-        if (!(mine_win && !bsel) && v0 && pos0 == j) pos0 = p;
-        if (!(mine_win && bsel) && v1 && pos1 == j) pos1 = p;
+        const int ql = lane & 31;
+        const float prq = prow[min(ql + 1, 31)];          // pivot-row value this lane contributes to the next row
+        const float rinv = *s_rinv;
+        lds_wait();                                       // prow is rewritten by wave 1 once the next heads are in
+        const float piv = pr[0];
+        if (pw == g) {
+          // this workgroup's candidate won: its remaining sub-block values (staged in crow) are final
+          const int rw = blane + (bsel ? 64 : 0);
+          const int qmax = min(32 - jj, kb - j);          // columns j .. j+qmax-1 exist in this sub-block
+          if (lane < qmax) P[(j + lane) * P2_PS + rw] = crow[lane];
+          if (lane == blane) { if (bsel) { pos1 = j; ps1 = jj; } else { pos0 = j; ps0 = jj; } }
+          if (lane == 0) piv_local[jj] = rw;
+        }
+        const bool win0 = (pw == g) && lane == blane && !bsel, win1 = (pw == g) && lane == blane && bsel;
+        if (!win0 && v0 && pos0 == j) pos0 = p;
+        if (!win1 && v1 && pos1 == j) pos1 = p;
         // rows still in play (position > j) take the multiplier; pivoted rows are finished and their
         // registers are no longer stored.  A zero pivot leaves every row unchanged.
+        const bool a0 = piv != 0.0f && v0 && pos0 > j, a1 = piv != 0.0f && v1 && pos1 > j;
+        const bool normal = fabsf(piv) >= 1.17549435e-38f;
+        const float l0 = normal ? x0[0] * rinv : x0[0] / piv;
+        const float l1 = normal ? x1[0] * rinv : x1[0] / piv;
+        if (jj + 1 < w) {
+          publish_head(j + 1, a0 ? fmaf(-l0, pr[1], x0[1]) : x0[1], a1 ? fmaf(-l1, pr[1], x1[1]) : x1[1]);
+          if (blane >= 0) {
+            // the new candidate's row after this step's update, one value per lane from its current values
+            // (same operands as the full update below), published right behind the head
+            if (lane == blane) {
+              float4* c4 = (float4*)crow;
+              if (bsel) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) c4[q] = make_float4(x1[4 * q], x1[4 * q + 1], x1[4 * q + 2], x1[4 * q + 3]);
+              } else {
+#pragma unroll
+                for (int q = 0; q < 8; q++) c4[q] = make_float4(x0[4 * q], x0[4 * q + 1], x0[4 * q + 2], x0[4 * q + 3]);
+              }
+            }
+            const float lc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bsel ? l1 : l0), blane));
+            lds_wait();
+            const float xo = crow[min(ql + 1, 31)];
+            float nv = (piv != 0.0f) ? fmaf(-lc, prq, xo) : xo;
+            if (ql == 31) nv = 0.0f;
+            lds_wait();                                   // every lane's read is done before crow is overwritten
+            if (lane < 32) crow[ql] = nv;                 // the candidate's row as the pivot row of step j + 1
+            const uint32_t tag1 = (uint32_t)(a.k + j + 2);
+            const int par1 = (j + 1) & 1, rep = lane >> 5;
+            put_word(rowsl + (((size_t)(par1 * P2_NREP + rep) * P2_MAXG + g) * 32 + ql), tag1, __float_as_uint(nv));
+            put_word(rowsl + (((size_t)(par1 * P2_NREP + rep + 2) * P2_MAXG + g) * 32 + ql), tag1, __float_as_uint(nv));
+          }
+        }
+        PANEL2_MARK(j, 1);
         if (piv != 0.0f) {
-          const bool a0 = v0 && pos0 > j, a1 = v1 && pos1 > j;
-          const bool normal = fabsf(piv) >= 1.17549435e-38f;
-          float l0 = normal ? x0[0] * rinv : x0[0] / piv;
-          float l1 = normal ? x1[0] * rinv : x1[0] / piv;
           if (a0) P[j * P2_PS + lane] = l0;
           if (a1) P[j * P2_PS + lane + 64] = l1;
-          l0 = a0 ? l0 : 0.0f; l1 = a1 ? l1 : 0.0f;
+          const float m0 = a0 ? l0 : 0.0f, m1 = a1 ? l1 : 0.0f;
 #pragma unroll
-          for (int q = 1; q < 32; q++) { x0[q - 1] = fmaf(-l0, pr[q], x0[q]); x1[q - 1] = fmaf(-l1, pr[q], x1[q]); }
+          for (int q = 1; q < 32; q++) { x0[q - 1] = fmaf(-m0, pr[q], x0[q]); x1[q - 1] = fmaf(-m1, pr[q], x1[q]); }
         } else {
           if (v0 && pos0 > j) P[j * P2_PS + lane] = x0[0];
           if (v1 && pos1 > j) P[j * P2_PS + lane + 64] = x1[0];
@@ -612,7 +651,7 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
           for (int q = 1; q < 32; q++) { x0[q - 1] = x0[q]; x1[q - 1] = x1[q]; }
         }
         x0[31] = 0.0f; x1[31] = 0.0f;
-        lds_wait();   // the prow reads of this step are complete before the next publication
+        lds_wait();
         PANEL2_MARK(j, 3);
       }
     } else {

@@ -1303,7 +1303,7 @@ static void pack_buffers(int kb, int64_t m2, Workspace& W) {
 
 // L21 (rows k+kb .. n-1 of the panel columns) into row tiles 0 ..
 template <int FMT>
-static void pack_L(float* A, int64_t lda, int64_t k, int kb, int64_t m2, bool tile_scale, Workspace& W) {
+static void pack_L(float* A, int64_t lda, int64_t k, int kb, int64_t m2, bool tile_scale, Workspace& W, const char* pname) {
   constexpr int ES = FmtInfo<FMT>::ES;
   const int nkb = (kb + 15) / 16;
   const int nT = (int)((cdiv(m2, 128)) * 4);
@@ -1313,13 +1313,14 @@ static void pack_L(float* A, int64_t lda, int64_t k, int kb, int64_t m2, bool ti
                      A + (k + kb) + k * lda, (int64_t)1, lda, (int)m2, kb, nkb, W.packL.p,
                      FMT == FMT_FP16X3 ? W.packLlo.p : nullptr, W.scaleL.p, W.cnt.p, tile_scale ? 1 : 0);
   LAUNCH_CK();
-  W.prof_end(evp, "pack", (4.0 + ES) * (double)m2 * kb, 0.0);
+  W.prof_end(evp, pname, (4.0 + ES) * (double)m2 * kb, 0.0);
 }
 
 // U12 columns [cb, ce) (cb - (k + kb) a multiple of 32) into column tiles starting at (cb - k - kb) / 32;
 // the tiles up to the next multiple of four past ce are written too (zero padding of the last workgroup tile)
 template <int FMT>
-static void pack_U(float* A, int64_t lda, int64_t k, int kb, int64_t cb, int64_t ce, bool last, bool tile_scale, Workspace& W) {
+static void pack_U(float* A, int64_t lda, int64_t k, int kb, int64_t cb, int64_t ce, bool last, bool tile_scale, Workspace& W,
+                   const char* pname) {
   constexpr int ES = FmtInfo<FMT>::ES;
   const int nkb = (kb + 15) / 16;
   const int64_t t0 = (cb - k - kb) / 32;
@@ -1332,12 +1333,12 @@ static void pack_U(float* A, int64_t lda, int64_t k, int kb, int64_t cb, int64_t
                      A + k + cb * lda, lda, (int64_t)1, (int)nidx, kb, nkb, W.packU.p + t0 * tb,
                      FMT == FMT_FP16X3 ? W.packUlo.p + t0 * tb : nullptr, W.scaleU.p + t0, W.cnt.p, tile_scale ? 1 : 0);
   LAUNCH_CK();
-  W.prof_end(evp, "pack", (4.0 + ES) * (double)nidx * kb, 0.0);
+  W.prof_end(evp, pname, (4.0 + ES) * (double)nidx * kb, 0.0);
 }
 
 // A22[:, cb .. ce) -= L21 * U12[:, cb .. ce)
 template <int FMT>
-static void gemm_cols(float* A, int64_t lda, int64_t k, int kb, int64_t m2, int64_t cb, int64_t ce, Workspace& W) {
+static void gemm_cols(float* A, int64_t lda, int64_t k, int kb, int64_t m2, int64_t cb, int64_t ce, Workspace& W, const char* gname) {
   constexpr int ES = FmtInfo<FMT>::ES, FB = FmtInfo<FMT>::FB;
   const int nkb = (kb + 15) / 16;
   const int64_t t0 = (cb - k - kb) / 32, n2 = ce - cb;
@@ -1351,15 +1352,17 @@ static void gemm_cols(float* A, int64_t lda, int64_t k, int kb, int64_t m2, int6
                      FMT == FMT_FP16X3 ? W.packLlo.p : nullptr, FMT == FMT_FP16X3 ? W.packUlo.p + t0 * tb : nullptr,
                      W.scaleL.p, W.scaleU.p + t0);
   LAUNCH_CK();
-  W.prof_end(ev, "trailing_gemm", gb, gf);
+  W.prof_end(ev, gname, gb, gf);
 }
 
 template <int FMT>
 static void update_cols(float* A, int64_t lda, int64_t k, int kb, int64_t m2, int64_t cb, int64_t ce, bool last,
                         bool with_L, bool tile_scale, Workspace& W) {
-  if (with_L) { pack_buffers<FMT>(kb, m2, W); pack_L<FMT>(A, lda, k, kb, m2, tile_scale, W); }
-  pack_U<FMT>(A, lda, k, kb, cb, ce, last, tile_scale, W);
-  gemm_cols<FMT>(A, lda, k, kb, m2, cb, ce, W);
+  // the next panel's columns (look-ahead) are profiled under their own names
+  const bool nxt = with_L && !last;
+  if (with_L) { pack_buffers<FMT>(kb, m2, W); pack_L<FMT>(A, lda, k, kb, m2, tile_scale, W, nxt ? "pack_next" : "pack"); }
+  pack_U<FMT>(A, lda, k, kb, cb, ce, last, tile_scale, W, nxt ? "pack_next" : "pack");
+  gemm_cols<FMT>(A, lda, k, kb, m2, cb, ce, W, nxt ? "gemm_next" : "trailing_gemm");
 }
 
 static void update_cols_fmt(int fmt, float* A, int64_t lda, int64_t k, int kb, int64_t m2, int64_t cb, int64_t ce,
@@ -1433,17 +1436,18 @@ static void run_panel(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int32
 }
 
 static void laswp_ranges(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int64_t ra0, int64_t ra1,
-                         int64_t rb0, int64_t rb1, bool perm, const int32_t* plan) {
+                         int64_t rb0, int64_t rb1, bool perm, const int32_t* plan, const char* pname = "laswp") {
   const int64_t ncols = (ra1 - ra0) + (rb1 - rb0) + (perm ? 1 : 0);
   if (ncols <= 0) return;
   hipEvent_t e1 = W.prof_begin();
   hipLaunchKernelGGL(k_laswp, dim3((unsigned)std::min<int64_t>(ncols, 4096)), dim3(256), 0, W.stream,
                      a.A, a.lda, k, ra0, ra1, rb0, rb1, perm ? 1 : 0, plan, a.perm);
   LAUNCH_CK();
-  W.prof_end(e1, "laswp", 16.0 * (double)(ncols - (perm ? 1 : 0)) * kb, 0.0);
+  W.prof_end(e1, pname, 16.0 * (double)(ncols - (perm ? 1 : 0)) * kb, 0.0);
 }
 
-static void trsm_cols(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int64_t cb, int64_t ce, bool fma) {
+static void trsm_cols(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int64_t cb, int64_t ce, bool fma,
+                      const char* pname = "trsm") {
   if (ce <= cb) return;
   hipEvent_t e2 = W.prof_begin();
   if (fma) {
@@ -1455,7 +1459,7 @@ static void trsm_cols(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int64
                        a.A, a.lda, cb, ce, k, kb, W.linv.p);
   }
   LAUNCH_CK();
-  W.prof_end(e2, "trsm", 8.0 * (double)kb * (ce - cb), (double)kb * kb * (ce - cb));
+  W.prof_end(e2, pname, 8.0 * (double)kb * (ce - cb), (double)kb * kb * (ce - cb));
 }
 
 // Right-looking blocked LU.  With look-ahead (default), step k first brings the next panel's columns up to
@@ -1511,7 +1515,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
     const int64_t cN = k + kb + kbn;                            // end of the next panel's columns
     const bool split = overlap && (kbn % 128) == 0 && m2 > kbn && panel2_usable(a, W, k + kb, kbn, allow_p2);
     if (split) {
-      laswp_ranges(a, W, k, kb, k + kb, cN, 0, 0, false, plan);
+      laswp_ranges(a, W, k, kb, k + kb, cN, 0, 0, false, plan, "laswp_next");
     } else {
       laswp_ranges(a, W, k, kb, 0, k, k + kb, n, true, plan);
     }
@@ -1519,10 +1523,10 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
       hipEvent_t e2 = W.prof_begin();
       hipLaunchKernelGGL(k_trsm_inv, dim3(cdiv(kb, 32)), dim3(64), 0, W.stream, a.A, a.lda, k, kb, W.linv.p);
       LAUNCH_CK();
-      W.prof_end(e2, "trsm", 0.0, 0.0);
+      W.prof_end(e2, "trsm_inv", 0.0, 0.0);
     }
     if (split) {
-      trsm_cols(a, W, k, kb, k + kb, cN, false);
+      trsm_cols(a, W, k, kb, k + kb, cN, false, "trsm_next");
       update_cols_fmt(a.fmt, a.A, a.lda, k, kb, m2, k + kb, cN, false, true, a.tile_scale, W);
       HIP_CK(hipEventRecord(W.ev_next, W.stream));
       HIP_CK(hipStreamWaitEvent(W.pstream, W.ev_next, 0));

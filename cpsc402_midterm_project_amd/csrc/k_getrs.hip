@@ -6,13 +6,16 @@
 //
 // Blocked by 128 rows.  The 128 x 128 diagonal blocks of L (unit lower) and U are inverted once per
 // factorization (k_trinv, one workgroup per block, FP32 substitution on the columns of the identity), so the
-// diagonal step of the solve is a parallel matrix-vector product.  k_trsv2: one workgroup of 512 threads
-// per block row.  It streams the off-diagonal blocks of its block row in the order their solution blocks
-// are produced, with the next block's loads in flight while the current one is applied; the solution
-// blocks are exchanged as tagged 64-bit words (FP32 value + launch tag), so a reader needs no flag and no
-// fence.  Only the immediately preceding block (preloaded into registers) and the inverse product sit on
-// the dependency chain between consecutive block rows.  Lower-numbered workgroups never wait for
-// higher-numbered ones, so in-order dispatch guarantees progress.
+// diagonal step of the solve is a parallel matrix-vector product, and so are the chain matrices
+// M_i = T_i^{-1} B_i (B_i the off-diagonal block next to the diagonal, k_chainmat): block i of the solution
+// is x_i = c_i - M_i x_prev with c_i = T_i^{-1} (r_i - sum of the other off-diagonal blocks' products), and
+// c_i does not depend on x_prev.  k_trsv2: one workgroup of 512 threads per block row.  It streams the
+// off-diagonal blocks of its block row in the order their solution blocks are produced, with the next
+// block's loads in flight while the current one is applied, and forms c_i; the solution blocks are
+// exchanged as tagged 64-bit words (FP32 value + launch tag), so a reader needs no flag and no fence.  Only
+// one 128 x 128 matrix-vector product (M_i, preloaded into registers) sits on the dependency chain between
+// consecutive block rows.  Lower-numbered workgroups never wait for higher-numbered ones, so in-order
+// dispatch guarantees progress.
 #include "mp_internal.hpp"
 #include "mp_kernels.hpp"
 
@@ -36,12 +39,12 @@ __device__ __forceinline__ uint64_t gs_load(const uint64_t* p) {
 }
 // dist = how many chain steps before this workgroup needs the block (0: it is on the dependency chain).
 // Every workgroup waiting at the frontier polls the same few cache lines, so the poll interval grows with
-// dist: only the next one or two block rows poll eagerly.
+// dist, but only slowly: a workgroup that oversleeps has to apply several blocks in a row once it wakes
+// (about 0.5 us each), and falling behind the chain that way costs more than the extra polling.
 __device__ __forceinline__ void gs_sleep(int dist) {
-  if (dist <= 0) __builtin_amdgcn_s_sleep(1);
-  else if (dist == 1) __builtin_amdgcn_s_sleep(4);
-  else if (dist < 4) __builtin_amdgcn_s_sleep(32);
-  else __builtin_amdgcn_s_sleep(127);
+  if (dist <= 1) __builtin_amdgcn_s_sleep(1);
+  else if (dist < 8) __builtin_amdgcn_s_sleep(2);
+  else __builtin_amdgcn_s_sleep(8);
 }
 // The word loaded ahead (`first`) is checked before the polling loop: the loop re-loads, and at its head the
 // compiler can only wait for every outstanding load, which would also wait for the prefetch in flight.
@@ -107,9 +110,62 @@ __global__ __launch_bounds__(128) void k_trinv(const float* __restrict__ LU, int
 
 // LOWER: y = Linv_i (P r)_i - sum_{j<i} L_ij y_j, output Y (FP32).   UPPER: x_i = Uinv_i (y_i - sum_{j>i} U_ij x_j),
 // output D = scale * x (FP64).  Thread t: row r = t % 128 of the block, quarter qq = t / 128 of the columns.
+// Chain matrix of block i: LOWER M_i = Linv_i L_{i,i-1} (i >= 1), UPPER M_i = Uinv_i U_{i,i+1} (i <= nbk-2);
+// zero for the first block in production order.  Output column-major per block like Tinv; rows and columns
+// beyond n are zero.  Thread (tr, tc) forms an 8 x 8 tile, each element summed over k in increasing order.
+template <bool LOWER>
+__global__ __launch_bounds__(256) void k_chainmat(const float* __restrict__ LU, int64_t lda, int64_t n,
+                                                  const float* __restrict__ Tinv, float* __restrict__ M) {
+  extern __shared__ float sm[];
+  float* Ts = sm;                     // [128][129]: Ts[row * 129 + k] = Tinv_i[row][k]
+  float* Bs = sm + GB * (GB + 1);     // [128][129]: Bs[k * 129 + col] = B_i[k][col]
+  const int nbk = (int)((n + GB - 1) / GB);
+  const int i = blockIdx.x, t = threadIdx.x;
+  const int j = LOWER ? i - 1 : i + 1;
+  float* Mo = M + (size_t)i * GB * GB;
+  if (j < 0 || j >= nbk) {
+    for (int idx = t; idx < GB * GB; idx += 256) Mo[idx] = 0.0f;
+    return;
+  }
+  const int64_t ro = (int64_t)i * GB, co = (int64_t)j * GB;
+  for (int idx = t; idx < GB * GB; idx += 256) {
+    const int col = idx >> 7, row = idx & (GB - 1);   // coalesced over rows
+    Ts[row * (GB + 1) + col] = Tinv[(size_t)i * GB * GB + idx];
+    float v = 0.0f;
+    if (ro + row < n && co + col < n) v = LU[(co + col) * lda + ro + row];
+    Bs[row * (GB + 1) + col] = v;
+  }
+  __syncthreads();
+  const int tr = t >> 4, tc = t & 15;
+  float acc[8][8];
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+#pragma unroll
+    for (int p = 0; p < 8; p++) acc[q][p] = 0.0f;
+  for (int k = 0; k < GB; k++) {
+    float av[8], bv[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) av[q] = Ts[(tr * 8 + q) * (GB + 1) + k];
+#pragma unroll
+    for (int p = 0; p < 8; p++) bv[p] = Bs[k * (GB + 1) + tc * 8 + p];
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+#pragma unroll
+      for (int p = 0; p < 8; p++) acc[q][p] = fmaf(av[q], bv[p], acc[q][p]);
+  }
+#pragma unroll
+  for (int p = 0; p < 8; p++)
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int row = tr * 8 + q, col = tc * 8 + p;
+      Mo[(size_t)col * GB + row] = (ro + row < n) ? acc[q][p] : 0.0f;
+    }
+}
+
 template <int NR, bool LOWER>
 __global__ __launch_bounds__(512) void k_trsv2(const float* __restrict__ LU, int64_t lda, int64_t n,
-                                               const float* __restrict__ Tinv, const int32_t* __restrict__ perm,
+                                               const float* __restrict__ Tinv, const float* __restrict__ Mch,
+                                               const int32_t* __restrict__ perm,
                                                const double* __restrict__ Rin, int64_t ldr,
                                                float* __restrict__ Y, int64_t ldy,
                                                double* __restrict__ D, int64_t ldd, double scale,
@@ -139,15 +195,12 @@ __global__ __launch_bounds__(512) void k_trsv2(const float* __restrict__ LU, int
       v[k] = GS_LOAD(base + (col < n ? col : n - 1) * lda);
     }
   };
-  // preloads: inverse block row, the chain block, the right-hand side
-  float Tv[32], Lt[32], rhs[NR];
+  // preloads: inverse block row, chain matrix row, the right-hand side
+  float Tv[32], Mt[32], rhs[NR];
 #pragma unroll
   for (int k = 0; k < 32; k++) Tv[k] = Tinv[(size_t)bi * GB * GB + (size_t)(qq * 32 + k) * GB + r];
-  if (ndep > 0) ldL(blk(ndep - 1), Lt);
-  else {
 #pragma unroll
-    for (int k = 0; k < 32; k++) Lt[k] = 0.0f;
-  }
+  for (int k = 0; k < 32; k++) Mt[k] = Mch[(size_t)bi * GB * GB + (size_t)(qq * 32 + k) * GB + r];
 #pragma unroll
   for (int c = 0; c < NR; c++) {
     rhs[c] = 0.0f;
@@ -190,31 +243,15 @@ __global__ __launch_bounds__(512) void k_trsv2(const float* __restrict__ LU, int
     step(s + 1, Lb, yb);
   }
   GS_MARK(0);
-  // the chain block
-  if (ndep > 0) {
-    const int bj = blk(ndep - 1);
-    if (t < GB) {
-#pragma unroll
-      for (int c = 0; c < NR; c++) {
-        const uint64_t* p = ybuf + ((int64_t)bj * NR + c) * GB + t;
-        ys[nbulk % 3][c][t] = gs_wait(p, gs_load(p), tag, LDS_FLAG(&s_abort), 0);
-      }
-    }
-    __syncthreads();
-    GS_MARK(1);
-#pragma unroll
-    for (int k = 0; k < 32; k++)
-#pragma unroll
-      for (int c = 0; c < NR; c++) acc[c] = fmaf(Lt[k], ys[nbulk % 3][c][qq * 32 + k], acc[c]);
-  }
+  // off the chain: b = rhs - (bulk sums), c = Tinv b
+  const int sb = (nbulk + 1) % 3, sc = nbulk % 3;     // slots not read by the last two bulk steps
 #pragma unroll
   for (int c = 0; c < NR; c++) part[qq][c][r] = acc[c];
   __syncthreads();
-  // b = rhs - sum, staged in ys[0]
   if (qq == 0) {
 #pragma unroll
     for (int c = 0; c < NR; c++)
-      ys[(nbulk + 1) % 3][c][r] = rhs[c] - ((part[0][c][r] + part[1][c][r]) + (part[2][c][r] + part[3][c][r]));
+      ys[sb][c][r] = rhs[c] - ((part[0][c][r] + part[1][c][r]) + (part[2][c][r] + part[3][c][r]));
   }
   __syncthreads();
   float px[NR];
@@ -223,15 +260,43 @@ __global__ __launch_bounds__(512) void k_trsv2(const float* __restrict__ LU, int
 #pragma unroll
   for (int k = 0; k < 32; k++)
 #pragma unroll
-    for (int c = 0; c < NR; c++) px[c] = fmaf(Tv[k], ys[(nbulk + 1) % 3][c][qq * 32 + k], px[c]);
+    for (int c = 0; c < NR; c++) px[c] = fmaf(Tv[k], ys[sb][c][qq * 32 + k], px[c]);
   // part was last read before the previous barrier
 #pragma unroll
   for (int c = 0; c < NR; c++) part[qq][c][r] = px[c];
   __syncthreads();
+  float xv[NR];
+#pragma unroll
+  for (int c = 0; c < NR; c++) xv[c] = (part[0][c][r] + part[1][c][r]) + (part[2][c][r] + part[3][c][r]);
+  // the chain: x = c - M x_prev
+  if (ndep > 0) {
+    const int bj = blk(ndep - 1);
+    if (t < GB) {
+#pragma unroll
+      for (int c = 0; c < NR; c++) {
+        const uint64_t* p = ybuf + ((int64_t)bj * NR + c) * GB + t;
+        ys[sc][c][t] = gs_wait(p, gs_load(p), tag, LDS_FLAG(&s_abort), 0);
+      }
+    }
+    __syncthreads();                                  // also orders the reads of part above before the writes below
+    GS_MARK(1);
+    float d[NR];
+#pragma unroll
+    for (int c = 0; c < NR; c++) d[c] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 32; k++)
+#pragma unroll
+      for (int c = 0; c < NR; c++) d[c] = fmaf(Mt[k], ys[sc][c][qq * 32 + k], d[c]);
+#pragma unroll
+    for (int c = 0; c < NR; c++) part[qq][c][r] = d[c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NR; c++) xv[c] -= (part[0][c][r] + part[1][c][r]) + (part[2][c][r] + part[3][c][r]);
+  }
   if (qq == 0) {
 #pragma unroll
     for (int c = 0; c < NR; c++) {
-      const float x = (part[0][c][r] + part[1][c][r]) + (part[2][c][r] + part[3][c][r]);
+      const float x = xv[c];
       gs_put(ybuf + ((int64_t)bi * NR + c) * GB + r, tag, rvalid ? x : 0.0f);
       if (rvalid) {
         if (LOWER) Y[(int64_t)c * ldy + row] = x;
@@ -250,11 +315,13 @@ static void trsv_pair(int64_t n, const float* LU, int64_t lda, const int32_t* pe
   const uint32_t tl = W.take_tags(1), tu = W.take_tags(1);
   const float* Ti = W.tinv.p;
   const float* Tu = W.tinv.p + (size_t)nbk * GB * GB;
+  const float* Mi = W.tchain.p;
+  const float* Mu = W.tchain.p + (size_t)nbk * GB * GB;
   hipEvent_t ev = W.prof_begin();
-  hipLaunchKernelGGL((k_trsv2<NR, true>), dim3(nbk), dim3(512), 0, W.stream, LU, lda, n, Ti, perm, R, ldr,
+  hipLaunchKernelGGL((k_trsv2<NR, true>), dim3(nbk), dim3(512), 0, W.stream, LU, lda, n, Ti, Mi, perm, R, ldr,
                      W.rhs32.p, n, (double*)nullptr, (int64_t)0, 1.0, W.ybuf.p, tl, err);
   LAUNCH_CK();
-  hipLaunchKernelGGL((k_trsv2<NR, false>), dim3(nbk), dim3(512), 0, W.stream, LU, lda, n, Tu, perm, (const double*)nullptr, (int64_t)0,
+  hipLaunchKernelGGL((k_trsv2<NR, false>), dim3(nbk), dim3(512), 0, W.stream, LU, lda, n, Tu, Mu, perm, (const double*)nullptr, (int64_t)0,
                      W.rhs32.p, n, D, ldd, scale, W.ybuf.p, tu, err);
   LAUNCH_CK();
   W.prof_end(ev, "getrs", 4.0 * (double)n * n + 24.0 * n * NR, 2.0 * (double)n * n * NR);
@@ -270,12 +337,16 @@ void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* per
   W.rhs32.ensure(*W.tracker, (size_t)n * 4, "getrs_y32");
   W.dinfo.ensure(*W.tracker, 4, "getrf_info");
   W.tinv.ensure(*W.tracker, (size_t)2 * nbk * GB * GB, "getrs_block_inverses");
+  W.tchain.ensure(*W.tracker, (size_t)2 * nbk * GB * GB, "getrs_chain_matrices");
   if (!reuse_inverse || W.tinv_lu != LU || W.tinv_n != n || W.tinv_lda != lda || W.tinv_gen != W.factor_gen) {
     static bool attr = false;
     const size_t lds = (size_t)(GB * (GB + 1) + GB * GB) * 4;
+    const size_t lds_cm = (size_t)2 * GB * (GB + 1) * 4;
     if (!attr) {
       HIP_CK(hipFuncSetAttribute((const void*)k_trinv<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       HIP_CK(hipFuncSetAttribute((const void*)k_trinv<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      HIP_CK(hipFuncSetAttribute((const void*)k_chainmat<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cm));
+      HIP_CK(hipFuncSetAttribute((const void*)k_chainmat<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cm));
       attr = true;
     }
     hipEvent_t ev = W.prof_begin();
@@ -283,7 +354,12 @@ void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* per
     LAUNCH_CK();
     hipLaunchKernelGGL(k_trinv<false>, dim3((unsigned)nbk), dim3(GB), lds, W.stream, LU, lda, n, W.tinv.p + (size_t)nbk * GB * GB);
     LAUNCH_CK();
-    W.prof_end(ev, "getrs_block_inverse", 8.0 * (double)nbk * GB * GB, 2.0 * (double)nbk * GB * GB * GB / 3.0);
+    hipLaunchKernelGGL(k_chainmat<true>, dim3((unsigned)nbk), dim3(256), lds_cm, W.stream, LU, lda, n, W.tinv.p, W.tchain.p);
+    LAUNCH_CK();
+    hipLaunchKernelGGL(k_chainmat<false>, dim3((unsigned)nbk), dim3(256), lds_cm, W.stream, LU, lda, n,
+                       W.tinv.p + (size_t)nbk * GB * GB, W.tchain.p + (size_t)nbk * GB * GB);
+    LAUNCH_CK();
+    W.prof_end(ev, "getrs_block_inverse", 16.0 * (double)nbk * GB * GB, 2.0 * (double)nbk * GB * GB * GB * (1.0 / 3.0 + 2.0));
     W.tinv_lu = LU; W.tinv_n = n; W.tinv_lda = lda; W.tinv_gen = W.factor_gen;
   }
   int c = 0;

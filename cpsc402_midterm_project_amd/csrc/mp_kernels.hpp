@@ -51,6 +51,7 @@ struct Workspace {
   DBuf<uint64_t> ybuf;   // tagged partial-solution words of the triangular solves
   DBuf<float> rhs32;
   DBuf<float> tinv;      // inverses of the 128 x 128 diagonal blocks of L and U
+  DBuf<float> tchain;    // chain matrices Linv_i L_{i,i-1} and Uinv_i U_{i,i+1} of the blocked substitution
   const float* tinv_lu = nullptr; int64_t tinv_n = 0, tinv_lda = 0; uint64_t tinv_gen = 0;
   uint64_t factor_gen = 1;   // incremented by every factorization written through getrf
   uint32_t tag = 1;      // next free exchange tag (never 0)

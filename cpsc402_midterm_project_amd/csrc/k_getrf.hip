@@ -407,6 +407,7 @@ __device__ __forceinline__ uint64_t head_word(uint32_t tag, int pos, uint32_t vb
   return ((uint64_t)(tag & 0x1FFFFu) << 47) | ((uint64_t)(pos & 0xFFFF) << 31) | (uint64_t)(vbits & 0x7FFFFFFFu);
 }
 
+__device__ __forceinline__ void p2_subst(const float* LsT, int zmask, float (&u)[32]);
 // Poll the 32 U-row words of one trailing column (stride 256 words) and turn them into the final U values
 // by forward substitution, right-looking so the 31 updates per step are independent (each u_q still
 // receives its updates in increasing d).  LsT[d][q] = l_{q,d}; steps whose pivot was zero are skipped.
@@ -432,9 +433,16 @@ __device__ __forceinline__ void p2_u_column(const uint64_t* xs, uint32_t stag, c
       for (int i = 0; i < 16; i++) if (need >> i & 1u) u[q16 + i] = 0.0f;
     }
   }
+  p2_subst(LsT, zmask, u);
+}
+// Forward substitution of one U column (u holds the pivot rows' values in the column, in step order),
+// right-looking so the updates of one step are independent (each u_q still receives its updates in
+// increasing d); steps whose pivot was zero leave the column unchanged.  No branch depends on the data, so
+// the LsT reads of later steps can be issued ahead of the arithmetic.
+__device__ __forceinline__ void p2_subst(const float* LsT, int zmask, float (&u)[32]) {
 #pragma unroll
   for (int d = 0; d < 31; d++) {
-    if (zmask >> d & 1) continue;
+    const bool skip = (zmask >> d & 1) != 0;
     const float4* l4 = (const float4*)(LsT + d * 32);
 #pragma unroll
     for (int q4 = 0; q4 < 8; q4++) {
@@ -444,7 +452,7 @@ __device__ __forceinline__ void p2_u_column(const uint64_t* xs, uint32_t stag, c
 #pragma unroll
       for (int e = 0; e < 4; e++) {
         const int q = 4 * q4 + e;
-        if (q > d) u[q] = fmaf(-le[e], u[d], u[q]);
+        if (q > d) { const float nv = fmaf(-le[e], u[d], u[q]); u[q] = skip ? u[q] : nv; }
       }
     }
   }
@@ -726,6 +734,7 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
     const int zmask = *s_zmask;
     uint64_t* xsp = a.xbuf + P2_OFF_SUB + (int64_t)((c0 >> 5) & 1) * P2_NREP * P2_SUBREP;
     if (t < P2_R) pstep_bg[t] = pstep[t];
+    const int nw = min(32, rem);                          // columns updated now
     if (wave >= 2) {
       // the owners publish their pivot rows (columns [c0, kb)) to every replica
       const int tq = t - 128;
@@ -739,51 +748,102 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
           }
       }
     } else {
-      // LsT[d][q] = l_{q,d}: sub-block values of pivot row q, waves 0-1
+      // waves 0-1, one polling round: LsT[d][q] = l_{q,d} (pivot row q's values in the sub-block's columns) and
+      // the raw U words of the next nw columns (Us[col][q], substituted below)
       const uint64_t* xs = xsp + myrep * P2_SUBREP;
-      uint32_t vals[8];
-      uint32_t need = 0xFFu;
-      for (long it = 0; it < SPIN_MAX && need; it++) {
-        uint64_t wv[8];
+      uint32_t vals[16];
+      uint32_t need = 0xFFFFu;
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-          const int idx = t + P2_R * i, q = idx >> 5, d = idx & 31;
-          wv[i] = (need >> i & 1u) ? __hip_atomic_load(xs + q * 256 + c0 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+      for (int i = 8; i < 16; i++) if (((t + P2_R * (i - 8)) & 31) >= nw) { need &= ~(1u << i); vals[i] = 0u; }
+      for (long it = 0; it < SPIN_MAX && need; it++) {
+        uint64_t wv[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+          const int idx = t + P2_R * (i & 7), q = idx >> 5, d = idx & 31;
+          const int col = (i < 8) ? c0 + d : c0 + 32 + d;
+          wv[i] = (need >> i & 1u) ? __hip_atomic_load(xs + q * 256 + col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
         }
 #pragma unroll
-        for (int i = 0; i < 8; i++)
+        for (int i = 0; i < 16; i++)
           if ((need >> i & 1u) && (uint32_t)(wv[i] >> 32) == stag) { vals[i] = (uint32_t)wv[i]; need &= ~(1u << i); }
         if (!need || *s_abort) break;
         __builtin_amdgcn_s_sleep(1);
       }
-      if (need) { *s_abort = 1; for (int i = 0; i < 8; i++) if (need >> i & 1u) vals[i] = 0u; }
+      if (need) { *s_abort = 1; for (int i = 0; i < 16; i++) if (need >> i & 1u) vals[i] = 0u; }
 #pragma unroll
-      for (int i = 0; i < 8; i++) { const int idx = t + P2_R * i; LsT[(idx & 31) * 32 + (idx >> 5)] = __uint_as_float(vals[i]); }
+      for (int i = 0; i < 8; i++) {
+        const int idx = t + P2_R * i, q = idx >> 5, d = idx & 31;
+        LsT[d * 32 + q] = __uint_as_float(vals[i]);
+        Us[d * P2_US + q] = __uint_as_float(vals[i + 8]);
+      }
     }
     lds_barrier();
     PANEL2_MARK(c0 + 2, 7);
-    // next 32 columns: U rows by 32 threads of wave 0, then all 256 threads (column t & 31, rows 16 (t >> 5) ..)
-    const int nw = min(32, rem);
+    // next nw columns: forward substitution of the U rows (thread per column), in place
     if (t < nw) {
       float u[32];
-      p2_u_column(xsp + myrep * P2_SUBREP + c0 + 32 + t, stag, LsT, zmask, s_abort, u);
+      float4* u4 = (float4*)(Us + t * P2_US);
 #pragma unroll
-      for (int q = 0; q < 32; q++) Us[t * P2_US + q] = u[q];
+      for (int q4 = 0; q4 < 8; q4++) { const float4 v = u4[q4]; u[4 * q4] = v.x; u[4 * q4 + 1] = v.y; u[4 * q4 + 2] = v.z; u[4 * q4 + 3] = v.w; }
+      p2_subst(LsT, zmask, u);
+#pragma unroll
+      for (int q4 = 0; q4 < 8; q4++) u4[q4] = make_float4(u[4 * q4], u[4 * q4 + 1], u[4 * q4 + 2], u[4 * q4 + 3]);
     }
     lds_barrier();
     PANEL2_MARK(c0 + 3, 7);
     {
-      const int col = t & 31, rg = t >> 5;
-      if (col < nw) {
-        float u[32];
-        const float4* u4 = (const float4*)(Us + col * P2_US);
+      // thread per (row, half of the columns): the row's 32 multipliers stay in registers, the U columns are
+      // read as broadcasts; each element gets a -= l_d u_d in increasing d, as in p2_apply
+      const int r = t & (P2_R - 1), ch = t >> 7;
+      const uint32_t em = p2_emask(pstep[r], zmask);
+      if (em != 0u) {
+        float l[32];
 #pragma unroll
-        for (int q4 = 0; q4 < 8; q4++) { const float4 v = u4[q4]; u[4 * q4] = v.x; u[4 * q4 + 1] = v.y; u[4 * q4 + 2] = v.z; u[4 * q4 + 3] = v.w; }
-        const int c = c0 + 32 + col;
-        for (int r = rg * 16; r < rg * 16 + 16; r++) {
-          const uint32_t em = p2_emask(pstep[r], zmask);
-          if (em == 0u) continue;
-          P[c * P2_PS + r] = p2_apply(P[c * P2_PS + r], P, c0, r, u, em);
+        for (int d = 0; d < 32; d++) l[d] = P[(c0 + d) * P2_PS + r];
+        // four columns at a time: independent accumulation chains (columns past nw are clamped reads whose
+        // results are not stored)
+        for (int c4 = 0; c4 < 16; c4 += 4) {
+          if (ch * 16 + c4 >= nw) break;
+          float acc[4];
+          const float4* u4[4];
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const int col = min(ch * 16 + c4 + i, nw - 1);
+            acc[i] = P[(c0 + 32 + col) * P2_PS + r];
+            u4[i] = (const float4*)(Us + col * P2_US);
+          }
+          if (em == 0xFFFFFFFFu) {
+#pragma unroll
+            for (int q4 = 0; q4 < 8; q4++) {
+#pragma unroll
+              for (int i = 0; i < 4; i++) {
+                const float4 v = u4[i][q4];
+                acc[i] = fmaf(-l[4 * q4], v.x, acc[i]); acc[i] = fmaf(-l[4 * q4 + 1], v.y, acc[i]);
+                acc[i] = fmaf(-l[4 * q4 + 2], v.z, acc[i]); acc[i] = fmaf(-l[4 * q4 + 3], v.w, acc[i]);
+              }
+            }
+          } else {
+            // a row that became a pivot in this sub-block, or a step with a zero pivot
+#pragma unroll
+            for (int q4 = 0; q4 < 8; q4++) {
+#pragma unroll
+              for (int i = 0; i < 4; i++) {
+                const float4 v = u4[i][q4];
+                const float ue[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                  const int d = 4 * q4 + e;
+                  const float nv = fmaf(-l[d], ue[e], acc[i]);
+                  acc[i] = (em >> d & 1u) ? nv : acc[i];
+                }
+              }
+            }
+          }
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const int col = ch * 16 + c4 + i;
+            if (col < nw) P[(c0 + 32 + col) * P2_PS + r] = acc[i];
+          }
         }
       }
     }

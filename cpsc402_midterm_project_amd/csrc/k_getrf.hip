@@ -880,15 +880,52 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
   if (t == 0 && *s_abort) a.dinfo[1] = 1;
 }
 
+// Inverse of diagonal block b of L11 into Linv (column-major 32 x 32); threads 0..31 compute, all threads
+// of the block (nthr of them, a multiple of 32) help load and take part in the barrier.
+__device__ __forceinline__ void trsm_inv_block(const float* __restrict__ A, int64_t lda, int64_t k, int kb,
+                                               float* __restrict__ Linv, int b, int c, int nthr, float (*T)[33]) {
+  const int rb = 32 * b, rib = min(32, kb - rb);
+  for (int idx = c; idx < 32 * 32; idx += nthr) {
+    const int q = idx / 32, r = idx - q * 32;
+    T[r][q] = (r < rib && q < rib) ? A[(k + rb + r) + (k + rb + q) * lda] : 0.0f;
+  }
+  __syncthreads();
+  if (c < 32) {
+    float x[32];
+#pragma unroll
+    for (int r = 0; r < 32; r++) x[r] = (r == c) ? 1.0f : 0.0f;
+#pragma unroll
+    for (int r = 1; r < 32; r++) {
+      float acc = x[r];
+#pragma unroll
+      for (int q = 0; q < r; q++) if (q >= c) acc = fmaf(-T[r][q], x[q], acc);
+      if (r > c) x[r] = acc;
+    }
+#pragma unroll
+    for (int r = 0; r < 32; r++) Linv[(size_t)b * 1024 + c * 32 + r] = (r < rib && c < rib) ? x[r] : (r == c ? 1.0f : 0.0f);
+  }
+}
+
 // ------------------------------------------------------------------ row interchanges outside the panel
 // plan[0] = number of moved rows, then (dst, src) pairs of panel-local rows: new[dst] = old[src].
 // Columns [ra0, ra1) and [rb0, rb1) of A (rows k .. n-1), plus, with do_perm, the composed permutation vector
-// (one extra "column").  Workgroups stride over the columns.
+// (one extra "column").  Workgroups stride over the columns.  The first ninv workgroups instead invert the
+// diagonal blocks of L11 for the TRSM (L11's columns are not among the interchanged ones), which saves the
+// TRSM path a dependent launch.  zero_plan (optional): the swap-plan counter of the next panel, cleared here
+// (its previous readers precede this launch on the stream).
 __global__ __launch_bounds__(256) void k_laswp(float* __restrict__ A, int64_t lda, int64_t k,
                                                int64_t ra0, int64_t ra1, int64_t rb0, int64_t rb1, int do_perm,
-                                               const int32_t* __restrict__ plan, int32_t* __restrict__ perm) {
+                                               const int32_t* __restrict__ plan, int32_t* __restrict__ perm,
+                                               int kb, int ninv, float* __restrict__ Linv, int32_t* __restrict__ zero_plan) {
+  if (zero_plan != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *zero_plan = 0;
+  if ((int)blockIdx.x < ninv) {
+    __shared__ float T[32][33];
+    trsm_inv_block(A, lda, k, kb, Linv, blockIdx.x, threadIdx.x, 256, T);
+    return;
+  }
   const int cnt = plan[0];
   const int t = threadIdx.x;
+  const int bx = (int)blockIdx.x - ninv, gx = (int)gridDim.x - ninv;
   const int64_t na = ra1 - ra0, nb2 = rb1 - rb0, ncols = na + nb2;
   int dst[4], src[4];
 #pragma unroll
@@ -897,7 +934,7 @@ __global__ __launch_bounds__(256) void k_laswp(float* __restrict__ A, int64_t ld
     dst[q] = (e < cnt) ? plan[1 + 2 * e] : -1;
     src[q] = (e < cnt) ? plan[2 + 2 * e] : -1;
   }
-  for (int64_t cc = blockIdx.x; cc < ncols + do_perm; cc += gridDim.x) {
+  for (int64_t cc = bx; cc < ncols + do_perm; cc += gx) {
     if (cc == ncols) {
       int32_t* pp = perm + k;
       int32_t v[4];
@@ -932,27 +969,7 @@ __global__ void k_iota(int32_t* p, int64_t n) { const int64_t i = (int64_t)block
 //   X_r -= L_{r,b} X_b, r below  (acc = fmaf(-L_rq, X_q, acc) for q = 0..31)
 __global__ __launch_bounds__(64) void k_trsm_inv(const float* __restrict__ A, int64_t lda, int64_t k, int kb, float* __restrict__ Linv) {
   __shared__ float T[32][33];
-  const int b = blockIdx.x, c = threadIdx.x;
-  const int rb = 32 * b, rib = min(32, kb - rb);
-  for (int idx = c; idx < 32 * 32; idx += 64) {
-    const int q = idx / 32, r = idx - q * 32;
-    T[r][q] = (r < rib && q < rib) ? A[(k + rb + r) + (k + rb + q) * lda] : 0.0f;
-  }
-  __syncthreads();
-  if (c < 32) {
-    float x[32];
-#pragma unroll
-    for (int r = 0; r < 32; r++) x[r] = (r == c) ? 1.0f : 0.0f;
-#pragma unroll
-    for (int r = 1; r < 32; r++) {
-      float acc = x[r];
-#pragma unroll
-      for (int q = 0; q < r; q++) if (q >= c) acc = fmaf(-T[r][q], x[q], acc);
-      if (r > c) x[r] = acc;
-    }
-#pragma unroll
-    for (int r = 0; r < 32; r++) Linv[(size_t)b * 1024 + c * 32 + r] = (r < rib && c < rib) ? x[r] : (r == c ? 1.0f : 0.0f);
-  }
+  trsm_inv_block(A, lda, k, kb, Linv, blockIdx.x, threadIdx.x, 64, T);
 }
 
 __global__ __launch_bounds__(256) void k_trsm(float* __restrict__ A, int64_t lda, int64_t n, int64_t k, int kb,
@@ -1456,7 +1473,6 @@ static void run_panel(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int32
   const int64_t n = a.n;
   const int m = (int)(n - k);
   const int64_t G2 = cdiv(m, P2_R);
-  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, stream, plan, 1);
   if (panel2_usable(a, W, k, kb, allow_p2)) {
     Panel2Args pa;
     pa.A = a.A; pa.lda = a.lda; pa.k = k; pa.kb = kb; pa.m = m; pa.G = (int)G2;
@@ -1485,6 +1501,7 @@ static void run_panel(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int32
   MP_REQUIRE(R > 0, "getrf: panel does not fit on the device (n too large for this nb)");
   MP_REQUIRE(cooperative, "getrf: the fallback panel needs a cooperative launch");
   const int G = (int)cdiv(m, R);
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, stream, plan, 1);
   PanelArgs pa;
   pa.A = a.A; pa.lda = a.lda; pa.k = k; pa.kb = kb; pa.m = m; pa.G = G; pa.R = R;
   pa.xbuf = W.xbuf.p; pa.ipiv = a.ipiv; pa.dinfo = W.dinfo.p; pa.plan = plan;
@@ -1496,12 +1513,14 @@ static void run_panel(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int32
 }
 
 static void laswp_ranges(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int64_t ra0, int64_t ra1,
-                         int64_t rb0, int64_t rb1, bool perm, const int32_t* plan, const char* pname = "laswp") {
+                         int64_t rb0, int64_t rb1, bool perm, const int32_t* plan, bool with_inv,
+                         int32_t* zero_plan, const char* pname = "laswp") {
   const int64_t ncols = (ra1 - ra0) + (rb1 - rb0) + (perm ? 1 : 0);
-  if (ncols <= 0) return;
+  const int ninv = with_inv ? (int)cdiv(kb, 32) : 0;
+  if (ncols <= 0 && ninv == 0 && zero_plan == nullptr) return;
   hipEvent_t e1 = W.prof_begin();
-  hipLaunchKernelGGL(k_laswp, dim3((unsigned)std::min<int64_t>(ncols, 4096)), dim3(256), 0, W.stream,
-                     a.A, a.lda, k, ra0, ra1, rb0, rb1, perm ? 1 : 0, plan, a.perm);
+  hipLaunchKernelGGL(k_laswp, dim3((unsigned)(ninv + std::max<int64_t>(1, std::min<int64_t>(ncols, 4096)))), dim3(256), 0, W.stream,
+                     a.A, a.lda, k, ra0, ra1, rb0, rb1, perm ? 1 : 0, plan, a.perm, kb, ninv, W.linv.p, zero_plan);
   LAUNCH_CK();
   W.prof_end(e1, pname, 16.0 * (double)(ncols - (perm ? 1 : 0)) * kb, 0.0);
 }
@@ -1538,6 +1557,8 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   W.affect.ensure(*W.tracker, 2 * PLAN_WORDS, "swap_plan");
   W.linv.ensure(*W.tracker, (size_t)cdiv(nb, 32) * 1024, "trsm_block_inverses");
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, W.stream, W.dinfo.p, 4);
+  // both swap plans start empty (later each is cleared by the interchange launch of the step before its panel)
+  hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)cdiv(2 * PLAN_WORDS, 256)), dim3(256), 0, W.stream, W.affect.p, 2 * PLAN_WORDS);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, W.stream, W.cnt.p, 2);
   fill_iota(a.perm, n, W);
   static bool attr_set = false;
@@ -1570,16 +1591,19 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
     const int kb = (int)std::min<int64_t>(nb, n - k);
     const int64_t m2 = n - k - kb;
     int32_t* plan = plan_of(k);
-    if (m2 == 0) { laswp_ranges(a, W, k, kb, 0, k, 0, 0, true, plan); break; }
+    if (m2 == 0) { laswp_ranges(a, W, k, kb, 0, k, 0, 0, true, plan, false, nullptr); break; }
     const int kbn = (int)std::min<int64_t>(nb, m2);            // next panel width
     const int64_t cN = k + kb + kbn;                            // end of the next panel's columns
     const bool split = overlap && (kbn % 128) == 0 && m2 > kbn && panel2_usable(a, W, k + kb, kbn, allow_p2);
+    // the diagonal-block inverses for the TRSM are formed by extra workgroups of the first interchange launch
+    // and the next panel's swap-plan counter is cleared by it (its last reader was the previous step's)
+    int32_t* next_plan = plan_of(k + kb);
     if (split) {
-      laswp_ranges(a, W, k, kb, k + kb, cN, 0, 0, false, plan, "laswp_next");
+      laswp_ranges(a, W, k, kb, k + kb, cN, 0, 0, false, plan, !use_fma_trsm, next_plan, "laswp_next");
     } else {
-      laswp_ranges(a, W, k, kb, 0, k, k + kb, n, true, plan);
+      laswp_ranges(a, W, k, kb, 0, k, k + kb, n, true, plan, !use_fma_trsm, next_plan);
     }
-    {
+    if (use_fma_trsm) {
       hipEvent_t e2 = W.prof_begin();
       hipLaunchKernelGGL(k_trsm_inv, dim3(cdiv(kb, 32)), dim3(64), 0, W.stream, a.A, a.lda, k, kb, W.linv.p);
       LAUNCH_CK();
@@ -1593,7 +1617,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
       run_panel(a, W, k + kb, kbn, plan_of(k + kb), W.pstream, false, allow_p2, rmin);
       panels++;
       HIP_CK(hipEventRecord(W.ev_panel, W.pstream));
-      laswp_ranges(a, W, k, kb, 0, k, cN, n, true, plan);
+      laswp_ranges(a, W, k, kb, 0, k, cN, n, true, plan, false, nullptr);
       trsm_cols(a, W, k, kb, cN, n, false);
       update_cols_fmt(a.fmt, a.A, a.lda, k, kb, m2, cN, n, true, false, a.tile_scale, W);
       HIP_CK(hipStreamWaitEvent(W.stream, W.ev_panel, 0));

@@ -12,6 +12,7 @@
 #include <rocsolver/rocsolver.h>
 #include <functional>
 #include <memory>
+#include <set>
 
 using namespace mpk;
 
@@ -88,6 +89,7 @@ struct mp_ctx {
   double* hpin = nullptr;
   std::string last_record, last_hist;
   std::unique_ptr<Krylov> kry;
+  std::set<mp_problem*> problems;       // live problems: their device memory is released by mp_destroy
 };
 
 struct mp_problem {
@@ -101,6 +103,7 @@ struct mp_problem {
   std::vector<std::string> tile_sha;
   std::vector<double> hXexact, hXref;
   const void* a_ptr() const { return fp32 ? (const void*)A32.p : (const void*)A64.p; }
+  void release_device() { A64.release(); A32.release(); B.release(); Xexact.release(); Xref.release(); }
 };
 
 // ------------------------------------------------------------------ methods
@@ -695,6 +698,9 @@ void mp_destroy(mp_ctx* C) {
   if (!C) return;
   (void)hipSetDevice(C->device);
   (void)hipStreamSynchronize(C->W.stream);
+  // problems that outlive their context keep only host state
+  for (mp_problem* p : C->problems) { p->release_device(); p->ctx = nullptr; }
+  C->problems.clear();
   C->kry.reset();
   if (C->rb) rocblas_destroy_handle(C->rb);
   if (C->hpin) (void)hipHostFree(C->hpin);
@@ -810,6 +816,7 @@ int mp_problem_from_host(mp_ctx* C, int64_t n, int32_t nrhs, int32_t a_bits, con
     if (x_ref) { P->hXref.assign(x_ref, x_ref + (size_t)n * nrhs); P->has_ref = true; }
     finish_setup(C, P.get(), src, hb);
     P->case_json = case_json ? case_json : "";
+    C->problems.insert(P.get());
     *out = P.release();
   });
 }
@@ -845,6 +852,7 @@ int mp_problem_generate(mp_ctx* C, const char* case_json, mp_problem** out) {
     std::string cj = case_json; while (!cj.empty() && isspace((unsigned char)cj.back())) cj.pop_back();
     std::string jj = j.obj();
     P->case_json = (cj.size() > 2) ? cj.substr(0, cj.size() - 1) + "," + jj.substr(1) : jj;
+    C->problems.insert(P.get());
     *out = P.release();
   });
 }
@@ -854,6 +862,7 @@ int mp_problem_generate(mp_ctx* C, const char* case_json, mp_problem** out) {
 int mp_problem_reference(mp_ctx* C, mp_problem* P) {
   return guard([&] {
     MP_REQUIRE(C && P, "NULL argument");
+    MP_REQUIRE(P->ctx == C, "problem belongs to another context, or its context was destroyed");
     MP_REQUIRE(!P->fp32, "reference solutions of FP32-stored systems are not implemented");
     const double t0 = wall_now();
     const int64_t n = P->n; const int nrhs = P->nrhs; const size_t nb = (size_t)n * nrhs;
@@ -906,11 +915,16 @@ int mp_problem_reference(mp_ctx* C, mp_problem* P) {
   });
 }
 
-void mp_problem_free(mp_problem* p) { delete p; }
+void mp_problem_free(mp_problem* p) {
+  if (!p) return;
+  if (p->ctx) { (void)hipSetDevice(p->ctx->device); p->ctx->problems.erase(p); }
+  delete p;                           // a problem whose context is gone holds no device memory any more
+}
 
 int mp_problem_copy_out(mp_ctx* C, mp_problem* P, void* A, double* B, double* x_exact) {
   return guard([&] {
     MP_REQUIRE(C && P, "NULL argument");
+    MP_REQUIRE(P->ctx == C, "problem belongs to another context, or its context was destroyed");
     const size_t nn = (size_t)P->n * P->n, nb = (size_t)P->n * P->nrhs;
     if (A) HIP_CK(hipMemcpy(A, P->a_ptr(), nn * (P->fp32 ? 4 : 8), hipMemcpyDeviceToHost));
     if (B) HIP_CK(hipMemcpy(B, P->B.p, nb * 8, hipMemcpyDeviceToHost));
@@ -961,6 +975,7 @@ int mp_methods_json(char* json, size_t cap) {
 int mp_eval_metrics_json(mp_ctx* C, mp_problem* P, const double* X_host, char* json, size_t cap) {
   return guard([&] {
     MP_REQUIRE(C && P && X_host, "NULL argument");
+    MP_REQUIRE(P->ctx == C, "problem belongs to another context, or its context was destroyed");
     DBuf<double> X; X.alloc(C->tracker, (size_t)P->n * P->nrhs, "eval_X");
     HIP_CK(hipMemcpy(X.p, X_host, (size_t)P->n * P->nrhs * 8, hipMemcpyHostToDevice));
     EvalBufs E; Metrics M;
@@ -978,6 +993,7 @@ int mp_last_hist_json(mp_ctx* C, char* json, size_t cap) { return guard([&] { MP
 int mp_solve(mp_ctx* C, mp_problem* P, const char* method_id, const char* params_json, mp_trial_out* out, double* X_host) {
   return guard([&] {
     MP_REQUIRE(C && P && method_id, "NULL argument");
+    MP_REQUIRE(P->ctx == C, "problem belongs to another context, or its context was destroyed");
     Params a; a.parse(params_json);
     MethodSpec M = make_method(method_id, a);
     const std::string target = a.get("target", P->fp32 ? "fp32" : "fp64");

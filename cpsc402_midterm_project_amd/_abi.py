@@ -23,7 +23,7 @@ FMT = {"fp16": 0, "bf16": 1, "fp8": 2, "fp8e4m3": 2, "fp32": 3, "fp16x3": 4}
 EXPORTS = [
     "mp_create", "mp_destroy", "mp_last_error", "mp_caps", "mp_synchronize", "mp_profile", "mp_profile_json", "mp_malloc", "mp_free",
     "mp_memcpy_h2d", "mp_memcpy_d2h", "mp_memcpy_d2d", "mp_residual_fp64", "mp_residual_dd", "mp_getrf",
-    "mp_getrs", "mp_problem_from_host", "mp_problem_generate", "mp_problem_reference", "mp_problem_free", "mp_problem_copy_out", "mp_problem_case_json", "mp_solve",
+    "mp_getrs", "mp_problem_from_host", "mp_problem_generate", "mp_problem_reference", "mp_problem_free", "mp_problem_copy_out", "mp_problem_copy_ref", "mp_problem_case_json", "mp_solve",
     "mp_last_record_json", "mp_last_hist_json", "mp_methods_json", "mp_eval_metrics_json",
 ]
 
@@ -76,6 +76,7 @@ def _load() -> C.CDLL:
         "mp_problem_generate": ([vp, C.c_char_p, C.POINTER(vp)], C.c_int),
         "mp_problem_reference": ([vp, vp], C.c_int),
         "mp_problem_copy_out": ([vp, vp, vp, dp, dp], C.c_int),
+        "mp_problem_copy_ref": ([vp, dp], C.c_int),
         "mp_problem_case_json": ([vp, C.c_char_p, sz], C.c_int),
         "mp_solve": ([vp, vp, C.c_char_p, C.c_char_p, C.POINTER(TrialOut), dp], C.c_int),
         "mp_last_record_json": ([vp, C.c_char_p, sz], C.c_int),

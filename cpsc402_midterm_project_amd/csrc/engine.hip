@@ -932,6 +932,14 @@ int mp_problem_copy_out(mp_ctx* C, mp_problem* P, void* A, double* B, double* x_
   });
 }
 
+int mp_problem_copy_ref(mp_problem* P, double* x_ref) {
+  return guard([&] {
+    MP_REQUIRE(P && x_ref, "NULL argument");
+    MP_REQUIRE(P->has_ref && !P->hXref.empty(), "problem has no reference solution (run mp_problem_reference first)");
+    std::memcpy(x_ref, P->hXref.data(), P->hXref.size() * 8);
+  });
+}
+
 int mp_problem_case_json(mp_problem* P, char* json, size_t cap) {
   return guard([&] {
     MP_REQUIRE(P, "problem is NULL");

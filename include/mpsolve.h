@@ -109,6 +109,9 @@ int mp_problem_reference(mp_ctx* ctx, mp_problem* p);
 void mp_problem_free(mp_problem* p);
 /* copy the stored system back to the host (column-major; A in the stored dtype); any pointer may be NULL */
 int mp_problem_copy_out(mp_ctx* ctx, mp_problem* p, void* A, double* B, double* x_exact);
+/* the reference solution of mp_problem_reference (n x nrhs FP64); MP_ERR_INVALID when there is none.  The
+ * reference writes the same vector to Xref_<case>.f64 (src/mpsolve.cu:168-170). */
+int mp_problem_copy_ref(mp_problem* p, double* x_ref);
 /* case record (JSON object: sizes, norms, hashes, case parameters passed at creation) */
 int mp_problem_case_json(mp_problem* p, char* json, size_t cap);
 

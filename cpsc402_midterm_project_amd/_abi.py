@@ -24,8 +24,9 @@ EXPORTS = [
     "mp_create", "mp_destroy", "mp_last_error", "mp_caps", "mp_synchronize", "mp_profile", "mp_profile_json", "mp_malloc", "mp_free",
     "mp_memcpy_h2d", "mp_memcpy_d2h", "mp_memcpy_d2d", "mp_residual_fp64", "mp_residual_dd", "mp_getrf",
     "mp_getrs", "mp_problem_from_host", "mp_problem_generate", "mp_problem_reference", "mp_problem_free", "mp_problem_copy_out", "mp_problem_copy_ref", "mp_problem_case_json", "mp_solve",
-    "mp_last_record_json", "mp_last_hist_json", "mp_methods_json", "mp_eval_metrics_json",
+    "mp_last_record_json", "mp_last_hist_json", "mp_methods_json", "mp_eval_metrics_json", "mp_batch_solve",
 ]
+BATCH_TERMINAL = ["converged", "nonfinite", "max_outer", "stagnation", "singular_factor"]
 
 
 class MPSolveError(RuntimeError):
@@ -47,6 +48,11 @@ class TrialOut(C.Structure):
                 ("refactorizations", C.c_int32), ("device_ready_wall_s", C.c_double), ("factor_total_s", C.c_double),
                 ("eta_max", C.c_double), ("omega_max", C.c_double), ("terminal_reason", C.c_char * 48),
                 ("final_stage", C.c_char * 48)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("terminal", C.c_int32), ("outer_iters", C.c_int32), ("info", C.c_int32), ("reserved", C.c_int32),
+                ("eta_max", C.c_double), ("omega_max", C.c_double)]
 
 
 def _load() -> C.CDLL:
@@ -83,6 +89,7 @@ def _load() -> C.CDLL:
         "mp_last_hist_json": ([vp, C.c_char_p, sz], C.c_int),
         "mp_methods_json": ([C.c_char_p, sz], C.c_int),
         "mp_eval_metrics_json": ([vp, vp, dp, C.c_char_p, sz], C.c_int),
+        "mp_batch_solve": ([vp, i32, vp, vp, dp, vp, dp, vp, dp, C.c_char_p, vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -211,6 +218,37 @@ class Context:
 
     def getrs(self, n, nrhs, LU, lda, perm, R, ldr, D, ldd, scale=1.0):
         check(lib.mp_getrs(self.h, n, nrhs, _ptr(LU), lda, _ptr(perm), _ptr(R), ldr, _ptr(D), ldd, float(scale)))
+
+    def batch_solve_packed(self, n, nrhs, A, a_off, B, b_off, X, **params):
+        """mp_batch_solve on packed device buffers; n, nrhs, a_off, b_off are host integer sequences."""
+        count = len(n)
+        n_ = np.ascontiguousarray(n, dtype=np.int32); r_ = np.ascontiguousarray(nrhs, dtype=np.int32)
+        ao = np.ascontiguousarray(a_off, dtype=np.int64); bo = np.ascontiguousarray(b_off, dtype=np.int64)
+        res = (BatchResult * max(count, 1))()
+        check(lib.mp_batch_solve(self.h, count, _ptr(n_), _ptr(r_), _ptr(A), _ptr(ao), _ptr(B), _ptr(bo), _ptr(X),
+                                 json.dumps(params).encode(), res))
+        out = [dict(terminal=BATCH_TERMINAL[r.terminal], outer_iters=r.outer_iters, info=r.info, eta_max=r.eta_max,
+                    omega_max=r.omega_max) for r in res[:count]]
+        return out, json.loads(_json_call(lib.mp_last_record_json, self.h))
+
+    def batch_solve(self, systems, **params):
+        """Solve a list of (A, B) host systems (n <= 512, nrhs <= 16) in one launch.
+        Returns ([X_i], [per-system result], batch record)."""
+        mats = [np.asfortranarray(A, dtype=np.float64) for A, _ in systems]
+        rhs = [np.asfortranarray(np.asarray(B, dtype=np.float64).reshape(A.shape[0], -1, order="F")) for A, B in systems]
+        n = [A.shape[0] for A in mats]; nrhs = [B.shape[1] for B in rhs]
+        a_off = np.cumsum([0] + [k * k for k in n])[:-1]; b_off = np.cumsum([0] + [k * r for k, r in zip(n, nrhs)])[:-1]
+        Ah = np.concatenate([A.ravel(order="F") for A in mats]) if mats else np.zeros(1)
+        Bh = np.concatenate([B.ravel(order="F") for B in rhs]) if rhs else np.zeros(1)
+        dA, dB, dX = self.buffer(Ah), self.buffer(Bh), self.empty(max(Bh.nbytes, 8))
+        try:
+            res, rec = self.batch_solve_packed(n, nrhs, dA, a_off, dB, b_off, dX, **params)
+            Xh = dX.to_array((Bh.size,), np.float64)
+        finally:
+            for b in (dA, dB, dX):
+                b.free()
+        X = [Xh[o:o + k * r].reshape(k, r, order="F") for o, k, r in zip(b_off, n, nrhs)]
+        return X, res, rec
 
     # ---- problems and solves
     def problem(self, A: np.ndarray, B: np.ndarray, x_exact=None, x_ref=None, case: dict | None = None) -> "Problem":

@@ -27,7 +27,8 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", f"--offload-arch={ARCH}", "-I", str(ROOT / "include"),
           "-Wno-unused-result", "-Wno-unused-command-line-argument"]
 
-LIB_SOURCES = ["k_resid.hip", "k_getrf.hip", "k_getrs.hip", "k_krylov.hip", "generate.hip", "engine.hip", "sha256.cpp"]
+LIB_SOURCES = ["k_resid.hip", "k_getrf.hip", "k_getrs.hip", "k_krylov.hip", "k_batch.hip", "generate.hip", "engine.hip",
+               "sha256.cpp"]
 CLI_SOURCES = ["cli.cpp"]
 
 

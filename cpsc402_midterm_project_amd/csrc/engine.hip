@@ -775,6 +775,39 @@ int mp_residual_dd(mp_ctx* C, int64_t n, int32_t nrhs, int32_t a_bits, const voi
   });
 }
 
+int mp_batch_solve(mp_ctx* C, int32_t count, const int32_t* n, const int32_t* nrhs, const double* A, const int64_t* a_off,
+                   const double* B, const int64_t* b_off, double* X, const char* params_json, mp_batch_result* results) {
+  return guard([&] {
+    MP_REQUIRE(C, "ctx is NULL");
+    Params a; a.parse(params_json);
+    BatchParams bp;
+    const double u = (a.get("target", "fp64") == "fp32") ? std::ldexp(1.0, -24) : std::ldexp(1.0, -53);
+    bp.tau = a.has("tau") ? a.getd("tau", 0) : a.getd("tau_mult", 100.0) * u;
+    bp.max_outer = a.geti("max_outer", 30);
+    bp.eta_only = a.get("stop_on", "eta_omega") == "eta";
+    bp.wg_per_cu = a.geti("wg_per_cu", 2);
+    MP_REQUIRE(bp.max_outer >= 0, "max_outer must be non-negative");
+    std::vector<BatchResult> r(std::max(count, 0));
+    double dev_s = 0;
+    batch_solve(count, n, nrhs, A, a_off, B, b_off, X, bp, C->W, r.data(), &dev_s);
+    static const char* names[] = {"converged", "nonfinite", "max_outer", "stagnation", "singular_factor"};
+    int64_t hist[5] = {0, 0, 0, 0, 0}; int64_t outer_total = 0, nsum = 0; int nmax = 0;
+    for (int i = 0; i < count; i++) {
+      results[i].terminal = r[i].terminal; results[i].outer_iters = r[i].outer_iters; results[i].info = r[i].info;
+      results[i].reserved = 0; results[i].eta_max = r[i].eta_max; results[i].omega_max = r[i].omega_max;
+      if (r[i].terminal >= 0 && r[i].terminal < 5) hist[r[i].terminal]++;
+      outer_total += r[i].outer_iters; nsum += n[i]; nmax = std::max(nmax, (int)n[i]);
+    }
+    Json j;
+    j.s("kind", "batch").i("systems", count).i("n_sum", nsum).i("n_max", nmax).d("device_s", dev_s)
+     .d("systems_per_s", dev_s > 0 ? count / dev_s : 0.0).d("tau", bp.tau).i("max_outer", bp.max_outer)
+     .s("stop_on", bp.eta_only ? "eta" : "eta_omega").i("wg_per_cu", bp.wg_per_cu).i("outer_iters_total", outer_total);
+    for (int t = 0; t < 5; t++) j.i(names[t], hist[t]);
+    C->last_record = j.obj();
+    C->last_hist.clear();
+  });
+}
+
 int mp_getrf(mp_ctx* C, int64_t n, float* LU, int64_t lda, int64_t* ipiv, int32_t* perm, const mp_getrf_params* prm, mp_getrf_info* info) {
   return guard([&] {
     MP_REQUIRE(C && prm, "NULL argument");

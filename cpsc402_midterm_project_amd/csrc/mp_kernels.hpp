@@ -53,6 +53,12 @@ struct Workspace {
   DBuf<float> tinv;      // inverses of the 128 x 128 diagonal blocks of L and U
   DBuf<float> tchain;    // chain matrices Linv_i L_{i,i-1} and Uinv_i U_{i,i+1} of the blocked substitution
   const float* tinv_lu = nullptr; int64_t tinv_n = 0, tinv_lda = 0; uint64_t tinv_gen = 0;
+  // batched small systems
+  DBuf<uint8_t> bsys;    // per-system descriptors in processing order
+  DBuf<float> bwork;     // per-system factors (row-major origin order + column-major LAPACK order)
+  DBuf<double> bresid;   // per-system residuals
+  DBuf<uint8_t> bres;    // per-system outcomes
+  DBuf<int32_t> bqueue;  // work-queue head of the persistent launch
   uint64_t factor_gen = 1;   // incremented by every factorization written through getrf
   uint32_t tag = 1;      // next free exchange tag (never 0)
   // bracket a launch with profiling events (no-op when profiling is off)
@@ -102,6 +108,17 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out);
 // reuse_inverse: the diagonal-block inverses computed for the same LU buffer since the last getrf are reused.
 void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* perm,
            const double* R, int64_t ldr, double* D, int64_t ldd, double scale, Workspace& W, bool reuse_inverse = false);
+
+// Many independent systems (n <= 512, nrhs <= 16) in one persistent launch: FP32 LU (nb = 32, FP32 trailing
+// update) and stationary refinement with FP64 residuals until eta <= tau (and omega <= tau unless eta_only).
+struct BatchParams { double tau = 100 * 1.1102230246251565e-16; int max_outer = 30; bool eta_only = false; int wg_per_cu = 2; };
+struct BatchResult { int32_t terminal = 0, outer_iters = 0, info = 0; double eta_max = 0, omega_max = 0; };
+int batch_max_n();
+int batch_max_rhs();
+// A, B, X: device buffers; system i uses A + a_off[i] (n x n, lda n) and B / X + b_off[i] (n x nrhs, ld n).
+void batch_solve(int count, const int32_t* n, const int32_t* nrhs, const double* A, const int64_t* a_off,
+                 const double* B, const int64_t* b_off, double* X, const BatchParams& p, Workspace& W,
+                 BatchResult* out, double* device_s);
 
 struct GenSpec {
   int64_t n = 1024; int nrhs = 1; std::string family = "randsvd", spectrum = "geometric", rhs = "xtrue_uniform";

@@ -138,6 +138,27 @@ int mp_last_hist_json(mp_ctx* ctx, char* jsonl, size_t cap);
 /* precision contracts of every method ID (methods_catalog.py:50-100), as a JSON array */
 int mp_methods_json(char* json, size_t cap);
 
+/* ---------------------------------------------------------------- batches of small systems
+ * Solves `count` independent systems A_i X_i = B_i (n_i <= 512, nrhs_i <= 16) in one persistent launch: each
+ * system is factorized in FP32 with partial pivoting (the reference's custom_blocked_lu with nb = 32 and the FP32
+ * trailing update of its cublasSgemm path, src/mp_core.cuh:129-150) and refined by the stationary loop of
+ * Engine::refine_stage (src/mp_core.cuh:281-296: FP64 residual, FP32 correction, the same stopping tests and
+ * terminal reasons).  The reference has no batched entry point; this replaces running its per-system path once
+ * per system.  A, B, X are device buffers of the context's GPU: system i reads A + a_off[i] (n_i x n_i,
+ * column-major, lda n_i) and B + b_off[i], and writes X + b_off[i] (n_i x nrhs_i, ld n_i).  n, nrhs, a_off,
+ * b_off and results are host arrays of `count` entries.  params_json: "tau" or "tau_mult" (x u of "target",
+ * default 100), "max_outer" (30), "stop_on" ("eta_omega" | "eta"), "wg_per_cu" (resident workgroups per CU, 2).
+ * mp_last_record_json then returns the batch record (device seconds, systems per second, terminal counts). */
+typedef struct {
+  int32_t terminal;       /* 0 converged, 1 nonfinite, 2 max_outer, 3 stagnation, 4 singular_factor */
+  int32_t outer_iters;    /* refinement steps taken */
+  int32_t info;           /* first zero pivot of the FP32 factorization (1-based column), 0 if none */
+  int32_t reserved;
+  double eta_max, omega_max;   /* backward errors of the returned X over its right-hand sides (-1 if singular) */
+} mp_batch_result;
+int mp_batch_solve(mp_ctx* ctx, int32_t count, const int32_t* n, const int32_t* nrhs, const double* A, const int64_t* a_off,
+                   const double* B, const int64_t* b_off, double* X, const char* params_json, mp_batch_result* results);
+
 /* FP64 metrics of a host solution on the stored system (eval_metrics, src/mp_core.cuh:193-205):
  * eta/omega per RHS and their maxima, written as a JSON object. */
 int mp_eval_metrics_json(mp_ctx* ctx, mp_problem* p, const double* X_host, char* json, size_t cap);

@@ -1,0 +1,103 @@
+"""Batched small systems (mp_batch_solve): parity with the oracle's FP32-LU stationary refinement, system by system.
+
+The kernel runs, for every system, the reference's custom_blocked_lu with nb = 32 and the FP32 trailing format
+(src/mp_core.cuh:129-150) followed by Engine::refine_stage with FP64 residuals (src/mp_core.cuh:281-296).  The
+oracle runs the same method on the CPU (oracle.ir_stationary(fmt="fp32", nb=32)) with the same operation
+order, so the bar is bitwise: the returned X, the number of refinement steps and the terminal reason must equal
+the oracle's for each system.  At batch scale the pass criterion is the one of the uneven-batch workload:
+every system's normwise backward error, recomputed by the oracle's FP64 metric, is <= tau.
+"""
+import numpy as np
+import pytest
+
+from cpsc402_midterm_project_amd import MPSolveError
+from cpsc402_midterm_project_amd.problems import randsvd
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems(specs, seed0):
+    out = []
+    for i, (n, nrhs, kappa) in enumerate(specs):
+        A, b, _, _ = randsvd(n, kappa, seed0 + i, nrhs=nrhs, gpu=False)
+        out.append((A, b))
+    return out
+
+
+def _oracle_solve(oracle, A, B, tau, max_outer, eta_only):
+    x, reason, hist = oracle.ir_stationary(np.asfortranarray(A), B, fmt="fp32", nb=32, tau=tau, max_outer=max_outer,
+                                           stop_eta_only=eta_only)
+    return x, reason, len(hist) - 1
+
+
+EDGE = [(1, 1, 10.0), (2, 3, 10.0), (31, 1, 1e3), (32, 2, 1e3), (33, 1, 1e4), (63, 4, 1e2), (64, 1, 1e5),
+        (65, 5, 1e3), (100, 16, 1e4), (127, 1, 1e5), (200, 2, 1e3), (257, 1, 1e4), (300, 7, 1e5), (511, 1, 1e3),
+        (512, 3, 1e5)]
+
+
+@pytest.mark.parametrize("stop_on", ["eta", "eta_omega"])
+def test_batch_matches_oracle_bitwise(ctx, oracle, stop_on):
+    print(f"[batch parity] {len(EDGE)} systems stop_on={stop_on}", flush=True)
+    systems = _systems(EDGE, 100)
+    tau = 1e-15 if stop_on == "eta" else 100 * 2.0 ** -53
+    X, res, rec = ctx.batch_solve(systems, tau=tau, stop_on=stop_on, max_outer=30)
+    assert rec["systems"] == len(systems) and rec["device_s"] > 0
+    for (A, B), x, r, spec in zip(systems, X, res, EDGE):
+        xo, reason, outer = _oracle_solve(oracle, A, B, tau, 30, stop_on == "eta")
+        assert (r["terminal"], r["outer_iters"]) == (reason, outer), (spec, r, reason, outer)
+        np.testing.assert_array_equal(x, xo, err_msg=str(spec))
+        m = oracle.metrics(np.asfortranarray(A), B, x)
+        assert r["eta_max"] == m["eta_max"] and r["omega_max"] == m["omega_max"], (spec, r, m["eta_max"], m["omega_max"])
+        assert r["info"] == 0
+
+
+def test_batch_alone_equals_in_mix(ctx):
+    """A system's result does not depend on what else is in the batch or on the launch shape."""
+    specs = [(n, 1 + n % 5, 10.0 ** (1 + n % 5)) for n in (40, 77, 150, 333, 480, 96, 250)]
+    systems = _systems(specs, 500)
+    print(f"[batch alone-vs-mix] {len(systems)} systems", flush=True)
+    Xmix, rmix, _ = ctx.batch_solve(systems, tau=1e-15, stop_on="eta")
+    Xone, rone, _ = ctx.batch_solve(systems, tau=1e-15, stop_on="eta", wg_per_cu=1)
+    for i, s in enumerate(systems):
+        Xa, ra, _ = ctx.batch_solve([s], tau=1e-15, stop_on="eta")
+        np.testing.assert_array_equal(Xa[0], Xmix[i])
+        np.testing.assert_array_equal(Xone[i], Xmix[i])
+        assert ra[0] == rmix[i] == rone[i]
+
+
+def test_batch_uneven_workload_reaches_tau(ctx, oracle):
+    """Uneven batch (n ~ U{32..512}, log10 kappa ~ U[1,5], seeds base + i): every system converges with
+    eta <= 1e-15 by the oracle's FP64 metric."""
+    rng = np.random.default_rng(7)
+    count = 256
+    specs = [(int(rng.integers(32, 513)), 1, float(10.0 ** rng.uniform(1, 5))) for _ in range(count)]
+    systems = _systems(specs, 9000)
+    print(f"[batch uneven] {count} systems", flush=True)
+    X, res, rec = ctx.batch_solve(systems, tau=1e-15, stop_on="eta")
+    assert rec["converged"] == count, rec
+    for (A, B), x, r in zip(systems, X, res):
+        m = oracle.metrics(np.asfortranarray(A), B, x)
+        assert r["terminal"] == "converged" and m["eta_max"] <= 1e-15, (r, m["eta_max"])
+
+
+def test_batch_singular_and_zero_rhs(ctx, oracle):
+    n = 70
+    Z = np.zeros((n, n), order="F")
+    A, b, _, _ = randsvd(n, 1e2, 3, gpu=False)
+    systems = [(Z, np.ones((n, 1))), (A, np.zeros((n, 2))), (A, b)]
+    X, res, _ = ctx.batch_solve(systems, tau=1e-15, stop_on="eta")
+    assert res[0]["terminal"] == "singular_factor" and res[0]["info"] == 1
+    assert not X[0].any()
+    assert res[1]["terminal"] == "converged" and res[1]["outer_iters"] == 0 and not X[1].any()
+    assert res[2]["terminal"] == "converged"
+
+
+def test_batch_rejects_out_of_range(ctx):
+    A, b, _, _ = randsvd(8, 10.0, 1, gpu=False)
+    with pytest.raises(MPSolveError, match="nrhs"):
+        ctx.batch_solve([(A, np.ones((8, 17)))])
+    big = np.eye(513, order="F")
+    with pytest.raises(MPSolveError, match="512"):
+        ctx.batch_solve([(big, np.ones((513, 1)))])
+    X, res, rec = ctx.batch_solve([])
+    assert X == [] and res == [] and rec["systems"] == 0

@@ -5,9 +5,9 @@ aloupeep/CPSC402_Midterm_Project): HIP kernels for gfx950 behind a C ABI (includ
 when it is not built; there is no CPU fallback.  Submodules that do not need the library (build, problems)
 import without it.
 """
-_ABI_NAMES = {"Context", "DeviceBuffer", "MPSolveError", "Problem", "methods", "lib", "EXPORTS", "LIB_PATH"}
+_ABI_NAMES = {"Context", "DeviceBuffer", "MPSolveError", "Problem", "Server", "methods", "lib", "EXPORTS", "LIB_PATH"}
 
-__all__ = ["Context", "DeviceBuffer", "MPSolveError", "Problem", "methods", "randsvd"]
+__all__ = ["Context", "DeviceBuffer", "MPSolveError", "Problem", "Server", "methods", "randsvd"]
 
 
 def __getattr__(name):

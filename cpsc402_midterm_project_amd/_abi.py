@@ -25,7 +25,10 @@ EXPORTS = [
     "mp_memcpy_h2d", "mp_memcpy_d2h", "mp_memcpy_d2d", "mp_residual_fp64", "mp_residual_dd", "mp_getrf",
     "mp_getrs", "mp_problem_from_host", "mp_problem_generate", "mp_problem_reference", "mp_problem_free", "mp_problem_copy_out", "mp_problem_copy_ref", "mp_problem_case_json", "mp_solve",
     "mp_last_record_json", "mp_last_hist_json", "mp_methods_json", "mp_eval_metrics_json", "mp_batch_solve",
+    "mp_batch_solve_ptrs", "mp_generate_into", "mp_problem_from_device", "mp_problem_set_rhs", "mp_server_clock",
+    "mp_server_create", "mp_server_submit", "mp_server_poll", "mp_server_stats_json", "mp_server_destroy",
 ]
+SERVER_TERMINAL = ["converged", "nonfinite", "max_outer", "stagnation", "singular_factor", "error", "other"]
 BATCH_TERMINAL = ["converged", "nonfinite", "max_outer", "stagnation", "singular_factor"]
 
 
@@ -53,6 +56,12 @@ class TrialOut(C.Structure):
 class BatchResult(C.Structure):
     _fields_ = [("terminal", C.c_int32), ("outer_iters", C.c_int32), ("info", C.c_int32), ("reserved", C.c_int32),
                 ("eta_max", C.c_double), ("omega_max", C.c_double)]
+
+
+class ServerCompletion(C.Structure):
+    _fields_ = [("request_id", C.c_uint64), ("terminal", C.c_int32), ("path", C.c_int32), ("cache_hit", C.c_int32),
+                ("outer_iters", C.c_int32), ("eta_max", C.c_double), ("omega_max", C.c_double), ("queue_s", C.c_double),
+                ("service_s", C.c_double), ("latency_s", C.c_double)]
 
 
 def _load() -> C.CDLL:
@@ -90,6 +99,16 @@ def _load() -> C.CDLL:
         "mp_methods_json": ([C.c_char_p, sz], C.c_int),
         "mp_eval_metrics_json": ([vp, vp, dp, C.c_char_p, sz], C.c_int),
         "mp_batch_solve": ([vp, i32, vp, vp, dp, vp, dp, vp, dp, C.c_char_p, vp], C.c_int),
+        "mp_batch_solve_ptrs": ([vp, i32, vp, vp, vp, vp, vp, C.c_char_p, vp], C.c_int),
+        "mp_generate_into": ([vp, C.c_char_p, dp, dp, dp, vp], C.c_int),
+        "mp_problem_from_device": ([vp, i64, i32, dp, dp, i32, C.c_char_p, C.POINTER(vp)], C.c_int),
+        "mp_problem_set_rhs": ([vp, vp, i32, dp], C.c_int),
+        "mp_server_clock": ([], C.c_double),
+        "mp_server_create": ([C.c_int, C.c_char_p, C.POINTER(vp)], C.c_int),
+        "mp_server_submit": ([vp, C.c_uint64, i64, i32, dp, dp, dp, C.c_double, C.POINTER(C.c_uint64)], C.c_int),
+        "mp_server_poll": ([vp, vp, i32, C.c_double, C.POINTER(i32)], C.c_int),
+        "mp_server_stats_json": ([vp, C.c_char_p, sz], C.c_int),
+        "mp_server_destroy": ([vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -250,6 +269,20 @@ class Context:
         X = [Xh[o:o + k * r].reshape(k, r, order="F") for o, k, r in zip(b_off, n, nrhs)]
         return X, res, rec
 
+    def generate_into(self, A, B, x_exact=None, **case) -> bool:
+        """Seeded system generated straight into device buffers (mp_generate_into); returns has_exact."""
+        he = C.c_int32(0)
+        check(lib.mp_generate_into(self.h, json.dumps(case).encode(), _ptr(A), _ptr(B), _ptr(x_exact), C.byref(he)))
+        return bool(he.value)
+
+    def problem_from_device(self, n, nrhs, A, B, borrow_a=True, case=None) -> "Problem":
+        self_p = Problem.__new__(Problem)
+        h = C.c_void_p()
+        check(lib.mp_problem_from_device(self.h, n, nrhs, _ptr(A), _ptr(B), 1 if borrow_a else 0,
+                                         json.dumps(case or {}).encode(), C.byref(h)))
+        self_p.ctx, self_p.h, self_p.n, self_p.nrhs = self, h, n, nrhs
+        return self_p
+
     # ---- problems and solves
     def problem(self, A: np.ndarray, B: np.ndarray, x_exact=None, x_ref=None, case: dict | None = None) -> "Problem":
         return Problem(self, A, B, x_exact, x_ref, case)
@@ -300,6 +333,10 @@ class Problem:
     def case_record(self) -> dict:
         return json.loads(_json_call(lib.mp_problem_case_json, self.h))
 
+    def set_rhs(self, B_dev, nrhs: int) -> None:
+        check(lib.mp_problem_set_rhs(self.ctx.h, self.h, nrhs, _ptr(B_dev)))
+        self.nrhs = nrhs
+
     def solve(self, method: str, **params) -> tuple[np.ndarray, dict]:
         X = np.zeros((self.n, self.nrhs), dtype=np.float64, order="F")
         out = TrialOut()
@@ -325,3 +362,53 @@ class Problem:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Continuous-batching request loop (mp_server_*).  A and B are device buffers that stay alive until their
+    request completes; X is a float64 Fortran-ordered host array filled on completion."""
+
+    def __init__(self, device: int = 0, **params):
+        h = C.c_void_p()
+        check(lib.mp_server_create(device, json.dumps(params).encode(), C.byref(h)))
+        self.h = h
+        self._pending = {}
+
+    @staticmethod
+    def clock() -> float:
+        return lib.mp_server_clock()
+
+    def submit(self, matrix_id: int, n: int, nrhs: int, A, B, X: np.ndarray, arrival: float = -1.0) -> int:
+        assert X.dtype == np.float64 and X.flags.f_contiguous and X.size == n * nrhs
+        rid = C.c_uint64()
+        check(lib.mp_server_submit(self.h, matrix_id, n, nrhs, _ptr(A), _ptr(B), _ptr(X), float(arrival), C.byref(rid)))
+        self._pending[rid.value] = X
+        return rid.value
+
+    def poll(self, max_count: int = 1024, wait: float = 0.05) -> list[dict]:
+        buf = (ServerCompletion * max_count)()
+        cnt = C.c_int32()
+        check(lib.mp_server_poll(self.h, buf, max_count, float(wait), C.byref(cnt)))
+        out = []
+        for c in buf[:cnt.value]:
+            self._pending.pop(c.request_id, None)
+            out.append(dict(id=c.request_id, terminal=SERVER_TERMINAL[c.terminal], path=("batch", "large")[c.path],
+                            cache_hit=bool(c.cache_hit), outer_iters=c.outer_iters, eta_max=c.eta_max,
+                            omega_max=c.omega_max, queue_s=c.queue_s, service_s=c.service_s, latency_s=c.latency_s))
+        return out
+
+    def stats(self) -> dict:
+        buf = C.create_string_buffer(4096)
+        check(lib.mp_server_stats_json(self.h, buf, len(buf)))
+        return json.loads(buf.value)
+
+    def close(self) -> None:
+        if self.h:
+            check(lib.mp_server_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

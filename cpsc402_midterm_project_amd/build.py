@@ -28,7 +28,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", f"--offload-arch={A
           "-Wno-unused-result", "-Wno-unused-command-line-argument"]
 
 LIB_SOURCES = ["k_resid.hip", "k_getrf.hip", "k_getrs.hip", "k_krylov.hip", "k_batch.hip", "generate.hip", "engine.hip",
-               "sha256.cpp"]
+               "sha256.cpp", "server.cpp"]
 CLI_SOURCES = ["cli.cpp"]
 
 
@@ -36,7 +36,7 @@ def _compile(src: Path, obj: Path) -> tuple[str, int, str]:
     if src.suffix == ".hip":
         cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE_FLAGS, "-c", str(src), "-o", str(obj)]
     else:
-        cmd = [HIPCC, "-O2", "-fPIC", "-std=c++17", "-I", str(ROOT / "include"), "-c", str(src), "-o", str(obj)]
+        cmd = [HIPCC, "-O2", "-fPIC", "-std=c++17", "-pthread", "-I", str(ROOT / "include"), "-c", str(src), "-o", str(obj)]
     p = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in (p.stderr or "").splitlines() if "packed-fp32-ops" not in l)
     return str(src.name), p.returncode, err
@@ -57,7 +57,7 @@ def build(jobs: int = 8, verbose: bool = False) -> Path:
     lib = PKG / "libmpsolve.so"
     tmp = PKG / "libmpsolve.so.tmp"
     link = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs),
-            "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+            "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-pthread", "-Wl,-rpath,/opt/rocm/lib"]
     p = subprocess.run(link, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError("link failed:\n" + p.stderr)

@@ -92,6 +92,8 @@ struct mp_ctx {
   std::set<mp_problem*> problems;       // live problems: their device memory is released by mp_destroy
 };
 
+struct Factors;
+
 struct mp_problem {
   mp_ctx* ctx = nullptr;
   int64_t n = 0; int nrhs = 1; bool fp32 = false;
@@ -103,7 +105,10 @@ struct mp_problem {
   std::vector<std::string> tile_sha;
   std::vector<double> hXexact, hXref;
   const void* a_ptr() const { return fp32 ? (const void*)A32.p : (const void*)A64.p; }
-  void release_device() { A64.release(); A32.release(); B.release(); Xexact.release(); Xref.release(); }
+  // factors kept for solves with "reuse" (the reference keeps its Engine, and so its factors, across the trials
+  // of a case, src/mpsolve.cu:173 and src/mp_core.cuh:224-227); a factor cache of repeated matrices uses the same
+  mutable std::shared_ptr<Factors> fcache;
+  void release_device() { A64.release(); A32.release(); B.release(); Xexact.release(); Xref.release(); fcache.reset(); }
 };
 
 // ------------------------------------------------------------------ methods
@@ -404,7 +409,8 @@ struct Engine {
   void sync() { HIP_CK(hipStreamSynchronize(C->W.stream)); }
 
   Factors& factors_for(const MethodSpec& M, const std::string& kind, TrialOut& T) {
-    Factors& F = fact;
+    if (reuse && !P.fcache) P.fcache = std::make_shared<Factors>();
+    Factors& F = reuse ? *P.fcache : fact;
     const double t0 = wall_now();
     if (reuse && F.valid && F.kind == kind && F.scale_mode == M.scale_mode && F.nb == M.nb) {
       T.stages.push_back(kind + "(reused)"); T.stage_times.push_back(0.0); T.growth = F.growth; T.factor_info = F.info; return F;
@@ -664,9 +670,45 @@ static void finish_setup(mp_ctx* C, mp_problem* Pp, const char* src, const std::
     P->b_sha = sha256_hex(hb.data(), hb.size() * 8);
 }
 
+// Norms of a problem whose A is already on the device (no host copy of A, so no A hashes): ||A||_inf, max|A|
+// on the device; B's norms and hash from its host copy.
+static void finish_setup_device(mp_ctx* C, mp_problem* P, const std::vector<double>& hb) {
+  const int64_t n = P->n; const int nrhs = P->nrhs;
+  {
+    DBuf<double> ones, Rt, Tt;
+    ones.alloc(C->tracker, n, "ones"); Rt.alloc(C->tracker, n, "rowsum_R"); Tt.alloc(C->tracker, n, "rowsum_T");
+    fill_f64(ones.p, n, 1.0, C->W);
+    ResidArgs ra; ra.n = n; ra.nrhs = 1; ra.a32 = false; ra.A = P->a_ptr(); ra.lda = n; ra.X = ones.p; ra.ldx = n;
+    ra.R = Rt.p; ra.ldr = n; ra.T = Tt.p; ra.ldt = n; ra.matvec = true;
+    residual(ra, C->W);
+    std::vector<double> rs(n);
+    HIP_CK(hipMemcpyAsync(rs.data(), Tt.p, n * 8, hipMemcpyDeviceToHost, C->W.stream));
+    HIP_CK(hipStreamSynchronize(C->W.stream));
+    double m = 0; for (double v : rs) { if (!(v <= m)) m = v; }
+    P->anorm_inf = m;
+  }
+  P->amax = dev_absmax(C, P->a_ptr(), false, n, n);
+  P->bnorm_inf.assign(nrhs, 0); P->bnorm_2.assign(nrhs, 0);
+  for (int c = 0; c < nrhs; c++) {
+    double m = 0, s2 = 0;
+    for (int64_t i = 0; i < n; i++) { const double v = std::fabs(hb[i + (size_t)c * n]); m = std::max(m, v); s2 += v * v; }
+    P->bnorm_inf[c] = m; P->bnorm_2[c] = std::sqrt(s2);
+  }
+  P->a_sha = "not_computed";
+  P->tile_sha.clear();
+  P->b_sha = sha256_hex(hb.data(), hb.size() * 8);
+}
+
 extern "C" {
 
 const char* mp_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"
+
+// error message of the calling thread, for entry points implemented in other files of the library
+void mp_set_last_error(const std::string& msg) { g_last_error = msg; }
+
+extern "C" {
 
 int mp_create(mp_ctx** out, int device) {
   return guard([&] {
@@ -775,36 +817,71 @@ int mp_residual_dd(mp_ctx* C, int64_t n, int32_t nrhs, int32_t a_bits, const voi
   });
 }
 
+static void batch_record(mp_ctx* C, int count, const int32_t* n, const BatchParams& bp, const std::vector<BatchResult>& r,
+                         double dev_s, mp_batch_result* results) {
+  static const char* names[] = {"converged", "nonfinite", "max_outer", "stagnation", "singular_factor"};
+  int64_t hist[5] = {0, 0, 0, 0, 0}; int64_t outer_total = 0, nsum = 0; int nmax = 0;
+  double eta_worst = 0;
+  for (int i = 0; i < count; i++) {
+    results[i].terminal = r[i].terminal; results[i].outer_iters = r[i].outer_iters; results[i].info = r[i].info;
+    results[i].reserved = 0; results[i].eta_max = r[i].eta_max; results[i].omega_max = r[i].omega_max;
+    if (r[i].terminal >= 0 && r[i].terminal < 5) hist[r[i].terminal]++;
+    outer_total += r[i].outer_iters; nsum += n[i]; nmax = std::max(nmax, (int)n[i]);
+    if (!(r[i].eta_max <= eta_worst)) eta_worst = r[i].eta_max;
+  }
+  Json j;
+  j.s("kind", "batch").i("systems", count).i("n_sum", nsum).i("n_max", nmax).d("device_s", dev_s)
+   .d("systems_per_s", dev_s > 0 ? count / dev_s : 0.0).d("tau", bp.tau).i("max_outer", bp.max_outer)
+   .s("stop_on", bp.eta_only ? "eta" : "eta_omega").i("wg_per_cu", bp.wg_per_cu).i("outer_iters_total", outer_total)
+   .d("eta_worst", eta_worst);
+  for (int t = 0; t < 5; t++) j.i(names[t], hist[t]);
+  C->last_record = j.obj();
+  C->last_hist.clear();
+}
+
+static BatchParams batch_params(const char* params_json) {
+  Params a; a.parse(params_json);
+  BatchParams bp;
+  const double u = (a.get("target", "fp64") == "fp32") ? std::ldexp(1.0, -24) : std::ldexp(1.0, -53);
+  bp.tau = a.has("tau") ? a.getd("tau", 0) : a.getd("tau_mult", 100.0) * u;
+  bp.max_outer = a.geti("max_outer", 30);
+  bp.eta_only = a.get("stop_on", "eta_omega") == "eta";
+  bp.wg_per_cu = a.geti("wg_per_cu", 2);
+  MP_REQUIRE(bp.max_outer >= 0, "max_outer must be non-negative");
+  return bp;
+}
+
+int mp_batch_solve_ptrs(mp_ctx* C, int32_t count, const int32_t* n, const int32_t* nrhs, const double* const* A,
+                        const double* const* B, double* const* X, const char* params_json, mp_batch_result* results) {
+  return guard([&] {
+    MP_REQUIRE(C, "ctx is NULL");
+    const BatchParams bp = batch_params(params_json);
+    std::vector<BatchResult> r(std::max(count, 0));
+    double dev_s = 0;
+    batch_solve(count, n, nrhs, A, B, X, bp, C->W, r.data(), &dev_s);
+    batch_record(C, count, n, bp, r, dev_s, results);
+  });
+}
+
 int mp_batch_solve(mp_ctx* C, int32_t count, const int32_t* n, const int32_t* nrhs, const double* A, const int64_t* a_off,
                    const double* B, const int64_t* b_off, double* X, const char* params_json, mp_batch_result* results) {
   return guard([&] {
     MP_REQUIRE(C, "ctx is NULL");
-    Params a; a.parse(params_json);
-    BatchParams bp;
-    const double u = (a.get("target", "fp64") == "fp32") ? std::ldexp(1.0, -24) : std::ldexp(1.0, -53);
-    bp.tau = a.has("tau") ? a.getd("tau", 0) : a.getd("tau_mult", 100.0) * u;
-    bp.max_outer = a.geti("max_outer", 30);
-    bp.eta_only = a.get("stop_on", "eta_omega") == "eta";
-    bp.wg_per_cu = a.geti("wg_per_cu", 2);
-    MP_REQUIRE(bp.max_outer >= 0, "max_outer must be non-negative");
+    MP_REQUIRE(count >= 0, "count must be non-negative");
+    const BatchParams bp = batch_params(params_json);
+    std::vector<const double*> pa(std::max(count, 0)), pb(std::max(count, 0));
+    std::vector<double*> px(std::max(count, 0));
+    if (count > 0) {
+      MP_REQUIRE(A && B && X && a_off && b_off, "NULL buffer or offset array");
+      for (int i = 0; i < count; i++) {
+        MP_REQUIRE(a_off[i] >= 0 && b_off[i] >= 0, "offsets must be non-negative");
+        pa[i] = A + a_off[i]; pb[i] = B + b_off[i]; px[i] = X + b_off[i];
+      }
+    }
     std::vector<BatchResult> r(std::max(count, 0));
     double dev_s = 0;
-    batch_solve(count, n, nrhs, A, a_off, B, b_off, X, bp, C->W, r.data(), &dev_s);
-    static const char* names[] = {"converged", "nonfinite", "max_outer", "stagnation", "singular_factor"};
-    int64_t hist[5] = {0, 0, 0, 0, 0}; int64_t outer_total = 0, nsum = 0; int nmax = 0;
-    for (int i = 0; i < count; i++) {
-      results[i].terminal = r[i].terminal; results[i].outer_iters = r[i].outer_iters; results[i].info = r[i].info;
-      results[i].reserved = 0; results[i].eta_max = r[i].eta_max; results[i].omega_max = r[i].omega_max;
-      if (r[i].terminal >= 0 && r[i].terminal < 5) hist[r[i].terminal]++;
-      outer_total += r[i].outer_iters; nsum += n[i]; nmax = std::max(nmax, (int)n[i]);
-    }
-    Json j;
-    j.s("kind", "batch").i("systems", count).i("n_sum", nsum).i("n_max", nmax).d("device_s", dev_s)
-     .d("systems_per_s", dev_s > 0 ? count / dev_s : 0.0).d("tau", bp.tau).i("max_outer", bp.max_outer)
-     .s("stop_on", bp.eta_only ? "eta" : "eta_omega").i("wg_per_cu", bp.wg_per_cu).i("outer_iters_total", outer_total);
-    for (int t = 0; t < 5; t++) j.i(names[t], hist[t]);
-    C->last_record = j.obj();
-    C->last_hist.clear();
+    batch_solve(count, n, nrhs, pa.data(), pb.data(), px.data(), bp, C->W, r.data(), &dev_s);
+    batch_record(C, count, n, bp, r, dev_s, results);
   });
 }
 
@@ -851,6 +928,71 @@ int mp_problem_from_host(mp_ctx* C, int64_t n, int32_t nrhs, int32_t a_bits, con
     P->case_json = case_json ? case_json : "";
     C->problems.insert(P.get());
     *out = P.release();
+  });
+}
+
+int mp_generate_into(mp_ctx* C, const char* case_json, double* A, double* B, double* x_exact, int32_t* has_exact) {
+  return guard([&] {
+    MP_REQUIRE(C && case_json && A && B, "NULL argument");
+    Params a; a.parse(case_json);
+    GenSpec g;
+    g.n = (int64_t)a.getd("n", 1024); g.nrhs = a.geti("nrhs", 1); g.family = a.get("family", "randsvd");
+    g.spectrum = a.get("spectrum", "geometric"); g.rhs = a.get("rhs", g.family == "dyadic" ? "exact_sign" : "xtrue_uniform");
+    g.kappa = a.getd("kappa", 100); g.seed = (uint64_t)a.getd("seed", 9200); g.ell = a.getd("ell", 0.25); g.nugget = a.getd("nugget", 1e-3);
+    g.dim = a.geti("dim", 3); g.intmax = a.geti("intmax", 255); g.dyadic_exp = a.geti("dyadic_exp", 8); g.grcar_k = a.geti("grcar_k", 3);
+    g.global_exp = a.geti("global_exp", 0);
+    MP_REQUIRE(g.n > 0 && g.nrhs > 0, "n and nrhs must be positive");
+    DBuf<double> Xe;
+    double* xe = x_exact;
+    if (!xe) { Xe.alloc(C->tracker, (size_t)g.n * g.nrhs, "Xexact"); xe = Xe.p; }
+    const bool exact = generate(g, (void*)C->rb, C->W, C->tracker, A, B, xe);
+    HIP_CK(hipStreamSynchronize(C->W.stream));
+    if (has_exact) *has_exact = exact ? 1 : 0;
+  });
+}
+
+int mp_problem_from_device(mp_ctx* C, int64_t n, int32_t nrhs, const double* A, const double* B, int32_t borrow_a,
+                           const char* case_json, mp_problem** out) {
+  return guard([&] {
+    MP_REQUIRE(C && A && B && out, "NULL argument");
+    MP_REQUIRE(n > 0 && nrhs > 0, "bad sizes");
+    std::unique_ptr<mp_problem> P(new mp_problem());
+    P->ctx = C; P->n = n; P->nrhs = nrhs; P->fp32 = false;
+    const size_t nn = (size_t)n * n;
+    if (borrow_a) { P->A64.p = const_cast<double*>(A); P->A64.n = nn; P->A64.tr = nullptr; }   // not freed by the problem
+    else { P->A64.alloc(C->tracker, nn, "A64"); HIP_CK(hipMemcpy(P->A64.p, A, nn * 8, hipMemcpyDeviceToDevice)); }
+    P->B.alloc(C->tracker, (size_t)n * nrhs, "B");
+    HIP_CK(hipMemcpy(P->B.p, B, (size_t)n * nrhs * 8, hipMemcpyDeviceToDevice));
+    std::vector<double> hb((size_t)n * nrhs);
+    HIP_CK(hipMemcpy(hb.data(), B, hb.size() * 8, hipMemcpyDeviceToHost));
+    finish_setup_device(C, P.get(), hb);
+    P->case_json = case_json ? case_json : "";
+    C->problems.insert(P.get());
+    *out = P.release();
+  });
+}
+
+int mp_problem_set_rhs(mp_ctx* C, mp_problem* P, int32_t nrhs, const double* B) {
+  return guard([&] {
+    MP_REQUIRE(C && P && B, "NULL argument");
+    MP_REQUIRE(P->ctx == C, "problem belongs to another context, or its context was destroyed");
+    MP_REQUIRE(nrhs > 0, "nrhs must be positive");
+    const size_t nb = (size_t)P->n * nrhs;
+    if (P->B.n < nb) P->B.alloc(C->tracker, nb, "B");
+    HIP_CK(hipMemcpy(P->B.p, B, nb * 8, hipMemcpyDeviceToDevice));
+    P->nrhs = nrhs;
+    std::vector<double> hb(nb);
+    HIP_CK(hipMemcpy(hb.data(), B, nb * 8, hipMemcpyDeviceToHost));
+    P->bnorm_inf.assign(nrhs, 0); P->bnorm_2.assign(nrhs, 0);
+    for (int c = 0; c < nrhs; c++) {
+      double m = 0, s2 = 0;
+      for (int64_t i = 0; i < P->n; i++) { const double v = std::fabs(hb[i + (size_t)c * P->n]); m = std::max(m, v); s2 += v * v; }
+      P->bnorm_inf[c] = m; P->bnorm_2[c] = std::sqrt(s2);
+    }
+    P->b_sha = sha256_hex(hb.data(), hb.size() * 8);
+    // solutions of the old right-hand sides no longer apply; the cached factors of A do
+    P->has_exact = false; P->has_ref = false; P->hXexact.clear(); P->hXref.clear();
+    P->Xexact.release(); P->Xref.release();
   });
 }
 
@@ -1064,7 +1206,7 @@ int mp_solve(mp_ctx* C, mp_problem* P, const char* method_id, const char* params
       }
       if (!skipped && T.ok) {
         Engine E(C, *P, tau, tau);
-        E.instrument = a.getb("instrument", false); E.stop_eta_only = eta_only;
+        E.instrument = a.getb("instrument", false); E.stop_eta_only = eta_only; E.reuse = a.getb("reuse", false);
         HIP_CK(hipStreamSynchronize(C->W.stream));
         const double t1 = wall_now();
         E.run(M, T);

@@ -36,7 +36,9 @@ constexpr int BNMAX = 512;
 constexpr int BRHS = 16;
 
 struct BSys {
-  int64_t a_off, b_off;   // A (n x n, lda n) and B / X (n x nrhs, ld n) offsets in doubles
+  const double* A;        // n x n, lda n
+  const double* B;        // n x nrhs, ld n
+  double* X;              // n x nrhs, ld n
   int64_t w_off;          // factor workspace offset in floats: row-major n x ldr, then column-major n x n
   int64_t r_off;          // residual scratch offset in doubles (n x nrhs)
   int32_t n, nrhs, idx, ldr;
@@ -538,8 +540,7 @@ __device__ void residual(Lds& S, const double* A, const double* B, const double*
 __global__ void k_batch_zero(int32_t* p) { if (threadIdx.x == 0) p[0] = 0; }
 
 __global__ __launch_bounds__(BT, 2) void k_batch(const BSys* __restrict__ sys, int count, int32_t* __restrict__ queue,
-                                                 const double* __restrict__ Aall, const double* __restrict__ Ball,
-                                                 double* __restrict__ Xall, float* __restrict__ Wall, double* __restrict__ Rall,
+                                                 float* __restrict__ Wall, double* __restrict__ Rall,
                                                  BRes* __restrict__ res, double tau, int max_outer, int eta_only) {
   __shared__ Lds S;
   const int t = threadIdx.x;
@@ -553,9 +554,9 @@ __global__ __launch_bounds__(BT, 2) void k_batch(const BSys* __restrict__ sys, i
     if (q >= count) return;
     const BSys d = sys[q];
     const int n = d.n, nrhs = d.nrhs, ldr = d.ldr;
-    const double* A = Aall + d.a_off;
-    const double* B = Ball + d.b_off;
-    double* X = Xall + d.b_off;
+    const double* A = d.A;
+    const double* B = d.B;
+    double* X = d.X;
     float* LU = Wall + d.w_off;
     float* LUc = LU + (int64_t)n * ldr;
     double* R = Rall + d.r_off;
@@ -608,19 +609,17 @@ __global__ __launch_bounds__(BT, 2) void k_batch(const BSys* __restrict__ sys, i
 int batch_max_n() { return BNMAX; }
 int batch_max_rhs() { return BRHS; }
 
-void batch_solve(int count, const int32_t* n, const int32_t* nrhs, const double* A, const int64_t* a_off,
-                 const double* B, const int64_t* b_off, double* X, const BatchParams& p, Workspace& W,
-                 BatchResult* out, double* device_s) {
+void batch_solve(int count, const int32_t* n, const int32_t* nrhs, const double* const* A, const double* const* B,
+                 double* const* X, const BatchParams& p, Workspace& W, BatchResult* out, double* device_s) {
   MP_REQUIRE(count >= 0, "count must be non-negative");
   if (device_s) *device_s = 0.0;
   if (count == 0) return;
-  MP_REQUIRE(n && nrhs && a_off && b_off && out, "NULL size/offset/result array");
-  MP_REQUIRE(A && B && X, "NULL device buffer");
+  MP_REQUIRE(n && nrhs && A && B && X && out, "NULL size/pointer/result array");
   MP_REQUIRE(p.wg_per_cu >= 1 && p.wg_per_cu <= 8, "wg_per_cu must be in [1, 8]");
   for (int i = 0; i < count; i++) {
     MP_REQUIRE(n[i] >= 1 && n[i] <= BNMAX, "every n must be in [1, 512] (larger systems go through mp_solve)");
     MP_REQUIRE(nrhs[i] >= 1 && nrhs[i] <= BRHS, "every nrhs must be in [1, 16]");
-    MP_REQUIRE(a_off[i] >= 0 && b_off[i] >= 0, "offsets must be non-negative");
+    MP_REQUIRE(A[i] && B[i] && X[i], "NULL device pointer of a system");
   }
   // longest first: n^3 dominates the work of a system
   std::vector<int> order(count);
@@ -632,7 +631,7 @@ void batch_solve(int count, const int32_t* n, const int32_t* nrhs, const double*
     const int i = order[q];
     BSys& s = tab[q];
     s.n = n[i]; s.nrhs = nrhs[i]; s.idx = i; s.ldr = (n[i] + 31) / 32 * 32;
-    s.a_off = a_off[i]; s.b_off = b_off[i];
+    s.A = A[i]; s.B = B[i]; s.X = X[i];
     s.w_off = woff; woff += (int64_t)s.n * s.ldr + (int64_t)s.n * s.n;
     woff = (woff + 63) / 64 * 64;                       // 256-byte aligned slices
     s.r_off = roff; roff += (int64_t)s.n * s.nrhs;
@@ -650,7 +649,7 @@ void batch_solve(int count, const int32_t* n, const int32_t* nrhs, const double*
   hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(64), 0, W.stream, W.bqueue.p);
   LAUNCH_CK();
   HIP_CK(hipEventRecord(e0, W.stream));
-  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(BT), 0, W.stream, (const BSys*)W.bsys.p, count, W.bqueue.p, A, B, X,
+  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(BT), 0, W.stream, (const BSys*)W.bsys.p, count, W.bqueue.p,
                      W.bwork.p, W.bresid.p, (BRes*)W.bres.p, p.tau, p.max_outer, p.eta_only ? 1 : 0);
   LAUNCH_CK();
   HIP_CK(hipEventRecord(e1, W.stream));

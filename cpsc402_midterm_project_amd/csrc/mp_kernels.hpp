@@ -115,10 +115,9 @@ struct BatchParams { double tau = 100 * 1.1102230246251565e-16; int max_outer = 
 struct BatchResult { int32_t terminal = 0, outer_iters = 0, info = 0; double eta_max = 0, omega_max = 0; };
 int batch_max_n();
 int batch_max_rhs();
-// A, B, X: device buffers; system i uses A + a_off[i] (n x n, lda n) and B / X + b_off[i] (n x nrhs, ld n).
-void batch_solve(int count, const int32_t* n, const int32_t* nrhs, const double* A, const int64_t* a_off,
-                 const double* B, const int64_t* b_off, double* X, const BatchParams& p, Workspace& W,
-                 BatchResult* out, double* device_s);
+// System i: A[i] (n x n, lda n), B[i] and X[i] (n x nrhs, ld n), device pointers in host arrays.
+void batch_solve(int count, const int32_t* n, const int32_t* nrhs, const double* const* A, const double* const* B,
+                 double* const* X, const BatchParams& p, Workspace& W, BatchResult* out, double* device_s);
 
 struct GenSpec {
   int64_t n = 1024; int nrhs = 1; std::string family = "randsvd", spectrum = "geometric", rhs = "xtrue_uniform";

@@ -103,6 +103,18 @@ int mp_problem_from_host(mp_ctx* ctx, int64_t n, int32_t nrhs, int32_t a_bits, c
  * exact_sign | unit_column), "intmax", "dyadic_exp", "ell", "nugget", "dim", "grcar_k", "global_exp".
  * Replaces generate_problem (src/mp_core.cuh:51-103); the random streams are this library's own. */
 int mp_problem_generate(mp_ctx* ctx, const char* case_json, mp_problem** out);
+/* The same seeded generation into caller-provided device buffers of the context's GPU: A (n x n), B and, if not
+ * NULL, x_exact (n x nrhs); *has_exact (may be NULL) tells whether x_exact holds the exact solution.  No
+ * problem object and no host copies (for building batches and request pools in HBM). */
+int mp_generate_into(mp_ctx* ctx, const char* case_json, double* A, double* B, double* x_exact, int32_t* has_exact);
+/* A problem over an FP64 system already in device memory (n x n A, n x nrhs B, leading dimension n).  B is
+ * copied; A is copied unless borrow_a != 0, in which case the caller keeps A alive and unchanged for the
+ * problem's lifetime.  Norms are computed on the device; the record's a_sha256 is "not_computed". */
+int mp_problem_from_device(mp_ctx* ctx, int64_t n, int32_t nrhs, const double* A, const double* B, int32_t borrow_a,
+                           const char* case_json, mp_problem** out);
+/* Replaces the right-hand sides (device, n x nrhs) and drops the exact / reference solutions; factors cached by
+ * solves with "reuse" stay valid (they depend on A only). */
+int mp_problem_set_rhs(mp_ctx* ctx, mp_problem* p, int32_t nrhs, const double* B);
 /* FP64 LU solution refined with double-double residuals (best of <= 12 steps), used for the forward-error
  * metrics; replaces compute_reference (src/mpsolve.cu:99-125) without the kappa_2 estimate. */
 int mp_problem_reference(mp_ctx* ctx, mp_problem* p);
@@ -120,6 +132,8 @@ int mp_problem_case_json(mp_problem* p, char* json, size_t cap);
  * mp_methods_json) with the reference's stopping rules and terminal reasons (Engine::run,
  * src/mp_core.cuh:299-314).  params_json overrides parameters ("nb", "gmres_m", "gmres_tol",
  * "gmres_budget", "gmres_cycles", "max_outer", "tau_mult", "target", "cost_ratio_next"); NULL = defaults.
+ * "reuse": true keeps the problem's factors and reuses them in later solves of the same factor kind, nb and
+ * scale mode (the reference's --reuse, src/mpsolve.cu:173; its Engine lives across the trials of a case).
  * The timed region (device_ready_wall_s) starts with inputs resident and ends after the last kernel.
  * X_host (n x nrhs FP64, may be NULL) receives the solution.  The full trial record (the reference's
  * field names, src/mpsolve.cu:221-229) is returned by mp_last_record_json, the histories by
@@ -158,6 +172,42 @@ typedef struct {
 } mp_batch_result;
 int mp_batch_solve(mp_ctx* ctx, int32_t count, const int32_t* n, const int32_t* nrhs, const double* A, const int64_t* a_off,
                    const double* B, const int64_t* b_off, double* X, const char* params_json, mp_batch_result* results);
+/* The same with one device pointer per system (host arrays A[i], B[i], X[i]), for systems that live in
+ * separate allocations. */
+int mp_batch_solve_ptrs(mp_ctx* ctx, int32_t count, const int32_t* n, const int32_t* nrhs, const double* const* A,
+                        const double* const* B, double* const* X, const char* params_json, mp_batch_result* results);
+
+/* ---------------------------------------------------------------- serving loop
+ * Continuous-batching request loop on one GPU (no counterpart in the reference, which solves one system per
+ * process, src/mpsolve.cu:130-240).  Requests name a matrix id and FP64 A (n x n) and B (n x nrhs) already in
+ * device memory (leading dimension n, alive and unchanged until the request completes; A for as long as its id
+ * may be repeated), and a host buffer that receives X (n x nrhs).  Systems with n <= 512 and nrhs <= 16 are
+ * grouped into persistent batched launches (mp_batch_solve) while larger ones are solved one at a time by
+ * "method" on a second context, so both kinds overlap on the device; a repeated matrix id reuses its cached
+ * factors.  Every request is refined to eta <= "tau" (default 1e-15).  Admission keeps the bytes the server
+ * holds within "hbm_budget_bytes" (default 200 GiB), releasing the least recently used cached factors first.
+ * params_json: "method", "tau", "hbm_budget_bytes", "max_batch", "solve" {...} and "batch" {...} (extra
+ * parameters of mp_solve / mp_batch_solve).  Latencies use the clock of mp_server_clock (seconds). */
+typedef struct mp_server mp_server;
+typedef struct {
+  uint64_t request_id;
+  int32_t terminal;       /* 0 converged, 1 nonfinite, 2 max_outer, 3 stagnation, 4 singular_factor, 5 error, 6 other */
+  int32_t path;           /* 0 batched small system, 1 large system */
+  int32_t cache_hit;      /* 1: factors reused from an earlier request with the same matrix id */
+  int32_t outer_iters;
+  double eta_max, omega_max;
+  double queue_s, service_s, latency_s;   /* arrival -> start, start -> done, arrival -> done */
+} mp_server_completion;
+double mp_server_clock(void);
+int mp_server_create(int device, const char* params_json, mp_server** out);
+/* arrival_s: the request's arrival time on mp_server_clock (negative: now) */
+int mp_server_submit(mp_server* s, uint64_t matrix_id, int64_t n, int32_t nrhs, const double* A, const double* B,
+                     double* X_host, double arrival_s, uint64_t* request_id);
+/* up to max completions; waits up to wait_s seconds when none is ready */
+int mp_server_poll(mp_server* s, mp_server_completion* out, int32_t max, double wait_s, int32_t* count);
+int mp_server_stats_json(mp_server* s, char* json, size_t cap);
+/* finishes the queued requests, then releases the cached factors and the server's contexts */
+int mp_server_destroy(mp_server* s);
 
 /* FP64 metrics of a host solution on the stored system (eval_metrics, src/mp_core.cuh:193-205):
  * eta/omega per RHS and their maxima, written as a JSON object. */

@@ -8,7 +8,11 @@ GPU.  A "step" is one complete solve from resident inputs to the stopped iterate
 time-to-accuracy of the headline case (kappa=1e6) in ms; every case's timings, iteration counts and errors
 are reported alongside.
 
-python bench.py [--steps K] [--warmup W] [--n N] [--kappas 1e2,1e6] [--quick]
+Two further legs report the rest of the metric: the uneven batch (4096 systems n <= 512 in one persistent
+launch) and the serving stream (p50/p99 arrival-to-completion latency at 0.5x/0.8x/0.95x of the measured
+saturation rate; serve_p50_ms / serve_p99_ms are the 0.8x figures).
+
+python bench.py [--steps K] [--warmup W] [--n N] [--kappas 1e2,1e6] [--batch-count C] [--serve-requests R]
 """
 from __future__ import annotations
 
@@ -56,6 +60,135 @@ def time_method(P, method, params, steps, warmup, require_eta=True):
                 converged=all(r["terminal_reason"] in ("converged", "direct_solve") for r in recs) and last["eta_max"] <= 1e-15)
 
 
+def bench_batch(ctx, count, steps, warmup, cpu):
+    """Uneven batch: `count` randsvd systems, n ~ U{32..512}, log10 kappa ~ U[1, 5], seeds 9300 + i, one right-hand
+    side each, generated on the GPU into one packed buffer; one mp_batch_solve launch per step (FP32 LU +
+    stationary IR with FP64 residuals, eta <= 1e-15 per system)."""
+    import concurrent.futures as cf
+    import threadpoolctl
+    from cpsc402_midterm_project_amd.problems import randsvd
+    rng = np.random.default_rng(9300)
+    n = rng.integers(32, 513, size=count).astype(np.int32)
+    kap = 10.0 ** rng.uniform(1, 5, size=count)
+    a_off = np.concatenate([[0], np.cumsum(n.astype(np.int64) ** 2)[:-1]])
+    b_off = np.concatenate([[0], np.cumsum(n.astype(np.int64))[:-1]])
+    tot_a, tot_b = int((n.astype(np.int64) ** 2).sum()), int(n.sum())
+    Ah, Bh = np.empty(tot_a), np.empty(tot_b)
+    t0 = time.time()
+
+    def gen(i):   # host randsvd (numpy QR of seeded Gaussians), written into the packed column-major buffers
+        k = int(n[i])
+        A, b, _, _ = randsvd(k, float(kap[i]), 9300 + i, nrhs=1, gpu=False)
+        Ah[a_off[i]:a_off[i] + k * k] = A.ravel(order="F"); Bh[b_off[i]:b_off[i] + k] = b.ravel(order="F")
+
+    workers = min(16, len(os.sched_getaffinity(0)))
+    with threadpoolctl.threadpool_limits(1), cf.ThreadPoolExecutor(workers) as ex:
+        list(ex.map(gen, range(count)))
+    dA, dB, dX = ctx.buffer(Ah), ctx.buffer(Bh), ctx.empty(8 * tot_b)
+    log(f"[batch] generated {count} systems (sum n^2 = {tot_a:.3e}) on {workers} host threads in {time.time()-t0:.1f}s")
+    recs = []
+    for i in range(warmup + steps):
+        res, rec = ctx.batch_solve_packed(n, np.ones(count, np.int32), dA, a_off, dB, b_off, dX, tau=1e-15, stop_on="eta")
+        log(f"  batch {'warm' if i < warmup else 'step'} {i}: {rec['device_s']*1e3:.2f} ms, converged {rec['converged']}/{count}, "
+            f"eta_worst {rec['eta_worst']:.2e}, outer total {rec['outer_iters_total']}")
+        if i >= warmup:
+            recs.append(rec)
+    ms = statistics.median(r["device_s"] * 1e3 for r in recs)
+    flops = float((2.0 / 3.0) * (n.astype(np.float64) ** 3).sum())
+    out = dict(workload=f"{count} randsvd systems, n~U{{32..512}}, log10 kappa~U[1,5], seeds 9300+i, nrhs=1",
+               method="FP32 LU (nb=32, FP32 trailing update) + stationary IR, FP64 residuals, one persistent launch",
+               ms_median=round(ms, 3), ms_all=[round(r["device_s"] * 1e3, 3) for r in recs],
+               systems_per_s=round(count / (ms * 1e-3), 1), converged=recs[-1]["converged"], systems=count,
+               eta_worst=recs[-1]["eta_worst"], outer_iters_total=recs[-1]["outer_iters_total"],
+               getrf_flops=flops, getrf_tflops_equiv=round(flops / (ms * 1e-3) / 1e12, 3))
+    if cpu:
+        cores = len(os.sched_getaffinity(0))
+        t = time.time()
+        with threadpoolctl.threadpool_limits(cores):
+            for i in range(count):
+                k = int(n[i]); A = Ah[a_off[i]:a_off[i] + k * k].reshape(k, k, order="F")
+                np.linalg.solve(A, Bh[b_off[i]:b_off[i] + k])
+        tc = time.time() - t
+        out["cpu_dgesv_ms"] = round(tc * 1e3, 1)
+        out["cpu_cores"] = cores
+        log(f"[batch cpu] numpy dgesv over {count} systems: {tc*1e3:.0f} ms on {cores} threads")
+    for b in (dA, dB, dX):
+        b.free()
+    return out
+
+
+def bench_serve(ctx, nreq, pool_size, loads=(0.5, 0.8, 0.95)):
+    """Serving stream (scaled): a pool of `pool_size` device-resident randsvd matrices, n log-uniform in
+    [256, 16384], kappa log-uniform in [1e1, 1e8]; each request has 1-16 right-hand sides; 20% of requests repeat
+    an earlier request's matrix id (cache hit if its factors are still resident), the others take a new id
+    (factorized afresh).  Saturation is measured by a burst of `nreq` requests; then Poisson streams of `nreq`
+    requests at 0.5x, 0.8x and 0.95x of it report p50/p99 arrival-to-completion latency and solves/s."""
+    from cpsc402_midterm_project_amd import Server
+    rng = np.random.default_rng(9400)
+    ns = np.exp(rng.uniform(np.log(256), np.log(16384), size=pool_size)).astype(np.int64)
+    ks = 10.0 ** rng.uniform(1, 8, size=pool_size)
+    t0 = time.time()
+    pool = []
+    for j in range(pool_size):
+        k = int(ns[j])
+        dA, dB = ctx.empty(8 * k * k), ctx.empty(8 * k * 16)
+        ctx.generate_into(dA, dB, None, family="randsvd", n=k, nrhs=16, kappa=float(ks[j]), seed=9400 + j, rhs="xtrue_uniform")
+        pool.append((k, dA, dB))
+    log(f"[serve] pool of {pool_size} matrices (n {ns.min()}..{ns.max()}) generated in {time.time()-t0:.1f}s")
+
+    def stream(rate, seed):
+        r = np.random.default_rng(seed)
+        reqs, ids = [], []
+        next_id = 1
+        for i in range(nreq):
+            if ids and r.random() < 0.2:
+                mid, j = ids[int(r.integers(len(ids)))]
+            else:
+                mid, j = next_id, int(r.integers(pool_size)); next_id += 1
+                ids.append((mid, j))
+            reqs.append((mid, j, int(r.integers(1, 17))))
+        gaps = r.exponential(1.0 / rate, size=nreq) if rate else np.zeros(nreq)
+        return reqs, np.cumsum(gaps)
+
+    def run(rate, seed, tag):
+        reqs, arr = stream(rate, seed)
+        with Server(tau=1e-15, hbm_budget_bytes=200 << 30, solve={"nb": 256}) as S:
+            outs = {}
+            t_start = Server.clock()
+            done = []
+            for (mid, j, nrhs), a in zip(reqs, arr):
+                k, dA, dB = pool[j]
+                while Server.clock() < t_start + a:
+                    done += S.poll(wait=min(0.002, max(0.0, t_start + a - Server.clock())))
+                X = np.zeros((k, nrhs), order="F")
+                rid = S.submit(mid, k, nrhs, dA, dB, X, arrival=t_start + a)
+                outs[rid] = X
+            while len(done) < len(reqs):
+                done += S.poll(wait=0.05)
+            t_end = Server.clock()
+            st = S.stats()
+        lat = np.array([c["latency_s"] for c in done]) * 1e3
+        ok = sum(1 for c in done if c["terminal"] == "converged" and c["eta_max"] <= 1e-15)
+        res = dict(offered_per_s=None if not rate else round(rate, 2), requests=len(done),
+                   solves_per_s=round(len(done) / (t_end - t_start), 2), p50_ms=round(float(np.percentile(lat, 50)), 2),
+                   p99_ms=round(float(np.percentile(lat, 99)), 2), mean_ms=round(float(lat.mean()), 2),
+                   converged=ok, cache_hits=st["cache_hits"], batches=st["batches"], small=st["small_requests"],
+                   large=st["large_requests"], errors=st["errors"])
+        log(f"[serve {tag}] {res}")
+        return res
+
+    burst = run(None, 9401, "burst")
+    sat = burst["solves_per_s"]
+    out = dict(workload=f"{nreq} requests per stream; pool of {pool_size} randsvd matrices n log-uniform [256,16384], "
+                        f"kappa log-uniform [1e1,1e8], 1-16 rhs, 20% repeated ids, eta<=1e-15, HBM budget 200 GiB",
+               saturation_solves_per_s=sat, burst=burst, loads={})
+    for f in loads:
+        out["loads"][str(f)] = run(f * sat, 9402 + int(f * 100), f"{f}x")
+    for _, dA, dB in pool:
+        dA.free(); dB.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +201,9 @@ def main():
                     help="low-precision candidates (FP16 MFMA trailing update); the fastest converged one is reported")
     ap.add_argument("--fallback", default="mx32_gmresir_w64_rdd")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=4096, help="uneven-batch leg: systems per launch (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=240, help="serving leg: requests per stream (0: skip)")
+    ap.add_argument("--serve-pool", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this solver build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -130,6 +266,12 @@ def main():
                                   f"kappa={kappa:g} system, one run, eta={eta:.2e}")
                 log(f"[cpu] dgesv {tc*1e3:.0f} ms on {cores} threads")
         P.close()
+    batch = None
+    if args.batch_count > 0:
+        batch = bench_batch(ctx, args.batch_count, args.steps, args.warmup, not args.no_cpu)
+    serve = None
+    if args.serve_requests > 0:
+        serve = bench_serve(ctx, args.serve_requests, args.serve_pool)
     ctx.close()
 
     # roofline of the dominant kernel (largest total time among the profiled launches)
@@ -162,7 +304,10 @@ def main():
         "fp64_lu_ms": round(hb["fp64_lu"]["ms_median"], 3),
         "speedup_vs_fp64_lu": None if hb["speedup_vs_fp64_lu"] is None else round(hb["speedup_vs_fp64_lu"], 3),
         "converged": bool(hb.get("best")),
-        "serve_p50_ms": None, "serve_p99_ms": None,
+        "serve_p50_ms": serve["loads"]["0.8"]["p50_ms"] if serve else None,
+        "serve_p99_ms": serve["loads"]["0.8"]["p99_ms"] if serve else None,
+        "serve": serve,
+        "batch": batch,
         "roofline": roof,
         "cpu_baseline": cpu,
         "cases": cases,

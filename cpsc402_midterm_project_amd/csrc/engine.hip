@@ -180,6 +180,7 @@ static MethodSpec make_method(const std::string& id, const Params& a) {
   }
   else if (id == "policy_mx8_ladder") { M.factor = "mx8"; M.scale_mode = 1; M.gmres = true; M.resid = 128; M.policy = "ladder"; }
   else if (id == "policy_mx16_ladder") { M.factor = "mx16"; M.scale_mode = 1; M.gmres = true; M.resid = 128; M.policy = "ladder16"; }
+  else if (id == "policy_mx16x3_ladder") { M.factor = "mx16x3"; M.scale_mode = 1; M.gmres = true; M.resid = 128; M.policy = "ladder16x3"; }
   else throw MPError("unknown method id " + id, -1);
   if (M.scale_mode == 2) unsupported("two-sided row/column scaling (scale_mode 2) is not implemented");
   if (M.apply64) unsupported("promoted FP64 factor application (apply64) is not implemented");
@@ -200,7 +201,7 @@ static const char* kMethodIds[] = {
   "mxbf16_lu_direct", "mxbf16_ir_w64_r64", "mxbf16_ir_w64_rdd", "mxbf16_gmresir_w64_r64", "mxbf16_gmresir_w64_rdd",
   "mx8_lu_direct", "mx8_ir_w64_r64", "mx8_ir_w64_rdd", "mx8_gmresir_w64_r64", "mx8_gmresir_w64_rdd",
   "mx32_ir_w64_rdd", "mx32_gmresir_w64_rdd", "mx16x3_lu_direct", "mx16x3_ir_w64_rdd", "mx16x3_gmresir_w64_rdd",
-  "policy_mx8_ladder", "policy_mx16_ladder"};
+  "policy_mx8_ladder", "policy_mx16_ladder", "policy_mx16x3_ladder"};
 
 static std::string factor_contract(const std::string& f) {
   if (f == "cs32") return "vendor FP32 LU with partial pivoting (rocSOLVER sgetrf) on the FP32 copy of A";
@@ -275,7 +276,8 @@ static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, i
     F.LU64.ensure(C->tracker, nn, "LU64");
     MP_REQUIRE(!P.fp32, "FP64 factors of FP32-stored problems are not implemented");
     HIP_CK(hipMemcpyAsync(F.LU64.p, P.A64.p, nn * 8, hipMemcpyDeviceToDevice, W.stream));
-    MP_REQUIRE(F.gscale == 1.0, "global scaling with FP64 vendor factors is not implemented");
+    // the reference scales the FP64 copy too (src/mp_core.cuh:167) and applies the factors with g
+    if (F.gscale != 1.0) axpy_scale_inplace(F.LU64.p, (int64_t)nn, F.gscale, W);
   }
   if (kind == "cs32" || kind == "cs64") {
     F.ipiv32.ensure(C->tracker, n + 1, "ipiv32");
@@ -312,6 +314,7 @@ static void apply_inverse(mp_ctx* C, const Factors& F, const double* r, double* 
     RB_CK(rocblas_set_stream(C->rb, W.stream));
     RB_CK(rocsolver_dgetrs(C->rb, rocblas_operation_none, (rocblas_int)n, (rocblas_int)nrhs, const_cast<double*>(F.LU64.p),
                            (rocblas_int)n, F.ipiv32.p, d, (rocblas_int)n));
+    if (F.gscale != 1.0) axpy_scale_inplace(d, (int64_t)n * nrhs, F.gscale, W);   // (gA)^{-1} r = A^{-1} r / g
   } else {
     getrs(n, nrhs, F.LU32.p, n, F.perm.p, r, n, d, n, F.gscale, W, true);
   }
@@ -397,7 +400,7 @@ struct TrialOut {
 };
 
 struct Engine {
-  mp_ctx* C; const mp_problem& P; Factors fact; bool reuse = false; bool instrument = false;
+  mp_ctx* C; const mp_problem& P; Factors fact; bool reuse = false; bool resume_ladder = false; bool instrument = false;
   int64_t n; int nrhs; double tau_eta, tau_omega; bool stop_eta_only = false;
   DBuf<double> x64, D, y; DBuf<float> x32;
   EvalBufs E;
@@ -578,6 +581,7 @@ struct Engine {
     const bool target64 = (tau_eta < 1e-10);
     if (M.policy == "ladder") { ladder = {"mx8", "mx16", "cus32"}; }
     else if (M.policy == "ladder16") { ladder = {"mx16", "mx16x3", "cus32"}; }
+    else if (M.policy == "ladder16x3") { ladder = {"mx16x3", "cus32", "cs64"}; }
     else if (M.policy != "none") {
       if (M.factor == "het16" || M.factor == "cus32") { ladder.push_back("cs32"); if (target64) ladder.push_back("cs64"); }
       else if (M.factor == "cs32") { if (target64) ladder.push_back("cs64"); }
@@ -586,7 +590,12 @@ struct Engine {
     if (M.working == 32) { x32.ensure(C->tracker, (size_t)n * nrhs, "x32"); HIP_CK(hipMemsetAsync(x32.p, 0, (size_t)n * nrhs * 4, W().stream)); }
     Metrics Mt; std::string reason = "unknown"; MethodSpec Ms = M;
     bool initialized = false;
-    for (int stage = 0; stage < (int)ladder.size(); stage++) {
+    int first = 0;
+    // resume_ladder: with reuse, start at the stage whose factors the problem keeps (the stage that served the
+    // last solve of this matrix), so repeated solves skip the stages that failed before
+    if (reuse && resume_ladder && P.fcache && P.fcache->valid)
+      for (int s = 0; s < (int)ladder.size(); s++) if (ladder[s] == P.fcache->kind) { first = s; break; }
+    for (int stage = first; stage < (int)ladder.size(); stage++) {
       Factors& F = factors_for(M, ladder[stage], T);
       T.final_stage = ladder[stage];
       if (stage > 0) T.fallback = true;
@@ -598,7 +607,7 @@ struct Engine {
       if (M.policy == "budget" && M.budget == 0) Ms.gmres = false;
       if (M.policy == "budget" && M.budget > 0) { if (stage == 0) { Ms.gmres = true; budget_left = M.budget; } else Ms.gmres = false; }
       if (M.policy == "cost_aware") Ms.gmres = false;
-      if (M.policy == "ladder" || M.policy == "ladder16") { Ms.gmres = true; budget_left = M.gmres_budget; }
+      if (M.policy == "ladder" || M.policy == "ladder16" || M.policy == "ladder16x3") { Ms.gmres = true; budget_left = M.gmres_budget; }
       const double ts = wall_now();
       reason = refine_stage(Ms, F, stage, T, t0, budget_left, !initialized, Mt);
       initialized = true;
@@ -1207,6 +1216,7 @@ int mp_solve(mp_ctx* C, mp_problem* P, const char* method_id, const char* params
       if (!skipped && T.ok) {
         Engine E(C, *P, tau, tau);
         E.instrument = a.getb("instrument", false); E.stop_eta_only = eta_only; E.reuse = a.getb("reuse", false);
+        E.resume_ladder = a.getb("resume_ladder", false);
         HIP_CK(hipStreamSynchronize(C->W.stream));
         const double t1 = wall_now();
         E.run(M, T);

@@ -7,9 +7,11 @@
 //   * systems with n <= 512 and nrhs <= 16 are collected while the previous batch runs and solved together by
 //     one persistent batched launch (mp_batch_solve_ptrs): requests that arrive while others are on the GPU
 //     join the next launch as soon as it starts;
-//   * larger systems are solved one at a time by the configured method (default: two-term FP16 LU +
-//     FGMRES-IR with double-double residuals) on the whole GPU; a matrix id seen before reuses its problem and
-//     the factors cached with it (mp_problem_set_rhs + "reuse"), so a repeat costs only the refinement.
+//   * larger systems are solved one at a time by the configured method on the whole GPU (default: two-term FP16
+//     LU + FGMRES-IR with double-double residuals, escalating to FP32 and then FP64 factors when a stage stops
+//     short of tau); a matrix id seen before reuses its problem and the factors cached with it
+//     (mp_problem_set_rhs + "reuse", starting at the stage that served it last), so a repeat costs only the
+//     refinement.
 // Admission: the bytes the server holds (cached factors, the running large solve, batch buffers) stay within
 // the HBM budget; the least recently used cached factors are released first, and a large request waits at
 // the head of its queue until its bytes fit.
@@ -59,7 +61,7 @@ struct mp_server {
   int device = 0;
   mp_ctx* big = nullptr;
   mp_ctx* small = nullptr;
-  std::string method = "mx16x3_gmresir_w64_rdd";
+  std::string method = "policy_mx16x3_ladder";
   std::string solve_params, batch_params;
   size_t budget = (size_t)200 << 30;
   int max_batch = 4096;
@@ -91,9 +93,10 @@ struct mp_server {
     done.push_back(c);
   }
 
-  // bytes a large request holds while it runs: FP32 factors, FGMRES basis (m = 30) and solve vectors
+  // bytes a large request holds while it runs: factors (FP32, or FP64 at the last stage of a ladder), the
+  // FGMRES basis (m = 30) and the solve vectors
   static size_t big_need(int64_t n, int nrhs) {
-    return (size_t)n * n * 4 + (size_t)n * (2 * 30 + 2) * 8 + (size_t)n * nrhs * 8 * 8;
+    return (size_t)n * n * 8 + (size_t)n * (2 * 30 + 2) * 8 + (size_t)n * nrhs * 8 * 8;
   }
 
   void evict_lru_locked() {
@@ -161,7 +164,7 @@ struct mp_server {
         if (P) mp_problem_free(P);
         finish(r, 1, hit ? 1 : 0, 5, 0, -1, -1, t_done);
       } else {
-        const size_t fb = (size_t)r.n * r.n * 4;
+        const size_t fb = (size_t)r.n * r.n * (std::strcmp(out.final_stage, "cs64") == 0 ? 8 : 4);   // factors kept
         cache[r.matrix] = Entry{P, fb, ++tick};
         cache_bytes += fb;
         finish(r, 1, hit ? 1 : 0, out.ok ? terminal_code(out.terminal_reason) : 5, out.outer_iters, out.eta_max,
@@ -282,7 +285,7 @@ int mp_server_create(int device, const char* params_json, mp_server** out) {
       if (o.size() <= 2) return "{" + extra + "}";
       return o.substr(0, o.size() - 1) + "," + extra + "}";
     };
-    S->solve_params = with(sp, "\"tau\":" + tau + ",\"stop_on\":\"eta\",\"reuse\":true");
+    S->solve_params = with(sp, "\"tau\":" + tau + ",\"stop_on\":\"eta\",\"reuse\":true,\"resume_ladder\":true");
     S->batch_params = with(bp, "\"tau\":" + tau + ",\"stop_on\":\"eta\"");
     if (mp_create(&S->big, device) != MP_OK) throw MPError(std::string("mp_create: ") + mp_last_error(), MP_ERR_NO_DEVICE);
     if (mp_create(&S->small, device) != MP_OK) {

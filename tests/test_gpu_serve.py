@@ -109,3 +109,21 @@ def test_server_results_equal_alone(ctx, budget):
             Xa, ra = _alone_large(ctx, mats[mid], dB, n, nrhs, 128)
             assert ra["outer_iters"] == c["outer_iters"]
         np.testing.assert_array_equal(X, Xa)
+
+
+def test_ladder_reaches_tau_and_resumes_at_final_stage(ctx):
+    """kappa = 1e8: the two-term FP16 stage stops short of 1e-15 and the ladder escalates; with reuse and
+    resume_ladder a second solve of the same matrix starts at the stage that converged, with its factors."""
+    n = 1000
+    print(f"[ladder] n={n} kappa=1e8", flush=True)
+    dA, dB = _gen(ctx, n, 1, 1e8, 77)
+    P = ctx.problem_from_device(n, 1, dA, dB)
+    params = dict(nb=128, tau=TAU, stop_on="eta", reuse=True, resume_ladder=True)
+    X1, r1 = P.solve("policy_mx16x3_ladder", **params)
+    assert r1["terminal_reason"] == "converged" and r1["eta_max"] <= TAU, (r1["terminal_reason"], r1["stages"], r1["error"])
+    X2, r2 = P.solve("policy_mx16x3_ladder", **params)
+    assert r2["terminal_reason"] == "converged" and r2["refactorizations"] == 0
+    assert r2["stages"][0] == r1["final_stage"] + "(reused)", (r1["stages"], r2["stages"])
+    if r1["stages"] == [r1["final_stage"]]:       # no escalation: the same stage from the same start
+        np.testing.assert_array_equal(X1, X2)
+    P.close()

@@ -74,6 +74,7 @@ struct Lds {
   double red[4][4][4];              // [wave][rhs][stat]
   double stat[BRHS][4];             // per right-hand side: rinf, xinf, omega, eta
   double binf[BRHS];
+  float ys[BRHS][BNMAX];            // right-hand sides / solutions of the FP32 substitutions
   double anorm, amax;
   int misc[8];
 };
@@ -96,6 +97,7 @@ __device__ void setup(Lds& S, const double* __restrict__ A, int n, int ldr, floa
   double am = 0.0, rs = 0.0;
   for (int i = t; i < n; i += BT) {
     double s = 0.0;
+#pragma unroll 16
     for (int j = 0; j < n; j++) {
       const double v = fabs(A[i + (int64_t)j * n]);
       am = nanmax(am, v);
@@ -349,102 +351,85 @@ __device__ void to_colmajor(Lds& S, const float* __restrict__ LU, int n, int ldr
     }
 }
 
-// ------------------------------------------------------------------ substitutions (one wave, NR right-hand sides)
-// X[:, c] (=|+=) g * U^{-1} L^{-1} (float) src[perm, c] for c = c_first + 4 r, r < NR.  Same FMA order as the
-// row-oriented substitution: row i receives column j's term in increasing (forward) / decreasing (backward) j.
-template <int NR>
-__device__ void wave_solve(const Lds& S, const float* __restrict__ LUc, int n, const double* __restrict__ src,
-                           double* __restrict__ X, int c_first, double g, bool first) {
+// ------------------------------------------------------------------ substitutions
+// X[:, c] (=|+=) g * U^{-1} L^{-1} (float) src[perm, c], blocked by 64 columns.  The diagonal block is solved
+// by one wave per right-hand side (row jb + lane in the lane, its 64 factor values in registers, each solved
+// value broadcast by readlane); the rows beyond the block then take the block's 64 terms in one parallel pass
+// (thread per row).  Every row receives its terms in increasing (forward) / decreasing (backward) column order,
+// so the result equals the sequential row-oriented substitution.
+__device__ void solve_all(Lds& S, const float* __restrict__ LUc, int n, int nrhs, const double* __restrict__ src,
+                          double* __restrict__ X, double g, bool first) {
 #pragma clang fp contract(off)
-  const int lane = threadIdx.x & 63;
-  float y[NR][8];
-#pragma unroll
-  for (int r = 0; r < NR; r++)
-#pragma unroll
-    for (int s = 0; s < 8; s++) {
-      const int i = lane + 64 * s;
-      const float v = (float)src[S.rowat[min(i, n - 1)] + (int64_t)(c_first + 4 * r) * n];
-      y[r][s] = (i < n) ? v : 0.0f;
-    }
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  for (int c = 0; c < nrhs; c++)
+    for (int i = t; i < n; i += BT) S.ys[c][i] = (float)src[S.rowat[i] + (int64_t)c * n];
+  __syncthreads();
   // forward, unit lower
+  for (int jb = 0; jb < n; jb += 64) {
+    const int nbk = min(64, n - jb);
+    if (w < nrhs) {
+      const int i = jb + lane, ic = min(i, n - 1);
+      float L[64];
 #pragma unroll
-  for (int s = 0; s < 8; s++) {
-    if (64 * s >= n) break;
-    const int jend = min(64, n - 64 * s);
-#pragma unroll 4
-    for (int jj = 0; jj < jend; jj++) {
-      const int j = 64 * s + jj;
-      float col[8];
+      for (int jj = 0; jj < 64; jj++) L[jj] = LUc[ic + (int64_t)min(jb + jj, n - 1) * n];
+      for (int c = w; c < nrhs; c += 4) {
+        float y = S.ys[c][ic];
 #pragma unroll
-      for (int s2 = 0; s2 < 8; s2++) {
-        if (s2 >= s) {
-          const int i = lane + 64 * s2;
-          const float v = LUc[min(i, n - 1) + (int64_t)j * n];
-          col[s2] = (i < n) ? v : 0.0f;
+        for (int jj = 0; jj < 64; jj++) {
+          if (jj < nbk) {
+            const float yj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(y), jj));
+            if (lane > jj) y = fmaf(-L[jj], yj, y);
+          }
         }
-      }
-#pragma unroll
-      for (int r = 0; r < NR; r++) {
-        const float yj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(y[r][s]), jj));
-        y[r][s] = (lane > jj) ? fmaf(-col[s], yj, y[r][s]) : y[r][s];
-#pragma unroll
-        for (int s2 = 0; s2 < 8; s2++)
-          if (s2 > s) y[r][s2] = fmaf(-col[s2], yj, y[r][s2]);
+        if (i < n) S.ys[c][i] = y;
       }
     }
+    __syncthreads();
+    for (int i = jb + 64 + t; i < n; i += BT)
+      for (int c = 0; c < nrhs; c++) {
+        float acc = S.ys[c][i];
+#pragma unroll 16
+        for (int jj = 0; jj < 64; jj++) acc = fmaf(-LUc[i + (int64_t)(jb + jj) * n], S.ys[c][jb + jj], acc);
+        S.ys[c][i] = acc;
+      }
+    __syncthreads();
   }
   // backward, upper with division by the diagonal
+  for (int jb = ((n - 1) / 64) * 64; jb >= 0; jb -= 64) {
+    const int nbk = min(64, n - jb);
+    if (w < nrhs) {
+      const int i = jb + lane, ic = min(i, n - 1);
+      float U[64];
 #pragma unroll
-  for (int s = 7; s >= 0; s--) {
-    if (64 * s >= n) continue;
-    const int jend = min(64, n - 64 * s);
-#pragma unroll 4
-    for (int jj = jend - 1; jj >= 0; jj--) {
-      const int j = 64 * s + jj;
-      float col[8];
+      for (int jj = 0; jj < 64; jj++) U[jj] = LUc[ic + (int64_t)min(jb + jj, n - 1) * n];
+      for (int c = w; c < nrhs; c += 4) {
+        float y = S.ys[c][ic];
 #pragma unroll
-      for (int s2 = 0; s2 < 8; s2++) {
-        if (s2 <= s) {
-          const int i = lane + 64 * s2;
-          const float v = LUc[min(i, n - 1) + (int64_t)j * n];
-          col[s2] = (i < n) ? v : 1.0f;
+        for (int jj = 63; jj >= 0; jj--) {
+          if (jj < nbk) {
+            if (lane == jj) y = y / U[jj];
+            const float yj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(y), jj));
+            if (lane < jj) y = fmaf(-U[jj], yj, y);
+          }
         }
-      }
-#pragma unroll
-      for (int r = 0; r < NR; r++) {
-        if (lane == jj) y[r][s] = y[r][s] / col[s];
-        const float yj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(y[r][s]), jj));
-        y[r][s] = (lane < jj) ? fmaf(-col[s], yj, y[r][s]) : y[r][s];
-#pragma unroll
-        for (int s2 = 0; s2 < 8; s2++)
-          if (s2 < s) y[r][s2] = fmaf(-col[s2], yj, y[r][s2]);
+        if (i < n) S.ys[c][i] = y;
       }
     }
+    __syncthreads();
+    for (int i = t; i < jb; i += BT)
+      for (int c = 0; c < nrhs; c++) {
+        float acc = S.ys[c][i];
+        for (int jj = nbk - 1; jj >= 0; jj--) acc = fmaf(-LUc[i + (int64_t)(jb + jj) * n], S.ys[c][jb + jj], acc);
+        S.ys[c][i] = acc;
+      }
+    __syncthreads();
   }
-#pragma unroll
-  for (int r = 0; r < NR; r++)
-#pragma unroll
-    for (int s = 0; s < 8; s++) {
-      const int i = lane + 64 * s;
-      if (i < n) {
-        double* xp = X + i + (int64_t)(c_first + 4 * r) * n;
-        const double d = g * (double)y[r][s];
-        *xp = first ? d : *xp + d;
-      }
+  for (int c = 0; c < nrhs; c++)
+    for (int i = t; i < n; i += BT) {
+      double* xp = X + i + (int64_t)c * n;
+      const double d = g * (double)S.ys[c][i];
+      *xp = first ? d : *xp + d;
     }
-}
-
-__device__ void solve_all(const Lds& S, const float* LUc, int n, int nrhs, const double* src, double* X, double g, bool first) {
-  const int w = threadIdx.x >> 6;
-  const int nr = (nrhs - w + 3) / 4;           // right-hand sides w, w+4, ... of this wave
-  if (w < nrhs) {
-    switch (nr) {
-      case 1: wave_solve<1>(S, LUc, n, src, X, w, g, first); break;
-      case 2: wave_solve<2>(S, LUc, n, src, X, w, g, first); break;
-      case 3: wave_solve<3>(S, LUc, n, src, X, w, g, first); break;
-      default: wave_solve<4>(S, LUc, n, src, X, w, g, first); break;
-    }
-  }
   __syncthreads();
 }
 
@@ -470,6 +455,7 @@ __device__ void resid_chunk(Lds& S, const double* __restrict__ A, const double* 
     __syncthreads();
     const int je = min(64, n - jb);
     if (i0 < n) {
+#pragma unroll 8
       for (int jl = 0; jl < je; jl++) {
         const int64_t off = (int64_t)(jb + jl) * n;
         const double a0 = A[i0 + off];

@@ -26,7 +26,7 @@ EXPORTS = [
     "mp_getrs", "mp_problem_from_host", "mp_problem_generate", "mp_problem_reference", "mp_problem_free", "mp_problem_copy_out", "mp_problem_copy_ref", "mp_problem_case_json", "mp_solve",
     "mp_last_record_json", "mp_last_hist_json", "mp_methods_json", "mp_eval_metrics_json", "mp_batch_solve",
     "mp_batch_solve_ptrs", "mp_generate_into", "mp_problem_from_device", "mp_problem_set_rhs", "mp_server_clock",
-    "mp_server_create", "mp_server_submit", "mp_server_poll", "mp_server_stats_json", "mp_server_destroy",
+    "mp_server_create", "mp_server_submit", "mp_server_poll", "mp_server_stats_json", "mp_server_destroy", "mp_getrf64",
 ]
 SERVER_TERMINAL = ["converged", "nonfinite", "max_outer", "stagnation", "singular_factor", "error", "other"]
 BATCH_TERMINAL = ["converged", "nonfinite", "max_outer", "stagnation", "singular_factor"]
@@ -109,6 +109,7 @@ def _load() -> C.CDLL:
         "mp_server_poll": ([vp, vp, i32, C.c_double, C.POINTER(i32)], C.c_int),
         "mp_server_stats_json": ([vp, C.c_char_p, sz], C.c_int),
         "mp_server_destroy": ([vp], C.c_int),
+        "mp_getrf64": ([vp, i64, dp, i64, vp, i32, C.POINTER(i32)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -234,6 +235,11 @@ class Context:
         info = GetrfInfo()
         check(lib.mp_getrf(self.h, n, _ptr(LU), lda, _ptr(ipiv), _ptr(perm), C.byref(p), C.byref(info)))
         return info
+
+    def getrf64(self, n, A, lda, ipiv, nb=128) -> int:
+        info = C.c_int32()
+        check(lib.mp_getrf64(self.h, n, _ptr(A), lda, _ptr(ipiv), nb, C.byref(info)))
+        return info.value
 
     def getrs(self, n, nrhs, LU, lda, perm, R, ldr, D, ldd, scale=1.0):
         check(lib.mp_getrs(self.h, n, nrhs, _ptr(LU), lda, _ptr(perm), _ptr(R), ldr, _ptr(D), ldd, float(scale)))

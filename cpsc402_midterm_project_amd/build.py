@@ -27,7 +27,7 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", f"--offload-arch={ARCH}", "-I", str(ROOT / "include"),
           "-Wno-unused-result", "-Wno-unused-command-line-argument"]
 
-LIB_SOURCES = ["k_resid.hip", "k_getrf.hip", "k_getrs.hip", "k_krylov.hip", "k_batch.hip", "generate.hip", "engine.hip",
+LIB_SOURCES = ["k_resid.hip", "k_getrf.hip", "k_getrs.hip", "k_krylov.hip", "k_batch.hip", "k_getrf64.hip", "generate.hip", "engine.hip",
                "sha256.cpp", "server.cpp"]
 CLI_SOURCES = ["cli.cpp"]
 

@@ -86,6 +86,7 @@ struct mp_ctx {
   DevTracker tracker;
   Workspace W;
   rocblas_handle rb = nullptr;
+  rocblas_handle rb_panel = nullptr;    // bound to W.pstream: FP64 LU look-ahead panels
   double* hpin = nullptr;
   std::string last_record, last_hist;
   std::unique_ptr<Krylov> kry;
@@ -167,6 +168,7 @@ static MethodSpec make_method(const std::string& id, const Params& a) {
   else if (starts("magma_")) { unsupported("MAGMA is not linked on this platform"); }
   // ---- MI355X method IDs: per-tile power-of-two scaled low-precision trailing updates
   //      mx{16,bf16,8}_{lu_direct | ir_w64_r64 | ir_w64_rdd | gmresir_w64_r64 | gmresir_w64_rdd}, policy_mx8_ladder
+  else if (id == "mx64_lu_direct") { M.kind = "direct"; M.factor = "mx64"; M.max_outer = 0; M.nb = a.geti("nb", 128); }
   else if (starts("mx16_") || starts("mxbf16_") || starts("mx8_") || starts("mx32_") || starts("mx16x3_")) {
     const size_t us = id.find('_');
     const std::string pre = id.substr(0, us), rest = id.substr(us + 1);
@@ -201,11 +203,13 @@ static const char* kMethodIds[] = {
   "mxbf16_lu_direct", "mxbf16_ir_w64_r64", "mxbf16_ir_w64_rdd", "mxbf16_gmresir_w64_r64", "mxbf16_gmresir_w64_rdd",
   "mx8_lu_direct", "mx8_ir_w64_r64", "mx8_ir_w64_rdd", "mx8_gmresir_w64_r64", "mx8_gmresir_w64_rdd",
   "mx32_ir_w64_rdd", "mx32_gmresir_w64_rdd", "mx16x3_lu_direct", "mx16x3_ir_w64_rdd", "mx16x3_gmresir_w64_rdd",
-  "policy_mx8_ladder", "policy_mx16_ladder", "policy_mx16x3_ladder"};
+  "policy_mx8_ladder", "policy_mx16_ladder", "policy_mx16x3_ladder", "mx64_lu_direct"};
 
 static std::string factor_contract(const std::string& f) {
   if (f == "cs32") return "vendor FP32 LU with partial pivoting (rocSOLVER sgetrf) on the FP32 copy of A";
   if (f == "cs64") return "vendor FP64 LU with partial pivoting (rocSOLVER dgetrf) on an FP64 copy of A";
+  if (f == "mx64") return "own FP64 blocked LU with partial pivoting: vendor panel (rocSOLVER dgetrf on m x nb) and U12 "
+                          "(rocBLAS dtrsm), trailing update A22 -= L21*U12 on v_mfma_f64_16x16x4_f64, look-ahead on two streams";
   const std::string base = "own right-looking blocked LU (HIP, gfx950): FP32 storage; FP32 panel with partial pivoting "
                            "(largest |a|, ties to the smallest row); FP32 laswp and unit-lower TRSM for U12; trailing update "
                            "A22 -= L21*U12 on MFMA with FP32 accumulation and operands rounded (RNE) to ";
@@ -256,7 +260,7 @@ static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, i
   W.factor_gen++;   // cached getrs block inverses of any earlier factors are stale from here on
   F.n = n; F.kind = kind; F.scale_mode = scale_mode; F.nb = nb; F.valid = false; F.growth = -1;
   F.conv_overflow = F.conv_subnormal = 0; F.info = 0; F.panels = 0;
-  F.prec = (kind == "cs64") ? 64 : 32;
+  F.prec = (kind == "cs64" || kind == "mx64") ? 64 : 32;
   F.ipiv.ensure(C->tracker, n, "ipiv");
   F.perm.ensure(C->tracker, n, "perm");
   F.gscale = 1.0; F.gscale_exp = 0;
@@ -279,15 +283,21 @@ static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, i
     // the reference scales the FP64 copy too (src/mp_core.cuh:167) and applies the factors with g
     if (F.gscale != 1.0) axpy_scale_inplace(F.LU64.p, (int64_t)nn, F.gscale, W);
   }
-  if (kind == "cs32" || kind == "cs64") {
+  if (kind == "cs32" || kind == "cs64" || kind == "mx64") {
     F.ipiv32.ensure(C->tracker, n + 1, "ipiv32");
     RB_CK(rocblas_set_stream(C->rb, W.stream));
+    int info64 = 0;
     if (kind == "cs32") RB_CK(rocsolver_sgetrf(C->rb, (rocblas_int)n, (rocblas_int)n, F.LU32.p, (rocblas_int)n, F.ipiv32.p, F.ipiv32.p + n));
-    else RB_CK(rocsolver_dgetrf(C->rb, (rocblas_int)n, (rocblas_int)n, F.LU64.p, (rocblas_int)n, F.ipiv32.p, F.ipiv32.p + n));
+    else if (kind == "cs64") RB_CK(rocsolver_dgetrf(C->rb, (rocblas_int)n, (rocblas_int)n, F.LU64.p, (rocblas_int)n, F.ipiv32.p, F.ipiv32.p + n));
+    else {
+      const char* ov = std::getenv("MPSOLVE_OVERLAP");
+      getrf64(n, F.LU64.p, n, F.ipiv32.p, nb, !(ov && ov[0] == '0'), (void*)C->rb, (void*)C->rb_panel, W, &info64);
+      RB_CK(rocblas_set_stream(C->rb, W.stream));
+    }
     std::vector<int> ip(n + 1);
-    HIP_CK(hipMemcpyAsync(ip.data(), F.ipiv32.p, (n + 1) * sizeof(int), hipMemcpyDeviceToHost, W.stream));
+    HIP_CK(hipMemcpyAsync(ip.data(), F.ipiv32.p, (kind == "mx64" ? n : n + 1) * sizeof(int), hipMemcpyDeviceToHost, W.stream));
     HIP_CK(hipStreamSynchronize(W.stream));
-    F.info = ip[n];
+    F.info = (kind == "mx64") ? info64 : ip[n];
     ip.resize(n);
     std::vector<int32_t> perm; perm_from_ipiv32(ip, perm);
     std::vector<int64_t> ip64(ip.begin(), ip.end());
@@ -737,6 +747,8 @@ int mp_create(mp_ctx** out, int device) {
     C->W.num_cus = prop.multiProcessorCount;
     RB_CK(rocblas_create_handle(&C->rb));
     RB_CK(rocblas_set_stream(C->rb, C->W.stream));
+    RB_CK(rocblas_create_handle(&C->rb_panel));
+    RB_CK(rocblas_set_stream(C->rb_panel, C->W.pstream));
     HIP_CK(hipHostMalloc((void**)&C->hpin, 4096 * sizeof(double), hipHostMallocDefault));
     C->W.dinfo.ensure(C->tracker, 4, "getrf_info");
     HIP_CK(hipMemsetAsync(C->W.dinfo.p, 0, 16, C->W.stream));
@@ -754,6 +766,7 @@ void mp_destroy(mp_ctx* C) {
   C->problems.clear();
   C->kry.reset();
   if (C->rb) rocblas_destroy_handle(C->rb);
+  if (C->rb_panel) rocblas_destroy_handle(C->rb_panel);
   if (C->hpin) (void)hipHostFree(C->hpin);
   (void)hipStreamSynchronize(C->W.pstream);
   hipStream_t s = C->W.stream, ps = C->W.pstream;
@@ -891,6 +904,18 @@ int mp_batch_solve(mp_ctx* C, int32_t count, const int32_t* n, const int32_t* nr
     double dev_s = 0;
     batch_solve(count, n, nrhs, pa.data(), pb.data(), px.data(), bp, C->W, r.data(), &dev_s);
     batch_record(C, count, n, bp, r, dev_s, results);
+  });
+}
+
+int mp_getrf64(mp_ctx* C, int64_t n, double* A, int64_t lda, int32_t* ipiv, int32_t nb, int32_t* info) {
+  return guard([&] {
+    MP_REQUIRE(C && A && ipiv, "NULL argument");
+    MP_REQUIRE(nb >= 4 && nb % 4 == 0, "nb must be a positive multiple of 4");
+    const char* ov = std::getenv("MPSOLVE_OVERLAP");
+    int inf = 0;
+    getrf64(n, A, lda, ipiv, nb, !(ov && ov[0] == '0'), (void*)C->rb, (void*)C->rb_panel, C->W, &inf);
+    RB_CK(rocblas_set_stream(C->rb, C->W.stream));
+    if (info) *info = inf;
   });
 }
 

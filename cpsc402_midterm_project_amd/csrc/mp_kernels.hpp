@@ -59,6 +59,8 @@ struct Workspace {
   DBuf<double> bresid;   // per-system residuals
   DBuf<uint8_t> bres;    // per-system outcomes
   DBuf<int32_t> bqueue;  // work-queue head of the persistent launch
+  // FP64 LU
+  DBuf<int32_t> getrf64_info;   // per-panel info of the vendor panel factorizations
   uint64_t factor_gen = 1;   // incremented by every factorization written through getrf
   uint32_t tag = 1;      // next free exchange tag (never 0)
   // bracket a launch with profiling events (no-op when profiling is off)
@@ -118,6 +120,14 @@ int batch_max_rhs();
 // System i: A[i] (n x n, lda n), B[i] and X[i] (n x nrhs, ld n), device pointers in host arrays.
 void batch_solve(int count, const int32_t* n, const int32_t* nrhs, const double* const* A, const double* const* B,
                  double* const* X, const BatchParams& p, Workspace& W, BatchResult* out, double* device_s);
+
+// FP64 blocked LU (P A = L U in place, 1-based global ipiv on the device) with the trailing update on FP64 MFMA;
+// rb_main / rb_panel: rocBLAS handles of the context (the second one serves the look-ahead panel stream).
+void getrf64(int64_t n, double* A, int64_t lda, int32_t* ipiv, int nb, bool lookahead, void* rb_main, void* rb_panel,
+             Workspace& W, int* info);
+// C (m2 x n2) -= L (m2 x kb) * U (kb x n2), FP64, column-major, on FP64 MFMA; kb % 4 == 0.
+void dgemm_update(int64_t m2, int64_t n2, int kb, const double* L, int64_t ldl, const double* U, int64_t ldu,
+                  double* C, int64_t ldc, hipStream_t stream);
 
 struct GenSpec {
   int64_t n = 1024; int nrhs = 1; std::string family = "randsvd", spectrum = "geometric", rhs = "xtrue_uniform";

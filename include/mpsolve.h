@@ -86,6 +86,12 @@ typedef struct {
  * Replaces custom_blocked_lu (src/mp_core.cuh:129-150). */
 int mp_getrf(mp_ctx* ctx, int64_t n, float* LU, int64_t lda, int64_t* ipiv, int32_t* perm,
              const mp_getrf_params* params, mp_getrf_info* info);
+/* Plain FP64 LU (P A = L U in place, A n x n FP64 device, column-major lda): the comparator the mixed-precision
+ * solve is measured against (native_lu64, src/mpsolve.cu:22 and src/mp_core.cuh:162-170, cusolverDnXgetrf).
+ * Blocked by nb (a multiple of 4): the vendor panel (rocSOLVER dgetrf on m x nb), rocBLAS dtrsm for U12, and
+ * the trailing update on this library's FP64 MFMA kernel, with look-ahead.  ipiv: n 1-based global LAPACK pivots
+ * (device); *info: first zero pivot (1-based) or 0. */
+int mp_getrf64(mp_ctx* ctx, int64_t n, double* A, int64_t lda, int32_t* ipiv, int32_t nb, int32_t* info);
 /* D = scale * U^{-1} L^{-1} P R with FP32 factors and FP32 arithmetic, FP64 R and D (device).
  * Replaces apply_inverse for FP32 factors (src/mp_core.cuh:182-190). */
 int mp_getrs(mp_ctx* ctx, int64_t n, int32_t nrhs, const float* LU, int64_t lda, const int32_t* perm,

@@ -93,6 +93,99 @@ __global__ __launch_bounds__(256) void k_dgemm(double* __restrict__ C, int64_t l
     }
 }
 
+// The same update with the operands staged through LDS: K is processed in chunks of 32, double-buffered; the
+// workgroup loads chunk s+1 from global memory into registers (coalesced: L by rows, U by k) while its waves
+// multiply chunk s from LDS, then stores it to the other buffer.  LDS: L chunk [32][128] and U chunk [32][129]
+// (padded against bank conflicts of the k-major U stores), two buffers, 132 KB.  kb % 32 == 0.
+constexpr int DG_KC = 32;
+constexpr int DG_US = 129;
+static size_t dgemm_lds_bytes() { return (size_t)2 * DG_KC * (128 + DG_US) * 8; }
+__global__ __launch_bounds__(256) void k_dgemm_lds(double* __restrict__ C, int64_t ldc, int m2, int n2, int kb,
+                                                   const double* __restrict__ L, int64_t ldl,
+                                                   const double* __restrict__ U, int64_t ldu) {
+  extern __shared__ double sm[];
+  double* Ls = sm;                           // [2][KC][128]
+  double* Us = sm + 2 * DG_KC * 128;         // [2][KC][129]
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int row0 = blockIdx.x * 128, col0 = blockIdx.y * 128;
+  const int wr = (w & 1) * 64, wc = (w >> 1) * 64;       // wave tile inside the workgroup tile
+  const int li = lane & 15, lk = lane >> 4;
+  // C^T accumulators (same lane layout as k_dgemm)
+  f64x4_t acc[4][4];
+#pragma unroll
+  for (int ci = 0; ci < 4; ci++)
+#pragma unroll
+    for (int ri = 0; ri < 4; ri++) {
+      const int row = min(row0 + wr + ri * 16 + li, m2 - 1);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int col = min(col0 + wc + ci * 16 + 4 * r + lk, n2 - 1);
+        acc[ci][ri][r] = C[row + (int64_t)col * ldc];
+      }
+    }
+  // global -> register staging: L element e = t + 256 m: k = e / 128, row = e % 128 (rows contiguous);
+  // U element e: k = e % 32, col = e / 32 (k contiguous within a column)
+  double gl[16], gu[16];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const int e = t + 256 * m;
+      const int kl = e >> 7, rl = e & 127;
+      gl[m] = L[min(row0 + rl, m2 - 1) + (int64_t)(k0 + kl) * ldl];
+      const int ku = e & 31, cu = e >> 5;
+      gu[m] = -U[(k0 + ku) + (int64_t)min(col0 + cu, n2 - 1) * ldu];
+    }
+  };
+  auto lstore = [&](int buf) {
+    double* lb = Ls + buf * DG_KC * 128;
+    double* ub = Us + buf * DG_KC * DG_US;
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const int e = t + 256 * m;
+      lb[(e >> 7) * 128 + (e & 127)] = gl[m];
+      ub[(e & 31) * DG_US + (e >> 5)] = gu[m];
+    }
+  };
+  const int nch = kb / DG_KC;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ch++) {
+    const int buf = ch & 1;
+    if (ch + 1 < nch) gload((ch + 1) * DG_KC);
+    const double* lb = Ls + buf * DG_KC * 128;
+    const double* ub = Us + buf * DG_KC * DG_US;
+#pragma unroll
+    for (int ks = 0; ks < DG_KC / 4; ks++) {
+      const int kk = ks * 4 + lk;
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        a[i] = ub[kk * DG_US + wc + i * 16 + li];
+        b[i] = lb[kk * 128 + wr + i * 16 + li];
+      }
+#pragma unroll
+      for (int ci = 0; ci < 4; ci++)
+#pragma unroll
+        for (int ri = 0; ri < 4; ri++)
+          acc[ci][ri] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ci], b[ri], acc[ci][ri], 0, 0, 0);
+    }
+    if (ch + 1 < nch) lstore(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int ci = 0; ci < 4; ci++)
+#pragma unroll
+    for (int ri = 0; ri < 4; ri++) {
+      const int row = row0 + wr + ri * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int col = col0 + wc + ci * 16 + 4 * r + lk;
+        if (row < m2 && col < n2) C[row + (int64_t)col * ldc] = acc[ci][ri][r];
+      }
+    }
+}
+
 // panel-local 1-based pivots -> global 1-based pivots (rows of the whole matrix)
 __global__ void k_ipiv_global(int32_t* ipiv, int kb, int off) {
   const int i = threadIdx.x + blockIdx.x * 256;
@@ -109,7 +202,17 @@ void dgemm_update(int64_t m2, int64_t n2, int kb, const double* L, int64_t ldl, 
   if (m2 <= 0 || n2 <= 0 || kb <= 0) return;
   MP_REQUIRE(kb % 4 == 0, "dgemm_update: kb must be a multiple of 4");
   MP_REQUIRE(m2 < (1ll << 31) && n2 < (1ll << 31), "dgemm_update: sizes too large");
-  hipLaunchKernelGGL(k_dgemm, dim3(cdiv(m2, 128), cdiv(n2, 128)), dim3(256), 0, stream, C, ldc, (int)m2, (int)n2, kb, L, ldl, U, ldu);
+  if (kb % DG_KC == 0) {
+    static bool lds_set = false;
+    if (!lds_set) {
+      HIP_CK(hipFuncSetAttribute((const void*)k_dgemm_lds, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dgemm_lds_bytes()));
+      lds_set = true;
+    }
+    hipLaunchKernelGGL(k_dgemm_lds, dim3(cdiv(m2, 128), cdiv(n2, 128)), dim3(256), dgemm_lds_bytes(), stream, C, ldc,
+                       (int)m2, (int)n2, kb, L, ldl, U, ldu);
+  } else {
+    hipLaunchKernelGGL(k_dgemm, dim3(cdiv(m2, 128), cdiv(n2, 128)), dim3(256), 0, stream, C, ldc, (int)m2, (int)n2, kb, L, ldl, U, ldu);
+  }
   LAUNCH_CK();
 }
 
@@ -129,21 +232,29 @@ void getrf64(int64_t n, double* A, int64_t lda, int32_t* ipiv, int nb, bool look
     const int kb = (int)std::min<int64_t>(nb, n - k);
     rocblas_handle h = (s == s1) ? rbp : rb;
     RB_CK2(rocblas_set_stream(h, s));
+    hipEvent_t e = W.prof_begin(s);
     RB_CK2(rocsolver_dgetrf(h, (rocblas_int)(n - k), kb, A_(k, k), (rocblas_int)lda, ipiv + k, W.getrf64_info.p + k / nb));
     hipLaunchKernelGGL(k_ipiv_global, dim3(cdiv(kb, 256)), dim3(256), 0, s, ipiv + k, kb, (int)k);
     LAUNCH_CK();
+    W.prof_end(e, "lu64_panel", 16.0 * (n - k) * kb, (double)(n - k) * kb * kb, s);
   };
   // interchanges of block k on columns [c0, c1), then U12 and the update of those columns
   auto apply = [&](int64_t k, int64_t c0, int64_t c1, hipStream_t s) {
     const int kb = (int)std::min<int64_t>(nb, n - k);
     if (c1 <= c0) return;
     RB_CK2(rocblas_set_stream(rb, s));
+    hipEvent_t e0 = W.prof_begin(s);
     RB_CK2(rocsolver_dlaswp(rb, (rocblas_int)(c1 - c0), A_(0, c0), (rocblas_int)lda, (rocblas_int)(k + 1), (rocblas_int)(k + kb), ipiv, 1));
+    W.prof_end(e0, "lu64_laswp", 16.0 * kb * (c1 - c0), 0, s);
     const double one = 1.0;
     RB_CK2(rocblas_set_pointer_mode(rb, rocblas_pointer_mode_host));
+    hipEvent_t e1 = W.prof_begin(s);
     RB_CK2(rocblas_dtrsm(rb, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit, kb,
                          (rocblas_int)(c1 - c0), &one, A_(k, k), (rocblas_int)lda, A_(k, c0), (rocblas_int)lda));
+    W.prof_end(e1, "lu64_trsm", 16.0 * kb * (c1 - c0), (double)kb * kb * (c1 - c0), s);
+    hipEvent_t e2 = W.prof_begin(s);
     dgemm_update(n - k - kb, c1 - c0, kb, A_(k + kb, k), lda, A_(k, c0), lda, A_(k + kb, c0), lda, s);
+    W.prof_end(e2, "lu64_gemm", 16.0 * (n - k - kb) * (c1 - c0), 2.0 * (n - k - kb) * (c1 - c0) * kb, s);
   };
   auto swap_left = [&](int64_t k, hipStream_t s) {   // block k's interchanges on the factored columns [0, k)
     const int kb = (int)std::min<int64_t>(nb, n - k);

@@ -204,6 +204,8 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096, help="uneven-batch leg: systems per launch (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=240, help="serving leg: requests per stream (0: skip)")
     ap.add_argument("--serve-pool", type=int, default=40)
+    ap.add_argument("--legs", default="headline,batch,serve",
+                    help="legs to run (a profiling run may select one; the headline fields are null without it)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this solver build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -211,8 +213,9 @@ def main():
 
     from cpsc402_midterm_project_amd import Context, Problem
 
+    legs = set(args.legs.split(","))
     n = args.n
-    kappas = [float(k) for k in args.kappas.split(",")]
+    kappas = [float(k) for k in args.kappas.split(",")] if "headline" in legs else []
     common = dict(nb=args.nb, gmres_m=30, gmres_tol=1e-4, tau=1e-15, stop_on="eta", max_outer=50)
     ctx = Context(0)
     caps = ctx.caps()
@@ -269,10 +272,10 @@ def main():
                 log(f"[cpu] dgesv {tc*1e3:.0f} ms on {cores} threads")
         P.close()
     batch = None
-    if args.batch_count > 0:
+    if args.batch_count > 0 and "batch" in legs:
         batch = bench_batch(ctx, args.batch_count, args.steps, args.warmup, not args.no_cpu)
     serve = None
-    if args.serve_requests > 0:
+    if args.serve_requests > 0 and "serve" in legs:
         serve = bench_serve(ctx, args.serve_requests, args.serve_pool)
     ctx.close()
 
@@ -288,6 +291,10 @@ def main():
                     all_kernels={k: dict(ms=round(x["ms"], 3), count=x["count"],
                                          GBps=round(x["bytes"] / max(x["ms"], 1e-9) / 1e6, 1),
                                          TFps=round(x["flops"] / max(x["ms"], 1e-9) / 1e9, 2)) for k, x in prof.items()})
+    if not cases:
+        print(json.dumps({"metric": "n=16384 time-to-FP64-accuracy: low-prec LU-IR vs FP64 LU; serve p50/p99 ms", "value": None,
+                          "unit": "ms", "legs": sorted(legs), "batch": batch, "serve": serve}), flush=True)
+        return
     hb = headline or cases[-1]
     cands = hb["low_precision"] + ([hb["fallback"]] if hb.get("fallback") else [])
     best = next((r for r in cands if r["method"] == hb.get("best")), None) or cands[0]

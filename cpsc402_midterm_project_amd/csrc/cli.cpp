@@ -8,7 +8,6 @@
 //   * --family prescribed (the reference's default) is generated as "randsvd": the same prescribed singular
 //     values with Haar-random orthogonal factors (QR of Gaussian matrices) instead of rotation stages, so the
 //     matrices differ from the reference's for the same seed; the case record says which generator ran;
-//   * --dtype fp32 storage and --rowcol-span scaling are not generated (fatal error, as for an unknown family);
 //   * --modes host times the upload of A and B plus the solve plus the download of X, with the solve itself
 //     reading the resident copy (the same bytes cross the bus as in the reference's host mode).
 #include "mpsolve.h"
@@ -129,13 +128,11 @@ struct Session {
 // The case object passed to mp_problem_generate: the reference's flag names with dashes as underscores.
 std::string case_spec(const Args& a, std::string& family_generated) {
   const std::string family = a.get("family", "prescribed");
-  if (a.get("dtype", "fp64") != "fp64") throw std::runtime_error("only --dtype fp64 stored systems are generated by this build");
-  if (a.geti("rowcol-span", 0) != 0) throw std::runtime_error("--rowcol-span scaling is not generated by this build");
   family_generated = (family == "prescribed") ? "randsvd" : family;
   Obj c;
   c.s("case_id", a.get("case-id", "case")).i("n", a.geti("n", 1024)).i("nrhs", a.geti("nrhs", 1))
    .s("family", family_generated).s("family_requested", family).s("spectrum", a.get("spectrum", "geometric"))
-   .s("dtype", "fp64").s("rhs", a.get("rhs", "random")).d("kappa", a.getd("kappa", 100)).d("kappa_nominal", a.getd("kappa", 100))
+   .s("dtype", a.get("dtype", "fp64")).s("rhs", a.get("rhs", "random")).d("kappa", a.getd("kappa", 100)).d("kappa_nominal", a.getd("kappa", 100))
    .i("seed", a.geti("seed", 9200)).d("ell", a.getd("ell", 0.25)).d("nugget", a.getd("nugget", 1e-3)).i("dim", a.geti("dim", 3))
    .i("intmax", a.geti("intmax", 255)).i("dyadic_exp", a.geti("dyadic-exp", 8)).i("grcar_k", a.geti("grcar-k", 3))
    .i("global_exp", a.geti("global-exp", 0)).i("rowcol_span", a.geti("rowcol-span", 0));
@@ -160,7 +157,8 @@ int cmd_regen(Session& S) {
 
 // Orthogonality / density check of the generator (src/mpsolve.cu:148): run with an identity spectrum, A is
 // orthogonal and max |A^T A - I| measures the generator's orthogonality.
-int cmd_selftest(Session& S, long long n) {
+int cmd_selftest(Session& S, long long n, bool f32) {
+  if (f32) throw std::runtime_error("selftest checks the FP64 generator: run it with --dtype fp64");
   std::vector<double> h((size_t)n * n);
   check(mp_problem_copy_out(S.C, S.P, h.data(), nullptr, nullptr), "mp_problem_copy_out");
   double maxdev = 0, minabs = INFINITY, anorm = 0; size_t zeros = 0;
@@ -214,18 +212,20 @@ int cmd_run(Session& S, const Args& a, const std::string& case_id, long long n, 
   }
   // B, X_exact, X_ref, optionally A
   const size_t nb = (size_t)n * nrhs;
+  const bool f32 = a.get("dtype", "fp64") == "fp32";       // files of A and B in the stored dtype
   {
     std::vector<double> hb(nb), hx(nb);
     std::string cj = problem_json(S.P);
     const bool has_exact = cj.find("\"has_exact_solution\":true") != std::string::npos;
     check(mp_problem_copy_out(S.C, S.P, nullptr, hb.data(), has_exact ? hx.data() : nullptr), "mp_problem_copy_out");
-    write_bin(out + "/B_" + case_id + ".f64", hb.data(), nb * 8);
+    if (f32) { std::vector<float> b32(hb.begin(), hb.end()); write_bin(out + "/B_" + case_id + ".f32", b32.data(), nb * 4); }
+    else write_bin(out + "/B_" + case_id + ".f64", hb.data(), nb * 8);
     if (has_exact) write_bin(out + "/Xexact_" + case_id + ".f64", hx.data(), nb * 8);
     if (mp_problem_copy_ref(S.P, hx.data()) == MP_OK) write_bin(out + "/Xref_" + case_id + ".f64", hx.data(), nb * 8);
     if (a.has("embed-a")) {
-      std::vector<double> ha((size_t)n * n);
+      std::vector<double> ha((size_t)n * n);       // large enough for either dtype
       check(mp_problem_copy_out(S.C, S.P, ha.data(), nullptr, nullptr), "mp_problem_copy_out");
-      write_bin(out + "/A_" + case_id + ".f64", ha.data(), ha.size() * 8);
+      write_bin(out + "/A_" + case_id + (f32 ? ".f32" : ".f64"), ha.data(), ha.size() * (f32 ? 4 : 8));
     }
   }
   std::ofstream trials(out + "/trials_" + case_id + ".jsonl", std::ios::app), hist(out + "/hist_" + case_id + ".jsonl", std::ios::app);
@@ -258,9 +258,10 @@ int cmd_run(Session& S, const Args& a, const std::string& case_id, long long n, 
         void* dA = nullptr; void* dB = nullptr;
         if (host) {
           // the host mode's bus traffic: A and B up, X down (src/mpsolve.cu:197, 212-213)
-          check(mp_malloc(S.C, &dA, (size_t)n * n * 8), "mp_malloc"); check(mp_malloc(S.C, &dB, nb * 8), "mp_malloc");
+          const size_t abytes = (size_t)n * n * (f32 ? 4 : 8), bbytes = nb * (f32 ? 4 : 8);   // stored dtype
+          check(mp_malloc(S.C, &dA, abytes), "mp_malloc"); check(mp_malloc(S.C, &dB, nb * 8), "mp_malloc");
           const double t0 = wall_now();
-          check(mp_memcpy_h2d(S.C, dA, hA.data(), (size_t)n * n * 8), "upload A"); check(mp_memcpy_h2d(S.C, dB, hB.data(), nb * 8), "upload B");
+          check(mp_memcpy_h2d(S.C, dA, hA.data(), abytes), "upload A"); check(mp_memcpy_h2d(S.C, dB, hB.data(), bbytes), "upload B");
           t_up = wall_now() - t0;
         }
         // the record's x_path names the file written below (after the solve reports the working dtype)
@@ -308,7 +309,7 @@ int main(int argc, char** argv) {
     check(mp_problem_generate(S.C, spec.c_str(), &S.P), "mp_problem_generate");
     const long long n = a.geti("n", 1024), nrhs = a.geti("nrhs", 1);
     if (a.cmd == "regen") return cmd_regen(S);
-    if (a.cmd == "selftest") return cmd_selftest(S, n);
+    if (a.cmd == "selftest") return cmd_selftest(S, n, a.get("dtype", "fp64") == "fp32");
     return cmd_run(S, a, a.get("case-id", "case"), n, nrhs);   // any other command runs trials, as the reference does
   } catch (std::exception& e) {
     std::fprintf(stderr, "FATAL: %s\n", e.what());

@@ -254,6 +254,16 @@ static void perm_from_ipiv32(const std::vector<int>& ip, std::vector<int32_t>& p
   for (int64_t i = 0; i < n; i++) { const int64_t p = ip[i] - 1; if (p != i) std::swap(perm[i], perm[p]); }
 }
 
+__global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ b, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) b[i] = (double)a[i];
+}
+
+__global__ void k_scale_f32(float* __restrict__ x, int64_t n, float s) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) x[i] *= s;
+}
+
 static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, int nb, int scale_mode, Factors& F) {
   Workspace& W = C->W;
   const int64_t n = P.n; const size_t nn = (size_t)n * n;
@@ -272,14 +282,22 @@ static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, i
     F.LU32.ensure(C->tracker, nn, "LU32");
     if (P.fp32) {
       HIP_CK(hipMemcpyAsync(F.LU32.p, P.A32.p, nn * 4, hipMemcpyDeviceToDevice, W.stream));
-      MP_REQUIRE(F.gscale == 1.0, "global scaling of FP32-stored problems is not implemented");
+      // the reference scales the FP32 copy in place (src/mp_core.cuh:167): a power of two, exact in range
+      if (F.gscale != 1.0) {
+        hipLaunchKernelGGL(k_scale_f32, dim3(cdiv(nn, 256)), dim3(256), 0, W.stream, F.LU32.p, (int64_t)nn, (float)F.gscale);
+        LAUNCH_CK();
+      }
     } else {
       scale_convert_f64_to_f32(P.A64.p, n, n, F.LU32.p, n, F.gscale, W);
     }
   } else {
     F.LU64.ensure(C->tracker, nn, "LU64");
-    MP_REQUIRE(!P.fp32, "FP64 factors of FP32-stored problems are not implemented");
-    HIP_CK(hipMemcpyAsync(F.LU64.p, P.A64.p, nn * 8, hipMemcpyDeviceToDevice, W.stream));
+    if (P.fp32) {   // FP64 factors of an FP32-stored system: the exact FP64 values of its entries (src/mp_core.cuh:163)
+      hipLaunchKernelGGL(k_f32_to_f64, dim3(cdiv(nn, 256)), dim3(256), 0, W.stream, P.A32.p, F.LU64.p, (int64_t)nn);
+      LAUNCH_CK();
+    } else {
+      HIP_CK(hipMemcpyAsync(F.LU64.p, P.A64.p, nn * 8, hipMemcpyDeviceToDevice, W.stream));
+    }
     // the reference scales the FP64 copy too (src/mp_core.cuh:167) and applies the factors with g
     if (F.gscale != 1.0) axpy_scale_inplace(F.LU64.p, (int64_t)nn, F.gscale, W);
   }
@@ -974,7 +992,7 @@ int mp_generate_into(mp_ctx* C, const char* case_json, double* A, double* B, dou
     g.spectrum = a.get("spectrum", "geometric"); g.rhs = a.get("rhs", g.family == "dyadic" ? "exact_sign" : "xtrue_uniform");
     g.kappa = a.getd("kappa", 100); g.seed = (uint64_t)a.getd("seed", 9200); g.ell = a.getd("ell", 0.25); g.nugget = a.getd("nugget", 1e-3);
     g.dim = a.geti("dim", 3); g.intmax = a.geti("intmax", 255); g.dyadic_exp = a.geti("dyadic_exp", 8); g.grcar_k = a.geti("grcar_k", 3);
-    g.global_exp = a.geti("global_exp", 0);
+    g.global_exp = a.geti("global_exp", 0); g.rowcol_span = a.geti("rowcol_span", 0);
     MP_REQUIRE(g.n > 0 && g.nrhs > 0, "n and nrhs must be positive");
     DBuf<double> Xe;
     double* xe = x_exact;
@@ -1039,9 +1057,11 @@ int mp_problem_generate(mp_ctx* C, const char* case_json, mp_problem** out) {
     g.spectrum = a.get("spectrum", "geometric"); g.rhs = a.get("rhs", g.family == "dyadic" ? "exact_sign" : "xtrue_uniform");
     g.kappa = a.getd("kappa", 100); g.seed = (uint64_t)a.getd("seed", 9200); g.ell = a.getd("ell", 0.25); g.nugget = a.getd("nugget", 1e-3);
     g.dim = a.geti("dim", 3); g.intmax = a.geti("intmax", 255); g.dyadic_exp = a.geti("dyadic_exp", 8); g.grcar_k = a.geti("grcar_k", 3);
-    g.global_exp = a.geti("global_exp", 0);
+    g.global_exp = a.geti("global_exp", 0); g.rowcol_span = a.geti("rowcol_span", 0);
     MP_REQUIRE(g.n > 0 && g.nrhs > 0, "n and nrhs must be positive");
-    MP_REQUIRE(a.get("dtype", "fp64") == "fp64", "only fp64 stored systems are generated on the device");
+    MP_REQUIRE(g.rowcol_span >= 0 && g.rowcol_span <= 200, "rowcol_span must be in [0, 200]");
+    const std::string dtype = a.get("dtype", "fp64");
+    MP_REQUIRE(dtype == "fp64" || dtype == "fp32", "dtype must be fp64 or fp32");
     std::unique_ptr<mp_problem> P(new mp_problem());
     P->ctx = C; P->n = g.n; P->nrhs = g.nrhs; P->fp32 = false;
     const size_t nn = (size_t)g.n * g.n, nb = (size_t)g.n * g.nrhs;
@@ -1055,6 +1075,22 @@ int mp_problem_generate(mp_ctx* C, const char* case_json, mp_problem** out) {
     HIP_CK(hipMemcpy(ha.data(), P->A64.p, nn * 8, hipMemcpyDeviceToHost));
     HIP_CK(hipMemcpy(hb.data(), P->B.p, nb * 8, hipMemcpyDeviceToHost));
     if (exact) { P->hXexact.resize(nb); HIP_CK(hipMemcpy(P->hXexact.data(), Xe.p, nb * 8, hipMemcpyDeviceToHost)); P->has_exact = true; }
+    if (dtype == "fp32") {
+      // FP32 storage (src/mp_core.cuh:87): A and B rounded to FP32, B kept as the FP64 value of its FP32 copy.
+      // x_exact stays exact only if the rounding changed nothing.
+      std::vector<float> ha32(nn);
+      bool same = true;
+      const double* a64 = (const double*)ha.data();
+      for (size_t i = 0; i < nn; i++) { ha32[i] = (float)a64[i]; same = same && ((double)ha32[i] == a64[i]); }
+      for (size_t i = 0; i < nb; i++) { const double v = (double)(float)hb[i]; same = same && (v == hb[i]); hb[i] = v; }
+      P->A64.release();
+      P->fp32 = true;
+      P->A32.alloc(C->tracker, nn, "A32");
+      HIP_CK(hipMemcpy(P->A32.p, ha32.data(), nn * 4, hipMemcpyHostToDevice));
+      HIP_CK(hipMemcpy(P->B.p, hb.data(), nb * 8, hipMemcpyHostToDevice));
+      if (!same) { P->has_exact = false; P->hXexact.clear(); }
+      ha.assign((const char*)ha32.data(), (const char*)ha32.data() + nn * 4);
+    }
     const double th = wall_now();
     finish_setup(C, P.get(), ha.data(), hb);
     Json j; j.d("gen_seconds", tg).d("hash_seconds", wall_now() - th).s("generator", "device: counter-based Gaussian/uniform streams, rocSOLVER geqrf/orgqr, rocBLAS dgemm");
@@ -1072,7 +1108,6 @@ int mp_problem_reference(mp_ctx* C, mp_problem* P) {
   return guard([&] {
     MP_REQUIRE(C && P, "NULL argument");
     MP_REQUIRE(P->ctx == C, "problem belongs to another context, or its context was destroyed");
-    MP_REQUIRE(!P->fp32, "reference solutions of FP32-stored systems are not implemented");
     const double t0 = wall_now();
     const int64_t n = P->n; const int nrhs = P->nrhs; const size_t nb = (size_t)n * nrhs;
     Factors F;
@@ -1088,8 +1123,8 @@ int mp_problem_reference(mp_ctx* C, mp_problem* P) {
       std::vector<double> st(4 * nrhs);
       DBuf<double> stats; stats.alloc(C->tracker, 4 * nrhs + 4, "ref_stats");
       for (it = 0; it < 12; it++) {
-        ResidArgs ra; ra.n = n; ra.nrhs = nrhs; ra.A = P->A64.p; ra.lda = n; ra.X = X.p; ra.ldx = n; ra.B = P->B.p; ra.ldb = n;
-        ra.R = Rh.p; ra.Rlo = Rl.p; ra.ldr = n; ra.dd = true; ra.stats = stats.p;
+        ResidArgs ra; ra.n = n; ra.nrhs = nrhs; ra.a32 = P->fp32; ra.A = P->a_ptr(); ra.lda = n; ra.X = X.p; ra.ldx = n;
+        ra.B = P->B.p; ra.ldb = n; ra.R = Rh.p; ra.Rlo = Rl.p; ra.ldr = n; ra.dd = true; ra.stats = stats.p;
         residual(ra, C->W);
         HIP_CK(hipMemcpyAsync(st.data(), stats.p, 32 * nrhs, hipMemcpyDeviceToHost, C->W.stream));
         HIP_CK(hipStreamSynchronize(C->W.stream));

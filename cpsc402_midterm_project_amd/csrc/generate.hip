@@ -13,6 +13,8 @@
 // (q, s, k) -> 53-bit uniform; Gaussians by Box-Muller.  Streams do not depend on the launch shape.
 #include "mp_internal.hpp"
 #include "mp_kernels.hpp"
+
+#include <random>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -100,6 +102,12 @@ double spectrum_value(const std::string& sp, int64_t i, int64_t n, double kappa)
   throw MPError("unknown spectrum " + sp, -1);
 }
 
+// A[i + j n] *= r[i] * c[j] (powers of two: exact)
+__global__ void k_rowcol_scale(double* __restrict__ A, int64_t n, const double* __restrict__ r, const double* __restrict__ c) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+  if (i < n) A[i + j * n] *= r[i] * c[j];
+}
+
 // Generates A (FP64, n x n), B (n x nrhs) and optionally X_exact into device buffers.  Returns has_exact.
 bool generate(const GenSpec& g, void* rb_handle, Workspace& W, DevTracker& tr, double* A, double* B, double* Xe) {
   rocblas_handle rb = (rocblas_handle)rb_handle;
@@ -138,6 +146,23 @@ bool generate(const GenSpec& g, void* rb_handle, Workspace& W, DevTracker& tr, d
     throw MPError("unknown or unsupported family " + g.family + " (supported: randsvd, dyadic, rbf, grcar, wilkinson)", -4);
   }
   LAUNCH_CK();
+  if (g.rowcol_span > 0) {
+    // the reference's --rowcol-span (src/mp_core.cuh:71-72): A <- D_r A D_c with row and column factors
+    // 2^(u - span/2), u uniform in [0, span]; applied before the right-hand side is formed.  The stream is
+    // this library's own (seeded from the case seed).
+    if (g.rhs == "exact_sign") throw MPError("exact_sign with scaling not supported", -1);
+    std::mt19937_64 rng(g.seed * 31ull + 5ull);
+    std::uniform_int_distribution<int> u(0, g.rowcol_span);
+    std::vector<double> r(n), c(n);
+    for (int64_t i = 0; i < n; i++) r[i] = std::ldexp(1.0, u(rng) - g.rowcol_span / 2);
+    for (int64_t i = 0; i < n; i++) c[i] = std::ldexp(1.0, u(rng) - g.rowcol_span / 2);
+    DBuf<double> dr, dc; dr.alloc(tr, n, "gen_dr"); dc.alloc(tr, n, "gen_dc");
+    HIP_CK(hipMemcpyAsync(dr.p, r.data(), n * 8, hipMemcpyHostToDevice, W.stream));
+    HIP_CK(hipMemcpyAsync(dc.p, c.data(), n * 8, hipMemcpyHostToDevice, W.stream));
+    hipLaunchKernelGGL(k_rowcol_scale, dim3(cdiv(n, 256), (unsigned)n), dim3(256), 0, W.stream, A, n, dr.p, dc.p);
+    LAUNCH_CK();
+    HIP_CK(hipStreamSynchronize(W.stream));
+  }
   const size_t nb = (size_t)n * nrhs;
   if (g.rhs == "random" || g.rhs == "random_scaled") {
     hipLaunchKernelGGL(k_gauss_fill, dim3(cdiv(nb, 256)), dim3(256), 0, W.stream, B, (int64_t)nb, g.seed, (uint64_t)3 + (uint64_t)nrhs);

@@ -132,7 +132,7 @@ void dgemm_update(int64_t m2, int64_t n2, int kb, const double* L, int64_t ldl, 
 struct GenSpec {
   int64_t n = 1024; int nrhs = 1; std::string family = "randsvd", spectrum = "geometric", rhs = "xtrue_uniform";
   double kappa = 100; uint64_t seed = 9200; double ell = 0.25, nugget = 1e-3; int dim = 3, intmax = 255, dyadic_exp = 8, grcar_k = 3;
-  int global_exp = 0;
+  int global_exp = 0, rowcol_span = 0;
 };
 // device generation of A (n x n), B and X_exact (n x nrhs); returns whether X_exact holds the exact solution
 bool generate(const GenSpec& g, void* rocblas_handle, Workspace& W, DevTracker& tr, double* A, double* B, double* Xe);

@@ -106,5 +106,26 @@ def test_cli_selftest_and_unsupported_flags(tmp_path):
     rec = json.loads(r.stdout)
     assert rec["selftest"] == "generator" and rec["n"] == 192
     assert rec["max_orthogonality_deviation"] < 1e-12
-    r = _run("run", "--n", 64, "--rowcol-span", 4, "--out", tmp_path)
-    assert r.returncode == 1 and "rowcol-span" in json.loads(r.stdout)["fatal"]
+    r = _run("selftest", "--n", 64, "--dtype", "fp32")
+    assert r.returncode == 1 and "fatal" in json.loads(r.stdout)
+
+
+@pytest.mark.gpu
+def test_cli_fp32_storage_and_rowcol_scaling(tmp_path):
+    """--dtype fp32 stores A and B in FP32 (files .f32); --rowcol-span scales rows and columns by powers of two;
+    both systems are solved to the FP32 target by the default method."""
+    n = 300
+    print(f"[cli fp32 rowcol] n={n}", flush=True)
+    r = _run("run", "--n", n, "--kappa", 1e3, "--dtype", "fp32", "--rowcol-span", 6, "--methods", "mx16_ir_w64_r64",
+             "--reps", 1, "--warmup", 0, "--case-id", "f", "--out", tmp_path, "--embed-a")
+    assert r.returncode == 0, r.stderr + r.stdout
+    case = json.loads((tmp_path / "case_f.json").read_text())
+    assert case["dtype"] == "fp32" and case["rowcol_span"] == 6 and case["target"] == "fp32"
+    A = np.fromfile(tmp_path / "A_f.f32", dtype=np.float32).reshape(n, n, order="F")
+    b = np.fromfile(tmp_path / "B_f.f32", dtype=np.float32)
+    assert A.size == n * n and b.size == n
+    # row and column scales are powers of two spanning up to 2^6: max/min of row maxima is well above 1
+    rm = np.abs(A).max(axis=1)
+    assert rm.max() / rm.min() > 4
+    rec = json.loads((tmp_path / "trials_f.jsonl").read_text().splitlines()[0])
+    assert rec["ok"] and rec["terminal_reason"] == "converged" and rec["eta_max"] <= 100 * 2.0 ** -24

@@ -256,7 +256,7 @@ __device__ void invert_l11(Lds& S) {
 }
 
 // U12 column c: u_r = a_r + sum_{q<r} Ti[r][q] a_q (FMA in increasing q), written back and returned.
-__device__ __forceinline__ void trsm_col(Lds& S, float* __restrict__ LU, int ldr, int k, int c, float (&ud)[32]) {
+__device__ __forceinline__ void trsm_col(Lds& S, float* __restrict__ LU, int ldr, int k, int c, float (&ud)[32], bool store) {
   float a[32];
 #pragma unroll
   for (int r = 0; r < 32; r++) a[r] = LU[urow_off(S, k + r, ldr) + c];
@@ -266,7 +266,7 @@ __device__ __forceinline__ void trsm_col(Lds& S, float* __restrict__ LU, int ldr
 #pragma unroll
     for (int q = 0; q < 32; q++)
       if (q < r) acc = fmaf(S.Ti[r][q], a[q], acc);
-    LU[urow_off(S, k + r, ldr) + c] = acc;
+    if (store) LU[urow_off(S, k + r, ldr) + c] = acc;
     ud[r] = acc;
     // one row's T^{-1} reads at a time (unbounded hoisting of the 496 LDS reads exhausts the registers)
     asm volatile("" ::: "memory");
@@ -276,15 +276,25 @@ __device__ __forceinline__ void trsm_col(Lds& S, float* __restrict__ LU, int ldr
 __device__ void update(Lds& S, float* __restrict__ LU, int n, int ldr, int k) {
   const int t = threadIdx.x;
   double* Lc = S.buf;                                   // [64 rows][32] FP64
-  // one column per thread per pass (256 columns a pass): its U12 column stays in registers as FP64
+  const int m2 = n - k - 32;
+  // G thread groups of BT/G threads: each group takes every column of the pass and a share of the rows, so
+  // narrow trailing matrices still keep all threads busy (each group forms its columns' U12 itself)
+  const int G = (m2 <= 64) ? 4 : (m2 <= 128 ? 2 : 1);
+  const int TG = BT / G, g = t / TG, tl = t % TG;
 #pragma unroll 1
-  for (int cb = k + 32; cb < n; cb += BT) {
-    const int c = cb + t;
+  for (int cb = k + 32; cb < n; cb += TG) {
+    const int c = cb + tl;
     const bool has = c < n;
     double ud[32];
     {
+      // every group reads the column and forms U12; group 0 writes it back once all groups have read it
       float uf[32];
-      if (has) trsm_col(S, LU, ldr, k, c, uf);
+      if (has) trsm_col(S, LU, ldr, k, c, uf, false);
+      __syncthreads();
+      if (has && g == 0) {
+#pragma unroll
+        for (int r = 0; r < 32; r++) LU[urow_off(S, k + r, ldr) + c] = uf[r];
+      }
 #pragma unroll
       for (int q = 0; q < 32; q++) ud[q] = has ? (double)uf[q] : 0.0;
     }
@@ -308,7 +318,7 @@ __device__ void update(Lds& S, float* __restrict__ LU, int n, int ldr, int k) {
         // its branch and hoist every LDS read above them, which exhausts the registers)
         constexpr int RG = 8;
 #pragma unroll 1
-        for (int r0 = 0; r0 < nr; r0 += RG) {
+        for (int r0 = g * RG; r0 < nr; r0 += G * RG) {
           int ro[RG], rl[RG]; float v[RG]; double acc[RG];
 #pragma unroll
           for (int i = 0; i < RG; i++) {

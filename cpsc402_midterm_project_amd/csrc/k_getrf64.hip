@@ -12,6 +12,8 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <mutex>
+
 namespace mpk {
 
 namespace {
@@ -203,11 +205,12 @@ void dgemm_update(int64_t m2, int64_t n2, int kb, const double* L, int64_t ldl, 
   MP_REQUIRE(kb % 4 == 0, "dgemm_update: kb must be a multiple of 4");
   MP_REQUIRE(m2 < (1ll << 31) && n2 < (1ll << 31), "dgemm_update: sizes too large");
   if (kb % DG_KC == 0) {
-    static bool lds_set = false;
-    if (!lds_set) {
-      HIP_CK(hipFuncSetAttribute((const void*)k_dgemm_lds, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dgemm_lds_bytes()));
-      lds_set = true;
-    }
+    static std::once_flag lds_once;      // contexts on several host threads may get here together
+    hipError_t attr = hipSuccess;
+    std::call_once(lds_once, [&] {
+      attr = hipFuncSetAttribute((const void*)k_dgemm_lds, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dgemm_lds_bytes());
+    });
+    HIP_CK(attr);
     hipLaunchKernelGGL(k_dgemm_lds, dim3(cdiv(m2, 128), cdiv(n2, 128)), dim3(256), dgemm_lds_bytes(), stream, C, ldc,
                        (int)m2, (int)n2, kb, L, ldl, U, ldu);
   } else {

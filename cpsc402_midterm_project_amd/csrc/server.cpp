@@ -65,6 +65,8 @@ struct mp_server {
   std::string solve_params, batch_params;
   size_t budget = (size_t)200 << 30;
   int max_batch = 4096;
+  bool sched_fifo = false;      // large requests in arrival order instead of smallest first
+  double max_wait_s = 0.6;      // a large request waiting longer than this goes next
 
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -121,7 +123,12 @@ struct mp_server {
         for (;;) {
           if (stop && qbig.empty()) return;
           if (!qbig.empty()) {
-            const Req& h = qbig.front();
+            // next large request: the smallest system first (its solve time grows like n^3), unless the oldest
+            // one has waited longer than max_wait_s, which then goes first
+            size_t pick = 0;
+            if (!(sched_fifo || now_s() - qbig.front().arrival > max_wait_s))
+              for (size_t q = 1; q < qbig.size(); q++) if (qbig[q].n < qbig[pick].n) pick = q;
+            const Req& h = qbig[pick];
             auto it = cache.find(h.matrix);
             const size_t full = big_need(h.n, h.nrhs);
             const size_t need = full - (it != cache.end() ? std::min(full, it->second.bytes) : 0);   // beyond what it holds
@@ -134,7 +141,7 @@ struct mp_server {
               if (keep != cache.end()) keep->second.last = tick;
             }
             if (held() + need <= budget || (big_bytes == 0 && small_bytes == 0)) {
-              r = h; qbig.pop_front();
+              r = h; qbig.erase(qbig.begin() + (std::ptrdiff_t)pick);   // the others keep their arrival order
               big_bytes = full;
               auto it2 = cache.find(r.matrix);
               if (it2 != cache.end()) {
@@ -275,6 +282,8 @@ int mp_server_create(int device, const char* params_json, mp_server** out) {
     S->method = jget(js, "method", S->method);
     S->budget = (size_t)std::stod(jget(js, "hbm_budget_bytes", std::to_string((double)S->budget)));
     S->max_batch = std::stoi(jget(js, "max_batch", "4096"));
+    S->sched_fifo = jget(js, "schedule", "shortest_first") == "fifo";
+    S->max_wait_s = std::stod(jget(js, "max_wait_s", "0.6"));
     MP_REQUIRE(S->max_batch >= 1, "max_batch must be positive");
     std::string sp = jget(js, "solve", "{}");
     std::string bp = jget(js, "batch", "{}");

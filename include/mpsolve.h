@@ -193,6 +193,8 @@ int mp_batch_solve_ptrs(mp_ctx* ctx, int32_t count, const int32_t* n, const int3
  * "method" on a second context, so both kinds overlap on the device; a repeated matrix id reuses its cached
  * factors.  Every request is refined to eta <= "tau" (default 1e-15).  Admission keeps the bytes the server
  * holds within "hbm_budget_bytes" (default 200 GiB), releasing the least recently used cached factors first.
+ * Large requests run smallest n first ("schedule": "shortest_first", or "fifo"); one that has waited longer than
+ * "max_wait_s" (default 0.6) goes next.
  * params_json: "method", "tau", "hbm_budget_bytes", "max_batch", "solve" {...} and "batch" {...} (extra
  * parameters of mp_solve / mp_batch_solve).  Latencies use the clock of mp_server_clock (seconds). */
 typedef struct mp_server mp_server;

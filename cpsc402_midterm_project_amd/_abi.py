@@ -27,6 +27,7 @@ EXPORTS = [
     "mp_last_record_json", "mp_last_hist_json", "mp_methods_json", "mp_eval_metrics_json", "mp_batch_solve",
     "mp_batch_solve_ptrs", "mp_generate_into", "mp_problem_from_device", "mp_problem_set_rhs", "mp_server_clock",
     "mp_server_create", "mp_server_submit", "mp_server_poll", "mp_server_stats_json", "mp_server_destroy", "mp_getrf64",
+    "mp_factor_cache_put", "mp_factor_cache_get", "mp_factor_cache_evict",
 ]
 SERVER_TERMINAL = ["converged", "nonfinite", "max_outer", "stagnation", "singular_factor", "error", "other"]
 BATCH_TERMINAL = ["converged", "nonfinite", "max_outer", "stagnation", "singular_factor"]
@@ -110,6 +111,9 @@ def _load() -> C.CDLL:
         "mp_server_stats_json": ([vp, C.c_char_p, sz], C.c_int),
         "mp_server_destroy": ([vp], C.c_int),
         "mp_getrf64": ([vp, i64, dp, i64, vp, i32, C.POINTER(i32)], C.c_int),
+        "mp_factor_cache_put": ([vp, C.c_uint64, vp], C.c_int),
+        "mp_factor_cache_get": ([vp, C.c_uint64, C.POINTER(vp), C.POINTER(i32)], C.c_int),
+        "mp_factor_cache_evict": ([vp, C.c_uint64], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -235,6 +239,18 @@ class Context:
         info = GetrfInfo()
         check(lib.mp_getrf(self.h, n, _ptr(LU), lda, _ptr(ipiv), _ptr(perm), C.byref(p), C.byref(info)))
         return info
+
+    def factor_cache_put(self, matrix_id: int, problem: "Problem") -> None:
+        check(lib.mp_factor_cache_put(self.h, matrix_id, problem.h))
+
+    def factor_cache_has(self, matrix_id: int) -> bool:
+        """True when the id is cached and its problem holds factors; MPSolveError when the id is unknown."""
+        p = C.c_void_p(); hf = C.c_int32()
+        check(lib.mp_factor_cache_get(self.h, matrix_id, C.byref(p), C.byref(hf)))
+        return bool(hf.value)
+
+    def factor_cache_evict(self, matrix_id: int) -> None:
+        check(lib.mp_factor_cache_evict(self.h, matrix_id))
 
     def getrf64(self, n, A, lda, ipiv, nb=128) -> int:
         info = C.c_int32()

@@ -91,6 +91,7 @@ struct mp_ctx {
   std::string last_record, last_hist;
   std::unique_ptr<Krylov> kry;
   std::set<mp_problem*> problems;       // live problems: their device memory is released by mp_destroy
+  std::map<uint64_t, mp_problem*> factor_ids;   // factor cache: matrix id -> problem holding its factors
 };
 
 struct Factors;
@@ -1024,6 +1025,34 @@ int mp_problem_from_device(mp_ctx* C, int64_t n, int32_t nrhs, const double* A, 
   });
 }
 
+int mp_factor_cache_put(mp_ctx* C, uint64_t matrix_id, mp_problem* P) {
+  return guard([&] {
+    MP_REQUIRE(C && P, "NULL argument");
+    MP_REQUIRE(P->ctx == C, "problem belongs to another context, or its context was destroyed");
+    C->factor_ids[matrix_id] = P;
+  });
+}
+
+int mp_factor_cache_get(mp_ctx* C, uint64_t matrix_id, mp_problem** out, int32_t* has_factors) {
+  return guard([&] {
+    MP_REQUIRE(C && out, "NULL argument");
+    auto it = C->factor_ids.find(matrix_id);
+    MP_REQUIRE(it != C->factor_ids.end(), "matrix id not in the factor cache");
+    *out = it->second;
+    if (has_factors) *has_factors = (it->second->fcache && it->second->fcache->valid) ? 1 : 0;
+  });
+}
+
+int mp_factor_cache_evict(mp_ctx* C, uint64_t matrix_id) {
+  return guard([&] {
+    MP_REQUIRE(C, "ctx is NULL");
+    auto it = C->factor_ids.find(matrix_id);
+    MP_REQUIRE(it != C->factor_ids.end(), "matrix id not in the factor cache");
+    it->second->fcache.reset();        // device memory of the factors is released here
+    C->factor_ids.erase(it);
+  });
+}
+
 int mp_problem_set_rhs(mp_ctx* C, mp_problem* P, int32_t nrhs, const double* B) {
   return guard([&] {
     MP_REQUIRE(C && P && B, "NULL argument");
@@ -1161,7 +1190,11 @@ int mp_problem_reference(mp_ctx* C, mp_problem* P) {
 
 void mp_problem_free(mp_problem* p) {
   if (!p) return;
-  if (p->ctx) { (void)hipSetDevice(p->ctx->device); p->ctx->problems.erase(p); }
+  if (p->ctx) {
+    (void)hipSetDevice(p->ctx->device); p->ctx->problems.erase(p);
+    for (auto it = p->ctx->factor_ids.begin(); it != p->ctx->factor_ids.end();)
+      it = (it->second == p) ? p->ctx->factor_ids.erase(it) : std::next(it);
+  }
   delete p;                           // a problem whose context is gone holds no device memory any more
 }
 

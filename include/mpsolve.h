@@ -122,6 +122,13 @@ int mp_problem_from_device(mp_ctx* ctx, int64_t n, int32_t nrhs, const double* A
 /* Replaces the right-hand sides (device, n x nrhs) and drops the exact / reference solutions; factors cached by
  * solves with "reuse" stay valid (they depend on A only). */
 int mp_problem_set_rhs(mp_ctx* ctx, mp_problem* p, int32_t nrhs, const double* B);
+/* Factor cache by matrix id (no counterpart in the reference; a serving extension of its --reuse).  put
+ * registers a problem under an id (the caller keeps ownership; freeing the problem drops its ids); solves of
+ * that problem with "reuse" keep their factors in it.  get returns the problem and whether it holds factors;
+ * evict releases the factors' device memory and forgets the id. */
+int mp_factor_cache_put(mp_ctx* ctx, uint64_t matrix_id, mp_problem* p);
+int mp_factor_cache_get(mp_ctx* ctx, uint64_t matrix_id, mp_problem** p, int32_t* has_factors);
+int mp_factor_cache_evict(mp_ctx* ctx, uint64_t matrix_id);
 /* FP64 LU solution refined with double-double residuals (best of <= 12 steps), used for the forward-error
  * metrics; replaces compute_reference (src/mpsolve.cu:99-125) without the kappa_2 estimate. */
 int mp_problem_reference(mp_ctx* ctx, mp_problem* p);

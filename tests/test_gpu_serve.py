@@ -127,3 +127,26 @@ def test_ladder_reaches_tau_and_resumes_at_final_stage(ctx):
     if r1["stages"] == [r1["final_stage"]]:       # no escalation: the same stage from the same start
         np.testing.assert_array_equal(X1, X2)
     P.close()
+
+
+def test_factor_cache_put_get_evict(ctx):
+    n = 900
+    print(f"[factor cache] n={n}", flush=True)
+    dA, dB = _gen(ctx, n, 1, 1e3, 44)
+    P = ctx.problem_from_device(n, 1, dA, dB)
+    ctx.factor_cache_put(7, P)
+    assert not ctx.factor_cache_has(7)
+    params = dict(nb=128, tau=TAU, stop_on="eta", reuse=True)
+    _, r1 = P.solve("mx16x3_gmresir_w64_rdd", **params)
+    assert ctx.factor_cache_has(7) and r1["refactorizations"] == 1
+    _, r2 = P.solve("mx16x3_gmresir_w64_rdd", **params)
+    assert r2["refactorizations"] == 0
+    ctx.factor_cache_evict(7)
+    with pytest.raises(Exception, match="not in the factor cache"):
+        ctx.factor_cache_has(7)
+    _, r3 = P.solve("mx16x3_gmresir_w64_rdd", **params)
+    assert r3["refactorizations"] == 1 and r3["terminal_reason"] == "converged"
+    ctx.factor_cache_put(8, P)
+    P.close()
+    with pytest.raises(Exception, match="not in the factor cache"):
+        ctx.factor_cache_has(8)

@@ -185,9 +185,6 @@ static MethodSpec make_method(const std::string& id, const Params& a) {
   else if (id == "policy_mx16_ladder") { M.factor = "mx16"; M.scale_mode = 1; M.gmres = true; M.resid = 128; M.policy = "ladder16"; }
   else if (id == "policy_mx16x3_ladder") { M.factor = "mx16x3"; M.scale_mode = 1; M.gmres = true; M.resid = 128; M.policy = "ladder16x3"; }
   else throw MPError("unknown method id " + id, -1);
-  if (M.scale_mode == 2) unsupported("two-sided row/column scaling (scale_mode 2) is not implemented");
-  if (M.apply64) unsupported("promoted FP64 factor application (apply64) is not implemented");
-  if (M.gmres_left) unsupported("left-preconditioned GMRES is not implemented");
   return M;
 }
 
@@ -231,6 +228,9 @@ struct Factors {
   double gscale = 1.0; int gscale_exp = 0; int scale_mode = 0; int nb = 0;
   int info = 0; double growth = -1; unsigned long long conv_overflow = 0, conv_subnormal = 0; int panels = 0;
   bool valid = false;
+  DBuf<double> LU64p; bool promoted = false;   // FP64 copy of FP32 factors, applied by FP64 getrs (apply64)
+  DBuf<double> dr, dc;                          // two-sided power-of-two scales (scale mode 2): F = g Dr A Dc
+  mutable DBuf<double> rs;                      // Dr r, the scaled right-hand side of an application
 };
 
 static int fmt_of(const std::string& kind) {
@@ -265,31 +265,116 @@ __global__ void k_scale_f32(float* __restrict__ x, int64_t n, float s) {
   if (i < n) x[i] *= s;
 }
 
+// ---- two-sided power-of-two equilibration (scale mode 2, src/mp_core.cuh:164-166 with the helpers of
+// src/mp_kernels.cuh:25,105-108): dr_i = 2^-(e-1) for the row maximum m_i in [2^(e-1), 2^e) (1 for a zero or
+// non-finite maximum), A <- Dr A; then dc_j the same for the columns of Dr A, A <- A Dc.  Maxima are taken
+// over the non-NaN magnitudes and gathered with 64-bit integer atomics on the bit patterns (non-negative
+// doubles order as integers); every scale is a power of two, so each scaled entry is exact.
+__device__ inline void atomic_max_nonneg(double* p, double v) {
+  atomicMax((unsigned long long*)p, (unsigned long long)__double_as_longlong(v));
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_eq_rowmax(const T* __restrict__ A, int64_t n, double* __restrict__ dr) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  const int64_t c0 = (int64_t)blockIdx.y * 64, c1 = c0 + 64 < n ? c0 + 64 : n;
+  double m = 0.0;
+  for (int64_t c = c0; c < c1; c++) m = fmax(m, fabs((double)A[c * n + r]));
+  if (m > 0.0) atomic_max_nonneg(dr + r, m);
+}
+__global__ void k_eq_pow2(double* __restrict__ v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double m = v[i];
+  if (!(m > 0.0) || !isfinite(m)) { v[i] = 1.0; return; }
+  int e; frexp(m, &e);
+  v[i] = ldexp(1.0, 1 - e);
+}
+// A <- diag(sr) A diag(sc) on column blockIdx.y, and the column's new absolute maximum into cmax[c]
+// (cmax == nullptr: not gathered)
+template <typename T>
+__global__ __launch_bounds__(256) void k_eq_scale(T* __restrict__ A, int64_t n, const double* __restrict__ sr,
+                                                  const double* __restrict__ sc, double* __restrict__ cmax) {
+  __shared__ double red[4];
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t c = blockIdx.y;
+  double m = 0.0;
+  if (r < n) {
+    const T v = (T)((double)A[c * n + r] * (sr ? sr[r] : 1.0) * (sc ? sc[c] : 1.0));
+    A[c * n + r] = v;
+    m = fabs((double)v);
+    if (m != m) m = 0.0;
+  }
+  if (!cmax) return;
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    if (m > 0.0) atomic_max_nonneg(cmax + c, m);
+  }
+}
+__global__ void k_scale_rows(const double* __restrict__ x, double* __restrict__ y, int64_t n, int nrhs,
+                             const double* __restrict__ s) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n * nrhs) y[i] = x[i] * s[i % n];
+}
+
+template <typename T>
+static void equilibrate(mp_ctx* C, T* A, int64_t n, Factors& F) {
+  Workspace& W = C->W;
+  F.dr.ensure(C->tracker, n, "dr"); F.dc.ensure(C->tracker, n, "dc");
+  fill_f64(F.dr.p, n, 0.0, W); fill_f64(F.dc.p, n, 0.0, W);
+  hipLaunchKernelGGL(k_eq_rowmax<T>, dim3(cdiv(n, 256), cdiv(n, 64)), dim3(256), 0, W.stream, A, n, F.dr.p);
+  LAUNCH_CK();
+  hipLaunchKernelGGL(k_eq_pow2, dim3(cdiv(n, 256)), dim3(256), 0, W.stream, F.dr.p, n);
+  LAUNCH_CK();
+  hipLaunchKernelGGL(k_eq_scale<T>, dim3(cdiv(n, 256), (unsigned)n), dim3(256), 0, W.stream, A, n, F.dr.p,
+                     (const double*)nullptr, F.dc.p);
+  LAUNCH_CK();
+  hipLaunchKernelGGL(k_eq_pow2, dim3(cdiv(n, 256)), dim3(256), 0, W.stream, F.dc.p, n);
+  LAUNCH_CK();
+  hipLaunchKernelGGL(k_eq_scale<T>, dim3(cdiv(n, 256), (unsigned)n), dim3(256), 0, W.stream, A, n,
+                     (const double*)nullptr, F.dc.p, (double*)nullptr);
+  LAUNCH_CK();
+}
+
 static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, int nb, int scale_mode, Factors& F) {
   Workspace& W = C->W;
   const int64_t n = P.n; const size_t nn = (size_t)n * n;
   W.factor_gen++;   // cached getrs block inverses of any earlier factors are stale from here on
-  F.n = n; F.kind = kind; F.scale_mode = scale_mode; F.nb = nb; F.valid = false; F.growth = -1;
+  F.n = n; F.kind = kind; F.scale_mode = scale_mode; F.nb = nb; F.valid = false; F.growth = -1; F.promoted = false;
   F.conv_overflow = F.conv_subnormal = 0; F.info = 0; F.panels = 0;
   F.prec = (kind == "cs64" || kind == "mx64") ? 64 : 32;
   F.ipiv.ensure(C->tracker, n, "ipiv");
   F.perm.ensure(C->tracker, n, "perm");
+  // Global scale g = 2^-e for the maximum m = f 2^e (f in [0.5, 1)) of the copy in the factor precision
+  // (src/mp_core.cuh:167).  Without two-sided scaling that maximum is A's maximum rounded to the factor
+  // precision (rounding is monotonic), known before the copy is made, so the FP64 -> FP32 convert and the
+  // scale run as one pass; with it, the maximum of the equilibrated copy is measured.
   F.gscale = 1.0; F.gscale_exp = 0;
-  if (scale_mode >= 1 && P.amax > 0 && std::isfinite(P.amax)) {
-    int e; std::frexp(P.amax, &e); F.gscale_exp = -e; F.gscale = std::ldexp(1.0, -e);
-  }
-  double amax_f = P.amax * F.gscale;
+  auto set_gscale = [&](double m) {
+    if (m > 0 && std::isfinite(m)) { int e; std::frexp(m, &e); F.gscale_exp = -e; F.gscale = std::ldexp(1.0, -e); }
+  };
+  const double amax_copy = (F.prec == 32) ? (double)(float)P.amax : P.amax;
+  if (scale_mode == 1) set_gscale(amax_copy);
+  double amax_f = amax_copy * F.gscale;
   if (F.prec == 32) {
     F.LU32.ensure(C->tracker, nn, "LU32");
     if (P.fp32) {
       HIP_CK(hipMemcpyAsync(F.LU32.p, P.A32.p, nn * 4, hipMemcpyDeviceToDevice, W.stream));
-      // the reference scales the FP32 copy in place (src/mp_core.cuh:167): a power of two, exact in range
-      if (F.gscale != 1.0) {
-        hipLaunchKernelGGL(k_scale_f32, dim3(cdiv(nn, 256)), dim3(256), 0, W.stream, F.LU32.p, (int64_t)nn, (float)F.gscale);
-        LAUNCH_CK();
-      }
     } else {
-      scale_convert_f64_to_f32(P.A64.p, n, n, F.LU32.p, n, F.gscale, W);
+      scale_convert_f64_to_f32(P.A64.p, n, n, F.LU32.p, n, F.gscale, W);   // fl32(a) * fl32(g), as convert-then-scale
+    }
+    if (scale_mode == 2) {
+      equilibrate<float>(C, F.LU32.p, n, F);
+      amax_f = dev_absmax(C, F.LU32.p, true, n, n);
+      set_gscale(amax_f); amax_f *= F.gscale;
+    }
+    // the reference scales the FP32 copy in place (src/mp_core.cuh:167): a power of two, exact in range
+    if (F.gscale != 1.0 && (P.fp32 || scale_mode == 2)) {
+      hipLaunchKernelGGL(k_scale_f32, dim3(cdiv(nn, 256)), dim3(256), 0, W.stream, F.LU32.p, (int64_t)nn, (float)F.gscale);
+      LAUNCH_CK();
     }
   } else {
     F.LU64.ensure(C->tracker, nn, "LU64");
@@ -298,6 +383,11 @@ static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, i
       LAUNCH_CK();
     } else {
       HIP_CK(hipMemcpyAsync(F.LU64.p, P.A64.p, nn * 8, hipMemcpyDeviceToDevice, W.stream));
+    }
+    if (scale_mode == 2) {
+      equilibrate<double>(C, F.LU64.p, n, F);
+      amax_f = dev_absmax(C, F.LU64.p, false, n, n);
+      set_gscale(amax_f); amax_f *= F.gscale;
     }
     // the reference scales the FP64 copy too (src/mp_core.cuh:167) and applies the factors with g
     if (F.gscale != 1.0) axpy_scale_inplace(F.LU64.p, (int64_t)nn, F.gscale, W);
@@ -334,19 +424,54 @@ static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, i
   F.valid = true;
 }
 
-// d = A^{-1} r through the factors (r, d FP64 device, n x nrhs, leading dimension n)
-static void apply_inverse(mp_ctx* C, const Factors& F, const double* r, double* d, int nrhs, int* trs) {
+// FP64 copy of FP32 factors with int32 LAPACK pivots, for the promoted application (the reference's
+// LU64_promoted, src/mp_core.cuh:178)
+static void promote_factors(mp_ctx* C, Factors& F) {
+  if (F.prec != 32 || F.promoted) return;
   Workspace& W = C->W;
-  const int64_t n = F.n;
-  if (F.prec == 64) {
-    if (d != r) HIP_CK(hipMemcpyAsync(d, r, (size_t)n * nrhs * 8, hipMemcpyDeviceToDevice, W.stream));
-    RB_CK(rocblas_set_stream(C->rb, W.stream));
-    RB_CK(rocsolver_dgetrs(C->rb, rocblas_operation_none, (rocblas_int)n, (rocblas_int)nrhs, const_cast<double*>(F.LU64.p),
-                           (rocblas_int)n, F.ipiv32.p, d, (rocblas_int)n));
-    if (F.gscale != 1.0) axpy_scale_inplace(d, (int64_t)n * nrhs, F.gscale, W);   // (gA)^{-1} r = A^{-1} r / g
-  } else {
-    getrs(n, nrhs, F.LU32.p, n, F.perm.p, r, n, d, n, F.gscale, W, true);
+  const int64_t n = F.n; const size_t nn = (size_t)n * n;
+  F.LU64p.ensure(C->tracker, nn, "LU64_promoted");
+  hipLaunchKernelGGL(k_f32_to_f64, dim3(cdiv(nn, 256)), dim3(256), 0, W.stream, F.LU32.p, F.LU64p.p, (int64_t)nn);
+  LAUNCH_CK();
+  if (F.kind != "cs32") {      // own factorizations report int64 pivots
+    std::vector<int64_t> ip(n);
+    HIP_CK(hipMemcpyAsync(ip.data(), F.ipiv.p, n * 8, hipMemcpyDeviceToHost, W.stream));
+    HIP_CK(hipStreamSynchronize(W.stream));
+    std::vector<int32_t> ip32(ip.begin(), ip.end());
+    F.ipiv32.ensure(C->tracker, n + 1, "ipiv32");
+    HIP_CK(hipMemcpyAsync(F.ipiv32.p, ip32.data(), n * 4, hipMemcpyHostToDevice, W.stream));
   }
+  HIP_CK(hipStreamSynchronize(W.stream));
+  F.promoted = true;
+}
+
+// d = A^{-1} r through the factors (r, d FP64 device, n x nrhs, leading dimension n); with apply64 the FP64
+// copy of FP32 factors is applied in FP64
+static void apply_inverse(mp_ctx* C, const Factors& F, const double* r, double* d, int nrhs, int* trs, bool apply64 = false) {
+  Workspace& W = C->W;
+  const int64_t n = F.n; const int64_t m = n * nrhs;
+  // with F = g Dr A Dc: A^{-1} r = g Dc F^{-1} (Dr r)  (src/mp_core.cuh:184,188-189)
+  if (F.scale_mode == 2) {
+    F.rs.ensure(C->tracker, (size_t)m, "rs");
+    hipLaunchKernelGGL(k_scale_rows, dim3(cdiv(m, 256)), dim3(256), 0, W.stream, r, F.rs.p, n, nrhs, F.dr.p);
+    LAUNCH_CK();
+    r = F.rs.p;
+  }
+  const bool fp64 = F.prec == 64 || (apply64 && F.promoted);
+  if (fp64) {
+    const double* LU = (F.prec == 64) ? F.LU64.p : F.LU64p.p;
+    if (d != r) HIP_CK(hipMemcpyAsync(d, r, (size_t)m * 8, hipMemcpyDeviceToDevice, W.stream));
+    RB_CK(rocblas_set_stream(C->rb, W.stream));
+    RB_CK(rocsolver_dgetrs(C->rb, rocblas_operation_none, (rocblas_int)n, (rocblas_int)nrhs, const_cast<double*>(LU),
+                           (rocblas_int)n, F.ipiv32.p, d, (rocblas_int)n));
+  } else {
+    getrs(n, nrhs, F.LU32.p, n, F.perm.p, r, n, d, n, F.gscale, W, true);    // the scale by g is fused
+  }
+  if (F.scale_mode == 2) {
+    hipLaunchKernelGGL(k_scale_rows, dim3(cdiv(m, 256)), dim3(256), 0, W.stream, d, d, n, nrhs, F.dc.p);
+    LAUNCH_CK();
+  }
+  if (fp64 && F.gscale != 1.0) axpy_scale_inplace(d, m, F.gscale, W);   // (gA)^{-1} r = A^{-1} r / g
   if (trs) (*trs)++;
 }
 
@@ -445,9 +570,11 @@ struct Engine {
     Factors& F = reuse ? *P.fcache : fact;
     const double t0 = wall_now();
     if (reuse && F.valid && F.kind == kind && F.scale_mode == M.scale_mode && F.nb == M.nb) {
+      if (M.apply64) promote_factors(C, F);
       T.stages.push_back(kind + "(reused)"); T.stage_times.push_back(0.0); T.growth = F.growth; T.factor_info = F.info; return F;
     }
     factorize(C, P, kind, M.nb, M.scale_mode, F);
+    if (M.apply64 && F.info == 0) promote_factors(C, F);
     sync();
     const double dt = wall_now() - t0;
     T.t_factor_total += dt; T.stages.push_back(kind); T.stage_times.push_back(dt); T.refactorizations++;
@@ -483,8 +610,10 @@ struct Engine {
     K.ensure(n, m);
     double* hp = C->hpin;
     fill_f64(d, n, 0.0, W());
-    copy_f64(r, K.rcur.p, n, W());
-    K.norm(r, K.scal());
+    // the left variant works on the preconditioned residual F^{-1} r (src/mp_core.cuh:240-241)
+    if (M.gmres_left) { apply_inverse(C, F, r, K.rcur.p, 1, &T.trs, M.apply64); T.precond_applies++; }
+    else copy_f64(r, K.rcur.p, n, W());
+    K.norm(K.rcur.p, K.scal());
     HIP_CK(hipMemcpyAsync(hp, K.scal(), 8, hipMemcpyDeviceToHost, W().stream));
     sync();
     const double beta0 = hp[0];
@@ -503,12 +632,20 @@ struct Engine {
       int j = 0; bool happy = false, nonfinite = false;
       for (j = 0; j < m && budget_left > 0; j++) {
         double* vj = K.V.p + (int64_t)j * n;
-        double* zj = K.Z.p + (int64_t)j * n;
-        apply_inverse(C, F, vj, zj, 1, &T.trs); T.precond_applies++;
-        // w = A z_j (FP64)
-        ResidArgs ra; ra.n = n; ra.nrhs = 1; ra.a32 = P.fp32; ra.A = P.a_ptr(); ra.lda = n; ra.X = zj; ra.ldx = n;
-        ra.R = K.w.p; ra.ldr = n; ra.matvec = true;
-        residual(ra, W()); T.matvecs++;
+        if (M.gmres_left) {
+          // w = F^{-1} (A v_j)
+          ResidArgs ra; ra.n = n; ra.nrhs = 1; ra.a32 = P.fp32; ra.A = P.a_ptr(); ra.lda = n; ra.X = vj; ra.ldx = n;
+          ra.R = K.sR.p; ra.ldr = n; ra.matvec = true;
+          residual(ra, W()); T.matvecs++;
+          apply_inverse(C, F, K.sR.p, K.w.p, 1, &T.trs, M.apply64); T.precond_applies++;
+        } else {
+          double* zj = K.Z.p + (int64_t)j * n;
+          apply_inverse(C, F, vj, zj, 1, &T.trs, M.apply64); T.precond_applies++;
+          // w = A z_j (FP64)
+          ResidArgs ra; ra.n = n; ra.nrhs = 1; ra.a32 = P.fp32; ra.A = P.a_ptr(); ra.lda = n; ra.X = zj; ra.ldx = n;
+          ra.R = K.w.p; ra.ldr = n; ra.matvec = true;
+          residual(ra, W()); T.matvecs++;
+        }
         // CGS2
         K.dots(K.V.p, j + 1, K.w.p, K.h(), false);
         K.update(K.V.p, j + 1, K.h(), K.w.p, -1.0);
@@ -534,7 +671,7 @@ struct Engine {
       const int k = j;
       if (k == 0) break;
       K.lsq(k, K.y());
-      K.update(K.Z.p, k, K.y(), d, 1.0);
+      K.update(M.gmres_left ? K.V.p : K.Z.p, k, K.y(), d, 1.0);   // left: d += V y; right (flexible): d += Z y
       // true residual of the correction equation: sR = r - A d (FP64)
       ResidArgs ra; ra.n = n; ra.nrhs = 1; ra.a32 = P.fp32; ra.A = P.a_ptr(); ra.lda = n; ra.X = d; ra.ldx = n;
       ra.B = r; ra.ldb = n; ra.R = K.sR.p; ra.ldr = n;
@@ -547,7 +684,8 @@ struct Engine {
       const double tn = hp[0], rn0 = hp[1];
       const double true_rel = (rn0 > 0) ? tn / rn0 : 0.0;
       if (instrument) T.ihist.push_back({outer, cycles, k - 1, rhs, hp[4], true_rel, "cycle_end"});
-      copy_f64(K.sR.p, K.rcur.p, n, W());
+      if (M.gmres_left) { apply_inverse(C, F, K.sR.p, K.rcur.p, 1, &T.trs, M.apply64); T.precond_applies++; }
+      else copy_f64(K.sR.p, K.rcur.p, n, W());
       if (!std::isfinite(tn)) { status = "nonfinite"; break; }
       if (true_rel <= M.gmres_tol) { status = happy ? "happy_breakdown_verified" : "converged_true"; break; }
       if (happy) { status = "happy_breakdown_unverified"; break; }
@@ -564,7 +702,7 @@ struct Engine {
                            bool initial_solve, Metrics& Mt) {
     if (initial_solve) {
       const double t0 = wall_now();
-      apply_inverse(C, F, P.B.p, D.p, nrhs, &T.trs);
+      apply_inverse(C, F, P.B.p, D.p, nrhs, &T.trs, M.apply64);
       set_x(M, D.p);
       sync();
       T.t_initial += wall_now() - t0;
@@ -593,7 +731,7 @@ struct Engine {
           if (st == "nonfinite") T.hist.back().note += ";inner_nonfinite";
         }
       } else {
-        apply_inverse(C, F, E.R.p, D.p, nrhs, &T.trs); T.precond_applies++;
+        apply_inverse(C, F, E.R.p, D.p, nrhs, &T.trs, M.apply64); T.precond_applies++;
       }
       T.hist.back().inner_in_step = T.inner_total - inner_before;
       add_update(M, D.p);
@@ -1246,9 +1384,13 @@ int mp_methods_json(char* json, size_t cap) {
       j.s("method_id", id).s("kind", M.kind).s("factorization", M.factor).s("factorization_contract", factor_contract(M.factor))
        .i("working", M.working).s("residual", M.resid == 128 ? "double-double (TwoProd via FMA, TwoSum), rounded to FP64" :
                                                  (M.resid == 64 ? "FP64 with FMA, |A||x|+|b| in the same pass" : "FP32"))
-       .i("scale_mode", M.scale_mode).b("gmres", M.gmres).s("gmres_variant", M.gmres ? "right-preconditioned flexible GMRES, CGS2, device Givens" : "none")
-       .s("policy", M.policy).s("unsupported_reason", M.unsupported_reason)
-       .s("triangular_solves", M.factor == "cs64" ? "rocSOLVER dgetrs (FP64)" : "own HIP getrs: FP64->FP32 cast, FP32 substitution, FP32->FP64 cast, power-of-two unscaling");
+       .i("scale_mode", M.scale_mode).b("gmres", M.gmres)
+       .s("gmres_variant", !M.gmres ? "none" : (M.gmres_left ? "left-preconditioned GMRES on F^-1 A, CGS2, device Givens"
+                                                            : "right-preconditioned flexible GMRES, CGS2, device Givens"))
+       .b("apply64", M.apply64).s("policy", M.policy).s("unsupported_reason", M.unsupported_reason)
+       .s("triangular_solves", (M.factor == "cs64" || M.factor == "mx64") ? "rocSOLVER dgetrs (FP64)"
+                               : (M.apply64 ? "FP32 factors promoted to FP64, rocSOLVER dgetrs, power-of-two unscaling"
+                                            : "own HIP getrs: FP64->FP32 cast, FP32 substitution, FP32->FP64 cast, power-of-two unscaling"));
       out += (first ? "" : ",") + j.obj();
       first = false;
     }

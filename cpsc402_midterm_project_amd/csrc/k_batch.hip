@@ -107,8 +107,11 @@ __device__ void setup(Lds& S, const double* __restrict__ A, int n, int ldr, floa
   }
   const double amax = block_nanmax(S, am, 0);
   const double anorm = block_nanmax(S, rs, 1);
+  // global scale from the maximum of the FP32 copy, fl32(amax) (rounding is monotonic); entries are rounded to
+  // FP32 first and then scaled in FP32, as the convert-then-scale of src/mp_core.cuh:161,167
   g = 1.0;
-  if (amax > 0.0 && amax <= DBL_MAX) { int e; frexp(amax, &e); g = ldexp(1.0, -e); }
+  const double amax32 = (double)(float)amax;
+  if (amax32 > 0.0 && amax32 <= DBL_MAX) { int e; frexp(amax32, &e); g = ldexp(1.0, -e); }
   if (t == 0) { S.anorm = anorm; S.amax = amax; }
   for (int i = t; i < n; i += BT) S.rowat[i] = i;
   // FP64 column-major -> FP32 row-major through a 64 x 65 LDS tile (coalesced on both sides)
@@ -119,7 +122,7 @@ __device__ void setup(Lds& S, const double* __restrict__ A, int n, int ldr, floa
       for (int m = 0; m < 16; m++) {
         const int cl = (t >> 6) + 4 * m, r = rb + rl, c = cb + cl;
         float v = 0.0f;
-        if (r < n && c < n) v = (float)(A[r + (int64_t)c * n] * g);
+        if (r < n && c < n) v = (float)A[r + (int64_t)c * n] * (float)g;
         tile[rl * 65 + cl] = v;
       }
       __syncthreads();

@@ -218,7 +218,7 @@ __global__ __launch_bounds__(256) void k_scale_convert(const double* __restrict_
                                                        float* __restrict__ out, int64_t ldo, double s) {
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t c = blockIdx.y;
-  if (r < n) out[c * ldo + r] = (float)(A[c * lda + r] * s);
+  if (r < n) out[c * ldo + r] = (float)A[c * lda + r] * (float)s;   // round to FP32, then the exact power-of-two scale
 }
 void scale_convert_f64_to_f32(const double* A, int64_t n, int64_t lda, float* out, int64_t ldo, double s, Workspace& W) {
   hipLaunchKernelGGL(k_scale_convert, dim3(cdiv(n, 256), (unsigned)n), dim3(256), 0, W.stream, A, n, lda, out, ldo, s);

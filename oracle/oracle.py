@@ -147,24 +147,59 @@ def exact_residual(A, x, b):
 
 
 # ------------------------------------------------------------------ refinement restatements
+def _pow2_recip(m):
+    """k_pow2_recip (src/mp_kernels.cuh:108): 2^-(e-1) for m in [2^(e-1), 2^e); 1 for m <= 0 or non-finite."""
+    m = np.asarray(m, dtype=np.float64)
+    out = np.ones_like(m)
+    ok = (m > 0) & np.isfinite(m)
+    out[ok] = np.ldexp(1.0, 1 - np.frexp(m[ok])[1])
+    return out
+
+
+def _absmax_rows(A, axis):
+    a = np.abs(A.astype(np.float64))
+    return np.where(np.isnan(a), 0.0, a).max(axis=axis)      # NaN never exceeds the running maximum
+
+
 def factor(A, fmt="fp16", nb=128, scale_mode=1, tile_scale=True):
-    amax = np.abs(A).max()
+    """factorize (src/mp_core.cuh:158-167) for the own blocked kinds: the FP32 copy of A, for scale mode 2 the
+    power-of-two row then column equilibration (src/mp_core.cuh:164-166), for modes 1 and 2 the global scale
+    g = 2^-e from the copy's maximum f 2^e, applied in FP32; then the blocked LU."""
+    A32 = np.asarray(A, dtype=np.float64).astype(np.float32)
+    dr = dc = None
+    if scale_mode == 2:
+        dr = _pow2_recip(_absmax_rows(A32, 1))
+        A32 = (A32.astype(np.float64) * dr[:, None]).astype(np.float32)
+        dc = _pow2_recip(_absmax_rows(A32, 0))
+        A32 = (A32.astype(np.float64) * dc[None, :]).astype(np.float32)
     g = 1.0
+    amax = float(_absmax_rows(A32, None)) if A32.size else 0.0
     if scale_mode >= 1 and amax > 0 and np.isfinite(amax):
-        e = np.frexp(amax)[1]
-        g = float(np.ldexp(1.0, -int(e)))
-    A32 = (np.asarray(A, dtype=np.float64) * g).astype(np.float32)
+        g = float(np.ldexp(1.0, -int(np.frexp(amax)[1])))
+        A32 = A32 * np.float32(g)
     LU, ipiv, perm, info, cnt = getrf(A32, nb=nb, fmt=fmt, tile_scale=tile_scale)
-    return dict(LU=LU, ipiv=ipiv, perm=perm, info=info, g=g, counts=cnt)
+    return dict(LU=LU, ipiv=ipiv, perm=perm, info=info, g=g, dr=dr, dc=dc, counts=cnt)
 
 
-def ir_stationary(A, B, fmt="fp16", nb=128, tau=1e-15, max_outer=50, dd=False, stop_eta_only=True, tile_scale=True):
+def apply_inverse(F, R):
+    """apply_inverse with FP32 factors (src/mp_core.cuh:182-190): (Dr r) -> FP32 substitution -> Dc, g."""
+    R = np.asarray(R, dtype=np.float64).reshape(F["LU"].shape[0], -1)
+    if F.get("dr") is not None:
+        R = R * F["dr"][:, None]
+    D = getrs(F["LU"], F["perm"], R, F["g"])
+    if F.get("dc") is not None:
+        D = D * F["dc"][:, None]
+    return D
+
+
+def ir_stationary(A, B, fmt="fp16", nb=128, tau=1e-15, max_outer=50, dd=False, stop_eta_only=True, tile_scale=True,
+                  scale_mode=1):
     """Stationary IR as in Engine::refine_stage (src/mp_core.cuh:281-296): x0 = A^-1 b, then residual+metrics,
     stopping tests (converged / nonfinite / max_outer / stagnation), correction, x += d."""
-    F = factor(A, fmt, nb, 1, tile_scale)
+    F = factor(A, fmt, nb, scale_mode, tile_scale)
     n = A.shape[0]
     B = np.asarray(B, dtype=np.float64).reshape(n, -1)
-    x = getrs(F["LU"], F["perm"], B, F["g"])
+    x = apply_inverse(F, B)
     hist = []
     prev, stall, outer = np.inf, 0, 0
     while True:
@@ -185,7 +220,7 @@ def ir_stationary(A, B, fmt="fp16", nb=128, tau=1e-15, max_outer=50, dd=False, s
             Rh, Rl = resid_dd(A, x, B); R = Rh + Rl
         else:
             R, _ = resid_fp64(A, x, B)
-        x = x + getrs(F["LU"], F["perm"], R, F["g"])
+        x = x + apply_inverse(F, R)
         outer += 1
 
 
@@ -244,7 +279,7 @@ def fgmres_ir(A, B, fmt="fp16", nb=128, tau=1e-15, max_outer=30, m=30, tol=1e-4,
     F = factor(A, fmt, nb, 1, tile_scale)
     n = A.shape[0]
     b = np.asarray(B, dtype=np.float64).reshape(n)
-    pre = lambda v: getrs(F["LU"], F["perm"], v, F["g"])[:, 0]
+    pre = lambda v: apply_inverse(F, v)[:, 0]
     x = pre(b)
     hist = []; inner = 0
     for outer in range(max_outer + 1):

@@ -61,5 +61,9 @@ def test_method_registry_covers_reference_ids():
     assert cat["mx16_gmresir_w64_rdd"]["residual"].startswith("double-double")
     assert cat["policy_mx8_ladder"]["policy"] == "ladder"
     # options the build does not implement are declared, not silently changed
-    assert cat["hetero16_ir_w64_r64_twosided"]["kind"] == "unsupported"
+    assert cat["hetero16_ir_w64_r64_twosided"]["kind"] != "unsupported" and cat["hetero16_ir_w64_r64_twosided"]["scale_mode"] == 2
+    for mid in ("native_leftgmres32_app32_w64_r64", "native_leftgmres32_app64_w64_r64",
+                "native_gmresir32_app64_w64_r64", "hetero16_leftgmres_app64_w64_r64", "hetero16_gmresir_app64_w64_r64"):
+        assert cat[mid]["kind"] != "unsupported" and cat[mid]["gmres"], mid
+    assert cat["native_leftgmres32_app64_w64_r64"]["apply64"]
     json.dumps(cat)

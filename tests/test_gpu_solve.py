@@ -56,6 +56,54 @@ def test_reference_methods_converge_kappa1e2(ctx, oracle, method):
     assert m["eta_max"] <= tau and m["omega_max"] <= tau
 
 
+@pytest.mark.parametrize("method", ["native_leftgmres32_app32_w64_r64", "native_leftgmres32_app64_w64_r64",
+                                    "native_gmresir32_app64_w64_r64", "hetero16_gmresir_app64_w64_r64",
+                                    "hetero16_leftgmres_app64_w64_r64"])
+def test_left_gmres_and_promoted_application(ctx, oracle, method):
+    """Left-preconditioned GMRES (Krylov space of F^-1 A from F^-1 r, correction d += V y) and the FP64
+    application of promoted FP32 factors (src/mp_core.cuh:178-186, 236-278) reach the FP64 stopping test."""
+    A, b, xt, _ = randsvd(640, 1e4, 9210, gpu=False)
+    print(f"[solve] {method} n=640 kappa=1e4", flush=True)
+    P = ctx.problem(A, b, x_exact=xt)
+    X, rec = P.solve(method, nb=128)
+    assert rec["ok"] and rec["terminal_reason"] == "converged", (rec["terminal_reason"], rec["error"], rec["eta_max"])
+    assert rec["precond_applies"] > 0 and rec["matvecs"] > 0
+    tau = 100 * 2.0 ** -53
+    m = oracle.metrics(A, b, X, dd=True)
+    assert m["eta_max"] <= tau and m["omega_max"] <= tau
+
+
+def _exact_lu_system(n, seed):
+    """A = L U with dyadic factors that one FP32 panel (nb >= n) reproduces exactly: unit L with |l| <= 1/4,
+    U with diagonal +-{1, 2, 4} and off-diagonals in {0, +-1/8}, so partial pivoting keeps the natural order
+    and every quotient, product and sum is exact.  (Blocked factorizations are not exact here: the triangular
+    solve for U12 goes through the inverse of L11, whose entries need more than 24 bits.)"""
+    rng = np.random.default_rng(seed)
+    L = np.tril(rng.choice([-0.25, 0.0, 0.25], size=(n, n)), -1) + np.eye(n)
+    U = np.triu(rng.choice([-0.125, 0.0, 0.125], size=(n, n)), 1)
+    U += np.diag(rng.choice([-4.0, -2.0, -1.0, 1.0, 2.0, 4.0], size=n))
+    return np.asfortranarray(L @ U), rng.standard_normal((n, 1))
+
+
+def test_promoted_application_is_fp64(ctx, oracle):
+    """The initial solve x0 = F^-1 b with FP32 factors that are exactly A's: applied in FP64 (apply64) x0 has an
+    FP64-level backward error and the stage stops converged before any refinement step; applied in FP32 it
+    carries FP32 rounding."""
+    A, b = _exact_lu_system(64, 12)
+    P = ctx.problem(A, b)
+    f = oracle.factor(A, fmt="fp16", nb=64, scale_mode=1, tile_scale=False)
+    L, U = oracle.unpack_lu(f["LU"])
+    assert f["info"] == 0 and np.array_equal(L @ U, f["g"] * A)       # the factors are exactly those of A
+    X64, r64 = P.solve("hetero16_leftgmres_app64_w64_r64", nb=64, max_outer=0)
+    X32, r32 = P.solve("hetero16_gmresir_w64_r64", nb=64, max_outer=0)
+    e64 = oracle.metrics(A, b, X64, dd=True)["eta_max"]
+    e32 = oracle.metrics(A, b, X32, dd=True)["eta_max"]
+    assert r64["terminal_reason"] == "converged" and r64["outer_iters"] == 0, (r64["terminal_reason"], e64)
+    assert e64 <= 64 * 2.0 ** -53, e64
+    assert e32 > 1e-10 and r32["terminal_reason"] == "max_outer", (e32, r32["terminal_reason"])
+    np.testing.assert_allclose(X64, np.linalg.solve(A, b), rtol=0, atol=1e-12 * np.abs(X64).max())
+
+
 def test_direct_fp64_lu(ctx, oracle):
     A, b, xt, _ = randsvd(1000, 1e3, 9202, gpu=False)
     P = ctx.problem(A, b)
@@ -75,8 +123,41 @@ def test_singular_and_unsupported_outcomes(ctx):
     assert rec["terminal_reason"] == "singular_factor" and rec["factor_info"] == 2
     X, rec = P.solve("cusolver_irs_m64_l16_classical_fb_default")
     assert not rec["ok"] and rec["terminal_reason"] == "unsupported_on_platform"
-    X, rec = P.solve("hetero16_ir_w64_r64_twosided")
+    X, rec = P.solve("magma_ht_ir_w64_r64")
     assert rec["terminal_reason"] == "unsupported_on_platform"
+
+
+def _rowcol_scaled(n, span, seed):
+    """A = Dr0 B Dc0 with B randsvd (kappa 10) and power-of-two Dr0, Dc0 drawn from 2^[-span, span]."""
+    rng = np.random.default_rng(seed)
+    B0, _, _, _ = randsvd(n, 10.0, seed, gpu=False)
+    A = np.ldexp(1.0, rng.integers(-span, span + 1, n))[:, None] * B0 * np.ldexp(1.0, rng.integers(-span, span + 1, n))[None, :]
+    return np.asfortranarray(A), rng.standard_normal((n, 2))
+
+
+@pytest.mark.parametrize("method,fmt", [("hetero16_ir_w64_r64_twosided", "fp16"), ("native_ir32_w64_r64_twosided", None)])
+def test_two_sided_scaling(ctx, oracle, method, fmt):
+    """Scale mode 2 on a system whose rows and columns span 2^+-24: the equilibrated factors solve it to the
+    FP64 stopping test.  The initial solve x0 = g Dc F^-1 (Dr b) is checked against the oracle's equilibration:
+    with the oracle's scales and (for the own factorization, bitwise equal to the oracle's) factors,
+    y = x0 / (g dc) solves L U y = P (dr b) to FP32 backward accuracy, which fails for any wrong scale."""
+    A, b = _rowcol_scaled(384, 24, 31)
+    print(f"[solve] {method} n=384 span=2^24", flush=True)
+    P = ctx.problem(A, b)
+    X, rec = P.solve(method, nb=64)
+    assert rec["ok"] and rec["terminal_reason"] == "converged", (rec["terminal_reason"], rec["error"], rec["eta_max"])
+    m = oracle.metrics(A, b, X, dd=True)
+    assert m["eta_max"] <= 100 * 2.0 ** -53 and m["omega_max"] <= 100 * 2.0 ** -53
+    if fmt is None:
+        return
+    X0, _ = P.solve(method, nb=64, max_outer=0)
+    F = oracle.factor(A, fmt, 64, scale_mode=2, tile_scale=False)
+    L, U = oracle.unpack_lu(F["LU"])
+    y = X0 / (F["g"] * F["dc"][:, None])
+    rhs = (F["dr"][:, None] * b)[F["perm"]]
+    res = np.abs(L @ (U @ y) - rhs).max(axis=0)
+    scale = (np.abs(L) @ (np.abs(U) @ np.abs(y))).max(axis=0) + np.abs(rhs).max(axis=0)
+    assert np.all(res / scale <= 4 * 384 * 2.0 ** -24), res / scale
 
 
 def test_unattainable_tau_stagnates(ctx):

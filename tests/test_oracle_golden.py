@@ -222,6 +222,45 @@ def test_getrs_is_lu_solve(oracle):
     assert np.abs(d - ref).max() / np.abs(ref).max() < 1e-4
 
 
+def test_two_sided_scaling_hand_example(oracle):
+    """Scale mode 2 (src/mp_core.cuh:164-167), worked by hand.  A = [[3, 1/2], [1/4, 1/100]]: row maxima 3 and
+    1/4 give dr = (1/2, 4) (each row maximum lands in [1, 2)); Dr A = [[3/2, 1/4], [1, 1/25]] has column maxima
+    3/2 and 1/4, so dc = (1, 4); Dr A Dc = [[3/2, 1], [1, 4/25]] has maximum 3/2 = 0.75 * 2^1, so g = 1/2."""
+    A = np.array([[3.0, 0.5], [0.25, 0.01]], order="F")
+    F = oracle.factor(A, "fp32", 32, scale_mode=2)
+    np.testing.assert_array_equal(F["dr"], [0.5, 4.0])
+    np.testing.assert_array_equal(F["dc"], [1.0, 4.0])
+    assert F["g"] == 0.5
+    M = np.array([[0.75, 0.5], [0.5, np.float32(0.01) * np.float32(8)]], dtype=np.float32)   # fl32(1/100) * 4 * 4 / 2
+    L, U = oracle.unpack_lu(F["LU"])
+    np.testing.assert_allclose(L @ U, M[F["perm"]].astype(np.float64), rtol=0, atol=1e-7)
+    # zero and non-finite maxima leave the row or column unscaled (k_pow2_recip)
+    Z = np.array([[0.0, 0.0], [5.0, 1.0]], order="F")
+    np.testing.assert_array_equal(oracle.factor(Z, "fp32", 32, scale_mode=2)["dr"], [1.0, 0.25])
+    # the application undoes every scale: x = A^-1 b at FP32 accuracy on a badly row/column-scaled system
+    rng = np.random.default_rng(3)
+    n = 96
+    B0, _, _, _ = randsvd(n, 10.0, 5, gpu=False)
+    As = np.asfortranarray(np.ldexp(1.0, rng.integers(-30, 31, n))[:, None] * B0 * np.ldexp(1.0, rng.integers(-30, 31, n))[None, :])
+    b = rng.standard_normal((n, 1))
+    F = oracle.factor(As, "fp32", 32, scale_mode=2)
+    x = oracle.apply_inverse(F, b)
+    ref = np.linalg.solve(As, b)
+    assert np.abs(x - ref).max() / np.abs(ref).max() < 1e-4
+    # without equilibration the same system has entries that FP16 cannot hold; with it the FP16 factors
+    # reach FP16-level accuracy
+    x16 = oracle.apply_inverse(oracle.factor(As, "fp16", 32, scale_mode=2, tile_scale=False), b)
+    assert np.abs(x16 - ref).max() / np.abs(ref).max() < 1e-1
+
+
+def test_global_scale_uses_fp32_rounded_maximum(oracle):
+    """The global scale comes from the maximum of the FP32 copy (src/mp_core.cuh:167): 1 - 2^-30 rounds to 1 in
+    FP32, whose exponent is one more than that of the FP64 value."""
+    A = np.array([[1.0 - 2.0 ** -30, 0.25], [0.5, 0.75]], order="F")
+    assert oracle.factor(A, "fp32", 32, scale_mode=1)["g"] == 0.5
+    assert oracle.factor(np.array([[0.75, 0.25], [0.5, 0.75]], order="F"), "fp32", 32, scale_mode=1)["g"] == 1.0
+
+
 # ------------------------------------------------------------------ generator and refinement restatements
 def test_randsvd_spectrum_and_hash():
     A, b, xt, rec = randsvd(64, 1e4, 9200, gpu=False)

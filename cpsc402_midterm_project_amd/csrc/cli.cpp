@@ -5,9 +5,6 @@
 // (src/mpsolve.cu:130-240).  This driver keeps that boundary: the same commands (run, regen, caps, selftest),
 // the same flags with the same defaults, the same output file names and formats, and trial records with the
 // reference's field names (they come from mp_solve).  Differences, by design of this build:
-//   * --family prescribed (the reference's default) is generated as "randsvd": the same prescribed singular
-//     values with Haar-random orthogonal factors (QR of Gaussian matrices) instead of rotation stages, so the
-//     matrices differ from the reference's for the same seed; the case record says which generator ran;
 //   * --modes host times the upload of A and B plus the solve plus the download of X, with the solve itself
 //     reading the resident copy (the same bytes cross the bus as in the reference's host mode).
 #include "mpsolve.h"
@@ -128,7 +125,7 @@ struct Session {
 // The case object passed to mp_problem_generate: the reference's flag names with dashes as underscores.
 std::string case_spec(const Args& a, std::string& family_generated) {
   const std::string family = a.get("family", "prescribed");
-  family_generated = (family == "prescribed") ? "randsvd" : family;
+  family_generated = family;
   Obj c;
   c.s("case_id", a.get("case-id", "case")).i("n", a.geti("n", 1024)).i("nrhs", a.geti("nrhs", 1))
    .s("family", family_generated).s("family_requested", family).s("spectrum", a.get("spectrum", "geometric"))

@@ -1,20 +1,26 @@
 // generate.hip — deterministic test-system generation on the device (setup, outside every timed region).
 //
 // Families (the case fields of the reference's CaseSpec / generate_problem, src/mp_core.cuh:14-103):
-//   randsvd    A = U diag(sigma) V^T, U and V the Q factors (rocSOLVER geqrf/orgqr) of Gaussian matrices,
-//              sigma from --spectrum (geometric kappa^(-i/(n-1)), clustered, one_small, identity)
+//   prescribed diag(sigma) rotated by ceil(log2 n) + 4 stages of random disjoint Givens pairs on the columns,
+//              transposed, then as many stages on the columns again (src/mp_core.cuh:36-58)
+//   randsvd    A = U diag(sigma) V^T, U and V the Q factors (rocSOLVER geqrf/orgqr) of Gaussian matrices
 //   dyadic     A = 2^-s C with integer C_ij in [-m, m]; with rhs=exact_sign, x* = +-1 and b = 2^-s C x* exactly
 //   rbf        exp(-|p_i - p_j|^2 / (2 ell^2)) + nugget*delta_ij, points uniform in [0,1]^dim
 //   grcar      -1 on the subdiagonal, 1 on the diagonal and k superdiagonals
 //   wilkinson  1 on the diagonal and last column, -1 below the diagonal
+// sigma from --spectrum (geometric kappa^(-i/(n-1)), clustered, one_small, identity).
 // RHS modes: random (Gaussian), random_scaled (column c scaled by 2^(4c-2 nrhs)), xtrue_uniform
 // (x_true ~ U(-1,1), b = A x_true), exact_sign (dyadic), unit_column (b = A e_j).
-// Random numbers are a counter-based stream: value k of stream s for seed q is splitmix-style mixing of
-// (q, s, k) -> 53-bit uniform; Gaussians by Box-Muller.  Streams do not depend on the launch shape.
+//
+// Random streams.  Everything the reference draws on the host (rotation stages, rbf points, row/column
+// scales, random right-hand sides, unit columns) is drawn here from the same xoshiro256** stream with the same
+// seeds and in the same order, and the dyadic entries use the same counter hash, so a case seed names the same
+// stored system in both builds (the transcendental functions come from the host C library).  The randsvd
+// family, which the reference does not have, uses a counter-based stream of its own: value k of stream s for
+// seed q is splitmix-style mixing of (q, s, k) -> 53-bit uniform; Gaussians by Box-Muller.
 #include "mp_internal.hpp"
 #include "mp_kernels.hpp"
 
-#include <random>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -36,6 +42,99 @@ __host__ __device__ inline double gauss(uint64_t seed, uint64_t stream, uint64_t
   return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
 
+// ---- the reference's streams (src/mp_common.cuh:34-43): splitmix64 steps and xoshiro256**
+__host__ __device__ inline uint64_t splitmix_step(uint64_t& x) {
+  x += 0x9E3779B97F4A7C15ull;
+  uint64_t z = x;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+// counter hash of three words: the second splitmix64 output from a mixed start
+__host__ __device__ inline uint64_t hash3(uint64_t a, uint64_t b, uint64_t c) {
+  uint64_t x = (a * 0x9E3779B97F4A7C15ull) ^ ((b + 0x7F4A7C15ull) * 0xBF58476D1CE4E5B9ull) ^ ((c + 0x1234567ull) * 0x94D049BB133111EBull);
+  splitmix_step(x);
+  return splitmix_step(x);
+}
+class HostStream {
+ public:
+  explicit HostStream(uint64_t seed) { uint64_t x = seed; for (auto& w : s_) w = splitmix_step(x); }
+  uint64_t next() {
+    const uint64_t out = rotl(s_[1] * 5, 7) * 9, t = s_[1] << 17;
+    s_[2] ^= s_[0]; s_[3] ^= s_[1]; s_[1] ^= s_[2]; s_[0] ^= s_[3]; s_[2] ^= t; s_[3] = rotl(s_[3], 45);
+    return out;
+  }
+  double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }          // [0, 1)
+  double normal() {                                                        // Box-Muller, first branch
+    double u1;
+    do { u1 = uniform(); } while (u1 <= 0.0);
+    const double u2 = uniform();
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+  }
+  int64_t below(int64_t n) { return (int64_t)(next() % (uint64_t)n); }
+ private:
+  static uint64_t rotl(uint64_t v, int k) { return (v << k) | (v >> (64 - k)); }
+  uint64_t s_[4];
+};
+
+// ---- prescribed family: stages of disjoint column rotations.  Pair p of a stage rotates columns (c1, c2) by
+// (c, s): a' = fma(c, a, s b), b' = fma(c, b, -(s a)), each product rounded once (src/mp_kernels.cuh:39-44).
+__global__ __launch_bounds__(256) void k_rotate_pairs(double* __restrict__ A, int64_t n, const int32_t* __restrict__ pairs,
+                                                      const double* __restrict__ cs) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  const int p = blockIdx.y;
+  const int64_t c1 = pairs[2 * p], c2 = pairs[2 * p + 1];
+  const double c = cs[2 * p], sn = cs[2 * p + 1];
+  const double a = A[c1 * n + r], b = A[c2 * n + r];
+  A[c1 * n + r] = fma(c, a, __dmul_rn(sn, b));
+  A[c2 * n + r] = fma(c, b, -__dmul_rn(sn, a));
+}
+// in-place transpose: workgroup (bi, bj), bj >= bi, swaps the 32 x 32 tiles (bi, bj) and (bj, bi) through LDS
+__global__ __launch_bounds__(256) void k_transpose_sq(double* __restrict__ A, int64_t n) {
+  const int bi = blockIdx.y, bj = blockIdx.x;
+  if (bj < bi) return;
+  __shared__ double tu[32][33], tl[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int64_t r0 = (int64_t)bi * 32, c0 = (int64_t)bj * 32;
+  for (int k = ty; k < 32; k += 8) {
+    if (r0 + tx < n && c0 + k < n) tu[k][tx] = A[(c0 + k) * n + r0 + tx];      // tile (bi, bj): tu[k][t] = A(r0+t, c0+k)
+    if (c0 + tx < n && r0 + k < n) tl[k][tx] = A[(r0 + k) * n + c0 + tx];      // tile (bj, bi): tl[k][t] = A(c0+t, r0+k)
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    if (r0 + tx < n && c0 + k < n) A[(c0 + k) * n + r0 + tx] = tl[tx][k];      // A(r0+tx, c0+k) <- A(c0+k, r0+tx)
+    if (bi != bj && c0 + tx < n && r0 + k < n) A[(r0 + k) * n + c0 + tx] = tu[tx][k];
+  }
+}
+static void rotation_stages(double* A, int64_t n, uint64_t seed, int side, int stages, Workspace& W, DevTracker& tr) {
+  HostStream rng(seed * 1000003ull + 17ull * (uint64_t)side + 0xC0FFEEull);
+  const int64_t np = n / 2;
+  if (np == 0) return;
+  std::vector<int32_t> perm(n), pairs((size_t)2 * np * stages);
+  std::vector<double> cs((size_t)2 * np * stages);
+  for (int st = 0; st < stages; st++) {
+    for (int64_t i = 0; i < n; i++) perm[i] = (int32_t)i;
+    for (int64_t i = n - 1; i > 0; i--) std::swap(perm[i], perm[rng.below(i + 1)]);
+    int32_t* pp = pairs.data() + (size_t)2 * np * st; double* cp = cs.data() + (size_t)2 * np * st;
+    for (int64_t p = 0; p < np; p++) {
+      pp[2 * p] = perm[2 * p]; pp[2 * p + 1] = perm[2 * p + 1];
+      const double th = rng.uniform() * 6.283185307179586;
+      cp[2 * p] = std::cos(th); cp[2 * p + 1] = std::sin(th);
+    }
+  }
+  DBuf<int32_t> dp; DBuf<double> dc;
+  dp.alloc(tr, pairs.size(), "gen_pairs"); dc.alloc(tr, cs.size(), "gen_cs");
+  HIP_CK(hipMemcpyAsync(dp.p, pairs.data(), pairs.size() * 4, hipMemcpyHostToDevice, W.stream));
+  HIP_CK(hipMemcpyAsync(dc.p, cs.data(), cs.size() * 8, hipMemcpyHostToDevice, W.stream));
+  for (int st = 0; st < stages; st++) {
+    hipLaunchKernelGGL(k_rotate_pairs, dim3(cdiv(n, 256), (unsigned)np), dim3(256), 0, W.stream, A, n,
+                       dp.p + (size_t)2 * np * st, dc.p + (size_t)2 * np * st);
+    LAUNCH_CK();
+  }
+  HIP_CK(hipStreamSynchronize(W.stream));
+}
+
 __global__ void k_gauss_fill(double* A, int64_t count, uint64_t seed, uint64_t stream) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i < count) A[i] = gauss(seed, stream, (uint64_t)i);
@@ -48,10 +147,11 @@ __global__ void k_scale_cols(double* A, int64_t n, const double* s) {
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x, c = blockIdx.y;
   if (r < n) A[c * n + r] *= s[c];
 }
+// dyadic entries and signs from the counter hash (src/mp_kernels.cuh:48-49)
 __device__ inline int dyadic_int(uint64_t seed, int64_t i, int64_t j, int m) {
-  return (int)(mix64(mix64(seed ^ 0xA24BAED4963EE407ull) + (uint64_t)i * 0x9FB21C651E98DF25ull + (uint64_t)j) % (uint64_t)(2 * m + 1)) - m;
+  return (int)(hash3(seed, (uint64_t)i, (uint64_t)j) % (uint64_t)(2 * m + 1)) - m;
 }
-__device__ inline int dyadic_sign(uint64_t seed, int64_t j) { return (mix64(seed * 31 + (uint64_t)j) & 1ull) ? 1 : -1; }
+__device__ inline int dyadic_sign(uint64_t seed, int64_t j) { return (hash3(seed ^ 0xA5A5A5A5ull, (uint64_t)j, 7ull) & 1ull) ? 1 : -1; }
 __global__ void k_dyadic_A(double* A, int64_t n, uint64_t seed, int m, double scale) {
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x, c = blockIdx.y;
   if (r < n) A[c * n + r] = (double)dyadic_int(seed, r, c, m) * scale;
@@ -116,7 +216,20 @@ bool generate(const GenSpec& g, void* rb_handle, Workspace& W, DevTracker& tr, d
   HIP_CK(hipStreamSynchronize(W.stream));
   if (rocblas_set_stream(rb, W.stream) != rocblas_status_success) throw MPError("rocblas_set_stream failed", -2);
   auto rbck = [](rocblas_status s, const char* w) { if (s != rocblas_status_success) throw MPError(std::string("rocBLAS/rocSOLVER failure in ") + w, -2); };
-  if (g.family == "randsvd") {
+  if (g.family == "prescribed") {
+    std::vector<double> hs(n);
+    for (int64_t i = 0; i < n; i++) hs[i] = spectrum_value(g.spectrum, i, n, g.kappa);
+    DBuf<double> sig; sig.alloc(tr, n, "gen_sigma");
+    HIP_CK(hipMemsetAsync(A, 0, nn * 8, W.stream));
+    HIP_CK(hipMemcpyAsync(sig.p, hs.data(), n * 8, hipMemcpyHostToDevice, W.stream));
+    HIP_CK(hipMemcpy2DAsync(A, (size_t)(n + 1) * 8, sig.p, 8, 8, (size_t)n, hipMemcpyDeviceToDevice, W.stream));
+    const int stages = (int)std::ceil(std::log2((double)n)) + 4;
+    rotation_stages(A, n, g.seed, 1, stages, W, tr);
+    const unsigned nt = (unsigned)cdiv(n, 32);
+    hipLaunchKernelGGL(k_transpose_sq, dim3(nt, nt), dim3(256), 0, W.stream, A, n);
+    LAUNCH_CK();
+    rotation_stages(A, n, g.seed, 2, stages, W, tr);
+  } else if (g.family == "randsvd") {
     DBuf<double> U, V, tau, sig;
     U.alloc(tr, nn, "gen_U"); V.alloc(tr, nn, "gen_V"); tau.alloc(tr, n, "gen_tau"); sig.alloc(tr, n, "gen_sigma");
     hipLaunchKernelGGL(k_gauss_fill, dim3(cdiv(nn, 256)), dim3(256), 0, W.stream, U.p, (int64_t)nn, g.seed, (uint64_t)1);
@@ -137,25 +250,26 @@ bool generate(const GenSpec& g, void* rb_handle, Workspace& W, DevTracker& tr, d
     hipLaunchKernelGGL(k_dyadic_A, dim3(cdiv(n, 256), (unsigned)n), dim3(256), 0, W.stream, A, n, g.seed, g.intmax, std::ldexp(1.0, -g.dyadic_exp));
   } else if (g.family == "rbf") {
     DBuf<double> pts; pts.alloc(tr, (size_t)n * g.dim, "gen_pts");
-    hipLaunchKernelGGL(k_unif_fill, dim3(cdiv((size_t)n * g.dim, 256)), dim3(256), 0, W.stream, pts.p, (int64_t)n * g.dim, g.seed, (uint64_t)7, 0.0, 1.0);
+    HostStream rng(g.seed * 7919ull + 0xB1ull);
+    std::vector<double> hp((size_t)n * g.dim);
+    for (auto& v : hp) v = rng.uniform();
+    HIP_CK(hipMemcpyAsync(pts.p, hp.data(), hp.size() * 8, hipMemcpyHostToDevice, W.stream));
     hipLaunchKernelGGL(k_rbf, dim3(cdiv(n, 256), (unsigned)n), dim3(256), 0, W.stream, A, n, pts.p, g.dim, g.ell, g.nugget);
     HIP_CK(hipStreamSynchronize(W.stream));
   } else if (g.family == "grcar" || g.family == "wilkinson") {
     hipLaunchKernelGGL(k_band, dim3(cdiv(n, 256), (unsigned)n), dim3(256), 0, W.stream, A, n, g.family == "grcar" ? 0 : 1, g.grcar_k);
   } else {
-    throw MPError("unknown or unsupported family " + g.family + " (supported: randsvd, dyadic, rbf, grcar, wilkinson)", -4);
+    throw MPError("unknown or unsupported family " + g.family + " (supported: prescribed, randsvd, dyadic, rbf, grcar, wilkinson)", -4);
   }
   LAUNCH_CK();
   if (g.rowcol_span > 0) {
     // the reference's --rowcol-span (src/mp_core.cuh:71-72): A <- D_r A D_c with row and column factors
-    // 2^(u - span/2), u uniform in [0, span]; applied before the right-hand side is formed.  The stream is
-    // this library's own (seeded from the case seed).
+    // 2^(u - span/2), u uniform in [0, span]; applied before the right-hand side is formed.
     if (g.rhs == "exact_sign") throw MPError("exact_sign with scaling not supported", -1);
-    std::mt19937_64 rng(g.seed * 31ull + 5ull);
-    std::uniform_int_distribution<int> u(0, g.rowcol_span);
+    HostStream rng(g.seed * 31ull + 5ull);
     std::vector<double> r(n), c(n);
-    for (int64_t i = 0; i < n; i++) r[i] = std::ldexp(1.0, u(rng) - g.rowcol_span / 2);
-    for (int64_t i = 0; i < n; i++) c[i] = std::ldexp(1.0, u(rng) - g.rowcol_span / 2);
+    for (int64_t i = 0; i < n; i++) r[i] = std::ldexp(1.0, (int)rng.below(g.rowcol_span + 1) - g.rowcol_span / 2);
+    for (int64_t i = 0; i < n; i++) c[i] = std::ldexp(1.0, (int)rng.below(g.rowcol_span + 1) - g.rowcol_span / 2);
     DBuf<double> dr, dc; dr.alloc(tr, n, "gen_dr"); dc.alloc(tr, n, "gen_dc");
     HIP_CK(hipMemcpyAsync(dr.p, r.data(), n * 8, hipMemcpyHostToDevice, W.stream));
     HIP_CK(hipMemcpyAsync(dc.p, c.data(), n * 8, hipMemcpyHostToDevice, W.stream));
@@ -165,15 +279,16 @@ bool generate(const GenSpec& g, void* rb_handle, Workspace& W, DevTracker& tr, d
   }
   const size_t nb = (size_t)n * nrhs;
   if (g.rhs == "random" || g.rhs == "random_scaled") {
-    hipLaunchKernelGGL(k_gauss_fill, dim3(cdiv(nb, 256)), dim3(256), 0, W.stream, B, (int64_t)nb, g.seed, (uint64_t)3 + (uint64_t)nrhs);
-    if (g.rhs == "random_scaled") {
-      std::vector<double> s(nrhs);
-      for (int c = 0; c < nrhs; c++) s[c] = std::ldexp(1.0, 4 * c - 2 * nrhs);
-      DBuf<double> ds; ds.alloc(tr, nrhs, "gen_cs");
-      HIP_CK(hipMemcpyAsync(ds.p, s.data(), nrhs * 8, hipMemcpyHostToDevice, W.stream));
-      hipLaunchKernelGGL(k_scale_cols, dim3(cdiv(n, 256), (unsigned)nrhs), dim3(256), 0, W.stream, B, n, ds.p);
-      HIP_CK(hipStreamSynchronize(W.stream));
-    }
+    // Gaussian B in column-major order from one stream; random_scaled scales column c by 2^(4c - 2 nrhs)
+    HostStream rng(g.seed * 104729ull + 0xBEEFull + (uint64_t)nrhs);
+    std::vector<double> hb(nb);
+    for (int c = 0; c < nrhs; c++)
+      for (int64_t i = 0; i < n; i++) {
+        const double v = rng.normal();
+        hb[i + (size_t)c * n] = (g.rhs == "random_scaled") ? std::ldexp(v, 4 * c - 2 * nrhs) : v;
+      }
+    HIP_CK(hipMemcpyAsync(B, hb.data(), nb * 8, hipMemcpyHostToDevice, W.stream));
+    HIP_CK(hipStreamSynchronize(W.stream));
   } else if (g.rhs == "xtrue_uniform") {
     hipLaunchKernelGGL(k_unif_fill, dim3(cdiv(nb, 256)), dim3(256), 0, W.stream, Xe, (int64_t)nb, g.seed, (uint64_t)4, -1.0, 1.0);
     const double one = 1.0, zero = 0.0;
@@ -187,8 +302,9 @@ bool generate(const GenSpec& g, void* rb_handle, Workspace& W, DevTracker& tr, d
     has_exact = true;
   } else if (g.rhs == "unit_column") {
     HIP_CK(hipMemsetAsync(Xe, 0, nb * 8, W.stream));
+    HostStream rng(g.seed * 17ull + 3ull);
     for (int c = 0; c < nrhs; c++) {
-      const int64_t j = (int64_t)(unif(g.seed, 5, (uint64_t)c) * (double)n) % n;
+      const int64_t j = rng.below(n);
       HIP_CK(hipMemcpyAsync(B + (size_t)c * n, A + (size_t)j * n, n * 8, hipMemcpyDeviceToDevice, W.stream));
       const double one = 1.0;
       HIP_CK(hipMemcpyAsync(Xe + (size_t)c * n + j, &one, 8, hipMemcpyHostToDevice, W.stream));

@@ -6,6 +6,11 @@ loops:
   metrics         eval_metrics_T        src/mp_core.cuh:193-205 (eta, omega, relres, worst row)
   ir_stationary   Engine::refine_stage  src/mp_core.cuh:281-296 (stopping tests and terminal reasons)
   fgmres          Engine::gmres_solve   src/mp_core.cuh:237-278 (right-preconditioned flexible GMRES)
+  factor / apply_inverse   factorize + apply_inverse, src/mp_core.cuh:158-190 (scale modes 0, 1, 2)
+  Xoshiro, hash3, prescribed, rhs_random, rowcol_factors, unit_columns, dyadic
+                  the reference's generator streams and families, src/mp_common.cuh:34-43,
+                  src/mp_core.cuh:29-103, src/mp_kernels.cuh:39-49 (pinned by published splitmix64 /
+                  xoshiro256** outputs, the prescribed spectrum and exact dyadic products)
 """
 from __future__ import annotations
 
@@ -295,3 +300,151 @@ def fgmres_ir(A, B, fmt="fp16", nb=128, tau=1e-15, max_outer=30, m=30, tol=1e-4,
         inner += its
         x = x + d
     return x, "max_outer", hist, inner
+
+
+# ------------------------------------------------------------------ generator streams (src/mp_common.cuh:34-43)
+# Python integers and exact rational arithmetic restate the reference's host streams and the prescribed family
+# (src/mp_core.cuh:29-58, src/mp_kernels.cuh:39-49) for small n.
+_M64 = (1 << 64) - 1
+
+
+def splitmix_step(state):
+    """One splitmix64 step: returns (new_state, output)."""
+    state = (state + 0x9E3779B97F4A7C15) & _M64
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return state, z ^ (z >> 31)
+
+
+def hash3(a, b, c):
+    x = ((a * 0x9E3779B97F4A7C15) & _M64) ^ (((b + 0x7F4A7C15) * 0xBF58476D1CE4E5B9) & _M64) \
+        ^ (((c + 0x1234567) * 0x94D049BB133111EB) & _M64)
+    x, _ = splitmix_step(x)
+    _, out = splitmix_step(x)
+    return out
+
+
+class Xoshiro:
+    """xoshiro256** with its state filled by four splitmix64 steps from the seed."""
+
+    def __init__(self, seed=None, state=None):
+        if state is None:
+            x, state = seed & _M64, []
+            for _ in range(4):
+                x, v = splitmix_step(x)
+                state.append(v)
+        self.s = list(state)
+
+    @staticmethod
+    def _rotl(v, k):
+        return ((v << k) | (v >> (64 - k))) & _M64
+
+    def next(self):
+        s = self.s
+        out = (self._rotl((s[1] * 5) & _M64, 7) * 9) & _M64
+        t = (s[1] << 17) & _M64
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t; s[3] = self._rotl(s[3], 45)
+        return out
+
+    def uniform(self):
+        return (self.next() >> 11) * 2.0 ** -53
+
+    def normal(self):
+        import math
+        u1 = self.uniform()
+        while u1 <= 0:
+            u1 = self.uniform()
+        u2 = self.uniform()
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(6.283185307179586 * u2)
+
+    def below(self, n):
+        return self.next() % n
+
+
+def fma(x, y, z):
+    """Correctly rounded x*y + z (exact rational sum, one rounding), with IEEE signed zeros."""
+    import math
+    r = Fraction(x) * Fraction(y) + Fraction(z)
+    if r != 0:
+        return float(r)
+    if (x == 0 or y == 0) and z == 0 and math.copysign(1, x) * math.copysign(1, y) < 0 and math.copysign(1, z) < 0:
+        return -0.0
+    return 0.0
+
+
+def spectrum(sp, n, kappa):
+    if sp == "geometric":
+        return [1.0 if n == 1 else kappa ** (-i / (n - 1)) for i in range(n)]
+    if sp == "clustered":
+        k = max(2, n // 64)
+        return [1.0 / kappa if i >= n - k else 1.0 for i in range(n)]
+    if sp == "one_small":
+        return [1.0 / kappa if i == n - 1 else 1.0 for i in range(n)]
+    if sp == "identity":
+        return [1.0] * n
+    raise ValueError(sp)
+
+
+def _rotation_stages(A, n, seed, side, stages):
+    import math
+    rng = Xoshiro((seed * 1000003 + 17 * side + 0xC0FFEE) & _M64)
+    for _ in range(stages):
+        perm = list(range(n))
+        for i in range(n - 1, 0, -1):
+            j = rng.below(i + 1)
+            perm[i], perm[j] = perm[j], perm[i]
+        for p in range(n // 2):
+            c1, c2 = perm[2 * p], perm[2 * p + 1]
+            th = rng.uniform() * 6.283185307179586
+            c, s = math.cos(th), math.sin(th)
+            for r in range(n):
+                a, b = A[r][c1], A[r][c2]
+                A[r][c1] = fma(c, a, s * b)
+                A[r][c2] = fma(c, b, -(s * a))
+
+
+def prescribed(n, kappa, seed, sp="geometric"):
+    """The prescribed family: diag(sigma), rotation stages on the columns, transpose, stages again."""
+    import math
+    sig = spectrum(sp, n, kappa)
+    A = [[sig[i] if i == j else 0.0 for j in range(n)] for i in range(n)]
+    stages = int(math.ceil(math.log2(n))) + 4
+    _rotation_stages(A, n, seed, 1, stages)
+    A = [list(col) for col in zip(*A)]
+    _rotation_stages(A, n, seed, 2, stages)
+    return np.asfortranarray(np.array(A, dtype=np.float64))
+
+
+def rhs_random(n, nrhs, seed, scaled=False):
+    rng = Xoshiro((seed * 104729 + 0xBEEF + nrhs) & _M64)
+    B = np.empty((n, nrhs), order="F")
+    for c in range(nrhs):
+        for i in range(n):
+            v = rng.normal()
+            B[i, c] = np.ldexp(v, 4 * c - 2 * nrhs) if scaled else v
+    return B
+
+
+def rowcol_factors(n, span, seed):
+    rng = Xoshiro((seed * 31 + 5) & _M64)
+    r = [np.ldexp(1.0, int(rng.below(span + 1)) - span // 2) for _ in range(n)]
+    c = [np.ldexp(1.0, int(rng.below(span + 1)) - span // 2) for _ in range(n)]
+    return np.array(r), np.array(c)
+
+
+def unit_columns(n, nrhs, seed):
+    rng = Xoshiro((seed * 17 + 3) & _M64)
+    return [int(rng.below(n)) for _ in range(nrhs)]
+
+
+def dyadic(n, seed, m, s, nrhs=1):
+    """A = 2^-s C, C_ij = hash3(seed, i, j) mod (2m+1) - m; x*_j = +-1 from hash3(seed' ^ 0xA5A5A5A5, j, 7),
+    seed' = 3 seed + 11 + c; b = 2^-s C x* (exact)."""
+    C = np.array([[int(hash3(seed, i, j) % (2 * m + 1)) - m for j in range(n)] for i in range(n)], dtype=np.int64)
+    X = np.empty((n, nrhs))
+    for c in range(nrhs):
+        rs = (seed * 3 + 11 + c) & _M64
+        X[:, c] = [1.0 if hash3(rs ^ 0xA5A5A5A5, j, 7) & 1 else -1.0 for j in range(n)]
+    B = (C @ X.astype(np.int64)).astype(np.float64) * 2.0 ** -s
+    return np.asfortranarray(C.astype(np.float64) * 2.0 ** -s), np.asfortranarray(B), X

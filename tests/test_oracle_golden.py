@@ -283,3 +283,45 @@ def test_fgmres_ir_converges_and_fp8_stagnates(oracle):
     assert st == "converged" and hist[-1] <= 1e-15
     x, st, hist = oracle.ir_stationary(A, b, "fp8", 64, tau=1e-15, dd=True, max_outer=10)
     assert st in ("stagnation", "max_outer")
+
+
+# ------------------------------------------------------------------ generator streams
+def test_generator_streams_known_answers(oracle):
+    """splitmix64 from state 0 and xoshiro256** from state (1, 2, 3, 4): the published first outputs."""
+    x, outs = 0, []
+    for _ in range(3):
+        x, v = oracle.splitmix_step(x)
+        outs.append(v)
+    assert outs == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    r = oracle.Xoshiro(state=[1, 2, 3, 4])
+    assert [r.next() for _ in range(4)] == [11520, 0, 1509978240, 1215971899390074240]
+    u = oracle.Xoshiro(seed=9200)
+    vals = [u.uniform() for _ in range(1000)]
+    assert min(vals) >= 0.0 and max(vals) < 1.0 and abs(np.mean(vals) - 0.5) < 0.05
+
+
+def test_oracle_fma_is_correctly_rounded(oracle):
+    e = 2.0 ** -30
+    assert oracle.fma(1 + e, 1 - e, -1.0) == -(e * e)            # the product's low part survives one rounding
+    assert (1 + e) * (1 - e) - 1.0 == 0.0                          # and is lost with two
+    assert np.signbit(oracle.fma(-1.0, 0.0, -0.0)) and not np.signbit(oracle.fma(1.0, 0.0, -0.0))
+
+
+@pytest.mark.parametrize("n,sp", [(16, "geometric"), (33, "identity"), (48, "clustered")])
+def test_prescribed_family_spectrum(oracle, n, sp):
+    """Rotation stages are orthogonal: the singular values are the prescribed ones; the identity spectrum
+    gives an orthogonal matrix that is dense after the stages."""
+    A = oracle.prescribed(n, 1e5, 9200, sp)
+    s = np.linalg.svd(A, compute_uv=False)
+    sig = np.sort(oracle.spectrum(sp, n, 1e5))[::-1]
+    assert np.abs(s - sig).max() < 10 * n * 2.0 ** -53
+    if sp == "identity":
+        assert np.abs(A.T @ A - np.eye(n)).max() < 10 * n * 2.0 ** -53
+        assert np.count_nonzero(A) > 0.9 * n * n
+    assert np.array_equal(A, oracle.prescribed(n, 1e5, 9200, sp)) and not np.array_equal(A, oracle.prescribed(n, 1e5, 9201, sp))
+
+
+def test_dyadic_rhs_is_exact(oracle):
+    A, B, X = oracle.dyadic(30, 5, 255, 8, nrhs=2)
+    assert np.all(np.abs(X) == 1.0) and np.all(np.abs(A * 256 - np.round(A * 256)) == 0) and np.abs(A * 256).max() <= 255
+    np.testing.assert_array_equal(A @ X, B)                        # every sum is exact in FP64

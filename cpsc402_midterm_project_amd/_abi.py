@@ -24,7 +24,7 @@ EXPORTS = [
     "mp_create", "mp_destroy", "mp_last_error", "mp_caps", "mp_synchronize", "mp_profile", "mp_profile_json", "mp_malloc", "mp_free",
     "mp_memcpy_h2d", "mp_memcpy_d2h", "mp_memcpy_d2d", "mp_residual_fp64", "mp_residual_dd", "mp_getrf",
     "mp_getrs", "mp_problem_from_host", "mp_problem_generate", "mp_problem_reference", "mp_problem_free", "mp_problem_copy_out", "mp_problem_copy_ref", "mp_problem_case_json", "mp_solve",
-    "mp_last_record_json", "mp_last_hist_json", "mp_methods_json", "mp_eval_metrics_json", "mp_batch_solve",
+    "mp_last_record_json", "mp_last_hist_json", "mp_methods_json", "mp_eval_metrics_json", "mp_audit_json", "mp_sha256_hex", "mp_batch_solve",
     "mp_batch_solve_ptrs", "mp_generate_into", "mp_problem_from_device", "mp_problem_set_rhs", "mp_server_clock",
     "mp_server_create", "mp_server_submit", "mp_server_poll", "mp_server_stats_json", "mp_server_destroy", "mp_getrf64",
     "mp_factor_cache_put", "mp_factor_cache_get", "mp_factor_cache_evict",
@@ -99,6 +99,8 @@ def _load() -> C.CDLL:
         "mp_last_hist_json": ([vp, C.c_char_p, sz], C.c_int),
         "mp_methods_json": ([C.c_char_p, sz], C.c_int),
         "mp_eval_metrics_json": ([vp, vp, dp, C.c_char_p, sz], C.c_int),
+        "mp_audit_json": ([vp, vp, dp, dp, dp, C.c_char_p, sz], C.c_int),
+        "mp_sha256_hex": ([vp, sz, C.c_char_p], C.c_int),
         "mp_batch_solve": ([vp, i32, vp, vp, dp, vp, dp, vp, dp, C.c_char_p, vp], C.c_int),
         "mp_batch_solve_ptrs": ([vp, i32, vp, vp, vp, vp, vp, C.c_char_p, vp], C.c_int),
         "mp_generate_into": ([vp, C.c_char_p, dp, dp, dp, vp], C.c_int),
@@ -194,6 +196,12 @@ class Context:
         h = C.c_void_p()
         check(lib.mp_create(C.byref(h), device))
         self.h = h
+
+    def audit(self, X: np.ndarray, x_ref: np.ndarray | None = None, x_exact: np.ndarray | None = None) -> dict:
+        """FP64 and double-double evaluation of a solution with forward errors (mp_audit_json)."""
+        f = lambda a: None if a is None else np.asfortranarray(a, dtype=np.float64).reshape(self.n, self.nrhs, order="F")
+        X, R, E = f(X), f(x_ref), f(x_exact)
+        return json.loads(_json_call(lib.mp_audit_json, self.ctx.h, self.h, _ptr(X), _ptr(R), _ptr(E)))
 
     def close(self) -> None:
         if self.h:

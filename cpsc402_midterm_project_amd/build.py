@@ -1,7 +1,7 @@
 """Build libmpsolve.so (HIP, gfx950) and the mpsolve CLI in-tree with hipcc.
 
 Usage: python -m cpsc402_midterm_project_amd.build [--jobs N]
-Outputs cpsc402_midterm_project_amd/libmpsolve.so and cpsc402_midterm_project_amd/mpsolve.
+Outputs cpsc402_midterm_project_amd/libmpsolve.so and the cpsc402_midterm_project_amd/mpsolve and mpaudit drivers.
 """
 from __future__ import annotations
 
@@ -70,6 +70,12 @@ def build(jobs: int = 8, verbose: bool = False) -> Path:
         p = subprocess.run(cmd, capture_output=True, text=True)
         if p.returncode != 0:
             raise RuntimeError("CLI build failed:\n" + p.stderr)
+        # mpaudit: the same driver with options only (the reference's separate audit executable)
+        cmd = [HIPCC, "-O2", "-std=c++17", "-DMPSOLVE_AUDIT_MAIN", "-I", str(ROOT / "include"), str(cli_src),
+               "-o", str(PKG / "mpaudit"), "-L", str(PKG), "-lmpsolve", "-Wl,-rpath,$ORIGIN"]
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0:
+            raise RuntimeError("mpaudit build failed:\n" + p.stderr)
     return lib
 
 

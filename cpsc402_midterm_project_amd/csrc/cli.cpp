@@ -10,11 +10,13 @@
 #include "mpsolve.h"
 
 #include <sys/stat.h>
+#include <unistd.h>
 
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <ctime>
 #include <fstream>
 #include <map>
@@ -36,10 +38,12 @@ struct Args {
   bool has(const std::string& k) const { return kv.count(k) > 0; }
 };
 
-Args parse_args(int argc, char** argv) {
+// mpsolve: argv[1] is the command.  mpaudit (src/mpaudit.cu:9) takes options only: first = 1, cmd "audit".
+Args parse_args(int argc, char** argv, int first = 2) {
   Args a;
-  if (argc > 1) a.cmd = argv[1];
-  for (int i = 2; i < argc; i++) {
+  if (first == 2 && argc > 1) a.cmd = argv[1];
+  if (first == 1) a.cmd = "audit";
+  for (int i = first; i < argc; i++) {
     const std::string s = argv[i];
     if (s.rfind("--", 0) != 0) continue;
     const std::string k = s.substr(2);
@@ -145,6 +149,77 @@ std::string problem_json(mp_problem* P) {
   std::vector<char> buf(1 << 20);
   check(mp_problem_case_json(P, buf.data(), buf.size()), "mp_problem_case_json");
   return std::string(buf.data());
+}
+
+// Raw text of a top-level field of a flat JSON object: a string with its quotes, a number, a literal or a
+// flat array; "null" when absent.
+std::string json_raw(const std::string& j, const std::string& key) {
+  const std::string pat = "\"" + key + "\":";
+  size_t p = j.find(pat);
+  if (p == std::string::npos) return "null";
+  p += pat.size();
+  size_t e = p;
+  if (j[p] == '"') { e = p + 1; while (e < j.size() && (j[e] != '"' || j[e - 1] == '\\')) e++; e++; }
+  else if (j[p] == '[') { e = j.find(']', p); e = (e == std::string::npos) ? j.size() : e + 1; }
+  else { while (e < j.size() && j[e] != ',' && j[e] != '}') e++; }
+  return j.substr(p, e - p);
+}
+
+bool read_bin(const std::string& path, void* data, size_t bytes) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return false;
+  f.read((char*)data, (std::streamsize)bytes);
+  return (bool)f;
+}
+
+// mpaudit (src/mpaudit.cu:8-39): the stored problem regenerated in this fresh process with its hashes, an
+// optional dump of A, then one record per saved solution of --x-list ("<path> <float32|float64>" lines) with
+// FP64 and double-double metrics and forward errors against --xref / --xexact.  Never runs a solver.
+int cmd_audit(Session& S, const Args& a, long long n, long long nrhs) {
+  const std::string cj = problem_json(S.P);
+  Obj head;
+  head.s("case_id", a.get("case-id", "case")).raw("regenerated_a_sha256", json_raw(cj, "a_sha256"))
+      .raw("regenerated_b_sha256", json_raw(cj, "b_sha256")).raw("regenerated_tile_sha256", json_raw(cj, "tile_sha256"))
+      .raw("gen_seconds", json_raw(cj, "gen_seconds")).raw("hash_seconds", json_raw(cj, "hash_seconds"))
+      .raw("anorm_inf", json_raw(cj, "anorm_inf")).s("timestamp", iso_now()).s("audit_process_pid", std::to_string((long)getpid()));
+  std::printf("%s\n", head.str().c_str());
+  std::fflush(stdout);
+  const bool f32 = json_raw(cj, "dtype") == "\"fp32\"";
+  const size_t nb = (size_t)n * nrhs;
+  if (a.has("dump-a")) {
+    std::vector<double> ha((size_t)n * n);
+    check(mp_problem_copy_out(S.C, S.P, ha.data(), nullptr, nullptr), "mp_problem_copy_out");
+    write_bin(a.get("dump-a"), ha.data(), ha.size() * (f32 ? 4 : 8));
+  }
+  std::vector<double> xr, xe;
+  if (a.has("xref")) { xr.resize(nb); if (!read_bin(a.get("xref"), xr.data(), nb * 8)) xr.clear(); }
+  if (a.has("xexact")) { xe.resize(nb); if (!read_bin(a.get("xexact"), xe.data(), nb * 8)) xe.clear(); }
+  std::ifstream lst(a.get("x-list"));
+  std::string path, dt;
+  std::vector<char> buf(1 << 16);
+  while (lst >> path >> dt) {
+    const bool x32 = dt == "float32";
+    std::vector<char> raw(nb * (x32 ? 4 : 8));
+    std::ifstream f(path, std::ios::binary);
+    if (!f) { std::printf("%s\n", Obj().s("x_path", path).s("error", "cannot open").str().c_str()); continue; }
+    f.read(raw.data(), (std::streamsize)raw.size());
+    if (!f) { std::printf("%s\n", Obj().s("x_path", path).s("error", "short read").str().c_str()); continue; }
+    char hex[65];
+    check(mp_sha256_hex(raw.data(), raw.size(), hex), "mp_sha256_hex");
+    std::vector<double> X(nb);
+    if (x32) { const float* p = (const float*)raw.data(); for (size_t i = 0; i < nb; i++) X[i] = p[i]; }
+    else std::memcpy(X.data(), raw.data(), nb * 8);
+    for (;;) {
+      const int st = mp_audit_json(S.C, S.P, X.data(), xr.empty() ? nullptr : xr.data(), xe.empty() ? nullptr : xe.data(),
+                                   buf.data(), buf.size());
+      if (st == MP_OK) break;
+      if (std::string(mp_last_error()).find("buffer too small") == std::string::npos || buf.size() > ((size_t)1 << 30)) check(st, "mp_audit_json");
+      buf.resize(buf.size() * 8);
+    }
+    std::printf("%s\n", merge(Obj().s("x_path", path).s("x_sha256", hex).s("x_dtype", dt).str(), std::string(buf.data())).c_str());
+    std::fflush(stdout);
+  }
+  return 0;
 }
 
 int cmd_regen(Session& S) {
@@ -295,8 +370,12 @@ int cmd_run(Session& S, const Args& a, const std::string& case_id, long long n, 
 }  // namespace
 
 int main(int argc, char** argv) {
+#ifdef MPSOLVE_AUDIT_MAIN
+  const Args a = parse_args(argc, argv, 1);
+#else
   const Args a = parse_args(argc, argv);
-  if (a.cmd.empty()) { std::fprintf(stderr, "usage: mpsolve run|regen|caps|selftest --key value ...\n"); return 2; }
+#endif
+  if (a.cmd.empty()) { std::fprintf(stderr, "usage: mpsolve run|regen|caps|selftest|audit --key value ...\n"); return 2; }
   try {
     Session S;
     check(mp_create(&S.C, (int)a.geti("device", 0)), "mp_create");
@@ -307,6 +386,7 @@ int main(int argc, char** argv) {
     const long long n = a.geti("n", 1024), nrhs = a.geti("nrhs", 1);
     if (a.cmd == "regen") return cmd_regen(S);
     if (a.cmd == "selftest") return cmd_selftest(S, n, a.get("dtype", "fp64") == "fp32");
+    if (a.cmd == "audit") return cmd_audit(S, a, n, nrhs);
     return cmd_run(S, a, a.get("case-id", "case"), n, nrhs);   // any other command runs trials, as the reference does
   } catch (std::exception& e) {
     std::fprintf(stderr, "FATAL: %s\n", e.what());

@@ -1260,7 +1260,9 @@ int mp_problem_generate(mp_ctx* C, const char* case_json, mp_problem** out) {
     }
     const double th = wall_now();
     finish_setup(C, P.get(), ha.data(), hb);
-    Json j; j.d("gen_seconds", tg).d("hash_seconds", wall_now() - th).s("generator", "device: counter-based Gaussian/uniform streams, rocSOLVER geqrf/orgqr, rocBLAS dgemm");
+    Json j; j.d("gen_seconds", tg).d("hash_seconds", wall_now() - th)
+             .s("generator", g.family == "randsvd" ? "device: counter-based Gaussian/uniform streams, rocSOLVER geqrf/orgqr, rocBLAS dgemm"
+                                                   : "device kernels over the reference's xoshiro256** host streams and counter hash");
     std::string cj = case_json; while (!cj.empty() && isspace((unsigned char)cj.back())) cj.pop_back();
     std::string jj = j.obj();
     P->case_json = (cj.size() > 2) ? cj.substr(0, cj.size() - 1) + "," + jj.substr(1) : jj;
@@ -1411,6 +1413,47 @@ int mp_eval_metrics_json(mp_ctx* C, mp_problem* P, const double* X_host, char* j
     j.d("eta_max", M.eta_max).d("omega_max", M.omega_max).kv("eta", Json::arr(M.eta)).kv("omega", Json::arr(M.omega))
      .kv("rinf", Json::arr(M.rinf)).kv("xinf", Json::arr(M.xinf)).b("metrics_finite", M.finite);
     copy_str(j.obj(), json, cap);
+  });
+}
+
+int mp_audit_json(mp_ctx* C, mp_problem* P, const double* X_host, const double* x_ref, const double* x_exact,
+                  char* json, size_t cap) {
+  return guard([&] {
+    MP_REQUIRE(C && P && X_host, "NULL argument");
+    MP_REQUIRE(P->ctx == C, "problem belongs to another context, or its context was destroyed");
+    const int64_t n = P->n; const int nrhs = P->nrhs; const size_t nb = (size_t)n * nrhs;
+    DBuf<double> X; X.alloc(C->tracker, nb, "audit_X");
+    HIP_CK(hipMemcpy(X.p, X_host, nb * 8, hipMemcpyHostToDevice));
+    EvalBufs E; Metrics M, Mdd;
+    residual_metrics(C, *P, X.p, false, E, M);
+    residual_metrics(C, *P, X.p, true, E, Mdd);      // double-double residual, rounded once to FP64
+    std::vector<double> eta_dd(nrhs);
+    for (int c = 0; c < nrhs; c++) {
+      const double den = P->anorm_inf * M.xinf[c] + P->bnorm_inf[c];
+      eta_dd[c] = (Mdd.rinf[c] == 0) ? 0.0 : (den == 0 ? INFINITY : Mdd.rinf[c] / den);
+    }
+    const std::vector<double> hx(X_host, X_host + nb);
+    std::vector<double> vr, ve;
+    if (x_ref) vr.assign(x_ref, x_ref + nb); else if (P->has_ref) vr = P->hXref;
+    if (x_exact) ve.assign(x_exact, x_exact + nb); else if (P->has_exact) ve = P->hXexact;
+    if (!vr.empty()) { fwd_stats(hx, vr, n, nrhs, M.finf, M.f2, M.fcomp, M.fzero_maxabs, M.fzero_count); M.has_fwd = true; }
+    if (!ve.empty()) { fwd_stats(hx, ve, n, nrhs, M.einf, M.e2, M.ecomp, M.ezero_maxabs, M.ezero_count); M.has_exact = true; }
+    Json j;
+    j.b("finite", M.finite).kv("eta_fp64", Json::arr(M.eta)).kv("omega_fp64", Json::arr(M.omega)).kv("rinf_fp64", Json::arr(M.rinf))
+     .kv("rinf_dd", Json::arr(Mdd.rinf)).kv("eta_dd", Json::arr(eta_dd)).kv("relres_inf", Json::arr(M.relres_inf))
+     .kv("relres_2", Json::arr(M.relres_2)).kv("xinf", Json::arr(M.xinf))
+     .b("has_ref", M.has_fwd).kv("finf_ref", Json::arr(M.finf)).kv("f2_ref", Json::arr(M.f2)).kv("fcomp_ref", Json::arr(M.fcomp))
+     .b("has_exact", M.has_exact).kv("finf_exact", Json::arr(M.einf)).kv("f2_exact", Json::arr(M.e2))
+     .kv("fcomp_exact", Json::arr(M.ecomp)).kv("ezero_count", Json::arr(M.ezero_count));
+    copy_str(j.obj(), json, cap);
+  });
+}
+
+int mp_sha256_hex(const void* data, size_t bytes, char* hex65) {
+  return guard([&] {
+    MP_REQUIRE(hex65 && (data || bytes == 0), "NULL argument");
+    const std::string h = sha256_hex(data, bytes);
+    std::memcpy(hex65, h.c_str(), 65);
   });
 }
 

@@ -229,6 +229,16 @@ int mp_server_destroy(mp_server* s);
  * eta/omega per RHS and their maxima, written as a JSON object. */
 int mp_eval_metrics_json(mp_ctx* ctx, mp_problem* p, const double* X_host, char* json, size_t cap);
 
+/* Independent evaluation of a saved solution (the per-solution record of mpaudit, src/mpaudit.cu:28-34):
+ * FP64 eta/omega/rinf, double-double rinf and eta_dd, relres, and forward errors against x_ref / x_exact
+ * (host n x nrhs; NULL: the problem's own reference / exact solution, if it has one). */
+int mp_audit_json(mp_ctx* ctx, mp_problem* p, const double* X_host, const double* x_ref, const double* x_exact,
+                  char* json, size_t cap);
+
+/* SHA-256 of host bytes as 64 hex digits plus NUL (the x_sha256 / a_sha256 of the records,
+ * src/mp_common.cuh:45-52). */
+int mp_sha256_hex(const void* data, size_t bytes, char* hex65);
+
 #ifdef __cplusplus
 }
 #endif

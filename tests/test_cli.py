@@ -26,7 +26,7 @@ def _run(*args, timeout=120):
 def test_cli_usage_without_command():
     r = _run()
     assert r.returncode == 2
-    assert "usage: mpsolve run|regen|caps|selftest" in r.stderr
+    assert "usage: mpsolve run|regen|caps|selftest|audit" in r.stderr
 
 
 def test_cli_caps_reports_device_or_fatal_json():
@@ -129,3 +129,47 @@ def test_cli_fp32_storage_and_rowcol_scaling(tmp_path):
     assert rm.max() / rm.min() > 4
     rec = json.loads((tmp_path / "trials_f.jsonl").read_text().splitlines()[0])
     assert rec["ok"] and rec["terminal_reason"] == "converged" and rec["eta_max"] <= 100 * 2.0 ** -24
+
+
+@pytest.mark.gpu
+def test_mpaudit_regenerates_and_reevaluates(tmp_path, oracle):
+    """mpaudit (src/mpaudit.cu:8-39) in a fresh process: the regenerated hashes equal the case record's, every
+    listed solution gets its SHA-256 and FP64 metrics equal to the trial record's, the double-double eta agrees
+    with the oracle's, forward errors come from --xexact, and unreadable entries are reported, not fatal."""
+    n, nrhs = 200, 2
+    case = ["--n", n, "--nrhs", nrhs, "--family", "prescribed", "--kappa", 1e4, "--rhs", "random", "--seed", 5]
+    print(f"[mpaudit] n={n}", flush=True)
+    r = _run("run", *case, "--methods", "mx16_ir_w64_r64,native_lu32", "--reps", 1, "--warmup", 0, "--case-id", "a1",
+             "--out", tmp_path, "--nb", 64, "--embed-a", "--save-x", 1)
+    assert r.returncode == 0, r.stderr + r.stdout
+    crec = json.loads((tmp_path / "case_a1.json").read_text())
+    trials = [json.loads(l) for l in (tmp_path / "trials_a1.jsonl").read_text().splitlines() if l.strip()]
+    sols = [t for t in trials if t.get("x_path")]
+    assert sols, "no saved solutions"
+    lst = tmp_path / "xlist.txt"
+    lst.write_text("".join(f"{tmp_path / t['x_path']} {t['x_dtype']}\n" for t in sols) + f"{tmp_path / 'missing.f64'} float64\n")
+    xe = np.linspace(-1, 1, n * nrhs)
+    (tmp_path / "xe.f64").write_bytes(xe.tobytes())
+    aud = ROOT / "cpsc402_midterm_project_amd" / "mpaudit"
+    p = subprocess.run([str(aud), "--case-id", "a1", "--x-list", str(lst), "--xexact", str(tmp_path / "xe.f64"),
+                        "--dump-a", str(tmp_path / "dump.f64"), *map(str, case)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr + p.stdout
+    lines = [json.loads(l) for l in p.stdout.splitlines() if l.startswith("{")]
+    head = lines[0]
+    assert head["regenerated_a_sha256"] == crec["a_sha256"] and head["regenerated_b_sha256"] == crec["b_sha256"]
+    assert head["regenerated_tile_sha256"] == crec["tile_sha256"] and head["audit_process_pid"]
+    A = np.fromfile(tmp_path / "dump.f64").reshape(n, n, order="F")
+    assert hashlib.sha256(A.tobytes(order="F")).hexdigest() == crec["a_sha256"]
+    B = np.fromfile(tmp_path / "B_a1.f64").reshape(n, nrhs, order="F")
+    by_path = {l["x_path"]: l for l in lines[1:]}
+    assert by_path[str(tmp_path / "missing.f64")]["error"] == "cannot open"
+    for t in sols:
+        rec = by_path[str(tmp_path / t["x_path"])]
+        raw = (tmp_path / t["x_path"]).read_bytes()
+        assert rec["x_sha256"] == hashlib.sha256(raw).hexdigest() == t["x_sha256"]
+        X = np.frombuffer(raw, dtype=np.float32 if t["x_dtype"] == "float32" else np.float64).astype(np.float64).reshape(n, nrhs, order="F")
+        assert max(rec["eta_fp64"]) == t["eta_max"]
+        m = oracle.metrics(A, B, X, dd=True)
+        np.testing.assert_allclose(rec["eta_dd"], m["eta"], rtol=1e-6, atol=1e-30)
+        assert rec["has_exact"] and np.allclose(rec["finf_exact"], np.abs(X - xe.reshape(n, nrhs, order="F")).max(axis=0)
+                                                 / np.abs(xe.reshape(n, nrhs, order="F")).max(axis=0), rtol=1e-12)

@@ -1227,24 +1227,44 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ S, int64
   if (t == 0) exps[tl] = s;
   const float sc = ldexpf(1.0f, s);
   unsigned long long n_over = 0, n_sub = 0;
-  for (int q = t; q < nkb * 64 * 8; q += 256) {
-    const int kblk = q >> 9, l = (q >> 3) & 63, e = q & 7;
-    const int ii = l & 31;
-    const int kk = (FMT == FMT_FP32) ? (kblk * 16 + 2 * e + (l >> 5)) : (kblk * 16 + (l >> 5) * 8 + e);   // FP16X3: as FP16
-    const float v = tile[ii * (KP + 1) + kk] * sc;
-    const float av = fabsf(v);
-    if (av > FI::MAXV) n_over++; else if (av > 0.0f && av < FI::MINN) n_sub++;
-    const size_t o = ((size_t)(tl * nkb + kblk) * 64 + l) * 8 + e;
-    if (FMT == FMT_FP16) { const _Float16 h = (_Float16)v; ((uint16_t*)out)[o] = __builtin_bit_cast(uint16_t, h); }
-    else if (FMT == FMT_FP16X3) {
-      const _Float16 h = (_Float16)v;
-      const _Float16 lo = (_Float16)(v - (float)h);   // v - hi is exact in FP32
-      ((uint16_t*)out)[o] = __builtin_bit_cast(uint16_t, h);
-      ((uint16_t*)out_lo)[o] = __builtin_bit_cast(uint16_t, lo);
+  // one (k-block, lane) fragment of 8 elements per thread and pass: 8 LDS reads, one vector store per output
+  for (int u = t; u < nkb * 64; u += 256) {
+    const int kblk = u >> 6, l = u & 63;
+    const float* trow = tile + (l & 31) * (KP + 1);
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int kk = (FMT == FMT_FP32) ? (kblk * 16 + 2 * e + (l >> 5)) : (kblk * 16 + (l >> 5) * 8 + e);   // FP16X3: as FP16
+      v[e] = trow[kk] * sc;
+      const float av = fabsf(v[e]);
+      if (av > FI::MAXV) n_over++; else if (av > 0.0f && av < FI::MINN) n_sub++;
     }
-    else if (FMT == FMT_BF16) ((uint16_t*)out)[o] = to_bf16_rne(v);
-    else if (FMT == FMT_FP8) out[o] = to_e4m3_rne(v);
-    else ((float*)out)[o] = v;
+    const size_t o = ((size_t)(tl * nkb + kblk) * 64 + l) * 8;
+    if (FMT == FMT_FP16 || FMT == FMT_FP16X3) {
+      union { uint16_t h[8]; uint4 w; } hi, lo;
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const _Float16 hv = (_Float16)v[e];
+        hi.h[e] = __builtin_bit_cast(uint16_t, hv);
+        if (FMT == FMT_FP16X3) lo.h[e] = __builtin_bit_cast(uint16_t, (_Float16)(v[e] - (float)hv));   // v - hi is exact in FP32
+      }
+      *(uint4*)((uint16_t*)out + o) = hi.w;
+      if (FMT == FMT_FP16X3) *(uint4*)((uint16_t*)out_lo + o) = lo.w;
+    } else if (FMT == FMT_BF16) {
+      union { uint16_t h[8]; uint4 w; } hi;
+#pragma unroll
+      for (int e = 0; e < 8; e++) hi.h[e] = to_bf16_rne(v[e]);
+      *(uint4*)((uint16_t*)out + o) = hi.w;
+    } else if (FMT == FMT_FP8) {
+      union { uint8_t h[8]; uint2 w; } q8;
+#pragma unroll
+      for (int e = 0; e < 8; e++) q8.h[e] = to_e4m3_rne(v[e]);
+      *(uint2*)(out + o) = q8.w;
+    } else {
+      float4* d = (float4*)((float*)out + o);
+      d[0] = make_float4(v[0], v[1], v[2], v[3]);
+      d[1] = make_float4(v[4], v[5], v[6], v[7]);
+    }
   }
   if (n_over) atomicAdd(cnt, n_over);
   if (n_sub) atomicAdd(cnt + 1, n_sub);

@@ -1201,15 +1201,29 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ S, int64
   const int tl = blockIdx.x;
   const int KP = nkb * 16;
   float m = 0.0f;
-  for (int q = t; q < 32 * KP; q += 256) {
+  // 32 x KP source elements, at most 32 per thread (KP <= 256): every load is issued before the first one is
+  // used (indices clamped in range, out-of-range elements zeroed afterwards), so a thread waits for memory
+  // once instead of once per element
+  float vals[32];
+#pragma unroll
+  for (int j = 0; j < 32; j++) {
+    const int q = t + 256 * j;
     int ii, kk;
     if (sI == 1) { kk = q / 32; ii = q - kk * 32; } else { ii = q / KP; kk = q - ii * KP; }
-    const int idx = tl * 32 + ii;
-    float v = 0.0f;
-    if (idx < nidx && kk < K) v = S[(int64_t)idx * sI + (int64_t)kk * sK];
-    tile[ii * (KP + 1) + kk] = v;
-    const float av = fabsf(v);
-    m = (av != av || av > m) ? av : m;
+    const int idx = min(tl * 32 + ii, nidx - 1);
+    vals[j] = (q < 32 * KP) ? S[(int64_t)idx * sI + (int64_t)min(kk, K - 1) * sK] : 0.0f;
+  }
+#pragma unroll
+  for (int j = 0; j < 32; j++) {
+    const int q = t + 256 * j;
+    if (q < 32 * KP) {
+      int ii, kk;
+      if (sI == 1) { kk = q / 32; ii = q - kk * 32; } else { ii = q / KP; kk = q - ii * KP; }
+      const float v = (tl * 32 + ii < nidx && kk < K) ? vals[j] : 0.0f;
+      tile[ii * (KP + 1) + kk] = v;
+      const float av = fabsf(v);
+      m = (av != av || av > m) ? av : m;
+    }
   }
   red[t] = m;
   __syncthreads();

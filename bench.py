@@ -189,6 +189,17 @@ def bench_serve(ctx, nreq, pool_size, loads=(0.5, 0.8, 0.95)):
     return out
 
 
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, calibrated against a copy of known size) from
+    the committed rocprofv3 PMC summaries; None when no summary covers it."""
+    for f in sorted((Path(__file__).resolve().parent / "profiles").glob("*_traffic.md")):
+        for line in f.read_text().splitlines():
+            w = line.split()
+            if len(w) == 3 and w[0] == "traffic_bytes_per_launch" and w[1] == kernel:
+                return int(w[2])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,7 +298,7 @@ def main():
         per_ms = v["ms"] / v["count"]
         gbs = v["bytes"] / v["ms"] / 1e6
         roof = dict(kernel=name, bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None, launches=v["count"], avg_launch_ms=round(per_ms, 4),
+                    frac=round(gbs / HBM_PEAK_GBS, 4), traffic=pmc_traffic(name), launches=v["count"], avg_launch_ms=round(per_ms, 4),
                     all_kernels={k: dict(ms=round(x["ms"], 3), count=x["count"],
                                          GBps=round(x["bytes"] / max(x["ms"], 1e-9) / 1e6, 1),
                                          TFps=round(x["flops"] / max(x["ms"], 1e-9) / 1e9, 2)) for k, x in prof.items()})

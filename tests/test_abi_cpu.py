@@ -67,3 +67,26 @@ def test_method_registry_covers_reference_ids():
         assert cat[mid]["kind"] != "unsupported" and cat[mid]["gmres"], mid
     assert cat["native_leftgmres32_app64_w64_r64"]["apply64"]
     json.dumps(cat)
+
+
+def test_sha256_known_answers():
+    """mp_sha256_hex (host-only) against the FIPS 180-2 examples and hashlib; the records' a_sha256 / x_sha256
+    come from the same routine."""
+    import hashlib
+    import cpsc402_midterm_project_amd as pkg
+    lib = ctypes.CDLL(str(pkg.LIB_PATH))
+    lib.mp_sha256_hex.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_char_p]
+
+    def sha(b):
+        out = ctypes.create_string_buffer(65)
+        buf = ctypes.create_string_buffer(b, len(b))
+        assert lib.mp_sha256_hex(buf, len(b), out) == 0
+        return out.value.decode()
+
+    assert sha(b"abc") == "ba7816bf8f01cfea414140de5dae2223b00361a396177a9cb410ff61f20015ad"
+    assert sha(b"") == "e3b0c44298fc1c149afbf4c8996fb92427ae41e4649b934ca495991b7852b855"
+    msg = b"abcdbcdecdefdefgefghfghighijhijkijkljklmklmnlmnomnopnopq"
+    assert sha(msg) == "248d6a61d20638b8e5c026930c3e6039a33ce45964ff2167f6ecedd419db06c1"
+    for n in (55, 56, 63, 64, 65, 1000, 1 << 16):
+        b = bytes((i * 7 + 3) & 0xFF for i in range(n))
+        assert sha(b) == hashlib.sha256(b).hexdigest(), n

@@ -324,7 +324,7 @@ int cmd_run(Session& S, const Args& a, const std::string& case_id, long long n, 
          .d("gmres_tol", a.getd("gmres-tol", 1e-4)).i("gmres_budget", a.geti("gmres-budget", 480))
          .i("gmres_cycles", a.geti("gmres-cycles", 4)).i("max_outer", a.geti("max-outer", 30))
          .d("cost_ratio_next", a.getd("cost-ratio-next", 3.0)).b("instrument", a.has("instrument")).b("reuse", a.has("reuse"))
-         .s("timestamp", iso_now());
+         .s("panel_requested", a.get("panel", "cusolver")).s("timestamp", iso_now());
         mp_trial_out T;
         double t_up = 0, t_down = 0;
         void* dA = nullptr; void* dB = nullptr;

@@ -1545,7 +1545,7 @@ int mp_solve(mp_ctx* C, mp_problem* P, const char* method_id, const char* params
      .s("timestamp", a.get("timestamp", "")).i("nb", M.nb).i("gmres_m", M.gmres_m).d("gmres_tol", M.gmres_tol).i("gmres_budget", M.gmres_budget)
      .i("gmres_cycles", M.gmres_cycles).i("max_outer", M.max_outer).s("policy", M.policy).i("budget", M.budget).i("scale_mode", M.scale_mode)
      .i("working", M.working).i("resid", M.resid == 128 ? 64 : M.resid).b("resid_dd", M.resid == 128).b("apply64", M.apply64)
-     .b("reuse", a.getb("reuse", false)).s("panel", "hip_lds_exchange").i("stages_count", (int64_t)T.stages.size())
+     .b("reuse", a.getb("reuse", false)).s("panel", "hip_lds_exchange").s("panel_requested", a.get("panel_requested", "cusolver")).i("stages_count", (int64_t)T.stages.size())
      .s("stop_on", eta_only ? "eta" : "eta_omega");
     C->last_record = j.obj();
     std::string h;

@@ -85,6 +85,8 @@ __global__ __launch_bounds__(128) void k_trinv(const float* __restrict__ LU, int
     for (int q = 0; q < GB; q++) if (q < c) X[q * GB + c] = 0.0f;
     for (int q = 1; q < GB; q++) {
       float s = 0.0f;
+      // unrolled so the LDS reads of later terms are in flight while the fmaf chain advances (same order)
+#pragma unroll 8
       for (int p = 0; p < q; p++) {
         const float ns = fmaf(T[q * (GB + 1) + p], X[p * GB + c], s);
         s = (p >= c) ? ns : s;
@@ -94,6 +96,7 @@ __global__ __launch_bounds__(128) void k_trinv(const float* __restrict__ LU, int
   } else {
     for (int q = GB - 1; q >= 0; q--) {
       float s = (q == c) ? 1.0f : 0.0f;
+#pragma unroll 8
       for (int p = q + 1; p < GB; p++) {
         const float ns = fmaf(-T[q * (GB + 1) + p], X[p * GB + c], s);
         s = (p <= c) ? ns : s;

@@ -111,6 +111,19 @@ def test_cli_selftest_and_unsupported_flags(tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n,seed", [(256, 9100), (384, 9101)])
+def test_cli_selftest_reference_anchor(n, seed):
+    """The reference's packaged generator check (finalize.py:80: `mpsolve selftest --n 256|384 --family
+    prescribed --spectrum identity --seed 9100|9101`) reports an orthogonality deviation of about 1.5e-15 and no
+    zero entries (finalize.py:85); the rotation-stage generator reproduces both."""
+    r = _run("selftest", "--n", n, "--family", "prescribed", "--spectrum", "identity", "--seed", seed)
+    assert r.returncode == 0, r.stderr
+    rec = json.loads(r.stdout)
+    assert 5e-16 <= rec["max_orthogonality_deviation"] <= 4e-15, rec
+    assert rec["zero_entries"] == 0 and rec["min_abs"] > 0
+
+
+@pytest.mark.gpu
 def test_cli_fp32_storage_and_rowcol_scaling(tmp_path):
     """--dtype fp32 stores A and B in FP32 (files .f32); --rowcol-span scales rows and columns by powers of two;
     both systems are solved to the FP32 target by the default method."""

@@ -1,0 +1,42 @@
+"""Magnitude anchors from the reference's own logs: its capability probe (src/probe/probe_cusolver.cu) solved an
+n=2048 Gaussian system with a 2 sqrt(n) diagonal boost by FP64 and FP32 LU and logged the normwise (eta) and
+componentwise (omega) backward errors (logs/probe_cusolver_2048.jsonl:2,4).  The same matrix, regenerated with
+libstdc++ by tests/golden/probe_matrix, solved here by native_lu64 (FP64 LU) and native_lu32 (FP32 LU, FP32
+solution), must land at the same magnitudes: the factorizations round differently, so the bar is a factor
+of 4, not equality."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+HELPER = Path(__file__).resolve().parent / "golden" / "probe_matrix"
+PROBE = {"native_lu64": (1.400e-16, 5.856e-16), "native_lu32": (5.694e-08, 2.320e-07)}   # (eta, omega)
+
+
+@pytest.fixture(scope="module")
+def probe_system(tmp_path_factory):
+    if not HELPER.exists():
+        pytest.fail(f"{HELPER} is not built (make -C tests/golden)")
+    d = tmp_path_factory.mktemp("probe")
+    n = 2048
+    subprocess.run([str(HELPER), str(n), str(d / "A.f64"), str(d / "b.f64")], check=True, timeout=60)
+    A = np.fromfile(d / "A.f64").reshape(n, n, order="F")
+    b = np.fromfile(d / "b.f64").reshape(n, 1)
+    return A, b
+
+
+@pytest.mark.parametrize("method", ["native_lu64", "native_lu32"])
+def test_probe_backward_error_magnitudes(ctx, oracle, probe_system, method):
+    A, b = probe_system
+    print(f"[probe anchor] {method} n=2048", flush=True)
+    P = ctx.problem(A, b)
+    X, rec = P.solve(method)
+    assert rec["terminal_reason"] == "direct_solve" and rec["factor_info"] == 0
+    m = oracle.metrics(A, b, X)
+    eta0, om0 = PROBE[method]
+    print(f"[probe anchor] {method}: eta {m['eta_max']:.3e} (probe {eta0:.3e}), omega {m['omega_max']:.3e} (probe {om0:.3e})", flush=True)
+    assert eta0 / 4 <= m["eta_max"] <= eta0 * 4, (m["eta_max"], eta0)
+    assert om0 / 4 <= m["omega_max"] <= om0 * 4, (m["omega_max"], om0)
+    P.close()

@@ -557,6 +557,8 @@ struct Engine {
   mp_ctx* C; const mp_problem& P; Factors fact; bool reuse = false; bool resume_ladder = false; bool instrument = false;
   int64_t n; int nrhs; double tau_eta, tau_omega; bool stop_eta_only = false;
   DBuf<double> x64, D, y; DBuf<float> x32;
+  // ladder policies of this build: the iterate with the smallest eta so far, where the next stage starts
+  bool keep_best = false; double best_eta = INFINITY; DBuf<double> xbest;
   EvalBufs E;
   Engine(mp_ctx* c, const mp_problem& p, double te, double to) : C(c), P(p), n(p.n), nrhs(p.nrhs), tau_eta(te), tau_omega(to) {
     x64.alloc(C->tracker, (size_t)n * nrhs, "x64");
@@ -696,6 +698,10 @@ struct Engine {
   void residual_and_metrics(const MethodSpec& M, Metrics& Mt) {
     if (M.resid == 32) throw MPError("fp32 residual control requires fp32 stored data", -4);
     residual_metrics(C, P, x64.p, M.resid == 128, E, Mt);
+    if (keep_best && Mt.finite && Mt.eta_max < best_eta) {
+      best_eta = Mt.eta_max;
+      copy_f64(x64.p, xbest.p, (int64_t)n * nrhs, W());
+    }
   }
 
   std::string refine_stage(const MethodSpec& M, const Factors& F, int stage_idx, TrialOut& T, double t_start, int& budget_left,
@@ -746,8 +752,9 @@ struct Engine {
     int budget_left = M.gmres ? M.gmres_budget : 0;
     std::vector<std::string> ladder{M.factor};
     const bool target64 = (tau_eta < 1e-10);
-    if (M.policy == "ladder") { ladder = {"mx8", "mx16", "cus32"}; }
-    else if (M.policy == "ladder16") { ladder = {"mx16", "mx16x3", "cus32"}; }
+    // this build's ladders end, like the reference's (src/mp_core.cuh:301), with FP64 factors for an FP64 target
+    if (M.policy == "ladder") { ladder = {"mx8", "mx16", "cus32"}; if (target64) ladder.push_back("cs64"); }
+    else if (M.policy == "ladder16") { ladder = {"mx16", "mx16x3", "cus32"}; if (target64) ladder.push_back("cs64"); }
     else if (M.policy == "ladder16x3") { ladder = {"mx16x3", "cus32", "cs64"}; }
     else if (M.policy != "none") {
       if (M.factor == "het16" || M.factor == "cus32") { ladder.push_back("cs32"); if (target64) ladder.push_back("cs64"); }
@@ -757,6 +764,13 @@ struct Engine {
     if (M.working == 32) { x32.ensure(C->tracker, (size_t)n * nrhs, "x32"); HIP_CK(hipMemsetAsync(x32.p, 0, (size_t)n * nrhs * 4, W().stream)); }
     Metrics Mt; std::string reason = "unknown"; MethodSpec Ms = M;
     bool initialized = false;
+    // A stage that stagnates or runs out of budget can leave an iterate far worse than one it passed through
+    // (low-precision factors of an ill-conditioned matrix); this build's ladders start the next stage from the
+    // best iterate seen (smallest eta), not from the last one.  The reference's policies keep continuing from
+    // the last iterate.  The returned solution is always the last iterate of the last stage.
+    keep_best = (M.policy == "ladder" || M.policy == "ladder16" || M.policy == "ladder16x3") && M.working == 64;
+    best_eta = INFINITY;
+    if (keep_best) xbest.ensure(C->tracker, (size_t)n * nrhs, "x_best");
     int first = 0;
     // resume_ladder: with reuse, start at the stage whose factors the problem keeps (the stage that served the
     // last solve of this matrix), so repeated solves skip the stages that failed before
@@ -793,8 +807,13 @@ struct Engine {
       }
       if (reason == "converged") break;
       if (M.policy == "none") break;
-      if (stage + 1 < (int)ladder.size())
+      if (stage + 1 < (int)ladder.size()) {
         T.hist.push_back({T.outer, stage, wall_now() - t0, Mt.eta_max, Mt.omega_max, -1, 0, "escalate_after:" + reason});
+        if (keep_best && std::isfinite(best_eta) && !(Mt.finite && Mt.eta_max <= best_eta)) {
+          copy_f64(xbest.p, x64.p, (int64_t)n * nrhs, W());
+          T.hist.back().note += ";restart_from_best_eta=" + std::to_string(best_eta);
+        }
+      }
     }
     T.terminal = (M.kind == "direct" && reason != "nonfinite" && reason != "singular_factor") ? "direct_solve" : reason;
     sync();

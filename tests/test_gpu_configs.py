@@ -1,0 +1,66 @@
+"""The workload configurations other than the headline, as parity cases at their stated shapes.
+
+C0: n=1024, kappa=1e4, one RHS, FP16 trailing update with per-tile scales (nb=128), stationary IR with FP64
+residuals, stop at eta <= 1e-15 or 50 steps; the oracle runs the same method on the CPU (ir_stationary:
+src/mp_core.cuh:281-296 over the blocked LU of :129-150), and both reach the same outcome.
+
+C2: n=32768, nrhs=4, kappa in {1e4, 1e8, 1e10}, FP8-e4m3 trailing update with per-tile power-of-two scales,
+FGMRES-IR with double-double residuals, ladder FP8 -> FP16 -> FP32 (-> FP64 for an FP64 target) on stagnation
+(policy_mx8_ladder).  Every RHS reaches eta <= 1e-15 by the oracle's FP64 metric; eta, omega, forward errors
+against x_true and against the FP64 reference solution, stages and iteration counts are printed per kappa.  The
+matrices are the prescribed family (rotation stages, the same geometric spectrum) instead of randsvd, whose
+QR-based generation takes tens of seconds per matrix at this size.
+"""
+import numpy as np
+import pytest
+
+from cpsc402_midterm_project_amd import Problem
+from cpsc402_midterm_project_amd.problems import randsvd
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("kappa", [1e3, 1e4])
+def test_c0_stationary_ir_agrees_with_cpu_restatement(ctx, oracle, kappa):
+    """At kappa=1e3 both converge, at step counts within 2.  At C0's kappa=1e4 single-term FP16 factors
+    contract the error by only about 0.6-0.9 per step: the oracle is still at eta ~1e-13 after 50 steps, and
+    neither reaches 1e-15 (the GPU may stop earlier on the stagnation rule, eta > 0.9 * previous for three
+    steps, which the rounding of each run decides); the first steps agree in magnitude."""
+    A, b, xt, _ = randsvd(1024, kappa, 9200, gpu=False)
+    print(f"[C0] mx16_ir_w64_r64 n=1024 kappa={kappa:g} nb=128", flush=True)
+    P = ctx.problem(A, b, x_exact=xt)
+    X, rec = P.solve("mx16_ir_w64_r64", nb=128, max_outer=50, tau=1e-15, stop_on="eta")
+    eta_gpu = [h["eta_max"] for h in P.history() if h.get("kind") == "outer"]
+    xo, reason, hist = oracle.ir_stationary(np.asfortranarray(A), b, fmt="fp16", nb=128, tau=1e-15, max_outer=50,
+                                            stop_eta_only=True)
+    print(f"[C0] gpu {rec['terminal_reason']} {rec['outer_iters']} steps, eta {eta_gpu[0]:.2e} -> {eta_gpu[-1]:.2e}; "
+          f"oracle {reason} {len(hist) - 1} steps, eta {hist[0]:.2e} -> {min(hist):.2e}", flush=True)
+    assert hist[0] / 4 <= eta_gpu[0] <= hist[0] * 4
+    if kappa < 1e4:
+        assert rec["terminal_reason"] == reason == "converged", (rec["terminal_reason"], reason)
+        assert abs(rec["outer_iters"] - (len(hist) - 1)) <= 2, (rec["outer_iters"], len(hist) - 1)
+        assert oracle.metrics(A, b, X)["eta_max"] <= 1e-15 and oracle.metrics(A, b, xo)["eta_max"] <= 1e-15
+    else:
+        assert reason in ("max_outer", "stagnation") and rec["terminal_reason"] in ("max_outer", "stagnation")
+        assert min(eta_gpu) > 1e-15 and min(hist) > 1e-15
+    P.close()
+
+
+@pytest.mark.parametrize("kappa", [1e4, 1e8, 1e10])
+def test_c2_fp8_ladder_full_shape(ctx, oracle, kappa):
+    n, nrhs = 32768, 4
+    print(f"[C2] policy_mx8_ladder n={n} nrhs={nrhs} kappa={kappa:g}", flush=True)
+    P = Problem.generate(ctx, family="prescribed", n=n, nrhs=nrhs, kappa=kappa, seed=9202, rhs="xtrue_uniform")
+    P.reference()
+    X, rec = P.solve("policy_mx8_ladder", nb=256, tau=1e-15, stop_on="eta")
+    print(f"[C2] kappa={kappa:g} terminal={rec['terminal_reason']} stages={rec['stages']} outer={rec['outer_iters']} "
+          f"inner={rec['inner_iters_total']} eta={rec['eta_max']:.2e} omega={rec['omega_max']:.2e} "
+          f"einf={max(rec['einf']):.2e} finf={max(rec['finf']):.2e} device_s={rec['device_ready_wall_s']:.2f}", flush=True)
+    assert rec["terminal_reason"] == "converged", (rec["terminal_reason"], rec["stages"], rec["error"])
+    assert rec["stages"][0] == "mx8"
+    A, B, xt = P.host_copy()
+    m = oracle.metrics(A, B, X)
+    assert m["eta_max"] <= 1e-15, m["eta_max"]
+    # forward error against x_true is bounded by kappa times the backward error (normwise, up to n)
+    assert max(rec["einf"]) <= max(kappa * 1e-14, 1e-12)
+    P.close()

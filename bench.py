@@ -208,8 +208,9 @@ def main():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--kappas", default="1e2,1e6")
     ap.add_argument("--nb", type=int, default=256)
-    ap.add_argument("--methods", default="mx16_gmresir_w64_rdd,mx16x3_gmresir_w64_rdd",
-                    help="low-precision candidates (FP16 MFMA trailing update); the fastest converged one is reported")
+    ap.add_argument("--methods", default="mx16_gmresir_w64_rdd,mxbf16_gmresir_w64_rdd,mx16x3_gmresir_w64_rdd",
+                    help="low-precision candidates (FP16 and, separately, BF16 MFMA trailing updates); the fastest "
+                         "converged one is reported")
     ap.add_argument("--fallback", default="mx32_gmresir_w64_rdd")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-count", type=int, default=4096, help="uneven-batch leg: systems per launch (0: skip)")

@@ -101,3 +101,42 @@ def test_batch_rejects_out_of_range(ctx):
         ctx.batch_solve([(big, np.ones((513, 1)))])
     X, res, rec = ctx.batch_solve([])
     assert X == [] and res == [] and rec["systems"] == 0
+
+
+def test_batch_full_uneven_workload_every_system_by_oracle(ctx, oracle):
+    """The uneven-batch configuration at its full size: 4096 randsvd systems, n ~ U{32..512}, log10 kappa ~
+    U[1, 5], seeds 9300 + i (the bench's workload), one packed launch; every system's eta, recomputed by the
+    oracle's FP64 metric from the returned X, is <= 1e-15."""
+    import concurrent.futures as cf
+    import os
+    import threadpoolctl
+    count = 4096
+    rng = np.random.default_rng(9300)
+    n = rng.integers(32, 513, size=count).astype(np.int32)
+    kap = 10.0 ** rng.uniform(1, 5, size=count)
+    a_off = np.concatenate([[0], np.cumsum(n.astype(np.int64) ** 2)[:-1]])
+    b_off = np.concatenate([[0], np.cumsum(n.astype(np.int64))[:-1]])
+    Ah, Bh = np.empty(int((n.astype(np.int64) ** 2).sum())), np.empty(int(n.sum()))
+    systems = [None] * count
+
+    def gen(i):
+        k = int(n[i])
+        A, b, _, _ = randsvd(k, float(kap[i]), 9300 + i, nrhs=1, gpu=False)
+        Ah[a_off[i]:a_off[i] + k * k] = A.ravel(order="F"); Bh[b_off[i]:b_off[i] + k] = b.ravel(order="F")
+        systems[i] = (A, b)
+
+    print(f"[batch full] generating {count} systems", flush=True)
+    with threadpoolctl.threadpool_limits(1), cf.ThreadPoolExecutor(min(16, len(os.sched_getaffinity(0)))) as ex:
+        list(ex.map(gen, range(count)))
+    dA, dB, dX = ctx.buffer(Ah), ctx.buffer(Bh), ctx.empty(8 * Bh.size)
+    print(f"[batch full] solving {count} systems", flush=True)
+    res, rec = ctx.batch_solve_packed(n, np.ones(count, np.int32), dA, a_off, dB, b_off, dX, tau=1e-15, stop_on="eta")
+    assert rec["converged"] == count, rec
+    X = dX.to_array((Bh.size,), np.float64)
+    worst = 0.0
+    for i, (A, b) in enumerate(systems):
+        k = int(n[i])
+        eta = oracle.metrics(A, b, X[b_off[i]:b_off[i] + k].reshape(k, 1))["eta_max"]
+        assert eta <= 1e-15, (i, k, kap[i], eta)
+        worst = max(worst, eta)
+    print(f"[batch full] every system converged; worst eta by the oracle {worst:.2e}", flush=True)

@@ -4,6 +4,9 @@ C0: n=1024, kappa=1e4, one RHS, FP16 trailing update with per-tile scales (nb=12
 residuals, stop at eta <= 1e-15 or 50 steps; the oracle runs the same method on the CPU (ir_stationary:
 src/mp_core.cuh:281-296 over the blocked LU of :129-150), and both reach the same outcome.
 
+C1: n=16384, the headline method at kappa=1e6 and the BF16 trailing update at kappa=1e2, checked by the
+oracle's double-double metric.
+
 C2: n=32768, nrhs=4, kappa in {1e4, 1e8, 1e10}, FP8-e4m3 trailing update with per-tile power-of-two scales,
 FGMRES-IR with double-double residuals, ladder FP8 -> FP16 -> FP32 (-> FP64 for an FP64 target) on stagnation
 (policy_mx8_ladder).  Every RHS reaches eta <= 1e-15 by the oracle's FP64 metric; eta, omega, forward errors
@@ -43,6 +46,23 @@ def test_c0_stationary_ir_agrees_with_cpu_restatement(ctx, oracle, kappa):
     else:
         assert reason in ("max_outer", "stagnation") and rec["terminal_reason"] in ("max_outer", "stagnation")
         assert min(eta_gpu) > 1e-15 and min(hist) > 1e-15
+    P.close()
+
+
+@pytest.mark.parametrize("method,kappa,seed", [("mx16x3_gmresir_w64_rdd", 1e6, 9201), ("mxbf16_gmresir_w64_rdd", 1e2, 9200)])
+def test_c1_headline_shape_by_oracle(ctx, oracle, method, kappa, seed):
+    """C1 at n=16384 (randsvd generated on the GPU, the bench's seeds): the headline method at kappa=1e6 and the
+    BF16 trailing update at kappa=1e2 reach eta <= 1e-15 by the oracle's double-double metric on the host."""
+    n = 16384
+    print(f"[C1] {method} n={n} kappa={kappa:g}", flush=True)
+    P = Problem.generate(ctx, family="randsvd", n=n, nrhs=1, kappa=kappa, seed=seed, rhs="xtrue_uniform")
+    X, rec = P.solve(method, nb=256, gmres_m=30, tau=1e-15, stop_on="eta")
+    print(f"[C1] {method} kappa={kappa:g}: {rec['terminal_reason']} outer={rec['outer_iters']} inner={rec['inner_iters_total']} "
+          f"eta={rec['eta_max']:.2e} device_s={rec['device_ready_wall_s']:.3f}", flush=True)
+    assert rec["terminal_reason"] == "converged", (rec["terminal_reason"], rec["eta_max"])
+    A, B, xt = P.host_copy()
+    m = oracle.metrics(A, B, X, dd=True)
+    assert m["eta_max"] <= 1e-15, m["eta_max"]
     P.close()
 
 

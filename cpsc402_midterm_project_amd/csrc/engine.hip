@@ -918,6 +918,7 @@ int mp_create(mp_ctx** out, int device) {
     HIP_CK(hipStreamCreateWithFlags(&C->W.pstream, hipStreamNonBlocking));
     HIP_CK(hipEventCreateWithFlags(&C->W.ev_next, hipEventDisableTiming));
     HIP_CK(hipEventCreateWithFlags(&C->W.ev_panel, hipEventDisableTiming));
+    HIP_CK(hipEventCreateWithFlags(&C->W.ev_pack, hipEventDisableTiming));
     C->W.tracker = &C->tracker;
     hipDeviceProp_t prop; HIP_CK(hipGetDeviceProperties(&prop, device));
     C->W.num_cus = prop.multiProcessorCount;
@@ -946,10 +947,11 @@ void mp_destroy(mp_ctx* C) {
   if (C->hpin) (void)hipHostFree(C->hpin);
   (void)hipStreamSynchronize(C->W.pstream);
   hipStream_t s = C->W.stream, ps = C->W.pstream;
-  hipEvent_t e1 = C->W.ev_next, e2 = C->W.ev_panel;
+  hipEvent_t e1 = C->W.ev_next, e2 = C->W.ev_panel, e3 = C->W.ev_pack;
   delete C;
   if (e1) (void)hipEventDestroy(e1);
   if (e2) (void)hipEventDestroy(e2);
+  if (e3) (void)hipEventDestroy(e3);
   if (ps) (void)hipStreamDestroy(ps);
   if (s) (void)hipStreamDestroy(s);
 }

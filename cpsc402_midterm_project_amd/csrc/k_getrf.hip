@@ -1406,25 +1406,38 @@ static void pack_buffers(int kb, int64_t m2, Workspace& W) {
   const int nT = (int)((cdiv(m2, 128)) * 4);
   const size_t bytes = (size_t)nT * nkb * 64 * 8 * ES;
   W.packL.ensure(*W.tracker, bytes, "packL");
+  W.packL2.ensure(*W.tracker, bytes, "packL_b");
   W.packU.ensure(*W.tracker, bytes, "packU");
-  if (FMT == FMT_FP16X3) { W.packLlo.ensure(*W.tracker, bytes, "packL_lo"); W.packUlo.ensure(*W.tracker, bytes, "packU_lo"); }
+  if (FMT == FMT_FP16X3) {
+    W.packLlo.ensure(*W.tracker, bytes, "packL_lo"); W.packLlo2.ensure(*W.tracker, bytes, "packL_lo_b");
+    W.packUlo.ensure(*W.tracker, bytes, "packU_lo");
+  }
   W.scaleL.ensure(*W.tracker, nT, "scaleL");
+  W.scaleL2.ensure(*W.tracker, nT, "scaleL_b");
   W.scaleU.ensure(*W.tracker, nT, "scaleU");
+}
+
+// the packed-L set of step k (k / nb even: the first set, odd: the second)
+struct LPack { uint8_t* L; uint8_t* Llo; int32_t* s; };
+static LPack lpack(Workspace& W, int64_t k, int nb) {
+  if (((k / nb) & 1) == 0) return {W.packL.p, W.packLlo.p, W.scaleL.p};
+  return {W.packL2.p, W.packLlo2.p, W.scaleL2.p};
 }
 
 // L21 (rows k+kb .. n-1 of the panel columns) into row tiles 0 ..
 template <int FMT>
-static void pack_L(float* A, int64_t lda, int64_t k, int kb, int64_t m2, bool tile_scale, Workspace& W, const char* pname) {
+static void pack_L(float* A, int64_t lda, int64_t k, int kb, int64_t m2, bool tile_scale, Workspace& W, const char* pname,
+                   LPack lp, hipStream_t stream) {
   constexpr int ES = FmtInfo<FMT>::ES;
   const int nkb = (kb + 15) / 16;
   const int nT = (int)((cdiv(m2, 128)) * 4);
   const size_t lds = (size_t)32 * (nkb * 16 + 1) * sizeof(float);
-  hipEvent_t evp = W.prof_begin();
-  hipLaunchKernelGGL((k_pack<FMT>), dim3(nT), dim3(256), lds, W.stream,
-                     A + (k + kb) + k * lda, (int64_t)1, lda, (int)m2, kb, nkb, W.packL.p,
-                     FMT == FMT_FP16X3 ? W.packLlo.p : nullptr, W.scaleL.p, W.cnt.p, tile_scale ? 1 : 0);
+  hipEvent_t evp = W.prof_begin(stream);
+  hipLaunchKernelGGL((k_pack<FMT>), dim3(nT), dim3(256), lds, stream,
+                     A + (k + kb) + k * lda, (int64_t)1, lda, (int)m2, kb, nkb, lp.L,
+                     FMT == FMT_FP16X3 ? lp.Llo : nullptr, lp.s, W.cnt.p, tile_scale ? 1 : 0);
   LAUNCH_CK();
-  W.prof_end(evp, pname, (4.0 + ES) * (double)m2 * kb, 0.0);
+  W.prof_end(evp, pname, (4.0 + ES) * (double)m2 * kb, 0.0, stream);
 }
 
 // U12 columns [cb, ce) (cb - (k + kb) a multiple of 32) into column tiles starting at (cb - k - kb) / 32;
@@ -1449,7 +1462,8 @@ static void pack_U(float* A, int64_t lda, int64_t k, int kb, int64_t cb, int64_t
 
 // A22[:, cb .. ce) -= L21 * U12[:, cb .. ce)
 template <int FMT>
-static void gemm_cols(float* A, int64_t lda, int64_t k, int kb, int64_t m2, int64_t cb, int64_t ce, Workspace& W, const char* gname) {
+static void gemm_cols(float* A, int64_t lda, int64_t k, int kb, int64_t m2, int64_t cb, int64_t ce, Workspace& W, const char* gname,
+                      LPack lp) {
   constexpr int ES = FmtInfo<FMT>::ES, FB = FmtInfo<FMT>::FB;
   const int nkb = (kb + 15) / 16;
   const int64_t t0 = (cb - k - kb) / 32, n2 = ce - cb;
@@ -1459,31 +1473,52 @@ static void gemm_cols(float* A, int64_t lda, int64_t k, int kb, int64_t m2, int6
   const double gf = (FMT == FMT_FP16X3 ? 6.0 : 2.0) * (double)m2 * n2 * kb;
   hipEvent_t ev = W.prof_begin();
   hipLaunchKernelGGL((k_gemm<FMT>), dim3((unsigned)cdiv(m2, 128), (unsigned)cdiv(n2, 128)), dim3(256), 0, W.stream,
-                     A + (k + kb) + cb * lda, lda, (int)m2, (int)n2, nkb, W.packL.p, W.packU.p + t0 * tb,
-                     FMT == FMT_FP16X3 ? W.packLlo.p : nullptr, FMT == FMT_FP16X3 ? W.packUlo.p + t0 * tb : nullptr,
-                     W.scaleL.p, W.scaleU.p + t0);
+                     A + (k + kb) + cb * lda, lda, (int)m2, (int)n2, nkb, lp.L, W.packU.p + t0 * tb,
+                     FMT == FMT_FP16X3 ? lp.Llo : nullptr, FMT == FMT_FP16X3 ? W.packUlo.p + t0 * tb : nullptr,
+                     lp.s, W.scaleU.p + t0);
   LAUNCH_CK();
   W.prof_end(ev, gname, gb, gf);
 }
 
+// with_L: pack L21 of step k here (on the main stream) before the update; otherwise it was packed ahead
 template <int FMT>
-static void update_cols(float* A, int64_t lda, int64_t k, int kb, int64_t m2, int64_t cb, int64_t ce, bool last,
-                        bool with_L, bool tile_scale, Workspace& W) {
-  // the next panel's columns (look-ahead) are profiled under their own names
-  const bool nxt = with_L && !last;
-  if (with_L) { pack_buffers<FMT>(kb, m2, W); pack_L<FMT>(A, lda, k, kb, m2, tile_scale, W, nxt ? "pack_next" : "pack"); }
+static void update_cols(float* A, int64_t lda, int64_t k, int kb, int nb, int64_t m2, int64_t cb, int64_t ce, bool last,
+                        bool with_L, bool nxt, bool tile_scale, Workspace& W) {
+  // the next panel's columns (look-ahead) are profiled under their own names.  The packed-L pointers are
+  // taken after pack_buffers, which allocates the sets on first use.
+  if (with_L) pack_buffers<FMT>(kb, m2, W);
+  const LPack lp = lpack(W, k, nb);
+  if (with_L) pack_L<FMT>(A, lda, k, kb, m2, tile_scale, W, nxt ? "pack_next" : "pack", lp, W.stream);
   pack_U<FMT>(A, lda, k, kb, cb, ce, last, tile_scale, W, nxt ? "pack_next" : "pack");
-  gemm_cols<FMT>(A, lda, k, kb, m2, cb, ce, W, nxt ? "gemm_next" : "trailing_gemm");
+  gemm_cols<FMT>(A, lda, k, kb, m2, cb, ce, W, nxt ? "gemm_next" : "trailing_gemm", lp);
 }
 
-static void update_cols_fmt(int fmt, float* A, int64_t lda, int64_t k, int kb, int64_t m2, int64_t cb, int64_t ce,
-                            bool last, bool with_L, bool tile_scale, Workspace& W) {
+static void update_cols_fmt(int fmt, float* A, int64_t lda, int64_t k, int kb, int nb, int64_t m2, int64_t cb, int64_t ce,
+                            bool last, bool with_L, bool nxt, bool tile_scale, Workspace& W) {
   switch (fmt) {
-    case FMT_FP16: update_cols<FMT_FP16>(A, lda, k, kb, m2, cb, ce, last, with_L, tile_scale, W); break;
-    case FMT_BF16: update_cols<FMT_BF16>(A, lda, k, kb, m2, cb, ce, last, with_L, tile_scale, W); break;
-    case FMT_FP8:  update_cols<FMT_FP8>(A, lda, k, kb, m2, cb, ce, last, with_L, tile_scale, W); break;
-    case FMT_FP16X3: update_cols<FMT_FP16X3>(A, lda, k, kb, m2, cb, ce, last, with_L, tile_scale, W); break;
-    default:       update_cols<FMT_FP32>(A, lda, k, kb, m2, cb, ce, last, with_L, tile_scale, W); break;
+    case FMT_FP16: update_cols<FMT_FP16>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W); break;
+    case FMT_BF16: update_cols<FMT_BF16>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W); break;
+    case FMT_FP8:  update_cols<FMT_FP8>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W); break;
+    case FMT_FP16X3: update_cols<FMT_FP16X3>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W); break;
+    default:       update_cols<FMT_FP32>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W); break;
+  }
+}
+
+// L21 of step k packed ahead on `stream` (the panel stream, right after step k's panel), into step k's set
+template <int FMT>
+static void pack_L_ahead(float* A, int64_t lda, int64_t k, int kb, int nb, int64_t m2, bool tile_scale, Workspace& W,
+                         hipStream_t stream) {
+  pack_buffers<FMT>(kb, m2, W);
+  pack_L<FMT>(A, lda, k, kb, m2, tile_scale, W, "pack_ahead", lpack(W, k, nb), stream);
+}
+static void pack_L_ahead_fmt(int fmt, float* A, int64_t lda, int64_t k, int kb, int nb, int64_t m2, bool tile_scale,
+                             Workspace& W, hipStream_t stream) {
+  switch (fmt) {
+    case FMT_FP16: pack_L_ahead<FMT_FP16>(A, lda, k, kb, nb, m2, tile_scale, W, stream); break;
+    case FMT_BF16: pack_L_ahead<FMT_BF16>(A, lda, k, kb, nb, m2, tile_scale, W, stream); break;
+    case FMT_FP8:  pack_L_ahead<FMT_FP8>(A, lda, k, kb, nb, m2, tile_scale, W, stream); break;
+    case FMT_FP16X3: pack_L_ahead<FMT_FP16X3>(A, lda, k, kb, nb, m2, tile_scale, W, stream); break;
+    default:       pack_L_ahead<FMT_FP32>(A, lda, k, kb, nb, m2, tile_scale, W, stream); break;
   }
 }
 
@@ -1621,6 +1656,9 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   int panels = 0;
   run_panel(a, W, 0, (int)std::min<int64_t>(nb, n), plan_of(0), W.stream, true, allow_p2, rmin);
   panels++;
+  // L21 of the current step was packed ahead on the panel stream (after its panel, while the previous step's
+  // remaining update ran on the main stream); steps alternate between two packed-L sets for that
+  bool l_ahead = false;
   for (int64_t k = 0; k < n; k += nb) {
     const int kb = (int)std::min<int64_t>(nb, n - k);
     const int64_t m2 = n - k - kb;
@@ -1643,23 +1681,35 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
       LAUNCH_CK();
       W.prof_end(e2, "trsm_inv", 0.0, 0.0);
     }
+    // the update needs this step's packed L21: when it was packed ahead, wait for that pack only here, so the
+    // interchanges and the TRSM above run while it finishes
     if (split) {
       trsm_cols(a, W, k, kb, k + kb, cN, false, "trsm_next");
-      update_cols_fmt(a.fmt, a.A, a.lda, k, kb, m2, k + kb, cN, false, true, a.tile_scale, W);
+      if (l_ahead) HIP_CK(hipStreamWaitEvent(W.stream, W.ev_pack, 0));
+      update_cols_fmt(a.fmt, a.A, a.lda, k, kb, nb, m2, k + kb, cN, false, !l_ahead, true, a.tile_scale, W);
       HIP_CK(hipEventRecord(W.ev_next, W.stream));
       HIP_CK(hipStreamWaitEvent(W.pstream, W.ev_next, 0));
       run_panel(a, W, k + kb, kbn, plan_of(k + kb), W.pstream, false, allow_p2, rmin);
       panels++;
       HIP_CK(hipEventRecord(W.ev_panel, W.pstream));
+      // the next step's L21 is final once its panel is written back (its own interchanges touch other columns)
+      const int64_t m2n = m2 - kbn;
+      l_ahead = m2n > 0;
+      if (l_ahead) {
+        pack_L_ahead_fmt(a.fmt, a.A, a.lda, k + kb, kbn, nb, m2n, a.tile_scale, W, W.pstream);
+        HIP_CK(hipEventRecord(W.ev_pack, W.pstream));
+      }
       laswp_ranges(a, W, k, kb, 0, k, cN, n, true, plan, false, nullptr);
       trsm_cols(a, W, k, kb, cN, n, false);
-      update_cols_fmt(a.fmt, a.A, a.lda, k, kb, m2, cN, n, true, false, a.tile_scale, W);
+      update_cols_fmt(a.fmt, a.A, a.lda, k, kb, nb, m2, cN, n, true, false, false, a.tile_scale, W);
       HIP_CK(hipStreamWaitEvent(W.stream, W.ev_panel, 0));
     } else {
       trsm_cols(a, W, k, kb, k + kb, n, use_fma_trsm);
-      update_cols_fmt(a.fmt, a.A, a.lda, k, kb, m2, k + kb, n, true, true, a.tile_scale, W);
+      if (l_ahead) HIP_CK(hipStreamWaitEvent(W.stream, W.ev_pack, 0));
+      update_cols_fmt(a.fmt, a.A, a.lda, k, kb, nb, m2, k + kb, n, true, !l_ahead, false, a.tile_scale, W);
       run_panel(a, W, k + kb, kbn, plan_of(k + kb), W.stream, true, allow_p2, rmin);
       panels++;
+      l_ahead = false;
     }
   }
   int32_t hinfo[2] = {0, 0};

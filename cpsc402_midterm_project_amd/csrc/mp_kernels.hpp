@@ -32,7 +32,7 @@ struct KernelProfiler {
 struct Workspace {
   hipStream_t stream = nullptr;
   hipStream_t pstream = nullptr;          // second stream: panel factorizations overlapped with trailing updates
-  hipEvent_t ev_next = nullptr, ev_panel = nullptr;   // cross-stream ordering (timing disabled)
+  hipEvent_t ev_next = nullptr, ev_panel = nullptr, ev_pack = nullptr;   // cross-stream ordering (timing disabled)
   KernelProfiler prof;
   DevTracker* tracker = nullptr;
   int num_cus = 256;
@@ -43,6 +43,9 @@ struct Workspace {
   DBuf<uint8_t> packL, packU;   // trailing-update operands in MFMA fragment order
   DBuf<uint8_t> packLlo, packUlo; // low parts of the two-term FP16 operands
   DBuf<int32_t> scaleL, scaleU; // per-32-row / per-32-column power-of-two exponents
+  // second set of the packed L operand: steps alternate between the two, so the next step's L21 can be packed
+  // on the panel stream while this step's remaining trailing update still reads its own
+  DBuf<uint8_t> packL2, packLlo2; DBuf<int32_t> scaleL2;
   DBuf<unsigned long long> cnt; // [0] overflow, [1] subnormal counts of the operand conversion
   DBuf<int32_t> dinfo;   // [0] info, [1] exchange timeout flag
   DBuf<int32_t> affect;  // per-panel swap plan (src,dst pairs)

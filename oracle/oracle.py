@@ -5,7 +5,8 @@ Wraps liboracle.so (mp_oracle.c) and restates, in numpy, the reference's metric 
 loops:
   metrics         eval_metrics_T        src/mp_core.cuh:193-205 (eta, omega, relres, worst row)
   ir_stationary   Engine::refine_stage  src/mp_core.cuh:281-296 (stopping tests and terminal reasons)
-  fgmres          Engine::gmres_solve   src/mp_core.cuh:237-278 (right-preconditioned flexible GMRES)
+  fgmres          Engine::gmres_solve   src/mp_core.cuh:237-278 (right-preconditioned flexible GMRES, and
+                                        the left-preconditioned variant)
   factor / apply_inverse   factorize + apply_inverse, src/mp_core.cuh:158-190 (scale modes 0, 1, 2)
   Xoshiro, hash3, prescribed, rhs_random, rowcol_factors, unit_columns, dyadic
                   the reference's generator streams and families, src/mp_common.cuh:34-43,
@@ -229,15 +230,18 @@ def ir_stationary(A, B, fmt="fp16", nb=128, tau=1e-15, max_outer=50, dd=False, s
         outer += 1
 
 
-def fgmres(A, r, precond, m=30, tol=1e-4, cycles=4, budget=480):
+def fgmres(A, r, precond, m=30, tol=1e-4, cycles=4, budget=480, left=False):
     """Right-preconditioned flexible GMRES on A d = r (Engine::gmres_solve, src/mp_core.cuh:237-278), with
-    classical Gram-Schmidt applied twice.  Returns (d, status, iterations)."""
+    classical Gram-Schmidt applied twice; left=True: GMRES on M^-1 A d = M^-1 r (src/mp_core.cuh:240-241,
+    249-250, 266, 271): the Krylov space of M^-1 A from M^-1 r, d += V y, and the restart residual
+    preconditioned.  Returns (d, status, iterations)."""
     n = A.shape[0]
     d = np.zeros(n)
-    beta0 = np.linalg.norm(r)
+    rcur = precond(r) if left else r.copy()
+    beta0 = np.linalg.norm(rcur)
     if beta0 == 0 or not np.isfinite(beta0):
         return d, ("zero_rhs" if beta0 == 0 else "nonfinite"), 0
-    rcur = r.copy(); its = 0; status = "budget"
+    its = 0; status = "budget"
     for cyc in range(cycles):
         if budget <= 0:
             break
@@ -249,8 +253,11 @@ def fgmres(A, r, precond, m=30, tol=1e-4, cycles=4, budget=480):
         V[:, 0] = rcur / betac
         j = 0; happy = False
         while j < m and budget > 0:
-            Z[:, j] = precond(V[:, j])
-            w = A @ Z[:, j]
+            if left:
+                w = precond(A @ V[:, j])
+            else:
+                Z[:, j] = precond(V[:, j])
+                w = A @ Z[:, j]
             h = V[:, :j + 1].T @ w; w = w - V[:, :j + 1] @ h
             h2 = V[:, :j + 1].T @ w; w = w - V[:, :j + 1] @ h2; h = h + h2
             hn = np.linalg.norm(w)
@@ -270,9 +277,10 @@ def fgmres(A, r, precond, m=30, tol=1e-4, cycles=4, budget=480):
                 break
         k = j
         y = np.linalg.solve(np.triu(H[:k, :k]), g[:k]) if k else np.zeros(0)
-        d = d + Z[:, :k] @ y
-        rcur = r - A @ d
-        true_rel = np.linalg.norm(rcur) / np.linalg.norm(r)
+        d = d + (V[:, :k] if left else Z[:, :k]) @ y
+        sr = r - A @ d
+        true_rel = np.linalg.norm(sr) / np.linalg.norm(r)
+        rcur = precond(sr) if left else sr
         if true_rel <= tol:
             return d, ("happy_breakdown_verified" if happy else "converged_true"), its
         if happy:
@@ -280,7 +288,8 @@ def fgmres(A, r, precond, m=30, tol=1e-4, cycles=4, budget=480):
     return d, status, its
 
 
-def fgmres_ir(A, B, fmt="fp16", nb=128, tau=1e-15, max_outer=30, m=30, tol=1e-4, dd=True, tile_scale=True):
+def fgmres_ir(A, B, fmt="fp16", nb=128, tau=1e-15, max_outer=30, m=30, tol=1e-4, dd=True, tile_scale=True,
+              left=False, cycles=4):
     F = factor(A, fmt, nb, 1, tile_scale)
     n = A.shape[0]
     b = np.asarray(B, dtype=np.float64).reshape(n)
@@ -296,7 +305,7 @@ def fgmres_ir(A, B, fmt="fp16", nb=128, tau=1e-15, max_outer=30, m=30, tol=1e-4,
             Rh, Rl = resid_dd(A, x, b); r = (Rh + Rl)[:, 0]
         else:
             r = resid_fp64(A, x, b)[0][:, 0]
-        d, st, its = fgmres(A, r, pre, m=m, tol=tol)
+        d, st, its = fgmres(A, r, pre, m=m, tol=tol, left=left, cycles=cycles)
         inner += its
         x = x + d
     return x, "max_outer", hist, inner

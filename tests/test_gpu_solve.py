@@ -218,3 +218,21 @@ def test_problem_outliving_its_context(oracle):
     finally:
         P.close()
         c2.close()
+
+
+def test_left_gmres_ir_against_oracle_restatement(ctx, oracle):
+    """Left-preconditioned GMRES-IR with FP32 factors and FP64 residuals against the oracle's left variant of
+    Engine::gmres_solve (src/mp_core.cuh:236-278) on the same system: both converge, with the same number of
+    outer steps (within 1) and inner iterations within 25% (the factorizations round differently)."""
+    A, b, xt, _ = randsvd(512, 1e4, 9212, gpu=False)
+    print("[solve] native_leftgmres32_app32_w64_r64 vs oracle n=512 kappa=1e4", flush=True)
+    P = ctx.problem(A, b)
+    X, rec = P.solve("native_leftgmres32_app32_w64_r64", nb=64, tau=1e-15, stop_on="eta")
+    xo, reason, hist, inner = oracle.fgmres_ir(np.asfortranarray(A), b, fmt="fp32", nb=64, tau=1e-15, m=40, dd=False,
+                                               tile_scale=False, left=True, cycles=1)
+    print(f"[solve] gpu {rec['terminal_reason']} outer={rec['outer_iters']} inner={rec['inner_iters_total']}; "
+          f"oracle {reason} outer={len(hist) - 1} inner={inner}", flush=True)
+    assert rec["terminal_reason"] == reason == "converged"
+    assert abs(rec["outer_iters"] - (len(hist) - 1)) <= 1
+    assert abs(rec["inner_iters_total"] - inner) <= max(2, 0.25 * inner)
+    assert oracle.metrics(A, b, X)["eta_max"] <= 1e-15

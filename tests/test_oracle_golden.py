@@ -325,3 +325,26 @@ def test_dyadic_rhs_is_exact(oracle):
     A, B, X = oracle.dyadic(30, 5, 255, 8, nrhs=2)
     assert np.all(np.abs(X) == 1.0) and np.all(np.abs(A * 256 - np.round(A * 256)) == 0) and np.abs(A * 256).max() <= 255
     np.testing.assert_array_equal(A @ X, B)                        # every sum is exact in FP64
+
+
+def test_left_gmres_pins(oracle):
+    """Left-preconditioned GMRES: with the exact inverse as preconditioner M^-1 A = I, so one step solves the
+    correction equation; with the identity it is plain GMRES and equals the right-preconditioned run."""
+    rng = np.random.default_rng(11)
+    n = 60
+    A = rng.standard_normal((n, n)) + n ** 0.5 * np.eye(n)
+    r = rng.standard_normal(n)
+    exact = lambda v: np.linalg.solve(A, v)
+    d, st, its = oracle.fgmres(A, r, exact, m=30, tol=1e-12, left=True)
+    assert its == 1 and st in ("converged_true", "happy_breakdown_verified")
+    np.testing.assert_allclose(d, np.linalg.solve(A, r), rtol=1e-12, atol=1e-12 * np.abs(d).max())
+    ident = lambda v: v.copy()
+    dl, stl, itl = oracle.fgmres(A, r, ident, m=40, tol=1e-10, cycles=1, left=True)
+    dr, str_, itr = oracle.fgmres(A, r, ident, m=40, tol=1e-10, cycles=1, left=False)
+    assert itl == itr and stl == str_
+    np.testing.assert_allclose(dl, dr, rtol=0, atol=1e-12 * np.abs(dr).max())
+    # a rough preconditioner (FP16 factors): left GMRES-IR reaches the FP64 test
+    B, b, _, _ = randsvd(200, 1e3, 4, gpu=False)
+    x, reason, hist, inner = oracle.fgmres_ir(np.asfortranarray(B), b, fmt="fp16", nb=64, tau=1e-15, m=40, left=True,
+                                              cycles=1)
+    assert reason == "converged" and inner > 0

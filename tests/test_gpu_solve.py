@@ -236,3 +236,22 @@ def test_left_gmres_ir_against_oracle_restatement(ctx, oracle):
     assert abs(rec["outer_iters"] - (len(hist) - 1)) <= 1
     assert abs(rec["inner_iters_total"] - inner) <= max(2, 0.25 * inner)
     assert oracle.metrics(A, b, X)["eta_max"] <= 1e-15
+
+
+@pytest.mark.parametrize("method,fmt,kappa", [("mx16_gmresir_w64_rdd", "fp16", 1e3), ("mx32_gmresir_w64_rdd", "fp32", 1e6)])
+def test_fgmres_ir_against_oracle_restatement(ctx, oracle, method, fmt, kappa):
+    """Right-preconditioned FGMRES-IR with double-double residuals against the oracle's restatement
+    (src/mp_core.cuh:237-278 over the blocked LU with the same trailing format): both converge, outer steps
+    within 1, inner iterations within 30% (the MFMA accumulation rounds differently from the oracle's
+    double-precision sums, so the factors are not bitwise equal on random matrices)."""
+    A, b, xt, _ = randsvd(512, kappa, 9213, gpu=False)
+    print(f"[solve] {method} vs oracle n=512 kappa={kappa:g}", flush=True)
+    P = ctx.problem(A, b)
+    X, rec = P.solve(method, nb=64, tau=1e-15, stop_on="eta")
+    xo, reason, hist, inner = oracle.fgmres_ir(np.asfortranarray(A), b, fmt=fmt, nb=64, tau=1e-15, m=40, dd=True)
+    print(f"[solve] gpu {rec['terminal_reason']} outer={rec['outer_iters']} inner={rec['inner_iters_total']}; "
+          f"oracle {reason} outer={len(hist) - 1} inner={inner}", flush=True)
+    assert rec["terminal_reason"] == reason == "converged"
+    assert abs(rec["outer_iters"] - (len(hist) - 1)) <= 1
+    assert abs(rec["inner_iters_total"] - inner) <= max(3, 0.3 * inner)
+    assert oracle.metrics(A, b, X, dd=True)["eta_max"] <= 1e-15

@@ -20,6 +20,20 @@ __global__ void k_set_x32(float* x32, double* x64, const double* d, int64_t m) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i < m) { const float v = (float)d[i]; x32[i] = v; x64[i] = (double)v; }
 }
+// FP32 residual control (k_resid_f32, src/mp_kernels.cuh:80-81): R = B - A X with every operation in FP32,
+// the row sum a sequential fmaf chain over the columns; B holds FP32 values (FP32-stored systems)
+__global__ void k_resid_f32_rows(const float* __restrict__ A, int64_t n, const float* __restrict__ X,
+                                 const double* __restrict__ B, float* __restrict__ R, int nrhs) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < nrhs; c++) {
+    const float* xp = X + (int64_t)c * n;
+    float s = 0.0f;
+    for (int64_t j = 0; j < n; j++) s = fmaf(A[i + j * n], xp[j], s);
+    R[i + (int64_t)c * n] = (float)B[i + (int64_t)c * n] - s;
+  }
+}
+
 __global__ void k_add_x32(float* x32, double* x64, const double* d, int64_t m) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i < m) { const float v = (float)((double)x32[i] + d[i]); x32[i] = v; x64[i] = (double)v; }
@@ -556,7 +570,7 @@ struct TrialOut {
 struct Engine {
   mp_ctx* C; const mp_problem& P; Factors fact; bool reuse = false; bool resume_ladder = false; bool instrument = false;
   int64_t n; int nrhs; double tau_eta, tau_omega; bool stop_eta_only = false;
-  DBuf<double> x64, D, y; DBuf<float> x32;
+  DBuf<double> x64, D, y; DBuf<float> x32, r32;
   // ladder policies of this build: the iterate with the smallest eta so far, where the next stage starts
   bool keep_best = false; double best_eta = INFINITY; DBuf<double> xbest;
   EvalBufs E;
@@ -695,8 +709,37 @@ struct Engine {
     return status;
   }
 
+  // In-loop metrics.  resid 32 (src/mp_core.cuh:231-235): the FP32 residual of the FP32 iterate on the
+  // FP32-stored system, eta = ||r||inf / (||A||inf ||x||inf + ||b||inf) from it, omega not evaluated (-1).
   void residual_and_metrics(const MethodSpec& M, Metrics& Mt) {
-    if (M.resid == 32) throw MPError("fp32 residual control requires fp32 stored data", -4);
+    if (M.resid == 32) {
+      if (!P.fp32) throw MPError("fp32 residual control requires fp32 stored data", -1);
+      if (M.working != 32) throw MPError("fp32 residual control needs the FP32 working iterate", -1);
+      const int64_t m = n * nrhs;
+      r32.ensure(C->tracker, (size_t)m, "R32");
+      hipLaunchKernelGGL(k_resid_f32_rows, dim3(cdiv(n, 256)), dim3(256), 0, W().stream, P.A32.p, n, x32.p, P.B.p, r32.p, nrhs);
+      LAUNCH_CK();
+      std::vector<float> hr(m); std::vector<double> hx(m);
+      HIP_CK(hipMemcpyAsync(hr.data(), r32.p, m * 4, hipMemcpyDeviceToHost, W().stream));
+      HIP_CK(hipMemcpyAsync(hx.data(), x64.p, m * 8, hipMemcpyDeviceToHost, W().stream));
+      sync();
+      Mt.eta.assign(nrhs, 0.0); Mt.omega.assign(nrhs, -1.0); Mt.rinf.assign(nrhs, 0.0); Mt.xinf.assign(nrhs, 0.0);
+      Mt.finite = true; Mt.eta_max = 0.0; Mt.omega_max = -1.0;
+      for (int c = 0; c < nrhs; c++) {
+        double ri = 0.0, xi = 0.0;
+        for (int64_t i = 0; i < n; i++) {
+          const double r = std::fabs((double)hr[i + c * n]), x = std::fabs(hx[i + c * n]);
+          if (!(r <= ri)) ri = r;             // a NaN is kept, as the column statistics do
+          if (!(x <= xi)) xi = x;
+        }
+        const double den = P.anorm_inf * xi + P.bnorm_inf[c];
+        Mt.eta[c] = (ri == 0) ? 0.0 : (den == 0 ? INFINITY : ri / den);
+        Mt.rinf[c] = ri; Mt.xinf[c] = xi;
+        if (!std::isfinite(ri) || !std::isfinite(xi)) Mt.finite = false;
+        Mt.eta_max = std::max(Mt.eta_max, Mt.eta[c]);
+      }
+      return;
+    }
     residual_metrics(C, P, x64.p, M.resid == 128, E, Mt);
     if (keep_best && Mt.finite && Mt.eta_max < best_eta) {
       best_eta = Mt.eta_max;

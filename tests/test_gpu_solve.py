@@ -310,3 +310,25 @@ def test_fp32_residual_control_on_an_fp32_system(ctx, oracle):
     m = oracle.metrics(A.astype(np.float64), B, X)
     assert m["eta_max"] <= 100 * 2.0 ** -24, m["eta_max"]
     P.close()
+
+
+@pytest.mark.parametrize("mid,ladder", [
+    ("policy_hetero16_escalate_immediate", ["het16", "cs32", "cs64"]),
+    ("policy_hetero16_budget120", ["het16", "cs32", "cs64"]),
+    ("policy_hetero16_cost_aware", ["het16", "cs32", "cs64"]),
+    ("policy_native32_escalate_immediate", ["cs32", "cs64"])])
+def test_reference_policies_escalate(ctx, oracle, mid, ladder):
+    """The reference's policies at kappa=1e6, where FP16-operand factors cannot reach the FP64 target:
+    they escalate along het16 -> cs32 -> cs64 (cs32 -> cs64), src/mp_core.cuh:299-314, stop at the first stage
+    that converges, and the final iterate meets tau = 100 u64 by the oracle's metric."""
+    A, b, xt, _ = randsvd(384, 1e6, 9216, gpu=False)
+    print(f"[policy] {mid} n=384 kappa=1e6", flush=True)
+    P = ctx.problem(A, b)
+    X, rec = P.solve(mid, nb=64)
+    assert rec["terminal_reason"] == "converged", (mid, rec["terminal_reason"], rec["stages"])
+    st = rec["stages"]
+    assert st == ladder[:len(st)] and rec["final_stage"] == st[-1], (mid, st)
+    assert rec["used_fallback"] == (len(st) > 1)
+    tau = 100 * 2.0 ** -53
+    m = oracle.metrics(A, b, X, dd=True)
+    assert m["eta_max"] <= tau and m["omega_max"] <= tau, (mid, m["eta_max"], m["omega_max"])

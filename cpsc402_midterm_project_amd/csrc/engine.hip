@@ -4,6 +4,7 @@
 // Engine with refine_stage, gmres_solve and run, and the method registry make_method (src/mpsolve.cu:17-72).
 // The numerical kernels are this library's HIP kernels; the vendor comparators (cs32 / cs64 factor kinds,
 // i.e. the plain FP32 / FP64 LU) are rocSOLVER getrf/getrs.
+#include <thread>
 #include "mp_internal.hpp"
 #include "mp_kernels.hpp"
 #include "mp_krylov.hpp"
@@ -899,12 +900,22 @@ static void finish_setup(mp_ctx* C, mp_problem* Pp, const char* src, const std::
       for (int64_t i = 0; i < n; i++) { const double v = std::fabs(hb[i + (size_t)c * n]); m = std::max(m, v); s += v * v; }
       P->bnorm_inf[c] = m; P->bnorm_2[c] = std::sqrt(s);
     }
-    P->a_sha = sha256_hex(src, (size_t)n * n * es);
+    // the whole-matrix digest is one sequential pass; the 16 column-tile digests run beside it on host threads
     const size_t colbytes = (size_t)n * es;
-    for (int t = 0; t < 16; t++) {
-      const size_t c0 = (size_t)n * t / 16, c1 = (size_t)n * (t + 1) / 16;
-      P->tile_sha.push_back(sha256_hex(src + c0 * colbytes, (c1 - c0) * colbytes));
+    std::vector<std::string> tiles(16);
+    {
+      std::vector<std::thread> th;
+      for (int g = 0; g < 4; g++)
+        th.emplace_back([&, g] {
+          for (int t = g; t < 16; t += 4) {
+            const size_t c0 = (size_t)n * t / 16, c1 = (size_t)n * (t + 1) / 16;
+            tiles[t] = sha256_hex(src + c0 * colbytes, (c1 - c0) * colbytes);
+          }
+        });
+      P->a_sha = sha256_hex(src, (size_t)n * n * es);
+      for (auto& t : th) t.join();
     }
+    P->tile_sha = tiles;
     P->b_sha = sha256_hex(hb.data(), hb.size() * 8);
 }
 

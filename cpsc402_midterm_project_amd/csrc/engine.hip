@@ -432,6 +432,8 @@ static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, i
     ga.tile_scale = (kind != "het16"); ga.ipiv = F.ipiv.p; ga.perm = F.perm.p;
     GetrfInfo gi;
     getrf(ga, W, &gi);
+    // a panel whose cross-workgroup exchange hit its spin limit left the factors incomplete
+    if (gi.timeout) throw MPError("factorization: the panel exchange timed out (workgroups not co-resident)", -2);
     F.info = gi.info; F.conv_overflow = gi.overflow; F.conv_subnormal = gi.subnormal; F.panels = gi.panels;
   }
   const double gm = (F.prec == 32) ? dev_absmax(C, F.LU32.p, true, n, n) : dev_absmax(C, F.LU64.p, false, n, n);
@@ -793,6 +795,9 @@ struct Engine {
   void run(const MethodSpec& M, TrialOut& T) {
     const double t0 = wall_now();
     T.outer = 0; T.inner_total = 0;
+    // word 2 of the info area is the substitutions' exchange-timeout flag: cleared here, checked at the end
+    W().dinfo.ensure(C->tracker, 4, "getrf_info");
+    HIP_CK(hipMemsetAsync(W().dinfo.p + 2, 0, 4, W().stream));
     int budget_left = M.gmres ? M.gmres_budget : 0;
     std::vector<std::string> ladder{M.factor};
     const bool target64 = (tau_eta < 1e-10);
@@ -860,7 +865,10 @@ struct Engine {
       }
     }
     T.terminal = (M.kind == "direct" && reason != "nonfinite" && reason != "singular_factor") ? "direct_solve" : reason;
+    int32_t trs_timeout = 0;
+    HIP_CK(hipMemcpyAsync(&trs_timeout, W().dinfo.p + 2, 4, hipMemcpyDeviceToHost, W().stream));
     sync();
+    if (trs_timeout) throw MPError("substitution: the block exchange timed out (workgroups not co-resident)", -2);
     T.t_total = wall_now() - t0;
   }
 };

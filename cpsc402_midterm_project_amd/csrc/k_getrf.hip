@@ -1549,9 +1549,16 @@ static void run_panel(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int32
     pa.ipiv = a.ipiv; pa.dinfo = W.dinfo.p; pa.plan = plan;
     void* args[] = {&pa};
     hipEvent_t e0 = W.prof_begin(stream);
-    if (cooperative)
-      HIP_CK(hipLaunchCooperativeKernel((const void*)k_panel2, dim3((unsigned)G2), dim3(256), args, (unsigned)panel2_lds_bytes(kb), stream));
-    else {
+    // The runtime refuses a cooperative launch while another process shares the device; the ordinary launch
+    // is then used.  Every wait of the exchange is bounded, so a workgroup that never became resident ends
+    // the panel with the timeout flag (reported as an error), never with a hang.
+    bool launched = false;
+    if (cooperative) {
+      launched = hipLaunchCooperativeKernel((const void*)k_panel2, dim3((unsigned)G2), dim3(256), args,
+                                            (unsigned)panel2_lds_bytes(kb), stream) == hipSuccess;
+      if (!launched) (void)hipGetLastError();
+    }
+    if (!launched) {
       // overlapped with trailing-update kernels that never wait on anything: workgroups that are not yet
       // resident start as those kernels drain, and the resident ones spin meanwhile
       hipLaunchKernelGGL(k_panel2, dim3((unsigned)G2), dim3(256), panel2_lds_bytes(kb), stream, pa);
@@ -1577,7 +1584,10 @@ static void run_panel(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int32
   void* args[] = {&pa};
   const size_t lds = (size_t)R * kb * 4 + (size_t)R * 4;
   hipEvent_t e0 = W.prof_begin(stream);
-  HIP_CK(hipLaunchCooperativeKernel((const void*)k_panel, dim3(G), dim3(256), args, (unsigned)lds, stream));
+  if (hipLaunchCooperativeKernel((const void*)k_panel, dim3(G), dim3(256), args, (unsigned)lds, stream) != hipSuccess) {
+    (void)hipGetLastError();      // refused while another process shares the device: bounded waits, as above
+    HIP_CK(hipLaunchKernel((const void*)k_panel, dim3(G), dim3(256), args, lds, stream));
+  }
   W.prof_end(e0, "panel", 8.0 * (double)m * kb, (double)m * kb * kb, stream);
 }
 

@@ -39,7 +39,9 @@ def test_cli_caps_reports_device_or_fatal_json():
 
 
 @pytest.mark.gpu
-def test_cli_run_campaign(tmp_path):
+def test_cli_run_campaign(tmp_path, ctx):
+    # `ctx` keeps a context of this process open on the device while the CLI runs: the runtime then refuses
+    # cooperative launches, and the factorization must fall back to ordinary ones
     n, nrhs, methods = 256, 2, ["native_lu64", "mx16_ir_w64_r64"]
     print(f"[cli run] n={n} nrhs={nrhs} methods={methods}", flush=True)
     r = _run("run", "--n", n, "--nrhs", nrhs, "--family", "dyadic", "--rhs", "exact_sign", "--seed", 17,
@@ -74,7 +76,8 @@ def test_cli_run_campaign(tmp_path):
                 rec = recs[i]; i += 1
                 assert (rec["mode"], rec["rep"], rec["warmup"]) == (mode, rep, rep < 0)
                 assert rec["method_id"] == methods[(mi + rot) % len(methods)]
-                assert rec["ok"] and rec["case_id"] == "c1" and rec["tag"] == "t" and rec["nb"] == 64
+                assert rec["ok"], (rec["method_id"], rec["mode"], rec["rep"], rec["terminal_reason"], rec["error"])
+                assert rec["case_id"] == "c1" and rec["tag"] == "t" and rec["nb"] == 64
                 assert rec["eta_max"] <= 100 * 2.0 ** -53, rec["eta_max"]
                 if mode == "host":
                     assert rec["host_e2e_s"] > rec["device_ready_wall_s"] > 0

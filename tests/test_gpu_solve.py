@@ -332,3 +332,28 @@ def test_reference_policies_escalate(ctx, oracle, mid, ladder):
     tau = 100 * 2.0 ** -53
     m = oracle.metrics(A, b, X, dd=True)
     assert m["eta_max"] <= tau and m["omega_max"] <= tau, (mid, m["eta_max"], m["omega_max"])
+
+
+def test_instrumented_history(ctx):
+    """--instrument (src/mp_core.cuh:264,270): one inner record per Arnoldi step with its recursive relative
+    residual, and a cycle_end record with the true relative residual of the correction equation."""
+    A, b, xt, _ = randsvd(512, 1e4, 9217, gpu=False)
+    P = ctx.problem(A, b)
+    X, rec = P.solve("mx16_gmresir_w64_rdd", nb=64, tau=1e-15, stop_on="eta", instrument=True)
+    assert rec["terminal_reason"] == "converged"
+    hist = P.history()
+    inner = [h for h in hist if h["kind"] == "inner" and h["note"] != "cycle_end"]
+    ends = [h for h in hist if h["kind"] == "inner" and h["note"] == "cycle_end"]
+    outer = [h for h in hist if h["kind"] == "outer"]
+    assert len(inner) == rec["inner_iters_total"] > 0
+    assert len(ends) >= rec["outer_iters"] and len(outer) == rec["outer_iters"] + 1
+    for e in ends:
+        assert e["true_rel"] is not None and 0 <= e["true_rel"] < 1
+    by_cycle = {}
+    for h in inner:
+        by_cycle.setdefault((h["outer"], h["rhs"], h["cycle"]), []).append(h["recursive_rel"])
+    for seq in by_cycle.values():
+        assert all(b <= a * (1 + 1e-12) for a, b in zip(seq, seq[1:])), seq     # GMRES residuals never grow
+    plain = P.solve("mx16_gmresir_w64_rdd", nb=64, tau=1e-15, stop_on="eta")
+    assert not [h for h in P.history() if h["kind"] == "inner"]
+    np.testing.assert_array_equal(plain[0], X)                                  # instrumentation changes nothing

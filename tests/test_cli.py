@@ -189,3 +189,15 @@ def test_mpaudit_regenerates_and_reevaluates(tmp_path, oracle):
         np.testing.assert_allclose(rec["eta_dd"], m["eta"], rtol=1e-6, atol=1e-30)
         assert rec["has_exact"] and np.allclose(rec["finf_exact"], np.abs(X - xe.reshape(n, nrhs, order="F")).max(axis=0)
                                                  / np.abs(xe.reshape(n, nrhs, order="F")).max(axis=0), rtol=1e-12)
+
+
+@pytest.mark.gpu
+def test_cli_reuse_keeps_factors_across_trials(tmp_path):
+    """--reuse (src/mpsolve.cu:173): the factors of the first trial of a method serve its later trials."""
+    r = _run("run", "--n", 300, "--kappa", 1e3, "--methods", "mx16_gmresir_w64_rdd", "--reps", 3, "--warmup", 0,
+             "--case-id", "ru", "--out", tmp_path, "--nb", 64, "--reuse")
+    assert r.returncode == 0, r.stderr + r.stdout
+    recs = [json.loads(l) for l in (tmp_path / "trials_ru.jsonl").read_text().splitlines() if l.strip()]
+    assert [x["refactorizations"] for x in recs] == [1, 0, 0], [x["refactorizations"] for x in recs]
+    assert all(x["reuse"] and x["terminal_reason"] == "converged" for x in recs)
+    assert len({x["x_sha256"] for x in recs}) == 1           # the same factors give the same solution

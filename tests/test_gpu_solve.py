@@ -357,3 +357,15 @@ def test_instrumented_history(ctx):
     plain = P.solve("mx16_gmresir_w64_rdd", nb=64, tau=1e-15, stop_on="eta")
     assert not [h for h in P.history() if h["kind"] == "inner"]
     np.testing.assert_array_equal(plain[0], X)                                  # instrumentation changes nothing
+
+
+def test_memory_admission_skips_trial(ctx):
+    """Admission (src/mpsolve.cu:182-190): a trial whose estimated extra device memory exceeds the budget is
+    recorded as memory_skip with its reason and no solve; within budget it runs."""
+    A, b, xt, _ = randsvd(256, 1e2, 9218, gpu=False)
+    P = ctx.problem(A, b)
+    X, rec = P.solve("mx16_gmresir_w64_rdd", nb=64, hbm_budget_bytes=1 << 10)
+    assert rec["terminal_reason"] == "memory_skip" and rec["skipped"] and "memory_skip" in rec["skip_reason"], rec
+    assert rec["outer_iters"] == 0 and rec["refactorizations"] == 0
+    X, rec = P.solve("mx16_gmresir_w64_rdd", nb=64, hbm_budget_bytes=64 << 30)
+    assert rec["terminal_reason"] == "converged" and not rec["skipped"]

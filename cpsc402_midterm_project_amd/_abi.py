@@ -23,7 +23,7 @@ FMT = {"fp16": 0, "bf16": 1, "fp8": 2, "fp8e4m3": 2, "fp32": 3, "fp16x3": 4}
 EXPORTS = [
     "mp_create", "mp_destroy", "mp_last_error", "mp_caps", "mp_synchronize", "mp_profile", "mp_profile_json", "mp_malloc", "mp_free",
     "mp_memcpy_h2d", "mp_memcpy_d2h", "mp_memcpy_d2d", "mp_residual_fp64", "mp_residual_dd", "mp_getrf",
-    "mp_getrs", "mp_problem_from_host", "mp_problem_generate", "mp_problem_reference", "mp_problem_free", "mp_problem_copy_out", "mp_problem_copy_ref", "mp_problem_case_json", "mp_solve",
+    "mp_getrs", "mp_pack_operand", "mp_problem_factor_bytes", "mp_problem_from_host", "mp_problem_generate", "mp_problem_reference", "mp_problem_free", "mp_problem_copy_out", "mp_problem_copy_ref", "mp_problem_case_json", "mp_solve",
     "mp_last_record_json", "mp_last_hist_json", "mp_methods_json", "mp_eval_metrics_json", "mp_audit_json", "mp_sha256_hex", "mp_batch_solve",
     "mp_batch_solve_ptrs", "mp_generate_into", "mp_problem_from_device", "mp_problem_set_rhs", "mp_server_clock",
     "mp_server_create", "mp_server_submit", "mp_server_poll", "mp_server_stats_json", "mp_server_destroy", "mp_getrf64",
@@ -87,6 +87,8 @@ def _load() -> C.CDLL:
         "mp_residual_dd": ([vp, i64, i32, i32, vp, i64, dp, i64, dp, i64, dp, dp, i64, dp, i64, dp, vp], C.c_int),
         "mp_getrf": ([vp, i64, vp, i64, vp, vp, C.POINTER(GetrfParams), C.POINTER(GetrfInfo)], C.c_int),
         "mp_getrs": ([vp, i64, i32, vp, i64, vp, dp, i64, dp, i64, C.c_double], C.c_int),
+        "mp_pack_operand": ([vp, i32, i32, vp, i64, i64, i32, i32, vp, vp, vp, vp], C.c_int),
+        "mp_problem_factor_bytes": ([vp, C.POINTER(C.c_uint64)], C.c_int),
         "mp_problem_from_host": ([vp, i64, i32, i32, vp, i64, dp, i64, dp, dp, C.c_char_p, C.POINTER(vp)], C.c_int),
         "mp_problem_free": ([vp], None),
         "mp_problem_generate": ([vp, C.c_char_p, C.POINTER(vp)], C.c_int),
@@ -267,6 +269,10 @@ class Context:
 
     def getrs(self, n, nrhs, LU, lda, perm, R, ldr, D, ldd, scale=1.0):
         check(lib.mp_getrs(self.h, n, nrhs, _ptr(LU), lda, _ptr(perm), _ptr(R), ldr, _ptr(D), ldd, float(scale)))
+
+    def pack_operand(self, S, s_idx, s_k, nidx, K, out, exps, counts, fmt="fp16", tile_scale=True, out_lo=None):
+        check(lib.mp_pack_operand(self.h, FMT[fmt], 1 if tile_scale else 0, _ptr(S), s_idx, s_k, nidx, K, _ptr(out),
+                                  _ptr(out_lo), _ptr(exps), _ptr(counts)))
 
     def batch_solve_packed(self, n, nrhs, A, a_off, B, b_off, X, **params):
         """mp_batch_solve on packed device buffers; n, nrhs, a_off, b_off are host integer sequences."""

@@ -242,6 +242,7 @@ struct Factors {
   DBuf<float> LU32; DBuf<double> LU64; DBuf<int64_t> ipiv; DBuf<int32_t> perm; DBuf<int32_t> ipiv32;
   double gscale = 1.0; int gscale_exp = 0; int scale_mode = 0; int nb = 0;
   int info = 0; double growth = -1; unsigned long long conv_overflow = 0, conv_subnormal = 0; int panels = 0;
+  int coop_refused = 0;   // panels launched as ordinary kernels after the runtime refused a cooperative launch
   bool valid = false;
   DBuf<double> LU64p; bool promoted = false;   // FP64 copy of FP32 factors, applied by FP64 getrs (apply64)
   DBuf<double> dr, dc;                          // two-sided power-of-two scales (scale mode 2): F = g Dr A Dc
@@ -359,7 +360,7 @@ static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, i
   const int64_t n = P.n; const size_t nn = (size_t)n * n;
   W.factor_gen++;   // cached getrs block inverses of any earlier factors are stale from here on
   F.n = n; F.kind = kind; F.scale_mode = scale_mode; F.nb = nb; F.valid = false; F.growth = -1; F.promoted = false;
-  F.conv_overflow = F.conv_subnormal = 0; F.info = 0; F.panels = 0;
+  F.conv_overflow = F.conv_subnormal = 0; F.info = 0; F.panels = 0; F.coop_refused = 0;
   F.prec = (kind == "cs64" || kind == "mx64") ? 64 : 32;
   F.ipiv.ensure(C->tracker, n, "ipiv");
   F.perm.ensure(C->tracker, n, "perm");
@@ -374,6 +375,9 @@ static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, i
   const double amax_copy = (F.prec == 32) ? (double)(float)P.amax : P.amax;
   if (scale_mode == 1) set_gscale(amax_copy);
   double amax_f = amax_copy * F.gscale;
+  // a stage that changes the factor precision frees the other precision's buffers: a ladder keeps only the
+  // factors it refines with (a cached problem then holds one factor matrix, which is what the serving loop counts)
+  if (F.prec == 32) F.LU64.release(); else { F.LU32.release(); F.LU64p.release(); }
   if (F.prec == 32) {
     F.LU32.ensure(C->tracker, nn, "LU32");
     if (P.fp32) {
@@ -433,8 +437,9 @@ static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, i
     GetrfInfo gi;
     getrf(ga, W, &gi);
     // a panel whose cross-workgroup exchange hit its spin limit left the factors incomplete
-    if (gi.timeout) throw MPError("factorization: the panel exchange timed out (workgroups not co-resident)", -2);
+    if (gi.timeout) throw MPError("factorization: the panel exchange timed out (workgroups not co-resident)", MP_ERR_INTERNAL);
     F.info = gi.info; F.conv_overflow = gi.overflow; F.conv_subnormal = gi.subnormal; F.panels = gi.panels;
+    F.coop_refused = gi.coop_refused;
   }
   const double gm = (F.prec == 32) ? dev_absmax(C, F.LU32.p, true, n, n) : dev_absmax(C, F.LU64.p, false, n, n);
   F.growth = (amax_f > 0) ? gm / amax_f : -1;
@@ -567,6 +572,7 @@ struct TrialOut {
   double t_total = 0, t_factor_total = 0, t_initial = 0, t_refine = 0, t_eval = 0;
   std::vector<HistRec> hist; std::vector<InnerRec> ihist;
   double growth = -1; unsigned long long conv_overflow = 0, conv_subnormal = 0; int factor_info = 0;
+  int panels = 0, panel_coop_refused = 0;   // panel launches of the own factorizations, and how many ran non-cooperatively
   std::string error; bool ok = true; double final_eta = 0, final_omega = 0;
 };
 
@@ -598,6 +604,7 @@ struct Engine {
     const double dt = wall_now() - t0;
     T.t_factor_total += dt; T.stages.push_back(kind); T.stage_times.push_back(dt); T.refactorizations++;
     T.growth = F.growth; T.conv_overflow = F.conv_overflow; T.conv_subnormal = F.conv_subnormal; T.factor_info = F.info;
+    T.panels += F.panels; T.panel_coop_refused += F.coop_refused;
     return F;
   }
 
@@ -868,7 +875,7 @@ struct Engine {
     int32_t trs_timeout = 0;
     HIP_CK(hipMemcpyAsync(&trs_timeout, W().dinfo.p + 2, 4, hipMemcpyDeviceToHost, W().stream));
     sync();
-    if (trs_timeout) throw MPError("substitution: the block exchange timed out (workgroups not co-resident)", -2);
+    if (trs_timeout) throw MPError("substitution: the block exchange timed out (workgroups not co-resident)", MP_ERR_INTERNAL);
     T.t_total = wall_now() - t0;
   }
 };
@@ -1169,9 +1176,40 @@ int mp_getrf(mp_ctx* C, int64_t n, float* LU, int64_t lda, int64_t* ipiv, int32_
   });
 }
 
+int mp_pack_operand(mp_ctx* C, int32_t fmt, int32_t tile_scale, const float* S, int64_t s_idx, int64_t s_k, int32_t nidx,
+                    int32_t K, void* out, void* out_lo, int32_t* exps, uint64_t* counts) {
+  return guard([&] {
+    MP_REQUIRE(C, "ctx is NULL");
+    pack_operand(fmt, tile_scale != 0, S, s_idx, s_k, nidx, K, out, out_lo, exps, (unsigned long long*)counts, C->W);
+  });
+}
+
+int mp_problem_factor_bytes(mp_problem* P, uint64_t* bytes) {
+  return guard([&] {
+    MP_REQUIRE(P && bytes, "NULL argument");
+    uint64_t b = 0;
+    if (P->fcache) {
+      const Factors& F = *P->fcache;
+      b = (uint64_t)F.LU32.n * 4 + F.LU64.n * 8 + F.LU64p.n * 8 + F.ipiv.n * 8 + F.perm.n * 4 + F.ipiv32.n * 4 +
+          F.dr.n * 8 + F.dc.n * 8 + F.rs.n * 8;
+    }
+    *bytes = b;
+  });
+}
+
 int mp_getrs(mp_ctx* C, int64_t n, int32_t nrhs, const float* LU, int64_t lda, const int32_t* perm, const double* R, int64_t ldr,
              double* D, int64_t ldd, double scale) {
-  return guard([&] { MP_REQUIRE(C, "ctx is NULL"); getrs(n, nrhs, LU, lda, perm, R, ldr, D, ldd, scale, C->W); });
+  return guard([&] {
+    MP_REQUIRE(C, "ctx is NULL");
+    // word 2 of the info area: the substitutions' exchange-timeout flag, cleared before and read after
+    C->W.dinfo.ensure(C->tracker, 4, "getrf_info");
+    HIP_CK(hipMemsetAsync(C->W.dinfo.p + 2, 0, 4, C->W.stream));
+    getrs(n, nrhs, LU, lda, perm, R, ldr, D, ldd, scale, C->W);
+    int32_t timeout = 0;
+    HIP_CK(hipMemcpyAsync(&timeout, C->W.dinfo.p + 2, 4, hipMemcpyDeviceToHost, C->W.stream));
+    HIP_CK(hipStreamSynchronize(C->W.stream));
+    if (timeout) throw MPError("getrs: the block exchange of the substitutions timed out (D is not valid)", MP_ERR_INTERNAL);
+  });
 }
 
 int mp_problem_from_host(mp_ctx* C, int64_t n, int32_t nrhs, int32_t a_bits, const void* A, int64_t lda, const double* B, int64_t ldb,
@@ -1567,8 +1605,18 @@ int mp_solve(mp_ctx* C, mp_problem* P, const char* method_id, const char* params
         // admission (src/mpsolve.cu:182-190), with an injectable budget
         const size_t nn = (size_t)n * n;
         size_t need = (M.factor == "cs64") ? nn * 8 : nn * 4;
+        if (M.apply64) need += nn * 8;
         if (M.policy != "none") need += nn * 8;
         if (M.gmres) need += (size_t)n * (2 * M.gmres_m + 2) * 8;
+        // factor buffers a reused problem already holds are not allocated again
+        size_t have = 0;
+        if (a.getb("reuse", false) && P->fcache) {
+          const Factors& F = *P->fcache;
+          if (M.factor == "cs64" && F.LU64.n >= nn) have += nn * 8;
+          if (M.factor != "cs64" && F.LU32.n >= nn) have += nn * 4;
+          if (M.apply64 && F.LU64p.n >= nn) have += nn * 8;
+        }
+        need = need > have ? need - have : 0;
         size_t fr = 0, tot = 0; HIP_CK(hipMemGetInfo(&fr, &tot));
         int64_t avail = (int64_t)fr - (int64_t)(512ull << 20);
         if (a.has("hbm_budget_bytes")) avail = std::min<int64_t>(avail, (int64_t)a.getd("hbm_budget_bytes", 0) - (int64_t)C->tracker.current);
@@ -1628,7 +1676,7 @@ int mp_solve(mp_ctx* C, mp_problem* P, const char* method_id, const char* params
      .s("timestamp", a.get("timestamp", "")).i("nb", M.nb).i("gmres_m", M.gmres_m).d("gmres_tol", M.gmres_tol).i("gmres_budget", M.gmres_budget)
      .i("gmres_cycles", M.gmres_cycles).i("max_outer", M.max_outer).s("policy", M.policy).i("budget", M.budget).i("scale_mode", M.scale_mode)
      .i("working", M.working).i("resid", M.resid == 128 ? 64 : M.resid).b("resid_dd", M.resid == 128).b("apply64", M.apply64)
-     .b("reuse", a.getb("reuse", false)).s("panel", "hip_lds_exchange").s("panel_requested", a.get("panel_requested", "cusolver")).i("stages_count", (int64_t)T.stages.size())
+     .b("reuse", a.getb("reuse", false)).s("panel", "hip_lds_exchange").i("panel_launches", T.panels).i("panel_coop_refused", T.panel_coop_refused).s("panel_requested", a.get("panel_requested", "cusolver")).i("stages_count", (int64_t)T.stages.size())
      .s("stop_on", eta_only ? "eta" : "eta_omega");
     C->last_record = j.obj();
     std::string h;

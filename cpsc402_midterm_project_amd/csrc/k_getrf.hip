@@ -1460,6 +1460,30 @@ static void pack_U(float* A, int64_t lda, int64_t k, int kb, int64_t cb, int64_t
   W.prof_end(evp, pname, (4.0 + ES) * (double)nidx * kb, 0.0);
 }
 
+// One operand pack on its own (the conversion step of the trailing update, exposed for parity checks):
+// nidx x K source elements S[idx * sI + kk * sK] into tiles of 32 indices in MFMA fragment order.
+void pack_operand(int fmt, bool tile_scale, const float* S, int64_t sI, int64_t sK, int nidx, int K, void* out,
+                  void* out_lo, int32_t* exps, unsigned long long* counts, Workspace& W) {
+  MP_REQUIRE(S && out && exps && counts, "pack: NULL argument");
+  MP_REQUIRE(nidx > 0 && K > 0 && K <= 256, "pack: need nidx > 0 and 0 < K <= 256");
+  MP_REQUIRE((sI == 1 && sK >= nidx) || (sK == 1 && sI >= K), "pack: one of the strides must be 1, the other >= the extent");
+  MP_REQUIRE(fmt >= FMT_FP16 && fmt <= FMT_FP16X3, "pack: unknown format");
+  MP_REQUIRE(fmt != FMT_FP16X3 || out_lo, "pack: the two-term format needs out_lo");
+  const int nkb = (K + 15) / 16;
+  const int nT = (int)cdiv(nidx, 32);
+  const size_t lds = (size_t)32 * (nkb * 16 + 1) * sizeof(float);
+  const int ts = tile_scale ? 1 : 0;
+  uint8_t* o = (uint8_t*)out; uint8_t* ol = (uint8_t*)out_lo;
+  switch (fmt) {
+    case FMT_FP16: hipLaunchKernelGGL((k_pack<FMT_FP16>), dim3(nT), dim3(256), lds, W.stream, S, sI, sK, nidx, K, nkb, o, nullptr, exps, counts, ts); break;
+    case FMT_BF16: hipLaunchKernelGGL((k_pack<FMT_BF16>), dim3(nT), dim3(256), lds, W.stream, S, sI, sK, nidx, K, nkb, o, nullptr, exps, counts, ts); break;
+    case FMT_FP8: hipLaunchKernelGGL((k_pack<FMT_FP8>), dim3(nT), dim3(256), lds, W.stream, S, sI, sK, nidx, K, nkb, o, nullptr, exps, counts, ts); break;
+    case FMT_FP32: hipLaunchKernelGGL((k_pack<FMT_FP32>), dim3(nT), dim3(256), lds, W.stream, S, sI, sK, nidx, K, nkb, o, nullptr, exps, counts, ts); break;
+    default: hipLaunchKernelGGL((k_pack<FMT_FP16X3>), dim3(nT), dim3(256), lds, W.stream, S, sI, sK, nidx, K, nkb, o, ol, exps, counts, ts); break;
+  }
+  LAUNCH_CK();
+}
+
 // A22[:, cb .. ce) -= L21 * U12[:, cb .. ce)
 template <int FMT>
 static void gemm_cols(float* A, int64_t lda, int64_t k, int kb, int64_t m2, int64_t cb, int64_t ce, Workspace& W, const char* gname,
@@ -1556,7 +1580,7 @@ static void run_panel(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int32
     if (cooperative) {
       launched = hipLaunchCooperativeKernel((const void*)k_panel2, dim3((unsigned)G2), dim3(256), args,
                                             (unsigned)panel2_lds_bytes(kb), stream) == hipSuccess;
-      if (!launched) (void)hipGetLastError();
+      if (!launched) { (void)hipGetLastError(); W.coop_refused++; }
     }
     if (!launched) {
       // overlapped with trailing-update kernels that never wait on anything: workgroups that are not yet
@@ -1586,6 +1610,7 @@ static void run_panel(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int32
   hipEvent_t e0 = W.prof_begin(stream);
   if (hipLaunchCooperativeKernel((const void*)k_panel, dim3(G), dim3(256), args, (unsigned)lds, stream) != hipSuccess) {
     (void)hipGetLastError();      // refused while another process shares the device: bounded waits, as above
+    W.coop_refused++;
     HIP_CK(hipLaunchKernel((const void*)k_panel, dim3(G), dim3(256), args, lds, stream));
   }
   W.prof_end(e0, "panel", 8.0 * (double)m * kb, (double)m * kb * kb, stream);
@@ -1631,11 +1656,14 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   MP_REQUIRE(a.fmt >= FMT_FP16 && a.fmt <= FMT_FP16X3, "getrf: unknown trailing format");
   const int nb = a.nb;
   W.factor_gen++;
+  W.coop_refused = 0;
   W.dinfo.ensure(*W.tracker, 4, "getrf_info");
   W.cnt.ensure(*W.tracker, 2, "getrf_counts");
   W.affect.ensure(*W.tracker, 2 * PLAN_WORDS, "swap_plan");
   W.linv.ensure(*W.tracker, (size_t)cdiv(nb, 32) * 1024, "trsm_block_inverses");
-  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, W.stream, W.dinfo.p, 4);
+  // words 0-1 (info, panel exchange timeout) belong to this factorization; word 2, the substitutions' timeout
+  // flag, is owned by whoever runs them (a refinement may refactorize between substitutions) and stays set
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, W.stream, W.dinfo.p, 2);
   // both swap plans start empty (later each is cleared by the interchange launch of the step before its panel)
   hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)cdiv(2 * PLAN_WORDS, 256)), dim3(256), 0, W.stream, W.affect.p, 2 * PLAN_WORDS);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, W.stream, W.cnt.p, 2);
@@ -1727,7 +1755,10 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   HIP_CK(hipMemcpyAsync(hinfo, W.dinfo.p, sizeof(hinfo), hipMemcpyDeviceToHost, W.stream));
   HIP_CK(hipMemcpyAsync(hc, W.cnt.p, sizeof(hc), hipMemcpyDeviceToHost, W.stream));
   HIP_CK(hipStreamSynchronize(W.stream));
-  if (out) { out->info = hinfo[0]; out->timeout = hinfo[1]; out->overflow = hc[0]; out->subnormal = hc[1]; out->panels = panels; }
+  if (out) {
+    out->info = hinfo[0]; out->timeout = hinfo[1]; out->overflow = hc[0]; out->subnormal = hc[1]; out->panels = panels;
+    out->coop_refused = W.coop_refused;
+  }
   if (hinfo[1]) throw MPError("getrf: panel exchange timed out (workgroups not co-resident?)", -5);
 }
 

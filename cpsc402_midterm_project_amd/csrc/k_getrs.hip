@@ -48,9 +48,10 @@ __device__ __forceinline__ void gs_sleep(int dist) {
 }
 // The word loaded ahead (`first`) is checked before the polling loop: the loop re-loads, and at its head the
 // compiler can only wait for every outstanding load, which would also wait for the prefetch in flight.
-__device__ __forceinline__ float gs_wait(const uint64_t* p, uint64_t first, uint32_t tag, volatile lds_i32* s_abort, int dist) {
+__device__ __forceinline__ float gs_wait(const uint64_t* p, uint64_t first, uint32_t tag, volatile lds_i32* s_abort, int dist,
+                                        long spin) {
   if ((uint32_t)(first >> 32) == tag) return __uint_as_float((uint32_t)first);
-  for (long it = 0; it < GS_SPIN_MAX; it++) {
+  for (long it = 0; it < spin; it++) {
     if (*s_abort) return 0.0f;
     gs_sleep(dist);
     const uint64_t w = gs_load(p);
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(512) void k_trsv2(const float* __restrict__ LU, int
                                                const double* __restrict__ Rin, int64_t ldr,
                                                float* __restrict__ Y, int64_t ldy,
                                                double* __restrict__ D, int64_t ldd, double scale,
-                                               uint64_t* __restrict__ ybuf, uint32_t tag, int32_t* __restrict__ err) {
+                                               uint64_t* __restrict__ ybuf, uint32_t tag, int32_t* __restrict__ err, long spin) {
   __shared__ float ys[3][NR][GB];
   __shared__ float part[4][NR][GB];
   __shared__ int s_abort;
@@ -229,7 +230,7 @@ __global__ __launch_bounds__(512) void k_trsv2(const float* __restrict__ LU, int
     const int bj = blk(s2);
     if (t < GB) {
 #pragma unroll
-      for (int c = 0; c < NR; c++) ys[s2 % 3][c][t] = gs_wait(ybuf + ((int64_t)bj * NR + c) * GB + t, y[c], tag, LDS_FLAG(&s_abort), ndep - 1 - s2);
+      for (int c = 0; c < NR; c++) ys[s2 % 3][c][t] = gs_wait(ybuf + ((int64_t)bj * NR + c) * GB + t, y[c], tag, LDS_FLAG(&s_abort), ndep - 1 - s2, spin);
     }
     __syncthreads();
 #pragma unroll
@@ -278,7 +279,7 @@ __global__ __launch_bounds__(512) void k_trsv2(const float* __restrict__ LU, int
 #pragma unroll
       for (int c = 0; c < NR; c++) {
         const uint64_t* p = ybuf + ((int64_t)bj * NR + c) * GB + t;
-        ys[sc][c][t] = gs_wait(p, gs_load(p), tag, LDS_FLAG(&s_abort), 0);
+        ys[sc][c][t] = gs_wait(p, gs_load(p), tag, LDS_FLAG(&s_abort), 0, spin);
       }
     }
     __syncthreads();                                  // also orders the reads of part above before the writes below
@@ -322,10 +323,10 @@ static void trsv_pair(int64_t n, const float* LU, int64_t lda, const int32_t* pe
   const float* Mu = W.tchain.p + (size_t)nbk * GB * GB;
   hipEvent_t ev = W.prof_begin();
   hipLaunchKernelGGL((k_trsv2<NR, true>), dim3(nbk), dim3(512), 0, W.stream, LU, lda, n, Ti, Mi, perm, R, ldr,
-                     W.rhs32.p, n, (double*)nullptr, (int64_t)0, 1.0, W.ybuf.p, tl, err);
+                     W.rhs32.p, n, (double*)nullptr, (int64_t)0, 1.0, W.ybuf.p, tl, err, W.gs_spin);
   LAUNCH_CK();
   hipLaunchKernelGGL((k_trsv2<NR, false>), dim3(nbk), dim3(512), 0, W.stream, LU, lda, n, Tu, Mu, perm, (const double*)nullptr, (int64_t)0,
-                     W.rhs32.p, n, D, ldd, scale, W.ybuf.p, tu, err);
+                     W.rhs32.p, n, D, ldd, scale, W.ybuf.p, tu, err, W.gs_spin);
   LAUNCH_CK();
   W.prof_end(ev, "getrs", 4.0 * (double)n * n + 24.0 * n * NR, 2.0 * (double)n * n * NR);
 }
@@ -341,6 +342,23 @@ void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* per
   W.dinfo.ensure(*W.tracker, 4, "getrf_info");
   W.tinv.ensure(*W.tracker, (size_t)2 * nbk * GB * GB, "getrs_block_inverses");
   W.tchain.ensure(*W.tracker, (size_t)2 * nbk * GB * GB, "getrs_chain_matrices");
+  // bound on the polls of one exchange wait.  A smaller bound only lets a test force the timeout path from the
+  // host: MPSOLVE_GS_SPIN = "bound" for every call, or "bound:calls" for the next `calls` calls after the
+  // variable takes a new value (every wait still ends; blocks that give up publish zeros and set the flag).
+  W.gs_spin = GS_SPIN_MAX;
+  if (const char* e = getenv("MPSOLVE_GS_SPIN")) {
+    if (W.gs_spin_env != e) {
+      W.gs_spin_env = e;
+      const char* c = strchr(e, ':');
+      W.gs_spin_calls = c ? atol(c + 1) : -1;
+    }
+    if (W.gs_spin_calls != 0) {
+      W.gs_spin = std::max(1L, std::min(GS_SPIN_MAX, atol(e)));
+      if (W.gs_spin_calls > 0) W.gs_spin_calls--;
+    }
+  } else {
+    W.gs_spin_env.clear();
+  }
   if (!reuse_inverse || W.tinv_lu != LU || W.tinv_n != n || W.tinv_lda != lda || W.tinv_gen != W.factor_gen) {
     static bool attr = false;
     const size_t lds = (size_t)(GB * (GB + 1) + GB * GB) * 4;

@@ -36,6 +36,7 @@ struct Workspace {
   KernelProfiler prof;
   DevTracker* tracker = nullptr;
   int num_cus = 256;
+  int coop_refused = 0;   // cooperative panel launches refused by the runtime in the current factorization
   DBuf<double> part;     // residual / absmax partial sums
   DBuf<double> statp;    // residual metric partials
   // getrf
@@ -47,7 +48,7 @@ struct Workspace {
   // on the panel stream while this step's remaining trailing update still reads its own
   DBuf<uint8_t> packL2, packLlo2; DBuf<int32_t> scaleL2;
   DBuf<unsigned long long> cnt; // [0] overflow, [1] subnormal counts of the operand conversion
-  DBuf<int32_t> dinfo;   // [0] info, [1] exchange timeout flag
+  DBuf<int32_t> dinfo;   // [0] info, [1] panel exchange timeout flag, [2] substitution exchange timeout flag
   DBuf<int32_t> affect;  // per-panel swap plan (src,dst pairs)
   DBuf<float> linv;      // inverses of the 32x32 diagonal blocks of L11
   // getrs
@@ -55,6 +56,8 @@ struct Workspace {
   DBuf<float> rhs32;
   DBuf<float> tinv;      // inverses of the 128 x 128 diagonal blocks of L and U
   DBuf<float> tchain;    // chain matrices Linv_i L_{i,i-1} and Uinv_i U_{i,i+1} of the blocked substitution
+  long gs_spin = 1L << 22;   // poll bound of one exchange wait of the substitutions
+  std::string gs_spin_env; long gs_spin_calls = -1;   // host-side override of that bound (tests only)
   const float* tinv_lu = nullptr; int64_t tinv_n = 0, tinv_lda = 0; uint64_t tinv_gen = 0;
   // batched small systems
   DBuf<uint8_t> bsys;    // per-system descriptors in processing order
@@ -106,8 +109,12 @@ struct GetrfArgs {
   int64_t* ipiv = nullptr;       // 1-based global LAPACK pivots (device)
   int32_t* perm = nullptr;       // composed row permutation: row i of P*A is row perm[i] of A (device)
 };
-struct GetrfInfo { int info = 0; int timeout = 0; unsigned long long overflow = 0, subnormal = 0; int panels = 0; };
+// coop_refused: panels whose cooperative launch the runtime refused (another process on the device), which then
+// ran as ordinary launches with bounded exchange waits
+struct GetrfInfo { int info = 0; int timeout = 0; unsigned long long overflow = 0, subnormal = 0; int panels = 0; int coop_refused = 0; };
 void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out);
+void pack_operand(int fmt, bool tile_scale, const float* S, int64_t sI, int64_t sK, int nidx, int K, void* out,
+                  void* out_lo, int32_t* exps, unsigned long long* counts, Workspace& W);
 
 // d = (LU)^{-1} P r  with FP32 factors and FP32 arithmetic, FP64 input/output; d is scaled by `scale`.
 // reuse_inverse: the diagonal-block inverses computed for the same LU buffer since the last getrf are reused.

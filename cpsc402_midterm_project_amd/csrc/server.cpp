@@ -171,7 +171,8 @@ struct mp_server {
         if (P) mp_problem_free(P);
         finish(r, 1, hit ? 1 : 0, 5, 0, -1, -1, t_done);
       } else {
-        const size_t fb = (size_t)r.n * r.n * (std::strcmp(out.final_stage, "cs64") == 0 ? 8 : 4);   // factors kept
+        uint64_t fb = (uint64_t)r.n * r.n * (std::strcmp(out.final_stage, "cs64") == 0 ? 8 : 4);   // factors kept
+        (void)mp_problem_factor_bytes(P, &fb);
         cache[r.matrix] = Entry{P, fb, ++tick};
         cache_bytes += fb;
         finish(r, 1, hit ? 1 : 0, out.ok ? terminal_code(out.terminal_reason) : 5, out.outer_iters, out.eta_max,

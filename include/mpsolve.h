@@ -86,6 +86,19 @@ typedef struct {
  * Replaces custom_blocked_lu (src/mp_core.cuh:129-150). */
 int mp_getrf(mp_ctx* ctx, int64_t n, float* LU, int64_t lda, int64_t* ipiv, int32_t* perm,
              const mp_getrf_params* params, mp_getrf_info* info);
+/* The operand conversion of the trailing update on its own (enqueued on the context stream): replaces
+ * k_f32_to_f16_2d (src/mp_kernels.cuh:14-18), the round-to-nearest FP32 -> FP16 pack of L21 and U12 with its
+ * overflow (|x| > 65504) and subnormal (0 < |x| < 2^-14) counters.  Source element (idx, kk) is
+ * S[idx * s_idx + kk * s_k] (device FP32; one stride 1, the other at least the extent), idx < nidx, kk < K <= 256.
+ * Each tile of 32 indices is scaled by 2^s (s = exps[tile], chosen from the tile's largest |x| so it lands in
+ * the top binade below the format's maximum; s = 0 when tile_scale is 0 or for fmt 3) and rounded to nearest
+ * even into fmt (0 FP16, 1 BF16, 2 OCP FP8-e4m3 saturating, 3 FP32, 4 two-term FP16: out = fp16(x), out_lo =
+ * fp16(x - out)).  Output element e of lane l of k-block b of tile t sits at ((t * nkb + b) * 64 + l) * 8 + e
+ * (element size of fmt, nkb = ceil(K / 16)) and holds idx = 32 t + l % 32 and kk = 16 b + 8 (l / 32) + e
+ * (FP32: kk = 16 b + 2 e + l / 32); positions past nidx or K hold 0.  counts[0] / counts[1] (device, caller
+ * zeroes) are incremented by the scaled values above the format's maximum / in its subnormal range. */
+int mp_pack_operand(mp_ctx* ctx, int32_t fmt, int32_t tile_scale, const float* S, int64_t s_idx, int64_t s_k, int32_t nidx,
+                    int32_t K, void* out, void* out_lo, int32_t* exps, uint64_t* counts);
 /* Plain FP64 LU (P A = L U in place, A n x n FP64 device, column-major lda): the comparator the mixed-precision
  * solve is measured against (native_lu64, src/mpsolve.cu:22 and src/mp_core.cuh:162-170, cusolverDnXgetrf).
  * Blocked by nb (a multiple of 4): the vendor panel (rocSOLVER dgetrf on m x nb), rocBLAS dtrsm for U12, and
@@ -93,7 +106,9 @@ int mp_getrf(mp_ctx* ctx, int64_t n, float* LU, int64_t lda, int64_t* ipiv, int3
  * (device); *info: first zero pivot (1-based) or 0. */
 int mp_getrf64(mp_ctx* ctx, int64_t n, double* A, int64_t lda, int32_t* ipiv, int32_t nb, int32_t* info);
 /* D = scale * U^{-1} L^{-1} P R with FP32 factors and FP32 arithmetic, FP64 R and D (device).
- * Replaces apply_inverse for FP32 factors (src/mp_core.cuh:182-190). */
+ * Replaces apply_inverse for FP32 factors (src/mp_core.cuh:182-190).  Returns MP_ERR_INTERNAL when a wait of
+ * the substitutions' block exchange reached its bound (workgroups that never became resident); D is then not
+ * valid.  Synchronizes the context stream. */
 int mp_getrs(mp_ctx* ctx, int64_t n, int32_t nrhs, const float* LU, int64_t lda, const int32_t* perm,
              const double* R, int64_t ldr, double* D, int64_t ldd, double scale);
 
@@ -161,6 +176,11 @@ typedef struct {
 } mp_trial_out;
 int mp_solve(mp_ctx* ctx, mp_problem* p, const char* method_id, const char* params_json,
              mp_trial_out* out, double* X_host);
+/* Device bytes held by the factors a problem keeps for "reuse" solves (0 when it keeps none).  The reference
+ * tracks every device buffer in one global tracker (g_tracker, src/mp_common.cuh:71) and subtracts reusable
+ * factor buffers in its admission estimate (src/mpsolve.cu:182-187); a factor cache of many problems needs
+ * the per-problem figure. */
+int mp_problem_factor_bytes(mp_problem* p, uint64_t* bytes);
 int mp_last_record_json(mp_ctx* ctx, char* json, size_t cap);
 int mp_last_hist_json(mp_ctx* ctx, char* jsonl, size_t cap);
 /* precision contracts of every method ID (methods_catalog.py:50-100), as a JSON array */

@@ -15,6 +15,8 @@
  *   or_getrs       apply_inverse      src/mp_core.cuh:182-190    FP64 -> FP32 cast, FP32 substitution, cast back, scale
  *   or_round_*     __float2half / the FP16 conversion of src/mp_kernels.cuh:16; BF16 and OCP FP8-e4m3 follow the
  *                  same round-to-nearest-even rule from the format definitions.
+ *   or_pack        k_f32_to_f16_2d    src/mp_kernels.cuh:14-18   one operand's conversion with its overflow / subnormal
+ *                  counters, after the per-32-index-tile power-of-two scale the trailing update of or_getrf applies
  * Operation order inside the panel and the TRSM is the one the HIP kernels use (documented per function), so
  * exact-arithmetic inputs give bitwise-identical factors.
  */
@@ -133,6 +135,31 @@ static int tile_exp(int fmt, float tmax, int tile_scale) {
   int s = top_exp(fmt) - (e - 1);
   if (s < -120) s = -120; if (s > 120) s = 120;
   return s;
+}
+
+/* One operand's conversion: S is nidx x K, element (i, q) at S[i * K + q].  Tiles of 32 indices are scaled by
+ * 2^exps[t] (tile_exp of the tile's largest |x|, NaN counting as the largest) and rounded to fmt; q_out / qlo_out
+ * (same layout) receive the rounded values as FP32 (qlo: the low FP16 term of format 4, else 0); counts[0] /
+ * counts[1] are incremented by the scaled values above the format's maximum / in its subnormal range, as the
+ * reference's pack counts them (src/mp_kernels.cuh:17) and as or_getrf does per trailing update. */
+void or_pack(int fmt, int tile_scale, int64_t nidx, int K, const float* S, float* q_out, float* qlo_out, int* exps,
+             unsigned long long* counts) {
+  for (int64_t t = 0; t * 32 < nidx; t++) {
+    float m = 0.0f;
+    for (int64_t i = t * 32; i < t * 32 + 32 && i < nidx; i++)
+      for (int q = 0; q < K; q++) { const float v = fabsf(S[i * K + q]); if (v != v || v > m) m = v; }
+    exps[t] = tile_exp(fmt, m, tile_scale);
+    const float sc = ldexpf(1.0f, exps[t]);
+    for (int64_t i = t * 32; i < t * 32 + 32 && i < nidx; i++)
+      for (int q = 0; q < K; q++) {
+        const float v = S[i * K + q] * sc;
+        const float av = fabsf(v);
+        if (av > fmt_max(fmt)) counts[0]++; else if (av > 0.0f && av < fmt_minnorm(fmt)) counts[1]++;
+        const float hv = round_fmt(fmt, v);
+        q_out[i * K + q] = hv;
+        qlo_out[i * K + q] = (fmt == 4) ? or_round_fp16(v - hv) : 0.0f;
+      }
+  }
 }
 
 /* ------------------------------------------------------------------ blocked LU

@@ -42,6 +42,7 @@ def _load():
     lib.or_getrf.argtypes = [i64, C.c_int, vp, i64, vp, vp, C.c_int, C.c_int, vp]
     lib.or_getrf.restype = C.c_int
     lib.or_getrs.argtypes = [i64, C.c_int, vp, i64, vp, vp, i64, vp, i64, C.c_double]
+    lib.or_pack.argtypes = [C.c_int, C.c_int, i64, C.c_int, vp, vp, vp, vp, vp]
     for f in ("or_round_fp16", "or_round_bf16", "or_round_e4m3"):
         getattr(lib, f).argtypes = [C.c_float]
         getattr(lib, f).restype = C.c_float
@@ -113,6 +114,19 @@ def round_fmt(x, fmt):
     f = {"fp16": lib().or_round_fp16, "bf16": lib().or_round_bf16, "fp8": lib().or_round_e4m3}[fmt]
     x = np.asarray(x, dtype=np.float32)
     return np.array([f(float(v)) for v in x.ravel()], dtype=np.float32).reshape(x.shape)
+
+
+def pack(S, fmt="fp16", tile_scale=True):
+    """One operand's conversion (or_pack; src/mp_kernels.cuh:14-18 with the per-tile power-of-two scale): S is
+    nidx x K (index-major).  Returns the rounded scaled values (nidx x K, FP32), the low terms of the two-term
+    format (zeros otherwise), the tile exponents and the [overflow, subnormal] counts."""
+    S = np.ascontiguousarray(S, dtype=np.float32)
+    nidx, K = S.shape
+    q = np.zeros_like(S); qlo = np.zeros_like(S)
+    ex = np.zeros((nidx + 31) // 32, dtype=np.int32)
+    cnt = np.zeros(2, dtype=np.uint64)
+    lib().or_pack(FMT[fmt], 1 if tile_scale else 0, nidx, K, _p(S), _p(q), _p(qlo), _p(ex), _p(cnt))
+    return q, qlo, ex, cnt
 
 
 def unpack_lu(LU):
@@ -203,13 +217,25 @@ def ir_stationary(A, B, fmt="fp16", nb=128, tau=1e-15, max_outer=50, dd=False, s
     """Stationary IR as in Engine::refine_stage (src/mp_core.cuh:281-296): x0 = A^-1 b, then residual+metrics,
     stopping tests (converged / nonfinite / max_outer / stagnation), correction, x += d."""
     F = factor(A, fmt, nb, scale_mode, tile_scale)
+    return ir_loop(A, B, F, tau=tau, max_outer=max_outer, dd=dd, stop_eta_only=stop_eta_only)
+
+
+def ir_loop(A, B, F, tau=1e-15, max_outer=50, dd=False, stop_eta_only=True, metric=None, apply=None):
+    """The loop of Engine::refine_stage (src/mp_core.cuh:283-296) over given factors F (a factor() dict, e.g.
+    built around another factorization's LU): x0 = F^-1 b; per step the metrics of x, then in this order
+    converged (finite, eta <= tau and, unless stop_eta_only, omega <= tau), nonfinite, max_outer, stagnation
+    (eta > 0.9 * previous eta for 3 consecutive steps, src/mp_core.cuh:290), then x += F^-1 r.
+    metric(x) -> dict(eta_max, omega_max, rinf, xinf) and apply(R) -> F^-1 R may be replaced (scripted
+    sequences that pin the stopping rules).  Returns (x, terminal reason, eta history)."""
     n = A.shape[0]
     B = np.asarray(B, dtype=np.float64).reshape(n, -1)
-    x = apply_inverse(F, B)
+    metric = metric or (lambda x: metrics(A, B, x, dd=dd))
+    apply = apply or (lambda R: apply_inverse(F, R))
+    x = apply(B)
     hist = []
     prev, stall, outer = np.inf, 0, 0
     while True:
-        m = metrics(A, B, x, dd=dd)
+        m = metric(x)
         hist.append(m["eta_max"])
         finite = np.isfinite(m["rinf"]).all() and np.isfinite(m["xinf"]).all()
         if finite and m["eta_max"] <= tau and (stop_eta_only or m["omega_max"] <= tau):
@@ -226,7 +252,7 @@ def ir_stationary(A, B, fmt="fp16", nb=128, tau=1e-15, max_outer=50, dd=False, s
             Rh, Rl = resid_dd(A, x, B); R = Rh + Rl
         else:
             R, _ = resid_fp64(A, x, B)
-        x = x + apply_inverse(F, R)
+        x = x + apply(R)
         outer += 1
 
 

@@ -49,6 +49,52 @@ def test_c0_stationary_ir_agrees_with_cpu_restatement(ctx, oracle, kappa):
     P.close()
 
 
+def test_c0_factors_versus_loop(ctx, oracle):
+    """C0 at kappa=1e4: separates the factorization from the refinement loop.  The GPU factors of the solve's
+    own FP32 matrix (fl32(A) * g, g from the FP32 maximum, src/mp_core.cuh:167) are exported through mp_getrf
+    and run through the oracle's loop (ir_loop: src/mp_core.cuh:283-296 with FP32 substitution); that eta
+    history must follow the GPU's own run step by step (the two loops differ only in the substitutions'
+    summation order), so any gap to the oracle's run on the oracle's factors is the factors'.  Both
+    factorizations are backward stable at the FP16-operand level; their backward errors are reported."""
+    n, nb = 1024, 128
+    A, b, xt, _ = randsvd(n, 1e4, 9200, gpu=False)
+    print(f"[C0 factors vs loop] n={n} kappa=1e4 nb={nb}", flush=True)
+    Fo = oracle.factor(A, "fp16", nb)
+    A32 = np.asfortranarray(np.asarray(A).astype(np.float32) * np.float32(Fo["g"]))
+    dLU = ctx.buffer(A32); dip, dpm = ctx.empty(8 * n), ctx.empty(4 * n)
+    info = ctx.getrf(n, dLU, n, dip, dpm, nb=nb, fmt="fp16", tile_scale=True)
+    assert info.info == 0
+    Fg = dict(Fo); Fg["LU"] = dLU.to_array((n, n), np.float32); Fg["perm"] = dpm.to_array((n,), np.int32)
+    berr = {}
+    for name, F in (("gpu", Fg), ("oracle", Fo)):
+        L, U = oracle.unpack_lu(F["LU"])
+        berr[name] = np.abs(A32[F["perm"]].astype(np.float64) - L @ U).max() / np.abs(A32).max()
+    agree = float(np.mean(Fg["perm"] == Fo["perm"]))
+    _, r_gl, h_gl = oracle.ir_loop(np.asfortranarray(A), b, Fg, tau=1e-15, max_outer=50)
+    _, r_ol, h_ol = oracle.ir_loop(np.asfortranarray(A), b, Fo, tau=1e-15, max_outer=50)
+    P = ctx.problem(A, b, x_exact=xt)
+    X, rec = P.solve("mx16_ir_w64_r64", nb=nb, max_outer=50, tau=1e-15, stop_on="eta")
+    h_gpu = [h["eta_max"] for h in P.history() if h.get("kind") == "outer"]
+    P.close()
+    rate = lambda h: (h[min(len(h), 15) - 1] / h[0]) ** (1.0 / (min(len(h), 15) - 1))
+    print(f"[C0] backward error of the factors: gpu {berr['gpu']:.2e} oracle {berr['oracle']:.2e}; "
+          f"perm agreement {agree:.3f}", flush=True)
+    print(f"[C0] gpu factors + gpu loop   : {rec['terminal_reason']} {len(h_gpu) - 1} steps, eta {h_gpu[0]:.2e} -> "
+          f"{min(h_gpu):.2e}, rate {rate(h_gpu):.3f}", flush=True)
+    print(f"[C0] gpu factors + oracle loop: {r_gl} {len(h_gl) - 1} steps, eta {h_gl[0]:.2e} -> {min(h_gl):.2e}, "
+          f"rate {rate(h_gl):.3f}", flush=True)
+    print(f"[C0] oracle factors + loop    : {r_ol} {len(h_ol) - 1} steps, eta {h_ol[0]:.2e} -> {min(h_ol):.2e}, "
+          f"rate {rate(h_ol):.3f}", flush=True)
+    # the two loops over the GPU's factors: same initial eta to rounding, same history within a factor 2 per step
+    assert abs(h_gpu[0] / h_gl[0] - 1) < 0.05, (h_gpu[0], h_gl[0])
+    k = min(len(h_gpu), len(h_gl), 15)
+    ratio = np.array(h_gpu[:k]) / np.array(h_gl[:k])
+    assert np.all((ratio > 0.5) & (ratio < 2.0)), ratio
+    # both factorizations are backward stable at the level of FP16 operands with FP32 accumulation
+    for v in berr.values():
+        assert v <= 4 * n * 2.0 ** -11 / 32, berr
+
+
 @pytest.mark.parametrize("method,kappa,seed", [("mx16x3_gmresir_w64_rdd", 1e6, 9201), ("mxbf16_gmresir_w64_rdd", 1e2, 9200)])
 def test_c1_headline_shape_by_oracle(ctx, oracle, method, kappa, seed):
     """C1 at n=16384 (randsvd generated on the GPU, the bench's seeds): the headline method at kappa=1e6 and the

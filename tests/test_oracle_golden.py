@@ -348,3 +348,104 @@ def test_left_gmres_pins(oracle):
     x, reason, hist, inner = oracle.fgmres_ir(np.asfortranarray(B), b, fmt="fp16", nb=64, tau=1e-15, m=40, left=True,
                                               cycles=1)
     assert reason == "converged" and inner > 0
+
+
+def test_metrics_hand_worked_2x2(oracle):
+    """eta = ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf) and omega = max_i |r_i| / (|A||x| + |b|)_i
+    (src/mp_core.cuh:198-202, src/mp_kernels.cuh:92-96) on A = [[2, -1], [3, 1]], x = (1, 2), b = (1, 4):
+    r = (1, -1), ||A||_inf = 4 (row sums 3, 4; the 1-norm would be 5), |A||x| + |b| = (5, 9), so eta = 1/12
+    and omega = 1/5, the worst row 0.  Norms taken by columns, or over x in the 1-norm, give other values."""
+    A = np.array([[2.0, -1.0], [3.0, 1.0]], order="F")
+    x = np.array([1.0, 2.0]); b = np.array([1.0, 4.0])
+    m = oracle.metrics(A, b, x)
+    assert m["eta_max"] == 1.0 / 12.0 and m["omega_max"] == 1.0 / 5.0
+    assert m["rinf"][0] == 1.0 and m["xinf"][0] == 2.0 and m["worst"][0] == 0
+    md = oracle.metrics(A, b, x, dd=True)
+    assert md["eta_max"] == 1.0 / 12.0 and md["omega_max"] == 1.0 / 5.0
+    # an exact solution has eta = omega = 0 (ri == 0 branch), a zero denominator gives inf
+    m0 = oracle.metrics(A, A @ x, x)
+    assert m0["eta_max"] == 0.0 and m0["omega_max"] == 0.0
+
+
+def _scripted_loop(oracle, etas, tau=1e-15, max_outer=50):
+    """ir_loop driven by a scripted eta sequence: the metric of the k-th evaluation is etas[k]."""
+    A = np.eye(2, order="F"); b = np.ones(2)
+    seq = iter(etas)
+
+    def metric(_x):
+        e = next(seq)
+        return dict(eta_max=e, omega_max=e, rinf=np.array([e]), xinf=np.array([1.0]))
+    _, reason, hist = oracle.ir_loop(A, b, None, tau=tau, max_outer=max_outer, metric=metric,
+                                     apply=lambda R: np.zeros_like(np.asarray(R, dtype=np.float64).reshape(2, -1)))
+    return reason, hist
+
+
+def test_ir_loop_stall_rule_scripted(oracle):
+    """Stagnation (src/mp_core.cuh:290): eta > 0.9 * previous for three consecutive steps.  The third such step
+    stops the loop (4 evaluations); a step at or below 0.9 * previous resets the count; exactly 0.9 * previous
+    is not a stall step; max_outer is tested before the stall rule and convergence before both."""
+    r, h = _scripted_loop(oracle, [1.0, 0.95, 0.95 ** 2, 0.95 ** 3, 0.95 ** 4])
+    assert r == "stagnation" and len(h) == 4
+    # a contraction below 0.9 in the middle resets the count
+    seq = [1.0, 0.95, 0.9025, 0.5, 0.49, 0.48, 0.47, 0.46]
+    r, h = _scripted_loop(oracle, seq)
+    assert r == "stagnation" and len(h) == 7
+    # exactly 0.9 * previous: not counted (> is strict); eventually max_outer
+    seq = [1.0]
+    for _ in range(11):
+        seq.append(0.9 * seq[-1])                               # the rule's own product, so equality is exact
+    r, h = _scripted_loop(oracle, seq, max_outer=10)
+    assert r == "max_outer" and len(h) == 11
+    # growing eta: stagnation after three steps, not divergence
+    r, h = _scripted_loop(oracle, [1e-3, 2e-3, 4e-3, 8e-3, 1.6e-2])
+    assert r == "stagnation" and len(h) == 4
+    # max_outer before stagnation at the same step
+    r, h = _scripted_loop(oracle, [1.0, 0.95, 0.95 ** 2, 0.95 ** 3], max_outer=3)
+    assert r == "max_outer" and len(h) == 4
+    # converged at the first evaluation below tau, even inside a stall run
+    r, h = _scripted_loop(oracle, [1.0, 0.99, 1e-16])
+    assert r == "converged" and len(h) == 3
+    r, h = _scripted_loop(oracle, [1.0, float("inf")])
+    assert r == "nonfinite" and len(h) == 2
+
+
+@pytest.mark.parametrize("fmt,tile_scale", [("fp16", True), ("fp16", False), ("bf16", True), ("fp8", True), ("fp8", False),
+                                            ("fp32", True), ("fp16x3", True)])
+def test_pack_restatement(oracle, fmt, tile_scale):
+    """or_pack: each 32-index tile's largest |x| lands in the format's top binade [2^T, 2^(T+1)) (T = 14 for
+    FP16/BF16, 7 for e4m3, none for FP32 or without tile scaling); values are round-to-nearest-even of x * 2^s
+    (numpy's float16 cast for FP16, truncation of the low 16 bits with rounding for BF16, the e4m3 rounding
+    pinned by enumeration above); the counters count scaled values above the maximum and in the subnormal range."""
+    rng = np.random.default_rng(5)
+    S = (rng.standard_normal((70, 40)) * 2.0 ** rng.integers(-20, 20, size=(70, 40))).astype(np.float32)
+    q, qlo, ex, cnt = oracle.pack(S, fmt, tile_scale)
+    top = {"fp16": 14, "fp16x3": 14, "bf16": 14, "fp8": 7}.get(fmt)
+    for t in range(len(ex)):
+        tile = S[32 * t:32 * t + 32]
+        if top is None or not tile_scale:
+            assert ex[t] == 0
+        else:
+            assert 2.0 ** top <= np.abs(tile).max() * 2.0 ** ex[t] < 2.0 ** (top + 1)
+    sc = np.repeat(np.ldexp(1.0, ex), 32)[:70, None].astype(np.float32)
+    V = S * sc
+    if fmt in ("fp16", "fp16x3"):
+        with np.errstate(over="ignore"):
+            ref = V.astype(np.float16).astype(np.float32)
+        np.testing.assert_array_equal(q, ref)
+        if fmt == "fp16x3":
+            np.testing.assert_array_equal(qlo, (V - ref).astype(np.float16).astype(np.float32))
+        big = np.abs(V) > 65504; sub = (np.abs(V) > 0) & (np.abs(V) < 2.0 ** -14)
+    elif fmt == "bf16":
+        u = V.view(np.uint32).astype(np.uint64)
+        ref = (((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)).view(np.float32)
+        np.testing.assert_array_equal(q, ref)
+        big = np.abs(V) > 3.3895313892515355e38; sub = (np.abs(V) > 0) & (np.abs(V) < 2.0 ** -126)
+    elif fmt == "fp8":
+        np.testing.assert_array_equal(q, oracle.round_fmt(V, "fp8"))
+        big = np.abs(V) > 448; sub = (np.abs(V) > 0) & (np.abs(V) < 2.0 ** -6)
+    else:
+        np.testing.assert_array_equal(q, V)
+        big = np.zeros_like(V, dtype=bool); sub = (np.abs(V) > 0) & (np.abs(V) < 2.0 ** -126)
+    assert int(cnt[0]) == int(big.sum()) and int(cnt[1]) == int(sub.sum())
+    if not tile_scale and fmt in ("fp16", "fp8"):
+        assert cnt[0] > 0                                       # the unscaled case overflows somewhere

@@ -173,7 +173,7 @@ __global__ __launch_bounds__(512) void k_trsv2(const float* __restrict__ LU, int
                                                const double* __restrict__ Rin, int64_t ldr,
                                                float* __restrict__ Y, int64_t ldy,
                                                double* __restrict__ D, int64_t ldd, double scale,
-                                               uint64_t* __restrict__ ybuf, uint32_t tag, int32_t* __restrict__ err, long spin) {
+                                               uint64_t* __restrict__ ybuf, uint32_t tag, uint32_t wtag, int32_t* __restrict__ err, long spin) {
   __shared__ float ys[3][NR][GB];
   __shared__ float part[4][NR][GB];
   __shared__ int s_abort;
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(512) void k_trsv2(const float* __restrict__ LU, int
     const int bj = blk(s2);
     if (t < GB) {
 #pragma unroll
-      for (int c = 0; c < NR; c++) ys[s2 % 3][c][t] = gs_wait(ybuf + ((int64_t)bj * NR + c) * GB + t, y[c], tag, LDS_FLAG(&s_abort), ndep - 1 - s2, spin);
+      for (int c = 0; c < NR; c++) ys[s2 % 3][c][t] = gs_wait(ybuf + ((int64_t)bj * NR + c) * GB + t, y[c], wtag, LDS_FLAG(&s_abort), ndep - 1 - s2, spin);
     }
     __syncthreads();
 #pragma unroll
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(512) void k_trsv2(const float* __restrict__ LU, int
 #pragma unroll
       for (int c = 0; c < NR; c++) {
         const uint64_t* p = ybuf + ((int64_t)bj * NR + c) * GB + t;
-        ys[sc][c][t] = gs_wait(p, gs_load(p), tag, LDS_FLAG(&s_abort), 0, spin);
+        ys[sc][c][t] = gs_wait(p, gs_load(p), wtag, LDS_FLAG(&s_abort), 0, spin);
       }
     }
     __syncthreads();                                  // also orders the reads of part above before the writes below
@@ -321,12 +321,15 @@ static void trsv_pair(int64_t n, const float* LU, int64_t lda, const int32_t* pe
   const float* Tu = W.tinv.p + (size_t)nbk * GB * GB;
   const float* Mi = W.tchain.p;
   const float* Mu = W.tchain.p + (size_t)nbk * GB * GB;
+  // the forced-timeout mode waits for a tag take_tags never hands out (it stays below 0xFFFFFFF0)
+  const long spin = W.gs_force ? 1L : GS_SPIN_MAX;
+  const uint32_t wl = W.gs_force ? 0xFFFFFFFFu : tl, wu = W.gs_force ? 0xFFFFFFFFu : tu;
   hipEvent_t ev = W.prof_begin();
   hipLaunchKernelGGL((k_trsv2<NR, true>), dim3(nbk), dim3(512), 0, W.stream, LU, lda, n, Ti, Mi, perm, R, ldr,
-                     W.rhs32.p, n, (double*)nullptr, (int64_t)0, 1.0, W.ybuf.p, tl, err, W.gs_spin);
+                     W.rhs32.p, n, (double*)nullptr, (int64_t)0, 1.0, W.ybuf.p, tl, wl, err, spin);
   LAUNCH_CK();
   hipLaunchKernelGGL((k_trsv2<NR, false>), dim3(nbk), dim3(512), 0, W.stream, LU, lda, n, Tu, Mu, perm, (const double*)nullptr, (int64_t)0,
-                     W.rhs32.p, n, D, ldd, scale, W.ybuf.p, tu, err, W.gs_spin);
+                     W.rhs32.p, n, D, ldd, scale, W.ybuf.p, tu, wu, err, spin);
   LAUNCH_CK();
   W.prof_end(ev, "getrs", 4.0 * (double)n * n + 24.0 * n * NR, 2.0 * (double)n * n * NR);
 }
@@ -342,22 +345,16 @@ void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* per
   W.dinfo.ensure(*W.tracker, 4, "getrf_info");
   W.tinv.ensure(*W.tracker, (size_t)2 * nbk * GB * GB, "getrs_block_inverses");
   W.tchain.ensure(*W.tracker, (size_t)2 * nbk * GB * GB, "getrs_chain_matrices");
-  // bound on the polls of one exchange wait.  A smaller bound only lets a test force the timeout path from the
-  // host: MPSOLVE_GS_SPIN = "bound" for every call, or "bound:calls" for the next `calls` calls after the
-  // variable takes a new value (every wait still ends; blocks that give up publish zeros and set the flag).
-  W.gs_spin = GS_SPIN_MAX;
-  if (const char* e = getenv("MPSOLVE_GS_SPIN")) {
-    if (W.gs_spin_env != e) {
-      W.gs_spin_env = e;
-      const char* c = strchr(e, ':');
-      W.gs_spin_calls = c ? atol(c + 1) : -1;
-    }
-    if (W.gs_spin_calls != 0) {
-      W.gs_spin = std::max(1L, std::min(GS_SPIN_MAX, atol(e)));
-      if (W.gs_spin_calls > 0) W.gs_spin_calls--;
-    }
+  // A test forces the timeout path from the host: MPSOLVE_GS_FAIL = k (k > 0: the next k calls after the
+  // variable takes a new value; "all": every call) makes the waits of the substitutions look for a tag that is
+  // never published, with a bound of one poll.  Every dependent block then gives up (publishing zeros and
+  // setting the timeout flag) and every workgroup exits normally.
+  W.gs_force = false;
+  if (const char* e = getenv("MPSOLVE_GS_FAIL")) {
+    if (W.gs_fail_env != e) { W.gs_fail_env = e; W.gs_fail_calls = (strcmp(e, "all") == 0) ? -1 : std::max(0L, atol(e)); }
+    if (W.gs_fail_calls != 0) { W.gs_force = true; if (W.gs_fail_calls > 0) W.gs_fail_calls--; }
   } else {
-    W.gs_spin_env.clear();
+    W.gs_fail_env.clear();
   }
   if (!reuse_inverse || W.tinv_lu != LU || W.tinv_n != n || W.tinv_lda != lda || W.tinv_gen != W.factor_gen) {
     static bool attr = false;

@@ -56,8 +56,8 @@ struct Workspace {
   DBuf<float> rhs32;
   DBuf<float> tinv;      // inverses of the 128 x 128 diagonal blocks of L and U
   DBuf<float> tchain;    // chain matrices Linv_i L_{i,i-1} and Uinv_i U_{i,i+1} of the blocked substitution
-  long gs_spin = 1L << 22;   // poll bound of one exchange wait of the substitutions
-  std::string gs_spin_env; long gs_spin_calls = -1;   // host-side override of that bound (tests only)
+  bool gs_force = false;     // forced exchange timeout of the substitutions (tests only)
+  std::string gs_fail_env; long gs_fail_calls = 0;
   const float* tinv_lu = nullptr; int64_t tinv_n = 0, tinv_lda = 0; uint64_t tinv_gen = 0;
   // batched small systems
   DBuf<uint8_t> bsys;    // per-system descriptors in processing order

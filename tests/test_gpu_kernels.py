@@ -176,9 +176,9 @@ def test_getrs_matches_oracle(ctx, oracle, n, nrhs):
 
 def test_getrs_exchange_timeout_is_an_error(ctx, oracle, monkeypatch):
     """A wait of the substitutions' block exchange that reaches its bound is reported (MP_ERR_INTERNAL), not
-    returned as a result.  The bound is lowered from the host to one poll, so later blocks give up waiting
-    for their predecessors; every workgroup still exits normally.  With the normal bound the same call
-    succeeds again (the flag is cleared per call)."""
+    returned as a result.  The host forces it (MPSOLVE_GS_FAIL: the waits look for a tag that is never
+    published, with a one-poll bound), so every block with a predecessor gives up; every workgroup still exits
+    normally.  Without it the same call succeeds again (the flag is cleared per call)."""
     from cpsc402_midterm_project_amd import MPSolveError
     n = 8192
     A, b, _, _ = randsvd(1024, 1e2, 3, gpu=False)
@@ -189,29 +189,29 @@ def test_getrs_exchange_timeout_is_an_error(ctx, oracle, monkeypatch):
     pm = np.arange(n, dtype=np.int32); pm[:1024] = F["perm"]
     bb = np.ones(n)
     dLU = _dev(ctx, LU); dpm = _dev(ctx, pm); dR = _dev(ctx, bb); dD = ctx.empty(8 * n)
-    monkeypatch.setenv("MPSOLVE_GS_SPIN", "1")
+    monkeypatch.setenv("MPSOLVE_GS_FAIL", "all")
     with pytest.raises(MPSolveError) as ei:
         ctx.getrs(n, 1, dLU, n, dpm, dR, n, dD, n)
     assert ei.value.status == -5 and "timed out" in str(ei.value)
-    monkeypatch.delenv("MPSOLVE_GS_SPIN")
+    monkeypatch.delenv("MPSOLVE_GS_FAIL")
     ctx.getrs(n, 1, dLU, n, dpm, dR, n, dD, n)
 
 
 def test_substitution_timeout_survives_refactorization(ctx, monkeypatch):
     """A ladder refactorizes between substitutions.  Only the first substitution of the trial (the initial
-    solve of the FP8 stage) is given a one-poll bound and times out; the ladder then escalates through own
+    solve of the FP8 stage) is forced to time out (MPSOLVE_GS_FAIL=1); the ladder then escalates through own
     factorizations (FP8 fails at kappa=1e8), and the trial must still end as an error: a factorization clears
     only its own info words.  The next solve, with the normal bound, starts clean and converges."""
     n = 2048
     print(f"[timeout sticky] n={n}", flush=True)
     A, b, xt, _ = randsvd(n, 1e8, 17, gpu=False)
     P = ctx.problem(A, b)
-    monkeypatch.setenv("MPSOLVE_GS_SPIN", "1:1")
+    monkeypatch.setenv("MPSOLVE_GS_FAIL", "1")
     X, rec = P.solve("policy_mx8_ladder", nb=256, tau=1e-15, stop_on="eta")
     print(f"[timeout sticky] ok={rec['ok']} stages={rec['stages']} error={rec['error']!r}", flush=True)
     assert len(rec["stages"]) >= 2, rec["stages"]                # a refactorization happened after the timeout
     assert not rec["ok"] and "timed out" in rec["error"], (rec["ok"], rec["error"], rec["stages"])
-    monkeypatch.delenv("MPSOLVE_GS_SPIN")
+    monkeypatch.delenv("MPSOLVE_GS_FAIL")
     X, rec = P.solve("policy_mx8_ladder", nb=256, tau=1e-15, stop_on="eta")
     assert rec["ok"] and rec["terminal_reason"] == "converged", (rec["error"], rec["terminal_reason"])
     P.close()

@@ -22,7 +22,7 @@ FMT = {"fp16": 0, "bf16": 1, "fp8": 2, "fp8e4m3": 2, "fp32": 3, "fp16x3": 4}
 # every function declared in include/mpsolve.h
 EXPORTS = [
     "mp_create", "mp_destroy", "mp_last_error", "mp_caps", "mp_synchronize", "mp_profile", "mp_profile_json", "mp_malloc", "mp_free",
-    "mp_memcpy_h2d", "mp_memcpy_d2h", "mp_memcpy_d2d", "mp_residual_fp64", "mp_residual_dd", "mp_getrf",
+    "mp_memcpy_h2d", "mp_memcpy_d2h", "mp_memcpy_d2d", "mp_residual_fp64", "mp_residual_dd", "mp_residual_noalloc", "mp_getrf",
     "mp_getrs", "mp_pack_operand", "mp_problem_factor_bytes", "mp_problem_from_host", "mp_problem_generate", "mp_problem_reference", "mp_problem_free", "mp_problem_copy_out", "mp_problem_copy_ref", "mp_problem_case_json", "mp_solve",
     "mp_last_record_json", "mp_last_hist_json", "mp_methods_json", "mp_eval_metrics_json", "mp_audit_json", "mp_sha256_hex", "mp_batch_solve",
     "mp_batch_solve_ptrs", "mp_generate_into", "mp_problem_from_device", "mp_problem_set_rhs", "mp_server_clock",
@@ -85,6 +85,7 @@ def _load() -> C.CDLL:
         "mp_memcpy_d2d": ([vp, vp, vp, sz], C.c_int),
         "mp_residual_fp64": ([vp, i64, i32, i32, vp, i64, dp, i64, dp, i64, dp, i64, dp, i64, dp, vp], C.c_int),
         "mp_residual_dd": ([vp, i64, i32, i32, vp, i64, dp, i64, dp, i64, dp, dp, i64, dp, i64, dp, vp], C.c_int),
+        "mp_residual_noalloc": ([vp, i64, i32, i32, vp, i64, dp, i64, dp, i64, dp, dp, i64, dp, i64, dp, vp], C.c_int),
         "mp_getrf": ([vp, i64, vp, i64, vp, vp, C.POINTER(GetrfParams), C.POINTER(GetrfInfo)], C.c_int),
         "mp_getrs": ([vp, i64, i32, vp, i64, vp, dp, i64, dp, i64, C.c_double], C.c_int),
         "mp_pack_operand": ([vp, i32, i32, vp, i64, i64, i32, i32, vp, vp, vp, vp], C.c_int),
@@ -243,6 +244,13 @@ class Context:
         else:
             check(lib.mp_residual_fp64(self.h, n, nrhs, a_bits, _ptr(A), lda, _ptr(X), ldx, _ptr(B), ldb if B is not None else n,
                                        _ptr(R), ldr, _ptr(T), ldt or n, _ptr(stats), _ptr(worst)))
+
+    def residual_noalloc(self, n, nrhs, A, lda, X, ldx, B, ldb, R, ldr, T=None, ldt=None, stats=None, worst=None,
+                         a_bits=64, Rlo=None):
+        """mp_residual_noalloc: Rlo given -> the double-double residual, else FP64."""
+        check(lib.mp_residual_noalloc(self.h, n, nrhs, a_bits, _ptr(A), lda, _ptr(X), ldx, _ptr(B),
+                                      ldb if B is not None else n, _ptr(R), _ptr(Rlo), ldr, _ptr(T), ldt or n,
+                                      _ptr(stats), _ptr(worst)))
 
     def getrf(self, n, LU, lda, ipiv, perm, nb=256, fmt="fp16", tile_scale=True) -> GetrfInfo:
         p = GetrfParams(nb, FMT[fmt], 1 if tile_scale else 0)

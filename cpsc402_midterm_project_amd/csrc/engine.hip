@@ -107,6 +107,7 @@ struct mp_ctx {
   std::unique_ptr<Krylov> kry;
   std::set<mp_problem*> problems;       // live problems: their device memory is released by mp_destroy
   std::map<uint64_t, mp_problem*> factor_ids;   // factor cache: matrix id -> problem holding its factors
+  double* resid_scratch = nullptr;      // device address of this GPU's allocation-free residual scratch
 };
 
 struct Factors;
@@ -991,6 +992,8 @@ int mp_create(mp_ctx** out, int device) {
     C->W.tracker = &C->tracker;
     hipDeviceProp_t prop; HIP_CK(hipGetDeviceProperties(&prop, device));
     C->W.num_cus = prop.multiProcessorCount;
+    // this device's copy of the allocation-free residual's scratch array (the device is current here)
+    C->resid_scratch = resid_scratch_init();
     RB_CK(rocblas_create_handle(&C->rb));
     RB_CK(rocblas_set_stream(C->rb, C->W.stream));
     RB_CK(rocblas_create_handle(&C->rb_panel));
@@ -1083,6 +1086,18 @@ int mp_residual_dd(mp_ctx* C, int64_t n, int32_t nrhs, int32_t a_bits, const voi
     ra.B = B; ra.ldb = B ? ldb : n; ra.R = Rhi; ra.Rlo = Rlo; ra.ldr = ldr; ra.T = T; ra.ldt = T ? ldt : n;
     ra.stats = colstats; ra.worst = worst; ra.dd = true;
     residual(ra, C->W);
+  });
+}
+
+int mp_residual_noalloc(mp_ctx* C, int64_t n, int32_t nrhs, int32_t a_bits, const void* A, int64_t lda, const double* X, int64_t ldx,
+                        const double* B, int64_t ldb, double* R, double* Rlo, int64_t ldr, double* T, int64_t ldt,
+                        double* colstats, int32_t* worst) {
+  return guard([&] {
+    MP_REQUIRE(C, "ctx is NULL"); MP_REQUIRE(a_bits == 32 || a_bits == 64, "a_bits must be 32 or 64");
+    ResidArgs ra; ra.n = n; ra.nrhs = nrhs; ra.a32 = (a_bits == 32); ra.A = A; ra.lda = lda; ra.X = X; ra.ldx = ldx;
+    ra.B = B; ra.ldb = B ? ldb : n; ra.R = R; ra.Rlo = Rlo; ra.ldr = ldr; ra.T = T; ra.ldt = T ? ldt : n;
+    ra.stats = colstats; ra.worst = worst; ra.dd = (Rlo != nullptr);
+    residual_noalloc(ra, C->W, C->resid_scratch);
   });
 }
 

@@ -259,28 +259,35 @@ static void pick_chunks(int64_t n, int64_t& cpc, int64_t& nchunks) {
 }
 
 template <typename TA, int NR, bool DD>
+static void launch_group_into(const ResidArgs& a, Workspace& W, int c0, double* part, double* statp, bool prof) {
+  int64_t cpc, nch;
+  pick_chunks(a.n, cpc, nch);
+  const int64_t rb = cdiv(a.n, 256);
+  const TA* A = (const TA*)a.A;
+  hipEvent_t ev = prof ? W.prof_begin() : nullptr;
+  hipLaunchKernelGGL((k_resid_partial<TA, NR, DD>), dim3(rb, nch), dim3(256), 0, W.stream,
+                     A, a.n, a.lda, a.X + (int64_t)c0 * a.ldx, a.ldx, cpc, part);
+  LAUNCH_CK();
+  // algorithmic bytes: A once, X once, the chunk partials written (the reduce pass reads them back)
+  if (prof)
+    W.prof_end(ev, DD ? "residual_dd_partial" : (a.matvec ? "matvec_partial" : "residual_partial"),
+               (double)sizeof(TA) * a.n * a.n + 8.0 * a.n * NR, (DD ? 20.0 : 4.0) * a.n * a.n * NR);
+  hipLaunchKernelGGL((k_resid_reduce<NR, DD>), dim3(rb, NR), dim3(256), 0, W.stream,
+                     part, a.n, nch, a.X + (int64_t)c0 * a.ldx, a.ldx,
+                     a.B ? a.B + (int64_t)c0 * a.ldb : nullptr, a.ldb,
+                     a.R + (int64_t)c0 * a.ldr, a.Rlo ? a.Rlo + (int64_t)c0 * a.ldr : nullptr, a.ldr,
+                     a.T ? a.T + (int64_t)c0 * a.ldt : nullptr, a.ldt, a.matvec ? 1 : 0,
+                     statp, c0);
+  LAUNCH_CK();
+}
+
+template <typename TA, int NR, bool DD>
 static void launch_group(const ResidArgs& a, Workspace& W, int c0) {
   int64_t cpc, nch;
   pick_chunks(a.n, cpc, nch);
   const int F = DD ? 3 : 2;
-  const size_t need = (size_t)nch * F * NR * a.n;
-  W.part.ensure(*W.tracker, need, "resid_partials");
-  const int64_t rb = cdiv(a.n, 256);
-  const TA* A = (const TA*)a.A;
-  hipEvent_t ev = W.prof_begin();
-  hipLaunchKernelGGL((k_resid_partial<TA, NR, DD>), dim3(rb, nch), dim3(256), 0, W.stream,
-                     A, a.n, a.lda, a.X + (int64_t)c0 * a.ldx, a.ldx, cpc, W.part.p);
-  LAUNCH_CK();
-  // algorithmic bytes: A once, X once, the chunk partials written (the reduce pass reads them back)
-  W.prof_end(ev, DD ? "residual_dd_partial" : (a.matvec ? "matvec_partial" : "residual_partial"),
-             (double)sizeof(TA) * a.n * a.n + 8.0 * a.n * NR, (DD ? 20.0 : 4.0) * a.n * a.n * NR);
-  hipLaunchKernelGGL((k_resid_reduce<NR, DD>), dim3(rb, NR), dim3(256), 0, W.stream,
-                     W.part.p, a.n, nch, a.X + (int64_t)c0 * a.ldx, a.ldx,
-                     a.B ? a.B + (int64_t)c0 * a.ldb : nullptr, a.ldb,
-                     a.R + (int64_t)c0 * a.ldr, a.Rlo ? a.Rlo + (int64_t)c0 * a.ldr : nullptr, a.ldr,
-                     a.T ? a.T + (int64_t)c0 * a.ldt : nullptr, a.ldt, a.matvec ? 1 : 0,
-                     W.statp.p, c0);
-  LAUNCH_CK();
+  W.part.ensure(*W.tracker, (size_t)nch * F * NR * a.n, "resid_partials");
+  launch_group_into<TA, NR, DD>(a, W, c0, W.part.p, W.statp.p, true);
 }
 
 template <typename TA, bool DD>
@@ -295,18 +302,68 @@ static void launch_all(const ResidArgs& a, Workspace& W) {
   }
 }
 
-void residual(const ResidArgs& a, Workspace& W) {
+static void check_resid_args(const ResidArgs& a) {
   MP_REQUIRE(a.n > 0 && a.nrhs > 0, "n and nrhs must be positive");
   MP_REQUIRE(a.lda >= a.n && a.ldx >= a.n && a.ldr >= a.n, "leading dimensions must be >= n");
   MP_REQUIRE(a.A && a.X && a.R, "A, X and R are required");
   MP_REQUIRE(!a.T || a.ldt >= a.n, "ldt must be >= n");
   MP_REQUIRE(!a.B || a.ldb >= a.n, "ldb must be >= n");
+}
+
+void residual(const ResidArgs& a, Workspace& W) {
+  check_resid_args(a);
   const int64_t nblk = cdiv(a.n, 256);
   W.statp.ensure(*W.tracker, (size_t)a.nrhs * nblk * 6, "resid_stat_partials");
   if (a.a32) { if (a.dd) launch_all<float, true>(a, W); else launch_all<float, false>(a, W); }
   else { if (a.dd) launch_all<double, true>(a, W); else launch_all<double, false>(a, W); }
   if (a.stats || a.worst) {
     hipLaunchKernelGGL(k_resid_stats_final, dim3(a.nrhs), dim3(256), 0, W.stream, W.statp.p, nblk, a.nrhs, a.stats, a.worst);
+    LAUNCH_CK();
+  }
+}
+
+// ------------------------------------------------------------------ residual without allocation
+// A fixed device array holds the chunk partials and the per-block statistics of residual_noalloc, so that
+// entry point allocates nothing.  Each GPU holds its own copy (at its own address) until the process ends.
+// In host code `scratch` is only a stand-in for the symbol: its device address comes from
+// hipGetSymbolAddress, taken once per device when a context is created (resid_scratch_init).
+static __device__ __align__(256) double scratch[RESID_SCRATCH_PART + RESID_SCRATCH_STAT];
+
+double* resid_scratch_init() {
+  double* ptr = nullptr;
+  HIP_CK(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+  MP_REQUIRE(ptr != nullptr, "residual scratch: no device address");
+  return ptr;
+}
+
+bool resid_noalloc_fits(int64_t n, int nrhs, bool dd) {
+  if (n <= 0 || nrhs <= 0) return false;
+  int64_t cpc, nch;
+  pick_chunks(n, cpc, nch);
+  const int64_t F = dd ? 3 : 2;
+  return nch * F * n <= RESID_SCRATCH_PART && (int64_t)nrhs * cdiv(n, 256) * 6 <= RESID_SCRATCH_STAT;
+}
+
+// The residual of `residual` with the right-hand sides taken one at a time (each right-hand side's sums do not
+// depend on how many are processed together, so the results are the same bits), partials in the scratch
+// array at `scratch_dev` (its device address).  No allocation, no memset, no profiling events.
+void residual_noalloc(const ResidArgs& a, Workspace& W, double* scratch_dev) {
+  check_resid_args(a);
+  MP_REQUIRE(scratch_dev != nullptr, "residual scratch was not set up for this device");
+  if (!resid_noalloc_fits(a.n, a.nrhs, a.dd)) {
+    std::ostringstream o;
+    o << "invalid argument: n = " << a.n << ", nrhs = " << a.nrhs << " exceeds the scratch of the allocation-free residual ("
+      << RESID_SCRATCH_PART << " partial words, " << RESID_SCRATCH_STAT << " statistics words)";
+    throw MPError(o.str(), -1);
+  }
+  double* part = scratch_dev;
+  double* statp = scratch_dev + RESID_SCRATCH_PART;
+  for (int c = 0; c < a.nrhs; c++) {
+    if (a.a32) { if (a.dd) launch_group_into<float, 1, true>(a, W, c, part, statp, false); else launch_group_into<float, 1, false>(a, W, c, part, statp, false); }
+    else { if (a.dd) launch_group_into<double, 1, true>(a, W, c, part, statp, false); else launch_group_into<double, 1, false>(a, W, c, part, statp, false); }
+  }
+  if (a.stats || a.worst) {
+    hipLaunchKernelGGL(k_resid_stats_final, dim3(a.nrhs), dim3(256), 0, W.stream, statp, cdiv(a.n, 256), a.nrhs, a.stats, a.worst);
     LAUNCH_CK();
   }
 }

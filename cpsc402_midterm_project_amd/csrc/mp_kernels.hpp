@@ -92,6 +92,14 @@ struct ResidArgs {
   double* stats = nullptr; int32_t* worst = nullptr; bool dd = false; bool matvec = false;
 };
 void residual(const ResidArgs& a, Workspace& W);
+// Allocation-free residual: partials in a fixed device array of RESID_SCRATCH_PART + RESID_SCRATCH_STAT doubles
+// (16 MiB + 2 MiB), shared by every call on one GPU.  resid_scratch_init (with the device current) returns that
+// array's device address; resid_noalloc_fits says whether a size fits it.
+constexpr int64_t RESID_SCRATCH_PART = 2LL << 20;
+constexpr int64_t RESID_SCRATCH_STAT = 256LL << 10;
+double* resid_scratch_init();
+bool resid_noalloc_fits(int64_t n, int nrhs, bool dd);
+void residual_noalloc(const ResidArgs& a, Workspace& W, double* scratch_dev);
 void absmax(const void* A, bool a32, int64_t n, int64_t lda, double* out_dev, Workspace& W);
 
 // FMT_FP16X3: operands split into FP16 hi + lo parts (same tile scale), update = hi*hi + hi*lo + lo*hi on

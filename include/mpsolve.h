@@ -70,6 +70,19 @@ int mp_residual_fp64(mp_ctx* ctx, int64_t n, int32_t nrhs, int32_t a_bits, const
 int mp_residual_dd(mp_ctx* ctx, int64_t n, int32_t nrhs, int32_t a_bits, const void* A, int64_t lda,
                    const double* X, int64_t ldx, const double* B, int64_t ldb,
                    double* Rhi, double* Rlo, int64_t ldr, double* T, int64_t ldt, double* colstats, int32_t* worst);
+/* The residual of mp_residual_fp64 (Rlo == NULL) or mp_residual_dd (Rlo != NULL) with the same arguments and
+ * the same result bits, allocating nothing: no device or host allocation, no memset, no profiling events.
+ * No counterpart in the reference; it is for callers that cannot allocate (code under HIP graph capture,
+ * serving loops that must not call hipMalloc).  Its partial sums live in one fixed device array per GPU
+ * (16 MiB + 2 MiB, held until the process ends; its address is taken when a context is created).  Limit: the
+ * chunk partials of one right-hand side must fit 2^21 doubles (every n <= 1048576 for FP64, n <= 699050 for
+ * double-double) and nrhs * ceil(n / 256) * 6 <= 2^18 (nrhs <= 682 at n = 16384); larger sizes return
+ * MP_ERR_INVALID before any launch.
+ * Right-hand sides are processed one at a time (A is read once per right-hand side).  Every call on one GPU
+ * uses the same array: calls that overlap on one GPU (other streams, other contexts or threads) race. */
+int mp_residual_noalloc(mp_ctx* ctx, int64_t n, int32_t nrhs, int32_t a_bits, const void* A, int64_t lda,
+                        const double* X, int64_t ldx, const double* B, int64_t ldb, double* R, double* Rlo,
+                        int64_t ldr, double* T, int64_t ldt, double* colstats, int32_t* worst);
 
 typedef struct {
   int32_t nb;          /* panel width (1..256) */

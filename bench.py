@@ -274,14 +274,33 @@ def main():
                 import threadpoolctl
                 A, b, _ = P.host_copy()
                 cores = len(os.sched_getaffinity(0))
+                blas = [f"{i.get('internal_api')} {i.get('version')}" for i in threadpoolctl.threadpool_info()
+                        if i.get("user_api") == "blas"]
+                times = []
                 with threadpoolctl.threadpool_limits(cores):
-                    t = time.time(); xc = np.linalg.solve(A, b); tc = time.time() - t
+                    for rep in range(6):                  # one warm-up, then the median of 5
+                        t = time.time(); xc = np.linalg.solve(A, b); tc = time.time() - t
+                        if rep > 0:
+                            times.append(tc)
                 r = b - A @ xc
                 eta = np.abs(r).max() / (np.abs(A).sum(axis=1).max() * np.abs(xc).max() + np.abs(b).max())
-                cpu = dict(value=tc * 1e3, unit="ms", cores=cores, kind="port",
-                           sample=f"numpy.linalg.solve (LAPACK dgesv, {np.__version__} OpenBLAS) on the same n={n} "
-                                  f"kappa={kappa:g} system, one run, eta={eta:.2e}")
-                log(f"[cpu] dgesv {tc*1e3:.0f} ms on {cores} threads")
+                cpu = dict(value=statistics.median(times) * 1e3, unit="ms", cores=cores, kind="port",
+                           sample=f"numpy.linalg.solve (LAPACK dgesv, numpy {np.__version__}, {', '.join(blas) or 'BLAS'}) "
+                                  f"on the same n={n} kappa={kappa:g} system, median of 5 runs after 1 warm-up "
+                                  f"({', '.join(f'{x * 1e3:.0f}' for x in times)} ms), eta={eta:.2e}")
+                log(f"[cpu] dgesv median {cpu['value']:.0f} ms on {cores} threads ({blas})")
+                # the CPU restatement of the configuration-0 mixed method (FP16-trailing blocked LU, nb=128, FP64-residual
+                # stationary IR, n=1024, kappa=1e4, at most 50 steps), single-threaded C + numpy
+                from cpsc402_midterm_project_amd.problems import randsvd
+                from oracle import oracle as O
+                A0, b0, _, _ = randsvd(1024, 1e4, 9200, gpu=False)
+                with threadpoolctl.threadpool_limits(1):
+                    t = time.time()
+                    _, reason0, hist0 = O.ir_stationary(np.asfortranarray(A0), b0, fmt="fp16", nb=128, tau=1e-15, max_outer=50)
+                    t0c = time.time() - t
+                cpu["c0_mixed_restatement"] = dict(ms=t0c * 1e3, cores=1, terminal=reason0, steps=len(hist0) - 1,
+                                                   eta_final=hist0[-1])
+                log(f"[cpu] C0 mixed restatement {t0c*1e3:.0f} ms, {reason0} after {len(hist0) - 1} steps")
         P.close()
     batch = None
     if args.batch_count > 0 and "batch" in legs:

@@ -8,8 +8,9 @@ GPU, and this library's blocked FP64 LU (FP64 MFMA trailing update, vendor panel
 time-to-accuracy of the headline case (kappa=1e6) in ms; every case's timings, iteration counts and errors
 are reported alongside.
 
-Two further legs report the rest of the metric: the uneven batch (4096 systems n <= 512 in one persistent
-launch) and the serving stream (p50/p99 arrival-to-completion latency at 0.5x/0.8x/0.95x of the measured
+Further legs report the rest of the metric: the larger system (randsvd n=32768, 4 right-hand sides, kappa in
+{1e4, 1e8, 1e10}, FP8 -> FP16 -> FP32 -> FP64 ladder, timed against rocSOLVER's FP64 LU), the uneven batch (4096
+systems n <= 512 in one persistent launch) and the serving stream (p50/p99 arrival-to-completion latency at 0.5x/0.8x/0.95x of the measured
 saturation rate; serve_p50_ms / serve_p99_ms are the 0.8x figures).
 
 python bench.py [--steps K] [--warmup W] [--n N] [--kappas 1e2,1e6] [--batch-count C] [--serve-requests R]
@@ -86,21 +87,34 @@ def bench_batch(ctx, count, steps, warmup, cpu):
         list(ex.map(gen, range(count)))
     dA, dB, dX = ctx.buffer(Ah), ctx.buffer(Bh), ctx.empty(8 * tot_b)
     log(f"[batch] generated {count} systems (sum n^2 = {tot_a:.3e}) on {workers} host threads in {time.time()-t0:.1f}s")
-    recs = []
-    for i in range(warmup + steps):
-        res, rec = ctx.batch_solve_packed(n, np.ones(count, np.int32), dA, a_off, dB, b_off, dX, tau=1e-15, stop_on="eta")
-        log(f"  batch {'warm' if i < warmup else 'step'} {i}: {rec['device_s']*1e3:.2f} ms, converged {rec['converged']}/{count}, "
-            f"eta_worst {rec['eta_worst']:.2e}, outer total {rec['outer_iters_total']}")
-        if i >= warmup:
-            recs.append(rec)
+    def timed(tune):
+        recs = []
+        for i in range(max(warmup, 1) + steps):      # the first call of the tuned shape runs the autotuner
+            res, rec = ctx.batch_solve_packed(n, np.ones(count, np.int32), dA, a_off, dB, b_off, dX, tau=1e-15, stop_on="eta",
+                                              tune=tune)
+            log(f"  batch tune={tune} {'warm' if i < max(warmup, 1) else 'step'} {i}: {rec['device_s']*1e3:.2f} ms, "
+                f"converged {rec['converged']}/{count}, eta_worst {rec['eta_worst']:.2e}, outer total {rec['outer_iters_total']}, "
+                f"wg_per_cu {rec['wg_per_cu']} split {rec['split_by_n_bucket']} tuning {rec['tuning'].get('source')}")
+            if i == 0:
+                first = rec
+            if i >= max(warmup, 1):
+                recs.append(rec)
+        return recs, first
+    fixed, _ = timed("off")
+    recs, tuned_first = timed("auto")
     ms = statistics.median(r["device_s"] * 1e3 for r in recs)
+    ms_fixed = statistics.median(r["device_s"] * 1e3 for r in fixed)
     flops = float((2.0 / 3.0) * (n.astype(np.float64) ** 3).sum())
     out = dict(workload=f"{count} randsvd systems, n~U{{32..512}}, log10 kappa~U[1,5], seeds 9300+i, nrhs=1",
                method="FP32 LU (nb=32, FP32 trailing update) + stationary IR, FP64 residuals, one persistent launch",
                ms_median=round(ms, 3), ms_all=[round(r["device_s"] * 1e3, 3) for r in recs],
                systems_per_s=round(count / (ms * 1e-3), 1), converged=recs[-1]["converged"], systems=count,
                eta_worst=recs[-1]["eta_worst"], outer_iters_total=recs[-1]["outer_iters_total"],
-               getrf_flops=flops, getrf_tflops_equiv=round(flops / (ms * 1e-3) / 1e12, 3))
+               getrf_flops=flops, getrf_tflops_equiv=round(flops / (ms * 1e-3) / 1e12, 3),
+               launch_shape=dict(wg_per_cu=recs[-1]["wg_per_cu"], split_by_n_bucket=recs[-1]["split_by_n_bucket"],
+                                 panel_width=recs[-1]["panel_width"], threads=recs[-1]["threads"],
+                                 tuning=tuned_first["tuning"]),
+               fixed_shape_ms_median=round(ms_fixed, 3), tuned_over_fixed=round(ms_fixed / ms, 3))
     if cpu:
         cores = len(os.sched_getaffinity(0))
         t = time.time()
@@ -189,6 +203,45 @@ def bench_serve(ctx, nreq, pool_size, loads=(0.5, 0.8, 0.95)):
     return out
 
 
+def bench_c2(ctx, n, kappas, nrhs=4, seed=9202):
+    """Larger-system leg: randsvd n=32768 (seed 9202, the context keeps U and V so the kappas share one pair of QR
+    factorizations), 4 right-hand sides, kappa in {1e4, 1e8, 1e10}; policy_mx8_ladder (FP8 -> FP16 -> FP32 -> FP64
+    factors, FGMRES-IR with double-double residuals, a stage left after one GMRES cycle that gains less than 10x),
+    eta <= 1e-15.  One warm-up solve, then one timed solve per kappa; rocSOLVER's FP64 LU + solve on the same system
+    timed once beside it (its time does not depend on kappa)."""
+    from cpsc402_midterm_project_amd import Problem
+    out = dict(workload=f"randsvd n={n} nrhs={nrhs} seed={seed}, kappa in {kappas}: policy_mx8_ladder (nb=256, FGMRES "
+                        f"restart 30, DD residual, eta<=1e-15) vs rocSOLVER FP64 LU", kappas={})
+    params = dict(nb=256, gmres_m=30, tau=1e-15, stop_on="eta")
+    for i, kappa in enumerate(kappas):
+        t0 = time.time()
+        P = Problem.generate(ctx, family="randsvd", n=n, nrhs=nrhs, kappa=kappa, seed=seed, rhs="xtrue_uniform", keep_q=True)
+        P.reference()
+        log(f"[c2] kappa={kappa:g} generated + FP64 reference in {time.time() - t0:.1f} s")
+        if i == 0:
+            _, w = P.solve("policy_mx8_ladder", **params)
+            log(f"[c2] warm-up {w['device_ready_wall_s']*1e3:.0f} ms {w['terminal_reason']} stages={w['stages']}")
+            recs = []
+            for _ in range(2):
+                _, r64 = P.solve("native_lu64")
+                recs.append(r64)
+            out["fp64_lu_ms"] = recs[-1]["device_ready_wall_s"] * 1e3
+            out["fp64_lu_eta"] = recs[-1]["eta_max"]
+            log(f"[c2] rocSOLVER FP64 LU {out['fp64_lu_ms']:.0f} ms eta={out['fp64_lu_eta']:.2e}")
+        _, r = P.solve("policy_mx8_ladder", **params)
+        ms = r["device_ready_wall_s"] * 1e3
+        out["kappas"][f"{kappa:g}"] = dict(ms=round(ms, 2), terminal=r["terminal_reason"], stages=r["stages"],
+                                           stage_times_s=[round(t, 4) for t in r["stage_times_s"]], outer=r["outer_iters"],
+                                           inner=r["inner_iters_total"], eta=r["eta_max"], omega=r["omega_max"],
+                                           einf_vs_xtrue=max(r["einf"]) if r.get("einf") else None,
+                                           finf_vs_fp64_reference=max(r["finf"]) if r.get("finf") else None,
+                                           speedup_vs_fp64_lu=round(out["fp64_lu_ms"] / ms, 3))
+        log(f"[c2] kappa={kappa:g}: {ms:.0f} ms {r['terminal_reason']} stages={r['stages']} outer={r['outer_iters']} "
+            f"inner={r['inner_iters_total']} eta={r['eta_max']:.2e}")
+        P.close()
+    return out
+
+
 def pmc_mfma(kernel):
     """Counter evidence for `kernel` from the committed rocprofv3 PMC summaries: MFMA FLOPs per factorization,
     MFMA-busy fraction of SIMD cycles and the rate when the kernel runs alone; None when no summary covers it."""
@@ -228,7 +281,9 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096, help="uneven-batch leg: systems per launch (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=240, help="serving leg: requests per stream (0: skip)")
     ap.add_argument("--serve-pool", type=int, default=40)
-    ap.add_argument("--legs", default="headline,batch,serve",
+    ap.add_argument("--c2-n", type=int, default=32768, help="larger-system leg: n")
+    ap.add_argument("--c2-kappas", default="1e4,1e8,1e10")
+    ap.add_argument("--legs", default="headline,batch,serve,c2",
                     help="legs to run (a profiling run may select one; the headline fields are null without it)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -320,6 +375,9 @@ def main():
     serve = None
     if args.serve_requests > 0 and "serve" in legs:
         serve = bench_serve(ctx, args.serve_requests, args.serve_pool)
+    c2 = None
+    if "c2" in legs and args.c2_n > 0:
+        c2 = bench_c2(ctx, args.c2_n, [float(k) for k in args.c2_kappas.split(",")])
     ctx.close()
 
     # roofline of the dominant kernel (largest total time among the profiled launches)
@@ -346,7 +404,7 @@ def main():
                     counters=pmc_mfma("trailing_gemm"))
     if not cases:
         print(json.dumps({"metric": "n=16384 time-to-FP64-accuracy: low-prec LU-IR vs FP64 LU; serve p50/p99 ms", "value": None,
-                          "unit": "ms", "legs": sorted(legs), "batch": batch, "serve": serve}), flush=True)
+                          "unit": "ms", "legs": sorted(legs), "batch": batch, "serve": serve, "c2": c2}), flush=True)
         return
     hb = headline or cases[-1]
     cands = hb["low_precision"] + ([hb["fallback"]] if hb.get("fallback") else [])
@@ -372,6 +430,7 @@ def main():
         "serve_p99_ms": serve["loads"]["0.8"]["p99_ms"] if serve else None,
         "serve": serve,
         "batch": batch,
+        "c2": c2,
         "roofline": roof,
         "mfma": mfma,
         "cpu_baseline": cpu,

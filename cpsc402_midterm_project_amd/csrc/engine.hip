@@ -579,6 +579,7 @@ struct TrialOut {
 
 struct Engine {
   mp_ctx* C; const mp_problem& P; Factors fact; bool reuse = false; bool resume_ladder = false; bool instrument = false;
+  bool stage_abandon = false;   // this build's ladders: leave a stage whose first GMRES cycle predicts it will not pay
   int64_t n; int nrhs; double tau_eta, tau_omega; bool stop_eta_only = false;
   DBuf<double> x64, D, y; DBuf<float> x32, r32;
   // ladder policies of this build: the iterate with the smallest eta so far, where the next stage starts
@@ -630,7 +631,16 @@ struct Engine {
   }
 
   // FGMRES (right, flexible) on A d = r for one RHS column (src/mp_core.cuh:237-278)
-  std::string gmres_solve(const MethodSpec& M, const Factors& F, const double* r, double* d, int& budget_left, TrialOut& T, int outer, int rhs) {
+  // Stage economics for this build's ladders: after the first cycle of a stage's first correction, the gain of
+  // that cycle (rho, the correction equation's true relative residual) and its time per iteration predict what
+  // finishing the stage costs: every remaining solve (right-hand sides x outer steps, one step gaining gmres_tol
+  // from the current eta down to tau) needs log(tol)/log(rho) cycles of m iterations.  When that exceeds the
+  // next stage's estimated factorization cost (this stage's factorization time x cost_ratio_next, the
+  // reference's cost-aware estimate, src/mp_core.cuh:304-311) or the remaining budget, the stage is left.
+  struct AbandonTest { double eta = 0, tau = 0, refactor_cost = 0; int nrhs = 1; };
+  std::string gmres_solve(const MethodSpec& M, const Factors& F, const double* r, double* d, int& budget_left, TrialOut& T, int outer, int rhs,
+                          const AbandonTest* ab = nullptr) {
+    const double t_start = wall_now();
     if (!C->kry) { C->kry.reset(new Krylov()); C->kry->W = &C->W; }
     Krylov& K = *C->kry;
     const int m = M.gmres_m;
@@ -715,6 +725,19 @@ struct Engine {
       else copy_f64(K.sR.p, K.rcur.p, n, W());
       if (!std::isfinite(tn)) { status = "nonfinite"; break; }
       if (true_rel <= M.gmres_tol) { status = happy ? "happy_breakdown_verified" : "converged_true"; break; }
+      if (ab && cycles == 1 && true_rel > M.gmres_tol) {
+        const double rho = std::min(std::max(true_rel, 1e-300), 0.999999);
+        const double per_solve = std::ceil(std::log(M.gmres_tol) / std::log(rho)) * m;
+        const double steps = std::max(1.0, std::ceil(std::log(ab->tau / std::max(ab->eta, ab->tau)) / std::log(M.gmres_tol)));
+        const double its = steps * ab->nrhs * per_solve - k;
+        const double t_it = (wall_now() - t_start) / std::max(1, k);
+        if (its * t_it > ab->refactor_cost || its > budget_left) {
+          std::ostringstream o;
+          o << "abandoned:rho=" << rho << ",pred_its=" << its << ",t_it=" << t_it << ",refactor_cost=" << ab->refactor_cost;
+          status = o.str();
+          break;
+        }
+      }
       if (happy) { status = "happy_breakdown_unverified"; break; }
     }
     return status;
@@ -758,8 +781,9 @@ struct Engine {
     }
   }
 
+  // last_stage: the stage the ladder ends with (never abandoned)
   std::string refine_stage(const MethodSpec& M, const Factors& F, int stage_idx, TrialOut& T, double t_start, int& budget_left,
-                           bool initial_solve, Metrics& Mt) {
+                           bool initial_solve, Metrics& Mt, bool last_stage = true) {
     if (initial_solve) {
       const double t0 = wall_now();
       apply_inverse(C, F, P.B.p, D.p, nrhs, &T.trs, M.apply64);
@@ -768,6 +792,7 @@ struct Engine {
       T.t_initial += wall_now() - t0;
     }
     double prev_eta = INFINITY; int stall = 0; std::string reason = "max_outer";
+    int local_step = 0;
     while (true) {
       const double te0 = wall_now();
       residual_and_metrics(M, Mt);
@@ -785,17 +810,27 @@ struct Engine {
       prev_eta = Mt.eta_max;
       if (M.gmres && budget_left <= 0) { reason = "gmres_budget_exhausted"; break; }
       if (M.gmres) {
+        bool abandoned = false;
+        AbandonTest ab;
+        ab.eta = Mt.eta_max; ab.tau = tau_eta; ab.nrhs = nrhs;
+        ab.refactor_cost = (T.stage_times.empty() ? 0.0 : T.stage_times.back()) * M.cost_ratio_next;
         for (int c = 0; c < nrhs; c++) {
-          const std::string st = gmres_solve(M, F, E.R.p + (int64_t)c * n, D.p + (int64_t)c * n, budget_left, T, T.outer, c);
+          const bool test = stage_abandon && !last_stage && local_step == 0 && c == 0;
+          const std::string st = gmres_solve(M, F, E.R.p + (int64_t)c * n, D.p + (int64_t)c * n, budget_left, T, T.outer, c,
+                                             test ? &ab : nullptr);
           T.hist.back().note += (c ? ";" : "") + st;
           if (st == "nonfinite") T.hist.back().note += ";inner_nonfinite";
+          if (st.rfind("abandoned", 0) == 0) { abandoned = true; break; }
         }
+        // the first cycle showed the factors too weak for this stage to pay: the iterate is left as it is
+        if (abandoned) { T.hist.back().inner_in_step = T.inner_total - inner_before; reason = "stage_abandoned"; break; }
       } else {
         apply_inverse(C, F, E.R.p, D.p, nrhs, &T.trs, M.apply64); T.precond_applies++;
       }
       T.hist.back().inner_in_step = T.inner_total - inner_before;
       add_update(M, D.p);
       T.outer++;
+      local_step++;
     }
     return reason;
   }
@@ -845,9 +880,14 @@ struct Engine {
       if (M.policy == "budget" && M.budget == 0) Ms.gmres = false;
       if (M.policy == "budget" && M.budget > 0) { if (stage == 0) { Ms.gmres = true; budget_left = M.budget; } else Ms.gmres = false; }
       if (M.policy == "cost_aware") Ms.gmres = false;
-      if (M.policy == "ladder" || M.policy == "ladder16" || M.policy == "ladder16x3") { Ms.gmres = true; budget_left = M.gmres_budget; }
+      // this build's ladders: FGMRES-IR for the low-precision stages; the closing FP64-factor stage refines
+      // stationarily (FP64 factors leave one substitution of all right-hand sides per step, where GMRES would
+      // apply them one vector at a time)
+      if (M.policy == "ladder" || M.policy == "ladder16" || M.policy == "ladder16x3") {
+        Ms.gmres = ladder[stage] != "cs64"; budget_left = M.gmres_budget;
+      }
       const double ts = wall_now();
-      reason = refine_stage(Ms, F, stage, T, t0, budget_left, !initialized, Mt);
+      reason = refine_stage(Ms, F, stage, T, t0, budget_left, !initialized, Mt, stage + 1 == (int)ladder.size());
       initialized = true;
       T.t_refine += wall_now() - ts;
       if (M.policy == "cost_aware" && reason != "converged" && reason != "nonfinite" && M.gmres_budget > 0 && stage == 0) {
@@ -1102,7 +1142,7 @@ int mp_residual_noalloc(mp_ctx* C, int64_t n, int32_t nrhs, int32_t a_bits, cons
 }
 
 static void batch_record(mp_ctx* C, int count, const int32_t* n, const BatchParams& bp, const std::vector<BatchResult>& r,
-                         double dev_s, mp_batch_result* results) {
+                         double dev_s, mp_batch_result* results, const std::string& tuning) {
   static const char* names[] = {"converged", "nonfinite", "max_outer", "stagnation", "singular_factor"};
   int64_t hist[5] = {0, 0, 0, 0, 0}; int64_t outer_total = 0, nsum = 0; int nmax = 0;
   double eta_worst = 0;
@@ -1119,13 +1159,22 @@ static void batch_record(mp_ctx* C, int count, const int32_t* n, const BatchPara
    .s("stop_on", bp.eta_only ? "eta" : "eta_omega").i("wg_per_cu", bp.wg_per_cu).i("outer_iters_total", outer_total)
    .d("eta_worst", eta_worst);
   for (int t = 0; t < 5; t++) j.i(names[t], hist[t]);
-  C->last_record = j.obj();
+  std::string rec = j.obj();
+  // the launch shape: row-split thresholds per n bucket and how they were chosen
+  std::ostringstream sh;
+  sh << ",\"split_by_n_bucket\":[";
+  for (int b = 0; b < 4; b++) sh << (b ? "," : "") << "[" << bp.split4[b] << "," << bp.split2[b] << "]";
+  sh << "],\"panel_width\":32,\"threads\":256,\"tuning\":" << (tuning.empty() ? std::string("{\"source\":\"off\"}") : tuning) << "}";
+  rec.pop_back();
+  C->last_record = rec + sh.str();
   C->last_hist.clear();
 }
 
-static BatchParams batch_params(const char* params_json) {
+static BatchParams batch_params(const char* params_json, bool* tune = nullptr, bool* tune_wg = nullptr) {
   Params a; a.parse(params_json);
   BatchParams bp;
+  if (tune) *tune = a.get("tune", "auto") != "off";
+  if (tune_wg) *tune_wg = !a.has("wg_per_cu");
   const double u = (a.get("target", "fp64") == "fp32") ? std::ldexp(1.0, -24) : std::ldexp(1.0, -53);
   bp.tau = a.has("tau") ? a.getd("tau", 0) : a.getd("tau_mult", 100.0) * u;
   bp.max_outer = a.geti("max_outer", 30);
@@ -1139,11 +1188,14 @@ int mp_batch_solve_ptrs(mp_ctx* C, int32_t count, const int32_t* n, const int32_
                         const double* const* B, double* const* X, const char* params_json, mp_batch_result* results) {
   return guard([&] {
     MP_REQUIRE(C, "ctx is NULL");
-    const BatchParams bp = batch_params(params_json);
+    bool tune = false, tune_wg = false;
+    BatchParams bp = batch_params(params_json, &tune, &tune_wg);
     std::vector<BatchResult> r(std::max(count, 0));
     double dev_s = 0;
+    std::string tuning;
+    if (tune && count > 0) batch_tune(count, n, nrhs, A, B, X, bp, C->W, tuning, tune_wg);
     batch_solve(count, n, nrhs, A, B, X, bp, C->W, r.data(), &dev_s);
-    batch_record(C, count, n, bp, r, dev_s, results);
+    batch_record(C, count, n, bp, r, dev_s, results, tuning);
   });
 }
 
@@ -1152,7 +1204,8 @@ int mp_batch_solve(mp_ctx* C, int32_t count, const int32_t* n, const int32_t* nr
   return guard([&] {
     MP_REQUIRE(C, "ctx is NULL");
     MP_REQUIRE(count >= 0, "count must be non-negative");
-    const BatchParams bp = batch_params(params_json);
+    bool tune = false, tune_wg = false;
+    BatchParams bp = batch_params(params_json, &tune, &tune_wg);
     std::vector<const double*> pa(std::max(count, 0)), pb(std::max(count, 0));
     std::vector<double*> px(std::max(count, 0));
     if (count > 0) {
@@ -1164,8 +1217,13 @@ int mp_batch_solve(mp_ctx* C, int32_t count, const int32_t* n, const int32_t* nr
     }
     std::vector<BatchResult> r(std::max(count, 0));
     double dev_s = 0;
+    std::string tuning;
+    if (tune && count > 0) {
+      for (int i = 0; i < count; i++) MP_REQUIRE(n[i] >= 1 && n[i] <= batch_max_n(), "every n must be in [1, 512]");
+      batch_tune(count, n, nrhs, pa.data(), pb.data(), px.data(), bp, C->W, tuning, tune_wg);
+    }
     batch_solve(count, n, nrhs, pa.data(), pb.data(), px.data(), bp, C->W, r.data(), &dev_s);
-    batch_record(C, count, n, bp, r, dev_s, results);
+    batch_record(C, count, n, bp, r, dev_s, results, tuning);
   });
 }
 
@@ -1267,7 +1325,7 @@ int mp_generate_into(mp_ctx* C, const char* case_json, double* A, double* B, dou
     g.spectrum = a.get("spectrum", "geometric"); g.rhs = a.get("rhs", g.family == "dyadic" ? "exact_sign" : "xtrue_uniform");
     g.kappa = a.getd("kappa", 100); g.seed = (uint64_t)a.getd("seed", 9200); g.ell = a.getd("ell", 0.25); g.nugget = a.getd("nugget", 1e-3);
     g.dim = a.geti("dim", 3); g.intmax = a.geti("intmax", 255); g.dyadic_exp = a.geti("dyadic_exp", 8); g.grcar_k = a.geti("grcar_k", 3);
-    g.global_exp = a.geti("global_exp", 0); g.rowcol_span = a.geti("rowcol_span", 0);
+    g.global_exp = a.geti("global_exp", 0); g.rowcol_span = a.geti("rowcol_span", 0); g.keep_q = a.getb("keep_q", false);
     MP_REQUIRE(g.n > 0 && g.nrhs > 0, "n and nrhs must be positive");
     DBuf<double> Xe;
     double* xe = x_exact;
@@ -1360,7 +1418,7 @@ int mp_problem_generate(mp_ctx* C, const char* case_json, mp_problem** out) {
     g.spectrum = a.get("spectrum", "geometric"); g.rhs = a.get("rhs", g.family == "dyadic" ? "exact_sign" : "xtrue_uniform");
     g.kappa = a.getd("kappa", 100); g.seed = (uint64_t)a.getd("seed", 9200); g.ell = a.getd("ell", 0.25); g.nugget = a.getd("nugget", 1e-3);
     g.dim = a.geti("dim", 3); g.intmax = a.geti("intmax", 255); g.dyadic_exp = a.geti("dyadic_exp", 8); g.grcar_k = a.geti("grcar_k", 3);
-    g.global_exp = a.geti("global_exp", 0); g.rowcol_span = a.geti("rowcol_span", 0);
+    g.global_exp = a.geti("global_exp", 0); g.rowcol_span = a.geti("rowcol_span", 0); g.keep_q = a.getb("keep_q", false);
     MP_REQUIRE(g.n > 0 && g.nrhs > 0, "n and nrhs must be positive");
     MP_REQUIRE(g.rowcol_span >= 0 && g.rowcol_span <= 200, "rowcol_span must be in [0, 200]");
     const std::string dtype = a.get("dtype", "fp64");
@@ -1641,6 +1699,8 @@ int mp_solve(mp_ctx* C, mp_problem* P, const char* method_id, const char* params
         Engine E(C, *P, tau, tau);
         E.instrument = a.getb("instrument", false); E.stop_eta_only = eta_only; E.reuse = a.getb("reuse", false);
         E.resume_ladder = a.getb("resume_ladder", false);
+        const bool own_ladder = M.policy == "ladder" || M.policy == "ladder16" || M.policy == "ladder16x3";
+        E.stage_abandon = own_ladder && a.getb("stage_abandon", true);
         HIP_CK(hipStreamSynchronize(C->W.stream));
         const double t1 = wall_now();
         E.run(M, T);
@@ -1690,6 +1750,7 @@ int mp_solve(mp_ctx* C, mp_problem* P, const char* method_id, const char* params
      .kv("ezero_maxabs", Json::arr(Mt.ezero_maxabs)).kv("ezero_count", Json::arr(Mt.ezero_count))
      .s("timestamp", a.get("timestamp", "")).i("nb", M.nb).i("gmres_m", M.gmres_m).d("gmres_tol", M.gmres_tol).i("gmres_budget", M.gmres_budget)
      .i("gmres_cycles", M.gmres_cycles).i("max_outer", M.max_outer).s("policy", M.policy).i("budget", M.budget).i("scale_mode", M.scale_mode)
+     .b("stage_abandon", (M.policy == "ladder" || M.policy == "ladder16" || M.policy == "ladder16x3") && a.getb("stage_abandon", true))
      .i("working", M.working).i("resid", M.resid == 128 ? 64 : M.resid).b("resid_dd", M.resid == 128).b("apply64", M.apply64)
      .b("reuse", a.getb("reuse", false)).s("panel", "hip_lds_exchange").i("panel_launches", T.panels).i("panel_coop_refused", T.panel_coop_refused).s("panel_requested", a.get("panel_requested", "cusolver")).i("stages_count", (int64_t)T.stages.size())
      .s("stop_on", eta_only ? "eta" : "eta_omega");

@@ -230,21 +230,41 @@ bool generate(const GenSpec& g, void* rb_handle, Workspace& W, DevTracker& tr, d
     LAUNCH_CK();
     rotation_stages(A, n, g.seed, 2, stages, W, tr);
   } else if (g.family == "randsvd") {
-    DBuf<double> U, V, tau, sig;
-    U.alloc(tr, nn, "gen_U"); V.alloc(tr, nn, "gen_V"); tau.alloc(tr, n, "gen_tau"); sig.alloc(tr, n, "gen_sigma");
-    hipLaunchKernelGGL(k_gauss_fill, dim3(cdiv(nn, 256)), dim3(256), 0, W.stream, U.p, (int64_t)nn, g.seed, (uint64_t)1);
-    hipLaunchKernelGGL(k_gauss_fill, dim3(cdiv(nn, 256)), dim3(256), 0, W.stream, V.p, (int64_t)nn, g.seed, (uint64_t)2);
-    rbck(rocsolver_dgeqrf(rb, (rocblas_int)n, (rocblas_int)n, U.p, (rocblas_int)n, tau.p), "dgeqrf");
-    rbck(rocsolver_dorgqr(rb, (rocblas_int)n, (rocblas_int)n, (rocblas_int)n, U.p, (rocblas_int)n, tau.p), "dorgqr");
-    rbck(rocsolver_dgeqrf(rb, (rocblas_int)n, (rocblas_int)n, V.p, (rocblas_int)n, tau.p), "dgeqrf");
-    rbck(rocsolver_dorgqr(rb, (rocblas_int)n, (rocblas_int)n, (rocblas_int)n, V.p, (rocblas_int)n, tau.p), "dorgqr");
+    // U and V depend on (n, seed) only: with keep_q they stay in the context, so systems that differ only in
+    // kappa (or the right-hand side) reuse them; the stored A is the same bits either way
+    DBuf<double> Ut, Vt, T, tau, sig;
+    const bool hit = g.keep_q && W.gen_valid && W.gen_n == n && W.gen_seed == g.seed;
+    double *U, *V;
+    if (g.keep_q) {
+      if (!hit) { W.gen_valid = false; W.genU.alloc(tr, nn, "gen_U_kept"); W.genV.alloc(tr, nn, "gen_V_kept"); }
+      U = W.genU.p; V = W.genV.p;
+    } else {
+      Ut.alloc(tr, nn, "gen_U"); Vt.alloc(tr, nn, "gen_V"); U = Ut.p; V = Vt.p;
+    }
+    if (!hit) {
+      tau.alloc(tr, n, "gen_tau");
+      hipLaunchKernelGGL(k_gauss_fill, dim3(cdiv(nn, 256)), dim3(256), 0, W.stream, U, (int64_t)nn, g.seed, (uint64_t)1);
+      hipLaunchKernelGGL(k_gauss_fill, dim3(cdiv(nn, 256)), dim3(256), 0, W.stream, V, (int64_t)nn, g.seed, (uint64_t)2);
+      rbck(rocsolver_dgeqrf(rb, (rocblas_int)n, (rocblas_int)n, U, (rocblas_int)n, tau.p), "dgeqrf");
+      rbck(rocsolver_dorgqr(rb, (rocblas_int)n, (rocblas_int)n, (rocblas_int)n, U, (rocblas_int)n, tau.p), "dorgqr");
+      rbck(rocsolver_dgeqrf(rb, (rocblas_int)n, (rocblas_int)n, V, (rocblas_int)n, tau.p), "dgeqrf");
+      rbck(rocsolver_dorgqr(rb, (rocblas_int)n, (rocblas_int)n, (rocblas_int)n, V, (rocblas_int)n, tau.p), "dorgqr");
+      if (g.keep_q) { W.gen_n = n; W.gen_seed = g.seed; W.gen_valid = true; }
+    }
+    sig.alloc(tr, n, "gen_sigma");
     std::vector<double> hs(n);
     for (int64_t i = 0; i < n; i++) hs[i] = spectrum_value(g.spectrum, i, n, g.kappa);
     HIP_CK(hipMemcpyAsync(sig.p, hs.data(), n * 8, hipMemcpyHostToDevice, W.stream));
-    hipLaunchKernelGGL(k_scale_cols, dim3(cdiv(n, 256), (unsigned)n), dim3(256), 0, W.stream, U.p, n, sig.p);
+    double* Us = U;
+    if (g.keep_q) {   // the kept U stays unscaled: U diag(sigma) goes to a temporary
+      T.alloc(tr, nn, "gen_U_sigma");
+      HIP_CK(hipMemcpyAsync(T.p, U, nn * 8, hipMemcpyDeviceToDevice, W.stream));
+      Us = T.p;
+    }
+    hipLaunchKernelGGL(k_scale_cols, dim3(cdiv(n, 256), (unsigned)n), dim3(256), 0, W.stream, Us, n, sig.p);
     const double one = 1.0, zero = 0.0;
     rbck(rocblas_dgemm(rb, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)n, (rocblas_int)n, (rocblas_int)n,
-                       &one, U.p, (rocblas_int)n, V.p, (rocblas_int)n, &zero, A, (rocblas_int)n), "dgemm");
+                       &one, Us, (rocblas_int)n, V, (rocblas_int)n, &zero, A, (rocblas_int)n), "dgemm");
     HIP_CK(hipStreamSynchronize(W.stream));
   } else if (g.family == "dyadic") {
     hipLaunchKernelGGL(k_dyadic_A, dim3(cdiv(n, 256), (unsigned)n), dim3(256), 0, W.stream, A, n, g.seed, g.intmax, std::ldexp(1.0, -g.dyadic_exp));

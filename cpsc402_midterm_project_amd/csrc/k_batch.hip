@@ -276,13 +276,15 @@ __device__ __forceinline__ void trsm_col(Lds& S, float* __restrict__ LU, int ldr
   }
 }
 
-__device__ void update(Lds& S, float* __restrict__ LU, int n, int ldr, int k) {
+// t4 / t2: trailing sizes m2 at or below which the rows are split over 4 / 2 thread groups (per n bucket, from
+// the launch's split table; the split changes which thread computes an element, never its operation order)
+__device__ void update(Lds& S, float* __restrict__ LU, int n, int ldr, int k, int t4, int t2) {
   const int t = threadIdx.x;
   double* Lc = S.buf;                                   // [64 rows][32] FP64
   const int m2 = n - k - 32;
   // G thread groups of BT/G threads: each group takes every column of the pass and a share of the rows, so
   // narrow trailing matrices still keep all threads busy (each group forms its columns' U12 itself)
-  const int G = (m2 <= 64) ? 4 : (m2 <= 128 ? 2 : 1);
+  const int G = (m2 <= t4) ? 4 : (m2 <= t2 ? 2 : 1);
   const int TG = BT / G, g = t / TG, tl = t % TG;
 #pragma unroll 1
   for (int cb = k + 32; cb < n; cb += TG) {
@@ -538,9 +540,11 @@ __device__ void residual(Lds& S, const double* A, const double* B, const double*
 
 __global__ void k_batch_zero(int32_t* p) { if (threadIdx.x == 0) p[0] = 0; }
 
+struct BSplit { int t4[4], t2[4]; };   // row-split thresholds per n bucket (n - 1) / 128
+
 __global__ __launch_bounds__(BT, 2) void k_batch(const BSys* __restrict__ sys, int count, int32_t* __restrict__ queue,
                                                  float* __restrict__ Wall, double* __restrict__ Rall,
-                                                 BRes* __restrict__ res, double tau, int max_outer, int eta_only) {
+                                                 BRes* __restrict__ res, double tau, int max_outer, int eta_only, BSplit split) {
   __shared__ Lds S;
   const int t = threadIdx.x;
   for (;;) {
@@ -559,6 +563,8 @@ __global__ __launch_bounds__(BT, 2) void k_batch(const BSys* __restrict__ sys, i
     float* LU = Wall + d.w_off;
     float* LUc = LU + (int64_t)n * ldr;
     double* R = Rall + d.r_off;
+    const int bucket = min(3, (n - 1) / 128);
+    const int t4 = split.t4[bucket], t2 = split.t2[bucket];
     double g;
     setup(S, A, n, ldr, LU, g);
     int info = 0;
@@ -567,7 +573,7 @@ __global__ __launch_bounds__(BT, 2) void k_batch(const BSys* __restrict__ sys, i
       info = panel(S, LU, n, ldr, k, kb, info);
       if (n - k - kb > 0) {
         invert_l11(S);
-        update(S, LU, n, ldr, k);
+        update(S, LU, n, ldr, k, t4, t2);
       }
     }
     BRes out; out.pad = 0; out.info = info; out.outer = 0;
@@ -648,8 +654,10 @@ void batch_solve(int count, const int32_t* n, const int32_t* nrhs, const double*
   hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(64), 0, W.stream, W.bqueue.p);
   LAUNCH_CK();
   HIP_CK(hipEventRecord(e0, W.stream));
+  BSplit split;
+  for (int b = 0; b < 4; b++) { split.t4[b] = p.split4[b]; split.t2[b] = p.split2[b]; }
   hipLaunchKernelGGL(k_batch, dim3(grid), dim3(BT), 0, W.stream, (const BSys*)W.bsys.p, count, W.bqueue.p,
-                     W.bwork.p, W.bresid.p, (BRes*)W.bres.p, p.tau, p.max_outer, p.eta_only ? 1 : 0);
+                     W.bwork.p, W.bresid.p, (BRes*)W.bres.p, p.tau, p.max_outer, p.eta_only ? 1 : 0, split);
   LAUNCH_CK();
   HIP_CK(hipEventRecord(e1, W.stream));
   std::vector<BRes> hr(count);
@@ -662,6 +670,94 @@ void batch_solve(int count, const int32_t* n, const int32_t* nrhs, const double*
     out[i].terminal = hr[i].terminal; out[i].outer_iters = hr[i].outer; out[i].info = hr[i].info;
     out[i].eta_max = hr[i].eta; out[i].omega_max = hr[i].omega;
   }
+}
+
+// ------------------------------------------------------------------ autotuner of the launch shape
+// The knobs that change only which thread or workgroup does a piece of work (never an element's operation
+// order, so every candidate returns the same bits): resident workgroups per CU, and per n bucket the trailing
+// sizes at which a system's update splits its rows over 2 or 4 thread groups.  The panel width (32) and the
+// workgroup (4 waves) are fixed by the kernel's structure: the 32-wide inverted diagonal block and U12 column
+// live in registers, and the panel's argmax is a 4-wave exchange.
+// A sample of the batch (up to 96 systems per bucket, spread over its sizes) is solved once per candidate;
+// per bucket the fastest split wins, then the fastest occupancy for the whole sample.  The choice is cached
+// per batch signature (the systems per bucket on a log2 scale and the largest right-hand-side count).
+static std::string batch_key(int count, const int32_t* n, const int32_t* nrhs) {
+  int cnt[4] = {0, 0, 0, 0}, rmax = 0;
+  for (int i = 0; i < count; i++) { cnt[std::min(3, (n[i] - 1) / 128)]++; rmax = std::max(rmax, (int)nrhs[i]); }
+  std::ostringstream o;
+  for (int b = 0; b < 4; b++) o << (cnt[b] ? 1 + (int)std::log2((double)cnt[b]) : 0) << "/";
+  o << (rmax <= 1 ? 1 : (rmax <= 4 ? 4 : 16));
+  return o.str();
+}
+
+void batch_tune(int count, const int32_t* n, const int32_t* nrhs, const double* const* A, const double* const* B,
+                double* const* X, BatchParams& p, Workspace& W, std::string& detail, bool tune_wg) {
+  MP_REQUIRE(count > 0 && n && nrhs && A && B && X, "batch_tune: empty batch or NULL array");
+  for (int i = 0; i < count; i++) {
+    MP_REQUIRE(n[i] >= 1 && n[i] <= BNMAX, "every n must be in [1, 512] (larger systems go through mp_solve)");
+    MP_REQUIRE(nrhs[i] >= 1 && nrhs[i] <= BRHS, "every nrhs must be in [1, 16]");
+  }
+  const std::string key = batch_key(count, n, nrhs);
+  auto it = W.batch_tuned.find(key);
+  if (it != W.batch_tuned.end()) {
+    const BatchParams& c = it->second;
+    for (int b = 0; b < 4; b++) { p.split4[b] = c.split4[b]; p.split2[b] = c.split2[b]; }
+    if (tune_wg) p.wg_per_cu = c.wg_per_cu;
+    detail = "{\"source\":\"cache\",\"key\":\"" + key + "\"}";
+    return;
+  }
+  const double t0 = wall_now();
+  std::vector<std::vector<int>> by(4);
+  for (int i = 0; i < count; i++) by[std::min(3, (n[i] - 1) / 128)].push_back(i);
+  std::vector<int> all;
+  std::vector<BatchResult> scratch(count);
+  auto run = [&](const std::vector<int>& ids, const BatchParams& q) {
+    std::vector<int32_t> nn, rr; std::vector<const double*> aa, bb; std::vector<double*> xx;
+    for (int i : ids) { nn.push_back(n[i]); rr.push_back(nrhs[i]); aa.push_back(A[i]); bb.push_back(B[i]); xx.push_back(X[i]); }
+    double s = 0;
+    batch_solve((int)ids.size(), nn.data(), rr.data(), aa.data(), bb.data(), xx.data(), q, W, scratch.data(), &s);
+    return s;
+  };
+  static const int cand[][2] = {{0, 0}, {32, 64}, {64, 128}, {128, 256}, {256, 512}};
+  std::ostringstream js;
+  js << "{\"source\":\"tuned\",\"key\":\"" << key << "\",\"buckets\":{";
+  bool first_b = true;
+  for (int b = 0; b < 4; b++) {
+    if (by[b].empty()) continue;
+    std::vector<int> ids = by[b];
+    std::sort(ids.begin(), ids.end(), [&](int x, int y) { return n[x] < n[y]; });
+    std::vector<int> smp;
+    const size_t want = std::min<size_t>(96, ids.size());
+    for (size_t s = 0; s < want; s++) smp.push_back(ids[s * ids.size() / want]);
+    all.insert(all.end(), smp.begin(), smp.end());
+    BatchParams q = p; q.wg_per_cu = 2;
+    run(smp, q);                                   // warm-up: workspace growth and first-launch costs
+    double best = 1e300; int bi = 2;
+    js << (first_b ? "" : ",") << "\"" << 128 * b + 1 << "-" << 128 * (b + 1) << "\":{";
+    first_b = false;
+    for (int c = 0; c < 5; c++) {
+      q.split4[b] = cand[c][0]; q.split2[b] = cand[c][1];
+      const double s = run(smp, q);
+      js << (c ? "," : "") << "\"" << cand[c][0] << "/" << cand[c][1] << "\":" << s * 1e3;
+      if (s < best) { best = s; bi = c; }
+    }
+    js << "}";
+    p.split4[b] = cand[bi][0]; p.split2[b] = cand[bi][1];
+  }
+  js << "},\"wg_per_cu\":{";
+  if (tune_wg) {
+    double best = 1e300; int bw = p.wg_per_cu;
+    for (int w = 1; w <= 3; w++) {
+      BatchParams q = p; q.wg_per_cu = w;
+      const double s = run(all, q);
+      js << (w > 1 ? "," : "") << "\"" << w << "\":" << s * 1e3;
+      if (s < best) { best = s; bw = w; }
+    }
+    p.wg_per_cu = bw;
+  }
+  js << "},\"sample_systems\":" << all.size() << ",\"tune_s\":" << wall_now() - t0 << "}";
+  detail = js.str();
+  W.batch_tuned[key] = p;
 }
 
 }  // namespace mpk

@@ -29,6 +29,14 @@ struct KernelProfiler {
   }
 };
 
+// Many independent systems (n <= 512, nrhs <= 16) in one persistent launch (batch_solve below).
+// split4 / split2 per n bucket (n - 1) / 128: trailing sizes at or below which a system's update splits its rows
+// over 4 / 2 thread groups
+struct BatchParams {
+  double tau = 100 * 1.1102230246251565e-16; int max_outer = 30; bool eta_only = false; int wg_per_cu = 2;
+  int split4[4] = {64, 64, 64, 64}, split2[4] = {128, 128, 128, 128};
+};
+
 struct Workspace {
   hipStream_t stream = nullptr;
   hipStream_t pstream = nullptr;          // second stream: panel factorizations overlapped with trailing updates
@@ -65,6 +73,9 @@ struct Workspace {
   DBuf<double> bresid;   // per-system residuals
   DBuf<uint8_t> bres;    // per-system outcomes
   DBuf<int32_t> bqueue;  // work-queue head of the persistent launch
+  std::map<std::string, BatchParams> batch_tuned;   // autotuned launch shapes by batch signature
+  // randsvd Q factors kept for later generations of the same (n, seed) (case option keep_q)
+  DBuf<double> genU, genV; int64_t gen_n = 0; uint64_t gen_seed = 0; bool gen_valid = false;
   // FP64 LU
   DBuf<int32_t> getrf64_info;   // per-panel info of the vendor panel factorizations
   uint64_t factor_gen = 1;   // incremented by every factorization written through getrf
@@ -131,13 +142,16 @@ void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* per
 
 // Many independent systems (n <= 512, nrhs <= 16) in one persistent launch: FP32 LU (nb = 32, FP32 trailing
 // update) and stationary refinement with FP64 residuals until eta <= tau (and omega <= tau unless eta_only).
-struct BatchParams { double tau = 100 * 1.1102230246251565e-16; int max_outer = 30; bool eta_only = false; int wg_per_cu = 2; };
 struct BatchResult { int32_t terminal = 0, outer_iters = 0, info = 0; double eta_max = 0, omega_max = 0; };
 int batch_max_n();
 int batch_max_rhs();
 // System i: A[i] (n x n, lda n), B[i] and X[i] (n x nrhs, ld n), device pointers in host arrays.
 void batch_solve(int count, const int32_t* n, const int32_t* nrhs, const double* const* A, const double* const* B,
                  double* const* X, const BatchParams& p, Workspace& W, BatchResult* out, double* device_s);
+// Picks the launch shape of a batch (workgroups per CU if tune_wg, row-split thresholds per n bucket) by timing
+// candidates on a sample of it; cached per batch signature in the workspace.  detail: JSON of the choice.
+void batch_tune(int count, const int32_t* n, const int32_t* nrhs, const double* const* A, const double* const* B,
+                double* const* X, BatchParams& p, Workspace& W, std::string& detail, bool tune_wg);
 
 // FP64 blocked LU (P A = L U in place, 1-based global ipiv on the device) with the trailing update on FP64 MFMA;
 // rb_main / rb_panel: rocBLAS handles of the context (the second one serves the look-ahead panel stream).
@@ -151,6 +165,7 @@ struct GenSpec {
   int64_t n = 1024; int nrhs = 1; std::string family = "randsvd", spectrum = "geometric", rhs = "xtrue_uniform";
   double kappa = 100; uint64_t seed = 9200; double ell = 0.25, nugget = 1e-3; int dim = 3, intmax = 255, dyadic_exp = 8, grcar_k = 3;
   int global_exp = 0, rowcol_span = 0;
+  bool keep_q = false;   // randsvd: keep U and V in the context for later generations with the same n and seed
 };
 // device generation of A (n x n), B and X_exact (n x nrhs); returns whether X_exact holds the exact solution
 bool generate(const GenSpec& g, void* rocblas_handle, Workspace& W, DevTracker& tr, double* A, double* B, double* Xe);

@@ -296,7 +296,10 @@ int mp_server_create(int device, const char* params_json, mp_server** out) {
       return o.substr(0, o.size() - 1) + "," + extra + "}";
     };
     S->solve_params = with(sp, "\"tau\":" + tau + ",\"stop_on\":\"eta\",\"reuse\":true,\"resume_ladder\":true");
-    S->batch_params = with(bp, "\"tau\":" + tau + ",\"stop_on\":\"eta\"");
+    // no launch-shape tuning inside the request loop unless asked for (a new batch signature would add its tuning
+    // time to the latency of the requests in that batch)
+    const std::string tune = bp.find("\"tune\"") == std::string::npos ? ",\"tune\":\"off\"" : "";
+    S->batch_params = with(bp, "\"tau\":" + tau + ",\"stop_on\":\"eta\"" + tune);
     if (mp_create(&S->big, device) != MP_OK) throw MPError(std::string("mp_create: ") + mp_last_error(), MP_ERR_NO_DEVICE);
     if (mp_create(&S->small, device) != MP_OK) {
       const std::string m = mp_last_error();

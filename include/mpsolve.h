@@ -135,7 +135,9 @@ int mp_problem_from_host(mp_ctx* ctx, int64_t n, int32_t nrhs, int32_t a_bits, c
 /* Device generation of a seeded test system from a case object (JSON): "family" (randsvd | dyadic | rbf |
  * grcar | wilkinson), "n", "nrhs", "kappa", "spectrum", "seed", "rhs" (xtrue_uniform | random | random_scaled |
  * exact_sign | unit_column), "intmax", "dyadic_exp", "ell", "nugget", "dim", "grcar_k", "global_exp",
- * "rowcol_span" (power-of-two row and column scaling), "dtype" (fp64 | fp32 storage).
+ * "rowcol_span" (power-of-two row and column scaling), "dtype" (fp64 | fp32 storage), "keep_q" (randsvd: the
+ * context keeps the two n x n orthogonal factors, 16 n^2 bytes, until a keep_q generation of another (n, seed)
+ * or mp_destroy, and reuses them for systems of the same n and seed, e.g. another kappa; the matrix is the same).
  * Replaces generate_problem (src/mp_core.cuh:51-103); the random streams are this library's own. */
 int mp_problem_generate(mp_ctx* ctx, const char* case_json, mp_problem** out);
 /* The same seeded generation into caller-provided device buffers of the context's GPU: A (n x n), B and, if not

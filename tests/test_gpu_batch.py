@@ -140,3 +140,28 @@ def test_batch_full_uneven_workload_every_system_by_oracle(ctx, oracle):
         assert eta <= 1e-15, (i, k, kap[i], eta)
         worst = max(worst, eta)
     print(f"[batch full] every system converged; worst eta by the oracle {worst:.2e}", flush=True)
+
+
+def test_batch_autotuned_shape_gives_the_same_bits(ctx):
+    """The autotuner (launch shape: workgroups per CU, row-split thresholds per n bucket, timed on a sample of
+    the batch and cached per batch signature) changes only which thread or workgroup computes what: X and every
+    per-system outcome equal the untuned launch bit for bit, the first tuned call reports the candidates it
+    timed, the second one the cached choice."""
+    rng = np.random.default_rng(41)
+    n = rng.integers(20, 513, size=300).astype(np.int32)
+    nrhs = rng.integers(1, 4, size=300).astype(np.int32)
+    systems = [randsvd(int(k), 10.0 ** rng.uniform(1, 4), 5000 + i, nrhs=int(r), gpu=False)[:2] for i, (k, r) in enumerate(zip(n, nrhs))]
+    print(f"[batch tune] {len(systems)} systems", flush=True)
+    X0, r0, rec0 = ctx.batch_solve(systems, tau=1e-15, stop_on="eta", tune="off")
+    assert rec0["tuning"]["source"] == "off" and rec0["split_by_n_bucket"] == [[64, 128]] * 4
+    X1, r1, rec1 = ctx.batch_solve(systems, tau=1e-15, stop_on="eta", tune="auto")
+    X2, r2, rec2 = ctx.batch_solve(systems, tau=1e-15, stop_on="eta")          # default: auto, now from the cache
+    print(f"[batch tune] off {rec0['device_s']*1e3:.2f} ms; tuned {rec1['device_s']*1e3:.2f} ms wg_per_cu={rec1['wg_per_cu']} "
+          f"split={rec1['split_by_n_bucket']} tuning={rec1['tuning']}", flush=True)
+    assert rec1["tuning"]["source"] == "tuned" and rec2["tuning"]["source"] == "cache"
+    assert rec1["wg_per_cu"] in (1, 2, 3) and rec2["wg_per_cu"] == rec1["wg_per_cu"]
+    assert rec2["split_by_n_bucket"] == rec1["split_by_n_bucket"]
+    assert set(rec1["tuning"]["buckets"]) == {"1-128", "129-256", "257-384", "385-512"}
+    for (a, b, c) in zip(X0, X1, X2):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+    assert r0 == r1 == r2

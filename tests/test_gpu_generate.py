@@ -63,3 +63,25 @@ def test_prescribed_large_keeps_spectrum(ctx):
     sig = 1e6 ** (-np.arange(n) / (n - 1))
     assert np.abs(s - sig).max() < 1e-13
     P.close()
+
+
+def test_randsvd_kept_q_factors_give_the_same_matrix(ctx):
+    """keep_q: the context keeps U and V of a randsvd generation and reuses them for the same (n, seed); the
+    generated systems (A, B, x_true) are bitwise those of a generation from scratch, for the same and for
+    another kappa, and a different seed replaces the kept factors."""
+    n = 384
+    print(f"[generate] randsvd keep_q n={n}", flush=True)
+    ref = {}
+    for kappa in (1e2, 1e6):
+        P = Problem.generate(ctx, family="randsvd", n=n, nrhs=2, kappa=kappa, seed=77, rhs="xtrue_uniform")
+        ref[kappa] = P.host_copy(); P.close()
+    for kappa in (1e2, 1e6, 1e2):                                 # first generation fills the cache, the rest reuse it
+        P = Problem.generate(ctx, family="randsvd", n=n, nrhs=2, kappa=kappa, seed=77, rhs="xtrue_uniform", keep_q=True)
+        got = P.host_copy(); P.close()
+        for a, b in zip(got, ref[kappa]):
+            assert _same_bits(a, b)
+    P = Problem.generate(ctx, family="randsvd", n=n, nrhs=2, kappa=1e2, seed=78, rhs="xtrue_uniform", keep_q=True)
+    A78 = P.host_copy()[0]; P.close()
+    P = Problem.generate(ctx, family="randsvd", n=n, nrhs=2, kappa=1e2, seed=78, rhs="xtrue_uniform")
+    assert _same_bits(A78, P.host_copy()[0]) and not _same_bits(A78, ref[1e2][0])
+    P.close()

@@ -632,15 +632,18 @@ struct Engine {
 
   // FGMRES (right, flexible) on A d = r for one RHS column (src/mp_core.cuh:237-278)
   // Stage economics for this build's ladders: after the first cycle of a stage's first correction, the gain of
-  // that cycle (rho, the correction equation's true relative residual) and its time per iteration predict what
-  // finishing the stage costs: every remaining solve (right-hand sides x outer steps, one step gaining gmres_tol
-  // from the current eta down to tau) needs log(tol)/log(rho) cycles of m iterations.  When that exceeds the
-  // next stage's estimated factorization cost (this stage's factorization time x cost_ratio_next, the
-  // reference's cost-aware estimate, src/mp_core.cuh:304-311) or the remaining budget, the stage is left.
-  struct AbandonTest { double eta = 0, tau = 0, refactor_cost = 0; int nrhs = 1; };
+  // that cycle (rho, the correction equation's true relative residual) predicts what finishing the stage costs:
+  // every remaining solve (right-hand sides x outer steps, one step gaining gmres_tol from the current eta down
+  // to tau) needs log(tol)/log(rho) cycles of m iterations.  When that exceeds the next factorization's cost in
+  // iterations (cost_ratio_next, the reference's cost-aware ratio, src/mp_core.cuh:304-311, times
+  // kFactorIters) or the remaining budget, the stage is left.  The rule uses no clock, so a solve's stages
+  // (and its result) do not depend on timing.
+  // One factorization costs about as much as 100 FGMRES iterations (a preconditioner application and an FP64
+  // matrix-vector product): 79 ms against 0.8 ms at n = 16384, 264 ms against 2.5 ms at n = 32768.
+  static constexpr double kFactorIters = 100.0;
+  struct AbandonTest { double eta = 0, tau = 0, refactor_iters = 0; int nrhs = 1; };
   std::string gmres_solve(const MethodSpec& M, const Factors& F, const double* r, double* d, int& budget_left, TrialOut& T, int outer, int rhs,
                           const AbandonTest* ab = nullptr) {
-    const double t_start = wall_now();
     if (!C->kry) { C->kry.reset(new Krylov()); C->kry->W = &C->W; }
     Krylov& K = *C->kry;
     const int m = M.gmres_m;
@@ -730,10 +733,9 @@ struct Engine {
         const double per_solve = std::ceil(std::log(M.gmres_tol) / std::log(rho)) * m;
         const double steps = std::max(1.0, std::ceil(std::log(ab->tau / std::max(ab->eta, ab->tau)) / std::log(M.gmres_tol)));
         const double its = steps * ab->nrhs * per_solve - k;
-        const double t_it = (wall_now() - t_start) / std::max(1, k);
-        if (its * t_it > ab->refactor_cost || its > budget_left) {
+        if (its > ab->refactor_iters || its > budget_left) {
           std::ostringstream o;
-          o << "abandoned:rho=" << rho << ",pred_its=" << its << ",t_it=" << t_it << ",refactor_cost=" << ab->refactor_cost;
+          o << "abandoned:rho=" << rho << ",pred_its=" << its << ",refactor_iters=" << ab->refactor_iters;
           status = o.str();
           break;
         }
@@ -813,7 +815,7 @@ struct Engine {
         bool abandoned = false;
         AbandonTest ab;
         ab.eta = Mt.eta_max; ab.tau = tau_eta; ab.nrhs = nrhs;
-        ab.refactor_cost = (T.stage_times.empty() ? 0.0 : T.stage_times.back()) * M.cost_ratio_next;
+        ab.refactor_iters = M.cost_ratio_next * kFactorIters;
         for (int c = 0; c < nrhs; c++) {
           const bool test = stage_abandon && !last_stage && local_step == 0 && c == 0;
           const std::string st = gmres_solve(M, F, E.R.p + (int64_t)c * n, D.p + (int64_t)c * n, budget_left, T, T.outer, c,

@@ -320,13 +320,15 @@ GPU_METHOD = {"hsd": "hetero16_ir_w32_r64", "sss": "native_ir32_w32_r32", "ssd":
 
 def gpu_trial(ctx, a32, b32, method: str, xref, max_outer: int = 30) -> dict:
     """Solve the stored FP32 system with the library's method of the same roles (through the C ABI) on the GPU
-    of `ctx`; the target is the working precision, as in the model."""
+    of `ctx`.  Like the model, which does not stop on a small backward error, the GPU leg refines until its
+    iterate stops improving: the backward-error target is 0, so the loop ends by its stagnation rule (eta above
+    0.9 x the previous one three times) or at max_outer."""
     mid = GPU_METHOD[method]
     target = "fp64" if METHODS[method].work == 64 else "fp32"
     P = ctx.problem(np.asfortranarray(a32, dtype=np.float32), np.asarray(b32, dtype=np.float64).reshape(-1, 1),
                     x_ref=np.asarray(xref, dtype=np.float64).reshape(-1, 1))
     try:
-        X, rec = P.solve(mid, target=target, max_outer=max_outer, nb=64)
+        X, rec = P.solve(mid, target=target, tau=0.0, max_outer=max_outer, nb=64)
     finally:
         P.close()
     x = X[:, 0].astype(np.float32 if target == "fp32" else np.float64)

@@ -2,7 +2,9 @@
 precision roles through the C ABI.  The GPU factorizations are not the scalar model (FP32 panel, FP16 operands
 with FP32 accumulation on MFMA), so the legs are compared by their errors: each GPU method meets its working
 precision's backward-error target, and its forward error is within a small factor of the model's for the same
-roles (or of 10 u of the working precision when the model's is smaller)."""
+roles (or of 10 u sqrt(kappa) of the working precision when the model's is smaller).  Both legs refine until
+their iterates stop improving (the model to its correction cap or an unchanged update, the GPU to its
+stagnation rule), so their final forward errors are comparable."""
 import pytest
 
 from cpsc402_midterm_project_amd import e1
@@ -21,6 +23,6 @@ def test_e1_gpu_leg(ctx, n, kappa):
         u = e1.UNIT[e1.METHODS[m].work]
         print(f"[e1] {m}: model {model[m]['status']} finf {model[m]['finf']:.2e}; gpu {g['gpu_method']} {g['terminal']} "
               f"eta {g['eta']:.2e} finf {g['finf']:.2e}", flush=True)
-        assert g["ok"] and g["terminal"] in ("converged", "direct_solve"), g
+        assert g["ok"] and g["terminal"] in ("stagnation", "max_outer", "direct_solve"), g
         assert g["eta"] <= 100 * u
         assert g["finf"] <= 8 * max(model[m]["finf"], 10 * u * kappa ** 0.5), (g["finf"], model[m]["finf"])

@@ -150,3 +150,63 @@ def test_factor_cache_put_get_evict(ctx):
     P.close()
     with pytest.raises(Exception, match="not in the factor cache"):
         ctx.factor_cache_has(8)
+
+
+def test_c4_shaped_stream_each_result_equals_its_replay(ctx):
+    """A stream shaped like the serving configuration: matrices with n log-uniform in [256, 16384] and kappa
+    log-uniform in [1e1, 1e8], 1-16 right-hand sides, 20% of the requests repeating an earlier matrix id, a
+    200 GB budget, all submitted at once so small batches and large solves overlap.  Every request converges to
+    eta <= 1e-15, and every result equals, bit for bit, the same system solved outside the server: a batch of
+    that system alone (n <= 512), or its matrix's requests replayed in arrival order on a fresh problem with the
+    server's parameters (reuse and resume_ladder: a repeat starts at the stage that served the matrix last)."""
+    rng = np.random.default_rng(9404)
+    nreq, npool = 40, 32
+    ns = np.exp(rng.uniform(np.log(256), np.log(16384), size=npool)).astype(np.int64)
+    ks = 10.0 ** rng.uniform(1, 8, size=npool)
+    mats, plan, used = {}, [], []
+    for i in range(nreq):
+        if used and rng.uniform() < 0.2:
+            mid = int(rng.choice(used))
+        else:
+            mid = len(used); used.append(mid)
+        plan.append((mid, int(rng.integers(1, 17))))
+    for mid in used:
+        mats[mid] = _gen(ctx, int(ns[mid]), 1, float(ks[mid]), 9500 + mid)[0]
+    print(f"[c4 stream] {nreq} requests over {len(used)} matrices, n {ns[used].min()}..{ns[used].max()}", flush=True)
+    reqs = []
+    for i, (mid, nrhs) in enumerate(plan):
+        n = int(ns[mid])
+        _, dB = _gen(ctx, n, nrhs, 10.0, 2000 + i)
+        reqs.append((mid, n, nrhs, dB, np.zeros((n, nrhs), order="F")))
+    nb = 256
+    with Server(tau=TAU, hbm_budget_bytes=200 << 30, solve={"nb": nb}) as S:
+        ids = {}
+        for r in reqs:
+            ids[S.submit(r[0], r[1], r[2], mats[r[0]], r[3], r[4])] = r
+        done = {}
+        for _ in range(3000):
+            for c in S.poll(wait=0.1):
+                done[c["id"]] = c
+            if len(done) == len(ids):
+                break
+        stats = S.stats()
+    print(f"[c4 stream] {stats}", flush=True)
+    assert len(done) == len(ids) and stats["errors"] == 0, stats
+    assert stats["cache_hits"] >= 1
+    order = sorted(ids)                                        # request ids follow submission order
+    replay = {}
+    for rid in order:
+        mid, n, nrhs, dB, X = ids[rid]
+        c = done[rid]
+        assert c["terminal"] == "converged" and c["eta_max"] <= TAU, (rid, n, c)
+        if n <= 512:
+            Xa, ra = _alone_small(ctx, mats[mid], dB, n, nrhs)
+        else:
+            if mid not in replay:
+                replay[mid] = ctx.problem_from_device(n, nrhs, mats[mid], dB)
+            else:
+                replay[mid].set_rhs(dB, nrhs)
+            Xa, ra = replay[mid].solve("policy_mx16x3_ladder", nb=nb, tau=TAU, stop_on="eta", reuse=True, resume_ladder=True)
+        np.testing.assert_array_equal(X, Xa)
+    for P in replay.values():
+        P.close()

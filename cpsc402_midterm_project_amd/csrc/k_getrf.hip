@@ -369,7 +369,7 @@ struct Panel2Args {
 __host__ __device__ __forceinline__ int p2_pfloats(int kb) { return (kb * P2_PS + 3) & ~3; }
 static size_t panel2_lds_bytes(int kb) {
   return (size_t)p2_pfloats(kb) * 4 + 32 * P2_US * 4 + 32 * 32 * 4 + 32 * 4 + 32 * 4 + 256 * 4 + 32 * 4 +
-         P2_R * 4 * 3 + 16 * 4;
+         P2_R * 4 * 3 + 16 * 4 + 2 * 8 * P2_US * 4;
 }
 
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
@@ -407,69 +407,75 @@ __device__ __forceinline__ uint64_t head_word(uint32_t tag, int pos, uint32_t vb
   return ((uint64_t)(tag & 0x1FFFFu) << 47) | ((uint64_t)(pos & 0xFFFF) << 31) | (uint64_t)(vbits & 0x7FFFFFFFu);
 }
 
-__device__ __forceinline__ void p2_subst(const float* LsT, int zmask, float (&u)[32]);
-// Poll the 32 U-row words of one trailing column (stride 256 words) and turn them into the final U values
-// by forward substitution, right-looking so the 31 updates per step are independent (each u_q still
-// receives its updates in increasing d).  LsT[d][q] = l_{q,d}; steps whose pivot was zero are skipped.
-__device__ __forceinline__ void p2_u_column(const uint64_t* xs, uint32_t stag, const float* LsT, int zmask,
-                                            volatile lds_i32* s_abort, float (&u)[32]) {
+// Poll four tagged U-row words (q, q + 1, q + 2, q + 3 of one column: stride 256 words) into u; a word that
+// never arrives sets the abort flag and reads as zero.
+__device__ __forceinline__ void p2_poll4(const uint64_t* xs, uint32_t tag, volatile lds_i32* s_abort, float (&u)[4]) {
+  uint32_t need = 0xFu;
+  for (long it = 0; it < SPIN_MAX && need; it++) {
+    uint64_t w[4];
 #pragma unroll
-  for (int q16 = 0; q16 < 32; q16 += 16) {
-    uint32_t need = 0xFFFFu;
-    for (long it = 0; it < SPIN_MAX && need; it++) {
-      uint64_t wq[16];
+    for (int i = 0; i < 4; i++)
+      w[i] = (need >> i & 1u) ? __hip_atomic_load(xs + i * 256, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
 #pragma unroll
-      for (int i = 0; i < 16; i++)
-        wq[i] = (need >> i & 1u) ? __hip_atomic_load(xs + (q16 + i) * 256, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+    for (int i = 0; i < 4; i++)
+      if ((need >> i & 1u) && (uint32_t)(w[i] >> 32) == tag) { u[i] = __uint_as_float((uint32_t)w[i]); need &= ~(1u << i); }
+    if (!need || *s_abort) break;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  if (need) {
+    *s_abort = 1;
 #pragma unroll
-      for (int i = 0; i < 16; i++)
-        if ((need >> i & 1u) && (uint32_t)(wq[i] >> 32) == stag) { u[q16 + i] = __uint_as_float((uint32_t)wq[i]); need &= ~(1u << i); }
-      if (!need || *s_abort) break;
-      __builtin_amdgcn_s_sleep(1);
-    }
-    if (need) {
-      *s_abort = 1;
+    for (int i = 0; i < 4; i++) if (need >> i & 1u) u[i] = 0.0f;
+  }
+}
+// Forward substitution of U columns spread over lanes: within each half-wave, lane = 8 * column + qb holds
+// the U rows 4 qb .. 4 qb + 3 of its column.  At step d the lane holding row d (final since step d - 1)
+// broadcasts u_d to its column's lanes (ds_swizzle), and every row q > d takes u_q -= l_{q,d} u_d: per row
+// the operations of sequential forward substitution, in increasing d.  Steps whose pivot was zero are
+// skipped.  LsT[d][q] = l_{q,d}.  All 64 lanes must be active.
+template <int D>
+__device__ __forceinline__ void p2_subst_step(const float* LsT, int zmask, int qb, float (&u)[4]) {
+  const float ud = __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(u[D & 3]), 0x18 | ((D >> 2) << 5)));
+  if (!((zmask >> D) & 1)) {
+    const float4 lq = *(const float4*)(LsT + D * 32 + 4 * qb);
+    const float le[4] = {lq.x, lq.y, lq.z, lq.w};
 #pragma unroll
-      for (int i = 0; i < 16; i++) if (need >> i & 1u) u[q16 + i] = 0.0f;
+    for (int e = 0; e < 4; e++) {
+      const float nv = fmaf(-le[e], ud, u[e]);
+      u[e] = (4 * qb + e > D) ? nv : u[e];
     }
   }
-  p2_subst(LsT, zmask, u);
+  if constexpr (D + 1 < 31) p2_subst_step<D + 1>(LsT, zmask, qb, u);
 }
-// Forward substitution of one U column (u holds the pivot rows' values in the column, in step order),
-// right-looking so the updates of one step are independent (each u_q still receives its updates in
-// increasing d); steps whose pivot was zero leave the column unchanged.  No branch depends on the data, so
-// the LsT reads of later steps can be issued ahead of the arithmetic.
-__device__ __forceinline__ void p2_subst(const float* LsT, int zmask, float (&u)[32]) {
-#pragma unroll
-  for (int d = 0; d < 31; d++) {
-    const bool skip = (zmask >> d & 1) != 0;
-    const float4* l4 = (const float4*)(LsT + d * 32);
+__device__ __forceinline__ void p2_subst_lanes(const float* LsT, int zmask, int qb, float (&u)[4]) {
+  p2_subst_step<0>(LsT, zmask, qb, u);
+}
+// Two rows' updates of one column: y -= sum_d l_d u_d in increasing d (steps in zmask skipped), u = Uc[0..31]
+// read as broadcasts shared by both rows.
+__device__ __forceinline__ void p2_chain2(float& ya, float& yb, const float (&la)[32], const float (&lb)[32],
+                                          const float* Uc, int zmask) {
+  const float4* u4 = (const float4*)Uc;
+  if (zmask == 0) {
 #pragma unroll
     for (int q4 = 0; q4 < 8; q4++) {
-      if (4 * q4 + 3 <= d) continue;
-      const float4 lq = l4[q4];
-      const float le[4] = {lq.x, lq.y, lq.z, lq.w};
+      const float4 v = u4[q4];
+      const float ue[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int e = 0; e < 4; e++) { ya = fmaf(-la[4 * q4 + e], ue[e], ya); yb = fmaf(-lb[4 * q4 + e], ue[e], yb); }
+    }
+  } else {
+#pragma unroll
+    for (int q4 = 0; q4 < 8; q4++) {
+      const float4 v = u4[q4];
+      const float ue[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
       for (int e = 0; e < 4; e++) {
-        const int q = 4 * q4 + e;
-        if (q > d) { const float nv = fmaf(-le[e], u[d], u[q]); u[q] = skip ? u[q] : nv; }
+        const bool skip = (zmask >> (4 * q4 + e) & 1) != 0;
+        const float na = fmaf(-la[4 * q4 + e], ue[e], ya), nb = fmaf(-lb[4 * q4 + e], ue[e], yb);
+        ya = skip ? ya : na; yb = skip ? yb : nb;
       }
     }
   }
-}
-// a -= sum_d l_d u_d over the steps in emask (bit d), l_d = P[(c0 + d) * P2_PS + r]
-__device__ __forceinline__ float p2_apply(float a, const float* P, int c0, int r, const float (&u)[32], uint32_t em) {
-  if (em == 0xFFFFFFFFu) {
-#pragma unroll
-    for (int d = 0; d < 32; d++) a = fmaf(-P[(c0 + d) * P2_PS + r], u[d], a);
-  } else {
-#pragma unroll
-    for (int d = 0; d < 32; d++) {
-      const float nv = fmaf(-P[(c0 + d) * P2_PS + r], u[d], a);
-      a = (em >> d & 1u) ? nv : a;
-    }
-  }
-  return a;
 }
 __device__ __forceinline__ uint32_t p2_emask(int dlim, int zmask) {
   if (dlim <= 0) return 0u;
@@ -490,6 +496,7 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
   int* pstep_bg = pstep + P2_R;                           // [128] copy for the background update
   int* spos = pstep_bg + P2_R;                            // [128]
   int* misc = spos + P2_R;                                // [16]
+  float* UsF = (float*)(misc + 16);                       // [2 waves][8][P2_US] background U columns
   volatile lds_i32* s_abort = LDS_FLAG(misc + 0);
   int* s_p = misc + 1; int* s_pw = misc + 2; int* s_info = misc + 3; int* s_zmask = misc + 4;
   float* s_rinv = (float*)(misc + 5);
@@ -517,17 +524,34 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
     const int w = min(32, kb - c0);
     int ps0 = -1, ps1 = -1;                               // -1: pivot before this sub-block (or no row)
     if (wave >= 2) {
-      // far columns of the previous sub-block: thread per column, all 128 rows
+      // far columns of the previous sub-block, 8 per pass per wave: the wave polls and substitutes the pass's 8
+      // U columns (p2_subst_lanes layout), stages them in LDS and updates rows lane and lane + 64 of those
+      // columns, the rows' multipliers held in registers.  A row that was a pivot step of that sub-block takes
+      // its substituted U value (the same operations on the same operands), earlier pivots stay as they are.
       if (bg_c0 >= 0) {
         const uint32_t btag = (uint32_t)(a.k + bg_c0 + 1);
         const uint64_t* xsb = a.xbuf + P2_OFF_SUB + (int64_t)(((bg_c0 >> 5) & 1) * P2_NREP + myrep) * P2_SUBREP;
-        for (int fc = bg_c0 + 64 + (t - 128); fc < kb; fc += 128) {
-          float u[32];
-          p2_u_column(xsb + fc, btag, LsT, bg_zmask, s_abort, u);
-          for (int r = 0; r < P2_R; r++) {
-            const uint32_t em = p2_emask(pstep_bg[r], bg_zmask);
-            if (em == 0u) continue;
-            P[fc * P2_PS + r] = p2_apply(P[fc * P2_PS + r], P, bg_c0, r, u, em);
+        const int wb = wave - 2, cl = lane >> 3, qb = lane & 7;
+        float* Uw = UsF + wb * 8 * P2_US;
+        const int psa = pstep_bg[lane], psb = pstep_bg[lane + 64];
+        float la[32], lb[32];
+#pragma unroll
+        for (int d = 0; d < 32; d++) { la[d] = P[(bg_c0 + d) * P2_PS + lane]; lb[d] = P[(bg_c0 + d) * P2_PS + lane + 64]; }
+        for (int fb = bg_c0 + 64 + 8 * wb; fb < kb; fb += 16) {
+          const int col = fb + cl;
+          float u[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+          if (col < kb) p2_poll4(xsb + (4 * qb) * 256 + col, btag, s_abort, u);
+          p2_subst_lanes(LsT, bg_zmask, qb, u);
+          *(float4*)(Uw + cl * P2_US + 4 * qb) = make_float4(u[0], u[1], u[2], u[3]);
+          lds_wait();
+          const int nc = min(8, kb - fb);
+          for (int c = 0; c < nc; c++) {
+            const int fc = fb + c;
+            const float* Uc = Uw + c * P2_US;
+            float ya = P[fc * P2_PS + lane], yb = P[fc * P2_PS + lane + 64];
+            p2_chain2(ya, yb, la, lb, Uc, bg_zmask);
+            if (psa >= 0) P[fc * P2_PS + lane] = (psa == 32) ? ya : Uc[min(psa, 31)];
+            if (psb >= 0) P[fc * P2_PS + lane + 64] = (psb == 32) ? yb : Uc[min(psb, 31)];
           }
         }
       }
@@ -779,24 +803,25 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
     }
     lds_barrier();
     PANEL2_MARK(c0 + 2, 7);
-    // next nw columns: forward substitution of the U rows (thread per column), in place
-    if (t < nw) {
-      float u[32];
-      float4* u4 = (float4*)(Us + t * P2_US);
-#pragma unroll
-      for (int q4 = 0; q4 < 8; q4++) { const float4 v = u4[q4]; u[4 * q4] = v.x; u[4 * q4 + 1] = v.y; u[4 * q4 + 2] = v.z; u[4 * q4 + 3] = v.w; }
-      p2_subst(LsT, zmask, u);
-#pragma unroll
-      for (int q4 = 0; q4 < 8; q4++) u4[q4] = make_float4(u[4 * q4], u[4 * q4 + 1], u[4 * q4 + 2], u[4 * q4 + 3]);
+    // next nw columns: forward substitution of the U rows, all four waves (8 columns each), in place
+    {
+      const int cl = lane >> 3, qb = lane & 7, col = 8 * wave + cl;
+      float4* up = (float4*)(Us + col * P2_US + 4 * qb);
+      const float4 v = *up;
+      float u[4] = {v.x, v.y, v.z, v.w};
+      p2_subst_lanes(LsT, zmask, qb, u);
+      if (col < nw) *up = make_float4(u[0], u[1], u[2], u[3]);
     }
     lds_barrier();
     PANEL2_MARK(c0 + 3, 7);
     {
       // thread per (row, half of the columns): the row's 32 multipliers stay in registers, the U columns are
-      // read as broadcasts; each element gets a -= l_d u_d in increasing d, as in p2_apply
+      // read as broadcasts; each element gets a -= l_d u_d in increasing d (steps with a zero pivot skipped).
+      // A row that was pivot step ps of this sub-block takes U row ps (the same operations on the same
+      // operands); rows pivoted earlier keep their values.
       const int r = t & (P2_R - 1), ch = t >> 7;
-      const uint32_t em = p2_emask(pstep[r], zmask);
-      if (em != 0u) {
+      const int ps = pstep[r];
+      if (ps >= 0) {
         float l[32];
 #pragma unroll
         for (int d = 0; d < 32; d++) l[d] = P[(c0 + d) * P2_PS + r];
@@ -812,7 +837,7 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
             acc[i] = P[(c0 + 32 + col) * P2_PS + r];
             u4[i] = (const float4*)(Us + col * P2_US);
           }
-          if (em == 0xFFFFFFFFu) {
+          if (zmask == 0) {
 #pragma unroll
             for (int q4 = 0; q4 < 8; q4++) {
 #pragma unroll
@@ -823,7 +848,6 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
               }
             }
           } else {
-            // a row that became a pivot in this sub-block, or a step with a zero pivot
 #pragma unroll
             for (int q4 = 0; q4 < 8; q4++) {
 #pragma unroll
@@ -834,7 +858,7 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
                 for (int e = 0; e < 4; e++) {
                   const int d = 4 * q4 + e;
                   const float nv = fmaf(-l[d], ue[e], acc[i]);
-                  acc[i] = (em >> d & 1u) ? nv : acc[i];
+                  acc[i] = (zmask >> d & 1) ? acc[i] : nv;
                 }
               }
             }
@@ -842,7 +866,7 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
 #pragma unroll
           for (int i = 0; i < 4; i++) {
             const int col = ch * 16 + c4 + i;
-            if (col < nw) P[(c0 + 32 + col) * P2_PS + r] = acc[i];
+            if (col < nw) P[(c0 + 32 + col) * P2_PS + r] = (ps == 32) ? acc[i] : Us[col * P2_US + min(ps, 31)];
           }
         }
       }
@@ -1284,6 +1308,246 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ S, int64
   if (n_sub) atomicAdd(cnt + 1, n_sub);
 }
 
+// ------------------------------------------------------------------ the next panel's columns in one launch
+// k_next_prep: for 32 columns per workgroup of the next panel's range [c0, c0 + ncols), everything the
+// look-ahead does before their trailing update, in one launch with the strip held in registers:
+//   1. the row interchanges of panel k (the swap plan: new[dst] = old[src], panel-local rows): the top kb rows
+//      are loaded through the map the plan defines, the moved rows below are gathered, and written back once
+//      every load of the workgroup has completed;
+//   2. U12 <- L11^{-1} U12 with the arithmetic of k_trsm_mfma: wave w owns 32-row block w of the strip in
+//      accumulator layout and, for b < w in increasing b, adds (-L_wb) X_b from a tile wave b published in LDS,
+//      then forms X_w = T_w^{-1} X_w.  The products and their order per block are k_trsm_mfma's, so the result
+//      is the same bits; the chain between blocks is one block's 16 products instead of a whole strip's;
+//   3. the packed FP16 / BF16 / FP8 / FP32 (or two-term FP16) U operand of the strip's 32-column tile, with its
+//      power-of-two exponent and the conversion counters, as k_pack writes it.
+// The diagonal-block inverses are formed per workgroup (one per wave) and the first workgroup stores them for
+// the TRSM of the remaining columns.  kb and ncols are multiples of 32, kb <= 256.
+constexpr int NP_THREADS = 512;
+// block row of block index blk in the order (0,0), (1,0), (1,1), (2,0), ... (rb * (rb + 1) / 2 + bb)
+__host__ __device__ constexpr int tri_row(int blk) { int rb = 0; while ((rb + 1) * (rb + 2) / 2 <= blk) rb++; return rb; }
+static size_t next_prep_lds_bytes() { return (size_t)36 * TM_BLK * 4 + 64 * 16 * 4 + 256 * 4; }
+
+template <int FMT>
+__global__ __launch_bounds__(NP_THREADS) void k_next_prep(float* __restrict__ A, int64_t lda, int64_t k, int kb, int64_t c0,
+                                                          const int32_t* __restrict__ plan, int32_t* __restrict__ zero_plan,
+                                                          float* __restrict__ Linv_out, uint8_t* __restrict__ outU,
+                                                          uint8_t* __restrict__ outUlo, int32_t* __restrict__ expsU,
+                                                          unsigned long long* __restrict__ cnt, int tile_scale) {
+  using FI = FmtInfo<FMT>;
+  extern __shared__ float sm[];
+  float* Xex = sm + 36 * TM_BLK;                     // [64 lanes][16]: the block a wave publishes
+  int* rmap = (int*)(Xex + 64 * 16);                 // [256]: source row of each of the top kb rows
+  __shared__ float red[NP_THREADS / 64];
+  __shared__ int s_misc[2];
+  const int t = threadIdx.x, w = t >> 6, l = t & 63, h = l >> 5, li = l & 31;
+  const int nrb = kb / 32;
+  const int64_t cs = c0 + (int64_t)blockIdx.x * 32;  // first column of this strip
+  if (blockIdx.x == 0 && t == 0 && zero_plan != nullptr) *zero_plan = 0;
+  const int npairs = plan[0];
+  // row map of the top kb rows: identity, then the plan's destinations above kb
+  for (int r = t; r < kb; r += NP_THREADS) rmap[r] = r;
+  __syncthreads();
+  for (int e = t; e < npairs; e += NP_THREADS) {
+    const int d = plan[1 + 2 * e];
+    if (d < kb) rmap[d] = plan[2 + 2 * e];
+  }
+  __syncthreads();
+  float* Ac = A + k;                                 // panel-local row r of column c: Ac[r + c * lda]
+  // (1) every load first: the moved rows below kb (two rounds of pairs per thread) and the strip rows
+  int dlo[2] = {-1, -1};
+  float vlo[2][32];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int e = t + NP_THREADS * q;
+    const int d = (e < npairs) ? plan[1 + 2 * e] : -1;
+    if (d >= kb) {
+      dlo[q] = d;
+      const int sr = plan[2 + 2 * e];
+#pragma unroll
+      for (int c = 0; c < 32; c++) vlo[q][c] = Ac[sr + (cs + c) * lda];
+    }
+  }
+  f32x16_t X;
+  if (w < nrb) {
+#pragma unroll
+    for (int a = 0; a < 16; a++) {
+      const int row = 32 * w + 8 * (a >> 2) + 4 * h + (a & 3);
+      X[a] = Ac[rmap[row] + (cs + li) * lda];
+    }
+  }
+  // L11 operand blocks into LDS: block (rb, bb), bb < rb, holds -L_rb,bb; the diagonal blocks get T^{-1} below.
+  // Element e = t + 512 j of the block sequence lies in block j / 2 (a compile-time index), at offset
+  // t + 512 (j % 2); the loads of 24 elements are issued before any of them is stored (a load consumed at once
+  // is waited for on its own).  Blocks past kb read block (0, 0) instead, which always lies in the panel, and
+  // store nothing.
+  {
+    const int nblk = nrb * (nrb + 1) / 2;
+#pragma unroll
+    for (int j0 = 0; j0 < 72; j0 += 24) {
+      float v[24];
+#pragma unroll
+      for (int jj = 0; jj < 24; jj++) {
+        const int j = j0 + jj, blk = j >> 1, rb = tri_row(blk), bb = blk - rb * (rb + 1) / 2;
+        const int off = t + NP_THREADS * (j & 1), col = off >> 5, row = off & 31;
+        const int rbc = blk < nblk ? rb : 0, bbc = blk < nblk ? bb : 0;
+        if (bb < rb) v[jj] = A[(k + rbc * 32 + row) + (k + bbc * 32 + col) * lda];
+      }
+#pragma unroll
+      for (int jj = 0; jj < 24; jj++) {
+        const int j = j0 + jj, blk = j >> 1, rb = tri_row(blk), bb = blk - rb * (rb + 1) / 2;
+        const int off = t + NP_THREADS * (j & 1), col = off >> 5, row = off & 31;
+        if (bb < rb && blk < nblk) sm[blk * TM_BLK + row * 33 + col] = -v[jj];
+      }
+    }
+  }
+  // diagonal-block inverses: wave w inverts T_w (lane c < 32: column c by forward substitution, the order of
+  // trsm_inv_block), staged in its own diagonal slot
+  if (w < nrb) {
+    float* T = sm + (w * (w + 1) / 2 + w) * TM_BLK;
+    {
+      float tv[16];
+#pragma unroll
+      for (int i = 0; i < 16; i++) { const int e = l + 64 * i, q = e >> 5, r = e & 31; tv[i] = A[(k + 32 * w + r) + (k + 32 * w + q) * lda]; }
+#pragma unroll
+      for (int i = 0; i < 16; i++) { const int e = l + 64 * i, q = e >> 5, r = e & 31; T[r * 33 + q] = tv[i]; }
+    }
+    lds_wait();
+    float x[32];
+    if (l < 32) {
+#pragma unroll
+      for (int r = 0; r < 32; r++) x[r] = (r == l) ? 1.0f : 0.0f;
+#pragma unroll
+      for (int r = 1; r < 32; r++) {
+        float acc = x[r];
+#pragma unroll
+        for (int q = 0; q < r; q++) if (q >= l) acc = fmaf(-T[r * 33 + q], x[q], acc);
+        if (r > l) x[r] = acc;
+      }
+    }
+    lds_wait();                                        // every lane's reads of T precede the overwrite
+    if (l < 32) {
+#pragma unroll
+      for (int r = 0; r < 32; r++) T[r * 33 + l] = x[r];
+      if (blockIdx.x == 0) {
+#pragma unroll
+        for (int r = 0; r < 32; r++) Linv_out[(size_t)w * 1024 + l * 32 + r] = x[r];
+      }
+    }
+  }
+  __syncthreads();                                     // loads of this workgroup complete; LDS operands ready
+  // the moved rows below kb
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    if (dlo[q] >= 0) {
+#pragma unroll
+      for (int c = 0; c < 32; c++) Ac[dlo[q] + (cs + c) * lda] = vlo[q][c];
+    }
+  }
+  // (2) the triangular solve, block by block
+  auto kidx = [&](int s2) { return 8 * (s2 >> 2) + 4 * h + (s2 & 3); };
+  for (int b = 0; b < nrb; b++) {
+    if (w == b) {
+      const float* M = sm + (b * (b + 1) / 2 + b) * TM_BLK + li * 33;
+      f32x16_t y;
+#pragma unroll
+      for (int a = 0; a < 16; a++) y[a] = 0.0f;
+#pragma unroll
+      for (int s2 = 0; s2 < 16; s2++) y = __builtin_amdgcn_mfma_f32_32x32x2f32(M[kidx(s2)], X[s2], y, 0, 0, 0);
+      X = y;
+#pragma unroll
+      for (int a = 0; a < 16; a++) Xex[l * 16 + a] = X[a];
+    }
+    __syncthreads();
+    if (w > b && w < nrb) {
+      const float* M = sm + (w * (w + 1) / 2 + b) * TM_BLK + li * 33;
+      float xb[16];
+#pragma unroll
+      for (int a = 0; a < 16; a++) xb[a] = Xex[l * 16 + a];
+#pragma unroll
+      for (int s2 = 0; s2 < 16; s2++) X = __builtin_amdgcn_mfma_f32_32x32x2f32(M[kidx(s2)], xb[s2], X, 0, 0, 0);
+    }
+    __syncthreads();                                   // Xex is rewritten by the next block's owner
+  }
+  // U12 back into A, and the strip staged row-major in LDS (the operand blocks are no longer needed)
+  float* St = sm;                                      // [256 rows][33]
+  if (w < nrb) {
+#pragma unroll
+    for (int a = 0; a < 16; a++) {
+      const int row = 32 * w + 8 * (a >> 2) + 4 * h + (a & 3);
+      Ac[row + (cs + li) * lda] = X[a];
+      St[row * 33 + li] = X[a];
+    }
+  }
+  // (3) tile maximum (NaN counts as the largest) and the power-of-two exponent of k_pack
+  float m = 0.0f;
+  if (w < nrb) {
+#pragma unroll
+    for (int a = 0; a < 16; a++) { const float av = fabsf(X[a]); m = (av != av || av > m) ? av : m; }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const float o = __shfl_xor(m, off, 64);
+    m = (o != o || o > m) ? o : m;
+  }
+  if (l == 0) red[w] = m;
+  __syncthreads();
+  if (t == 0) {
+    float tm = red[0];
+    for (int i = 1; i < NP_THREADS / 64; i++) { const float o = red[i]; tm = (o != o || o > tm) ? o : tm; }
+    int sc = 0;
+    if (tile_scale && FMT != FMT_FP32 && tm > 0.0f && tm <= 3.40282347e+38f) {
+      int e2; frexpf(tm, &e2);
+      sc = FI::TOPEXP - (e2 - 1);
+      sc = max(-120, min(120, sc));
+    }
+    s_misc[0] = sc;
+    expsU[blockIdx.x] = sc;
+  }
+  __syncthreads();
+  const float scl = ldexpf(1.0f, s_misc[0]);
+  const int nkb = kb / 16;
+  unsigned long long n_over = 0, n_sub = 0;
+  for (int u = t; u < nkb * 64; u += NP_THREADS) {
+    const int kblk = u >> 6, ln = u & 63;
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int kk = (FMT == FMT_FP32) ? (kblk * 16 + 2 * e + (ln >> 5)) : (kblk * 16 + (ln >> 5) * 8 + e);
+      v[e] = St[kk * 33 + (ln & 31)] * scl;
+      const float av = fabsf(v[e]);
+      if (av > FI::MAXV) n_over++; else if (av > 0.0f && av < FI::MINN) n_sub++;
+    }
+    const size_t o = ((size_t)((size_t)blockIdx.x * nkb + kblk) * 64 + ln) * 8;
+    if (FMT == FMT_FP16 || FMT == FMT_FP16X3) {
+      union { uint16_t h[8]; uint4 w4; } hi, lo;
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const _Float16 hv = (_Float16)v[e];
+        hi.h[e] = __builtin_bit_cast(uint16_t, hv);
+        if (FMT == FMT_FP16X3) lo.h[e] = __builtin_bit_cast(uint16_t, (_Float16)(v[e] - (float)hv));
+      }
+      *(uint4*)((uint16_t*)outU + o) = hi.w4;
+      if (FMT == FMT_FP16X3) *(uint4*)((uint16_t*)outUlo + o) = lo.w4;
+    } else if (FMT == FMT_BF16) {
+      union { uint16_t h[8]; uint4 w4; } hi;
+#pragma unroll
+      for (int e = 0; e < 8; e++) hi.h[e] = to_bf16_rne(v[e]);
+      *(uint4*)((uint16_t*)outU + o) = hi.w4;
+    } else if (FMT == FMT_FP8) {
+      union { uint8_t h[8]; uint2 w2; } q8;
+#pragma unroll
+      for (int e = 0; e < 8; e++) q8.h[e] = to_e4m3_rne(v[e]);
+      *(uint2*)(outU + o) = q8.w2;
+    } else {
+      float4* d = (float4*)((float*)outU + o);
+      d[0] = make_float4(v[0], v[1], v[2], v[3]);
+      d[1] = make_float4(v[4], v[5], v[6], v[7]);
+    }
+  }
+  if (n_over) atomicAdd(cnt, n_over);
+  if (n_sub) atomicAdd(cnt + 1, n_sub);
+}
+
 // ------------------------------------------------------------------ trailing update on MFMA
 
 template <int FMT> struct Frag;
@@ -1507,24 +1771,57 @@ static void gemm_cols(float* A, int64_t lda, int64_t k, int kb, int64_t m2, int6
 // with_L: pack L21 of step k here (on the main stream) before the update; otherwise it was packed ahead
 template <int FMT>
 static void update_cols(float* A, int64_t lda, int64_t k, int kb, int nb, int64_t m2, int64_t cb, int64_t ce, bool last,
-                        bool with_L, bool nxt, bool tile_scale, Workspace& W) {
+                        bool with_L, bool nxt, bool tile_scale, Workspace& W, bool u_packed = false) {
   // the next panel's columns (look-ahead) are profiled under their own names.  The packed-L pointers are
   // taken after pack_buffers, which allocates the sets on first use.
   if (with_L) pack_buffers<FMT>(kb, m2, W);
   const LPack lp = lpack(W, k, nb);
   if (with_L) pack_L<FMT>(A, lda, k, kb, m2, tile_scale, W, nxt ? "pack_next" : "pack", lp, W.stream);
-  pack_U<FMT>(A, lda, k, kb, cb, ce, last, tile_scale, W, nxt ? "pack_next" : "pack");
+  if (!u_packed) pack_U<FMT>(A, lda, k, kb, cb, ce, last, tile_scale, W, nxt ? "pack_next" : "pack");
   gemm_cols<FMT>(A, lda, k, kb, m2, cb, ce, W, nxt ? "gemm_next" : "trailing_gemm", lp);
 }
 
+// The next panel's columns [cb, cb + ncols) brought up to date except for their trailing update (k_next_prep):
+// interchanges, TRSM and the packed U operand of step k, in one launch.
+template <int FMT>
+static void next_prep(float* A, int64_t lda, int64_t k, int kb, int nb, int64_t m2, int64_t cb, int ncols, const int32_t* plan,
+                      int32_t* zero_plan, bool tile_scale, Workspace& W) {
+  static bool attr = false;
+  if (!attr) {
+    HIP_CK(hipFuncSetAttribute((const void*)k_next_prep<FMT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)next_prep_lds_bytes()));
+    attr = true;
+  }
+  pack_buffers<FMT>(kb, m2, W);
+  constexpr int ES = FmtInfo<FMT>::ES;
+  const int nkb = kb / 16;
+  const size_t tb = (size_t)nkb * 64 * 8 * ES;
+  const int64_t t0 = (cb - k - kb) / 32;
+  hipEvent_t ev = W.prof_begin();
+  hipLaunchKernelGGL((k_next_prep<FMT>), dim3((unsigned)(ncols / 32)), dim3(NP_THREADS), next_prep_lds_bytes(), W.stream,
+                     A, lda, k, kb, cb, plan, zero_plan, W.linv.p, W.packU.p + t0 * tb,
+                     FMT == FMT_FP16X3 ? W.packUlo.p + t0 * tb : nullptr, W.scaleU.p + t0, W.cnt.p, tile_scale ? 1 : 0);
+  LAUNCH_CK();
+  W.prof_end(ev, "prep_next", (16.0 + 4.0 + ES) * (double)kb * ncols, (double)kb * kb * ncols);
+}
+
 static void update_cols_fmt(int fmt, float* A, int64_t lda, int64_t k, int kb, int nb, int64_t m2, int64_t cb, int64_t ce,
-                            bool last, bool with_L, bool nxt, bool tile_scale, Workspace& W) {
+                            bool last, bool with_L, bool nxt, bool tile_scale, Workspace& W, bool u_packed = false) {
   switch (fmt) {
-    case FMT_FP16: update_cols<FMT_FP16>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W); break;
-    case FMT_BF16: update_cols<FMT_BF16>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W); break;
-    case FMT_FP8:  update_cols<FMT_FP8>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W); break;
-    case FMT_FP16X3: update_cols<FMT_FP16X3>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W); break;
-    default:       update_cols<FMT_FP32>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W); break;
+    case FMT_FP16: update_cols<FMT_FP16>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W, u_packed); break;
+    case FMT_BF16: update_cols<FMT_BF16>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W, u_packed); break;
+    case FMT_FP8:  update_cols<FMT_FP8>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W, u_packed); break;
+    case FMT_FP16X3: update_cols<FMT_FP16X3>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W, u_packed); break;
+    default:       update_cols<FMT_FP32>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W, u_packed); break;
+  }
+}
+static void next_prep_fmt(int fmt, float* A, int64_t lda, int64_t k, int kb, int nb, int64_t m2, int64_t cb, int ncols,
+                          const int32_t* plan, int32_t* zero_plan, bool tile_scale, Workspace& W) {
+  switch (fmt) {
+    case FMT_FP16: next_prep<FMT_FP16>(A, lda, k, kb, nb, m2, cb, ncols, plan, zero_plan, tile_scale, W); break;
+    case FMT_BF16: next_prep<FMT_BF16>(A, lda, k, kb, nb, m2, cb, ncols, plan, zero_plan, tile_scale, W); break;
+    case FMT_FP8:  next_prep<FMT_FP8>(A, lda, k, kb, nb, m2, cb, ncols, plan, zero_plan, tile_scale, W); break;
+    case FMT_FP16X3: next_prep<FMT_FP16X3>(A, lda, k, kb, nb, m2, cb, ncols, plan, zero_plan, tile_scale, W); break;
+    default:       next_prep<FMT_FP32>(A, lda, k, kb, nb, m2, cb, ncols, plan, zero_plan, tile_scale, W); break;
   }
 }
 
@@ -1690,6 +1987,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   const bool allow_p2 = !(getenv("MPSOLVE_PANEL") && atoi(getenv("MPSOLVE_PANEL")) == 1);
   const bool use_fma_trsm = getenv("MPSOLVE_TRSM") && atoi(getenv("MPSOLVE_TRSM")) == 1;
   const bool overlap = !(getenv("MPSOLVE_OVERLAP") && atoi(getenv("MPSOLVE_OVERLAP")) == 0) && W.pstream && !use_fma_trsm;
+  const bool fused_next = !(getenv("MPSOLVE_FUSED_NEXT") && atoi(getenv("MPSOLVE_FUSED_NEXT")) == 0);
   auto plan_of = [&](int64_t k) { return W.affect.p + ((k / nb) & 1) * PLAN_WORDS; };
   int panels = 0;
   run_panel(a, W, 0, (int)std::min<int64_t>(nb, n), plan_of(0), W.stream, true, allow_p2, rmin);
@@ -1708,7 +2006,11 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
     // the diagonal-block inverses for the TRSM are formed by extra workgroups of the first interchange launch
     // and the next panel's swap-plan counter is cleared by it (its last reader was the previous step's)
     int32_t* next_plan = plan_of(k + kb);
-    if (split) {
+    // the next panel's interchanges, TRSM and U pack in one launch (k_next_prep) when the widths allow
+    const bool fused = split && fused_next && (kb % 32) == 0 && (kbn % 32) == 0;
+    if (fused) {
+      next_prep_fmt(a.fmt, a.A, a.lda, k, kb, nb, m2, k + kb, kbn, plan, next_plan, a.tile_scale, W);
+    } else if (split) {
       laswp_ranges(a, W, k, kb, k + kb, cN, 0, 0, false, plan, !use_fma_trsm, next_plan, "laswp_next");
     } else {
       laswp_ranges(a, W, k, kb, 0, k, k + kb, n, true, plan, !use_fma_trsm, next_plan);
@@ -1722,9 +2024,9 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
     // the update needs this step's packed L21: when it was packed ahead, wait for that pack only here, so the
     // interchanges and the TRSM above run while it finishes
     if (split) {
-      trsm_cols(a, W, k, kb, k + kb, cN, false, "trsm_next");
+      if (!fused) trsm_cols(a, W, k, kb, k + kb, cN, false, "trsm_next");
       if (l_ahead) HIP_CK(hipStreamWaitEvent(W.stream, W.ev_pack, 0));
-      update_cols_fmt(a.fmt, a.A, a.lda, k, kb, nb, m2, k + kb, cN, false, !l_ahead, true, a.tile_scale, W);
+      update_cols_fmt(a.fmt, a.A, a.lda, k, kb, nb, m2, k + kb, cN, false, !l_ahead, true, a.tile_scale, W, fused);
       HIP_CK(hipEventRecord(W.ev_next, W.stream));
       HIP_CK(hipStreamWaitEvent(W.pstream, W.ev_next, 0));
       run_panel(a, W, k + kb, kbn, plan_of(k + kb), W.pstream, false, allow_p2, rmin);

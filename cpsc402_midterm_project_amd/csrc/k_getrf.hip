@@ -104,14 +104,16 @@ __device__ __forceinline__ uint32_t get_word_b(const uint64_t* p, uint32_t tag, 
 
 // Wave-wide maximum with DPP (row_shr 1/2/4/8 inside rows of 16 lanes, then row_bcast 15/31 across rows):
 // a handful of VALU ops instead of a chain of LDS-routed shuffles.  Inputs are never NaN here.
+// Inputs are non-negative floats or the marker -1: on those, the order of the bit patterns as signed integers
+// is the order of the values, so an integer max (one DPP-fused instruction per stage, no canonicalization).
 __device__ __forceinline__ float wave_max_dpp(float v) {
   int x = __float_as_int(v);
-  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x111, 0xf, 0xf, false))); x = __float_as_int(v);
-  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x112, 0xf, 0xf, false))); x = __float_as_int(v);
-  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x114, 0xf, 0xf, false))); x = __float_as_int(v);
-  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x118, 0xf, 0xf, false))); x = __float_as_int(v);
-  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x142, 0xa, 0xf, false))); x = __float_as_int(v);
-  v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x143, 0xc, 0xf, false))); x = __float_as_int(v);
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x111, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x112, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x114, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x118, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x142, 0xa, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x143, 0xc, 0xf, false));
   return __int_as_float(__builtin_amdgcn_readlane(x, 63));
 }
 // argmax over the wave: largest value, ties to the lowest lane (callers lay indices out increasing with the
@@ -364,13 +366,23 @@ struct Panel2Args {
   int64_t* ipiv; int32_t* dinfo; int32_t* plan;
 };
 
-// P occupies kb * P2_PS floats rounded up to a multiple of 4, so the float4-accessed arrays after it stay
-// 16-byte aligned.
+// LDS layout (bytes): the fixed-size arrays first, at compile-time offsets (16-byte aligned), so their
+// addresses are instruction immediates; the kb-dependent panel slice P last.
+constexpr int P2L_MISC = 0;                              // 16 ints: flags and step values
+constexpr int P2L_PROW = P2L_MISC + 16 * 4;              // 32 floats
+constexpr int P2L_CROW = P2L_PROW + 32 * 4;              // 32 floats
+constexpr int P2L_SPIV = P2L_CROW + 32 * 4;              // 256 ints
+constexpr int P2L_PIVL = P2L_SPIV + 256 * 4;             // 32 ints
+constexpr int P2L_PSTEP = P2L_PIVL + 32 * 4;             // P2_R ints
+constexpr int P2L_PSTEPBG = P2L_PSTEP + P2_R * 4;        // P2_R ints
+constexpr int P2L_SPOS = P2L_PSTEPBG + P2_R * 4;         // P2_R ints
+constexpr int P2L_US = P2L_SPOS + P2_R * 4;              // [32][P2_US] floats
+constexpr int P2L_LST = P2L_US + 32 * P2_US * 4;         // [32][32] floats
+constexpr int P2L_USF = P2L_LST + 32 * 32 * 4;           // [2][8][P2_US] floats
+constexpr int P2L_P = P2L_USF + 2 * 8 * P2_US * 4;       // [kb][P2_PS] floats
+static_assert(P2L_US % 16 == 0 && P2L_LST % 16 == 0 && P2L_USF % 16 == 0 && P2L_P % 16 == 0, "LDS alignment");
 __host__ __device__ __forceinline__ int p2_pfloats(int kb) { return (kb * P2_PS + 3) & ~3; }
-static size_t panel2_lds_bytes(int kb) {
-  return (size_t)p2_pfloats(kb) * 4 + 32 * P2_US * 4 + 32 * 32 * 4 + 32 * 4 + 32 * 4 + 256 * 4 + 32 * 4 +
-         P2_R * 4 * 3 + 16 * 4 + 2 * 8 * P2_US * 4;
-}
+static size_t panel2_lds_bytes(int kb) { return (size_t)P2L_P + (size_t)p2_pfloats(kb) * 4; }
 
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 __device__ __forceinline__ void lds_wait() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
@@ -485,22 +497,21 @@ __device__ __forceinline__ uint32_t p2_emask(int dlim, int zmask) {
 __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
   extern __shared__ float lds[];
   const int kb = a.kb, m = a.m, G = a.G;
-  float* P = lds;                                         // [kb][P2_PS]
-  float* Us = P + p2_pfloats(kb);                         // [32][P2_US] U rows of the next 32 columns
-  float* LsT = Us + 32 * P2_US;                           // [32][32]: LsT[d][q] = l_{q,d}
-  float* prow = LsT + 32 * 32;                            // [32] winner's row of the current step
-  float* crow = prow + 32;                                // [32] this workgroup's candidate row
-  int* s_piv = (int*)(crow + 32);                         // [256]
-  int* piv_local = s_piv + 256;                           // [32]
-  int* pstep = piv_local + 32;                            // [128] step at which a row became pivot (or -1 / 32)
-  int* pstep_bg = pstep + P2_R;                           // [128] copy for the background update
-  int* spos = pstep_bg + P2_R;                            // [128]
-  int* misc = spos + P2_R;                                // [16]
-  float* UsF = (float*)(misc + 16);                       // [2 waves][8][P2_US] background U columns
+  char* lb = (char*)lds;
+  float* P = (float*)(lb + P2L_P);                        // [kb][P2_PS]
+  float* Us = (float*)(lb + P2L_US);                      // [32][P2_US] U rows of the next 32 columns
+  float* LsT = (float*)(lb + P2L_LST);                    // [32][32]: LsT[d][q] = l_{q,d}
+  float* prow = (float*)(lb + P2L_PROW);                  // [32] previous step's pivot row
+  float* crow = (float*)(lb + P2L_CROW);                  // [32] this workgroup's pivot row staging
+  int* s_piv = (int*)(lb + P2L_SPIV);                     // [256]
+  int* piv_local = (int*)(lb + P2L_PIVL);                 // [32]
+  int* pstep = (int*)(lb + P2L_PSTEP);                    // [128] step at which a row became pivot (or -1 / 32)
+  int* pstep_bg = (int*)(lb + P2L_PSTEPBG);               // [128] copy for the background update
+  int* spos = (int*)(lb + P2L_SPOS);                      // [128]
+  int* misc = (int*)(lb + P2L_MISC);                      // [16]
+  float* UsF = (float*)(lb + P2L_USF);                    // [2 waves][8][P2_US] background U columns
   volatile lds_i32* s_abort = LDS_FLAG(misc + 0);
-  int* s_p = misc + 1; int* s_pw = misc + 2; int* s_info = misc + 3; int* s_zmask = misc + 4;
-  float* s_rinv = (float*)(misc + 5);
-  volatile lds_i32* s_step = LDS_FLAG(misc + 6);
+  int* s_info = misc + 3; int* s_zmask = misc + 4;
 
   const int t = threadIdx.x, g = blockIdx.x, lane = t & 63, wave = t >> 6;
   const int r0 = g * P2_R;
@@ -509,7 +520,7 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
   uint64_t* mbox = a.xbuf;
   uint64_t* rowsl = a.xbuf + P2_OFF_ROWS;
   const int myrep = g % P2_NREP;
-  if (t == 0) { *s_abort = 0; *s_info = 0; *s_zmask = 0; *s_step = -1; }
+  if (t == 0) { *s_abort = 0; *s_info = 0; *s_zmask = 0; }
   for (int idx = t; idx < kb * P2_R; idx += 256) {
     const int c = idx >> 7, r = idx & (P2_R - 1);
     P[c * P2_PS + r] = (r < rows) ? Ap[(int64_t)(r0 + r) + (int64_t)c * a.lda] : 0.0f;
@@ -556,6 +567,10 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
         }
       }
     } else if (wave == 0) {
+      // Wave 0 runs every column step end to end: it polls its mailbox, picks the winner, reads the winner's
+      // row, updates its two rows per lane and pushes the next head, so no hand-off between waves lies on the
+      // chain of steps.  The only stores it has in flight while polling are the previous head and candidate
+      // row, which complete within the exchange round trip.
       float x0[32], x1[32];
 #pragma unroll
       for (int q = 0; q < 32; q++) {
@@ -564,6 +579,8 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
       }
       ps0 = (v0 && pos0 >= c0) ? 32 : -1;
       ps1 = (v1 && pos1 >= c0) ? 32 : -1;
+      if (lane == 0) *s_zmask = 0;
+      if (lane < 32) piv_local[lane] = -1;
       // Step order: the candidate of column j+1 is found from the rows' column-j+1 values alone (updated
       // first) and its head pushed out; the rest of the rank-1 update, the multipliers and the candidate's
       // row (which readers fetch only after the heads round trip) follow while the exchange is in flight.
@@ -608,94 +625,11 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
       publish_row(c0);
       for (int jj = 0; jj < w; jj++) {
         const int j = c0 + jj;
-        PANEL2_MARK(j, 0);
-        // wait for wave 1's step flag: winner, its row, 1 / pivot
-        for (long it = 0; it < SPIN_MAX && *s_step != j; it++) {}
-        PANEL2_MARK(j, 2);
-        const int p = *s_p, pw = *s_pw;
-        float pr[32];
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-          const float4 v4 = ((const float4*)prow)[q];
-          pr[4 * q] = v4.x; pr[4 * q + 1] = v4.y; pr[4 * q + 2] = v4.z; pr[4 * q + 3] = v4.w;
-        }
-        const int ql = lane & 31;
-        const float prq = prow[min(ql + 1, 31)];          // pivot-row value this lane contributes to the next row
-        const float rinv = *s_rinv;
-        lds_wait();                                       // prow is rewritten by wave 1 once the next heads are in
-        const float piv = pr[0];
-        if (pw == g) {
-          // this workgroup's candidate won: its remaining sub-block values (staged in crow) are final
-          const int rw = blane + (bsel ? 64 : 0);
-          const int qmax = min(32 - jj, kb - j);          // columns j .. j+qmax-1 exist in this sub-block
-          if (lane < qmax) P[(j + lane) * P2_PS + rw] = crow[lane];
-          if (lane == blane) { if (bsel) { pos1 = j; ps1 = jj; } else { pos0 = j; ps0 = jj; } }
-          if (lane == 0) piv_local[jj] = rw;
-        }
-        const bool win0 = (pw == g) && lane == blane && !bsel, win1 = (pw == g) && lane == blane && bsel;
-        if (!win0 && v0 && pos0 == j) pos0 = p;
-        if (!win1 && v1 && pos1 == j) pos1 = p;
-        // rows still in play (position > j) take the multiplier; pivoted rows are finished and their
-        // registers are no longer stored.  A zero pivot leaves every row unchanged.
-        const bool a0 = piv != 0.0f && v0 && pos0 > j, a1 = piv != 0.0f && v1 && pos1 > j;
-        const bool normal = fabsf(piv) >= 1.17549435e-38f;
-        const float l0 = normal ? x0[0] * rinv : x0[0] / piv;
-        const float l1 = normal ? x1[0] * rinv : x1[0] / piv;
-        if (jj + 1 < w) {
-          publish_head(j + 1, a0 ? fmaf(-l0, pr[1], x0[1]) : x0[1], a1 ? fmaf(-l1, pr[1], x1[1]) : x1[1]);
-          if (blane >= 0) {
-            // the new candidate's row after this step's update, one value per lane from its current values
-            // (same operands as the full update below), published right behind the head
-            if (lane == blane) {
-              float4* c4 = (float4*)crow;
-              if (bsel) {
-#pragma unroll
-                for (int q = 0; q < 8; q++) c4[q] = make_float4(x1[4 * q], x1[4 * q + 1], x1[4 * q + 2], x1[4 * q + 3]);
-              } else {
-#pragma unroll
-                for (int q = 0; q < 8; q++) c4[q] = make_float4(x0[4 * q], x0[4 * q + 1], x0[4 * q + 2], x0[4 * q + 3]);
-              }
-            }
-            const float lc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bsel ? l1 : l0), blane));
-            lds_wait();
-            const float xo = crow[min(ql + 1, 31)];
-            float nv = (piv != 0.0f) ? fmaf(-lc, prq, xo) : xo;
-            if (ql == 31) nv = 0.0f;
-            lds_wait();                                   // every lane's read is done before crow is overwritten
-            if (lane < 32) crow[ql] = nv;                 // the candidate's row as the pivot row of step j + 1
-            const uint32_t tag1 = (uint32_t)(a.k + j + 2);
-            const int par1 = (j + 1) & 1, rep = lane >> 5;
-            put_word(rowsl + (((size_t)(par1 * P2_NREP + rep) * P2_MAXG + g) * 32 + ql), tag1, __float_as_uint(nv));
-            put_word(rowsl + (((size_t)(par1 * P2_NREP + rep + 2) * P2_MAXG + g) * 32 + ql), tag1, __float_as_uint(nv));
-          }
-        }
-        PANEL2_MARK(j, 1);
-        if (piv != 0.0f) {
-          if (a0) P[j * P2_PS + lane] = l0;
-          if (a1) P[j * P2_PS + lane + 64] = l1;
-          const float m0 = a0 ? l0 : 0.0f, m1 = a1 ? l1 : 0.0f;
-#pragma unroll
-          for (int q = 1; q < 32; q++) { x0[q - 1] = fmaf(-m0, pr[q], x0[q]); x1[q - 1] = fmaf(-m1, pr[q], x1[q]); }
-        } else {
-          if (v0 && pos0 > j) P[j * P2_PS + lane] = x0[0];
-          if (v1 && pos1 > j) P[j * P2_PS + lane + 64] = x1[0];
-#pragma unroll
-          for (int q = 1; q < 32; q++) { x0[q - 1] = x0[q]; x1[q - 1] = x1[q]; }
-        }
-        x0[31] = 0.0f; x1[31] = 0.0f;
-        lds_wait();
-        PANEL2_MARK(j, 3);
-      }
-    } else {
-      // wave 1: the exchange side of every step
-      if (lane == 0) *s_zmask = 0;
-      if (lane < 32) piv_local[lane] = -1;
-      for (int jj = 0; jj < w; jj++) {
-        const int j = c0 + jj;
         const uint32_t tag = (uint32_t)(a.k + j + 1);
         const int par = j & 1;
+        const int ql = lane & 31;
         PANEL2_MARK(j, 4);
-        // (2) own mailbox -> winner -> the winner's 32 values
+        // own mailbox: the G heads of column j
         const uint64_t* mb = mbox + ((size_t)par * P2_MAXG + g) * P2_MBOX;
         uint64_t hw[4] = {0ull, 0ull, 0ull, 0ull};
         uint32_t need = 0;
@@ -729,21 +663,91 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
           }
         }
         PANEL2_MARK(j, 5);
-        float bv; int p;
-        const int wl = wave_argmax_pos(gv, gpos, bv, p);
-        int wpp = (wl >= 0) ? __builtin_amdgcn_readlane(gpp, wl) : -1;
-        if (wl < 0 || p < j || p >= m) { *s_abort = 1; p = j; wpp = -1; }
+        float bvw; int p;
+        const int wl = wave_argmax_pos(gv, gpos, bvw, p);
+        int pw = (wl >= 0) ? __builtin_amdgcn_readlane(gpp, wl) : -1;
+        if (wl < 0 || p < j || p >= m) { *s_abort = 1; p = j; pw = -1; }
+        // the winner's 32 values (lane q < 32: column j + q), published before its head
         float pv = 0.0f;
-        if (lane < 32 && wpp >= 0)
-          pv = __uint_as_float(get_word_b(rowsl + (((size_t)(par * P2_NREP + myrep) * P2_MAXG + wpp) * 32 + lane), tag, s_abort));
+        if (lane < 32 && pw >= 0)
+          pv = __uint_as_float(get_word_b(rowsl + (((size_t)(par * P2_NREP + myrep) * P2_MAXG + pw) * 32 + lane), tag, s_abort));
         if (lane < 32) prow[lane] = pv;
+        const float piv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pv), 0));
+        const float pr1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pv), 1));
+        const float rinv = 1.0f / piv;
         if (lane == 0) {
-          *s_p = p; *s_pw = wpp; s_piv[j] = p; *s_rinv = 1.0f / pv;
-          if (pv == 0.0f) { *s_zmask |= 1 << jj; if (*s_info == 0) *s_info = (int)(a.k + j + 1); }
+          s_piv[j] = p;
+          if (piv == 0.0f) { *s_zmask |= 1 << jj; if (*s_info == 0) *s_info = (int)(a.k + j + 1); }
         }
+        PANEL2_MARK(j, 2);
+        if (pw == g) {
+          // this workgroup's candidate won: its remaining sub-block values (staged in crow) are final
+          const int rw = blane + (bsel ? 64 : 0);
+          const int qmax = min(32 - jj, kb - j);          // columns j .. j+qmax-1 exist in this sub-block
+          if (lane < qmax) P[(j + lane) * P2_PS + rw] = crow[lane];
+          if (lane == blane) { if (bsel) { pos1 = j; ps1 = jj; } else { pos0 = j; ps0 = jj; } }
+          if (lane == 0) piv_local[jj] = rw;
+        }
+        const bool win0 = (pw == g) && lane == blane && !bsel, win1 = (pw == g) && lane == blane && bsel;
+        if (!win0 && v0 && pos0 == j) pos0 = p;
+        if (!win1 && v1 && pos1 == j) pos1 = p;
+        // rows still in play (position > j) take the multiplier; pivoted rows are finished and their
+        // registers are no longer stored.  A zero pivot leaves every row unchanged.
+        const bool a0 = piv != 0.0f && v0 && pos0 > j, a1 = piv != 0.0f && v1 && pos1 > j;
+        const bool normal = fabsf(piv) >= 1.17549435e-38f;
+        const float l0 = normal ? x0[0] * rinv : x0[0] / piv;
+        const float l1 = normal ? x1[0] * rinv : x1[0] / piv;
+        if (jj + 1 < w) {
+          publish_head(j + 1, a0 ? fmaf(-l0, pr1, x0[1]) : x0[1], a1 ? fmaf(-l1, pr1, x1[1]) : x1[1]);
+          if (blane >= 0) {
+            // the new candidate's row after this step's update, one value per lane from its current values
+            // (same operands as the full update below), published right behind the head
+            if (lane == blane) {
+              float4* c4 = (float4*)crow;
+              if (bsel) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) c4[q] = make_float4(x1[4 * q], x1[4 * q + 1], x1[4 * q + 2], x1[4 * q + 3]);
+              } else {
+#pragma unroll
+                for (int q = 0; q < 8; q++) c4[q] = make_float4(x0[4 * q], x0[4 * q + 1], x0[4 * q + 2], x0[4 * q + 3]);
+              }
+            }
+            const float lc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bsel ? l1 : l0), blane));
+            lds_wait();
+            const float xo = crow[min(ql + 1, 31)];
+            const float prq = prow[min(ql + 1, 31)];      // pivot-row value this lane contributes to the next row
+            float nv = (piv != 0.0f) ? fmaf(-lc, prq, xo) : xo;
+            if (ql == 31) nv = 0.0f;
+            lds_wait();                                   // every lane's read is done before crow is overwritten
+            if (lane < 32) crow[ql] = nv;                 // the candidate's row as the pivot row of step j + 1
+            const uint32_t tag1 = (uint32_t)(a.k + j + 2);
+            const int par1 = (j + 1) & 1, rep = lane >> 5;
+            put_word(rowsl + (((size_t)(par1 * P2_NREP + rep) * P2_MAXG + g) * 32 + ql), tag1, __float_as_uint(nv));
+            put_word(rowsl + (((size_t)(par1 * P2_NREP + rep + 2) * P2_MAXG + g) * 32 + ql), tag1, __float_as_uint(nv));
+          }
+        }
+        PANEL2_MARK(j, 1);
+        float pr[32];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const float4 v4 = ((const float4*)prow)[q];
+          pr[4 * q] = v4.x; pr[4 * q + 1] = v4.y; pr[4 * q + 2] = v4.z; pr[4 * q + 3] = v4.w;
+        }
+        if (piv != 0.0f) {
+          if (a0) P[j * P2_PS + lane] = l0;
+          if (a1) P[j * P2_PS + lane + 64] = l1;
+          const float m0 = a0 ? l0 : 0.0f, m1 = a1 ? l1 : 0.0f;
+#pragma unroll
+          for (int q = 1; q < 32; q++) { x0[q - 1] = fmaf(-m0, pr[q], x0[q]); x1[q - 1] = fmaf(-m1, pr[q], x1[q]); }
+        } else {
+          if (v0 && pos0 > j) P[j * P2_PS + lane] = x0[0];
+          if (v1 && pos1 > j) P[j * P2_PS + lane + 64] = x1[0];
+#pragma unroll
+          for (int q = 1; q < 32; q++) { x0[q - 1] = x0[q]; x1[q - 1] = x1[q]; }
+        }
+        x0[31] = 0.0f; x1[31] = 0.0f;
         lds_wait();
-        if (lane == 0) *s_step = j;                      // releases wave 0 (LDS operations of a wave complete in order)
-        PANEL2_MARK(j, 6);
+        PANEL2_MARK(j, 3);
       }
     }
     if (wave == 0) { pstep[lane] = ps0; pstep[lane + 64] = ps1; }

@@ -42,6 +42,9 @@ typedef float f32x8_t __attribute__((ext_vector_type(8)));
 #ifndef PANEL2_COUNT
 #define PANEL2_COUNT(col, value)
 #endif
+#ifndef NP_MARK
+#define NP_MARK(phase)
+#endif
 constexpr long SPIN_MAX = 1L << 22;
 
 // ------------------------------------------------------------------ tagged-word exchange
@@ -667,6 +670,7 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
         const int wl = wave_argmax_pos(gv, gpos, bvw, p);
         int pw = (wl >= 0) ? __builtin_amdgcn_readlane(gpp, wl) : -1;
         if (wl < 0 || p < j || p >= m) { *s_abort = 1; p = j; pw = -1; }
+        PANEL2_MARK(j, 0);
         // the winner's 32 values (lane q < 32: column j + q), published before its head
         float pv = 0.0f;
         if (lane < 32 && pw >= 0)
@@ -699,6 +703,7 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
         const float l1 = normal ? x1[0] * rinv : x1[0] / piv;
         if (jj + 1 < w) {
           publish_head(j + 1, a0 ? fmaf(-l0, pr1, x0[1]) : x0[1], a1 ? fmaf(-l1, pr1, x1[1]) : x1[1]);
+          PANEL2_MARK(j, 6);
           if (blane >= 0) {
             // the new candidate's row after this step's update, one value per lane from its current values
             // (same operands as the full update below), published right behind the head
@@ -1315,7 +1320,9 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ S, int64
 // ------------------------------------------------------------------ the next panel's columns in one launch
 // k_next_prep: for 32 columns per workgroup of the next panel's range [c0, c0 + ncols), everything the
 // look-ahead does before their trailing update, in one launch with the strip held in registers:
-//   1. the row interchanges of panel k (the swap plan: new[dst] = old[src], panel-local rows): the top kb rows
+//   1. when given the swap plan (new[dst] = old[src], panel-local rows), the row interchanges of panel k (the
+//      caller may instead have applied them with a wide k_laswp launch, each column's scattered rows being one
+//      cache line per element, which 8 workgroups cannot fetch fast enough): the top kb rows
 //      are loaded through the map the plan defines, the moved rows below are gathered, and written back once
 //      every load of the workgroup has completed;
 //   2. U12 <- L11^{-1} U12 with the arithmetic of k_trsm_mfma: wave w owns 32-row block w of the strip in
@@ -1346,8 +1353,9 @@ __global__ __launch_bounds__(NP_THREADS) void k_next_prep(float* __restrict__ A,
   const int t = threadIdx.x, w = t >> 6, l = t & 63, h = l >> 5, li = l & 31;
   const int nrb = kb / 32;
   const int64_t cs = c0 + (int64_t)blockIdx.x * 32;  // first column of this strip
+  NP_MARK(0);
   if (blockIdx.x == 0 && t == 0 && zero_plan != nullptr) *zero_plan = 0;
-  const int npairs = plan[0];
+  const int npairs = plan != nullptr ? plan[0] : 0;   // no plan: the rows are already interchanged
   // row map of the top kb rows: identity, then the plan's destinations above kb
   for (int r = t; r < kb; r += NP_THREADS) rmap[r] = r;
   __syncthreads();
@@ -1356,6 +1364,7 @@ __global__ __launch_bounds__(NP_THREADS) void k_next_prep(float* __restrict__ A,
     if (d < kb) rmap[d] = plan[2 + 2 * e];
   }
   __syncthreads();
+  NP_MARK(1);
   float* Ac = A + k;                                 // panel-local row r of column c: Ac[r + c * lda]
   // (1) every load first: the moved rows below kb (two rounds of pairs per thread) and the strip rows
   int dlo[2] = {-1, -1};
@@ -1404,6 +1413,7 @@ __global__ __launch_bounds__(NP_THREADS) void k_next_prep(float* __restrict__ A,
       }
     }
   }
+  NP_MARK(7);
   // diagonal-block inverses: wave w inverts T_w (lane c < 32: column c by forward substitution, the order of
   // trsm_inv_block), staged in its own diagonal slot
   if (w < nrb) {
@@ -1416,16 +1426,24 @@ __global__ __launch_bounds__(NP_THREADS) void k_next_prep(float* __restrict__ A,
       for (int i = 0; i < 16; i++) { const int e = l + 64 * i, q = e >> 5, r = e & 31; T[r * 33 + q] = tv[i]; }
     }
     lds_wait();
+    // right-looking: at step q, x[q] has all its terms and every later row takes -T[r][q] x[q].  Per row the
+    // terms arrive in increasing q from the same starting value, so each x[r] is the fmaf chain of the
+    // row-by-row substitution above, while a step's 31 - q updates are independent (31 dependent steps
+    // instead of 496).
     float x[32];
     if (l < 32) {
 #pragma unroll
       for (int r = 0; r < 32; r++) x[r] = (r == l) ? 1.0f : 0.0f;
 #pragma unroll
-      for (int r = 1; r < 32; r++) {
-        float acc = x[r];
+      for (int q = 0; q < 31; q++) {
+        const float xq = x[q];
+        const bool on = q >= l;
 #pragma unroll
-        for (int q = 0; q < r; q++) if (q >= l) acc = fmaf(-T[r * 33 + q], x[q], acc);
-        if (r > l) x[r] = acc;
+        for (int r = q + 1; r < 32; r++) {
+          const float nv = fmaf(-T[r * 33 + q], xq, x[r]);
+          x[r] = on ? nv : x[r];
+        }
+        __builtin_amdgcn_sched_barrier(0);             // a step's reads stay with it (no hoisting of all 496)
       }
     }
     lds_wait();                                        // every lane's reads of T precede the overwrite
@@ -1439,6 +1457,7 @@ __global__ __launch_bounds__(NP_THREADS) void k_next_prep(float* __restrict__ A,
     }
   }
   __syncthreads();                                     // loads of this workgroup complete; LDS operands ready
+  NP_MARK(2);
   // the moved rows below kb
 #pragma unroll
   for (int q = 0; q < 2; q++) {
@@ -1447,6 +1466,7 @@ __global__ __launch_bounds__(NP_THREADS) void k_next_prep(float* __restrict__ A,
       for (int c = 0; c < 32; c++) Ac[dlo[q] + (cs + c) * lda] = vlo[q][c];
     }
   }
+  NP_MARK(3);
   // (2) the triangular solve, block by block
   auto kidx = [&](int s2) { return 8 * (s2 >> 2) + 4 * h + (s2 & 3); };
   for (int b = 0; b < nrb; b++) {
@@ -1472,6 +1492,7 @@ __global__ __launch_bounds__(NP_THREADS) void k_next_prep(float* __restrict__ A,
     }
     __syncthreads();                                   // Xex is rewritten by the next block's owner
   }
+  NP_MARK(4);
   // U12 back into A, and the strip staged row-major in LDS (the operand blocks are no longer needed)
   float* St = sm;                                      // [256 rows][33]
   if (w < nrb) {
@@ -1508,6 +1529,7 @@ __global__ __launch_bounds__(NP_THREADS) void k_next_prep(float* __restrict__ A,
     expsU[blockIdx.x] = sc;
   }
   __syncthreads();
+  NP_MARK(5);
   const float scl = ldexpf(1.0f, s_misc[0]);
   const int nkb = kb / 16;
   unsigned long long n_over = 0, n_sub = 0;
@@ -1548,6 +1570,7 @@ __global__ __launch_bounds__(NP_THREADS) void k_next_prep(float* __restrict__ A,
       d[1] = make_float4(v[4], v[5], v[6], v[7]);
     }
   }
+  NP_MARK(6);
   if (n_over) atomicAdd(cnt, n_over);
   if (n_sub) atomicAdd(cnt + 1, n_sub);
 }
@@ -1786,7 +1809,7 @@ static void update_cols(float* A, int64_t lda, int64_t k, int kb, int nb, int64_
 }
 
 // The next panel's columns [cb, cb + ncols) brought up to date except for their trailing update (k_next_prep):
-// interchanges, TRSM and the packed U operand of step k, in one launch.
+// TRSM and the packed U operand of step k in one launch (and the interchanges too when `plan` is given).
 template <int FMT>
 static void next_prep(float* A, int64_t lda, int64_t k, int kb, int nb, int64_t m2, int64_t cb, int ncols, const int32_t* plan,
                       int32_t* zero_plan, bool tile_scale, Workspace& W) {
@@ -2013,7 +2036,10 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
     // the next panel's interchanges, TRSM and U pack in one launch (k_next_prep) when the widths allow
     const bool fused = split && fused_next && (kb % 32) == 0 && (kbn % 32) == 0;
     if (fused) {
-      next_prep_fmt(a.fmt, a.A, a.lda, k, kb, nb, m2, k + kb, kbn, plan, next_plan, a.tile_scale, W);
+      // the interchanges touch one cache line per element: they go wide (a workgroup per column) ahead of the
+      // fused TRSM + pack, which then reads contiguous rows
+      laswp_ranges(a, W, k, kb, k + kb, cN, 0, 0, false, plan, false, next_plan, "laswp_next");
+      next_prep_fmt(a.fmt, a.A, a.lda, k, kb, nb, m2, k + kb, kbn, nullptr, nullptr, a.tile_scale, W);
     } else if (split) {
       laswp_ranges(a, W, k, kb, k + kb, cN, 0, 0, false, plan, !use_fma_trsm, next_plan, "laswp_next");
     } else {

@@ -42,6 +42,12 @@ typedef float f32x8_t __attribute__((ext_vector_type(8)));
 #ifndef PANEL2_COUNT
 #define PANEL2_COUNT(col, value)
 #endif
+#ifndef PANEL2_WGMARK
+#define PANEL2_WGMARK(slot)
+#endif
+#ifndef PANEL2_MARKB
+#define PANEL2_MARKB(col, phase)
+#endif
 #ifndef NP_MARK
 #define NP_MARK(phase)
 #endif
@@ -93,16 +99,6 @@ __device__ __forceinline__ uint64_t get_head(const uint64_t* p, uint32_t tag24, 
   }
   *s_abort = 1;
   return 0ull;
-}
-__device__ __forceinline__ uint32_t get_word_b(const uint64_t* p, uint32_t tag, volatile lds_i32* s_abort) {
-  for (long it = 0; it < SPIN_MAX; it++) {
-    const uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((uint32_t)(w >> 32) == tag) return (uint32_t)w;
-    if (*s_abort) return 0u;
-    __builtin_amdgcn_s_sleep(1);
-  }
-  *s_abort = 1;
-  return 0u;
 }
 
 // Wave-wide maximum with DPP (row_shr 1/2/4/8 inside rows of 16 lanes, then row_bcast 15/31 across rows):
@@ -338,13 +334,14 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs a) {
 //   * Rows never move: each carries its current position `pos` (LAPACK's sequential-interchange order).
 //     At step j the winner's position becomes j and the row that held position j takes the winner's old
 //     position, so "ties to the smallest index" compares positions.
-//   * Exchange per step: wave 0 finds its workgroup's candidate (DPP argmax), writes the candidate's 32
-//     values to its row slot (four replicas) and pushes one head word (tag, position, |value|) into every
-//     workgroup's mailbox.  Wave 1 reads its own mailbox, picks the winner, reads the winner's row from
-//     replica g % 4 into LDS and raises an LDS step flag that releases wave 0's rank-1 update.  Mailboxes
-//     sit 4 KB apart and replicas 64 KB apart, so the polling of 100+ workgroups spreads over memory
-//     channels instead of queueing on a few cache lines.  Wave 0 never loads from global memory inside
-//     the step loop (gfx9 counts loads and stores with one counter).
+//   * Exchange per step, all in wave 0: it finds its workgroup's candidate (DPP argmax) and pushes a
+//     16-byte head (tag, position, the candidate's value and its next-column value) into every workgroup's
+//     mailbox, then writes the candidate's 32 values to its row slot (four replicas).  After reading its own
+//     mailbox every workgroup picks the same winner; the winner's value and next-column value are enough to
+//     form the multipliers and column j+1 and to find the next candidate, while the winner's row (needed
+//     for the rest of the rank-1 update and for the next head's next-column value) is being fetched.
+//     Mailboxes sit 4 KB apart and replicas 64 KB apart, so the polling of 100+ workgroups spreads over
+//     memory channels instead of queueing on a few cache lines.
 //   * After 32 steps the panel columns right of the sub-block get the deferred update
 //     a -= sum_d l_d u_d (d in step order, so each element sees the same fmaf chain as 32 rank-1 updates):
 //     the owners publish their pivot rows (four replicas, double-buffered by sub-block), every workgroup
@@ -399,27 +396,77 @@ __device__ __forceinline__ int wave_min_dpp(int x) {
   x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x143, 0xc, 0xf, false));
   return __builtin_amdgcn_readlane(x, 63);
 }
-// Wave argmax of (v, pos): largest v, ties to the smallest pos; v < 0 marks "no candidate".  Returns the
-// winning lane (wave-uniform) or -1, and the winner's v / pos in every lane.
-__device__ __forceinline__ int wave_argmax_pos(float v, int pos, float& bv, int& bpos) {
-  const float mx = wave_max_dpp(v);
-  if (mx < 0.0f) { bv = -1.0f; bpos = 0x7FFFFFFF; return -1; }
-  const unsigned long long tie = __ballot(v == mx);
+// Pivot-search key of a value: its |value| bits, NaN counted as infinity.  On non-negative floats the order
+// of the bit patterns as integers is the order of the values; -1 marks "no candidate".
+__device__ __forceinline__ int p2_key(float v) {
+  const uint32_t b = __float_as_uint(v) & 0x7FFFFFFFu;
+  return (int)(b > 0x7F800000u ? 0x7F800000u : b);
+}
+// Wave-wide maximum of a signed integer: one DPP-modified v_max per stage (row_shr 1/2/4/8 inside rows of 16
+// lanes, then row_bcast 15/31 across rows; a lane whose DPP source is invalid keeps its value), two wait
+// states before each stage reads the previous stage's result.  All 64 lanes must be active.
+__device__ __forceinline__ int wave_max_i32(int x) {
+  int r;
+  asm volatile(
+      "v_mov_b32 %0, %1\n\t"
+      "s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_shr:2 row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_shr:4 row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_shr:8 row_mask:0xf bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_max_i32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+      "s_nop 1"
+      : "=&v"(r) : "v"(x));
+  return __builtin_amdgcn_readlane(r, 63);
+}
+// Wave argmax of (key, pos): largest key, ties to the smallest pos; key < 0 marks "no candidate".  Returns the
+// winning lane (wave-uniform) or -1, and the winner's pos.  All 64 lanes must be active.
+__device__ __forceinline__ int wave_argmax_key(int key, int pos, int& bpos) {
+  const int mx = wave_max_i32(key);
+  if (mx < 0) { bpos = 0x7FFFFFFF; return -1; }
+  const unsigned long long tie = __ballot(key == mx);
   int lane;
   if (__popcll(tie) == 1) {
     lane = __ffsll((long long)tie) - 1;
   } else {
-    const int mn = wave_min_dpp(v == mx ? pos : 0x7FFFFFFF);
-    lane = __ffsll((long long)__ballot(v == mx && pos == mn)) - 1;
+    const int mn = wave_min_dpp(key == mx ? pos : 0x7FFFFFFF);
+    lane = __ffsll((long long)__ballot(key == mx && pos == mn)) - 1;
   }
   lane = __builtin_amdgcn_readfirstlane(lane);
-  bv = mx;
   bpos = __builtin_amdgcn_readlane(pos, lane);
   return lane;
 }
-// head word: [63:47] 17-bit tag, [46:31] position, [30:0] |value| bits (0x7FFFFFFF: no candidate)
-__device__ __forceinline__ uint64_t head_word(uint32_t tag, int pos, uint32_t vbits) {
-  return ((uint64_t)(tag & 0x1FFFFu) << 47) | ((uint64_t)(pos & 0xFFFF) << 31) | (uint64_t)(vbits & 0x7FFFFFFFu);
+// 16-byte head slots of the panel exchange, written and read with one vector access at agent scope (sc1):
+// both halves carry the tag, so a slot read while it is being written is simply read again.  The loads wait
+// for their data inside the asm block (their destination registers are written by the memory system later).
+typedef unsigned long long p2u64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void p2_head_load2(const uint64_t* pa, const uint64_t* pb, p2u64x2& ra, p2u64x2& rb) {
+  asm volatile("global_load_dwordx4 %0, %2, off sc1\n\t"
+               "global_load_dwordx4 %1, %3, off sc1\n\t"
+               "s_waitcnt vmcnt(0)"
+               : "=&v"(ra), "=&v"(rb) : "v"(pa), "v"(pb) : "memory");
+}
+__device__ __forceinline__ void p2_head_store(uint64_t* p, uint64_t w0, uint64_t w1) {
+  const p2u64x2 v = {w0, w1};
+  asm volatile("global_store_dwordx4 %0, %1, off sc1\n\ts_nop 0" :: "v"(p), "v"(v) : "memory");
+}
+// A tagged word whose first load (`first`) was issued earlier: polled until it carries the tag.
+__device__ __forceinline__ uint32_t p2_wait_word(const uint64_t* p, uint64_t first, uint32_t tag, volatile lds_i32* s_abort) {
+  if ((uint32_t)(first >> 32) == tag) return (uint32_t)first;
+  for (long it = 0; it < SPIN_MAX; it++) {
+    if (*s_abort) return 0u;
+    __builtin_amdgcn_s_sleep(1);
+    const uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((uint32_t)(w >> 32) == tag) return (uint32_t)w;
+  }
+  *s_abort = 1;
+  return 0u;
 }
 
 // Poll four tagged U-row words (q, q + 1, q + 2, q + 3 of one column: stride 256 words) into u; a word that
@@ -523,16 +570,31 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
   uint64_t* mbox = a.xbuf;
   uint64_t* rowsl = a.xbuf + P2_OFF_ROWS;
   const int myrep = g % P2_NREP;
+  PANEL2_WGMARK(0);
   if (t == 0) { *s_abort = 0; *s_info = 0; *s_zmask = 0; }
-  for (int idx = t; idx < kb * P2_R; idx += 256) {
-    const int c = idx >> 7, r = idx & (P2_R - 1);
-    P[c * P2_PS + r] = (r < rows) ? Ap[(int64_t)(r0 + r) + (int64_t)c * a.lda] : 0.0f;
+  {
+    // the slice into LDS: thread (row r, column parity ch) keeps 32 loads in flight before its first LDS write
+    // (one load per iteration would wait a full memory round trip 128 times).  Rows past the panel read the
+    // last row and store zero; columns past kb read column kb - 1 and are not stored.
+    const int r = t & (P2_R - 1), ch = t >> 7;
+    const float* src = Ap + (int64_t)(r0 + min(r, rows - 1));
+    for (int cb = 0; cb < kb; cb += 64) {
+      float v[32];
+#pragma unroll
+      for (int i = 0; i < 32; i++) v[i] = src[(int64_t)min(cb + 2 * i + ch, kb - 1) * a.lda];
+#pragma unroll
+      for (int i = 0; i < 32; i++) {
+        const int c = cb + 2 * i + ch;
+        if (c < kb) P[c * P2_PS + r] = (r < rows) ? v[i] : 0.0f;
+      }
+    }
   }
   // rows of wave 0: set 0 = row lane, set 1 = row lane + 64
   const bool v0 = (wave == 0) && lane < rows, v1 = (wave == 0) && lane + 64 < rows;
   int pos0 = r0 + lane, pos1 = r0 + lane + 64;
   int bg_c0 = -1, bg_zmask = 0;                           // background update pending for waves 2-3
   lds_barrier();
+  PANEL2_WGMARK(1);
 
   for (int c0 = 0; c0 < kb; c0 += 32) {
     const int w = min(32, kb - c0);
@@ -589,19 +651,31 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
       // row (which readers fetch only after the heads round trip) follow while the exchange is in flight.
       int blane = -1;                                     // candidate lane of the current column (-1: none)
       bool bsel = false;                                  // candidate is the lane's second row
-      auto publish_head = [&](int j, float cur0, float cur1) {
-        const uint32_t tag = (uint32_t)(a.k + j + 1);
-        float cv0 = -1.0f, cv1 = -1.0f;
-        if (v0 && pos0 >= j) { float xa = fabsf(cur0); if (xa != xa) xa = INFINITY; cv0 = xa; }
-        if (v1 && pos1 >= j) { float xa = fabsf(cur1); if (xa != xa) xa = INFINITY; cv1 = xa; }
-        const bool take1 = (cv1 > cv0) || (cv1 == cv0 && cv1 >= 0.0f && pos1 < pos0);
-        float bv; int bpos;
-        blane = wave_argmax_pos(take1 ? cv1 : cv0, take1 ? pos1 : pos0, bv, bpos);
+      float cur_c = 0.0f;                                 // the candidate's value in the column it competes for
+      int pos_c = 0xFFFF;                                 // its position (0xFFFF: no candidate)
+      // The candidate among the rows in play (cur*: their values in the column), by a wave argmax of |value|
+      // (NaN counted as infinity; ties to the smaller position, as isamax).
+      auto find_candidate = [&](float cur0, float cur1, bool in0, bool in1) {
+        const int k0 = in0 ? p2_key(cur0) : -1, k1 = in1 ? p2_key(cur1) : -1;
+        const bool take1 = (k1 > k0) || (k1 == k0 && k1 >= 0 && pos1 < pos0);
+        int bpos;
+        PANEL2_MARKB(c0 + 31, 2);
+        blane = wave_argmax_key(take1 ? k1 : k0, take1 ? pos1 : pos0, bpos);
+        PANEL2_MARKB(c0 + 31, 3);
         bsel = (blane >= 0) && (__builtin_amdgcn_readlane((int)take1, blane) != 0);
-        const uint64_t hwv = head_word(tag, blane >= 0 ? bpos : 0, blane >= 0 ? __float_as_uint(bv) : 0x7FFFFFFFu);
-        uint64_t* mrow = mbox + (size_t)(j & 1) * P2_MAXG * P2_MBOX + g;
-        for (int rd = lane; rd < G; rd += 64)
-          __hip_atomic_store(mrow + (size_t)rd * P2_MBOX, hwv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        pos_c = blane >= 0 ? bpos : 0xFFFF;
+        cur_c = blane >= 0 ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(take1 ? cur1 : cur0), blane)) : 0.0f;
+      };
+      // The head of column j: [tag | position | value] and [tag | next-column value], one 16-byte slot in every
+      // reader's mailbox.  The next-column value is the candidate's column j+1 value with every update through
+      // step j-1, so the step that picks the winner can update its rows' column j+1 before the winner's row
+      // arrives.  Both halves carry the tag: a reader accepts a slot only when both do.
+      auto head_push = [&](int j, float nxt_c) {
+        const uint64_t tg = (uint64_t)((uint32_t)(a.k + j + 1) & 0xFFFFu) << 48;
+        const uint64_t w0 = tg | ((uint64_t)(uint32_t)(pos_c & 0xFFFF) << 32) | (uint64_t)__float_as_uint(cur_c);
+        const uint64_t w1 = tg | (uint64_t)__float_as_uint(nxt_c);
+        uint64_t* mrow = mbox + (size_t)(j & 1) * P2_MAXG * P2_MBOX + 2 * g;
+        for (int rd = lane; rd < G; rd += 64) p2_head_store(mrow + (size_t)rd * P2_MBOX, w0, w1);
       };
       // the candidate's 32 values (x[0] = column j), staged in LDS, four replicas
       auto publish_row = [&](int j) {
@@ -624,29 +698,36 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
         put_word(rowsl + (((size_t)(par * P2_NREP + rep) * P2_MAXG + g) * 32 + (lane & 31)), tag, mine);
         put_word(rowsl + (((size_t)(par * P2_NREP + rep + 2) * P2_MAXG + g) * 32 + (lane & 31)), tag, mine);
       };
-      publish_head(c0, x0[0], x1[0]);
+      find_candidate(x0[0], x1[0], v0 && pos0 >= c0, v1 && pos1 >= c0);
+      head_push(c0, blane >= 0 ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bsel ? x1[1] : x0[1]), blane)) : 0.0f);
       publish_row(c0);
       for (int jj = 0; jj < w; jj++) {
         const int j = c0 + jj;
         const uint32_t tag = (uint32_t)(a.k + j + 1);
+        const uint32_t tag16 = tag & 0xFFFFu;
         const int par = j & 1;
         const int ql = lane & 31;
         PANEL2_MARK(j, 4);
-        // own mailbox: the G heads of column j
+        // own mailbox: the G heads of column j (slot s at words 2s, 2s + 1)
         const uint64_t* mb = mbox + ((size_t)par * P2_MAXG + g) * P2_MBOX;
-        uint64_t hw[4] = {0ull, 0ull, 0ull, 0ull};
+        uint64_t hw0[4] = {0ull, 0ull, 0ull, 0ull}, hw1[4] = {0ull, 0ull, 0ull, 0ull};
         uint32_t need = 0;
 #pragma unroll
         for (int i = 0; i < 4; i++) if (lane + 64 * i < G) need |= 1u << i;
+        const int npair = G > 128 ? 2 : 1;
         long it_heads = 0;
         for (long it = 0; it < SPIN_MAX; it++, it_heads++) {
-          uint64_t wv[4];
-#pragma unroll
-          for (int i = 0; i < 4; i++)
-            wv[i] = (need >> i & 1u) ? __hip_atomic_load(mb + lane + 64 * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
-#pragma unroll
-          for (int i = 0; i < 4; i++)
-            if ((need >> i & 1u) && (uint32_t)(wv[i] >> 47) == (tag & 0x1FFFFu)) { hw[i] = wv[i]; need &= ~(1u << i); }
+          for (int h = 0; h < npair; h++) {
+            p2u64x2 sa, sb;
+            p2_head_load2(mb + 2 * (lane + 128 * h), mb + 2 * (lane + 128 * h + 64), sa, sb);
+            const int ia = 2 * h, ib = 2 * h + 1;
+            if ((need >> ia & 1u) && (uint32_t)(sa.x >> 48) == tag16 && (uint32_t)(sa.y >> 48) == tag16) {
+              hw0[ia] = sa.x; hw1[ia] = sa.y; need &= ~(1u << ia);
+            }
+            if ((need >> ib & 1u) && (uint32_t)(sb.x >> 48) == tag16 && (uint32_t)(sb.y >> 48) == tag16) {
+              hw0[ib] = sb.x; hw1[ib] = sb.y; need &= ~(1u << ib);
+            }
+          }
           if (__ballot(need != 0u) == 0ull) break;
           if (*s_abort) break;
           __builtin_amdgcn_s_sleep(1);
@@ -654,82 +735,106 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
         if (__ballot(need != 0u) != 0ull) *s_abort = 1;
         PANEL2_COUNT(j, it_heads);
         (void)it_heads;
-        float gv = -1.0f; int gpos = 0x7FFFFFFF, gpp = -1;
+        // the winner: the lane's best head, then the wave's
+        int kk = -1, kp = 0x7FFFFFFF, ks = -1;
+        uint32_t kv = 0u, kn = 0u;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
           const int pp = lane + 64 * i;
-          const uint32_t vb = (uint32_t)(hw[i] & 0x7FFFFFFFull);
-          if (pp < G && !(need >> i & 1u) && vb != 0x7FFFFFFFu) {
-            const float hvf = __uint_as_float(vb);
-            const int hpos = (int)((hw[i] >> 31) & 0xFFFFull);
-            if (hvf > gv || (hvf == gv && hpos < gpos)) { gv = hvf; gpos = hpos; gpp = pp; }
-          }
+          const int hpos = (int)((hw0[i] >> 32) & 0xFFFFull);
+          const uint32_t vb = (uint32_t)hw0[i];
+          const int key = (pp < G && !(need >> i & 1u) && hpos != 0xFFFF) ? p2_key(__uint_as_float(vb)) : -1;
+          const bool better = key > kk || (key == kk && key >= 0 && hpos < kp);
+          kk = better ? key : kk; kp = better ? hpos : kp; ks = better ? pp : ks;
+          kv = better ? vb : kv; kn = better ? (uint32_t)hw1[i] : kn;
         }
         PANEL2_MARK(j, 5);
-        float bvw; int p;
-        const int wl = wave_argmax_pos(gv, gpos, bvw, p);
-        int pw = (wl >= 0) ? __builtin_amdgcn_readlane(gpp, wl) : -1;
+        int p;
+        PANEL2_MARKB(j, 5);
+        const int wl = wave_argmax_key(kk, kp, p);
+        PANEL2_MARKB(j, 6);
+        int pw = (wl >= 0) ? __builtin_amdgcn_readlane(ks, wl) : -1;
         if (wl < 0 || p < j || p >= m) { *s_abort = 1; p = j; pw = -1; }
+        // the pivot and the pivot row's column j+1 value come with the head
+        const float piv = (wl >= 0) ? __uint_as_float(__builtin_amdgcn_readlane(kv, wl)) : 0.0f;
+        const float u1 = (wl >= 0) ? __uint_as_float(__builtin_amdgcn_readlane(kn, wl)) : 0.0f;
         PANEL2_MARK(j, 0);
-        // the winner's 32 values (lane q < 32: column j + q), published before its head
-        float pv = 0.0f;
-        if (lane < 32 && pw >= 0)
-          pv = __uint_as_float(get_word_b(rowsl + (((size_t)(par * P2_NREP + myrep) * P2_MAXG + pw) * 32 + lane), tag, s_abort));
-        if (lane < 32) prow[lane] = pv;
-        const float piv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pv), 0));
-        const float pr1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pv), 1));
+        // the winner's 32 values (lane q < 32: column j + q; lanes 32-63 read the same words): the load is in
+        // flight while the multipliers, column j+1 and the next candidate are formed
+        const uint64_t* rp = rowsl + (((size_t)(par * P2_NREP + myrep) * P2_MAXG + max(pw, 0)) * 32 + ql);
+        const uint64_t rw_first = __hip_atomic_load(rp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const float rinv = 1.0f / piv;
+        PANEL2_MARKB(j, 0);
+        // positions after the interchange: the winner takes position j, the row that held j takes the
+        // winner's old position p; rows in play for column j+1 are those past position j
+        const bool win0 = (pw == g) && lane == blane && !bsel, win1 = (pw == g) && lane == blane && bsel;
+        if (win0) { pos0 = j; ps0 = jj; }
+        if (win1) { pos1 = j; ps1 = jj; }
+        if (!win0 && v0 && pos0 == j) pos0 = p;
+        if (!win1 && v1 && pos1 == j) pos1 = p;
+        const bool in0 = v0 && pos0 > j, in1 = v1 && pos1 > j;
+        // rows in play take the multiplier; a zero pivot leaves every row unchanged
+        const bool a0 = piv != 0.0f && in0, a1 = piv != 0.0f && in1;
+        const bool normal = fabsf(piv) >= 1.17549435e-38f;
+        const float l0 = normal ? x0[0] * rinv : x0[0] / piv;
+        const float l1 = normal ? x1[0] * rinv : x1[0] / piv;
+        PANEL2_MARKB(j, 1);
+        const int pblane = blane;                          // this workgroup's candidate of column j
+        const bool pbsel = bsel;
+        const bool more = jj + 1 < w;
+        // column j+1 first (same operands as the full update below), then the candidate of column j+1
+        if (more) find_candidate(a0 ? fmaf(-l0, u1, x0[1]) : x0[1], a1 ? fmaf(-l1, u1, x1[1]) : x1[1], in0, in1);
+        float pv = 0.0f;
+        if (pw >= 0) pv = __uint_as_float(p2_wait_word(rp, rw_first, tag, s_abort));
+        PANEL2_MARK(j, 2);
+        if (lane < 32) prow[lane] = pv;
+        const float lc = (more && blane >= 0) ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bsel ? l1 : l0), blane)) : 0.0f;
+        if (more) {
+          // the candidate's column j+2 value after this step: the operands of its row published below
+          float nxt = 0.0f;
+          if (blane >= 0) {
+            const float x2c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bsel ? x1[2] : x0[2]), blane));
+            const float pr2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pv), 2));
+            nxt = (piv != 0.0f) ? fmaf(-lc, pr2, x2c) : x2c;
+          }
+          head_push(j + 1, nxt);
+        }
+        PANEL2_MARK(j, 6);
         if (lane == 0) {
           s_piv[j] = p;
           if (piv == 0.0f) { *s_zmask |= 1 << jj; if (*s_info == 0) *s_info = (int)(a.k + j + 1); }
         }
-        PANEL2_MARK(j, 2);
         if (pw == g) {
           // this workgroup's candidate won: its remaining sub-block values (staged in crow) are final
-          const int rw = blane + (bsel ? 64 : 0);
+          const int rw = pblane + (pbsel ? 64 : 0);
           const int qmax = min(32 - jj, kb - j);          // columns j .. j+qmax-1 exist in this sub-block
           if (lane < qmax) P[(j + lane) * P2_PS + rw] = crow[lane];
-          if (lane == blane) { if (bsel) { pos1 = j; ps1 = jj; } else { pos0 = j; ps0 = jj; } }
           if (lane == 0) piv_local[jj] = rw;
         }
-        const bool win0 = (pw == g) && lane == blane && !bsel, win1 = (pw == g) && lane == blane && bsel;
-        if (!win0 && v0 && pos0 == j) pos0 = p;
-        if (!win1 && v1 && pos1 == j) pos1 = p;
-        // rows still in play (position > j) take the multiplier; pivoted rows are finished and their
-        // registers are no longer stored.  A zero pivot leaves every row unchanged.
-        const bool a0 = piv != 0.0f && v0 && pos0 > j, a1 = piv != 0.0f && v1 && pos1 > j;
-        const bool normal = fabsf(piv) >= 1.17549435e-38f;
-        const float l0 = normal ? x0[0] * rinv : x0[0] / piv;
-        const float l1 = normal ? x1[0] * rinv : x1[0] / piv;
-        if (jj + 1 < w) {
-          publish_head(j + 1, a0 ? fmaf(-l0, pr1, x0[1]) : x0[1], a1 ? fmaf(-l1, pr1, x1[1]) : x1[1]);
-          PANEL2_MARK(j, 6);
-          if (blane >= 0) {
-            // the new candidate's row after this step's update, one value per lane from its current values
-            // (same operands as the full update below), published right behind the head
-            if (lane == blane) {
-              float4* c4 = (float4*)crow;
-              if (bsel) {
+        if (more && blane >= 0) {
+          // the new candidate's row after this step's update, one value per lane from its current values
+          // (same operands as the full update below), published right behind the head
+          if (lane == blane) {
+            float4* c4 = (float4*)crow;
+            if (bsel) {
 #pragma unroll
-                for (int q = 0; q < 8; q++) c4[q] = make_float4(x1[4 * q], x1[4 * q + 1], x1[4 * q + 2], x1[4 * q + 3]);
-              } else {
+              for (int q = 0; q < 8; q++) c4[q] = make_float4(x1[4 * q], x1[4 * q + 1], x1[4 * q + 2], x1[4 * q + 3]);
+            } else {
 #pragma unroll
-                for (int q = 0; q < 8; q++) c4[q] = make_float4(x0[4 * q], x0[4 * q + 1], x0[4 * q + 2], x0[4 * q + 3]);
-              }
+              for (int q = 0; q < 8; q++) c4[q] = make_float4(x0[4 * q], x0[4 * q + 1], x0[4 * q + 2], x0[4 * q + 3]);
             }
-            const float lc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bsel ? l1 : l0), blane));
-            lds_wait();
-            const float xo = crow[min(ql + 1, 31)];
-            const float prq = prow[min(ql + 1, 31)];      // pivot-row value this lane contributes to the next row
-            float nv = (piv != 0.0f) ? fmaf(-lc, prq, xo) : xo;
-            if (ql == 31) nv = 0.0f;
-            lds_wait();                                   // every lane's read is done before crow is overwritten
-            if (lane < 32) crow[ql] = nv;                 // the candidate's row as the pivot row of step j + 1
-            const uint32_t tag1 = (uint32_t)(a.k + j + 2);
-            const int par1 = (j + 1) & 1, rep = lane >> 5;
-            put_word(rowsl + (((size_t)(par1 * P2_NREP + rep) * P2_MAXG + g) * 32 + ql), tag1, __float_as_uint(nv));
-            put_word(rowsl + (((size_t)(par1 * P2_NREP + rep + 2) * P2_MAXG + g) * 32 + ql), tag1, __float_as_uint(nv));
           }
+          lds_wait();
+          const float xo = crow[min(ql + 1, 31)];
+          const float prq = prow[min(ql + 1, 31)];        // pivot-row value this lane contributes to the next row
+          float nv = (piv != 0.0f) ? fmaf(-lc, prq, xo) : xo;
+          if (ql == 31) nv = 0.0f;
+          lds_wait();                                     // every lane's read is done before crow is overwritten
+          if (lane < 32) crow[ql] = nv;                   // the candidate's row as the pivot row of step j + 1
+          const uint32_t tag1 = (uint32_t)(a.k + j + 2);
+          const int par1 = (j + 1) & 1, rep = lane >> 5;
+          put_word(rowsl + (((size_t)(par1 * P2_NREP + rep) * P2_MAXG + g) * 32 + ql), tag1, __float_as_uint(nv));
+          put_word(rowsl + (((size_t)(par1 * P2_NREP + rep + 2) * P2_MAXG + g) * 32 + ql), tag1, __float_as_uint(nv));
         }
         PANEL2_MARK(j, 1);
         float pr[32];
@@ -887,13 +992,25 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
   }
 
   // results: rows to their final positions, pivots, info, swap plan
+  PANEL2_WGMARK(2);
   if (wave == 0) { spos[lane] = pos0; spos[lane + 64] = pos1; }
   lds_barrier();
   const bool ok = (*s_abort == 0);
   if (ok) {
-    for (int idx = t; idx < kb * P2_R; idx += 256) {
-      const int c = idx >> 7, r = idx & (P2_R - 1);
-      if (r < rows) Ap[(int64_t)spos[r] + (int64_t)c * a.lda] = P[c * P2_PS + r];
+    // thread (row r, column parity ch): 32 LDS reads in flight, then 32 stores to the row's final position
+    const int r = t & (P2_R - 1), ch = t >> 7;
+    if (r < rows) {
+      float* dst = Ap + (int64_t)spos[r];
+      for (int cb = 0; cb < kb; cb += 64) {
+        float v[32];
+#pragma unroll
+        for (int i = 0; i < 32; i++) v[i] = P[min(cb + 2 * i + ch, kb - 1) * P2_PS + r];
+#pragma unroll
+        for (int i = 0; i < 32; i++) {
+          const int c = cb + 2 * i + ch;
+          if (c < kb) dst[(int64_t)c * a.lda] = v[i];
+        }
+      }
     }
     if (g == 0) {
       for (int jj = t; jj < kb; jj += 256) a.ipiv[a.k + jj] = a.k + s_piv[jj] + 1;
@@ -911,6 +1028,7 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
     }
   }
   if (t == 0 && *s_abort) a.dinfo[1] = 1;
+  PANEL2_WGMARK(3);
 }
 
 // Inverse of diagonal block b of L11 into Linv (column-major 32 x 32); threads 0..31 compute, all threads
@@ -967,26 +1085,47 @@ __global__ __launch_bounds__(256) void k_laswp(float* __restrict__ A, int64_t ld
     dst[q] = (e < cnt) ? plan[1 + 2 * e] : -1;
     src[q] = (e < cnt) ? plan[2 + 2 * e] : -1;
   }
-  for (int64_t cc = bx; cc < ncols + do_perm; cc += gx) {
-    if (cc == ncols) {
-      int32_t* pp = perm + k;
-      int32_t v[4];
+  // LB columns per pass: every load of the pass (unconditional, from row src or, for an unused slot, row 0 of
+  // the column) is in flight before the barrier, then the stores.  Passes touch disjoint columns, so only the
+  // load -> store order inside a pass needs the barrier.  The permutation vector is "column" ncols.
+  constexpr int LB = 8;
+  int sl[4];
 #pragma unroll
-      for (int q = 0; q < 4; q++) if (dst[q] >= 0) v[q] = pp[src[q]];
-      __syncthreads();
+  for (int q = 0; q < 4; q++) sl[q] = dst[q] >= 0 ? src[q] : 0;
+  for (int64_t c0 = (int64_t)bx * LB; c0 < ncols + do_perm; c0 += (int64_t)gx * LB) {
+    float v[LB][4];
+    float* cp[LB];
 #pragma unroll
-      for (int q = 0; q < 4; q++) if (dst[q] >= 0) pp[dst[q]] = v[q];
-    } else {
+    for (int j = 0; j < LB; j++) {
+      const int64_t cc = min(c0 + j, ncols - 1 >= 0 ? ncols - 1 : 0);
       const int64_t col = (cc < na) ? ra0 + cc : rb0 + (cc - na);
-      float* cp = A + col * lda + k;
-      float v[4];
+      cp[j] = A + col * lda + k;
+    }
+    if (ncols > 0) {
 #pragma unroll
-      for (int q = 0; q < 4; q++) if (dst[q] >= 0) v[q] = cp[src[q]];
-      __syncthreads();
+      for (int j = 0; j < LB; j++)
 #pragma unroll
-      for (int q = 0; q < 4; q++) if (dst[q] >= 0) cp[dst[q]] = v[q];
+        for (int q = 0; q < 4; q++) v[j][q] = cp[j][sl[q]];
+    }
+    const bool with_perm = do_perm && c0 <= ncols && ncols < c0 + LB;
+    int32_t pv[4];
+    int32_t* pp = perm + k;
+    if (with_perm) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) pv[q] = pp[sl[q]];
     }
     __syncthreads();
+#pragma unroll
+    for (int j = 0; j < LB; j++) {
+      if (c0 + j < ncols) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) if (dst[q] >= 0) cp[j][dst[q]] = v[j][q];
+      }
+    }
+    if (with_perm) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) if (dst[q] >= 0) pp[dst[q]] = pv[q];
+    }
   }
 }
 
@@ -1946,8 +2085,11 @@ static void laswp_ranges(const GetrfArgs& a, Workspace& W, int64_t k, int kb, in
   const int64_t ncols = (ra1 - ra0) + (rb1 - rb0) + (perm ? 1 : 0);
   const int ninv = with_inv ? (int)cdiv(kb, 32) : 0;
   if (ncols <= 0 && ninv == 0 && zero_plan == nullptr) return;
+  // at most two workgroups per CU, 8 columns per pass each: the panel launched beside this one on the other
+  // stream finds room on every CU instead of queueing behind thousands of short workgroups
+  const int64_t nwg = std::max<int64_t>(1, std::min<int64_t>(cdiv(ncols, 8), 2 * (int64_t)W.num_cus));
   hipEvent_t e1 = W.prof_begin();
-  hipLaunchKernelGGL(k_laswp, dim3((unsigned)(ninv + std::max<int64_t>(1, std::min<int64_t>(ncols, 4096)))), dim3(256), 0, W.stream,
+  hipLaunchKernelGGL(k_laswp, dim3((unsigned)(ninv + nwg)), dim3(256), 0, W.stream,
                      a.A, a.lda, k, ra0, ra1, rb0, rb1, perm ? 1 : 0, plan, a.perm, kb, ninv, W.linv.p, zero_plan);
   LAUNCH_CK();
   W.prof_end(e1, pname, 16.0 * (double)(ncols - (perm ? 1 : 0)) * kb, 0.0);
@@ -2069,9 +2211,13 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
         pack_L_ahead_fmt(a.fmt, a.A, a.lda, k + kb, kbn, nb, m2n, a.tile_scale, W, W.pstream);
         HIP_CK(hipEventRecord(W.ev_pack, W.pstream));
       }
-      laswp_ranges(a, W, k, kb, 0, k, cN, n, true, plan, false, nullptr);
+      // interchanges of the remaining columns: the trailing ones now (the TRSM and the update read them); the
+      // factored L columns left of the panel, whose scattered one-element-per-line traffic would otherwise
+      // meet the next panel's start, after the update (nothing reads them before the next step's interchanges)
+      laswp_ranges(a, W, k, kb, 0, 0, cN, n, false, plan, false, nullptr);
       trsm_cols(a, W, k, kb, cN, n, false);
       update_cols_fmt(a.fmt, a.A, a.lda, k, kb, nb, m2, cN, n, true, false, false, a.tile_scale, W);
+      laswp_ranges(a, W, k, kb, 0, k, 0, 0, true, plan, false, nullptr, "laswp_left");
       HIP_CK(hipStreamWaitEvent(W.stream, W.ev_panel, 0));
     } else {
       trsm_cols(a, W, k, kb, k + kb, n, use_fma_trsm);

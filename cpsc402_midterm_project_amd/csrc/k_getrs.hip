@@ -74,11 +74,21 @@ __global__ __launch_bounds__(128) void k_trinv(const float* __restrict__ LU, int
   const int b = blockIdx.x, c = threadIdx.x;
   const int64_t o = (int64_t)b * GB;
   const int nb = (int)min((int64_t)GB, n - o);
-  for (int idx = c; idx < GB * GB; idx += GB) {
-    const int p = idx >> 7, q = idx & (GB - 1);    // column p, row q: coalesced over q
-    float v = (q == p) ? 1.0f : 0.0f;
-    if (q < nb && p < nb) v = LU[(o + p) * lda + o + q];
-    T[q * (GB + 1) + p] = v;
+  {
+    // thread c reads row q = c of every column p (coalesced over q), 32 columns in flight at a time; rows and
+    // columns past n read a clamped element and store the identity
+    const int q = c;
+    const float* src = LU + o + min(q, nb - 1);
+    for (int pb = 0; pb < GB; pb += 32) {
+      float v[32];
+#pragma unroll
+      for (int i = 0; i < 32; i++) v[i] = src[(o + min(pb + i, nb - 1)) * lda];
+#pragma unroll
+      for (int i = 0; i < 32; i++) {
+        const int p = pb + i;
+        T[q * (GB + 1) + p] = (q < nb && p < nb) ? v[i] : ((q == p) ? 1.0f : 0.0f);
+      }
+    }
   }
   __syncthreads();
   if (LOWER) {
@@ -132,12 +142,27 @@ __global__ __launch_bounds__(256) void k_chainmat(const float* __restrict__ LU, 
     return;
   }
   const int64_t ro = (int64_t)i * GB, co = (int64_t)j * GB;
-  for (int idx = t; idx < GB * GB; idx += 256) {
-    const int col = idx >> 7, row = idx & (GB - 1);   // coalesced over rows
-    Ts[row * (GB + 1) + col] = Tinv[(size_t)i * GB * GB + idx];
-    float v = 0.0f;
-    if (ro + row < n && co + col < n) v = LU[(co + col) * lda + ro + row];
-    Bs[row * (GB + 1) + col] = v;
+  {
+    // thread (row, column parity): 16 columns of both operands in flight at a time, coalesced over rows;
+    // elements past n read a clamped address and store zero
+    const int row = t & (GB - 1), ch = t >> 7;
+    const float* tsrc = Tinv + (size_t)i * GB * GB + row;
+    const float* bsrc = LU + min(ro + row, n - 1);
+    for (int cb = 0; cb < GB; cb += 32) {
+      float tv[16], bv[16];
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        const int col = cb + 2 * e + ch;
+        tv[e] = tsrc[(size_t)col * GB];
+        bv[e] = bsrc[min(co + col, n - 1) * lda];
+      }
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        const int col = cb + 2 * e + ch;
+        Ts[row * (GB + 1) + col] = tv[e];
+        Bs[row * (GB + 1) + col] = (ro + row < n && co + col < n) ? bv[e] : 0.0f;
+      }
+    }
   }
   __syncthreads();
   const int tr = t >> 4, tc = t & 15;

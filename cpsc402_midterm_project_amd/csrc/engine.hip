@@ -39,7 +39,8 @@ __global__ void k_add_x32(float* x32, double* x64, const double* d, int64_t m) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i < m) { const float v = (float)((double)x32[i] + d[i]); x32[i] = v; x64[i] = (double)v; }
 }
-__global__ void k_add_vec(double* h, const double* h2, int cnt) {
+__global__ void k_add_vec(double* h, const double* h2, int cnt, const int* __restrict__ act) {
+  if (act && *act == 0) return;
   const int i = threadIdx.x + blockIdx.x * 64;
   if (i < cnt) h[i] += h2[i];
 }
@@ -470,7 +471,9 @@ static void promote_factors(mp_ctx* C, Factors& F) {
 
 // d = A^{-1} r through the factors (r, d FP64 device, n x nrhs, leading dimension n); with apply64 the FP64
 // copy of FP32 factors is applied in FP64
-static void apply_inverse(mp_ctx* C, const Factors& F, const double* r, double* d, int nrhs, int* trs, bool apply64 = false) {
+// active (device word, own FP32 substitutions only): the substitutions return at once when it is 0.
+static void apply_inverse(mp_ctx* C, const Factors& F, const double* r, double* d, int nrhs, int* trs, bool apply64 = false,
+                          const int* active = nullptr) {
   Workspace& W = C->W;
   const int64_t n = F.n; const int64_t m = n * nrhs;
   // with F = g Dr A Dc: A^{-1} r = g Dc F^{-1} (Dr r)  (src/mp_core.cuh:184,188-189)
@@ -488,7 +491,7 @@ static void apply_inverse(mp_ctx* C, const Factors& F, const double* r, double* 
     RB_CK(rocsolver_dgetrs(C->rb, rocblas_operation_none, (rocblas_int)n, (rocblas_int)nrhs, const_cast<double*>(LU),
                            (rocblas_int)n, F.ipiv32.p, d, (rocblas_int)n));
   } else {
-    getrs(n, nrhs, F.LU32.p, n, F.perm.p, r, n, d, n, F.gscale, W, true);    // the scale by g is fused
+    getrs(n, nrhs, F.LU32.p, n, F.perm.p, r, n, d, n, F.gscale, W, true, active);    // the scale by g is fused
   }
   if (F.scale_mode == 2) {
     hipLaunchKernelGGL(k_scale_rows, dim3(cdiv(m, 256)), dim3(256), 0, W.stream, d, d, n, nrhs, F.dc.p);
@@ -668,9 +671,48 @@ struct Engine {
       const double betac = hp[0];
       if (betac == 0) { status = "converged_true"; break; }
       K.scale_copy(K.rcur.p, K.V.p, K.scal(), true);   // v0 = rcur / betac
-      K.init_cycle(betac);
+      // Right-preconditioned steps on the library's own FP32 substitutions run one ahead of the host: step j+1 is
+      // enqueued before step j's numbers are read, and the Givens kernel of the step that ends the cycle clears
+      // the activity word, so the kernels of a step enqueued past it return at once.  The arithmetic of every
+      // step is the one below.
+      const bool fp64_apply = F.prec == 64 || (M.apply64 && F.promoted);
+      const bool ahead = !M.gmres_left && !fp64_apply && F.scale_mode != 2 && m <= 255;
+      K.init_cycle(betac, ahead ? K.active() : nullptr);
       int j = 0; bool happy = false, nonfinite = false;
-      for (j = 0; j < m && budget_left > 0; j++) {
+      if (ahead) {
+        int* act = K.active();
+        double* slots = hp + 64;                        // pinned: per step rec, hn, finite
+        for (int e = 0; e < 2; e++)
+          if (!K.ev[e]) HIP_CK(hipEventCreateWithFlags(&K.ev[e], hipEventDisableTiming));
+        auto enqueue = [&](int s) {
+          double* vs = K.V.p + (int64_t)s * n;
+          double* zs = K.Z.p + (int64_t)s * n;
+          apply_inverse(C, F, vs, zs, 1, nullptr, M.apply64, act);
+          ResidArgs ra; ra.n = n; ra.nrhs = 1; ra.a32 = P.fp32; ra.A = P.a_ptr(); ra.lda = n; ra.X = zs; ra.ldx = n;
+          ra.R = K.w.p; ra.ldr = n; ra.matvec = true; ra.active = act;
+          residual(ra, W());
+          K.arnoldi_tail(s, beta0, M.gmres_tol, act);     // CGS2, ||w||, Givens (the step that ends the cycle clears act)
+          HIP_CK(hipMemcpyAsync(slots + 4 * s, K.scal() + 2, 24, hipMemcpyDeviceToHost, W().stream));
+          HIP_CK(hipEventRecord(K.ev[s & 1], W().stream));
+          K.scale_copy(K.w.p, K.V.p + (int64_t)(s + 1) * n, K.scal(), true, act);   // v_{s+1} = w / hn
+        };
+        // a step goes out only if the budget lets it run after the steps already in flight
+        int enq = 0;
+        if (m > 0 && budget_left > 0) { enqueue(0); enq = 1; }
+        for (j = 0; j < enq; j++) {
+          if (enq < m && enq - j < budget_left) { enqueue(enq); enq++; }
+          HIP_CK(hipEventSynchronize(K.ev[j & 1]));
+          const double rec = slots[4 * j], hn = slots[4 * j + 1];
+          const bool fin = slots[4 * j + 2] != 0.0;
+          T.trs++; T.precond_applies++; T.matvecs++;
+          if (!fin) { nonfinite = true; break; }
+          T.inner_total++; budget_left--;
+          if (instrument) T.ihist.push_back({outer, cycles, j, rhs, rec, -1, ""});
+          if (hn == 0.0) { happy = true; j++; break; }
+          if (rec <= M.gmres_tol) { j++; break; }
+        }
+      }
+      for (; !ahead && j < m && budget_left > 0; j++) {
         double* vj = K.V.p + (int64_t)j * n;
         if (M.gmres_left) {
           // w = F^{-1} (A v_j)
@@ -691,7 +733,7 @@ struct Engine {
         K.update(K.V.p, j + 1, K.h(), K.w.p, -1.0);
         K.dots(K.V.p, j + 1, K.w.p, K.h2(), false);
         K.update(K.V.p, j + 1, K.h2(), K.w.p, -1.0);
-        hipLaunchKernelGGL(k_add_vec, dim3(cdiv(j + 1, 64)), dim3(64), 0, W().stream, K.h(), K.h2(), j + 1);
+        hipLaunchKernelGGL(k_add_vec, dim3(cdiv(j + 1, 64)), dim3(64), 0, W().stream, K.h(), K.h2(), j + 1, (const int*)nullptr);
         LAUNCH_CK();
         K.norm(K.w.p, K.scal());
         K.givens(j, K.h(), K.scal(), beta0, K.scal() + 2);

@@ -198,7 +198,10 @@ __global__ __launch_bounds__(512) void k_trsv2(const float* __restrict__ LU, int
                                                const double* __restrict__ Rin, int64_t ldr,
                                                float* __restrict__ Y, int64_t ldy,
                                                double* __restrict__ D, int64_t ldd, double scale,
-                                               uint64_t* __restrict__ ybuf, uint32_t tag, uint32_t wtag, int32_t* __restrict__ err, long spin) {
+                                               uint64_t* __restrict__ ybuf, uint32_t tag, uint32_t wtag, int32_t* __restrict__ err, long spin,
+                                               const int* __restrict__ act) {
+  // a step of a Krylov cycle that has already ended: every workgroup returns (none waits for another)
+  if (act && *act == 0) return;
   __shared__ float ys[3][NR][GB];
   __shared__ float part[4][NR][GB];
   __shared__ int s_abort;
@@ -339,7 +342,7 @@ __global__ __launch_bounds__(512) void k_trsv2(const float* __restrict__ LU, int
 
 template <int NR>
 static void trsv_pair(int64_t n, const float* LU, int64_t lda, const int32_t* perm, const double* R, int64_t ldr,
-                      double* D, int64_t ldd, double scale, Workspace& W, int32_t* err) {
+                      double* D, int64_t ldd, double scale, Workspace& W, int32_t* err, const int* act) {
   const int nbk = (int)cdiv(n, GB);
   const uint32_t tl = W.take_tags(1), tu = W.take_tags(1);
   const float* Ti = W.tinv.p;
@@ -351,16 +354,17 @@ static void trsv_pair(int64_t n, const float* LU, int64_t lda, const int32_t* pe
   const uint32_t wl = W.gs_force ? 0xFFFFFFFFu : tl, wu = W.gs_force ? 0xFFFFFFFFu : tu;
   hipEvent_t ev = W.prof_begin();
   hipLaunchKernelGGL((k_trsv2<NR, true>), dim3(nbk), dim3(512), 0, W.stream, LU, lda, n, Ti, Mi, perm, R, ldr,
-                     W.rhs32.p, n, (double*)nullptr, (int64_t)0, 1.0, W.ybuf.p, tl, wl, err, spin);
+                     W.rhs32.p, n, (double*)nullptr, (int64_t)0, 1.0, W.ybuf.p, tl, wl, err, spin, act);
   LAUNCH_CK();
   hipLaunchKernelGGL((k_trsv2<NR, false>), dim3(nbk), dim3(512), 0, W.stream, LU, lda, n, Tu, Mu, perm, (const double*)nullptr, (int64_t)0,
-                     W.rhs32.p, n, D, ldd, scale, W.ybuf.p, tu, wu, err, spin);
+                     W.rhs32.p, n, D, ldd, scale, W.ybuf.p, tu, wu, err, spin, act);
   LAUNCH_CK();
   W.prof_end(ev, "getrs", 4.0 * (double)n * n + 24.0 * n * NR, 2.0 * (double)n * n * NR);
 }
 
 void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* perm,
-           const double* R, int64_t ldr, double* D, int64_t ldd, double scale, Workspace& W, bool reuse_inverse) {
+           const double* R, int64_t ldr, double* D, int64_t ldd, double scale, Workspace& W, bool reuse_inverse,
+           const int* active) {
   MP_REQUIRE(n > 0 && nrhs > 0, "getrs: n, nrhs > 0");
   MP_REQUIRE(lda >= n && ldr >= n && ldd >= n, "getrs: leading dimensions >= n");
   const int64_t nbk = cdiv(n, GB);
@@ -411,8 +415,8 @@ void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* per
     const double* Rc = R + (int64_t)c * ldr;
     double* Dc = D + (int64_t)c * ldd;
     int32_t* err = W.dinfo.p + 2;
-    if (left >= 2) { trsv_pair<2>(n, LU, lda, perm, Rc, ldr, Dc, ldd, scale, W, err); c += 2; }
-    else { trsv_pair<1>(n, LU, lda, perm, Rc, ldr, Dc, ldd, scale, W, err); c += 1; }
+    if (left >= 2) { trsv_pair<2>(n, LU, lda, perm, Rc, ldr, Dc, ldd, scale, W, err, active); c += 2; }
+    else { trsv_pair<1>(n, LU, lda, perm, Rc, ldr, Dc, ldd, scale, W, err, active); c += 1; }
   }
 }
 

@@ -5,8 +5,11 @@
 // and copies every Hessenberg column to the host for the Givens rotations (src/mp_core.cuh:255-263).
 // Here each Arnoldi step does classical Gram-Schmidt twice (CGS2): one block GEMV-T + one block GEMV per
 // pass, with all reductions in a fixed order (chunk partials combined in chunk order) so results do not
-// depend on scheduling.  The rotations and the recursive residual live on the device; the host reads back
-// three numbers per step (|g_{j+1}|/beta0, h_{j+1,j}, a finiteness flag) to steer the loop.
+// depend on scheduling.  The rotations and the recursive residual live on the device, and so does the decision
+// to end a cycle: the Givens kernel clears an activity word when the step converged, broke down or met a
+// non-finite value, and every kernel of an Arnoldi step given that word returns at once when it is clear.  The
+// host can therefore enqueue step j+1 before it reads step j's three numbers (|g_{j+1}|/beta0, h_{j+1,j}, a
+// finiteness flag); a step enqueued past the end of the cycle does nothing.
 #include "mp_internal.hpp"
 #include "mp_kernels.hpp"
 #include "mp_krylov.hpp"
@@ -17,7 +20,9 @@ constexpr int KCH = 256;   // rows per chunk of the reductions
 
 // part[chunk*ncols + i] = sum over the chunk's rows of V[r, i] * w[r], i < ncols
 __global__ __launch_bounds__(256) void k_gemvt_partial(const double* __restrict__ V, int64_t ldv, int ncols,
-                                                       const double* __restrict__ w, int64_t n, double* __restrict__ part) {
+                                                       const double* __restrict__ w, int64_t n, double* __restrict__ part,
+                                                       const int* __restrict__ act) {
+  if (act && *act == 0) return;
   __shared__ double red[4][64];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int64_t r = (int64_t)blockIdx.x * KCH + t;
@@ -38,7 +43,9 @@ __global__ __launch_bounds__(256) void k_gemvt_partial(const double* __restrict_
 }
 
 // out[i] (+)= sum over chunks in order; if acc != 0 the result is added to out[i]
-__global__ void k_gemvt_final(const double* __restrict__ part, int nchunks, int ncols, double* __restrict__ out, int acc) {
+__global__ void k_gemvt_final(const double* __restrict__ part, int nchunks, int ncols, double* __restrict__ out, int acc,
+                              const int* __restrict__ act) {
+  if (act && *act == 0) return;
   const int i = blockIdx.x * 64 + threadIdx.x;
   if (i >= ncols) return;
   double s = 0.0;
@@ -48,7 +55,9 @@ __global__ void k_gemvt_final(const double* __restrict__ part, int nchunks, int 
 
 // w[r] += sign * sum_i V[r, i] * h[i]
 __global__ __launch_bounds__(256) void k_gemv_update(const double* __restrict__ V, int64_t ldv, int ncols,
-                                                     const double* __restrict__ h, double* __restrict__ w, int64_t n, double sign) {
+                                                     const double* __restrict__ h, double* __restrict__ w, int64_t n, double sign,
+                                                     const int* __restrict__ act) {
+  if (act && *act == 0) return;
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (r >= n) return;
   double s = 0.0;
@@ -56,7 +65,46 @@ __global__ __launch_bounds__(256) void k_gemv_update(const double* __restrict__ 
   w[r] = fma(sign, s, w[r]);
 }
 
-__global__ __launch_bounds__(256) void k_sumsq_partial(const double* __restrict__ w, int64_t n, double* __restrict__ part) {
+// One CGS pass's second half: h[i] = sum of the chunk partials of V^T w in chunk order (k_gemvt_final's sum,
+// formed by every workgroup into LDS; workgroup 0 stores it), then w[r] += sign * sum_i V[r, i] h[i] (k_gemv_update's
+// order).  With sq != nullptr the chunk partials of sum w^2 of the updated w follow (k_sumsq_partial's order; the
+// workgroup of 256 rows is one chunk).  ncols <= 256.
+__global__ __launch_bounds__(256) void k_cgs_update(const double* __restrict__ part, int nchunks, int ncols,
+                                                    const double* __restrict__ V, int64_t ldv, double* __restrict__ hout,
+                                                    double* __restrict__ w, int64_t n, double sign, double* __restrict__ sq,
+                                                    const int* __restrict__ act) {
+  if (act && *act == 0) return;
+  __shared__ double hs[256];
+  __shared__ double red[4];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (t < ncols) {
+    double s = 0.0;
+    for (int c = 0; c < nchunks; c++) s += part[(int64_t)c * ncols + t];
+    hs[t] = s;
+    if (blockIdx.x == 0) hout[t] = s;
+  }
+  __syncthreads();
+  const int64_t r = (int64_t)blockIdx.x * 256 + t;
+  double wn = 0.0;
+  if (r < n) {
+    double s = 0.0;
+    for (int i = 0; i < ncols; i++) s = fma(V[(int64_t)i * ldv + r], hs[i], s);
+    wn = fma(sign, s, w[r]);
+    w[r] = wn;
+  }
+  if (sq) {
+    double q = (r < n) ? wn * wn : 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) q += __shfl_xor(q, off, 64);
+    if (lane == 0) red[wv] = q;
+    __syncthreads();
+    if (t == 0) sq[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sumsq_partial(const double* __restrict__ w, int64_t n, double* __restrict__ part,
+                                                       const int* __restrict__ act) {
+  if (act && *act == 0) return;
   __shared__ double red[4];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int64_t r = (int64_t)blockIdx.x * KCH + t;
@@ -69,7 +117,8 @@ __global__ __launch_bounds__(256) void k_sumsq_partial(const double* __restrict_
 }
 
 // out[0] = sqrt(sum of partials)
-__global__ void k_norm_final(const double* __restrict__ part, int nchunks, double* __restrict__ out) {
+__global__ void k_norm_final(const double* __restrict__ part, int nchunks, double* __restrict__ out, const int* __restrict__ act) {
+  if (act && *act == 0) return;
   __shared__ double red[256];
   const int t = threadIdx.x;
   double s = 0.0;
@@ -81,7 +130,9 @@ __global__ void k_norm_final(const double* __restrict__ part, int nchunks, doubl
 }
 
 // y[r] = x[r] * (*s) ; s == nullptr -> y = x * sv
-__global__ void k_scale_copy(const double* __restrict__ x, double* __restrict__ y, int64_t n, const double* __restrict__ s, int inv) {
+__global__ void k_scale_copy(const double* __restrict__ x, double* __restrict__ y, int64_t n, const double* __restrict__ s, int inv,
+                             const int* __restrict__ act) {
+  if (act && *act == 0) return;
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (r >= n) return;
   const double v = s[0];
@@ -103,11 +154,13 @@ __global__ void k_fill_f64(double* __restrict__ x, int64_t n, double v) {
 
 // Hessenberg column j: h[0..j] (projections) and hn = ||w||.  State: H[(m+1) x m] column-major, cs, sn, g.
 // Applies the previous rotations, forms a new one (rho = hypot(a, b); rho == 0 -> (1, 0)), updates g and
-// writes out[0] = |g[j+1]| / beta0, out[1] = hn, out[2] = 1 if every value involved is finite.
+// writes out[0] = |g[j+1]| / beta0, out[1] = hn, out[2] = 1 if every value involved is finite.  With an activity
+// word, the step ends the cycle (clears it) when out[0] <= tol, hn == 0 or a value is not finite.
 __global__ void k_givens_step(double* __restrict__ H, int m, int j, const double* __restrict__ h, const double* __restrict__ hn,
                               double* __restrict__ cs, double* __restrict__ sn, double* __restrict__ g, double beta0,
-                              double* __restrict__ out) {
+                              double* __restrict__ out, double tol, int* __restrict__ act) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (act && *act == 0) return;
   double* col = H + (int64_t)j * (m + 1);
   bool fin = true;
   for (int i = 0; i <= j; i++) { col[i] = h[i]; fin = fin && isfinite(h[i]); }
@@ -129,6 +182,50 @@ __global__ void k_givens_step(double* __restrict__ H, int m, int j, const double
   out[0] = fabs(g[j + 1]) / beta0;
   out[1] = hnv;
   out[2] = fin ? 1.0 : 0.0;
+  if (act && (!fin || hnv == 0.0 || out[0] <= tol)) *act = 0;
+}
+
+// The Arnoldi step's tail in one launch of 256 threads: h[i] += h2[i] (the two CGS passes), hn = sqrt of the sum
+// of the w^2 chunk partials (k_norm_final's order), then k_givens_step's rotation update on thread 0; hn is also
+// stored at hnout for the scaling of the next basis vector.
+__global__ __launch_bounds__(256) void k_givens_fused(double* __restrict__ H, int m, int j, double* __restrict__ h,
+                                                      const double* __restrict__ h2, const double* __restrict__ sq, int nchunks,
+                                                      double* __restrict__ hnout, double* __restrict__ cs, double* __restrict__ sn,
+                                                      double* __restrict__ g, double beta0, double* __restrict__ out, double tol,
+                                                      int* __restrict__ act) {
+  if (act && *act == 0) return;
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  if (t <= j) h[t] += h2[t];
+  double s = 0.0;
+  for (int c = t; c < nchunks; c += 256) s += sq[c];
+  red[t] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) { if (t < st) red[t] += red[t + st]; __syncthreads(); }
+  if (t != 0) return;
+  const double hnv = sqrt(red[0]);
+  hnout[0] = hnv;
+  double* col = H + (int64_t)j * (m + 1);
+  bool fin = true;
+  for (int i = 0; i <= j; i++) { col[i] = h[i]; fin = fin && isfinite(h[i]); }
+  col[j + 1] = hnv;
+  fin = fin && isfinite(hnv);
+  for (int i = 0; i < j; i++) {
+    const double a = col[i], b = col[i + 1];
+    col[i] = cs[i] * a + sn[i] * b;
+    col[i + 1] = -sn[i] * a + cs[i] * b;
+  }
+  const double a = col[j], b = col[j + 1];
+  const double rho = hypot(a, b);
+  if (rho == 0.0) { cs[j] = 1.0; sn[j] = 0.0; } else { cs[j] = a / rho; sn[j] = b / rho; }
+  col[j] = rho; col[j + 1] = 0.0;
+  const double g0 = g[j];
+  g[j] = cs[j] * g0;
+  g[j + 1] = -sn[j] * g0;
+  out[0] = fabs(g[j + 1]) / beta0;
+  out[1] = hnv;
+  out[2] = fin ? 1.0 : 0.0;
+  if (act && (!fin || hnv == 0.0 || out[0] <= tol)) *act = 0;
 }
 
 // Solve the k x k upper-triangular H y = g (back substitution, one thread).
@@ -141,9 +238,11 @@ __global__ void k_hess_solve(const double* __restrict__ H, int m, int k, const d
   }
 }
 
-__global__ void k_gmres_init(double* __restrict__ g, double* __restrict__ H, int m, double beta) {
+// Start of a cycle: g = beta e_1, H = 0, and the activity word (if any) set.
+__global__ void k_gmres_init(double* __restrict__ g, double* __restrict__ H, int m, double beta, int* __restrict__ act) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int64_t hsz = (int64_t)(m + 1) * m;
+  if (i == 0 && act) *act = 1;
   if (i <= m) g[i] = (i == 0) ? beta : 0.0;
   for (int64_t q = i; q < hsz; q += (int64_t)gridDim.x * 256) H[q] = 0.0;
 }
@@ -162,36 +261,56 @@ void Krylov::ensure(int64_t n_, int m_) {
   part.alloc(*W->tracker, (size_t)cdiv(n, KCH) * (m + 2) + 64, "gmres_part");
 }
 
-void Krylov::dots(const double* Vb, int ncols, const double* x, double* out, bool accumulate) {
+void Krylov::dots(const double* Vb, int ncols, const double* x, double* out, bool accumulate, const int* act) {
   const unsigned nch = cdiv(n, KCH);
-  hipLaunchKernelGGL(k_gemvt_partial, dim3(nch), dim3(256), 0, W->stream, Vb, n, ncols, x, n, part.p);
+  hipLaunchKernelGGL(k_gemvt_partial, dim3(nch), dim3(256), 0, W->stream, Vb, n, ncols, x, n, part.p, act);
   LAUNCH_CK();
-  hipLaunchKernelGGL(k_gemvt_final, dim3(cdiv(ncols, 64)), dim3(64), 0, W->stream, part.p, (int)nch, ncols, out, accumulate ? 1 : 0);
-  LAUNCH_CK();
-}
-void Krylov::update(const double* Vb, int ncols, const double* h, double* x, double sign) {
-  hipLaunchKernelGGL(k_gemv_update, dim3(cdiv(n, 256)), dim3(256), 0, W->stream, Vb, n, ncols, h, x, n, sign);
+  hipLaunchKernelGGL(k_gemvt_final, dim3(cdiv(ncols, 64)), dim3(64), 0, W->stream, part.p, (int)nch, ncols, out, accumulate ? 1 : 0, act);
   LAUNCH_CK();
 }
-void Krylov::norm(const double* x, double* out) {
+void Krylov::update(const double* Vb, int ncols, const double* h, double* x, double sign, const int* act) {
+  hipLaunchKernelGGL(k_gemv_update, dim3(cdiv(n, 256)), dim3(256), 0, W->stream, Vb, n, ncols, h, x, n, sign, act);
+  LAUNCH_CK();
+}
+void Krylov::norm(const double* x, double* out, const int* act) {
   const unsigned nch = cdiv(n, KCH);
-  hipLaunchKernelGGL(k_sumsq_partial, dim3(nch), dim3(256), 0, W->stream, x, n, part.p);
+  hipLaunchKernelGGL(k_sumsq_partial, dim3(nch), dim3(256), 0, W->stream, x, n, part.p, act);
   LAUNCH_CK();
-  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, W->stream, part.p, (int)nch, out);
-  LAUNCH_CK();
-}
-void Krylov::scale_copy(const double* x, double* y, const double* s, bool inv) {
-  hipLaunchKernelGGL(k_scale_copy, dim3(cdiv(n, 256)), dim3(256), 0, W->stream, x, y, n, s, inv ? 1 : 0);
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, W->stream, part.p, (int)nch, out, act);
   LAUNCH_CK();
 }
-void Krylov::givens(int j, const double* h, const double* hn, double beta0, double* out) {
+void Krylov::scale_copy(const double* x, double* y, const double* s, bool inv, const int* act) {
+  hipLaunchKernelGGL(k_scale_copy, dim3(cdiv(n, 256)), dim3(256), 0, W->stream, x, y, n, s, inv ? 1 : 0, act);
+  LAUNCH_CK();
+}
+void Krylov::givens(int j, const double* h, const double* hn, double beta0, double* out, double tol, int* act) {
   double* cs = small.p; double* sn = cs + (m + 1); double* g = sn + (m + 1);
-  hipLaunchKernelGGL(k_givens_step, dim3(1), dim3(64), 0, W->stream, H.p, m, j, h, hn, cs, sn, g, beta0, out);
+  hipLaunchKernelGGL(k_givens_step, dim3(1), dim3(64), 0, W->stream, H.p, m, j, h, hn, cs, sn, g, beta0, out, tol, act);
   LAUNCH_CK();
 }
-void Krylov::init_cycle(double beta) {
+void Krylov::init_cycle(double beta, int* act) {
   double* g = small.p + 2 * (m + 1);
-  hipLaunchKernelGGL(k_gmres_init, dim3(cdiv((size_t)(m + 1) * m + 1, 256)), dim3(256), 0, W->stream, g, H.p, m, beta);
+  hipLaunchKernelGGL(k_gmres_init, dim3(cdiv((size_t)(m + 1) * m + 1, 256)), dim3(256), 0, W->stream, g, H.p, m, beta, act);
+  LAUNCH_CK();
+}
+// CGS2 of w against V[:, 0..j] and the step's Givens update: 5 launches with the sums of the unfused kernels.
+void Krylov::arnoldi_tail(int j, double beta0, double tol, int* act) {
+  const unsigned nch = cdiv(n, KCH);
+  const int ncols = j + 1;
+  double* sq = part.p + (size_t)nch * (m + 1);
+  double* cs = small.p; double* sn = cs + (m + 1); double* g = sn + (m + 1);
+  hipLaunchKernelGGL(k_gemvt_partial, dim3(nch), dim3(256), 0, W->stream, V.p, n, ncols, w.p, n, part.p, (const int*)act);
+  LAUNCH_CK();
+  hipLaunchKernelGGL(k_cgs_update, dim3(nch), dim3(256), 0, W->stream, part.p, (int)nch, ncols, V.p, n, h(), w.p, n, -1.0,
+                     (double*)nullptr, (const int*)act);
+  LAUNCH_CK();
+  hipLaunchKernelGGL(k_gemvt_partial, dim3(nch), dim3(256), 0, W->stream, V.p, n, ncols, w.p, n, part.p, (const int*)act);
+  LAUNCH_CK();
+  hipLaunchKernelGGL(k_cgs_update, dim3(nch), dim3(256), 0, W->stream, part.p, (int)nch, ncols, V.p, n, h2(), w.p, n, -1.0,
+                     sq, (const int*)act);
+  LAUNCH_CK();
+  hipLaunchKernelGGL(k_givens_fused, dim3(1), dim3(256), 0, W->stream, H.p, m, j, h(), h2(), sq, (int)nch, scal(), cs, sn, g,
+                     beta0, scal() + 2, tol, act);
   LAUNCH_CK();
 }
 void Krylov::lsq(int k, double* y) {

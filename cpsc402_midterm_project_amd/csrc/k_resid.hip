@@ -30,7 +30,8 @@ __device__ __forceinline__ double dmax_nan(double a, double b) {  // NaN-propaga
 template <typename TA, int NR, bool DD>
 __global__ __launch_bounds__(256) void k_resid_partial(const TA* __restrict__ A, int64_t n, int64_t lda,
                                                        const double* __restrict__ X, int64_t ldx, int64_t cpc,
-                                                       double* __restrict__ P) {
+                                                       double* __restrict__ P, const int* __restrict__ act) {
+  if (act && *act == 0) return;
   constexpr int F = DD ? 3 : 2;
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t chunk = blockIdx.y;
@@ -104,7 +105,8 @@ __global__ __launch_bounds__(256) void k_resid_reduce(const double* __restrict__
                                                       const double* __restrict__ B, int64_t ldb,
                                                       double* __restrict__ R, double* __restrict__ Rlo, int64_t ldr,
                                                       double* __restrict__ T, int64_t ldt, int matvec,
-                                                      double* __restrict__ Sp, int c_base) {
+                                                      double* __restrict__ Sp, int c_base, const int* __restrict__ act) {
+  if (act && *act == 0) return;
   constexpr int F = DD ? 3 : 2;
   const int c = blockIdx.y;
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -266,7 +268,7 @@ static void launch_group_into(const ResidArgs& a, Workspace& W, int c0, double* 
   const TA* A = (const TA*)a.A;
   hipEvent_t ev = prof ? W.prof_begin() : nullptr;
   hipLaunchKernelGGL((k_resid_partial<TA, NR, DD>), dim3(rb, nch), dim3(256), 0, W.stream,
-                     A, a.n, a.lda, a.X + (int64_t)c0 * a.ldx, a.ldx, cpc, part);
+                     A, a.n, a.lda, a.X + (int64_t)c0 * a.ldx, a.ldx, cpc, part, a.active);
   LAUNCH_CK();
   // algorithmic bytes: A once, X once, the chunk partials written (the reduce pass reads them back)
   if (prof)
@@ -277,7 +279,7 @@ static void launch_group_into(const ResidArgs& a, Workspace& W, int c0, double* 
                      a.B ? a.B + (int64_t)c0 * a.ldb : nullptr, a.ldb,
                      a.R + (int64_t)c0 * a.ldr, a.Rlo ? a.Rlo + (int64_t)c0 * a.ldr : nullptr, a.ldr,
                      a.T ? a.T + (int64_t)c0 * a.ldt : nullptr, a.ldt, a.matvec ? 1 : 0,
-                     statp, c0);
+                     statp, c0, a.active);
   LAUNCH_CK();
 }
 

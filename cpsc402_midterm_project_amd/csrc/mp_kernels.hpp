@@ -101,6 +101,7 @@ struct ResidArgs {
   const double* X = nullptr; int64_t ldx = 0; const double* B = nullptr; int64_t ldb = 0;
   double* R = nullptr; double* Rlo = nullptr; int64_t ldr = 0; double* T = nullptr; int64_t ldt = 0;
   double* stats = nullptr; int32_t* worst = nullptr; bool dd = false; bool matvec = false;
+  const int* active = nullptr;   // device word: the kernels return at once when it is 0 (nullptr: always run)
 };
 void residual(const ResidArgs& a, Workspace& W);
 // Allocation-free residual: partials in a fixed device array of RESID_SCRATCH_PART + RESID_SCRATCH_STAT doubles
@@ -137,8 +138,10 @@ void pack_operand(int fmt, bool tile_scale, const float* S, int64_t sI, int64_t 
 
 // d = (LU)^{-1} P r  with FP32 factors and FP32 arithmetic, FP64 input/output; d is scaled by `scale`.
 // reuse_inverse: the diagonal-block inverses computed for the same LU buffer since the last getrf are reused.
+// active: device word; the substitutions return at once when it is 0 (nullptr: always run).
 void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* perm,
-           const double* R, int64_t ldr, double* D, int64_t ldd, double scale, Workspace& W, bool reuse_inverse = false);
+           const double* R, int64_t ldr, double* D, int64_t ldd, double scale, Workspace& W, bool reuse_inverse = false,
+           const int* active = nullptr);
 
 // Many independent systems (n <= 512, nrhs <= 16) in one persistent launch: FP32 LU (nb = 32, FP32 trailing
 // update) and stationary refinement with FP64 residuals until eta <= tau (and omega <= tau unless eta_only).

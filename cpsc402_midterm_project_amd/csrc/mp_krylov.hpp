@@ -8,6 +8,11 @@ struct Krylov {
   Workspace* W = nullptr;
   int64_t n = 0; int m = 0;
   DBuf<double> V, Z, w, sR, rcur, H, small, part;
+  hipEvent_t ev[2] = {nullptr, nullptr};             // step-output events of the steps run ahead of the host
+  Krylov() = default;
+  Krylov(const Krylov&) = delete;
+  Krylov& operator=(const Krylov&) = delete;
+  ~Krylov() { for (hipEvent_t& e : ev) if (e) (void)hipEventDestroy(e); }
   // small layout: cs[m+1] | sn[m+1] | g[m+1] | h[m+1] (projections) | y[m+1] | scalars...
   double* cs() { return small.p; }
   double* g() { return small.p + 2 * (m + 1); }
@@ -15,13 +20,19 @@ struct Krylov {
   double* h2() { return small.p + 4 * (m + 1); }
   double* y() { return small.p + 5 * (m + 1); }
   double* scal() { return small.p + 6 * (m + 1); }   // 8 scalars: [0] hn, [1] norm tmp, [2..4] givens out
+  // activity word of the Arnoldi steps (set at a cycle's start, cleared by the step that ends the cycle)
+  int* active() { return (int*)(small.p + 6 * (m + 1) + 8); }
   void ensure(int64_t n_, int m_);
-  void dots(const double* Vb, int ncols, const double* x, double* out, bool accumulate);
-  void update(const double* Vb, int ncols, const double* h, double* x, double sign);
-  void norm(const double* x, double* out);
-  void scale_copy(const double* x, double* y, const double* s, bool inv);
-  void givens(int j, const double* h, const double* hn, double beta0, double* out);
-  void init_cycle(double beta);
+  // act: kernels return at once when *act == 0 (nullptr: always run)
+  void dots(const double* Vb, int ncols, const double* x, double* out, bool accumulate, const int* act = nullptr);
+  void update(const double* Vb, int ncols, const double* h, double* x, double sign, const int* act = nullptr);
+  void norm(const double* x, double* out, const int* act = nullptr);
+  void scale_copy(const double* x, double* y, const double* s, bool inv, const int* act = nullptr);
+  void givens(int j, const double* h, const double* hn, double beta0, double* out, double tol = -1.0, int* act = nullptr);
+  void init_cycle(double beta, int* act = nullptr);
+  // One Arnoldi step after w = A z_j: CGS2 against V[:, 0..j] (h, h2, w), hn = ||w|| in scal()[0] and the Givens
+  // update with its outputs in scal()[2..4], in 5 launches; the same sums as dots / update / norm / givens.  j < 256.
+  void arnoldi_tail(int j, double beta0, double tol, int* act);
   void lsq(int k, double* y);
 };
 

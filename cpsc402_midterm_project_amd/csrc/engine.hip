@@ -245,6 +245,7 @@ struct Factors {
   double gscale = 1.0; int gscale_exp = 0; int scale_mode = 0; int nb = 0;
   int info = 0; double growth = -1; unsigned long long conv_overflow = 0, conv_subnormal = 0; int panels = 0;
   int coop_refused = 0;   // panels launched as ordinary kernels after the runtime refused a cooperative launch
+  std::string coop_error; // the runtime's code for the first refusal
   bool valid = false;
   DBuf<double> LU64p; bool promoted = false;   // FP64 copy of FP32 factors, applied by FP64 getrs (apply64)
   DBuf<double> dr, dc;                          // two-sided power-of-two scales (scale mode 2): F = g Dr A Dc
@@ -362,7 +363,7 @@ static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, i
   const int64_t n = P.n; const size_t nn = (size_t)n * n;
   W.factor_gen++;   // cached getrs block inverses of any earlier factors are stale from here on
   F.n = n; F.kind = kind; F.scale_mode = scale_mode; F.nb = nb; F.valid = false; F.growth = -1; F.promoted = false;
-  F.conv_overflow = F.conv_subnormal = 0; F.info = 0; F.panels = 0; F.coop_refused = 0;
+  F.conv_overflow = F.conv_subnormal = 0; F.info = 0; F.panels = 0; F.coop_refused = 0; F.coop_error.clear();
   F.prec = (kind == "cs64" || kind == "mx64") ? 64 : 32;
   F.ipiv.ensure(C->tracker, n, "ipiv");
   F.perm.ensure(C->tracker, n, "perm");
@@ -441,7 +442,7 @@ static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, i
     // a panel whose cross-workgroup exchange hit its spin limit left the factors incomplete
     if (gi.timeout) throw MPError("factorization: the panel exchange timed out (workgroups not co-resident)", MP_ERR_INTERNAL);
     F.info = gi.info; F.conv_overflow = gi.overflow; F.conv_subnormal = gi.subnormal; F.panels = gi.panels;
-    F.coop_refused = gi.coop_refused;
+    F.coop_refused = gi.coop_refused; F.coop_error = gi.coop_error;
   }
   const double gm = (F.prec == 32) ? dev_absmax(C, F.LU32.p, true, n, n) : dev_absmax(C, F.LU64.p, false, n, n);
   F.growth = (amax_f > 0) ? gm / amax_f : -1;
@@ -577,6 +578,7 @@ struct TrialOut {
   std::vector<HistRec> hist; std::vector<InnerRec> ihist;
   double growth = -1; unsigned long long conv_overflow = 0, conv_subnormal = 0; int factor_info = 0;
   int panels = 0, panel_coop_refused = 0;   // panel launches of the own factorizations, and how many ran non-cooperatively
+  std::string panel_coop_error;             // the runtime's code for the first refused cooperative launch
   std::string error; bool ok = true; double final_eta = 0, final_omega = 0;
 };
 
@@ -610,6 +612,7 @@ struct Engine {
     T.t_factor_total += dt; T.stages.push_back(kind); T.stage_times.push_back(dt); T.refactorizations++;
     T.growth = F.growth; T.conv_overflow = F.conv_overflow; T.conv_subnormal = F.conv_subnormal; T.factor_info = F.info;
     T.panels += F.panels; T.panel_coop_refused += F.coop_refused;
+    if (T.panel_coop_error.empty()) T.panel_coop_error = F.coop_error;
     return F;
   }
 
@@ -1796,7 +1799,7 @@ int mp_solve(mp_ctx* C, mp_problem* P, const char* method_id, const char* params
      .i("gmres_cycles", M.gmres_cycles).i("max_outer", M.max_outer).s("policy", M.policy).i("budget", M.budget).i("scale_mode", M.scale_mode)
      .b("stage_abandon", (M.policy == "ladder" || M.policy == "ladder16" || M.policy == "ladder16x3") && a.getb("stage_abandon", true))
      .i("working", M.working).i("resid", M.resid == 128 ? 64 : M.resid).b("resid_dd", M.resid == 128).b("apply64", M.apply64)
-     .b("reuse", a.getb("reuse", false)).s("panel", "hip_lds_exchange").i("panel_launches", T.panels).i("panel_coop_refused", T.panel_coop_refused).s("panel_requested", a.get("panel_requested", "cusolver")).i("stages_count", (int64_t)T.stages.size())
+     .b("reuse", a.getb("reuse", false)).s("panel", "hip_lds_exchange").i("panel_launches", T.panels).i("panel_coop_refused", T.panel_coop_refused).s("panel_coop_error", T.panel_coop_error).s("panel_requested", a.get("panel_requested", "cusolver")).i("stages_count", (int64_t)T.stages.size())
      .s("stop_on", eta_only ? "eta" : "eta_omega");
     C->last_record = j.obj();
     std::string h;

@@ -2016,6 +2016,19 @@ void fill_iota(int32_t* p, int64_t n, Workspace& W) {
 
 constexpr int PLAN_WORDS = 1 + 4 * 256 + 8;
 
+// A refused cooperative launch of a valid panel grid (its size was checked against the CUs) means the runtime
+// cannot promise co-residency now (another process holds the device, or its resources are taken): the panel then
+// takes an ordinary launch with bounded waits.  Any other failure is an error of the call and is raised.  The
+// first refusal's code is kept for the trial record.
+static bool coop_refusal(hipError_t e, Workspace& W) {
+  (void)hipGetLastError();
+  const bool refusal = e == hipErrorCooperativeLaunchTooLarge || e == hipErrorLaunchOutOfResources ||
+                       e == hipErrorNotSupported || e == hipErrorOutOfMemory || e == hipErrorNotReady;
+  if (!refusal) throw MPError(std::string("getrf: cooperative panel launch failed: ") + hipGetErrorName(e), -5);
+  if (W.coop_refused++ == 0) W.coop_error = hipGetErrorName(e);
+  return true;
+}
+
 static bool panel2_usable(const GetrfArgs& a, const Workspace& W, int64_t k, int kb, bool allow_p2) {
   const int64_t G2 = cdiv(a.n - k, P2_R);
   return allow_p2 && a.n < 65536 && G2 <= std::min<int64_t>(P2_MAXG, W.num_cus) &&
@@ -2041,9 +2054,10 @@ static void run_panel(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int32
     // the panel with the timeout flag (reported as an error), never with a hang.
     bool launched = false;
     if (cooperative) {
-      launched = hipLaunchCooperativeKernel((const void*)k_panel2, dim3((unsigned)G2), dim3(256), args,
-                                            (unsigned)panel2_lds_bytes(kb), stream) == hipSuccess;
-      if (!launched) { (void)hipGetLastError(); W.coop_refused++; }
+      const hipError_t e = hipLaunchCooperativeKernel((const void*)k_panel2, dim3((unsigned)G2), dim3(256), args,
+                                                      (unsigned)panel2_lds_bytes(kb), stream);
+      launched = e == hipSuccess;
+      if (!launched) coop_refusal(e, W);
     }
     if (!launched) {
       // overlapped with trailing-update kernels that never wait on anything: workgroups that are not yet
@@ -2071,9 +2085,9 @@ static void run_panel(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int32
   void* args[] = {&pa};
   const size_t lds = (size_t)R * kb * 4 + (size_t)R * 4;
   hipEvent_t e0 = W.prof_begin(stream);
-  if (hipLaunchCooperativeKernel((const void*)k_panel, dim3(G), dim3(256), args, (unsigned)lds, stream) != hipSuccess) {
-    (void)hipGetLastError();      // refused while another process shares the device: bounded waits, as above
-    W.coop_refused++;
+  const hipError_t ce = hipLaunchCooperativeKernel((const void*)k_panel, dim3(G), dim3(256), args, (unsigned)lds, stream);
+  if (ce != hipSuccess) {
+    coop_refusal(ce, W);          // refused: an ordinary launch with bounded waits, as above
     HIP_CK(hipLaunchKernel((const void*)k_panel, dim3(G), dim3(256), args, lds, stream));
   }
   W.prof_end(e0, "panel", 8.0 * (double)m * kb, (double)m * kb * kb, stream);
@@ -2123,6 +2137,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   const int nb = a.nb;
   W.factor_gen++;
   W.coop_refused = 0;
+  W.coop_error.clear();
   W.dinfo.ensure(*W.tracker, 4, "getrf_info");
   W.cnt.ensure(*W.tracker, 2, "getrf_counts");
   W.affect.ensure(*W.tracker, 2 * PLAN_WORDS, "swap_plan");
@@ -2236,6 +2251,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   if (out) {
     out->info = hinfo[0]; out->timeout = hinfo[1]; out->overflow = hc[0]; out->subnormal = hc[1]; out->panels = panels;
     out->coop_refused = W.coop_refused;
+    out->coop_error = W.coop_error;
   }
   if (hinfo[1]) throw MPError("getrf: panel exchange timed out (workgroups not co-resident?)", -5);
 }

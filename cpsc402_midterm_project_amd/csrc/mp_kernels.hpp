@@ -45,6 +45,7 @@ struct Workspace {
   DevTracker* tracker = nullptr;
   int num_cus = 256;
   int coop_refused = 0;   // cooperative panel launches refused by the runtime in the current factorization
+  std::string coop_error;  // the runtime's code for the first refusal
   DBuf<double> part;     // residual / absmax partial sums
   DBuf<double> statp;    // residual metric partials
   // getrf
@@ -131,7 +132,8 @@ struct GetrfArgs {
 };
 // coop_refused: panels whose cooperative launch the runtime refused (another process on the device), which then
 // ran as ordinary launches with bounded exchange waits
-struct GetrfInfo { int info = 0; int timeout = 0; unsigned long long overflow = 0, subnormal = 0; int panels = 0; int coop_refused = 0; };
+struct GetrfInfo { int info = 0; int timeout = 0; unsigned long long overflow = 0, subnormal = 0; int panels = 0; int coop_refused = 0;
+                   std::string coop_error; };
 void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out);
 void pack_operand(int fmt, bool tile_scale, const float* S, int64_t sI, int64_t sK, int nidx, int K, void* out,
                   void* out_lo, int32_t* exps, unsigned long long* counts, Workspace& W);

@@ -78,6 +78,13 @@ def test_cli_run_campaign(tmp_path, ctx):
                 assert rec["method_id"] == methods[(mi + rot) % len(methods)]
                 assert rec["ok"], (rec["method_id"], rec["mode"], rec["rep"], rec["terminal_reason"], rec["error"])
                 assert rec["case_id"] == "c1" and rec["tag"] == "t" and rec["nb"] == 64
+                if rec["method_id"] == "mx16_ir_w64_r64":
+                    # this process's context shares the device: every panel ran, and a refused cooperative launch
+                    # is recorded with the runtime's code
+                    assert rec["panel_launches"] >= 1
+                    assert (rec["panel_coop_refused"] == 0) == (rec["panel_coop_error"] == ""), rec["panel_coop_error"]
+                    print(f"[cli run] panels {rec['panel_launches']}, refused {rec['panel_coop_refused']} "
+                          f"({rec['panel_coop_error'] or 'none'})", flush=True)
                 assert rec["eta_max"] <= 100 * 2.0 ** -53, rec["eta_max"]
                 if mode == "host":
                     assert rec["host_e2e_s"] > rec["device_ready_wall_s"] > 0

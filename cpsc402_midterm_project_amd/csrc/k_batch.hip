@@ -678,9 +678,12 @@ void batch_solve(int count, const int32_t* n, const int32_t* nrhs, const double*
 // sizes at which a system's update splits its rows over 2 or 4 thread groups.  The panel width (32) and the
 // workgroup (4 waves) are fixed by the kernel's structure: the 32-wide inverted diagonal block and U12 column
 // live in registers, and the panel's argmax is a 4-wave exchange.
-// A sample of the batch (up to 96 systems per bucket, spread over its sizes) is solved once per candidate;
-// per bucket the fastest split wins, then the fastest occupancy for the whole sample.  The choice is cached
-// per batch signature (the systems per bucket on a log2 scale and the largest right-hand-side count).
+// Per n bucket, a sample of the batch (at least twice the CU count of systems, spread over the bucket's sizes,
+// so the device is as full as in the real launch) is solved once per split candidate and the fastest split wins.
+// Occupancy (resident workgroups per CU) is timed on the whole batch: a sample too small to fill the device
+// cannot tell one workgroup per CU from two.  Last, the tuned shape is kept only if the whole batch runs
+// faster with it than with the default shape.  The choice is cached per batch signature (the systems per
+// bucket on a log2 scale and the largest right-hand-side count).
 static std::string batch_key(int count, const int32_t* n, const int32_t* nrhs) {
   int cnt[4] = {0, 0, 0, 0}, rmax = 0;
   for (int i = 0; i < count; i++) { cnt[std::min(3, (n[i] - 1) / 128)]++; rmax = std::max(rmax, (int)nrhs[i]); }
@@ -727,7 +730,7 @@ void batch_tune(int count, const int32_t* n, const int32_t* nrhs, const double* 
     std::vector<int> ids = by[b];
     std::sort(ids.begin(), ids.end(), [&](int x, int y) { return n[x] < n[y]; });
     std::vector<int> smp;
-    const size_t want = std::min<size_t>(96, ids.size());
+    const size_t want = std::min<size_t>(std::max<size_t>(96, (size_t)2 * W.num_cus), ids.size());
     for (size_t s = 0; s < want; s++) smp.push_back(ids[s * ids.size() / want]);
     all.insert(all.end(), smp.begin(), smp.end());
     BatchParams q = p; q.wg_per_cu = 2;
@@ -744,18 +747,35 @@ void batch_tune(int count, const int32_t* n, const int32_t* nrhs, const double* 
     js << "}";
     p.split4[b] = cand[bi][0]; p.split2[b] = cand[bi][1];
   }
+  std::vector<int> every(count);
+  for (int i = 0; i < count; i++) every[i] = i;
   js << "},\"wg_per_cu\":{";
+  double tuned_s = -1;
   if (tune_wg) {
     double best = 1e300; int bw = p.wg_per_cu;
     for (int w = 1; w <= 3; w++) {
       BatchParams q = p; q.wg_per_cu = w;
-      const double s = run(all, q);
+      const double s = run(every, q);
       js << (w > 1 ? "," : "") << "\"" << w << "\":" << s * 1e3;
       if (s < best) { best = s; bw = w; }
     }
     p.wg_per_cu = bw;
+    tuned_s = best;
+  } else {
+    tuned_s = run(every, p);
   }
-  js << "},\"sample_systems\":" << all.size() << ",\"tune_s\":" << wall_now() - t0 << "}";
+  // the default shape on the whole batch: the tuned one must beat it
+  BatchParams d0;
+  d0.tau = p.tau; d0.max_outer = p.max_outer; d0.eta_only = p.eta_only;
+  if (!tune_wg) d0.wg_per_cu = p.wg_per_cu;
+  const double default_s = run(every, d0);
+  const bool keep = tuned_s < default_s;
+  if (!keep) {
+    for (int b = 0; b < 4; b++) { p.split4[b] = d0.split4[b]; p.split2[b] = d0.split2[b]; }
+    if (tune_wg) p.wg_per_cu = d0.wg_per_cu;
+  }
+  js << "},\"tuned_ms\":" << tuned_s * 1e3 << ",\"default_ms\":" << default_s * 1e3 << ",\"kept\":\""
+     << (keep ? "tuned" : "default") << "\",\"sample_systems\":" << all.size() << ",\"tune_s\":" << wall_now() - t0 << "}";
   detail = js.str();
   W.batch_tuned[key] = p;
 }

@@ -340,6 +340,332 @@ __global__ __launch_bounds__(512) void k_trsv2(const float* __restrict__ LU, int
   if (t == 0 && s_abort) err[0] = 1;
 }
 
+// ------------------------------------------------------------------ 256-row blocks, four workgroups per block row
+// The chain of the substitution has one step per block row and each step costs a cross-CU exchange under the
+// load of the streaming workgroups (about 1.8 us), so 256-row blocks halve the chain of 128-row blocks.  A
+// block row's 256 rows are split over four workgroups of 64 rows, so each still streams one 64-row x 256-column
+// slice (64 KB) of a previous block per chain step.  T_i^{-1} of the 256 x 256 diagonal block mixes the four
+// slices: after the bulk sums, each workgroup publishes its 64-row slice of s_i = r_i - sum_{j < i-1} L_ij x_j
+// (tagged words), reads the slices its rows of T_i^{-1} need (lower: the slices above, upper: below) and
+// forms c_i; the chain step is x_i = c_i - M_i x_{i-1} with M_i = T_i^{-1} L_{i,i-1} (upper: U_{i,i+1}).
+// Workgroup b of the launch: block row b / 4 in production order, slice 0..3 in dependency order (upper:
+// from the bottom), so no workgroup waits for a higher-numbered one.
+constexpr int GB3 = 256;           // rows per block
+constexpr int SR3 = 64;            // rows per workgroup
+
+// acc (thread (tr, tc)'s 8 x 8 tile of a 128 x 128 product) += As * Bs, both staged in LDS with row stride 129:
+// As[row * 129 + k], Bs[k * 129 + col]; each element summed over k in increasing order.
+__device__ __forceinline__ void mm128_lds(const float* As, const float* Bs, int tr, int tc, float (&acc)[8][8]) {
+  for (int k = 0; k < GB; k++) {
+    float av[8], bv[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) av[q] = As[(tr * 8 + q) * (GB + 1) + k];
+#pragma unroll
+    for (int p = 0; p < 8; p++) bv[p] = Bs[k * (GB + 1) + tc * 8 + p];
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+#pragma unroll
+      for (int p = 0; p < 8; p++) acc[q][p] = fmaf(av[q], bv[p], acc[q][p]);
+  }
+}
+// Stage a 128 x 128 block into LDS (row stride 129) from a loader f(row, col) -> value: thread (row, column
+// parity) keeps 16 loads in flight.
+template <class F>
+__device__ __forceinline__ void stage128(float* S, int t, F&& f) {
+  const int row = t & (GB - 1), ch = t >> 7;
+  for (int cb = 0; cb < GB; cb += 32) {
+    float v[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = f(row, cb + 2 * e + ch);
+#pragma unroll
+    for (int e = 0; e < 16; e++) S[row * (GB + 1) + cb + 2 * e + ch] = v[e];
+  }
+}
+
+// T_i^{-1} of 256-row block i (column-major 256 x 256 per block) from the 128-row inverses T128 (k_trinv) and the
+// off-diagonal 128 x 128 part B of the block: lower [A 0; B C]^{-1} = [A^-1 0; -C^-1 B A^-1  C^-1],
+// upper [A B; 0 C]^{-1} = [A^-1  -A^-1 B C^-1; 0 C^-1].  A half block past n is the identity (B = 0).
+template <bool LOWER>
+__global__ __launch_bounds__(256) void k_tinv256(const float* __restrict__ LU, int64_t lda, int64_t n,
+                                                 const float* __restrict__ T128, float* __restrict__ T256) {
+  extern __shared__ float sm[];
+  float* S0 = sm;                     // [128][129]
+  float* S1 = sm + GB * (GB + 1);     // [128][129]
+  const int i = blockIdx.x, t = threadIdx.x;
+  const int nbk128 = (int)((n + GB - 1) / GB);
+  const int b0 = 2 * i, b1 = 2 * i + 1;
+  const bool has1 = b1 < nbk128;
+  float* out = T256 + (size_t)i * GB3 * GB3;
+  // the two diagonal 128-blocks and the zero block
+  for (int idx = t; idx < GB * GB; idx += 256) {
+    const int col = idx >> 7, row = idx & (GB - 1);
+    const float a = T128[(size_t)b0 * GB * GB + idx];
+    const float c = has1 ? T128[(size_t)b1 * GB * GB + idx] : (row == col ? 1.0f : 0.0f);
+    out[(size_t)col * GB3 + row] = a;
+    out[(size_t)(col + GB) * GB3 + row + GB] = c;
+    if (LOWER) out[(size_t)(col + GB) * GB3 + row] = 0.0f; else out[(size_t)col * GB3 + row + GB] = 0.0f;
+  }
+  const int tr = t >> 4, tc = t & 15;
+  float acc[8][8];
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+#pragma unroll
+    for (int p = 0; p < 8; p++) acc[q][p] = 0.0f;
+  if (has1) {
+    const int64_t o = (int64_t)i * GB3;
+    // B: lower rows o+128.., cols o..; upper rows o.., cols o+128..  (entries past n are zero)
+    auto ldB = [&](int row, int col) -> float {
+      const int64_t gr = LOWER ? o + GB + row : o + row, gc = LOWER ? o + col : o + GB + col;
+      const float v = LU[min(gc, n - 1) * lda + min(gr, n - 1)];
+      return (gr < n && gc < n) ? v : 0.0f;
+    };
+    auto ldT = [&](int b) { return [=](int row, int col) -> float { return T128[(size_t)b * GB * GB + (size_t)col * GB + row]; }; };
+    // first product P = B A^-1 (lower) or A^-1 B (upper)
+    if (LOWER) { stage128(S0, t, ldB); stage128(S1, t, ldT(b0)); }
+    else { stage128(S0, t, ldT(b0)); stage128(S1, t, ldB); }
+    __syncthreads();
+    mm128_lds(S0, S1, tr, tc, acc);
+    __syncthreads();
+    // P into S1 (lower: it is the right operand of C^-1 P; upper: the left operand of P C^-1)
+    float* P = LOWER ? S1 : S0;
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+#pragma unroll
+      for (int p = 0; p < 8; p++) {
+        // lower: S1 holds P as Bs[k * 129 + col]; upper: S0 holds P as As[row * 129 + k]
+        P[(tr * 8 + q) * (GB + 1) + tc * 8 + p] = acc[q][p];
+        acc[q][p] = 0.0f;
+      }
+    if (LOWER) stage128(S0, t, ldT(b1)); else stage128(S1, t, ldT(b1));
+    __syncthreads();
+    mm128_lds(S0, S1, tr, tc, acc);
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+#pragma unroll
+    for (int p = 0; p < 8; p++) {
+      const int row = tr * 8 + q, col = tc * 8 + p;
+      const float v = has1 ? -acc[q][p] : 0.0f;
+      if (LOWER) out[(size_t)col * GB3 + row + GB] = v;
+      else out[(size_t)(col + GB) * GB3 + row] = v;
+    }
+}
+
+// M_i = T_i^{-1} B_i for 256-row blocks (lower: B_i = L_{i,i-1}, upper: U_{i,i+1}; zero for the first block in
+// production order).  One workgroup per 128 x 128 output quarter (sub-row sr, sub-column sc): the sum over the
+// two inner quarters in increasing order.  Rows or columns past n are zero.
+template <bool LOWER>
+__global__ __launch_bounds__(256) void k_chain256(const float* __restrict__ LU, int64_t lda, int64_t n,
+                                                  const float* __restrict__ T256, float* __restrict__ M256) {
+  extern __shared__ float sm[];
+  float* S0 = sm;
+  float* S1 = sm + GB * (GB + 1);
+  const int nbk = (int)((n + GB3 - 1) / GB3);
+  const int i = blockIdx.x >> 2, sr = (blockIdx.x >> 1) & 1, sc = blockIdx.x & 1;
+  const int t = threadIdx.x, tr = t >> 4, tc = t & 15;
+  const int j = LOWER ? i - 1 : i + 1;
+  float* out = M256 + (size_t)i * GB3 * GB3;
+  float acc[8][8];
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+#pragma unroll
+    for (int p = 0; p < 8; p++) acc[q][p] = 0.0f;
+  if (j >= 0 && j < nbk) {
+    const int64_t ro = (int64_t)i * GB3, co = (int64_t)j * GB3;
+    for (int kq = 0; kq < 2; kq++) {
+      // T_i^{-1} quarter (sr, kq) is zero above (lower) / below (upper) the diagonal quarters
+      if (LOWER ? (kq > sr) : (kq < sr)) continue;
+      stage128(S0, t, [&](int row, int col) -> float {
+        return T256[(size_t)i * GB3 * GB3 + (size_t)(kq * GB + col) * GB3 + sr * GB + row];
+      });
+      stage128(S1, t, [&](int row, int col) -> float {
+        const int64_t gr = ro + kq * GB + row, gc = co + sc * GB + col;
+        const float v = LU[min(gc, n - 1) * lda + min(gr, n - 1)];
+        return (gr < n && gc < n) ? v : 0.0f;
+      });
+      __syncthreads();
+      mm128_lds(S0, S1, tr, tc, acc);
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++)
+#pragma unroll
+    for (int p = 0; p < 8; p++) {
+      const int row = sr * GB + tr * 8 + q, col = sc * GB + tc * 8 + p;
+      const bool valid = (int64_t)i * GB3 + row < n;
+      out[(size_t)col * GB3 + row] = valid ? acc[q][p] : 0.0f;
+    }
+}
+
+template <int NR, bool LOWER>
+__global__ __launch_bounds__(512) void k_trsv3(const float* __restrict__ LU, int64_t lda, int64_t n,
+                                               const float* __restrict__ Tinv, const float* __restrict__ Mch,
+                                               const int32_t* __restrict__ perm,
+                                               const double* __restrict__ Rin, int64_t ldr,
+                                               float* __restrict__ Y, int64_t ldy,
+                                               double* __restrict__ D, int64_t ldd, double scale,
+                                               uint64_t* __restrict__ ybuf, uint32_t tag, uint32_t wtag, int32_t* __restrict__ err, long spin,
+                                               const int* __restrict__ act) {
+  if (act && *act == 0) return;
+  __shared__ float ys[3][NR][GB3];
+  __shared__ float part[8][NR][SR3];
+  __shared__ float sv[NR][GB3];
+  __shared__ int s_abort;
+  const int nbk = (int)((n + GB3 - 1) / GB3);
+  const int t = threadIdx.x, r = t & (SR3 - 1), qq = t >> 6;
+  const int pi = (int)blockIdx.x >> 2, wp = (int)blockIdx.x & 3;
+  const int bi = LOWER ? pi : (nbk - 1 - pi);
+  const int w = LOWER ? wp : 3 - wp;
+  const int64_t row = (int64_t)bi * GB3 + SR3 * w + r;
+  const bool rvalid = row < n;
+  if (t == 0) s_abort = 0;
+  const int ndep = LOWER ? bi : (nbk - 1 - bi);
+  const int nbulk = ndep > 0 ? ndep - 1 : 0;
+  auto blk = [&](int s) { return LOWER ? s : (nbk - 1 - s); };
+  uint64_t* xw = ybuf;                                             // x words: [(block * NR + c) * 256 + row in block]
+  uint64_t* sw = ybuf + (int64_t)nbk * NR * GB3;                   // s slices, same layout
+  const int64_t rowc = rvalid ? row : n - 1;
+  auto ldL = [&](int bj, float (&v)[32]) {
+    const float* base = LU + rowc;
+#pragma unroll
+    for (int k = 0; k < 32; k++) {
+      const int64_t col = (int64_t)bj * GB3 + qq * 32 + k;
+      v[k] = GS_LOAD(base + (col < n ? col : n - 1) * lda);
+    }
+  };
+  float Tv[32], Mt[32], rhs[NR];
+#pragma unroll
+  for (int k = 0; k < 32; k++) Tv[k] = Tinv[(size_t)bi * GB3 * GB3 + (size_t)(qq * 32 + k) * GB3 + SR3 * w + r];
+#pragma unroll
+  for (int k = 0; k < 32; k++) Mt[k] = Mch[(size_t)bi * GB3 * GB3 + (size_t)(qq * 32 + k) * GB3 + SR3 * w + r];
+#pragma unroll
+  for (int c = 0; c < NR; c++) {
+    rhs[c] = 0.0f;
+    if (qq == 0 && rvalid) rhs[c] = LOWER ? (float)Rin[(int64_t)c * ldr + perm[row]] : Y[(int64_t)c * ldy + row];
+  }
+  float acc[NR];
+#pragma unroll
+  for (int c = 0; c < NR; c++) acc[c] = 0.0f;
+  __syncthreads();
+
+  // bulk, as in k_trsv2: dependencies 0 .. nbulk-1 in production order, the next block's loads in flight
+  float La[32], Lb[32];
+  uint64_t ya[NR], yb[NR];
+  auto ldY = [&](int s2, uint64_t (&y)[NR]) {
+    const int bj = blk(s2 < nbulk ? s2 : 0);
+#pragma unroll
+    for (int c = 0; c < NR; c++) y[c] = gs_load(xw + ((int64_t)bj * NR + c) * GB3 + (t & (GB3 - 1)));
+  };
+  auto step = [&](int s2, const float (&L)[32], const uint64_t (&y)[NR]) {
+    const int bj = blk(s2);
+    if (t < GB3) {
+#pragma unroll
+      for (int c = 0; c < NR; c++) ys[s2 % 3][c][t] = gs_wait(xw + ((int64_t)bj * NR + c) * GB3 + t, y[c], wtag, LDS_FLAG(&s_abort), ndep - 1 - s2, spin);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 32; k++)
+#pragma unroll
+      for (int c = 0; c < NR; c++) acc[c] = fmaf(L[k], ys[s2 % 3][c][qq * 32 + k], acc[c]);
+  };
+  if (nbulk > 0) { ldL(blk(0), La); ldY(0, ya); }
+  for (int s = 0; s < nbulk; s += 2) {
+    ldL(blk(s + 1 < nbulk ? s + 1 : 0), Lb); ldY(s + 1, yb);
+    step(s, La, ya);
+    if (s + 1 >= nbulk) break;
+    ldL(blk(s + 2 < nbulk ? s + 2 : 0), La); ldY(s + 2, ya);
+    step(s + 1, Lb, yb);
+  }
+  GS_MARK(0);
+  // this workgroup's slice of s_i = r_i - (bulk sums): published, and kept in sv
+#pragma unroll
+  for (int c = 0; c < NR; c++) part[qq][c][r] = acc[c];
+  __syncthreads();
+  if (qq == 0) {
+#pragma unroll
+    for (int c = 0; c < NR; c++) {
+      const float s = rhs[c] - (((part[0][c][r] + part[1][c][r]) + (part[2][c][r] + part[3][c][r])) +
+                                ((part[4][c][r] + part[5][c][r]) + (part[6][c][r] + part[7][c][r])));
+      sv[c][SR3 * w + r] = s;
+      gs_put(sw + ((int64_t)bi * NR + c) * GB3 + SR3 * w + r, tag, rvalid ? s : 0.0f);
+    }
+  }
+  // the slices this workgroup's rows of T_i^{-1} need from the other three; the rest of sv is zero
+  if (t < GB3) {
+    const int ws = t >> 6;
+    if (ws != w) {
+      const bool need = LOWER ? (ws < w) : (ws > w);
+#pragma unroll
+      for (int c = 0; c < NR; c++) {
+        float v = 0.0f;
+        if (need) {
+          const uint64_t* p = sw + ((int64_t)bi * NR + c) * GB3 + t;
+          v = gs_wait(p, gs_load(p), wtag, LDS_FLAG(&s_abort), 0, spin);
+        }
+        sv[c][t] = v;
+      }
+    }
+  }
+  __syncthreads();
+  float px[NR];
+#pragma unroll
+  for (int c = 0; c < NR; c++) px[c] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 32; k++)
+#pragma unroll
+    for (int c = 0; c < NR; c++) px[c] = fmaf(Tv[k], sv[c][qq * 32 + k], px[c]);
+#pragma unroll
+  for (int c = 0; c < NR; c++) part[qq][c][r] = px[c];    // part's previous reads precede the last barrier
+  __syncthreads();
+  float xv[NR];
+#pragma unroll
+  for (int c = 0; c < NR; c++)
+    xv[c] = ((part[0][c][r] + part[1][c][r]) + (part[2][c][r] + part[3][c][r])) +
+            ((part[4][c][r] + part[5][c][r]) + (part[6][c][r] + part[7][c][r]));
+  // the chain: x = c - M x_prev
+  if (ndep > 0) {
+    const int bj = blk(ndep - 1);
+    const int sc = nbulk % 3;                              // a slot the last bulk step did not read
+    if (t < GB3) {
+#pragma unroll
+      for (int c = 0; c < NR; c++) {
+        const uint64_t* p = xw + ((int64_t)bj * NR + c) * GB3 + t;
+        ys[sc][c][t] = gs_wait(p, gs_load(p), wtag, LDS_FLAG(&s_abort), 0, spin);
+      }
+    }
+    __syncthreads();                                       // also orders the reads of part above before the writes below
+    GS_MARK(1);
+    float d[NR];
+#pragma unroll
+    for (int c = 0; c < NR; c++) d[c] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 32; k++)
+#pragma unroll
+      for (int c = 0; c < NR; c++) d[c] = fmaf(Mt[k], ys[sc][c][qq * 32 + k], d[c]);
+#pragma unroll
+    for (int c = 0; c < NR; c++) part[qq][c][r] = d[c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NR; c++)
+      xv[c] -= ((part[0][c][r] + part[1][c][r]) + (part[2][c][r] + part[3][c][r])) +
+               ((part[4][c][r] + part[5][c][r]) + (part[6][c][r] + part[7][c][r]));
+  }
+  if (qq == 0) {
+#pragma unroll
+    for (int c = 0; c < NR; c++) {
+      const float x = xv[c];
+      gs_put(xw + ((int64_t)bi * NR + c) * GB3 + SR3 * w + r, tag, rvalid ? x : 0.0f);
+      if (rvalid) {
+        if (LOWER) Y[(int64_t)c * ldy + row] = x;
+        else D[(int64_t)c * ldd + row] = scale * (double)x;
+      }
+    }
+  }
+  GS_MARK(2);
+  if (t == 0 && s_abort) err[0] = 1;
+}
+
 template <int NR>
 static void trsv_pair(int64_t n, const float* LU, int64_t lda, const int32_t* perm, const double* R, int64_t ldr,
                       double* D, int64_t ldd, double scale, Workspace& W, int32_t* err, const int* act) {
@@ -362,18 +688,48 @@ static void trsv_pair(int64_t n, const float* LU, int64_t lda, const int32_t* pe
   W.prof_end(ev, "getrs", 4.0 * (double)n * n + 24.0 * n * NR, 2.0 * (double)n * n * NR);
 }
 
+template <int NR>
+static void trsv_pair3(int64_t n, const float* LU, int64_t lda, const int32_t* perm, const double* R, int64_t ldr,
+                       double* D, int64_t ldd, double scale, Workspace& W, int32_t* err, const int* act) {
+  const int nbk = (int)cdiv(n, GB3);
+  const uint32_t tl = W.take_tags(1), tu = W.take_tags(1);
+  const float* Ti = W.tinv3.p;
+  const float* Tu = W.tinv3.p + (size_t)nbk * GB3 * GB3;
+  const float* Mi = W.tchain3.p;
+  const float* Mu = W.tchain3.p + (size_t)nbk * GB3 * GB3;
+  const long spin = W.gs_force ? 1L : GS_SPIN_MAX;
+  const uint32_t wl = W.gs_force ? 0xFFFFFFFFu : tl, wu = W.gs_force ? 0xFFFFFFFFu : tu;
+  hipEvent_t ev = W.prof_begin();
+  hipLaunchKernelGGL((k_trsv3<NR, true>), dim3(4 * nbk), dim3(512), 0, W.stream, LU, lda, n, Ti, Mi, perm, R, ldr,
+                     W.rhs32.p, n, (double*)nullptr, (int64_t)0, 1.0, W.ybuf.p, tl, wl, err, spin, act);
+  LAUNCH_CK();
+  hipLaunchKernelGGL((k_trsv3<NR, false>), dim3(4 * nbk), dim3(512), 0, W.stream, LU, lda, n, Tu, Mu, perm, (const double*)nullptr, (int64_t)0,
+                     W.rhs32.p, n, D, ldd, scale, W.ybuf.p, tu, wu, err, spin, act);
+  LAUNCH_CK();
+  W.prof_end(ev, "getrs", 4.0 * (double)n * n + 24.0 * n * NR, 2.0 * (double)n * n * NR);
+}
+
 void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* perm,
            const double* R, int64_t ldr, double* D, int64_t ldd, double scale, Workspace& W, bool reuse_inverse,
            const int* active) {
   MP_REQUIRE(n > 0 && nrhs > 0, "getrs: n, nrhs > 0");
   MP_REQUIRE(lda >= n && ldr >= n && ldd >= n, "getrs: leading dimensions >= n");
   const int64_t nbk = cdiv(n, GB);
-  if (W.ybuf.ensure(*W.tracker, (size_t)nbk * 2 * GB, "getrs_exchange"))
+  // 256-row blocks (four workgroups per block row) from n = 512 on; MPSOLVE_GETRS=2 keeps 128-row blocks
+  const bool v3 = n >= 2 * GB3 && !(getenv("MPSOLVE_GETRS") && atoi(getenv("MPSOLVE_GETRS")) == 2);
+  const int64_t nbk3 = cdiv(n, GB3);
+  // exchange words: 128-row solve [block][rhs][row]; 256-row solve x and s slices, two of each per block row
+  const size_t yw = std::max<size_t>((size_t)nbk * 2 * GB, (size_t)nbk3 * 2 * 2 * GB3);
+  if (W.ybuf.ensure(*W.tracker, yw, "getrs_exchange"))
     HIP_CK(hipMemsetAsync(W.ybuf.p, 0, W.ybuf.n * 8, W.stream));
+  if (v3) {
+    W.tinv3.ensure(*W.tracker, (size_t)2 * nbk3 * GB3 * GB3, "getrs_block_inverses_256");
+    W.tchain3.ensure(*W.tracker, (size_t)2 * nbk3 * GB3 * GB3, "getrs_chain_matrices_256");
+  }
   W.rhs32.ensure(*W.tracker, (size_t)n * 4, "getrs_y32");
   W.dinfo.ensure(*W.tracker, 4, "getrf_info");
   W.tinv.ensure(*W.tracker, (size_t)2 * nbk * GB * GB, "getrs_block_inverses");
-  W.tchain.ensure(*W.tracker, (size_t)2 * nbk * GB * GB, "getrs_chain_matrices");
+  if (!v3) W.tchain.ensure(*W.tracker, (size_t)2 * nbk * GB * GB, "getrs_chain_matrices");
   // A test forces the timeout path from the host: MPSOLVE_GS_FAIL = k (k > 0: the next k calls after the
   // variable takes a new value; "all": every call) makes the waits of the substitutions look for a tag that is
   // never published, with a bound of one poll.  Every dependent block then gives up (publishing zeros and
@@ -385,7 +741,7 @@ void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* per
   } else {
     W.gs_fail_env.clear();
   }
-  if (!reuse_inverse || W.tinv_lu != LU || W.tinv_n != n || W.tinv_lda != lda || W.tinv_gen != W.factor_gen) {
+  if (!reuse_inverse || W.tinv_lu != LU || W.tinv_n != n || W.tinv_lda != lda || W.tinv_gen != W.factor_gen || W.tinv_v3 != v3) {
     static bool attr = false;
     const size_t lds = (size_t)(GB * (GB + 1) + GB * GB) * 4;
     const size_t lds_cm = (size_t)2 * GB * (GB + 1) * 4;
@@ -394,6 +750,10 @@ void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* per
       HIP_CK(hipFuncSetAttribute((const void*)k_trinv<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       HIP_CK(hipFuncSetAttribute((const void*)k_chainmat<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cm));
       HIP_CK(hipFuncSetAttribute((const void*)k_chainmat<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cm));
+      HIP_CK(hipFuncSetAttribute((const void*)k_tinv256<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cm));
+      HIP_CK(hipFuncSetAttribute((const void*)k_tinv256<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cm));
+      HIP_CK(hipFuncSetAttribute((const void*)k_chain256<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cm));
+      HIP_CK(hipFuncSetAttribute((const void*)k_chain256<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cm));
       attr = true;
     }
     hipEvent_t ev = W.prof_begin();
@@ -401,13 +761,28 @@ void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* per
     LAUNCH_CK();
     hipLaunchKernelGGL(k_trinv<false>, dim3((unsigned)nbk), dim3(GB), lds, W.stream, LU, lda, n, W.tinv.p + (size_t)nbk * GB * GB);
     LAUNCH_CK();
-    hipLaunchKernelGGL(k_chainmat<true>, dim3((unsigned)nbk), dim3(256), lds_cm, W.stream, LU, lda, n, W.tinv.p, W.tchain.p);
-    LAUNCH_CK();
-    hipLaunchKernelGGL(k_chainmat<false>, dim3((unsigned)nbk), dim3(256), lds_cm, W.stream, LU, lda, n,
-                       W.tinv.p + (size_t)nbk * GB * GB, W.tchain.p + (size_t)nbk * GB * GB);
-    LAUNCH_CK();
+    if (v3) {
+      // 256-row inverses from the 128-row ones, then the 256-row chain matrices
+      const float* Tl = W.tinv.p; const float* Tu = W.tinv.p + (size_t)nbk * GB * GB;
+      float* T3l = W.tinv3.p; float* T3u = W.tinv3.p + (size_t)nbk3 * GB3 * GB3;
+      hipLaunchKernelGGL(k_tinv256<true>, dim3((unsigned)nbk3), dim3(256), lds_cm, W.stream, LU, lda, n, Tl, T3l);
+      LAUNCH_CK();
+      hipLaunchKernelGGL(k_tinv256<false>, dim3((unsigned)nbk3), dim3(256), lds_cm, W.stream, LU, lda, n, Tu, T3u);
+      LAUNCH_CK();
+      hipLaunchKernelGGL(k_chain256<true>, dim3((unsigned)(4 * nbk3)), dim3(256), lds_cm, W.stream, LU, lda, n, T3l, W.tchain3.p);
+      LAUNCH_CK();
+      hipLaunchKernelGGL(k_chain256<false>, dim3((unsigned)(4 * nbk3)), dim3(256), lds_cm, W.stream, LU, lda, n, T3u,
+                         W.tchain3.p + (size_t)nbk3 * GB3 * GB3);
+      LAUNCH_CK();
+    } else {
+      hipLaunchKernelGGL(k_chainmat<true>, dim3((unsigned)nbk), dim3(256), lds_cm, W.stream, LU, lda, n, W.tinv.p, W.tchain.p);
+      LAUNCH_CK();
+      hipLaunchKernelGGL(k_chainmat<false>, dim3((unsigned)nbk), dim3(256), lds_cm, W.stream, LU, lda, n,
+                         W.tinv.p + (size_t)nbk * GB * GB, W.tchain.p + (size_t)nbk * GB * GB);
+      LAUNCH_CK();
+    }
     W.prof_end(ev, "getrs_block_inverse", 16.0 * (double)nbk * GB * GB, 2.0 * (double)nbk * GB * GB * GB * (1.0 / 3.0 + 2.0));
-    W.tinv_lu = LU; W.tinv_n = n; W.tinv_lda = lda; W.tinv_gen = W.factor_gen;
+    W.tinv_lu = LU; W.tinv_n = n; W.tinv_lda = lda; W.tinv_gen = W.factor_gen; W.tinv_v3 = v3;
   }
   int c = 0;
   while (c < nrhs) {
@@ -415,8 +790,13 @@ void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* per
     const double* Rc = R + (int64_t)c * ldr;
     double* Dc = D + (int64_t)c * ldd;
     int32_t* err = W.dinfo.p + 2;
-    if (left >= 2) { trsv_pair<2>(n, LU, lda, perm, Rc, ldr, Dc, ldd, scale, W, err, active); c += 2; }
-    else { trsv_pair<1>(n, LU, lda, perm, Rc, ldr, Dc, ldd, scale, W, err, active); c += 1; }
+    if (v3) {
+      if (left >= 2) { trsv_pair3<2>(n, LU, lda, perm, Rc, ldr, Dc, ldd, scale, W, err, active); c += 2; }
+      else { trsv_pair3<1>(n, LU, lda, perm, Rc, ldr, Dc, ldd, scale, W, err, active); c += 1; }
+    } else {
+      if (left >= 2) { trsv_pair<2>(n, LU, lda, perm, Rc, ldr, Dc, ldd, scale, W, err, active); c += 2; }
+      else { trsv_pair<1>(n, LU, lda, perm, Rc, ldr, Dc, ldd, scale, W, err, active); c += 1; }
+    }
   }
 }
 

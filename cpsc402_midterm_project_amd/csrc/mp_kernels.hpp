@@ -65,9 +65,10 @@ struct Workspace {
   DBuf<float> rhs32;
   DBuf<float> tinv;      // inverses of the 128 x 128 diagonal blocks of L and U
   DBuf<float> tchain;    // chain matrices Linv_i L_{i,i-1} and Uinv_i U_{i,i+1} of the blocked substitution
+  DBuf<float> tinv3, tchain3;   // the same for 256-row blocks
   bool gs_force = false;     // forced exchange timeout of the substitutions (tests only)
   std::string gs_fail_env; long gs_fail_calls = 0;
-  const float* tinv_lu = nullptr; int64_t tinv_n = 0, tinv_lda = 0; uint64_t tinv_gen = 0;
+  const float* tinv_lu = nullptr; int64_t tinv_n = 0, tinv_lda = 0; uint64_t tinv_gen = 0; bool tinv_v3 = false;
   // batched small systems
   DBuf<uint8_t> bsys;    // per-system descriptors in processing order
   DBuf<float> bwork;     // per-system factors (row-major origin order + column-major LAPACK order)

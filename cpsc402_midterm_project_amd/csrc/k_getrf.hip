@@ -1063,11 +1063,14 @@ __device__ __forceinline__ void trsm_inv_block(const float* __restrict__ A, int6
 // (one extra "column").  Workgroups stride over the columns.  The first ninv workgroups instead invert the
 // diagonal blocks of L11 for the TRSM (L11's columns are not among the interchanged ones), which saves the
 // TRSM path a dependent launch.  zero_plan (optional): the swap-plan counter of the next panel, cleared here
-// (its previous readers precede this launch on the stream).
+// (its previous readers precede this launch on the stream).  inv_snap (optional, with do_perm): after the
+// permutation vector is updated, the workgroup that updated it records its inverse for the rows at positions
+// k .. n-1 (inv_snap[perm[p]] = p), the map k_lperm uses to put the L columns of this step in final order.
 __global__ __launch_bounds__(256) void k_laswp(float* __restrict__ A, int64_t lda, int64_t k,
                                                int64_t ra0, int64_t ra1, int64_t rb0, int64_t rb1, int do_perm,
                                                const int32_t* __restrict__ plan, int32_t* __restrict__ perm,
-                                               int kb, int ninv, float* __restrict__ Linv, int32_t* __restrict__ zero_plan) {
+                                               int kb, int ninv, float* __restrict__ Linv, int32_t* __restrict__ zero_plan,
+                                               int32_t* __restrict__ inv_snap, int64_t n) {
   if (zero_plan != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *zero_plan = 0;
   if ((int)blockIdx.x < ninv) {
     __shared__ float T[32][33];
@@ -1125,7 +1128,51 @@ __global__ __launch_bounds__(256) void k_laswp(float* __restrict__ A, int64_t ld
     if (with_perm) {
 #pragma unroll
       for (int q = 0; q < 4; q++) if (dst[q] >= 0) pp[dst[q]] = pv[q];
+      if (inv_snap != nullptr) {
+        __syncthreads();   // this workgroup's permutation stores before its reads below
+        for (int64_t i = t; i < n - k; i += 256) inv_snap[pp[i]] = (int32_t)(k + i);
+      }
     }
+  }
+}
+
+// ------------------------------------------------------------------ deferred interchanges of the L columns
+// Step b's panel writes its L columns in the row order after step b (position p holds original row perm_b[p]);
+// the later steps' interchanges, which permute only positions >= (b + 1) * nb, are not applied to them on the
+// way.  Here each column block b < nblk - 1 is put in final order at once: position q >= r0 = (b + 1) * nb
+// takes the value of position inv_b[perm[q]] (inv_b: the inverse of perm_b, recorded by step b's interchange
+// launch), which is the same data movement as the sequence of interchanges.  A workgroup owns `cpw` columns of
+// one block: the gather map in LDS, then per column the segment rows r0 .. n-1 staged in LDS by coalesced loads
+// and written back in the new order by coalesced stores.  LDS: 8 * (n - r0) bytes.
+__global__ __launch_bounds__(1024) void k_lperm(float* __restrict__ A, int64_t lda, int64_t n, int nb, int cpw,
+                                                const int32_t* __restrict__ perm, const int32_t* __restrict__ inv_all) {
+  extern __shared__ int32_t lp_sm[];
+  const int wpb = nb / cpw;
+  const int b = (int)blockIdx.x / wpb, cw = (int)blockIdx.x - b * wpb;
+  const int64_t r0 = (int64_t)(b + 1) * nb;
+  const int len = (int)(n - r0);
+  if (len <= 0) return;
+  int32_t* map = lp_sm;
+  float* col = (float*)(lp_sm + len);
+  const int32_t* inv = inv_all + (int64_t)b * n;
+  const int t = threadIdx.x;
+  for (int i = t; i < len; i += 1024) {
+    const int32_t s = inv[perm[r0 + i]] - (int32_t)r0;
+    map[i] = min(max(s, 0), len - 1);
+  }
+  constexpr int U = 8;
+  for (int c = 0; c < cpw; c++) {
+    float* Ac = A + ((int64_t)b * nb + (int64_t)cw * cpw + c) * lda + r0;
+    __syncthreads();   // the map is complete, and the previous column's gathers are done
+    for (int i0 = 0; i0 < len; i0 += 1024 * U) {
+      float v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) { const int i = i0 + u * 1024 + t; v[u] = i < len ? Ac[i] : 0.0f; }
+#pragma unroll
+      for (int u = 0; u < U; u++) { const int i = i0 + u * 1024 + t; if (i < len) col[i] = v[u]; }
+    }
+    __syncthreads();
+    for (int i = t; i < len; i += 1024) Ac[i] = col[map[i]];
   }
 }
 
@@ -1745,15 +1792,33 @@ __device__ __forceinline__ void mma(f32x16_t& acc, const Frag<FMT_FP32>& a, cons
 // Workgroup tile 128 x 128, four waves in 2 x 2, each wave 2 x 2 MFMA tiles of 32 x 32.
 // The MFMA A operand is the U12 fragment (its 32 indices are C columns) and the B operand the L21 fragment
 // (C rows), so accumulator register a of lane l holds C[row = l%32][col = 8(a/4) + 4(l/32) + a%4].
+//
+// Tile order (xcdmap): workgroups are dealt to the 8 XCDs in turn (launch index mod 8), each XCD with its own
+// L2.  Tile ids are renumbered so that XCD x takes the x-th eighth of an order that walks bands of
+// ceil(gx / 8) row tiles column by column: an XCD then keeps its band's packed L21 (about 2 MB at m2 = 16K) in
+// its L2 and reads each U12 column tile once per band, instead of cycling through every row tile.  A
+// bijection of the tiles: every tile is computed once, with the same arithmetic.
 template <int FMT>
 __global__ __launch_bounds__(256) void k_gemm(float* __restrict__ C, int64_t ldc, int m2, int n2, int nkb,
                                               const uint8_t* __restrict__ PL, const uint8_t* __restrict__ PU,
                                               const uint8_t* __restrict__ PLlo, const uint8_t* __restrict__ PUlo,
-                                              const int32_t* __restrict__ eL, const int32_t* __restrict__ eU) {
+                                              const int32_t* __restrict__ eL, const int32_t* __restrict__ eU, int xcdmap) {
   constexpr int FB = FmtInfo<FMT>::FB;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int rt0 = (blockIdx.x * 128 + (w & 1) * 64) / 32;
-  const int ct0 = (blockIdx.y * 128 + (w >> 1) * 64) / 32;
+  int bx = blockIdx.x, by = blockIdx.y;
+  if (xcdmap) {
+    const int gx = gridDim.x, gy = gridDim.y, T = gx * gy;
+    const int pid = blockIdx.x + blockIdx.y * gx;
+    const int q = T / 8, r = T % 8, xcd = pid % 8;
+    const int np = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + pid / 8;
+    const int H = (gx + 7) / 8;
+    const int band = np / (H * gy), off = np - band * H * gy;
+    const int hb = min(H, gx - band * H);
+    by = off / hb;
+    bx = band * H + (off - by * hb);
+  }
+  const int rt0 = (bx * 128 + (w & 1) * 64) / 32;
+  const int ct0 = (by * 128 + (w >> 1) * 64) / 32;
   f32x16_t acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; i++)
@@ -1826,6 +1891,13 @@ __global__ __launch_bounds__(256) void k_gemm(float* __restrict__ C, int64_t ldc
 
 // ------------------------------------------------------------------ host driver
 static int lds_limit_bytes() { return 160 * 1024; }
+
+// XCD-banded tile order of the trailing update (MPSOLVE_GEMM_XCD=0: launch order)
+static bool gemm_xcd_order() {
+  static const bool on = !(getenv("MPSOLVE_GEMM_XCD") && atoi(getenv("MPSOLVE_GEMM_XCD")) == 0);
+  return on;
+}
+
 
 // Packed operand buffers for a trailing update of m2 rows x m2 columns: tiles padded to whole 128-wide
 // workgroup tiles.
@@ -1929,7 +2001,7 @@ static void gemm_cols(float* A, int64_t lda, int64_t k, int kb, int64_t m2, int6
   hipLaunchKernelGGL((k_gemm<FMT>), dim3((unsigned)cdiv(m2, 128), (unsigned)cdiv(n2, 128)), dim3(256), 0, W.stream,
                      A + (k + kb) + cb * lda, lda, (int)m2, (int)n2, nkb, lp.L, W.packU.p + t0 * tb,
                      FMT == FMT_FP16X3 ? lp.Llo : nullptr, FMT == FMT_FP16X3 ? W.packUlo.p + t0 * tb : nullptr,
-                     lp.s, W.scaleU.p + t0);
+                     lp.s, W.scaleU.p + t0, gemm_xcd_order() ? 1 : 0);
   LAUNCH_CK();
   W.prof_end(ev, gname, gb, gf);
 }
@@ -2095,7 +2167,7 @@ static void run_panel(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int32
 
 static void laswp_ranges(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int64_t ra0, int64_t ra1,
                          int64_t rb0, int64_t rb1, bool perm, const int32_t* plan, bool with_inv,
-                         int32_t* zero_plan, const char* pname = "laswp") {
+                         int32_t* zero_plan, const char* pname = "laswp", int32_t* inv_snap = nullptr) {
   const int64_t ncols = (ra1 - ra0) + (rb1 - rb0) + (perm ? 1 : 0);
   const int ninv = with_inv ? (int)cdiv(kb, 32) : 0;
   if (ncols <= 0 && ninv == 0 && zero_plan == nullptr) return;
@@ -2104,7 +2176,8 @@ static void laswp_ranges(const GetrfArgs& a, Workspace& W, int64_t k, int kb, in
   const int64_t nwg = std::max<int64_t>(1, std::min<int64_t>(cdiv(ncols, 8), 2 * (int64_t)W.num_cus));
   hipEvent_t e1 = W.prof_begin();
   hipLaunchKernelGGL(k_laswp, dim3((unsigned)(ninv + nwg)), dim3(256), 0, W.stream,
-                     a.A, a.lda, k, ra0, ra1, rb0, rb1, perm ? 1 : 0, plan, a.perm, kb, ninv, W.linv.p, zero_plan);
+                     a.A, a.lda, k, ra0, ra1, rb0, rb1, perm ? 1 : 0, plan, a.perm, kb, ninv, W.linv.p, zero_plan,
+                     perm ? inv_snap : nullptr, a.n);
   LAUNCH_CK();
   W.prof_end(e1, pname, 16.0 * (double)(ncols - (perm ? 1 : 0)) * kb, 0.0);
 }
@@ -2156,6 +2229,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
     HIP_CK(hipFuncSetAttribute((const void*)k_trsm, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 8192));
     HIP_CK(hipFuncSetAttribute((const void*)k_panel2, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes()));
     HIP_CK(hipFuncSetAttribute((const void*)k_trsm_mfma, hipFuncAttributeMaxDynamicSharedMemorySize, (int)trsm_mfma_lds_bytes()));
+    HIP_CK(hipFuncSetAttribute((const void*)k_lperm, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes()));
     attr_set = true;
   }
   // exchange area for the largest panel, cleared once per factorization (tags restart at 1)
@@ -2172,6 +2246,14 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   const bool use_fma_trsm = getenv("MPSOLVE_TRSM") && atoi(getenv("MPSOLVE_TRSM")) == 1;
   const bool overlap = !(getenv("MPSOLVE_OVERLAP") && atoi(getenv("MPSOLVE_OVERLAP")) == 0) && W.pstream && !use_fma_trsm;
   const bool fused_next = !(getenv("MPSOLVE_FUSED_NEXT") && atoi(getenv("MPSOLVE_FUSED_NEXT")) == 0);
+  // The factored L columns are put in final row order once, after the last panel (k_lperm), when a column
+  // segment and its gather map fit in LDS; otherwise (or with MPSOLVE_LPERM=0) every step's interchanges are
+  // applied to them as they come.  Each step's interchange launch then records the inverse permutation.
+  const bool defer_left = n > nb && (int64_t)8 * (n - nb) <= (int64_t)lds_limit_bytes() &&
+                          !(getenv("MPSOLVE_LPERM") && atoi(getenv("MPSOLVE_LPERM")) == 0);
+  if (defer_left) W.psnap.ensure(*W.tracker, (size_t)cdiv(n, nb) * (size_t)n, "perm_inverse_steps");
+  auto snap_of = [&](int64_t k) { return defer_left ? W.psnap.p + (k / nb) * n : nullptr; };
+  auto left_of = [&](int64_t k) { return defer_left ? (int64_t)0 : k; };   // end of the left column range
   auto plan_of = [&](int64_t k) { return W.affect.p + ((k / nb) & 1) * PLAN_WORDS; };
   int panels = 0;
   run_panel(a, W, 0, (int)std::min<int64_t>(nb, n), plan_of(0), W.stream, true, allow_p2, rmin);
@@ -2183,7 +2265,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
     const int kb = (int)std::min<int64_t>(nb, n - k);
     const int64_t m2 = n - k - kb;
     int32_t* plan = plan_of(k);
-    if (m2 == 0) { laswp_ranges(a, W, k, kb, 0, k, 0, 0, true, plan, false, nullptr); break; }
+    if (m2 == 0) { laswp_ranges(a, W, k, kb, 0, left_of(k), 0, 0, true, plan, false, nullptr, "laswp", snap_of(k)); break; }
     const int kbn = (int)std::min<int64_t>(nb, m2);            // next panel width
     const int64_t cN = k + kb + kbn;                            // end of the next panel's columns
     const bool split = overlap && (kbn % 128) == 0 && m2 > kbn && panel2_usable(a, W, k + kb, kbn, allow_p2);
@@ -2200,7 +2282,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
     } else if (split) {
       laswp_ranges(a, W, k, kb, k + kb, cN, 0, 0, false, plan, !use_fma_trsm, next_plan, "laswp_next");
     } else {
-      laswp_ranges(a, W, k, kb, 0, k, k + kb, n, true, plan, !use_fma_trsm, next_plan);
+      laswp_ranges(a, W, k, kb, 0, left_of(k), k + kb, n, true, plan, !use_fma_trsm, next_plan, "laswp", snap_of(k));
     }
     if (use_fma_trsm) {
       hipEvent_t e2 = W.prof_begin();
@@ -2227,12 +2309,13 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
         HIP_CK(hipEventRecord(W.ev_pack, W.pstream));
       }
       // interchanges of the remaining columns: the trailing ones now (the TRSM and the update read them); the
-      // factored L columns left of the panel, whose scattered one-element-per-line traffic would otherwise
-      // meet the next panel's start, after the update (nothing reads them before the next step's interchanges)
-      laswp_ranges(a, W, k, kb, 0, 0, cN, n, false, plan, false, nullptr);
+      // factored L columns left of the panel (nothing reads them before the end) either at the end, all steps
+      // at once (k_lperm), or after the update, where their scattered one-element-per-line traffic does not
+      // meet the next panel's start
+      laswp_ranges(a, W, k, kb, 0, 0, cN, n, defer_left, plan, false, nullptr, "laswp", snap_of(k));
       trsm_cols(a, W, k, kb, cN, n, false);
       update_cols_fmt(a.fmt, a.A, a.lda, k, kb, nb, m2, cN, n, true, false, false, a.tile_scale, W);
-      laswp_ranges(a, W, k, kb, 0, k, 0, 0, true, plan, false, nullptr, "laswp_left");
+      if (!defer_left) laswp_ranges(a, W, k, kb, 0, k, 0, 0, true, plan, false, nullptr, "laswp_left");
       HIP_CK(hipStreamWaitEvent(W.stream, W.ev_panel, 0));
     } else {
       trsm_cols(a, W, k, kb, k + kb, n, use_fma_trsm);
@@ -2242,6 +2325,19 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
       panels++;
       l_ahead = false;
     }
+  }
+  if (defer_left) {
+    const int64_t nblk = cdiv(n, nb);
+    int cpw = 16;
+    while (nb % cpw) cpw >>= 1;
+    // algorithmic bytes: every L element below its diagonal block read and written once
+    double moved = 0.0;
+    for (int64_t b = 0; b + 1 < nblk; b++) moved += (double)nb * (double)(n - (b + 1) * nb);
+    hipEvent_t e = W.prof_begin();
+    hipLaunchKernelGGL(k_lperm, dim3((unsigned)((nblk - 1) * (nb / cpw))), dim3(1024), (size_t)8 * (n - nb), W.stream,
+                       a.A, a.lda, n, nb, cpw, a.perm, W.psnap.p);
+    LAUNCH_CK();
+    W.prof_end(e, "lperm", 8.0 * moved, 0.0);
   }
   int32_t hinfo[2] = {0, 0};
   unsigned long long hc[2] = {0, 0};

@@ -59,6 +59,7 @@ struct Workspace {
   DBuf<unsigned long long> cnt; // [0] overflow, [1] subnormal counts of the operand conversion
   DBuf<int32_t> dinfo;   // [0] info, [1] panel exchange timeout flag, [2] substitution exchange timeout flag
   DBuf<int32_t> affect;  // per-panel swap plan (src,dst pairs)
+  DBuf<int32_t> psnap;   // per step: inverse of the row permutation after that step (deferred L interchanges)
   DBuf<float> linv;      // inverses of the 32x32 diagonal blocks of L11
   // getrs
   DBuf<uint64_t> ybuf;   // tagged partial-solution words of the triangular solves

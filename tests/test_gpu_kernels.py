@@ -129,6 +129,34 @@ def test_getrf_lookahead_matches_sequential(ctx, fmt, n, nb, monkeypatch):
         np.testing.assert_array_equal(x, y)
 
 
+@pytest.mark.parametrize("fmt,n,nb,overlap", [("fp16x3", 4160, 256, "1"), ("fp16", 3000, 256, "0"), ("fp32", 1001, 40, "1"),
+                                              ("fp16", 150, 7, "0")])
+def test_getrf_deferred_left_interchanges(ctx, fmt, n, nb, overlap, monkeypatch):
+    """The interchanges of the factored L columns applied all at once after the last panel (k_lperm, from the
+    recorded inverse permutation of every step) move the same data as applying each step's interchanges as it
+    comes: factors, ipiv and perm bit-identical, with and without look-ahead, for panel widths that are not a
+    multiple of the kernel's 16 columns per workgroup and a narrower last block."""
+    print(f"[getrf deferred L] fmt={fmt} n={n} nb={nb} overlap={overlap}", flush=True)
+    rng = np.random.default_rng(n + nb)
+    A = np.asfortranarray(rng.standard_normal((n, n)).astype(np.float32))
+    monkeypatch.setenv("MPSOLVE_OVERLAP", overlap)
+    out = []
+    for mode in ("0", "1"):
+        monkeypatch.setenv("MPSOLVE_LPERM", mode)
+        dLU = _dev(ctx, A); dip, dpm = ctx.empty(8 * n), ctx.empty(4 * n)
+        info = ctx.getrf(n, dLU, n, dip, dpm, nb=nb, fmt=fmt)
+        assert info.info == 0 and info.panels == -(-n // nb)
+        out.append((dLU.to_array((n, n), np.float32), dip.to_array((n,), np.int64), dpm.to_array((n,), np.int32)))
+        for b in (dLU, dip, dpm):
+            b.free()
+    for x, y in zip(out[0], out[1]):
+        np.testing.assert_array_equal(x, y)
+    # and they factor the row-permuted matrix (a misplaced L row leaves an error of the order of |A| itself)
+    LU, pm = out[1][0].astype(np.float64), out[1][2]
+    L = np.tril(LU, -1) + np.eye(n); U = np.triu(LU)
+    assert np.abs(A[pm].astype(np.float64) - L @ U).max() <= 0.25 * np.abs(A).max()
+
+
 def test_getrf_singular_info(ctx, oracle):
     n = 96
     A = np.ones((n, n), dtype=np.float32, order="F")

@@ -364,6 +364,7 @@ struct Panel2Args {
   float* A; int64_t lda; int64_t k; int kb; int m; int G;
   uint64_t* xbuf;    // P2_WORDS words, zeroed before the factorization (tags are unique within it)
   int64_t* ipiv; int32_t* dinfo; int32_t* plan;
+  int32_t* arrive;   // incremented once by every workgroup as it starts (nullptr: not counted)
 };
 
 // LDS layout (bytes): the fixed-size arrays first, at compile-time offsets (16-byte aligned), so their
@@ -571,6 +572,7 @@ __global__ __launch_bounds__(256) void k_panel2(Panel2Args a) {
   uint64_t* rowsl = a.xbuf + P2_OFF_ROWS;
   const int myrep = g % P2_NREP;
   PANEL2_WGMARK(0);
+  if (t == 0 && a.arrive != nullptr) __hip_atomic_fetch_add(a.arrive, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (t == 0) { *s_abort = 0; *s_info = 0; *s_zmask = 0; }
   {
     // the slice into LDS: thread (row r, column parity ch) keeps 32 loads in flight before its first LDS write
@@ -1174,6 +1176,15 @@ __global__ __launch_bounds__(1024) void k_lperm(float* __restrict__ A, int64_t l
     __syncthreads();
     for (int i = t; i < len; i += 1024) Ac[i] = col[map[i]];
   }
+}
+
+// Holds the stream until `target` workgroups of a panel have started (or about a millisecond has passed), so
+// that kernels launched behind it, whose LDS request does not fit beside a panel workgroup, take the other CUs
+// instead of the CUs the panel is waiting for.
+__global__ void k_gate(const int32_t* cnt, int target) {
+  if (threadIdx.x == 0)
+    for (int it = 0; it < 1024 && __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; it++)
+      __builtin_amdgcn_s_sleep(8);
 }
 
 __global__ void k_zero_i32(int32_t* p, int cnt) { const int i = blockIdx.x * 256 + threadIdx.x; if (i < cnt) p[i] = 0; }
@@ -1989,7 +2000,7 @@ void pack_operand(int fmt, bool tile_scale, const float* S, int64_t sI, int64_t 
 // A22[:, cb .. ce) -= L21 * U12[:, cb .. ce)
 template <int FMT>
 static void gemm_cols(float* A, int64_t lda, int64_t k, int kb, int64_t m2, int64_t cb, int64_t ce, Workspace& W, const char* gname,
-                      LPack lp) {
+                      LPack lp, size_t apart_lds = 0) {
   constexpr int ES = FmtInfo<FMT>::ES, FB = FmtInfo<FMT>::FB;
   const int nkb = (kb + 15) / 16;
   const int64_t t0 = (cb - k - kb) / 32, n2 = ce - cb;
@@ -1997,8 +2008,13 @@ static void gemm_cols(float* A, int64_t lda, int64_t k, int kb, int64_t m2, int6
   // algorithmic traffic of the update: C read + written (8 B/element) plus both packed operands once
   const double gb = 8.0 * (double)m2 * n2 + (FMT == FMT_FP16X3 ? 2.0 : 1.0) * ES * (double)(m2 + n2) * kb;
   const double gf = (FMT == FMT_FP16X3 ? 6.0 : 2.0) * (double)m2 * n2 * kb;
+  static bool attr = false;
+  if (!attr) {
+    HIP_CK(hipFuncSetAttribute((const void*)k_gemm<FMT>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 8192));
+    attr = true;
+  }
   hipEvent_t ev = W.prof_begin();
-  hipLaunchKernelGGL((k_gemm<FMT>), dim3((unsigned)cdiv(m2, 128), (unsigned)cdiv(n2, 128)), dim3(256), 0, W.stream,
+  hipLaunchKernelGGL((k_gemm<FMT>), dim3((unsigned)cdiv(m2, 128), (unsigned)cdiv(n2, 128)), dim3(256), apart_lds, W.stream,
                      A + (k + kb) + cb * lda, lda, (int)m2, (int)n2, nkb, lp.L, W.packU.p + t0 * tb,
                      FMT == FMT_FP16X3 ? lp.Llo : nullptr, FMT == FMT_FP16X3 ? W.packUlo.p + t0 * tb : nullptr,
                      lp.s, W.scaleU.p + t0, gemm_xcd_order() ? 1 : 0);
@@ -2009,14 +2025,14 @@ static void gemm_cols(float* A, int64_t lda, int64_t k, int kb, int64_t m2, int6
 // with_L: pack L21 of step k here (on the main stream) before the update; otherwise it was packed ahead
 template <int FMT>
 static void update_cols(float* A, int64_t lda, int64_t k, int kb, int nb, int64_t m2, int64_t cb, int64_t ce, bool last,
-                        bool with_L, bool nxt, bool tile_scale, Workspace& W, bool u_packed = false) {
+                        bool with_L, bool nxt, bool tile_scale, Workspace& W, bool u_packed = false, size_t apart_lds = 0) {
   // the next panel's columns (look-ahead) are profiled under their own names.  The packed-L pointers are
   // taken after pack_buffers, which allocates the sets on first use.
   if (with_L) pack_buffers<FMT>(kb, m2, W);
   const LPack lp = lpack(W, k, nb);
   if (with_L) pack_L<FMT>(A, lda, k, kb, m2, tile_scale, W, nxt ? "pack_next" : "pack", lp, W.stream);
   if (!u_packed) pack_U<FMT>(A, lda, k, kb, cb, ce, last, tile_scale, W, nxt ? "pack_next" : "pack");
-  gemm_cols<FMT>(A, lda, k, kb, m2, cb, ce, W, nxt ? "gemm_next" : "trailing_gemm", lp);
+  gemm_cols<FMT>(A, lda, k, kb, m2, cb, ce, W, nxt ? "gemm_next" : "trailing_gemm", lp, apart_lds);
 }
 
 // The next panel's columns [cb, cb + ncols) brought up to date except for their trailing update (k_next_prep):
@@ -2043,13 +2059,14 @@ static void next_prep(float* A, int64_t lda, int64_t k, int kb, int nb, int64_t 
 }
 
 static void update_cols_fmt(int fmt, float* A, int64_t lda, int64_t k, int kb, int nb, int64_t m2, int64_t cb, int64_t ce,
-                            bool last, bool with_L, bool nxt, bool tile_scale, Workspace& W, bool u_packed = false) {
+                            bool last, bool with_L, bool nxt, bool tile_scale, Workspace& W, bool u_packed = false,
+                            size_t apart_lds = 0) {
   switch (fmt) {
-    case FMT_FP16: update_cols<FMT_FP16>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W, u_packed); break;
-    case FMT_BF16: update_cols<FMT_BF16>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W, u_packed); break;
-    case FMT_FP8:  update_cols<FMT_FP8>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W, u_packed); break;
-    case FMT_FP16X3: update_cols<FMT_FP16X3>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W, u_packed); break;
-    default:       update_cols<FMT_FP32>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W, u_packed); break;
+    case FMT_FP16: update_cols<FMT_FP16>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W, u_packed, apart_lds); break;
+    case FMT_BF16: update_cols<FMT_BF16>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W, u_packed, apart_lds); break;
+    case FMT_FP8:  update_cols<FMT_FP8>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W, u_packed, apart_lds); break;
+    case FMT_FP16X3: update_cols<FMT_FP16X3>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W, u_packed, apart_lds); break;
+    default:       update_cols<FMT_FP32>(A, lda, k, kb, nb, m2, cb, ce, last, with_L, nxt, tile_scale, W, u_packed, apart_lds); break;
   }
 }
 static void next_prep_fmt(int fmt, float* A, int64_t lda, int64_t k, int kb, int nb, int64_t m2, int64_t cb, int ncols,
@@ -2119,6 +2136,7 @@ static void run_panel(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int32
     pa.A = a.A; pa.lda = a.lda; pa.k = k; pa.kb = kb; pa.m = m; pa.G = (int)G2;
     pa.xbuf = W.xbuf.p;
     pa.ipiv = a.ipiv; pa.dinfo = W.dinfo.p; pa.plan = plan;
+    pa.arrive = W.parrive.p ? W.parrive.p + k / a.nb : nullptr;
     void* args[] = {&pa};
     hipEvent_t e0 = W.prof_begin(stream);
     // The runtime refuses a cooperative launch while another process shares the device; the ordinary launch
@@ -2167,7 +2185,8 @@ static void run_panel(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int32
 
 static void laswp_ranges(const GetrfArgs& a, Workspace& W, int64_t k, int kb, int64_t ra0, int64_t ra1,
                          int64_t rb0, int64_t rb1, bool perm, const int32_t* plan, bool with_inv,
-                         int32_t* zero_plan, const char* pname = "laswp", int32_t* inv_snap = nullptr) {
+                         int32_t* zero_plan, const char* pname = "laswp", int32_t* inv_snap = nullptr,
+                         size_t apart_lds = 0) {
   const int64_t ncols = (ra1 - ra0) + (rb1 - rb0) + (perm ? 1 : 0);
   const int ninv = with_inv ? (int)cdiv(kb, 32) : 0;
   if (ncols <= 0 && ninv == 0 && zero_plan == nullptr) return;
@@ -2175,7 +2194,7 @@ static void laswp_ranges(const GetrfArgs& a, Workspace& W, int64_t k, int kb, in
   // stream finds room on every CU instead of queueing behind thousands of short workgroups
   const int64_t nwg = std::max<int64_t>(1, std::min<int64_t>(cdiv(ncols, 8), 2 * (int64_t)W.num_cus));
   hipEvent_t e1 = W.prof_begin();
-  hipLaunchKernelGGL(k_laswp, dim3((unsigned)(ninv + nwg)), dim3(256), 0, W.stream,
+  hipLaunchKernelGGL(k_laswp, dim3((unsigned)(ninv + nwg)), dim3(256), apart_lds, W.stream,
                      a.A, a.lda, k, ra0, ra1, rb0, rb1, perm ? 1 : 0, plan, a.perm, kb, ninv, W.linv.p, zero_plan,
                      perm ? inv_snap : nullptr, a.n);
   LAUNCH_CK();
@@ -2215,6 +2234,8 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   W.cnt.ensure(*W.tracker, 2, "getrf_counts");
   W.affect.ensure(*W.tracker, 2 * PLAN_WORDS, "swap_plan");
   W.linv.ensure(*W.tracker, (size_t)cdiv(nb, 32) * 1024, "trsm_block_inverses");
+  W.parrive.ensure(*W.tracker, (size_t)cdiv(n, nb), "panel_arrivals");
+  hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)cdiv(cdiv(n, nb), 256)), dim3(256), 0, W.stream, W.parrive.p, (int)cdiv(n, nb));
   // words 0-1 (info, panel exchange timeout) belong to this factorization; word 2, the substitutions' timeout
   // flag, is owned by whoever runs them (a refinement may refactorize between substitutions) and stays set
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, W.stream, W.dinfo.p, 2);
@@ -2230,6 +2251,7 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
     HIP_CK(hipFuncSetAttribute((const void*)k_panel2, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes()));
     HIP_CK(hipFuncSetAttribute((const void*)k_trsm_mfma, hipFuncAttributeMaxDynamicSharedMemorySize, (int)trsm_mfma_lds_bytes()));
     HIP_CK(hipFuncSetAttribute((const void*)k_lperm, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes()));
+    HIP_CK(hipFuncSetAttribute((const void*)k_laswp, hipFuncAttributeMaxDynamicSharedMemorySize, lds_limit_bytes() - 8192));
     attr_set = true;
   }
   // exchange area for the largest panel, cleared once per factorization (tags restart at 1)
@@ -2254,6 +2276,10 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   if (defer_left) W.psnap.ensure(*W.tracker, (size_t)cdiv(n, nb) * (size_t)n, "perm_inverse_steps");
   auto snap_of = [&](int64_t k) { return defer_left ? W.psnap.p + (k / nb) * n : nullptr; };
   auto left_of = [&](int64_t k) { return defer_left ? (int64_t)0 : k; };   // end of the left column range
+  const bool laswp_apart = !(getenv("MPSOLVE_LASWP_APART") && atoi(getenv("MPSOLVE_LASWP_APART")) == 0);
+  // the remaining columns' update is kept off the panel's CUs too once its rows are few enough that it still
+  // finishes within the panel on the CUs left over (its time grows as m2^2, a panel's about as m2)
+  const int64_t gemm_apart_m = getenv("MPSOLVE_GEMM_APART_M") ? atoll(getenv("MPSOLVE_GEMM_APART_M")) : 12288;
   auto plan_of = [&](int64_t k) { return W.affect.p + ((k / nb) & 1) * PLAN_WORDS; };
   int panels = 0;
   run_panel(a, W, 0, (int)std::min<int64_t>(nb, n), plan_of(0), W.stream, true, allow_p2, rmin);
@@ -2312,9 +2338,19 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
       // factored L columns left of the panel (nothing reads them before the end) either at the end, all steps
       // at once (k_lperm), or after the update, where their scattered one-element-per-line traffic does not
       // meet the next panel's start
-      laswp_ranges(a, W, k, kb, 0, 0, cN, n, defer_left, plan, false, nullptr, "laswp", snap_of(k));
+      // Kept off the panel's CUs: an LDS request that does not fit beside a panel workgroup, issued once all of
+      // the panel's workgroups have started (their polling loads would otherwise queue behind these scattered
+      // one-element accesses in the same CU's memory pipeline)
+      size_t apart = 0;
+      if (laswp_apart) {
+        apart = (size_t)std::min<int64_t>(lds_limit_bytes() - 8192,
+                                          std::max<int64_t>(0, (int64_t)lds_limit_bytes() - (int64_t)panel2_lds_bytes(kbn) + 256));
+        hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, W.stream, W.parrive.p + (k + kb) / nb, (int)cdiv(n - (k + kb), P2_R));
+      }
+      laswp_ranges(a, W, k, kb, 0, 0, cN, n, defer_left, plan, false, nullptr, "laswp", snap_of(k), apart);
       trsm_cols(a, W, k, kb, cN, n, false);
-      update_cols_fmt(a.fmt, a.A, a.lda, k, kb, nb, m2, cN, n, true, false, false, a.tile_scale, W);
+      update_cols_fmt(a.fmt, a.A, a.lda, k, kb, nb, m2, cN, n, true, false, false, a.tile_scale, W, false,
+                      m2 <= gemm_apart_m ? apart : 0);
       if (!defer_left) laswp_ranges(a, W, k, kb, 0, k, 0, 0, true, plan, false, nullptr, "laswp_left");
       HIP_CK(hipStreamWaitEvent(W.stream, W.ev_panel, 0));
     } else {

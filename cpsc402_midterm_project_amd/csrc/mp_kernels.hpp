@@ -60,6 +60,7 @@ struct Workspace {
   DBuf<int32_t> dinfo;   // [0] info, [1] panel exchange timeout flag, [2] substitution exchange timeout flag
   DBuf<int32_t> affect;  // per-panel swap plan (src,dst pairs)
   DBuf<int32_t> psnap;   // per step: inverse of the row permutation after that step (deferred L interchanges)
+  DBuf<int32_t> parrive; // per panel: workgroups started
   DBuf<float> linv;      // inverses of the 32x32 diagonal blocks of L11
   // getrs
   DBuf<uint64_t> ybuf;   // tagged partial-solution words of the triangular solves

@@ -227,8 +227,19 @@ __global__ __launch_bounds__(256) void k_absmax_partial(const TA* __restrict__ A
   const int64_t j0 = blockIdx.y * cpc;
   const int64_t jl = (j0 + cpc <= n) ? cpc : (n - j0);
   double m = 0.0;
-  if (i < n && jl > 0)
-    for (int64_t j = 0; j < jl; j++) m = dmax_nan(m, fabs((double)A[(j0 + j) * lda + i]));
+  if (i < n && jl > 0) {
+    // eight columns' loads in flight per thread (the maximum does not depend on the order)
+    const TA* col = A + j0 * lda + i;
+    int64_t j = 0;
+    for (; j + 8 <= jl; j += 8) {
+      TA v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = col[(j + u) * lda];
+#pragma unroll
+      for (int u = 0; u < 8; u++) m = dmax_nan(m, fabs((double)v[u]));
+    }
+    for (; j < jl; j++) m = dmax_nan(m, fabs((double)col[j * lda]));
+  }
   __shared__ double sh[256];
   sh[threadIdx.x] = m;
   __syncthreads();
@@ -251,9 +262,11 @@ __global__ void k_max_final(const double* __restrict__ in, int64_t cnt, double* 
 // ------------------------------------------------------------------ host launchers
 namespace mpk {
 
+// Column chunks of the partial-sum pass: about 1024 workgroups in all (16 chunks of 1024 columns at n = 16384
+// ran at 6.9 TB/s against 6.5 TB/s for 32 chunks of 512), at least 64 columns each.
 static void pick_chunks(int64_t n, int64_t& cpc, int64_t& nchunks) {
   const int64_t rb = cdiv(n, 256);
-  int64_t want = std::max<int64_t>(1, 2048 / rb);
+  int64_t want = std::max<int64_t>(1, 1024 / rb);
   want = std::min<int64_t>(want, std::max<int64_t>(1, (n + 63) / 64));
   cpc = ((cdiv(n, want) + 7) / 8) * 8;
   if (cpc < 8) cpc = 8;

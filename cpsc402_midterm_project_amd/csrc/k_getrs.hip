@@ -67,11 +67,11 @@ __device__ __forceinline__ float gs_wait(const uint64_t* p, uint64_t first, uint
 //                             upper  x_q = (delta_qc - sum_{p=q+1}^{c} U_qp x_p) / U_qq, q = c .. 0,
 // each sum accumulated in increasing p.
 template <bool LOWER>
-__global__ __launch_bounds__(128) void k_trinv(const float* __restrict__ LU, int64_t lda, int64_t n, float* __restrict__ Tinv) {
-  extern __shared__ float sm[];
+__device__ __forceinline__ void trinv_block(const float* __restrict__ LU, int64_t lda, int64_t n, float* __restrict__ Tinv,
+                                            int b, float* sm) {
   float* T = sm;                      // [128][129] row-major T[q * 129 + p]
   float* X = sm + GB * (GB + 1);      // [128][128] X[p * 128 + c]
-  const int b = blockIdx.x, c = threadIdx.x;
+  const int c = threadIdx.x;
   const int64_t o = (int64_t)b * GB;
   const int nb = (int)min((int64_t)GB, n - o);
   {
@@ -120,6 +120,15 @@ __global__ __launch_bounds__(128) void k_trinv(const float* __restrict__ LU, int
     const int col = idx >> 7, row = idx & (GB - 1);
     Tinv[(size_t)b * GB * GB + idx] = X[row * GB + col];
   }
+}
+// L's block inverses (workgroups 0 .. nbk-1) and U's (nbk .. 2 nbk-1) in one launch: each is one workgroup's
+// serial work, so the two triangles take the time of one
+__global__ __launch_bounds__(128) void k_trinv(const float* __restrict__ LU, int64_t lda, int64_t n, float* __restrict__ TinvL,
+                                               float* __restrict__ TinvU, int nbk) {
+  extern __shared__ float sm[];
+  const int bx = (int)blockIdx.x;
+  if (bx < nbk) trinv_block<true>(LU, lda, n, TinvL, bx, sm);
+  else trinv_block<false>(LU, lda, n, TinvU, bx - nbk, sm);
 }
 
 // LOWER: y = Linv_i (P r)_i - sum_{j<i} L_ij y_j, output Y (FP32).   UPPER: x_i = Uinv_i (y_i - sum_{j>i} U_ij x_j),
@@ -386,12 +395,11 @@ __device__ __forceinline__ void stage128(float* S, int t, F&& f) {
 // off-diagonal 128 x 128 part B of the block: lower [A 0; B C]^{-1} = [A^-1 0; -C^-1 B A^-1  C^-1],
 // upper [A B; 0 C]^{-1} = [A^-1  -A^-1 B C^-1; 0 C^-1].  A half block past n is the identity (B = 0).
 template <bool LOWER>
-__global__ __launch_bounds__(256) void k_tinv256(const float* __restrict__ LU, int64_t lda, int64_t n,
-                                                 const float* __restrict__ T128, float* __restrict__ T256) {
-  extern __shared__ float sm[];
+__device__ __forceinline__ void tinv256_block(const float* __restrict__ LU, int64_t lda, int64_t n,
+                                              const float* __restrict__ T128, float* __restrict__ T256, int i, float* sm) {
   float* S0 = sm;                     // [128][129]
   float* S1 = sm + GB * (GB + 1);     // [128][129]
-  const int i = blockIdx.x, t = threadIdx.x;
+  const int t = threadIdx.x;
   const int nbk128 = (int)((n + GB - 1) / GB);
   const int b0 = 2 * i, b1 = 2 * i + 1;
   const bool has1 = b1 < nbk128;
@@ -450,18 +458,26 @@ __global__ __launch_bounds__(256) void k_tinv256(const float* __restrict__ LU, i
       else out[(size_t)(col + GB) * GB3 + row] = v;
     }
 }
+// L's 256-row inverses (workgroups 0 .. nbk3-1) and U's in one launch
+__global__ __launch_bounds__(256) void k_tinv256(const float* __restrict__ LU, int64_t lda, int64_t n,
+                                                 const float* __restrict__ T128L, const float* __restrict__ T128U,
+                                                 float* __restrict__ T256L, float* __restrict__ T256U, int nbk3) {
+  extern __shared__ float sm[];
+  const int bx = (int)blockIdx.x;
+  if (bx < nbk3) tinv256_block<true>(LU, lda, n, T128L, T256L, bx, sm);
+  else tinv256_block<false>(LU, lda, n, T128U, T256U, bx - nbk3, sm);
+}
 
 // M_i = T_i^{-1} B_i for 256-row blocks (lower: B_i = L_{i,i-1}, upper: U_{i,i+1}; zero for the first block in
 // production order).  One workgroup per 128 x 128 output quarter (sub-row sr, sub-column sc): the sum over the
 // two inner quarters in increasing order.  Rows or columns past n are zero.
 template <bool LOWER>
-__global__ __launch_bounds__(256) void k_chain256(const float* __restrict__ LU, int64_t lda, int64_t n,
-                                                  const float* __restrict__ T256, float* __restrict__ M256) {
-  extern __shared__ float sm[];
+__device__ __forceinline__ void chain256_quarter(const float* __restrict__ LU, int64_t lda, int64_t n,
+                                                 const float* __restrict__ T256, float* __restrict__ M256, int bx, float* sm) {
   float* S0 = sm;
   float* S1 = sm + GB * (GB + 1);
   const int nbk = (int)((n + GB3 - 1) / GB3);
-  const int i = blockIdx.x >> 2, sr = (blockIdx.x >> 1) & 1, sc = blockIdx.x & 1;
+  const int i = bx >> 2, sr = (bx >> 1) & 1, sc = bx & 1;
   const int t = threadIdx.x, tr = t >> 4, tc = t & 15;
   const int j = LOWER ? i - 1 : i + 1;
   float* out = M256 + (size_t)i * GB3 * GB3;
@@ -496,6 +512,15 @@ __global__ __launch_bounds__(256) void k_chain256(const float* __restrict__ LU, 
       const bool valid = (int64_t)i * GB3 + row < n;
       out[(size_t)col * GB3 + row] = valid ? acc[q][p] : 0.0f;
     }
+}
+// L's chain matrices (workgroups 0 .. 4 nbk3 - 1) and U's in one launch
+__global__ __launch_bounds__(256) void k_chain256(const float* __restrict__ LU, int64_t lda, int64_t n,
+                                                  const float* __restrict__ T256L, const float* __restrict__ T256U,
+                                                  float* __restrict__ M256L, float* __restrict__ M256U, int nbk3) {
+  extern __shared__ float sm[];
+  const int bx = (int)blockIdx.x;
+  if (bx < 4 * nbk3) chain256_quarter<true>(LU, lda, n, T256L, M256L, bx, sm);
+  else chain256_quarter<false>(LU, lda, n, T256U, M256U, bx - 4 * nbk3, sm);
 }
 
 template <int NR, bool LOWER>
@@ -746,33 +771,25 @@ void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* per
     const size_t lds = (size_t)(GB * (GB + 1) + GB * GB) * 4;
     const size_t lds_cm = (size_t)2 * GB * (GB + 1) * 4;
     if (!attr) {
-      HIP_CK(hipFuncSetAttribute((const void*)k_trinv<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      HIP_CK(hipFuncSetAttribute((const void*)k_trinv<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      HIP_CK(hipFuncSetAttribute((const void*)k_trinv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       HIP_CK(hipFuncSetAttribute((const void*)k_chainmat<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cm));
       HIP_CK(hipFuncSetAttribute((const void*)k_chainmat<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cm));
-      HIP_CK(hipFuncSetAttribute((const void*)k_tinv256<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cm));
-      HIP_CK(hipFuncSetAttribute((const void*)k_tinv256<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cm));
-      HIP_CK(hipFuncSetAttribute((const void*)k_chain256<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cm));
-      HIP_CK(hipFuncSetAttribute((const void*)k_chain256<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cm));
+      HIP_CK(hipFuncSetAttribute((const void*)k_tinv256, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cm));
+      HIP_CK(hipFuncSetAttribute((const void*)k_chain256, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cm));
       attr = true;
     }
     hipEvent_t ev = W.prof_begin();
-    hipLaunchKernelGGL(k_trinv<true>, dim3((unsigned)nbk), dim3(GB), lds, W.stream, LU, lda, n, W.tinv.p);
-    LAUNCH_CK();
-    hipLaunchKernelGGL(k_trinv<false>, dim3((unsigned)nbk), dim3(GB), lds, W.stream, LU, lda, n, W.tinv.p + (size_t)nbk * GB * GB);
+    hipLaunchKernelGGL(k_trinv, dim3((unsigned)(2 * nbk)), dim3(GB), lds, W.stream, LU, lda, n, W.tinv.p,
+                       W.tinv.p + (size_t)nbk * GB * GB, (int)nbk);
     LAUNCH_CK();
     if (v3) {
       // 256-row inverses from the 128-row ones, then the 256-row chain matrices
       const float* Tl = W.tinv.p; const float* Tu = W.tinv.p + (size_t)nbk * GB * GB;
       float* T3l = W.tinv3.p; float* T3u = W.tinv3.p + (size_t)nbk3 * GB3 * GB3;
-      hipLaunchKernelGGL(k_tinv256<true>, dim3((unsigned)nbk3), dim3(256), lds_cm, W.stream, LU, lda, n, Tl, T3l);
+      hipLaunchKernelGGL(k_tinv256, dim3((unsigned)(2 * nbk3)), dim3(256), lds_cm, W.stream, LU, lda, n, Tl, Tu, T3l, T3u, (int)nbk3);
       LAUNCH_CK();
-      hipLaunchKernelGGL(k_tinv256<false>, dim3((unsigned)nbk3), dim3(256), lds_cm, W.stream, LU, lda, n, Tu, T3u);
-      LAUNCH_CK();
-      hipLaunchKernelGGL(k_chain256<true>, dim3((unsigned)(4 * nbk3)), dim3(256), lds_cm, W.stream, LU, lda, n, T3l, W.tchain3.p);
-      LAUNCH_CK();
-      hipLaunchKernelGGL(k_chain256<false>, dim3((unsigned)(4 * nbk3)), dim3(256), lds_cm, W.stream, LU, lda, n, T3u,
-                         W.tchain3.p + (size_t)nbk3 * GB3 * GB3);
+      hipLaunchKernelGGL(k_chain256, dim3((unsigned)(8 * nbk3)), dim3(256), lds_cm, W.stream, LU, lda, n, T3l, T3u, W.tchain3.p,
+                         W.tchain3.p + (size_t)nbk3 * GB3 * GB3, (int)nbk3);
       LAUNCH_CK();
     } else {
       hipLaunchKernelGGL(k_chainmat<true>, dim3((unsigned)nbk), dim3(256), lds_cm, W.stream, LU, lda, n, W.tinv.p, W.tchain.p);

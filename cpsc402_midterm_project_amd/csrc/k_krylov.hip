@@ -27,17 +27,26 @@ __global__ __launch_bounds__(256) void k_gemvt_partial(const double* __restrict_
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int64_t r = (int64_t)blockIdx.x * KCH + t;
   const double wr = (r < n) ? w[r] : 0.0;
-  for (int i = 0; i < ncols; i++) {
-    double s = (r < n) ? V[(int64_t)i * ldv + r] * wr : 0.0;
+  // eight columns' loads are issued before their eight wave reductions
+  for (int i0 = 0; i0 < ncols; i0 += 8) {
+    double v[8];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) red[wv][i & 63] = s;
-    if ((i & 63) == 63 || i == ncols - 1) {
-      __syncthreads();
-      const int base = i & ~63;
-      const int cnt = i - base + 1;
-      if (t < cnt) part[(int64_t)blockIdx.x * ncols + base + t] = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
-      __syncthreads();
+    for (int u = 0; u < 8; u++) v[u] = (r < n && i0 + u < ncols) ? V[(int64_t)(i0 + u) * ldv + r] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int i = i0 + u;
+      if (i >= ncols) break;
+      double s = v[u] * wr;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+      if (lane == 0) red[wv][i & 63] = s;
+      if ((i & 63) == 63 || i == ncols - 1) {
+        __syncthreads();
+        const int base = i & ~63;
+        const int cnt = i - base + 1;
+        if (t < cnt) part[(int64_t)blockIdx.x * ncols + base + t] = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+        __syncthreads();
+      }
     }
   }
 }
@@ -77,9 +86,18 @@ __global__ __launch_bounds__(256) void k_cgs_update(const double* __restrict__ p
   __shared__ double hs[256];
   __shared__ double red[4];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  // Both sums keep their ascending order; each issues eight loads before the eight additions that use them.
   if (t < ncols) {
     double s = 0.0;
-    for (int c = 0; c < nchunks; c++) s += part[(int64_t)c * ncols + t];
+    int c = 0;
+    for (; c + 8 <= nchunks; c += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = part[(int64_t)(c + u) * ncols + t];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += v[u];
+    }
+    for (; c < nchunks; c++) s += part[(int64_t)c * ncols + t];
     hs[t] = s;
     if (blockIdx.x == 0) hout[t] = s;
   }
@@ -88,7 +106,15 @@ __global__ __launch_bounds__(256) void k_cgs_update(const double* __restrict__ p
   double wn = 0.0;
   if (r < n) {
     double s = 0.0;
-    for (int i = 0; i < ncols; i++) s = fma(V[(int64_t)i * ldv + r], hs[i], s);
+    int i = 0;
+    for (; i + 8 <= ncols; i += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = V[(int64_t)(i + u) * ldv + r];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s = fma(v[u], hs[i + u], s);
+    }
+    for (; i < ncols; i++) s = fma(V[(int64_t)i * ldv + r], hs[i], s);
     wn = fma(sign, s, w[r]);
     w[r] = wn;
   }

@@ -114,12 +114,32 @@ __global__ __launch_bounds__(256) void k_resid_reduce(const double* __restrict__
   if (i < n) {
     const double bv = (B && !matvec) ? B[(int64_t)c * ldb + i] : 0.0;
     double r, rlo = 0.0, tt = 0.0;
+    // Chunks are combined in ascending order; eight chunks' loads are issued before their additions, so a
+    // thread waits on one memory latency per eight chunks instead of one per chunk.
+    const int64_t fs = (int64_t)F * NR * n;  // stride between consecutive chunks' fields
+    const double* base = P + (int64_t)c * n + i;
     if (DD) {
       double H = bv, L = 0.0;
-      for (int64_t k = 0; k < nchunks; k++) {
-        const double* base = P + ((k * F) * NR + c) * n + i;
-        const double h = base[0], l = base[(int64_t)2 * NR * n];
-        tt += base[(int64_t)NR * n];
+      int64_t k = 0;
+      for (; k + 8 <= nchunks; k += 8) {
+        double hv[8], lv[8], tv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const double* b = base + (k + u) * fs;
+          hv[u] = b[0]; tv[u] = b[(int64_t)NR * n]; lv[u] = b[(int64_t)2 * NR * n];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          tt += tv[u];
+          const double h = hv[u];
+          const double sn = H + h; const double bb = sn - H; const double e = (H - (sn - bb)) + (h - bb);
+          H = sn; L = L + (lv[u] + e);
+        }
+      }
+      for (; k < nchunks; k++) {
+        const double* b = base + k * fs;
+        const double h = b[0], l = b[(int64_t)2 * NR * n];
+        tt += b[(int64_t)NR * n];
         const double sn = H + h; const double bb = sn - H; const double e = (H - (sn - bb)) + (h - bb);
         H = sn; L = L + (l + e);
       }
@@ -127,10 +147,15 @@ __global__ __launch_bounds__(256) void k_resid_reduce(const double* __restrict__
       r = H + L; rlo = L - (r - H);
     } else {
       double s = 0.0;
-      for (int64_t k = 0; k < nchunks; k++) {
-        const double* base = P + ((k * F) * NR + c) * n + i;
-        s += base[0]; tt += base[(int64_t)NR * n];
+      int64_t k = 0;
+      for (; k + 8 <= nchunks; k += 8) {
+        double sv[8], tv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) { const double* b = base + (k + u) * fs; sv[u] = b[0]; tv[u] = b[(int64_t)NR * n]; }
+#pragma unroll
+        for (int u = 0; u < 8; u++) { s += sv[u]; tt += tv[u]; }
       }
+      for (; k < nchunks; k++) { const double* b = base + k * fs; s += b[0]; tt += b[(int64_t)NR * n]; }
       r = matvec ? s : (bv - s);
     }
     tt += fabs(bv);

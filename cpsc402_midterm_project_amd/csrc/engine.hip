@@ -97,6 +97,8 @@ struct Params {
 };
 
 // ------------------------------------------------------------------ context / problem
+struct Factors;
+
 struct mp_ctx {
   int device = 0;
   DevTracker tracker;
@@ -109,9 +111,12 @@ struct mp_ctx {
   std::set<mp_problem*> problems;       // live problems: their device memory is released by mp_destroy
   std::map<uint64_t, mp_problem*> factor_ids;   // factor cache: matrix id -> problem holding its factors
   double* resid_scratch = nullptr;      // device address of this GPU's allocation-free residual scratch
+  // The growth factor of the last own factorization, measured on the panel stream while the work after it runs:
+  // the factors it belongs to and max|A| of their copy.  Pinned double 3080 receives max|LU|; ev_growth marks
+  // the end of the measurement.  growth_resolve waits for it.
+  Factors* growth_target = nullptr; double growth_den = 0;
+  hipEvent_t ev_growth = nullptr;
 };
-
-struct Factors;
 
 struct mp_problem {
   mp_ctx* ctx = nullptr;
@@ -252,6 +257,17 @@ struct Factors {
   mutable DBuf<double> rs;                      // Dr r, the scaled right-hand side of an application
 };
 
+// Waits for a growth measurement still running on the panel stream and stores it with its factors (nothing
+// when none is pending).  Runs before anything writes or frees those factors and before the growth is reported.
+static void growth_resolve(mp_ctx* C) {
+  if (!C->growth_target) return;
+  Factors* F = C->growth_target;
+  C->growth_target = nullptr;
+  HIP_CK(hipEventSynchronize(C->ev_growth));
+  const double gm = C->hpin[3080];
+  F->growth = (C->growth_den > 0) ? gm / C->growth_den : -1;
+}
+
 static int fmt_of(const std::string& kind) {
   if (kind == "mx16x3") return FMT_FP16X3;
   if (kind == "het16" || kind == "mx16") return FMT_FP16;
@@ -361,6 +377,7 @@ static void equilibrate(mp_ctx* C, T* A, int64_t n, Factors& F) {
 static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, int nb, int scale_mode, Factors& F) {
   Workspace& W = C->W;
   const int64_t n = P.n; const size_t nn = (size_t)n * n;
+  growth_resolve(C);   // an earlier factorization's growth measurement may still be reading its factors
   W.factor_gen++;   // cached getrs block inverses of any earlier factors are stale from here on
   F.n = n; F.kind = kind; F.scale_mode = scale_mode; F.nb = nb; F.valid = false; F.growth = -1; F.promoted = false;
   F.conv_overflow = F.conv_subnormal = 0; F.info = 0; F.panels = 0; F.coop_refused = 0; F.coop_error.clear();
@@ -381,10 +398,23 @@ static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, i
   // a stage that changes the factor precision frees the other precision's buffers: a ladder keeps only the
   // factors it refines with (a cached problem then holds one factor matrix, which is what the serving loop counts)
   if (F.prec == 32) F.LU64.release(); else { F.LU32.release(); F.LU64p.release(); }
+  const bool own_getrf = !(kind == "cs32" || kind == "cs64" || kind == "mx64");
+  // the factor copy and the growth measurement use the second stream unless MPSOLVE_OVERLAP=0
+  const bool two_streams = W.pstream && !(std::getenv("MPSOLVE_OVERLAP") && std::getenv("MPSOLVE_OVERLAP")[0] == '0');
+  hipEvent_t cols_ready = nullptr;
   if (F.prec == 32) {
     F.LU32.ensure(C->tracker, nn, "LU32");
     if (P.fp32) {
       HIP_CK(hipMemcpyAsync(F.LU32.p, P.A32.p, nn * 4, hipMemcpyDeviceToDevice, W.stream));
+    } else if (own_getrf && scale_mode != 2 && two_streams && W.ev_conv && n > nb) {
+      // fl32(a) * fl32(g), as convert-then-scale.  The first panel's columns are converted first; the rest are
+      // converted on the panel stream while that panel runs, and getrf waits for them after launching it.
+      HIP_CK(hipEventRecord(W.ev_conv, W.stream));
+      HIP_CK(hipStreamWaitEvent(W.pstream, W.ev_conv, 0));
+      scale_convert_cols(P.A64.p, n, nb, n, F.LU32.p, n, F.gscale, W.stream);
+      scale_convert_cols(P.A64.p + (int64_t)nb * n, n, n - nb, n, F.LU32.p + (int64_t)nb * n, n, F.gscale, W.pstream);
+      HIP_CK(hipEventRecord(W.ev_conv, W.pstream));
+      cols_ready = W.ev_conv;
     } else {
       scale_convert_f64_to_f32(P.A64.p, n, n, F.LU32.p, n, F.gscale, W);   // fl32(a) * fl32(g), as convert-then-scale
     }
@@ -436,7 +466,7 @@ static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, i
     HIP_CK(hipMemcpyAsync(F.ipiv.p, ip64.data(), n * 8, hipMemcpyHostToDevice, W.stream));
   } else {
     GetrfArgs ga; ga.n = n; ga.A = F.LU32.p; ga.lda = n; ga.nb = nb; ga.fmt = fmt_of(kind);
-    ga.tile_scale = (kind != "het16"); ga.ipiv = F.ipiv.p; ga.perm = F.perm.p;
+    ga.tile_scale = (kind != "het16"); ga.ipiv = F.ipiv.p; ga.perm = F.perm.p; ga.cols_ready = cols_ready;
     GetrfInfo gi;
     getrf(ga, W, &gi);
     // a panel whose cross-workgroup exchange hit its spin limit left the factors incomplete
@@ -444,8 +474,20 @@ static void factorize(mp_ctx* C, const mp_problem& P, const std::string& kind, i
     F.info = gi.info; F.conv_overflow = gi.overflow; F.conv_subnormal = gi.subnormal; F.panels = gi.panels;
     F.coop_refused = gi.coop_refused; F.coop_error = gi.coop_error;
   }
-  const double gm = (F.prec == 32) ? dev_absmax(C, F.LU32.p, true, n, n) : dev_absmax(C, F.LU64.p, false, n, n);
-  F.growth = (amax_f > 0) ? gm / amax_f : -1;
+  if (own_getrf && F.prec == 32 && two_streams && C->ev_growth && C->hpin) {
+    // max|LU| on the panel stream (its own partials), beside the substitution set-up and first applications
+    // that follow on the main stream; growth_resolve collects it
+    W.gpart.ensure(*W.tracker, (size_t)absmax_partials(n), "growth_partials");
+    HIP_CK(hipEventRecord(C->ev_growth, W.stream));
+    HIP_CK(hipStreamWaitEvent(W.pstream, C->ev_growth, 0));
+    absmax_on(F.LU32.p, true, n, n, C->hpin + 3080, W.gpart.p, W.pstream);
+    HIP_CK(hipEventRecord(C->ev_growth, W.pstream));
+    F.growth = -1;
+    C->growth_target = &F; C->growth_den = amax_f;
+  } else {
+    const double gm = (F.prec == 32) ? dev_absmax(C, F.LU32.p, true, n, n) : dev_absmax(C, F.LU64.p, false, n, n);
+    F.growth = (amax_f > 0) ? gm / amax_f : -1;
+  }
   F.valid = true;
 }
 
@@ -525,9 +567,21 @@ static void residual_metrics(mp_ctx* C, const mp_problem& P, const double* X, bo
   ra.B = P.B.p; ra.ldb = n; ra.R = E.R.p; ra.Rlo = dd ? E.Rlo.p : nullptr; ra.ldr = n; ra.T = E.T.p; ra.ldt = n;
   ra.stats = E.stats.p; ra.worst = E.worst.p; ra.dd = dd;
   residual(ra, W);
-  std::vector<double> st(4 * nrhs); std::vector<int32_t> wi(nrhs);
-  HIP_CK(hipMemcpyAsync(st.data(), E.stats.p, 32 * nrhs, hipMemcpyDeviceToHost, W.stream));
-  HIP_CK(hipMemcpyAsync(wi.data(), E.worst.p, 4 * nrhs, hipMemcpyDeviceToHost, W.stream));
+  // read back through the context's pinned buffer (doubles [2048, 3072)) when the right-hand sides fit
+  std::vector<double> st_v; std::vector<int32_t> wi_v;
+  const double* st; const int32_t* wi;
+  if (C->hpin && nrhs <= 200) {
+    double* hs = C->hpin + 2048;
+    int32_t* hw = (int32_t*)(hs + 4 * nrhs);
+    HIP_CK(hipMemcpyAsync(hs, E.stats.p, 32 * nrhs, hipMemcpyDeviceToHost, W.stream));
+    HIP_CK(hipMemcpyAsync(hw, E.worst.p, 4 * nrhs, hipMemcpyDeviceToHost, W.stream));
+    st = hs; wi = hw;
+  } else {
+    st_v.resize(4 * nrhs); wi_v.resize(nrhs);
+    HIP_CK(hipMemcpyAsync(st_v.data(), E.stats.p, 32 * nrhs, hipMemcpyDeviceToHost, W.stream));
+    HIP_CK(hipMemcpyAsync(wi_v.data(), E.worst.p, 4 * nrhs, hipMemcpyDeviceToHost, W.stream));
+    st = st_v.data(); wi = wi_v.data();
+  }
   HIP_CK(hipStreamSynchronize(W.stream));
   M.rinf.assign(nrhs, 0); M.r2.assign(nrhs, 0); M.eta.assign(nrhs, 0); M.omega.assign(nrhs, 0);
   M.relres_inf.assign(nrhs, 0); M.relres_2.assign(nrhs, 0); M.xinf.assign(nrhs, 0); M.worst.assign(nrhs, -1);
@@ -594,6 +648,11 @@ struct Engine {
     x64.alloc(C->tracker, (size_t)n * nrhs, "x64");
     D.alloc(C->tracker, (size_t)n * nrhs, "D");
   }
+  Factors* growth_src = nullptr;   // factors of the last stage: their growth is reported
+  ~Engine() {
+    // a run that ended early (an error) may leave the growth measurement reading factors freed below
+    if (C->growth_target) { (void)hipEventSynchronize(C->ev_growth); C->growth_target = nullptr; }
+  }
   Workspace& W() { return C->W; }
   void sync() { HIP_CK(hipStreamSynchronize(C->W.stream)); }
 
@@ -603,8 +662,10 @@ struct Engine {
     const double t0 = wall_now();
     if (reuse && F.valid && F.kind == kind && F.scale_mode == M.scale_mode && F.nb == M.nb) {
       if (M.apply64) promote_factors(C, F);
+      growth_src = &F;
       T.stages.push_back(kind + "(reused)"); T.stage_times.push_back(0.0); T.growth = F.growth; T.factor_info = F.info; return F;
     }
+    growth_src = &F;
     factorize(C, P, kind, M.nb, M.scale_mode, F);
     if (M.apply64 && F.info == 0) promote_factors(C, F);
     sync();
@@ -668,10 +729,15 @@ struct Engine {
     int cycles = 0;
     while (cycles < M.gmres_cycles && budget_left > 0) {
       cycles++; if (cycles > 1) T.restarts++;
-      K.norm(K.rcur.p, K.scal());
-      HIP_CK(hipMemcpyAsync(hp, K.scal(), 8, hipMemcpyDeviceToHost, W().stream));
-      sync();
-      const double betac = hp[0];
+      // the first cycle starts from the vector whose norm is beta0 (the same kernels on the same data give the
+      // same bits), so only a restart measures it again
+      double betac = beta0;
+      if (cycles > 1) {
+        K.norm(K.rcur.p, K.scal());
+        HIP_CK(hipMemcpyAsync(hp, K.scal(), 8, hipMemcpyDeviceToHost, W().stream));
+        sync();
+        betac = hp[0];
+      }
       if (betac == 0) { status = "converged_true"; break; }
       K.scale_copy(K.rcur.p, K.V.p, K.scal(), true);   // v0 = rcur / betac
       // Right-preconditioned steps on the library's own FP32 substitutions run one ahead of the host: step j+1 is
@@ -965,6 +1031,9 @@ struct Engine {
     sync();
     if (trs_timeout) throw MPError("substitution: the block exchange timed out (workgroups not co-resident)", MP_ERR_INTERNAL);
     T.t_total = wall_now() - t0;
+    // the last stage's growth factor (measured beside the refinement, long finished by now)
+    growth_resolve(C);
+    if (growth_src) T.growth = growth_src->growth;
   }
 };
 
@@ -1076,6 +1145,8 @@ int mp_create(mp_ctx** out, int device) {
     HIP_CK(hipEventCreateWithFlags(&C->W.ev_next, hipEventDisableTiming));
     HIP_CK(hipEventCreateWithFlags(&C->W.ev_panel, hipEventDisableTiming));
     HIP_CK(hipEventCreateWithFlags(&C->W.ev_pack, hipEventDisableTiming));
+    HIP_CK(hipEventCreateWithFlags(&C->W.ev_conv, hipEventDisableTiming));
+    HIP_CK(hipEventCreateWithFlags(&C->ev_growth, hipEventDisableTiming));
     C->W.tracker = &C->tracker;
     hipDeviceProp_t prop; HIP_CK(hipGetDeviceProperties(&prop, device));
     C->W.num_cus = prop.multiProcessorCount;
@@ -1086,6 +1157,7 @@ int mp_create(mp_ctx** out, int device) {
     RB_CK(rocblas_create_handle(&C->rb_panel));
     RB_CK(rocblas_set_stream(C->rb_panel, C->W.pstream));
     HIP_CK(hipHostMalloc((void**)&C->hpin, 4096 * sizeof(double), hipHostMallocDefault));
+    C->W.hpin = C->hpin + 3072;   // doubles [3072, 3076): getrf's info readback
     C->W.dinfo.ensure(C->tracker, 4, "getrf_info");
     HIP_CK(hipMemsetAsync(C->W.dinfo.p, 0, 16, C->W.stream));
     HIP_CK(hipStreamSynchronize(C->W.stream));
@@ -1106,8 +1178,10 @@ void mp_destroy(mp_ctx* C) {
   if (C->hpin) (void)hipHostFree(C->hpin);
   (void)hipStreamSynchronize(C->W.pstream);
   hipStream_t s = C->W.stream, ps = C->W.pstream;
-  hipEvent_t e1 = C->W.ev_next, e2 = C->W.ev_panel, e3 = C->W.ev_pack;
+  hipEvent_t e1 = C->W.ev_next, e2 = C->W.ev_panel, e3 = C->W.ev_pack, e4 = C->W.ev_conv, e5 = C->ev_growth;
   delete C;
+  if (e5) (void)hipEventDestroy(e5);
+  if (e4) (void)hipEventDestroy(e4);
   if (e1) (void)hipEventDestroy(e1);
   if (e2) (void)hipEventDestroy(e2);
   if (e3) (void)hipEventDestroy(e3);

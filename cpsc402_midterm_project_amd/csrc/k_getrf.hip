@@ -1188,8 +1188,19 @@ __global__ void k_gate(const int32_t* cnt, int target) {
 }
 
 __global__ void k_zero_i32(int32_t* p, int cnt) { const int i = blockIdx.x * 256 + threadIdx.x; if (i < cnt) p[i] = 0; }
-__global__ void k_zero_u64(unsigned long long* p, int cnt) { const int i = blockIdx.x * 256 + threadIdx.x; if (i < cnt) p[i] = 0ull; }
-__global__ void k_iota(int32_t* p, int64_t n) { const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; if (i < n) p[i] = (int32_t)i; }
+// The state a factorization starts from, in one launch: the panel exchange area and both swap plans cleared,
+// the panel arrival counters, info words 0-1 and the conversion counters cleared, the permutation the identity.
+__global__ __launch_bounds__(256) void k_getrf_init(unsigned long long* __restrict__ xbuf, int64_t nx, int32_t* __restrict__ plans,
+                                                    int64_t nplan, int32_t* __restrict__ parrive, int64_t narrive,
+                                                    int32_t* __restrict__ perm, int64_t n, int32_t* __restrict__ dinfo,
+                                                    unsigned long long* __restrict__ cnt) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x, st = (int64_t)gridDim.x * 256;
+  for (int64_t i = t; i < nx; i += st) xbuf[i] = 0ull;
+  for (int64_t i = t; i < nplan; i += st) plans[i] = 0;
+  for (int64_t i = t; i < narrive; i += st) parrive[i] = 0;
+  for (int64_t i = t; i < n; i += st) perm[i] = (int32_t)i;
+  if (t < 2) { dinfo[t] = 0; cnt[t] = 0ull; }
+}
 
 // ------------------------------------------------------------------ U12 <- L11^{-1} U12 (unit lower, FP32)
 // Two kernels.  k_trsm_inv: one wave per 32x32 diagonal block T of L11 computes T^{-1} column by column
@@ -2098,11 +2109,6 @@ static void pack_L_ahead_fmt(int fmt, float* A, int64_t lda, int64_t k, int kb, 
   }
 }
 
-void fill_iota(int32_t* p, int64_t n, Workspace& W) {
-  hipLaunchKernelGGL(k_iota, dim3(cdiv(n, 256)), dim3(256), 0, W.stream, p, n);
-  LAUNCH_CK();
-}
-
 constexpr int PLAN_WORDS = 1 + 4 * 256 + 8;
 
 // A refused cooperative launch of a valid panel grid (its size was checked against the CUs) means the runtime
@@ -2235,14 +2241,6 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   W.affect.ensure(*W.tracker, 2 * PLAN_WORDS, "swap_plan");
   W.linv.ensure(*W.tracker, (size_t)cdiv(nb, 32) * 1024, "trsm_block_inverses");
   W.parrive.ensure(*W.tracker, (size_t)cdiv(n, nb), "panel_arrivals");
-  hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)cdiv(cdiv(n, nb), 256)), dim3(256), 0, W.stream, W.parrive.p, (int)cdiv(n, nb));
-  // words 0-1 (info, panel exchange timeout) belong to this factorization; word 2, the substitutions' timeout
-  // flag, is owned by whoever runs them (a refinement may refactorize between substitutions) and stays set
-  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, W.stream, W.dinfo.p, 2);
-  // both swap plans start empty (later each is cleared by the interchange launch of the step before its panel)
-  hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)cdiv(2 * PLAN_WORDS, 256)), dim3(256), 0, W.stream, W.affect.p, 2 * PLAN_WORDS);
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, W.stream, W.cnt.p, 2);
-  fill_iota(a.perm, n, W);
   static bool attr_set = false;
   if (!attr_set) {
     // dynamic limit leaves room for each kernel's static shared memory (at most 8 KB here)
@@ -2259,7 +2257,14 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
     const int64_t Gmax = std::min<int64_t>(128, cdiv(n, 64));
     const int64_t need = std::max<int64_t>(2 * (Gmax + Gmax * (nb + 1) + (nb + 1)), P2_WORDS);
     W.xbuf.ensure(*W.tracker, (size_t)need, "panel_exchange");
-    hipLaunchKernelGGL(k_zero_u64, dim3(cdiv(need, 256)), dim3(256), 0, W.stream, (unsigned long long*)W.xbuf.p, (int)need);
+    // Words 0-1 of the info area (info, panel exchange timeout) belong to this factorization; word 2, the
+    // substitutions' timeout flag, is owned by whoever runs them (a refinement may refactorize between
+    // substitutions) and stays set.  Both swap plans start empty (later each is cleared by the interchange launch
+    // of the step before its panel).
+    const int64_t most = std::max<int64_t>(need, std::max<int64_t>(n, 2 * PLAN_WORDS));
+    hipLaunchKernelGGL(k_getrf_init, dim3((unsigned)std::min<int64_t>(2048, cdiv(most, 256))), dim3(256), 0, W.stream,
+                       (unsigned long long*)W.xbuf.p, need, W.affect.p, (int64_t)(2 * PLAN_WORDS), W.parrive.p, cdiv(n, nb),
+                       a.perm, n, W.dinfo.p, (unsigned long long*)W.cnt.p);
     LAUNCH_CK();
   }
   int rmin = 64;
@@ -2284,6 +2289,8 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
   int panels = 0;
   run_panel(a, W, 0, (int)std::min<int64_t>(nb, n), plan_of(0), W.stream, true, allow_p2, rmin);
   panels++;
+  // the columns right of the first panel may still be in preparation on another stream
+  if (a.cols_ready) HIP_CK(hipStreamWaitEvent(W.stream, a.cols_ready, 0));
   // L21 of the current step was packed ahead on the panel stream (after its panel, while the previous step's
   // remaining update ran on the main stream); steps alternate between two packed-L sets for that
   bool l_ahead = false;
@@ -2375,10 +2382,14 @@ void getrf(const GetrfArgs& a, Workspace& W, GetrfInfo* out) {
     LAUNCH_CK();
     W.prof_end(e, "lperm", 8.0 * moved, 0.0);
   }
-  int32_t hinfo[2] = {0, 0};
-  unsigned long long hc[2] = {0, 0};
-  HIP_CK(hipMemcpyAsync(hinfo, W.dinfo.p, sizeof(hinfo), hipMemcpyDeviceToHost, W.stream));
-  HIP_CK(hipMemcpyAsync(hc, W.cnt.p, sizeof(hc), hipMemcpyDeviceToHost, W.stream));
+  // info words and conversion counters, read back through pinned memory where the context provides it (a copy
+  // into pageable memory is staged and synchronous)
+  int32_t hinfo_s[2] = {0, 0};
+  unsigned long long hc_s[2] = {0, 0};
+  unsigned long long* hc = W.hpin ? (unsigned long long*)W.hpin : hc_s;
+  int32_t* hinfo = W.hpin ? (int32_t*)((char*)W.hpin + 16) : hinfo_s;
+  HIP_CK(hipMemcpyAsync(hinfo, W.dinfo.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, W.stream));
+  HIP_CK(hipMemcpyAsync(hc, W.cnt.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, W.stream));
   HIP_CK(hipStreamSynchronize(W.stream));
   if (out) {
     out->info = hinfo[0]; out->timeout = hinfo[1]; out->overflow = hc[0]; out->subnormal = hc[1]; out->panels = panels;

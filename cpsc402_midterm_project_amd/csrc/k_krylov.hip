@@ -359,15 +359,34 @@ void fill_f64(double* x, int64_t n, double v, Workspace& W) {
 }
 
 // ------------------------------------------------------------------ FP64 -> FP32 copy with a power-of-two scale
-__global__ __launch_bounds__(256) void k_scale_convert(const double* __restrict__ A, int64_t n, int64_t lda,
+// out = fl32(A) * fl32(s) for an n-row x ncols block.  A workgroup covers 512 rows x 4 columns: each thread
+// issues its eight loads (rows r and r + 256 of four columns) before it stores.
+__global__ __launch_bounds__(256) void k_scale_convert(const double* __restrict__ A, int64_t n, int64_t ncols, int64_t lda,
                                                        float* __restrict__ out, int64_t ldo, double s) {
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t c = blockIdx.y;
-  if (r < n) out[c * ldo + r] = (float)A[c * lda + r] * (float)s;   // round to FP32, then the exact power-of-two scale
+  const int64_t r0 = (int64_t)blockIdx.x * 512 + threadIdx.x;
+  const int64_t c0 = (int64_t)blockIdx.y * 4;
+  const float sf = (float)s;
+  double v[8];
+#pragma unroll
+  for (int u = 0; u < 8; u++) {
+    const int64_t r = r0 + 256 * (u & 1), c = c0 + (u >> 1);
+    v[u] = (r < n && c < ncols) ? A[c * lda + r] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < 8; u++) {
+    const int64_t r = r0 + 256 * (u & 1), c = c0 + (u >> 1);
+    if (r < n && c < ncols) out[c * ldo + r] = (float)v[u] * sf;   // round to FP32, then the exact power-of-two scale
+  }
+}
+void scale_convert_cols(const double* A, int64_t n, int64_t ncols, int64_t lda, float* out, int64_t ldo, double s,
+                        hipStream_t stream) {
+  if (n <= 0 || ncols <= 0) return;
+  hipLaunchKernelGGL(k_scale_convert, dim3(cdiv(n, 512), (unsigned)cdiv(ncols, 4)), dim3(256), 0, stream, A, n, ncols, lda,
+                     out, ldo, s);
+  LAUNCH_CK();
 }
 void scale_convert_f64_to_f32(const double* A, int64_t n, int64_t lda, float* out, int64_t ldo, double s, Workspace& W) {
-  hipLaunchKernelGGL(k_scale_convert, dim3(cdiv(n, 256), (unsigned)n), dim3(256), 0, W.stream, A, n, lda, out, ldo, s);
-  LAUNCH_CK();
+  scale_convert_cols(A, n, n, lda, out, ldo, s, W.stream);
 }
 
 }  // namespace mpk

@@ -408,16 +408,26 @@ void residual_noalloc(const ResidArgs& a, Workspace& W, double* scratch_dev) {
   }
 }
 
-void absmax(const void* A, bool a32, int64_t n, int64_t lda, double* out_dev, Workspace& W) {
+int64_t absmax_partials(int64_t n) {
+  int64_t cpc, nch;
+  pick_chunks(n, cpc, nch);
+  return cdiv(n, 256) * nch;
+}
+
+void absmax_on(const void* A, bool a32, int64_t n, int64_t lda, double* out_dev, double* part, hipStream_t stream) {
   int64_t cpc, nch;
   pick_chunks(n, cpc, nch);
   const int64_t rb = cdiv(n, 256);
-  W.part.ensure(*W.tracker, (size_t)rb * nch, "absmax_partials");
-  if (a32) hipLaunchKernelGGL((k_absmax_partial<float>), dim3(rb, nch), dim3(256), 0, W.stream, (const float*)A, n, lda, cpc, W.part.p);
-  else hipLaunchKernelGGL((k_absmax_partial<double>), dim3(rb, nch), dim3(256), 0, W.stream, (const double*)A, n, lda, cpc, W.part.p);
+  if (a32) hipLaunchKernelGGL((k_absmax_partial<float>), dim3(rb, nch), dim3(256), 0, stream, (const float*)A, n, lda, cpc, part);
+  else hipLaunchKernelGGL((k_absmax_partial<double>), dim3(rb, nch), dim3(256), 0, stream, (const double*)A, n, lda, cpc, part);
   LAUNCH_CK();
-  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, W.stream, W.part.p, (int64_t)rb * nch, out_dev);
+  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, stream, part, (int64_t)rb * nch, out_dev);
   LAUNCH_CK();
+}
+
+void absmax(const void* A, bool a32, int64_t n, int64_t lda, double* out_dev, Workspace& W) {
+  W.part.ensure(*W.tracker, (size_t)absmax_partials(n), "absmax_partials");
+  absmax_on(A, a32, n, lda, out_dev, W.part.p, W.stream);
 }
 
 }  // namespace mpk

@@ -41,13 +41,16 @@ struct Workspace {
   hipStream_t stream = nullptr;
   hipStream_t pstream = nullptr;          // second stream: panel factorizations overlapped with trailing updates
   hipEvent_t ev_next = nullptr, ev_panel = nullptr, ev_pack = nullptr;   // cross-stream ordering (timing disabled)
+  hipEvent_t ev_conv = nullptr;   // the factor copy's columns past the first panel converted (second stream)
   KernelProfiler prof;
   DevTracker* tracker = nullptr;
+  void* hpin = nullptr;   // 32 pinned host bytes for the factorization's info readback (nullptr: pageable copies)
   int num_cus = 256;
   int coop_refused = 0;   // cooperative panel launches refused by the runtime in the current factorization
   std::string coop_error;  // the runtime's code for the first refusal
   DBuf<double> part;     // residual / absmax partial sums
   DBuf<double> statp;    // residual metric partials
+  DBuf<double> gpart;    // partials of the factors' maximum, measured on the second stream
   // getrf
   DBuf<uint64_t> xbuf;   // tagged-word exchange area of the panel kernel (zeroed at allocation)
   DBuf<uint8_t> packL, packU;   // trailing-update operands in MFMA fragment order
@@ -117,6 +120,9 @@ double* resid_scratch_init();
 bool resid_noalloc_fits(int64_t n, int nrhs, bool dd);
 void residual_noalloc(const ResidArgs& a, Workspace& W, double* scratch_dev);
 void absmax(const void* A, bool a32, int64_t n, int64_t lda, double* out_dev, Workspace& W);
+// the same on another stream, with its own partials buffer (at least absmax_partials(n) doubles)
+void absmax_on(const void* A, bool a32, int64_t n, int64_t lda, double* out_dev, double* part, hipStream_t stream);
+int64_t absmax_partials(int64_t n);
 
 // FMT_FP16X3: operands split into FP16 hi + lo parts (same tile scale), update = hi*hi + hi*lo + lo*hi on
 // FP16 MFMA with FP32 accumulation.
@@ -132,6 +138,9 @@ struct GetrfArgs {
   int64_t n = 0; float* A = nullptr; int64_t lda = 0; int nb = 256; int fmt = FMT_FP16; bool tile_scale = true;
   int64_t* ipiv = nullptr;       // 1-based global LAPACK pivots (device)
   int32_t* perm = nullptr;       // composed row permutation: row i of P*A is row perm[i] of A (device)
+  // when set, columns [nb, n) of A are still being written on another stream and this event marks their end:
+  // the first panel (columns [0, nb)) starts at once and everything else waits for the event
+  hipEvent_t cols_ready = nullptr;
 };
 // coop_refused: panels whose cooperative launch the runtime refused (another process on the device), which then
 // ran as ordinary launches with bounded exchange waits
@@ -181,6 +190,8 @@ void axpy_scale_inplace(double* x, int64_t n, double s, Workspace& W);
 double spectrum_value(const std::string& sp, int64_t i, int64_t n, double kappa);
 
 void scale_convert_f64_to_f32(const double* A, int64_t n, int64_t lda, float* out, int64_t ldo, double s, Workspace& W);
-void fill_iota(int32_t* p, int64_t n, Workspace& W);
+// the same for columns [0, ncols) of an n-row block, on the given stream
+void scale_convert_cols(const double* A, int64_t n, int64_t ncols, int64_t lda, float* out, int64_t ldo, double s,
+                        hipStream_t stream);
 
 }  // namespace mpk

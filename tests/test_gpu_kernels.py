@@ -129,6 +129,41 @@ def test_getrf_lookahead_matches_sequential(ctx, fmt, n, nb, monkeypatch):
         np.testing.assert_array_equal(x, y)
 
 
+@pytest.mark.parametrize("method,fmt", [("mx16x3_gmresir_w64_rdd", "fp16x3"), ("mx16_gmresir_w64_rdd", "fp16")])
+def test_solve_growth_and_split_copy(ctx, method, fmt, monkeypatch):
+    """A solve's factor copy is converted in two parts (the first panel's columns, then the rest on the second
+    stream beside that panel), and its growth factor max|LU| / max|copy| is measured on the second stream beside
+    the refinement.  The growth must be the same bits as in the one-stream schedule and as max|LU| / max|g fl32(A)|
+    formed on the host from getrf of the same scaled FP32 copy (getrf is deterministic, the power-of-two scale g
+    and both maxima are exact)."""
+    n, nb = 1000, 256
+    print(f"[solve growth] method={method} n={n}", flush=True)
+    A, b, _, _ = randsvd(n, 1e4, 11, gpu=False)
+    P = ctx.problem(A, b)
+    g = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("MPSOLVE_OVERLAP", mode)
+        X, rec = P.solve(method, nb=nb)
+        assert rec["ok"] and rec["terminal_reason"] == "converged", rec["terminal_reason"]
+        g[mode] = (rec["growth_proxy"], rec["scale_mode"])
+    P.close()
+    assert g["0"] == g["1"], g
+    growth, scale_mode = g["1"]
+    assert scale_mode in (0, 1), scale_mode
+    A32 = np.asarray(A, dtype=np.float64).astype(np.float32)
+    amax = float(np.abs(A32).max())
+    gs = 2.0 ** -np.frexp(amax)[1] if scale_mode == 1 else 1.0
+    S = (A32 * np.float32(gs)).astype(np.float32)
+    dLU = _dev(ctx, S); dip, dpm = ctx.empty(8 * n), ctx.empty(4 * n)
+    info = ctx.getrf(n, dLU, n, dip, dpm, nb=nb, fmt=fmt, tile_scale=True)
+    assert info.info == 0
+    LU = dLU.to_array((n, n), np.float32)
+    for x in (dLU, dip, dpm):
+        x.free()
+    expect = float(np.abs(LU).max()) / (amax * gs)
+    assert growth == expect, (growth, expect)
+
+
 @pytest.mark.parametrize("fmt,n,nb,overlap", [("fp16x3", 4160, 256, "1"), ("fp16", 3000, 256, "0"), ("fp32", 1001, 40, "1"),
                                               ("fp16", 150, 7, "0")])
 def test_getrf_deferred_left_interchanges(ctx, fmt, n, nb, overlap, monkeypatch):

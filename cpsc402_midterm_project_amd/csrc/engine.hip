@@ -750,9 +750,15 @@ struct Engine {
       int j = 0; bool happy = false, nonfinite = false;
       if (ahead) {
         int* act = K.active();
-        double* slots = hp + 64;                        // pinned: per step rec, hn, finite
-        for (int e = 0; e < 2; e++)
-          if (!K.ev[e]) HIP_CK(hipEventCreateWithFlags(&K.ev[e], hipEventDisableTiming));
+        // Each step's Givens kernel writes rec, hn, finite and then the step's tag into a coherent pinned slot,
+        // and the host waits by reading the tag: no copy kernel and no event in the stream between steps.
+        // Without such slots the outputs are copied back and waited for with events.
+        const bool poll = K.hstep != nullptr && m <= 255;
+        double* slots = poll ? K.hstep : hp + 64;       // pinned: per step rec, hn, finite (, tag)
+        unsigned long long tags[256] = {};
+        if (!poll)
+          for (int e = 0; e < 2; e++)
+            if (!K.ev[e]) HIP_CK(hipEventCreateWithFlags(&K.ev[e], hipEventDisableTiming));
         auto enqueue = [&](int s) {
           double* vs = K.V.p + (int64_t)s * n;
           double* zs = K.Z.p + (int64_t)s * n;
@@ -760,17 +766,33 @@ struct Engine {
           ResidArgs ra; ra.n = n; ra.nrhs = 1; ra.a32 = P.fp32; ra.A = P.a_ptr(); ra.lda = n; ra.X = zs; ra.ldx = n;
           ra.R = K.w.p; ra.ldr = n; ra.matvec = true; ra.active = act;
           residual(ra, W());
-          K.arnoldi_tail(s, beta0, M.gmres_tol, act);     // CGS2, ||w||, Givens (the step that ends the cycle clears act)
-          HIP_CK(hipMemcpyAsync(slots + 4 * s, K.scal() + 2, 24, hipMemcpyDeviceToHost, W().stream));
-          HIP_CK(hipEventRecord(K.ev[s & 1], W().stream));
+          // CGS2, ||w||, Givens (the step that ends the cycle clears act)
+          if (poll) {
+            tags[s] = ++K.tag;
+            K.arnoldi_tail(s, beta0, M.gmres_tol, act, slots + 4 * s, tags[s]);
+          } else {
+            K.arnoldi_tail(s, beta0, M.gmres_tol, act);
+            HIP_CK(hipMemcpyAsync(slots + 4 * s, K.scal() + 2, 24, hipMemcpyDeviceToHost, W().stream));
+            HIP_CK(hipEventRecord(K.ev[s & 1], W().stream));
+          }
           K.scale_copy(K.w.p, K.V.p + (int64_t)(s + 1) * n, K.scal(), true, act);   // v_{s+1} = w / hn
+        };
+        // waits until step s has written its slot; a stream that drained without it is an error
+        auto wait_step = [&](int s) {
+          const unsigned long long* tp = (const unsigned long long*)(slots + 4 * s + 3);
+          for (long it = 1;; it++) {
+            if (__atomic_load_n(tp, __ATOMIC_ACQUIRE) == tags[s]) return;
+            if ((it & 4095) == 0 && hipStreamQuery(W().stream) == hipSuccess &&
+                __atomic_load_n(tp, __ATOMIC_ACQUIRE) != tags[s])
+              throw MPError("FGMRES: an Arnoldi step ended without reporting its outputs", MP_ERR_INTERNAL);
+          }
         };
         // a step goes out only if the budget lets it run after the steps already in flight
         int enq = 0;
         if (m > 0 && budget_left > 0) { enqueue(0); enq = 1; }
         for (j = 0; j < enq; j++) {
           if (enq < m && enq - j < budget_left) { enqueue(enq); enq++; }
-          HIP_CK(hipEventSynchronize(K.ev[j & 1]));
+          if (poll) wait_step(j); else HIP_CK(hipEventSynchronize(K.ev[j & 1]));
           const double rec = slots[4 * j], hn = slots[4 * j + 1];
           const bool fin = slots[4 * j + 2] != 0.0;
           T.trs++; T.precond_applies++; T.matvecs++;
@@ -828,15 +850,16 @@ struct Engine {
       ra.B = r; ra.ldb = n; ra.R = K.sR.p; ra.ldr = n;
       residual(ra, W()); T.matvecs++; T.true_resid_checks++;
       K.norm(K.sR.p, K.scal());
-      K.norm(r, K.scal() + 1);
-      HIP_CK(hipMemcpyAsync(hp, K.scal(), 16, hipMemcpyDeviceToHost, W().stream));
+      // ||r||: the right-preconditioned cycle started from a copy of r, whose norm is beta0 (the same kernels on
+      // the same data); the left one started from F^{-1} r
+      if (M.gmres_left) K.norm(r, K.scal() + 1);
+      HIP_CK(hipMemcpyAsync(hp, K.scal(), M.gmres_left ? 16 : 8, hipMemcpyDeviceToHost, W().stream));
       HIP_CK(hipMemcpyAsync(hp + 4, K.scal() + 2, 8, hipMemcpyDeviceToHost, W().stream));
       sync();
-      const double tn = hp[0], rn0 = hp[1];
+      const double tn = hp[0], rn0 = M.gmres_left ? hp[1] : beta0;
       const double true_rel = (rn0 > 0) ? tn / rn0 : 0.0;
       if (instrument) T.ihist.push_back({outer, cycles, k - 1, rhs, hp[4], true_rel, "cycle_end"});
       if (M.gmres_left) { apply_inverse(C, F, K.sR.p, K.rcur.p, 1, &T.trs, M.apply64); T.precond_applies++; }
-      else copy_f64(K.sR.p, K.rcur.p, n, W());
       if (!std::isfinite(tn)) { status = "nonfinite"; break; }
       if (true_rel <= M.gmres_tol) { status = happy ? "happy_breakdown_verified" : "converged_true"; break; }
       if (ab && cycles == 1 && true_rel > M.gmres_tol) {
@@ -852,6 +875,7 @@ struct Engine {
         }
       }
       if (happy) { status = "happy_breakdown_unverified"; break; }
+      if (!M.gmres_left) copy_f64(K.sR.p, K.rcur.p, n, W());   // the next cycle restarts from this residual
     }
     return status;
   }

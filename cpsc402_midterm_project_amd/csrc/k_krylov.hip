@@ -213,12 +213,14 @@ __global__ void k_givens_step(double* __restrict__ H, int m, int j, const double
 
 // The Arnoldi step's tail in one launch of 256 threads: h[i] += h2[i] (the two CGS passes), hn = sqrt of the sum
 // of the w^2 chunk partials (k_norm_final's order), then k_givens_step's rotation update on thread 0; hn is also
-// stored at hnout for the scaling of the next basis vector.
+// stored at hnout for the scaling of the next basis vector.  With hslot (coherent pinned host memory) the three
+// step outputs also go there, followed by the step's tag with a system-scope release, so the host can wait for
+// the step by reading the tag.
 __global__ __launch_bounds__(256) void k_givens_fused(double* __restrict__ H, int m, int j, double* __restrict__ h,
                                                       const double* __restrict__ h2, const double* __restrict__ sq, int nchunks,
                                                       double* __restrict__ hnout, double* __restrict__ cs, double* __restrict__ sn,
                                                       double* __restrict__ g, double beta0, double* __restrict__ out, double tol,
-                                                      int* __restrict__ act) {
+                                                      int* __restrict__ act, double* __restrict__ hslot, unsigned long long tag) {
   if (act && *act == 0) return;
   __shared__ double red[256];
   const int t = threadIdx.x;
@@ -252,6 +254,10 @@ __global__ __launch_bounds__(256) void k_givens_fused(double* __restrict__ H, in
   out[1] = hnv;
   out[2] = fin ? 1.0 : 0.0;
   if (act && (!fin || hnv == 0.0 || out[0] <= tol)) *act = 0;
+  if (hslot) {
+    hslot[0] = out[0]; hslot[1] = hnv; hslot[2] = out[2];
+    __hip_atomic_store((unsigned long long*)(hslot + 3), tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 // Solve the k x k upper-triangular H y = g (back substitution, one thread).
@@ -274,7 +280,18 @@ __global__ void k_gmres_init(double* __restrict__ g, double* __restrict__ H, int
 }
 
 // ------------------------------------------------------------------ host wrappers
+Krylov::~Krylov() {
+  for (hipEvent_t& e : ev) if (e) (void)hipEventDestroy(e);
+  if (hstep) (void)hipHostFree(hstep);
+}
+
 void Krylov::ensure(int64_t n_, int m_) {
+  // step slots in coherent pinned host memory (m <= 255 steps of 4 words); without them the steps' outputs
+  // are copied back and waited for with events
+  if (!hstep && hipHostMalloc((void**)&hstep, 256 * 4 * sizeof(double), hipHostMallocCoherent) != hipSuccess) {
+    (void)hipGetLastError();
+    hstep = nullptr;
+  }
   if (n == n_ && m >= m_ && V.p) return;
   n = n_; m = m_;
   V.alloc(*W->tracker, (size_t)n * (m + 1), "gmres_V");
@@ -320,7 +337,7 @@ void Krylov::init_cycle(double beta, int* act) {
   LAUNCH_CK();
 }
 // CGS2 of w against V[:, 0..j] and the step's Givens update: 5 launches with the sums of the unfused kernels.
-void Krylov::arnoldi_tail(int j, double beta0, double tol, int* act) {
+void Krylov::arnoldi_tail(int j, double beta0, double tol, int* act, double* hslot, unsigned long long tag) {
   const unsigned nch = cdiv(n, KCH);
   const int ncols = j + 1;
   double* sq = part.p + (size_t)nch * (m + 1);
@@ -336,7 +353,7 @@ void Krylov::arnoldi_tail(int j, double beta0, double tol, int* act) {
                      sq, (const int*)act);
   LAUNCH_CK();
   hipLaunchKernelGGL(k_givens_fused, dim3(1), dim3(256), 0, W->stream, H.p, m, j, h(), h2(), sq, (int)nch, scal(), cs, sn, g,
-                     beta0, scal() + 2, tol, act);
+                     beta0, scal() + 2, tol, act, hslot, tag);
   LAUNCH_CK();
 }
 void Krylov::lsq(int k, double* y) {

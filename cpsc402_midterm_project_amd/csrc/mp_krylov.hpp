@@ -9,10 +9,13 @@ struct Krylov {
   int64_t n = 0; int m = 0;
   DBuf<double> V, Z, w, sR, rcur, H, small, part;
   hipEvent_t ev[2] = {nullptr, nullptr};             // step-output events of the steps run ahead of the host
+  // coherent pinned host slots [step][rec, hn, finite, tag] written by the step's Givens kernel, and the last tag
+  // handed out (tags only grow, so a slot never shows a stale step as the awaited one)
+  double* hstep = nullptr; unsigned long long tag = 0;
   Krylov() = default;
   Krylov(const Krylov&) = delete;
   Krylov& operator=(const Krylov&) = delete;
-  ~Krylov() { for (hipEvent_t& e : ev) if (e) (void)hipEventDestroy(e); }
+  ~Krylov();
   // small layout: cs[m+1] | sn[m+1] | g[m+1] | h[m+1] (projections) | y[m+1] | scalars...
   double* cs() { return small.p; }
   double* g() { return small.p + 2 * (m + 1); }
@@ -32,7 +35,8 @@ struct Krylov {
   void init_cycle(double beta, int* act = nullptr);
   // One Arnoldi step after w = A z_j: CGS2 against V[:, 0..j] (h, h2, w), hn = ||w|| in scal()[0] and the Givens
   // update with its outputs in scal()[2..4], in 5 launches; the same sums as dots / update / norm / givens.  j < 256.
-  void arnoldi_tail(int j, double beta0, double tol, int* act);
+  // hslot (coherent pinned host memory, 4 words): the step's outputs and then its tag are also written there.
+  void arnoldi_tail(int j, double beta0, double tol, int* act, double* hslot = nullptr, unsigned long long tag = 0);
   void lsq(int k, double* y);
 };
 

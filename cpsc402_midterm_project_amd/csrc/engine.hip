@@ -716,11 +716,14 @@ struct Engine {
     const int m = M.gmres_m;
     K.ensure(n, m);
     double* hp = C->hpin;
-    fill_f64(d, n, 0.0, W());
     // the left variant works on the preconditioned residual F^{-1} r (src/mp_core.cuh:240-241)
-    if (M.gmres_left) { apply_inverse(C, F, r, K.rcur.p, 1, &T.trs, M.apply64); T.precond_applies++; }
-    else copy_f64(r, K.rcur.p, n, W());
-    K.norm(K.rcur.p, K.scal());
+    if (M.gmres_left) {
+      fill_f64(d, n, 0.0, W());
+      apply_inverse(C, F, r, K.rcur.p, 1, &T.trs, M.apply64); T.precond_applies++;
+      K.norm(K.rcur.p, K.scal());
+    } else {
+      K.start(r, d, K.scal());   // d = 0, rcur = r, ||r||
+    }
     HIP_CK(hipMemcpyAsync(hp, K.scal(), 8, hipMemcpyDeviceToHost, W().stream));
     sync();
     const double beta0 = hp[0];
@@ -1193,6 +1196,7 @@ void mp_destroy(mp_ctx* C) {
   if (!C) return;
   (void)hipSetDevice(C->device);
   (void)hipStreamSynchronize(C->W.stream);
+  if (C->W.pstream) (void)hipStreamSynchronize(C->W.pstream);   // nothing may still read what is freed below
   // problems that outlive their context keep only host state
   for (mp_problem* p : C->problems) { p->release_device(); p->ctx = nullptr; }
   C->problems.clear();

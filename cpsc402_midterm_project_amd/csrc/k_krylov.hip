@@ -142,6 +142,23 @@ __global__ __launch_bounds__(256) void k_sumsq_partial(const double* __restrict_
   if (t == 0) part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
+// Start of a right-preconditioned solve in one pass: d = 0, rc = r, and k_sumsq_partial's chunk partials of r^2
+// (the same order, so the norm that follows is that of rc bit for bit).
+__global__ __launch_bounds__(256) void k_gmres_start(const double* __restrict__ r, double* __restrict__ rc,
+                                                     double* __restrict__ d, int64_t n, double* __restrict__ part) {
+  __shared__ double red[4];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int64_t i = (int64_t)blockIdx.x * KCH + t;
+  double v = 0.0;
+  if (i < n) { v = r[i]; rc[i] = v; d[i] = 0.0; }
+  double s = (i < n) ? v * v : 0.0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) red[wv] = s;
+  __syncthreads();
+  if (t == 0) part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
 // out[0] = sqrt(sum of partials)
 __global__ void k_norm_final(const double* __restrict__ part, int nchunks, double* __restrict__ out, const int* __restrict__ act) {
   if (act && *act == 0) return;
@@ -320,6 +337,13 @@ void Krylov::norm(const double* x, double* out, const int* act) {
   hipLaunchKernelGGL(k_sumsq_partial, dim3(nch), dim3(256), 0, W->stream, x, n, part.p, act);
   LAUNCH_CK();
   hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, W->stream, part.p, (int)nch, out, act);
+  LAUNCH_CK();
+}
+void Krylov::start(const double* r, double* d, double* out) {
+  const unsigned nch = cdiv(n, KCH);
+  hipLaunchKernelGGL(k_gmres_start, dim3(nch), dim3(256), 0, W->stream, r, rcur.p, d, n, part.p);
+  LAUNCH_CK();
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, W->stream, part.p, (int)nch, out, (const int*)nullptr);
   LAUNCH_CK();
 }
 void Krylov::scale_copy(const double* x, double* y, const double* s, bool inv, const int* act) {

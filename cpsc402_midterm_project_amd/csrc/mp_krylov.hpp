@@ -30,6 +30,8 @@ struct Krylov {
   void dots(const double* Vb, int ncols, const double* x, double* out, bool accumulate, const int* act = nullptr);
   void update(const double* Vb, int ncols, const double* h, double* x, double sign, const int* act = nullptr);
   void norm(const double* x, double* out, const int* act = nullptr);
+  // d = 0, rcur = r and out[0] = ||r|| (norm's sums), in two launches
+  void start(const double* r, double* d, double* out);
   void scale_copy(const double* x, double* y, const double* s, bool inv, const int* act = nullptr);
   void givens(int j, const double* h, const double* hn, double beta0, double* out, double tol = -1.0, int* act = nullptr);
   void init_cycle(double beta, int* act = nullptr);

@@ -762,13 +762,26 @@ struct Engine {
         if (!poll)
           for (int e = 0; e < 2; e++)
             if (!K.ev[e]) HIP_CK(hipEventCreateWithFlags(&K.ev[e], hipEventDisableTiming));
+        // z = F^{-1} v and w = A z: with FP32 factors applied unscaled, the product is formed by the upper solve's
+        // own extra workgroups as its blocks appear (the same bits as the separate product); MPSOLVE_GETRS_MV=0
+        // keeps them apart
+        const bool fuse_product = F.prec == 32 && F.scale_mode != 2 &&
+                                  !(std::getenv("MPSOLVE_GETRS_MV") && std::getenv("MPSOLVE_GETRS_MV")[0] == '0');
         auto enqueue = [&](int s) {
           double* vs = K.V.p + (int64_t)s * n;
           double* zs = K.Z.p + (int64_t)s * n;
-          apply_inverse(C, F, vs, zs, 1, nullptr, M.apply64, act);
-          ResidArgs ra; ra.n = n; ra.nrhs = 1; ra.a32 = P.fp32; ra.A = P.a_ptr(); ra.lda = n; ra.X = zs; ra.ldx = n;
-          ra.R = K.w.p; ra.ldr = n; ra.matvec = true; ra.active = act;
-          residual(ra, W());
+          bool formed = false;
+          if (fuse_product) {
+            ProductReq pq; pq.A = P.a_ptr(); pq.a32 = P.fp32; pq.w = K.w.p;
+            formed = getrs(n, 1, F.LU32.p, n, F.perm.p, vs, n, zs, n, F.gscale, W(), true, act, &pq);
+          } else {
+            apply_inverse(C, F, vs, zs, 1, nullptr, M.apply64, act);
+          }
+          if (!formed) {
+            ResidArgs ra; ra.n = n; ra.nrhs = 1; ra.a32 = P.fp32; ra.A = P.a_ptr(); ra.lda = n; ra.X = zs; ra.ldx = n;
+            ra.R = K.w.p; ra.ldr = n; ra.matvec = true; ra.active = act;
+            residual(ra, W());
+          }
           // CGS2, ||w||, Givens (the step that ends the cycle clears act)
           if (poll) {
             tags[s] = ++K.tag;

@@ -523,7 +523,71 @@ __global__ __launch_bounds__(256) void k_chain256(const float* __restrict__ LU, 
   else chain256_quarter<false>(LU, lda, n, T256U, M256U, bx - 4 * nbk3, sm);
 }
 
-template <int NR, bool LOWER>
+// The product A z that follows an upper solve, formed in the same launch (MvArgs, MV kernels): workgroups past
+// the substitution's own take 512 rows and one column chunk of k_resid_partial's split (chunks nearest the
+// bottom first, since the upper solve produces its blocks from the bottom).  Each waits until the substitution
+// has published every solution word of its chunk, stages z = scale * x in LDS (the value the substitution
+// stores in D) and forms the chunk's partial sums exactly as k_resid_partial does in product mode (same loads,
+// same fma order), so the partials, and the reduction after them, are the same bits as for the separate
+// product.  Substitution workgroups never wait for these, so in-order dispatch still guarantees progress.
+constexpr int MV_ROWS = 512;       // rows per product workgroup
+constexpr int MV_MAXC = 2048;      // largest column chunk staged in LDS
+struct MvArgs {
+  const void* A = nullptr; int64_t lda = 0; int64_t cpc = 0; int nch = 0; double* P = nullptr;
+};
+
+template <typename TA>
+__device__ __forceinline__ void mv_chunk(const TA* __restrict__ A, int64_t lda, int64_t n, int64_t cpc, int nch, int id,
+                                         const uint64_t* __restrict__ xw, uint32_t wtag, double scale,
+                                         double* __restrict__ P, double* zs, volatile lds_i32* s_abort, long spin) {
+  const int t = threadIdx.x;
+  const int nrb = (int)((n + MV_ROWS - 1) / MV_ROWS);
+  const int64_t chunk = nch - 1 - id / nrb;
+  const int64_t i = (int64_t)(id % nrb) * MV_ROWS + t;
+  const int64_t j0 = chunk * cpc;
+  const int64_t jl = (j0 + cpc <= n) ? cpc : (n - j0);
+  // one sentinel per 64-row slice of each block the chunk touches (the last row of the slice, published by the
+  // same store instruction as the rest), polled by one lane each with long sleeps
+  const int b0 = (int)(j0 / GB3), b1 = (int)((j0 + jl - 1) / GB3);
+  const int nsent = (b1 - b0 + 1) * (GB3 / SR3);
+  if (t < nsent) {
+    const uint64_t* p = xw + (int64_t)(b0 + t / (GB3 / SR3)) * GB3 + SR3 * (t % (GB3 / SR3)) + SR3 - 1;
+    (void)gs_wait(p, gs_load(p), wtag, s_abort, 8, spin);
+  }
+  __syncthreads();
+  for (int64_t q = t; q < jl; q += MV_ROWS) {
+    const uint64_t* p = xw + (j0 + q);                    // one right-hand side: word of column j is xw[j]
+    zs[q] = scale * (double)gs_wait(p, gs_load(p), wtag, s_abort, 0, spin);
+  }
+  __syncthreads();
+  if (i >= n || jl <= 0) return;
+  double s = 0.0, tt = 0.0;
+  const TA* ap = A + j0 * lda + i;
+  int64_t jj = 0;
+  for (; jj + 8 <= jl; jj += 8) {
+    double a[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) a[u] = (double)__builtin_nontemporal_load(ap + (jj + u) * lda);
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const double aa = fabs(a[u]);
+      const double x = zs[jj + u];
+      s = fma(a[u], x, s);
+      tt = fma(aa, fabs(x), tt);
+    }
+  }
+  for (; jj < jl; jj++) {
+    const double av = (double)ap[jj * lda];
+    const double x = zs[jj];
+    s = fma(av, x, s);
+    tt = fma(fabs(av), fabs(x), tt);
+  }
+  double* base = P + (chunk * 2) * n + i;
+  base[0] = s;
+  base[n] = tt;
+}
+
+template <int NR, bool LOWER, typename TA = double, bool MV = false>
 __global__ __launch_bounds__(512) void k_trsv3(const float* __restrict__ LU, int64_t lda, int64_t n,
                                                const float* __restrict__ Tinv, const float* __restrict__ Mch,
                                                const int32_t* __restrict__ perm,
@@ -531,12 +595,26 @@ __global__ __launch_bounds__(512) void k_trsv3(const float* __restrict__ LU, int
                                                float* __restrict__ Y, int64_t ldy,
                                                double* __restrict__ D, int64_t ldd, double scale,
                                                uint64_t* __restrict__ ybuf, uint32_t tag, uint32_t wtag, int32_t* __restrict__ err, long spin,
-                                               const int* __restrict__ act) {
+                                               const int* __restrict__ act, MvArgs mv) {
   if (act && *act == 0) return;
+  __shared__ int s_abort;
+  if constexpr (MV) {
+    static_assert(NR == 1 && !LOWER, "the product follows a one-column upper solve");
+    const int nbk0 = (int)((n + GB3 - 1) / GB3);
+    if ((int)blockIdx.x >= 4 * nbk0) {
+      __shared__ double zs[MV_MAXC];
+      if (threadIdx.x == 0) s_abort = 0;
+      __syncthreads();
+      mv_chunk<TA>((const TA*)mv.A, mv.lda, n, mv.cpc, mv.nch, (int)blockIdx.x - 4 * nbk0, ybuf, wtag, scale, mv.P, zs,
+                   LDS_FLAG(&s_abort), spin);
+      __syncthreads();
+      if (threadIdx.x == 0 && s_abort) err[0] = 1;
+      return;
+    }
+  }
   __shared__ float ys[3][NR][GB3];
   __shared__ float part[8][NR][SR3];
   __shared__ float sv[NR][GB3];
-  __shared__ int s_abort;
   const int nbk = (int)((n + GB3 - 1) / GB3);
   const int t = threadIdx.x, r = t & (SR3 - 1), qq = t >> 6;
   const int pi = (int)blockIdx.x >> 2, wp = (int)blockIdx.x & 3;
@@ -726,17 +804,49 @@ static void trsv_pair3(int64_t n, const float* LU, int64_t lda, const int32_t* p
   const uint32_t wl = W.gs_force ? 0xFFFFFFFFu : tl, wu = W.gs_force ? 0xFFFFFFFFu : tu;
   hipEvent_t ev = W.prof_begin();
   hipLaunchKernelGGL((k_trsv3<NR, true>), dim3(4 * nbk), dim3(512), 0, W.stream, LU, lda, n, Ti, Mi, perm, R, ldr,
-                     W.rhs32.p, n, (double*)nullptr, (int64_t)0, 1.0, W.ybuf.p, tl, wl, err, spin, act);
+                     W.rhs32.p, n, (double*)nullptr, (int64_t)0, 1.0, W.ybuf.p, tl, wl, err, spin, act, MvArgs{});
   LAUNCH_CK();
   hipLaunchKernelGGL((k_trsv3<NR, false>), dim3(4 * nbk), dim3(512), 0, W.stream, LU, lda, n, Tu, Mu, perm, (const double*)nullptr, (int64_t)0,
-                     W.rhs32.p, n, D, ldd, scale, W.ybuf.p, tu, wu, err, spin, act);
+                     W.rhs32.p, n, D, ldd, scale, W.ybuf.p, tu, wu, err, spin, act, MvArgs{});
   LAUNCH_CK();
   W.prof_end(ev, "getrs", 4.0 * (double)n * n + 24.0 * n * NR, 2.0 * (double)n * n * NR);
 }
 
-void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* perm,
+// One right-hand side, then w = A d by the upper solve's extra workgroups and the product reduction.
+static void trsv_pair3_product(int64_t n, const float* LU, int64_t lda, const int32_t* perm, const double* R, int64_t ldr,
+                               double* D, double scale, Workspace& W, int32_t* err, const int* act, const ProductReq& pr,
+                               int64_t cpc, int64_t nch) {
+  const int nbk = (int)cdiv(n, GB3);
+  const uint32_t tl = W.take_tags(1), tu = W.take_tags(1);
+  const float* Ti = W.tinv3.p;
+  const float* Tu = W.tinv3.p + (size_t)nbk * GB3 * GB3;
+  const float* Mi = W.tchain3.p;
+  const float* Mu = W.tchain3.p + (size_t)nbk * GB3 * GB3;
+  const long spin = W.gs_force ? 1L : GS_SPIN_MAX;
+  const uint32_t wl = W.gs_force ? 0xFFFFFFFFu : tl, wu = W.gs_force ? 0xFFFFFFFFu : tu;
+  W.part.ensure(*W.tracker, (size_t)nch * 2 * n, "resid_partials");
+  MvArgs mv; mv.A = pr.A; mv.lda = n; mv.cpc = cpc; mv.nch = (int)nch; mv.P = W.part.p;
+  const unsigned grid = (unsigned)(4 * nbk + cdiv(n, MV_ROWS) * nch);
+  hipEvent_t ev = W.prof_begin();
+  hipLaunchKernelGGL((k_trsv3<1, true>), dim3(4 * nbk), dim3(512), 0, W.stream, LU, lda, n, Ti, Mi, perm, R, ldr,
+                     W.rhs32.p, n, (double*)nullptr, (int64_t)0, 1.0, W.ybuf.p, tl, wl, err, spin, act, MvArgs{});
+  LAUNCH_CK();
+  if (pr.a32)
+    hipLaunchKernelGGL((k_trsv3<1, false, float, true>), dim3(grid), dim3(512), 0, W.stream, LU, lda, n, Tu, Mu, perm,
+                       (const double*)nullptr, (int64_t)0, W.rhs32.p, n, D, n, scale, W.ybuf.p, tu, wu, err, spin, act, mv);
+  else
+    hipLaunchKernelGGL((k_trsv3<1, false, double, true>), dim3(grid), dim3(512), 0, W.stream, LU, lda, n, Tu, Mu, perm,
+                       (const double*)nullptr, (int64_t)0, W.rhs32.p, n, D, n, scale, W.ybuf.p, tu, wu, err, spin, act, mv);
+  LAUNCH_CK();
+  // algorithmic bytes: the two triangles, then A once
+  W.prof_end(ev, "getrs_product", 4.0 * (double)n * n + (pr.a32 ? 4.0 : 8.0) * (double)n * n + 32.0 * n,
+             2.0 * (double)n * n + 2.0 * (double)n * n);
+  product_reduce(W.part.p, n, nch, D, pr.w, W, act);
+}
+
+bool getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* perm,
            const double* R, int64_t ldr, double* D, int64_t ldd, double scale, Workspace& W, bool reuse_inverse,
-           const int* active) {
+           const int* active, const ProductReq* product) {
   MP_REQUIRE(n > 0 && nrhs > 0, "getrs: n, nrhs > 0");
   MP_REQUIRE(lda >= n && ldr >= n && ldd >= n, "getrs: leading dimensions >= n");
   const int64_t nbk = cdiv(n, GB);
@@ -801,6 +911,14 @@ void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* per
     W.prof_end(ev, "getrs_block_inverse", 16.0 * (double)nbk * GB * GB, 2.0 * (double)nbk * GB * GB * GB * (1.0 / 3.0 + 2.0));
     W.tinv_lu = LU; W.tinv_n = n; W.tinv_lda = lda; W.tinv_gen = W.factor_gen; W.tinv_v3 = v3;
   }
+  if (product && v3 && nrhs == 1 && ldd == n) {
+    int64_t cpc, nch;
+    product_chunks(n, cpc, nch);
+    if (cpc <= MV_MAXC) {
+      trsv_pair3_product(n, LU, lda, perm, R, ldr, D, scale, W, W.dinfo.p + 2, active, *product, cpc, nch);
+      return true;
+    }
+  }
   int c = 0;
   while (c < nrhs) {
     const int left = nrhs - c;
@@ -815,6 +933,7 @@ void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* per
       else { trsv_pair<1>(n, LU, lda, perm, Rc, ldr, Dc, ldd, scale, W, err, active); c += 1; }
     }
   }
+  return false;
 }
 
 }  // namespace mpk

@@ -298,6 +298,16 @@ static void pick_chunks(int64_t n, int64_t& cpc, int64_t& nchunks) {
   nchunks = cdiv(n, cpc);
 }
 
+void product_chunks(int64_t n, int64_t& cpc, int64_t& nch) { pick_chunks(n, cpc, nch); }
+
+void product_reduce(const double* P, int64_t n, int64_t nch, const double* X, double* w, Workspace& W, const int* act) {
+  W.statp.ensure(*W.tracker, (size_t)cdiv(n, 256) * 6, "resid_stat_partials");
+  hipLaunchKernelGGL((k_resid_reduce<1, false>), dim3(cdiv(n, 256), 1), dim3(256), 0, W.stream, P, n, nch, X, n,
+                     (const double*)nullptr, (int64_t)0, w, (double*)nullptr, n, (double*)nullptr, (int64_t)0, 1,
+                     W.statp.p, 0, act);
+  LAUNCH_CK();
+}
+
 template <typename TA, int NR, bool DD>
 static void launch_group_into(const ResidArgs& a, Workspace& W, int c0, double* part, double* statp, bool prof) {
   int64_t cpc, nch;

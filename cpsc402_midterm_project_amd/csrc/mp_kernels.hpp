@@ -153,9 +153,16 @@ void pack_operand(int fmt, bool tile_scale, const float* S, int64_t sI, int64_t 
 // d = (LU)^{-1} P r  with FP32 factors and FP32 arithmetic, FP64 input/output; d is scaled by `scale`.
 // reuse_inverse: the diagonal-block inverses computed for the same LU buffer since the last getrf are reused.
 // active: device word; the substitutions return at once when it is 0 (nullptr: always run).
-void getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* perm,
+// product: when given (one right-hand side, 256-row blocks), also w = A d in FP64 (the residual kernels' product
+// mode, the same bits), formed by extra workgroups of the upper solve as its blocks appear; returns whether it was.
+struct ProductReq { const void* A = nullptr; bool a32 = false; double* w = nullptr; };
+bool getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* perm,
            const double* R, int64_t ldr, double* D, int64_t ldd, double scale, Workspace& W, bool reuse_inverse = false,
-           const int* active = nullptr);
+           const int* active = nullptr, const ProductReq* product = nullptr);
+// the residual kernels' column split of an n x n product (chunks of cpc columns) and the reduction of its
+// partials into w = A x (product mode)
+void product_chunks(int64_t n, int64_t& cpc, int64_t& nch);
+void product_reduce(const double* P, int64_t n, int64_t nch, const double* X, double* w, Workspace& W, const int* act);
 
 // Many independent systems (n <= 512, nrhs <= 16) in one persistent launch: FP32 LU (nb = 32, FP32 trailing
 // update) and stationary refinement with FP64 residuals until eta <= tau (and omega <= tau unless eta_only).

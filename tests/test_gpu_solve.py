@@ -369,3 +369,27 @@ def test_memory_admission_skips_trial(ctx):
     assert rec["outer_iters"] == 0 and rec["refactorizations"] == 0
     X, rec = P.solve("mx16_gmresir_w64_rdd", nb=64, hbm_budget_bytes=64 << 30)
     assert rec["terminal_reason"] == "converged" and not rec["skipped"]
+
+
+@pytest.mark.parametrize("n,dtype,method", [(1000, "fp64", "mx16x3_gmresir_w64_rdd"), (6144, "fp64", "mx16_gmresir_w64_rdd"),
+                                            (777, "fp32", "native_gmresir32_w64_r64")])
+def test_fgmres_product_beside_upper_solve(ctx, n, dtype, method, monkeypatch):
+    """Each FGMRES step's product w = A z can be formed by extra workgroups of the upper substitution as its
+    blocks appear (column chunks of 64 or 152 columns here, so some straddle two 256-row blocks, and a partial
+    last block at n=777).  Its chunk partials are the residual kernel's, in the same order, so the iterate, the
+    iteration counts and the residual history must be the same bits as with the separate product
+    (MPSOLVE_GETRS_MV=0)."""
+    print(f"[fgmres product] {method} n={n} {dtype}", flush=True)
+    P = Problem.generate(ctx, family="randsvd", n=n, nrhs=1, kappa=1e4, seed=9230 + n, rhs="random", dtype=dtype)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("MPSOLVE_GETRS_MV", mode)
+        X, rec = P.solve(method, nb=128, tau=1e-15, stop_on="eta")
+        assert rec["ok"] and rec["terminal_reason"] == "converged", (mode, rec["terminal_reason"])
+        assert rec["inner_iters_total"] > 0
+        out[mode] = (X, rec, [h.get("eta_max") for h in P.history() if h.get("kind") == "outer"])
+    P.close()
+    np.testing.assert_array_equal(out["0"][0], out["1"][0])
+    for k in ("inner_iters_total", "outer_iters", "matvecs", "x_sha256"):
+        assert out["0"][1][k] == out["1"][1][k], k
+    assert out["0"][2] == out["1"][2]

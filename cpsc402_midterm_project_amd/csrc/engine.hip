@@ -553,20 +553,29 @@ struct Metrics {
   std::vector<int64_t> fzero_count, ezero_count;
 };
 
-struct EvalBufs { DBuf<double> R, Rlo, T, stats; DBuf<int32_t> worst; };
+struct EvalBufs {
+  DBuf<double> R, Rlo, T, stats; DBuf<int32_t> worst;
+  void ensure(DevTracker& tr, int64_t n, int nrhs) {
+    R.ensure(tr, (size_t)n * nrhs, "R"); T.ensure(tr, (size_t)n * nrhs, "T"); Rlo.ensure(tr, (size_t)n * nrhs, "Rlo");
+    stats.ensure(tr, (size_t)4 * nrhs + 8, "stats"); worst.ensure(tr, (size_t)nrhs + 1, "worst");
+  }
+  void swap(EvalBufs& o) { R.swap(o.R); Rlo.swap(o.Rlo); T.swap(o.T); stats.swap(o.stats); worst.swap(o.worst); }
+};
 
 // Residual of the stored system for device X, FP64 or double-double, and the per-RHS metrics
-// (eval_metrics_T, src/mp_core.cuh:193-205).  R receives the FP64 (rounded) residual.
-static void residual_metrics(mp_ctx* C, const mp_problem& P, const double* X, bool dd, EvalBufs& E, Metrics& M) {
+// (eval_metrics_T, src/mp_core.cuh:193-205).  R receives the FP64 (rounded) residual.  launch = false: E already
+// holds that residual and its statistics (formed by an earlier residual_pair); only the metrics are read.
+static void residual_metrics(mp_ctx* C, const mp_problem& P, const double* X, bool dd, EvalBufs& E, Metrics& M,
+                             bool launch = true) {
   Workspace& W = C->W;
   const int64_t n = P.n; const int nrhs = P.nrhs;
-  E.R.ensure(C->tracker, (size_t)n * nrhs, "R"); E.T.ensure(C->tracker, (size_t)n * nrhs, "T");
-  E.Rlo.ensure(C->tracker, (size_t)n * nrhs, "Rlo");
-  E.stats.ensure(C->tracker, (size_t)4 * nrhs + 8, "stats"); E.worst.ensure(C->tracker, (size_t)nrhs + 1, "worst");
-  ResidArgs ra; ra.n = n; ra.nrhs = nrhs; ra.a32 = P.fp32; ra.A = P.a_ptr(); ra.lda = n; ra.X = X; ra.ldx = n;
-  ra.B = P.B.p; ra.ldb = n; ra.R = E.R.p; ra.Rlo = dd ? E.Rlo.p : nullptr; ra.ldr = n; ra.T = E.T.p; ra.ldt = n;
-  ra.stats = E.stats.p; ra.worst = E.worst.p; ra.dd = dd;
-  residual(ra, W);
+  E.ensure(C->tracker, n, nrhs);
+  if (launch) {
+    ResidArgs ra; ra.n = n; ra.nrhs = nrhs; ra.a32 = P.fp32; ra.A = P.a_ptr(); ra.lda = n; ra.X = X; ra.ldx = n;
+    ra.B = P.B.p; ra.ldb = n; ra.R = E.R.p; ra.Rlo = dd ? E.Rlo.p : nullptr; ra.ldr = n; ra.T = E.T.p; ra.ldt = n;
+    ra.stats = E.stats.p; ra.worst = E.worst.p; ra.dd = dd;
+    residual(ra, W);
+  }
   // read back through the context's pinned buffer (doubles [2048, 3072)) when the right-hand sides fit
   std::vector<double> st_v; std::vector<int32_t> wi_v;
   const double* st; const int32_t* wi;
@@ -644,6 +653,10 @@ struct Engine {
   // ladder policies of this build: the iterate with the smallest eta so far, where the next stage starts
   bool keep_best = false; double best_eta = INFINITY; DBuf<double> xbest;
   EvalBufs E;
+  // The end of an FGMRES solve and the next outer residual share one pass over A (residual_pair): the cycle's
+  // check r - A d and the residual of x + d (into E2, with xn = x + d).  pair_done: E2 holds that residual for
+  // the solve's final d; pair_ready: x was then updated by that d, so the next outer residual is E2.
+  EvalBufs E2; DBuf<double> xn; bool pair_done = false, pair_ready = false;
   Engine(mp_ctx* c, const mp_problem& p, double te, double to) : C(c), P(p), n(p.n), nrhs(p.nrhs), tau_eta(te), tau_omega(to) {
     x64.alloc(C->tracker, (size_t)n * nrhs, "x64");
     D.alloc(C->tracker, (size_t)n * nrhs, "D");
@@ -716,6 +729,11 @@ struct Engine {
     const int m = M.gmres_m;
     K.ensure(n, m);
     double* hp = C->hpin;
+    pair_done = false;
+    // the cycle's check and the next outer residual share a pass over A when the update x += d follows as it is
+    // (one right-hand side, FP64 iterate, FP64 or double-double residual); MPSOLVE_RESID_PAIR=0 keeps them apart
+    const bool pair_ok = nrhs == 1 && M.working == 64 && (M.resid == 64 || M.resid == 128) &&
+                         !(std::getenv("MPSOLVE_RESID_PAIR") && std::getenv("MPSOLVE_RESID_PAIR")[0] == '0');
     // the left variant works on the preconditioned residual F^{-1} r (src/mp_core.cuh:240-241)
     if (M.gmres_left) {
       fill_f64(d, n, 0.0, W());
@@ -862,9 +880,22 @@ struct Engine {
       K.lsq(k, K.y());
       K.update(M.gmres_left ? K.V.p : K.Z.p, k, K.y(), d, 1.0);   // left: d += V y; right (flexible): d += Z y
       // true residual of the correction equation: sR = r - A d (FP64)
-      ResidArgs ra; ra.n = n; ra.nrhs = 1; ra.a32 = P.fp32; ra.A = P.a_ptr(); ra.lda = n; ra.X = d; ra.ldx = n;
-      ra.B = r; ra.ldb = n; ra.R = K.sR.p; ra.ldr = n;
-      residual(ra, W()); T.matvecs++; T.true_resid_checks++;
+      if (pair_ok) {
+        // with the residual of x + d, which becomes the next outer residual if this is the solve's last cycle
+        xn.ensure(C->tracker, (size_t)n, "x_next");
+        E2.ensure(C->tracker, n, 1);
+        axpy_to(xn.p, x64.p, d, n, 1.0, W());
+        ResidPairArgs pa; pa.n = n; pa.a32 = P.fp32; pa.A = P.a_ptr(); pa.X1 = d; pa.B1 = r; pa.R1 = K.sR.p;
+        pa.X2 = xn.p; pa.B2 = P.B.p; pa.R2 = E2.R.p; pa.R2lo = E2.Rlo.p; pa.T2 = E2.T.p; pa.stats2 = E2.stats.p;
+        pa.worst2 = E2.worst.p; pa.dd2 = M.resid == 128;
+        residual_pair(pa, W());
+        pair_done = true;
+      } else {
+        ResidArgs ra; ra.n = n; ra.nrhs = 1; ra.a32 = P.fp32; ra.A = P.a_ptr(); ra.lda = n; ra.X = d; ra.ldx = n;
+        ra.B = r; ra.ldb = n; ra.R = K.sR.p; ra.ldr = n;
+        residual(ra, W());
+      }
+      T.matvecs++; T.true_resid_checks++;
       K.norm(K.sR.p, K.scal());
       // ||r||: the right-preconditioned cycle started from a copy of r, whose norm is beta0 (the same kernels on
       // the same data); the left one started from F^{-1} r
@@ -927,7 +958,13 @@ struct Engine {
       }
       return;
     }
-    residual_metrics(C, P, x64.p, M.resid == 128, E, Mt);
+    if (pair_ready) {   // formed with the last FGMRES check, for this very iterate
+      pair_ready = false;
+      E.swap(E2);
+      residual_metrics(C, P, x64.p, M.resid == 128, E, Mt, false);
+    } else {
+      residual_metrics(C, P, x64.p, M.resid == 128, E, Mt);
+    }
     if (keep_best && Mt.finite && Mt.eta_max < best_eta) {
       best_eta = Mt.eta_max;
       copy_f64(x64.p, xbest.p, (int64_t)n * nrhs, W());
@@ -982,6 +1019,8 @@ struct Engine {
       }
       T.hist.back().inner_in_step = T.inner_total - inner_before;
       add_update(M, D.p);
+      pair_ready = pair_done && M.gmres;   // x is now the x + d whose residual the last check formed
+      pair_done = false;
       T.outer++;
       local_step++;
     }

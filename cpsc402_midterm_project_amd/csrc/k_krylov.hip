@@ -186,6 +186,10 @@ __global__ void k_axpby(double* __restrict__ x, const double* __restrict__ d, in
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (r < n) x[r] = fma(a, d[r], x[r]);
 }
+__global__ void k_axpby_to(double* __restrict__ y, const double* __restrict__ x, const double* __restrict__ d, int64_t n, double a) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r < n) y[r] = fma(a, d[r], x[r]);   // k_axpby's value, into another vector
+}
 __global__ void k_copy_f64(const double* __restrict__ x, double* __restrict__ y, int64_t n) {
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (r < n) y[r] = x[r];
@@ -388,6 +392,10 @@ void Krylov::lsq(int k, double* y) {
 
 void axpy(double* x, const double* d, int64_t n, double a, Workspace& W) {
   hipLaunchKernelGGL(k_axpby, dim3(cdiv(n, 256)), dim3(256), 0, W.stream, x, d, n, a);
+  LAUNCH_CK();
+}
+void axpy_to(double* y, const double* x, const double* d, int64_t n, double a, Workspace& W) {
+  hipLaunchKernelGGL(k_axpby_to, dim3(cdiv(n, 256)), dim3(256), 0, W.stream, y, x, d, n, a);
   LAUNCH_CK();
 }
 void copy_f64(const double* x, double* y, int64_t n, Workspace& W) {

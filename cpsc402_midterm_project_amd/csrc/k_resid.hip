@@ -96,6 +96,55 @@ __global__ __launch_bounds__(256) void k_resid_partial(const TA* __restrict__ A,
   }
 }
 
+// One pass over A for two single-column products: column 1 with k_resid_partial<TA, 1, false>'s FP64 sums into
+// P1, column 2 with k_resid_partial<TA, 1, DD2>'s sums into P2 (each in its own layout).  Every accumulator sees
+// the operations of its own kernel in the same order, so both sets of partials are bitwise theirs.
+template <typename TA, bool DD2>
+__global__ __launch_bounds__(256) void k_resid_partial_pair(const TA* __restrict__ A, int64_t n, int64_t lda,
+                                                            const double* __restrict__ X1, const double* __restrict__ X2,
+                                                            int64_t cpc, double* __restrict__ P1, double* __restrict__ P2) {
+  constexpr int F2 = DD2 ? 3 : 2;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t chunk = blockIdx.y;
+  const int64_t j0 = chunk * cpc;
+  const int64_t jl = (j0 + cpc <= n) ? cpc : (n - j0);
+  if (i >= n || jl <= 0) return;
+  double s1 = 0.0, t1 = 0.0, s2 = 0.0, t2 = 0.0, lo2 = 0.0;
+  const TA* ap = A + j0 * lda + i;
+  const double* x1p = X1 + j0;
+  const double* x2p = X2 + j0;
+  auto col = [&](double av, double x1, double x2) {
+    const double aa = fabs(av);
+    s1 = fma(av, x1, s1);
+    t1 = fma(aa, fabs(x1), t1);
+    if (DD2) {
+      const double p = -av * x2;
+      const double pe = fma(-av, x2, -p);
+      const double sn = s2 + p;
+      const double bb = sn - s2;
+      const double e = (s2 - (sn - bb)) + (p - bb);
+      s2 = sn; lo2 += e + pe;
+    } else {
+      s2 = fma(av, x2, s2);
+    }
+    t2 = fma(aa, fabs(x2), t2);
+  };
+  int64_t jj = 0;
+  for (; jj + 8 <= jl; jj += 8) {
+    double a[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) a[u] = (double)__builtin_nontemporal_load(ap + (jj + u) * lda);
+#pragma unroll
+    for (int u = 0; u < 8; u++) col(a[u], x1p[jj + u], x2p[jj + u]);
+  }
+  for (; jj < jl; jj++) col((double)ap[jj * lda], x1p[jj], x2p[jj]);
+  double* b1 = P1 + (chunk * 2) * n + i;
+  b1[0] = s1; b1[n] = t1;
+  double* b2 = P2 + (chunk * F2) * n + i;
+  b2[0] = s2; b2[n] = t2;
+  if (DD2) b2[2 * n] = lo2;
+}
+
 // Combine the chunk partials in chunk order, write R/T (and the DD low part), and reduce the per-block
 // metric partials.  grid = (ceil(n/256), NR); one workgroup handles 256 rows of one right-hand side.
 // Sp layout: Sp[(c_global*nblk + blk)*6 + k], k: 0 max|R|, 1 max|X|, 2 max q, 3 sum R^2, 4 worst row.
@@ -358,6 +407,44 @@ static void check_resid_args(const ResidArgs& a) {
   MP_REQUIRE(a.A && a.X && a.R, "A, X and R are required");
   MP_REQUIRE(!a.T || a.ldt >= a.n, "ldt must be >= n");
   MP_REQUIRE(!a.B || a.ldb >= a.n, "ldb must be >= n");
+}
+
+void residual_pair(const ResidPairArgs& a, Workspace& W) {
+  MP_REQUIRE(a.n > 0 && a.A && a.X1 && a.B1 && a.R1 && a.X2 && a.B2 && a.R2 && a.T2 && a.stats2 && a.worst2,
+             "residual_pair: every operand is required");
+  MP_REQUIRE(!a.dd2 || a.R2lo, "residual_pair: the double-double residual needs its low part");
+  const int64_t n = a.n;
+  int64_t cpc, nch;
+  pick_chunks(n, cpc, nch);
+  const int64_t rb = cdiv(n, 256);
+  const int F2 = a.dd2 ? 3 : 2;
+  W.part.ensure(*W.tracker, (size_t)nch * (2 + F2) * n, "resid_partials");
+  W.statp.ensure(*W.tracker, (size_t)rb * 6, "resid_stat_partials");
+  double* P1 = W.part.p;
+  double* P2 = W.part.p + (size_t)nch * 2 * n;
+  hipEvent_t ev = W.prof_begin();
+  if (a.a32) {
+    if (a.dd2) hipLaunchKernelGGL((k_resid_partial_pair<float, true>), dim3(rb, nch), dim3(256), 0, W.stream, (const float*)a.A, n, n, a.X1, a.X2, cpc, P1, P2);
+    else hipLaunchKernelGGL((k_resid_partial_pair<float, false>), dim3(rb, nch), dim3(256), 0, W.stream, (const float*)a.A, n, n, a.X1, a.X2, cpc, P1, P2);
+  } else {
+    if (a.dd2) hipLaunchKernelGGL((k_resid_partial_pair<double, true>), dim3(rb, nch), dim3(256), 0, W.stream, (const double*)a.A, n, n, a.X1, a.X2, cpc, P1, P2);
+    else hipLaunchKernelGGL((k_resid_partial_pair<double, false>), dim3(rb, nch), dim3(256), 0, W.stream, (const double*)a.A, n, n, a.X1, a.X2, cpc, P1, P2);
+  }
+  LAUNCH_CK();
+  W.prof_end(ev, "residual_pair_partial", (a.a32 ? 4.0 : 8.0) * n * n + 16.0 * n, (a.dd2 ? 24.0 : 8.0) * n * n);
+  // column 1 as residual() reduces a one-column FP64 residual without statistics, then column 2 with them
+  hipLaunchKernelGGL((k_resid_reduce<1, false>), dim3(rb, 1), dim3(256), 0, W.stream, P1, n, nch, a.X1, n, a.B1, n, a.R1,
+                     (double*)nullptr, n, (double*)nullptr, (int64_t)0, 0, W.statp.p, 0, (const int*)nullptr);
+  LAUNCH_CK();
+  if (a.dd2)
+    hipLaunchKernelGGL((k_resid_reduce<1, true>), dim3(rb, 1), dim3(256), 0, W.stream, P2, n, nch, a.X2, n, a.B2, n, a.R2,
+                       a.R2lo, n, a.T2, n, 0, W.statp.p, 0, (const int*)nullptr);
+  else
+    hipLaunchKernelGGL((k_resid_reduce<1, false>), dim3(rb, 1), dim3(256), 0, W.stream, P2, n, nch, a.X2, n, a.B2, n, a.R2,
+                       (double*)nullptr, n, a.T2, n, 0, W.statp.p, 0, (const int*)nullptr);
+  LAUNCH_CK();
+  hipLaunchKernelGGL(k_resid_stats_final, dim3(1), dim3(256), 0, W.stream, W.statp.p, rb, 1, a.stats2, a.worst2);
+  LAUNCH_CK();
 }
 
 void residual(const ResidArgs& a, Workspace& W) {

@@ -64,6 +64,7 @@ template <typename T> struct DBuf {
   // grow-only: keeps the buffer when it is already large enough
   bool ensure(DevTracker& t, size_t n_, const char* what) { if (p && n >= n_) return false; alloc(t, n_, what); return true; }
   void release() { if (p && tr) tr->free(p); p = nullptr; n = 0; }
+  void swap(DBuf& o) { std::swap(p, o.p); std::swap(n, o.n); std::swap(tr, o.tr); }
 };
 
 // ------------------------------------------------------------------ JSON emitter (finite numbers with %.17g, non-finite -> null)

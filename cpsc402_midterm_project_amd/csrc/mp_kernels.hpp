@@ -110,6 +110,16 @@ struct ResidArgs {
   double* stats = nullptr; int32_t* worst = nullptr; bool dd = false; bool matvec = false;
   const int* active = nullptr;   // device word: the kernels return at once when it is 0 (nullptr: always run)
 };
+// Two one-column residuals of the same n x n A (leading dimension n) in one pass over A: R1 = B1 - A X1 in FP64
+// (no statistics) and R2 = B2 - A X2 in FP64 or double-double with T2 and the statistics, each the same bits as
+// residual() gives for it alone.
+struct ResidPairArgs {
+  int64_t n = 0; bool a32 = false; const void* A = nullptr;
+  const double* X1 = nullptr; const double* B1 = nullptr; double* R1 = nullptr;
+  const double* X2 = nullptr; const double* B2 = nullptr; double* R2 = nullptr; double* R2lo = nullptr; double* T2 = nullptr;
+  double* stats2 = nullptr; int32_t* worst2 = nullptr; bool dd2 = false;
+};
+void residual_pair(const ResidPairArgs& a, Workspace& W);
 void residual(const ResidArgs& a, Workspace& W);
 // Allocation-free residual: partials in a fixed device array of RESID_SCRATCH_PART + RESID_SCRATCH_STAT doubles
 // (16 MiB + 2 MiB), shared by every call on one GPU.  resid_scratch_init (with the device current) returns that

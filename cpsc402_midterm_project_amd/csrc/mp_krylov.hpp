@@ -44,6 +44,8 @@ struct Krylov {
 
 void axpy(double* x, const double* d, int64_t n, double a, Workspace& W);
 void copy_f64(const double* x, double* y, int64_t n, Workspace& W);
+// y = x + a d with axpy's rounding (y may not alias x or d)
+void axpy_to(double* y, const double* x, const double* d, int64_t n, double a, Workspace& W);
 void fill_f64(double* x, int64_t n, double v, Workspace& W);
 
 }  // namespace mpk

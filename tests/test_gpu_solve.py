@@ -393,3 +393,29 @@ def test_fgmres_product_beside_upper_solve(ctx, n, dtype, method, monkeypatch):
     for k in ("inner_iters_total", "outer_iters", "matvecs", "x_sha256"):
         assert out["0"][1][k] == out["1"][1][k], k
     assert out["0"][2] == out["1"][2]
+
+
+@pytest.mark.parametrize("n,dtype,method", [(1000, "fp64", "mx16x3_gmresir_w64_rdd"), (1500, "fp64", "hetero16_gmresir_w64_r64"),
+                                            (777, "fp32", "native_gmresir32_w64_r64"),
+                                            (900, "fp64", "native_leftgmres32_app32_w64_r64")])
+def test_fgmres_check_and_next_residual_share_a_pass(ctx, n, dtype, method, monkeypatch):
+    """The FP64 check r - A d at the end of an FGMRES cycle and the residual of x + d (double-double or FP64),
+    which the next outer step evaluates after x += d, can be formed in one pass over A.  Each keeps its own
+    kernel's sums in order, so the iterate, the counts and the history must be the same bits as with the two
+    passes apart (MPSOLVE_RESID_PAIR=0), for the right- and left-preconditioned solves and FP32 storage."""
+    print(f"[resid pair] {method} n={n} {dtype}", flush=True)
+    P = Problem.generate(ctx, family="randsvd", n=n, nrhs=1, kappa=1e4, seed=9240 + n, rhs="random", dtype=dtype)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("MPSOLVE_RESID_PAIR", mode)
+        X, rec = P.solve(method, nb=128, tau=1e-15, stop_on="eta")
+        assert rec["ok"] and rec["terminal_reason"] == "converged", (mode, rec["terminal_reason"])
+        assert rec["outer_iters"] >= 2 and rec["inner_iters_total"] > 0, (rec["outer_iters"], rec["inner_iters_total"])
+        hist = [(h.get("eta_max"), h.get("omega_max"), h.get("note")) for h in P.history() if h.get("kind") == "outer"]
+        out[mode] = (X, rec, hist)
+    P.close()
+    np.testing.assert_array_equal(out["0"][0], out["1"][0])
+    for k in ("inner_iters_total", "outer_iters", "matvecs", "true_residual_checks", "x_sha256", "eta", "omega", "rinf",
+              "worst_row"):
+        assert out["0"][1][k] == out["1"][1][k], k
+    assert out["0"][2] == out["1"][2]

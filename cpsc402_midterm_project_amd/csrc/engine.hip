@@ -789,8 +789,8 @@ struct Engine {
           double* vs = K.V.p + (int64_t)s * n;
           double* zs = K.Z.p + (int64_t)s * n;
           bool formed = false;
+          ProductReq pq; pq.A = P.a_ptr(); pq.a32 = P.fp32; pq.w = K.w.p; pq.partials_only = true;
           if (fuse_product) {
-            ProductReq pq; pq.A = P.a_ptr(); pq.a32 = P.fp32; pq.w = K.w.p;
             formed = getrs(n, 1, F.LU32.p, n, F.perm.p, vs, n, zs, n, F.gscale, W(), true, act, &pq);
           } else {
             apply_inverse(C, F, vs, zs, 1, nullptr, M.apply64, act);
@@ -800,12 +800,15 @@ struct Engine {
             ra.R = K.w.p; ra.ldr = n; ra.matvec = true; ra.active = act;
             residual(ra, W());
           }
-          // CGS2, ||w||, Givens (the step that ends the cycle clears act)
+          // CGS2, ||w||, Givens (the step that ends the cycle clears act); a product formed beside the upper solve
+          // left its column-chunk partials, which the first CGS launch reduces into w
+          const double* pp = formed ? pq.part : nullptr;
+          const int64_t pn = formed ? pq.nch : 0;
           if (poll) {
             tags[s] = ++K.tag;
-            K.arnoldi_tail(s, beta0, M.gmres_tol, act, slots + 4 * s, tags[s]);
+            K.arnoldi_tail(s, beta0, M.gmres_tol, act, slots + 4 * s, tags[s], pp, pn);
           } else {
-            K.arnoldi_tail(s, beta0, M.gmres_tol, act);
+            K.arnoldi_tail(s, beta0, M.gmres_tol, act, nullptr, 0, pp, pn);
             HIP_CK(hipMemcpyAsync(slots + 4 * s, K.scal() + 2, 24, hipMemcpyDeviceToHost, W().stream));
             HIP_CK(hipEventRecord(K.ev[s & 1], W().stream));
           }

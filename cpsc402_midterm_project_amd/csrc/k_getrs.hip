@@ -814,7 +814,7 @@ static void trsv_pair3(int64_t n, const float* LU, int64_t lda, const int32_t* p
 
 // One right-hand side, then w = A d by the upper solve's extra workgroups and the product reduction.
 static void trsv_pair3_product(int64_t n, const float* LU, int64_t lda, const int32_t* perm, const double* R, int64_t ldr,
-                               double* D, double scale, Workspace& W, int32_t* err, const int* act, const ProductReq& pr,
+                               double* D, double scale, Workspace& W, int32_t* err, const int* act, ProductReq& pr,
                                int64_t cpc, int64_t nch) {
   const int nbk = (int)cdiv(n, GB3);
   const uint32_t tl = W.take_tags(1), tu = W.take_tags(1);
@@ -841,12 +841,13 @@ static void trsv_pair3_product(int64_t n, const float* LU, int64_t lda, const in
   // algorithmic bytes: the two triangles, then A once
   W.prof_end(ev, "getrs_product", 4.0 * (double)n * n + (pr.a32 ? 4.0 : 8.0) * (double)n * n + 32.0 * n,
              2.0 * (double)n * n + 2.0 * (double)n * n);
-  product_reduce(W.part.p, n, nch, D, pr.w, W, act);
+  if (pr.partials_only) { pr.part = W.part.p; pr.nch = nch; }
+  else product_reduce(W.part.p, n, nch, D, pr.w, W, act);
 }
 
 bool getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* perm,
            const double* R, int64_t ldr, double* D, int64_t ldd, double scale, Workspace& W, bool reuse_inverse,
-           const int* active, const ProductReq* product) {
+           const int* active, ProductReq* product) {
   MP_REQUIRE(n > 0 && nrhs > 0, "getrs: n, nrhs > 0");
   MP_REQUIRE(lda >= n && ldr >= n && ldd >= n, "getrs: leading dimensions >= n");
   const int64_t nbk = cdiv(n, GB);

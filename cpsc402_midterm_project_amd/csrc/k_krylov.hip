@@ -51,6 +51,57 @@ __global__ __launch_bounds__(256) void k_gemvt_partial(const double* __restrict_
   }
 }
 
+// The product's reduction and the first CGS pass's dot products in one launch (256 rows per workgroup, the
+// row block of both): w[r] = sum over the product's column chunks of their partials in chunk order (the
+// residual reduction's product mode), then k_gemvt_partial's partials of V^T w for these rows.  Both are the
+// same bits as the two kernels apart.
+__global__ __launch_bounds__(256) void k_reduce_dots(const double* __restrict__ P, int64_t nch, double* __restrict__ w,
+                                                     const double* __restrict__ V, int64_t ldv, int ncols, int64_t n,
+                                                     double* __restrict__ part, const int* __restrict__ act) {
+  if (act && *act == 0) return;
+  __shared__ double red[4][64];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int64_t r = (int64_t)blockIdx.x * KCH + t;
+  double wr = 0.0;
+  if (r < n) {
+    const int64_t fs = 2 * n;            // field stride of the one-column FP64 partials
+    const double* base = P + r;
+    double s = 0.0;
+    int64_t k = 0;
+    for (; k + 8 <= nch; k += 8) {
+      double sv[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) sv[u] = base[(k + u) * fs];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += sv[u];
+    }
+    for (; k < nch; k++) s += base[k * fs];
+    wr = s;
+    w[r] = s;
+  }
+  for (int i0 = 0; i0 < ncols; i0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = (r < n && i0 + u < ncols) ? V[(int64_t)(i0 + u) * ldv + r] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int i = i0 + u;
+      if (i >= ncols) break;
+      double sm = v[u] * wr;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) sm += __shfl_xor(sm, off, 64);
+      if (lane == 0) red[wv][i & 63] = sm;
+      if ((i & 63) == 63 || i == ncols - 1) {
+        __syncthreads();
+        const int base = i & ~63;
+        const int cnt = i - base + 1;
+        if (t < cnt) part[(int64_t)blockIdx.x * ncols + base + t] = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+        __syncthreads();
+      }
+    }
+  }
+}
+
 // out[i] (+)= sum over chunks in order; if acc != 0 the result is added to out[i]
 __global__ void k_gemvt_final(const double* __restrict__ part, int nchunks, int ncols, double* __restrict__ out, int acc,
                               const int* __restrict__ act) {
@@ -365,12 +416,17 @@ void Krylov::init_cycle(double beta, int* act) {
   LAUNCH_CK();
 }
 // CGS2 of w against V[:, 0..j] and the step's Givens update: 5 launches with the sums of the unfused kernels.
-void Krylov::arnoldi_tail(int j, double beta0, double tol, int* act, double* hslot, unsigned long long tag) {
+void Krylov::arnoldi_tail(int j, double beta0, double tol, int* act, double* hslot, unsigned long long tag,
+                          const double* prod_part, int64_t prod_nch) {
   const unsigned nch = cdiv(n, KCH);
   const int ncols = j + 1;
   double* sq = part.p + (size_t)nch * (m + 1);
   double* cs = small.p; double* sn = cs + (m + 1); double* g = sn + (m + 1);
-  hipLaunchKernelGGL(k_gemvt_partial, dim3(nch), dim3(256), 0, W->stream, V.p, n, ncols, w.p, n, part.p, (const int*)act);
+  if (prod_part)
+    hipLaunchKernelGGL(k_reduce_dots, dim3(nch), dim3(256), 0, W->stream, prod_part, prod_nch, w.p, V.p, n, ncols, n, part.p,
+                       (const int*)act);
+  else
+    hipLaunchKernelGGL(k_gemvt_partial, dim3(nch), dim3(256), 0, W->stream, V.p, n, ncols, w.p, n, part.p, (const int*)act);
   LAUNCH_CK();
   hipLaunchKernelGGL(k_cgs_update, dim3(nch), dim3(256), 0, W->stream, part.p, (int)nch, ncols, V.p, n, h(), w.p, n, -1.0,
                      (double*)nullptr, (const int*)act);

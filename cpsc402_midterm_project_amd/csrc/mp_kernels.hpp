@@ -165,10 +165,12 @@ void pack_operand(int fmt, bool tile_scale, const float* S, int64_t sI, int64_t 
 // active: device word; the substitutions return at once when it is 0 (nullptr: always run).
 // product: when given (one right-hand side, 256-row blocks), also w = A d in FP64 (the residual kernels' product
 // mode, the same bits), formed by extra workgroups of the upper solve as its blocks appear; returns whether it was.
-struct ProductReq { const void* A = nullptr; bool a32 = false; double* w = nullptr; };
+// partials_only: the product's column-chunk partials are left in part (nch chunks) for the caller to reduce.
+struct ProductReq { const void* A = nullptr; bool a32 = false; double* w = nullptr; bool partials_only = false;
+                    const double* part = nullptr; int64_t nch = 0; };
 bool getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* perm,
            const double* R, int64_t ldr, double* D, int64_t ldd, double scale, Workspace& W, bool reuse_inverse = false,
-           const int* active = nullptr, const ProductReq* product = nullptr);
+           const int* active = nullptr, ProductReq* product = nullptr);
 // the residual kernels' column split of an n x n product (chunks of cpc columns) and the reduction of its
 // partials into w = A x (product mode)
 void product_chunks(int64_t n, int64_t& cpc, int64_t& nch);

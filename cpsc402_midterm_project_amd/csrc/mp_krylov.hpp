@@ -38,7 +38,10 @@ struct Krylov {
   // One Arnoldi step after w = A z_j: CGS2 against V[:, 0..j] (h, h2, w), hn = ||w|| in scal()[0] and the Givens
   // update with its outputs in scal()[2..4], in 5 launches; the same sums as dots / update / norm / givens.  j < 256.
   // hslot (coherent pinned host memory, 4 words): the step's outputs and then its tag are also written there.
-  void arnoldi_tail(int j, double beta0, double tol, int* act, double* hslot = nullptr, unsigned long long tag = 0);
+  // prod_part: w is first formed from the product's column-chunk partials (prod_nch chunks, one FP64 column),
+  // in the same launch as the first dot products.
+  void arnoldi_tail(int j, double beta0, double tol, int* act, double* hslot = nullptr, unsigned long long tag = 0,
+                    const double* prod_part = nullptr, int64_t prod_nch = 0);
   void lsq(int k, double* y);
 };
 

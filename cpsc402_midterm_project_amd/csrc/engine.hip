@@ -780,6 +780,7 @@ struct Engine {
         if (!poll)
           for (int e = 0; e < 2; e++)
             if (!K.ev[e]) HIP_CK(hipEventCreateWithFlags(&K.ev[e], hipEventDisableTiming));
+        bool prev_formed = false;
         // z = F^{-1} v and w = A z: with FP32 factors applied unscaled, the product is formed by the upper solve's
         // own extra workgroups as its blocks appear (the same bits as the separate product); MPSOLVE_GETRS_MV=0
         // keeps them apart
@@ -790,6 +791,8 @@ struct Engine {
           double* zs = K.Z.p + (int64_t)s * n;
           bool formed = false;
           ProductReq pq; pq.A = P.a_ptr(); pq.a32 = P.fp32; pq.w = K.w.p; pq.partials_only = true;
+          // v_s = w / hn of the previous step, formed by this step's lower solve as it reads it (and stored in V)
+          if (s > 0 && prev_formed) { pq.scale_src = K.w.p; pq.scale_div = K.scal(); }
           if (fuse_product) {
             formed = getrs(n, 1, F.LU32.p, n, F.perm.p, vs, n, zs, n, F.gscale, W(), true, act, &pq);
           } else {
@@ -812,7 +815,9 @@ struct Engine {
             HIP_CK(hipMemcpyAsync(slots + 4 * s, K.scal() + 2, 24, hipMemcpyDeviceToHost, W().stream));
             HIP_CK(hipEventRecord(K.ev[s & 1], W().stream));
           }
-          K.scale_copy(K.w.p, K.V.p + (int64_t)(s + 1) * n, K.scal(), true, act);   // v_{s+1} = w / hn
+          // v_{s+1} = w / hn: here, or by the next step's lower solve when this step's product was fused
+          if (!formed) K.scale_copy(K.w.p, K.V.p + (int64_t)(s + 1) * n, K.scal(), true, act);
+          prev_formed = formed;
         };
         // waits until step s has written its slot; a stream that drained without it is an error
         auto wait_step = [&](int s) {

@@ -532,8 +532,11 @@ __global__ __launch_bounds__(256) void k_chain256(const float* __restrict__ LU, 
 // product.  Substitution workgroups never wait for these, so in-order dispatch still guarantees progress.
 constexpr int MV_ROWS = 512;       // rows per product workgroup
 constexpr int MV_MAXC = 2048;      // largest column chunk staged in LDS
+// wsrc (lower solves, one right-hand side): the right-hand side is v = wsrc / hn[0] (k_scale_copy's division),
+// formed by the thread that reads it and also stored into vout.
 struct MvArgs {
   const void* A = nullptr; int64_t lda = 0; int64_t cpc = 0; int nch = 0; double* P = nullptr;
+  const double* wsrc = nullptr; const double* hn = nullptr; double* vout = nullptr;
 };
 
 template <typename TA>
@@ -645,7 +648,16 @@ __global__ __launch_bounds__(512) void k_trsv3(const float* __restrict__ LU, int
 #pragma unroll
   for (int c = 0; c < NR; c++) {
     rhs[c] = 0.0f;
-    if (qq == 0 && rvalid) rhs[c] = LOWER ? (float)Rin[(int64_t)c * ldr + perm[row]] : Y[(int64_t)c * ldy + row];
+    if (qq == 0 && rvalid) {
+      if (LOWER && NR == 1 && mv.wsrc) {
+        const int64_t pr = perm[row];
+        const double v = mv.wsrc[pr] / mv.hn[0];
+        mv.vout[pr] = v;
+        rhs[c] = (float)v;
+      } else {
+        rhs[c] = LOWER ? (float)Rin[(int64_t)c * ldr + perm[row]] : Y[(int64_t)c * ldy + row];
+      }
+    }
   }
   float acc[NR];
 #pragma unroll
@@ -828,8 +840,10 @@ static void trsv_pair3_product(int64_t n, const float* LU, int64_t lda, const in
   MvArgs mv; mv.A = pr.A; mv.lda = n; mv.cpc = cpc; mv.nch = (int)nch; mv.P = W.part.p;
   const unsigned grid = (unsigned)(4 * nbk + cdiv(n, MV_ROWS) * nch);
   hipEvent_t ev = W.prof_begin();
+  MvArgs lv;
+  if (pr.scale_src) { lv.wsrc = pr.scale_src; lv.hn = pr.scale_div; lv.vout = (double*)R; }
   hipLaunchKernelGGL((k_trsv3<1, true>), dim3(4 * nbk), dim3(512), 0, W.stream, LU, lda, n, Ti, Mi, perm, R, ldr,
-                     W.rhs32.p, n, (double*)nullptr, (int64_t)0, 1.0, W.ybuf.p, tl, wl, err, spin, act, MvArgs{});
+                     W.rhs32.p, n, (double*)nullptr, (int64_t)0, 1.0, W.ybuf.p, tl, wl, err, spin, act, lv);
   LAUNCH_CK();
   if (pr.a32)
     hipLaunchKernelGGL((k_trsv3<1, false, float, true>), dim3(grid), dim3(512), 0, W.stream, LU, lda, n, Tu, Mu, perm,

@@ -166,8 +166,11 @@ void pack_operand(int fmt, bool tile_scale, const float* S, int64_t sI, int64_t 
 // product: when given (one right-hand side, 256-row blocks), also w = A d in FP64 (the residual kernels' product
 // mode, the same bits), formed by extra workgroups of the upper solve as its blocks appear; returns whether it was.
 // partials_only: the product's column-chunk partials are left in part (nch chunks) for the caller to reduce.
+// scale_src: the right-hand side is first formed as scale_src / scale_div[0] and stored into R (which must then be
+// writable), by the lower solve's threads as they read it.
 struct ProductReq { const void* A = nullptr; bool a32 = false; double* w = nullptr; bool partials_only = false;
-                    const double* part = nullptr; int64_t nch = 0; };
+                    const double* part = nullptr; int64_t nch = 0;
+                    const double* scale_src = nullptr; const double* scale_div = nullptr; };
 bool getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* perm,
            const double* R, int64_t ldr, double* D, int64_t ldd, double scale, Workspace& W, bool reuse_inverse = false,
            const int* active = nullptr, ProductReq* product = nullptr);

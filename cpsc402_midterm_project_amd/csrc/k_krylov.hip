@@ -179,6 +179,72 @@ __global__ __launch_bounds__(256) void k_cgs_update(const double* __restrict__ p
   }
 }
 
+// The first CGS pass's update and the second pass's dot products in one launch (256 rows per workgroup, the row
+// block of both): k_cgs_update without the norm partials, then k_gemvt_partial's partials of V^T w for the
+// updated rows, into part_out (not part_in: other workgroups may still be summing part_in).  The same bits as the
+// two kernels apart.
+__global__ __launch_bounds__(256) void k_cgs_update_dots(const double* __restrict__ part_in, int nchunks, int ncols,
+                                                         const double* __restrict__ V, int64_t ldv, double* __restrict__ hout,
+                                                         double* __restrict__ w, int64_t n, double sign,
+                                                         double* __restrict__ part_out, const int* __restrict__ act) {
+  if (act && *act == 0) return;
+  __shared__ double hs[256];
+  __shared__ double red[4][64];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (t < ncols) {
+    double s = 0.0;
+    int c = 0;
+    for (; c + 8 <= nchunks; c += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = part_in[(int64_t)(c + u) * ncols + t];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += v[u];
+    }
+    for (; c < nchunks; c++) s += part_in[(int64_t)c * ncols + t];
+    hs[t] = s;
+    if (blockIdx.x == 0) hout[t] = s;
+  }
+  __syncthreads();
+  const int64_t r = (int64_t)blockIdx.x * 256 + t;
+  double wr = 0.0;
+  if (r < n) {
+    double s = 0.0;
+    int i = 0;
+    for (; i + 8 <= ncols; i += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = V[(int64_t)(i + u) * ldv + r];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s = fma(v[u], hs[i + u], s);
+    }
+    for (; i < ncols; i++) s = fma(V[(int64_t)i * ldv + r], hs[i], s);
+    wr = fma(sign, s, w[r]);
+    w[r] = wr;
+  }
+  for (int i0 = 0; i0 < ncols; i0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = (r < n && i0 + u < ncols) ? V[(int64_t)(i0 + u) * ldv + r] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int i = i0 + u;
+      if (i >= ncols) break;
+      double sm = v[u] * wr;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) sm += __shfl_xor(sm, off, 64);
+      if (lane == 0) red[wv][i & 63] = sm;
+      if ((i & 63) == 63 || i == ncols - 1) {
+        __syncthreads();
+        const int base = i & ~63;
+        const int cnt = i - base + 1;
+        if (t < cnt) part_out[(int64_t)blockIdx.x * ncols + base + t] = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+        __syncthreads();
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_sumsq_partial(const double* __restrict__ w, int64_t n, double* __restrict__ part,
                                                        const int* __restrict__ act) {
   if (act && *act == 0) return;
@@ -374,6 +440,7 @@ void Krylov::ensure(int64_t n_, int m_) {
   H.alloc(*W->tracker, (size_t)(m + 1) * m + 8, "gmres_H");
   small.alloc(*W->tracker, (size_t)8 * (m + 2) + 64, "gmres_small");
   part.alloc(*W->tracker, (size_t)cdiv(n, KCH) * (m + 2) + 64, "gmres_part");
+  part2.alloc(*W->tracker, (size_t)cdiv(n, KCH) * (m + 1) + 64, "gmres_part2");
 }
 
 void Krylov::dots(const double* Vb, int ncols, const double* x, double* out, bool accumulate, const int* act) {
@@ -415,7 +482,7 @@ void Krylov::init_cycle(double beta, int* act) {
   hipLaunchKernelGGL(k_gmres_init, dim3(cdiv((size_t)(m + 1) * m + 1, 256)), dim3(256), 0, W->stream, g, H.p, m, beta, act);
   LAUNCH_CK();
 }
-// CGS2 of w against V[:, 0..j] and the step's Givens update: 5 launches with the sums of the unfused kernels.
+// CGS2 of w against V[:, 0..j] and the step's Givens update: 4 launches with the sums of the unfused kernels.
 void Krylov::arnoldi_tail(int j, double beta0, double tol, int* act, double* hslot, unsigned long long tag,
                           const double* prod_part, int64_t prod_nch) {
   const unsigned nch = cdiv(n, KCH);
@@ -428,12 +495,20 @@ void Krylov::arnoldi_tail(int j, double beta0, double tol, int* act, double* hsl
   else
     hipLaunchKernelGGL(k_gemvt_partial, dim3(nch), dim3(256), 0, W->stream, V.p, n, ncols, w.p, n, part.p, (const int*)act);
   LAUNCH_CK();
-  hipLaunchKernelGGL(k_cgs_update, dim3(nch), dim3(256), 0, W->stream, part.p, (int)nch, ncols, V.p, n, h(), w.p, n, -1.0,
-                     (double*)nullptr, (const int*)act);
-  LAUNCH_CK();
-  hipLaunchKernelGGL(k_gemvt_partial, dim3(nch), dim3(256), 0, W->stream, V.p, n, ncols, w.p, n, part.p, (const int*)act);
-  LAUNCH_CK();
-  hipLaunchKernelGGL(k_cgs_update, dim3(nch), dim3(256), 0, W->stream, part.p, (int)nch, ncols, V.p, n, h2(), w.p, n, -1.0,
+  // MPSOLVE_CGS_FUSE=0: the first pass's update and the second pass's dot products as two launches (same sums)
+  const bool cgs_fuse = !(std::getenv("MPSOLVE_CGS_FUSE") && std::getenv("MPSOLVE_CGS_FUSE")[0] == '0');
+  if (cgs_fuse) {
+    hipLaunchKernelGGL(k_cgs_update_dots, dim3(nch), dim3(256), 0, W->stream, part.p, (int)nch, ncols, V.p, n, h(), w.p, n, -1.0,
+                       part2.p, (const int*)act);
+    LAUNCH_CK();
+  } else {
+    hipLaunchKernelGGL(k_cgs_update, dim3(nch), dim3(256), 0, W->stream, part.p, (int)nch, ncols, V.p, n, h(), w.p, n, -1.0,
+                       (double*)nullptr, (const int*)act);
+    LAUNCH_CK();
+    hipLaunchKernelGGL(k_gemvt_partial, dim3(nch), dim3(256), 0, W->stream, V.p, n, ncols, w.p, n, part2.p, (const int*)act);
+    LAUNCH_CK();
+  }
+  hipLaunchKernelGGL(k_cgs_update, dim3(nch), dim3(256), 0, W->stream, part2.p, (int)nch, ncols, V.p, n, h2(), w.p, n, -1.0,
                      sq, (const int*)act);
   LAUNCH_CK();
   hipLaunchKernelGGL(k_givens_fused, dim3(1), dim3(256), 0, W->stream, H.p, m, j, h(), h2(), sq, (int)nch, scal(), cs, sn, g,

@@ -419,3 +419,22 @@ def test_fgmres_check_and_next_residual_share_a_pass(ctx, n, dtype, method, monk
               "worst_row"):
         assert out["0"][1][k] == out["1"][1][k], k
     assert out["0"][2] == out["1"][2]
+
+
+def test_cgs_update_and_next_dots_fused(ctx, monkeypatch):
+    """The first CGS pass's update of w and the second pass's dot products in one launch per 256-row block: the
+    iterate, counts and history must be the same bits as with the two launches apart (MPSOLVE_CGS_FUSE=0)."""
+    print("[cgs fuse] mx16x3_gmresir_w64_rdd n=1200", flush=True)
+    P = Problem.generate(ctx, family="randsvd", n=1200, nrhs=1, kappa=1e5, seed=9260, rhs="random", dtype="fp64")
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("MPSOLVE_CGS_FUSE", mode)
+        X, rec = P.solve("mx16x3_gmresir_w64_rdd", nb=128, tau=1e-15, stop_on="eta")
+        assert rec["ok"] and rec["inner_iters_total"] > 3, (mode, rec["terminal_reason"], rec["inner_iters_total"])
+        hist = [(h.get("eta_max"), h.get("note")) for h in P.history() if h.get("kind") == "outer"]
+        out[mode] = (X, rec, hist)
+    P.close()
+    np.testing.assert_array_equal(out["0"][0], out["1"][0])
+    for k in ("terminal_reason", "inner_iters_total", "outer_iters", "x_sha256", "eta"):
+        assert out["0"][1][k] == out["1"][1][k], k
+    assert out["0"][2] == out["1"][2]

@@ -129,11 +129,9 @@ __global__ __launch_bounds__(256) void k_gemv_update(const double* __restrict__ 
 // formed by every workgroup into LDS; workgroup 0 stores it), then w[r] += sign * sum_i V[r, i] h[i] (k_gemv_update's
 // order).  With sq != nullptr the chunk partials of sum w^2 of the updated w follow (k_sumsq_partial's order; the
 // workgroup of 256 rows is one chunk).  ncols <= 256.
-__global__ __launch_bounds__(256) void k_cgs_update(const double* __restrict__ part, int nchunks, int ncols,
-                                                    const double* __restrict__ V, int64_t ldv, double* __restrict__ hout,
-                                                    double* __restrict__ w, int64_t n, double sign, double* __restrict__ sq,
-                                                    const int* __restrict__ act) {
-  if (act && *act == 0) return;
+__device__ __forceinline__ void cgs_update_body(const double* __restrict__ part, int nchunks, int ncols,
+                                                const double* __restrict__ V, int64_t ldv, double* __restrict__ hout,
+                                                double* __restrict__ w, int64_t n, double sign, double* __restrict__ sq) {
   __shared__ double hs[256];
   __shared__ double red[4];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -177,6 +175,14 @@ __global__ __launch_bounds__(256) void k_cgs_update(const double* __restrict__ p
     __syncthreads();
     if (t == 0) sq[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
   }
+}
+
+__global__ __launch_bounds__(256) void k_cgs_update(const double* __restrict__ part, int nchunks, int ncols,
+                                                    const double* __restrict__ V, int64_t ldv, double* __restrict__ hout,
+                                                    double* __restrict__ w, int64_t n, double sign, double* __restrict__ sq,
+                                                    const int* __restrict__ act) {
+  if (act && *act == 0) return;
+  cgs_update_body(part, nchunks, ncols, V, ldv, hout, w, n, sign, sq);
 }
 
 // The first CGS pass's update and the second pass's dot products in one launch (256 rows per workgroup, the row
@@ -354,13 +360,12 @@ __global__ void k_givens_step(double* __restrict__ H, int m, int j, const double
 // stored at hnout for the scaling of the next basis vector.  With hslot (coherent pinned host memory) the three
 // step outputs also go there, followed by the step's tag with a system-scope release, so the host can wait for
 // the step by reading the tag.
-__global__ __launch_bounds__(256) void k_givens_fused(double* __restrict__ H, int m, int j, double* __restrict__ h,
-                                                      const double* __restrict__ h2, const double* __restrict__ sq, int nchunks,
-                                                      double* __restrict__ hnout, double* __restrict__ cs, double* __restrict__ sn,
-                                                      double* __restrict__ g, double beta0, double* __restrict__ out, double tol,
-                                                      int* __restrict__ act, double* __restrict__ hslot, unsigned long long tag) {
-  if (act && *act == 0) return;
-  __shared__ double red[256];
+__device__ __forceinline__ void givens_tail(double* __restrict__ H, int m, int j, double* __restrict__ h,
+                                            const double* __restrict__ h2, const double* __restrict__ sq, int nchunks,
+                                            double* __restrict__ hnout, double* __restrict__ cs, double* __restrict__ sn,
+                                            double* __restrict__ g, double beta0, double* __restrict__ out, double tol,
+                                            int* __restrict__ act, double* __restrict__ hslot, unsigned long long tag,
+                                            double* red) {
   const int t = threadIdx.x;
   if (t <= j) h[t] += h2[t];
   double s = 0.0;
@@ -396,6 +401,44 @@ __global__ __launch_bounds__(256) void k_givens_fused(double* __restrict__ H, in
     hslot[0] = out[0]; hslot[1] = hnv; hslot[2] = out[2];
     __hip_atomic_store((unsigned long long*)(hslot + 3), tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
+}
+
+__global__ __launch_bounds__(256) void k_givens_fused(double* __restrict__ H, int m, int j, double* __restrict__ h,
+                                                      const double* __restrict__ h2, const double* __restrict__ sq, int nchunks,
+                                                      double* __restrict__ hnout, double* __restrict__ cs, double* __restrict__ sn,
+                                                      double* __restrict__ g, double beta0, double* __restrict__ out, double tol,
+                                                      int* __restrict__ act, double* __restrict__ hslot, unsigned long long tag) {
+  if (act && *act == 0) return;
+  __shared__ double red[256];
+  givens_tail(H, m, j, h, h2, sq, nchunks, hnout, cs, sn, g, beta0, out, tol, act, hslot, tag, red);
+}
+
+// The second CGS pass (k_cgs_update with the norm's partials) and the step's Givens update in one launch: every
+// workgroup releases its writes and counts itself; the last one to finish acquires them, clears the counter for
+// the next step and runs k_givens_fused's code.  The same bits as the two launches; nothing waits.
+__global__ __launch_bounds__(256) void k_cgs_update_givens(const double* __restrict__ part, int nchunks, int ncols,
+                                                           const double* __restrict__ V, int64_t ldv, double* __restrict__ hout,
+                                                           double* __restrict__ w, int64_t n, double sign, double* __restrict__ sq,
+                                                           unsigned int* __restrict__ counter, double* __restrict__ H, int m, int j,
+                                                           double* __restrict__ h, double* __restrict__ hnout,
+                                                           double* __restrict__ cs, double* __restrict__ sn, double* __restrict__ g,
+                                                           double beta0, double* __restrict__ out, double tol, int* __restrict__ act,
+                                                           double* __restrict__ hslot, unsigned long long tag) {
+  if (act && *act == 0) return;
+  cgs_update_body(part, nchunks, ncols, V, ldv, hout, w, n, sign, sq);
+  __shared__ int s_last;
+  __shared__ double red[256];
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (prev == gridDim.x - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  givens_tail(H, m, j, h, hout, sq, nchunks, hnout, cs, sn, g, beta0, out, tol, act, hslot, tag, red);
 }
 
 // Solve the k x k upper-triangular H y = g (back substitution, one thread).
@@ -441,6 +484,11 @@ void Krylov::ensure(int64_t n_, int m_) {
   small.alloc(*W->tracker, (size_t)8 * (m + 2) + 64, "gmres_small");
   part.alloc(*W->tracker, (size_t)cdiv(n, KCH) * (m + 2) + 64, "gmres_part");
   part2.alloc(*W->tracker, (size_t)cdiv(n, KCH) * (m + 1) + 64, "gmres_part2");
+  // the fused CGS / Givens launch's finish counter: zero here, and put back to zero by each step's last workgroup
+  if (!gcount.p) {
+    gcount.alloc(*W->tracker, 1, "gmres_finish_count");
+    HIP_CK(hipMemsetAsync(gcount.p, 0, sizeof(unsigned), W->stream));
+  }
 }
 
 void Krylov::dots(const double* Vb, int ncols, const double* x, double* out, bool accumulate, const int* act) {
@@ -482,7 +530,8 @@ void Krylov::init_cycle(double beta, int* act) {
   hipLaunchKernelGGL(k_gmres_init, dim3(cdiv((size_t)(m + 1) * m + 1, 256)), dim3(256), 0, W->stream, g, H.p, m, beta, act);
   LAUNCH_CK();
 }
-// CGS2 of w against V[:, 0..j] and the step's Givens update: 4 launches with the sums of the unfused kernels.
+// CGS2 of w against V[:, 0..j] and the step's Givens update: 3 launches (5 with MPSOLVE_CGS_FUSE=0) with the sums
+// of the unfused kernels.
 void Krylov::arnoldi_tail(int j, double beta0, double tol, int* act, double* hslot, unsigned long long tag,
                           const double* prod_part, int64_t prod_nch) {
   const unsigned nch = cdiv(n, KCH);
@@ -508,12 +557,18 @@ void Krylov::arnoldi_tail(int j, double beta0, double tol, int* act, double* hsl
     hipLaunchKernelGGL(k_gemvt_partial, dim3(nch), dim3(256), 0, W->stream, V.p, n, ncols, w.p, n, part2.p, (const int*)act);
     LAUNCH_CK();
   }
-  hipLaunchKernelGGL(k_cgs_update, dim3(nch), dim3(256), 0, W->stream, part2.p, (int)nch, ncols, V.p, n, h2(), w.p, n, -1.0,
-                     sq, (const int*)act);
-  LAUNCH_CK();
-  hipLaunchKernelGGL(k_givens_fused, dim3(1), dim3(256), 0, W->stream, H.p, m, j, h(), h2(), sq, (int)nch, scal(), cs, sn, g,
-                     beta0, scal() + 2, tol, act, hslot, tag);
-  LAUNCH_CK();
+  if (cgs_fuse) {
+    hipLaunchKernelGGL(k_cgs_update_givens, dim3(nch), dim3(256), 0, W->stream, part2.p, (int)nch, ncols, V.p, n, h2(), w.p, n,
+                       -1.0, sq, gcount.p, H.p, m, j, h(), scal(), cs, sn, g, beta0, scal() + 2, tol, act, hslot, tag);
+    LAUNCH_CK();
+  } else {
+    hipLaunchKernelGGL(k_cgs_update, dim3(nch), dim3(256), 0, W->stream, part2.p, (int)nch, ncols, V.p, n, h2(), w.p, n, -1.0,
+                       sq, (const int*)act);
+    LAUNCH_CK();
+    hipLaunchKernelGGL(k_givens_fused, dim3(1), dim3(256), 0, W->stream, H.p, m, j, h(), h2(), sq, (int)nch, scal(), cs, sn, g,
+                       beta0, scal() + 2, tol, act, hslot, tag);
+    LAUNCH_CK();
+  }
 }
 void Krylov::lsq(int k, double* y) {
   const double* g = small.p + 2 * (m + 1);

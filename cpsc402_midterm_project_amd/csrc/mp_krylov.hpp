@@ -9,6 +9,7 @@ struct Krylov {
   int64_t n = 0; int m = 0;
   DBuf<double> V, Z, w, sR, rcur, H, small, part;
   DBuf<double> part2;   // the second CGS pass's dot-product partials (written while part is still being read)
+  DBuf<unsigned> gcount;  // workgroups of the fused second CGS pass that have finished (the last runs the Givens step)
   hipEvent_t ev[2] = {nullptr, nullptr};             // step-output events of the steps run ahead of the host
   // coherent pinned host slots [step][rec, hn, finite, tag] written by the step's Givens kernel, and the last tag
   // handed out (tags only grow, so a slot never shows a stale step as the awaited one)
@@ -37,7 +38,7 @@ struct Krylov {
   void givens(int j, const double* h, const double* hn, double beta0, double* out, double tol = -1.0, int* act = nullptr);
   void init_cycle(double beta, int* act = nullptr);
   // One Arnoldi step after w = A z_j: CGS2 against V[:, 0..j] (h, h2, w), hn = ||w|| in scal()[0] and the Givens
-  // update with its outputs in scal()[2..4], in 4 launches; the same sums as dots / update / norm / givens.  j < 256.
+  // update with its outputs in scal()[2..4], in 3 launches; the same sums as dots / update / norm / givens.  j < 256.
   // hslot (coherent pinned host memory, 4 words): the step's outputs and then its tag are also written there.
   // prod_part: w is first formed from the product's column-chunk partials (prod_nch chunks, one FP64 column),
   // in the same launch as the first dot products.

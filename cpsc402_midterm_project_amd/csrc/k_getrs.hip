@@ -63,21 +63,25 @@ __device__ __forceinline__ float gs_wait(const uint64_t* p, uint64_t first, uint
 
 // Inverse of the 128 x 128 diagonal block b of L (unit lower, LOWER) or U (upper).  Output column-major per
 // block: Tinv[b * 128 * 128 + col * 128 + row]; rows / columns beyond n are the identity.
-// Thread c computes column c: lower  x_q = -sum_{p=c}^{q-1} L_qp x_p (x_c = 1),
-//                             upper  x_q = (delta_qc - sum_{p=q+1}^{c} U_qp x_p) / U_qq, q = c .. 0,
-// each sum accumulated in increasing p.
+// Blocked, so the serial chain is short: the eight 16 x 16 diagonal blocks are inverted by substitution (thread c
+// takes column c % 16 of block c / 16, each sum accumulated in increasing p), then three levels combine pairs of
+// s x s inverses into 2s x 2s ones (s = 16, 32, 64):
+//   lower [A 0; B C]^{-1} = [A^{-1} 0; -C^{-1} (B A^{-1})  C^{-1}],
+//   upper [A B; 0 C]^{-1} = [A^{-1}  -A^{-1} (B C^{-1}); 0 C^{-1}],
+// every product summed in increasing k over its nonzero range.
 template <bool LOWER>
 __device__ __forceinline__ void trinv_block(const float* __restrict__ LU, int64_t lda, int64_t n, float* __restrict__ Tinv,
                                             int b, float* sm) {
   float* T = sm;                      // [128][129] row-major T[q * 129 + p]
-  float* X = sm + GB * (GB + 1);      // [128][128] X[p * 128 + c]
-  const int c = threadIdx.x;
+  float* X = sm + GB * (GB + 1);      // [128][128] X[p * 128 + c], the inverse being built
+  float* Wt = X + GB * GB;            // [64 * 64] products of one level
+  const int t = threadIdx.x;
   const int64_t o = (int64_t)b * GB;
   const int nb = (int)min((int64_t)GB, n - o);
   {
-    // thread c reads row q = c of every column p (coalesced over q), 32 columns in flight at a time; rows and
+    // thread q reads row q of every column p (coalesced over q), 32 columns in flight at a time; rows and
     // columns past n read a clamped element and store the identity
-    const int q = c;
+    const int q = t;
     const float* src = LU + o + min(q, nb - 1);
     for (int pb = 0; pb < GB; pb += 32) {
       float v[32];
@@ -89,34 +93,71 @@ __device__ __forceinline__ void trinv_block(const float* __restrict__ LU, int64_
         T[q * (GB + 1) + p] = (q < nb && p < nb) ? v[i] : ((q == p) ? 1.0f : 0.0f);
       }
     }
+    for (int idx = t; idx < GB * GB; idx += GB) X[idx] = 0.0f;
   }
   __syncthreads();
-  if (LOWER) {
-    X[c * GB + c] = 1.0f;
-    for (int q = 0; q < GB; q++) if (q < c) X[q * GB + c] = 0.0f;
-    for (int q = 1; q < GB; q++) {
-      float s = 0.0f;
-      // unrolled so the LDS reads of later terms are in flight while the fmaf chain advances (same order)
-#pragma unroll 8
-      for (int p = 0; p < q; p++) {
-        const float ns = fmaf(T[q * (GB + 1) + p], X[p * GB + c], s);
-        s = (p >= c) ? ns : s;
+  {
+    // 16 x 16 leaves: block lb = t / 16, column c = t % 16
+    const int lb = t >> 4, c = t & 15, ob = lb * 16;
+    float x[16];
+    if (LOWER) {
+#pragma unroll
+      for (int q = 0; q < 16; q++) x[q] = (q == c) ? 1.0f : 0.0f;
+#pragma unroll
+      for (int q = 1; q < 16; q++) {
+        float s = 0.0f;
+#pragma unroll
+        for (int p = 0; p < q; p++) {
+          const float ns = fmaf(T[(ob + q) * (GB + 1) + ob + p], x[p], s);
+          s = (p >= c) ? ns : s;
+        }
+        x[q] = (q > c) ? -s : x[q];
       }
-      if (q > c) X[q * GB + c] = -s;
-    }
-  } else {
-    for (int q = GB - 1; q >= 0; q--) {
-      float s = (q == c) ? 1.0f : 0.0f;
-#pragma unroll 8
-      for (int p = q + 1; p < GB; p++) {
-        const float ns = fmaf(-T[q * (GB + 1) + p], X[p * GB + c], s);
-        s = (p <= c) ? ns : s;
+    } else {
+#pragma unroll
+      for (int q = 15; q >= 0; q--) {
+        float s = (q == c) ? 1.0f : 0.0f;
+#pragma unroll
+        for (int p = q + 1; p < 16; p++) {
+          const float ns = fmaf(-T[(ob + q) * (GB + 1) + ob + p], x[p], s);
+          s = (p <= c) ? ns : s;
+        }
+        x[q] = (q <= c) ? s / T[(ob + q) * (GB + 1) + ob + q] : 0.0f;
       }
-      X[q * GB + c] = (q <= c) ? s / T[q * (GB + 1) + q] : 0.0f;
     }
+#pragma unroll
+    for (int q = 0; q < 16; q++) X[(ob + q) * GB + ob + c] = x[q];
   }
   __syncthreads();
-  for (int idx = c; idx < GB * GB; idx += GB) {
+  for (int s = 16; s < GB; s *= 2) {
+    const int npair = GB / (2 * s), ss = s * s;
+    // W = B A^{-1} (lower, B = T[o2 + s.., o2..]) or B C^{-1} (upper, B = T[o2.., o2 + s..]), pair by pair
+    for (int e = t; e < npair * ss; e += GB) {
+      const int pr = e / ss, i = (e % ss) / s, j = e % s, o2 = pr * 2 * s;
+      float acc = 0.0f;
+      if (LOWER) {
+        for (int k = j; k < s; k++) acc = fmaf(T[(o2 + s + i) * (GB + 1) + o2 + k], X[(o2 + k) * GB + o2 + j], acc);
+      } else {
+        for (int k = 0; k <= j; k++) acc = fmaf(T[(o2 + i) * (GB + 1) + o2 + s + k], X[(o2 + s + k) * GB + o2 + s + j], acc);
+      }
+      Wt[pr * ss + i * s + j] = acc;
+    }
+    __syncthreads();
+    // the off-diagonal block: lower -C^{-1} W into X[o2 + s.., o2..], upper -A^{-1} W into X[o2.., o2 + s..]
+    for (int e = t; e < npair * ss; e += GB) {
+      const int pr = e / ss, i = (e % ss) / s, j = e % s, o2 = pr * 2 * s;
+      float acc = 0.0f;
+      if (LOWER) {
+        for (int k = 0; k <= i; k++) acc = fmaf(X[(o2 + s + i) * GB + o2 + s + k], Wt[pr * ss + k * s + j], acc);
+        X[(o2 + s + i) * GB + o2 + j] = -acc;
+      } else {
+        for (int k = i; k < s; k++) acc = fmaf(X[(o2 + i) * GB + o2 + k], Wt[pr * ss + k * s + j], acc);
+        X[(o2 + i) * GB + o2 + s + j] = -acc;
+      }
+    }
+    __syncthreads();
+  }
+  for (int idx = t; idx < GB * GB; idx += GB) {
     const int col = idx >> 7, row = idx & (GB - 1);
     Tinv[(size_t)b * GB * GB + idx] = X[row * GB + col];
   }
@@ -893,7 +934,7 @@ bool getrs(int64_t n, int nrhs, const float* LU, int64_t lda, const int32_t* per
   }
   if (!reuse_inverse || W.tinv_lu != LU || W.tinv_n != n || W.tinv_lda != lda || W.tinv_gen != W.factor_gen || W.tinv_v3 != v3) {
     static bool attr = false;
-    const size_t lds = (size_t)(GB * (GB + 1) + GB * GB) * 4;
+    const size_t lds = (size_t)(GB * (GB + 1) + GB * GB + 64 * 64) * 4;
     const size_t lds_cm = (size_t)2 * GB * (GB + 1) * 4;
     if (!attr) {
       HIP_CK(hipFuncSetAttribute((const void*)k_trinv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -5,7 +5,8 @@
 // permutation gather, the FP64->FP32 cast, the scaling and the FP32->FP64 cast are fused into them.
 //
 // Blocked by 128 rows.  The 128 x 128 diagonal blocks of L (unit lower) and U are inverted once per
-// factorization (k_trinv, one workgroup per block, FP32 substitution on the columns of the identity), so the
+// factorization (k_trinv, one workgroup per block: FP32 substitution on 16 x 16 diagonal blocks, then 2 x 2 block
+// products up to 128), so the
 // diagonal step of the solve is a parallel matrix-vector product, and so are the chain matrices
 // M_i = T_i^{-1} B_i (B_i the off-diagonal block next to the diagonal, k_chainmat): block i of the solution
 // is x_i = c_i - M_i x_prev with c_i = T_i^{-1} (r_i - sum of the other off-diagonal blocks' products), and
